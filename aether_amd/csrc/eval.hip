@@ -1,0 +1,464 @@
+// eval.hip -- fused SoA evaluation and compensated reductions (gfx950).
+//
+// The reference evaluates `dv[i] = expr` one sample per thread with the
+// component loop unrolled (expr/Assign.h:124-160), and its 128-bit bundle
+// path only applies to layout_right views (backend/cuda/bundle/LoadStore.h:
+// 150-208). Here each whole-array assignment is one kernel whose unit of
+// work is a (component, tile) pair: a tile is 2048 consecutive samples of
+// one component plane, 256 lanes each issuing 16-byte loads (double2 /
+// float4) along the sample axis of the plane, so every load instruction of a
+// wave covers 1 KiB of contiguous memory. The optional per-component sum is
+// folded into the same pass (0 extra HBM bytes), in the fixed tile shape of
+// accum.hpp, and finished by a small group tree.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/aether_hip.h"
+#include "accum.hpp"
+#include "common.hpp"
+#include "dmath.hpp"
+
+namespace aeth {
+namespace {
+
+constexpr int kTile = 2048;
+constexpr int kThreads = 256;
+constexpr int kGroup = 1024;
+
+template<class T>
+struct V16;
+template<>
+struct V16<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+template<>
+struct V16<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+
+template<class T>
+__device__ __forceinline__ void vload(const T* p, T (&v)[16 / sizeof(T)])
+{
+    using VT = typename V16<T>::type;
+    const VT x = *reinterpret_cast<const VT*>(p);
+    if constexpr (sizeof(T) == 8) {
+        v[0] = x.x;
+        v[1] = x.y;
+    } else {
+        v[0] = x.x;
+        v[1] = x.y;
+        v[2] = x.z;
+        v[3] = x.w;
+    }
+}
+
+template<class T>
+__device__ __forceinline__ void vstore(T* p, const T (&v)[16 / sizeof(T)])
+{
+    using VT = typename V16<T>::type;
+    VT x;
+    if constexpr (sizeof(T) == 8) {
+        x.x = v[0];
+        x.y = v[1];
+    } else {
+        x.x = v[0];
+        x.y = v[1];
+        x.z = v[2];
+        x.w = v[3];
+    }
+    *reinterpret_cast<VT*>(p) = x;
+}
+
+// The benchmark expression, spelled as aether evaluates it:
+// Sum(Sum(a, Product(b, c)), Product(CWiseSin(a), CWiseExp(CWiseScale(-1, Product(b, b))))).
+template<class T>
+__device__ __forceinline__ T cfg0_expr(T a, T b, T c)
+{
+    const T lhs = a + b * c;
+    const T nb2 = T(-1) * (b * b);
+    if constexpr (sizeof(T) == 8) {
+        return lhs + dm::sin(a) * dm::exp(nb2);
+    } else {
+        return lhs + dm::sinf(a) * dm::expf(nb2);
+    }
+}
+
+// One block per (component, tile) unit, grid-stride over units.
+// VECTOR: all planes 16-byte aligned (pitch * sizeof(T) % 16 == 0).
+template<class T, int M, bool SUM, bool VECTOR>
+__global__ __launch_bounds__(kThreads) void cfg0_kernel(const T* __restrict__ a, const T* __restrict__ b,
+                                                        const T* __restrict__ c, T* __restrict__ d, uint32_t n,
+                                                        uint64_t pitch, uint32_t tiles, uint32_t units,
+                                                        acc::Pair<T>* __restrict__ part)
+{
+    constexpr int VN = V16<T>::n;
+    constexpr int ROUNDS = kTile / (kThreads * VN);
+    __shared__ acc::Pair<T> lds[4];
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const uint32_t comp = u / tiles;
+        const uint32_t tile = u - comp * tiles;
+        const uint32_t s0 = tile * kTile;
+        const uint64_t base = (uint64_t)comp * pitch + s0;
+        acc::Pair<T> p{T(0), T(0)};
+        if (VECTOR && s0 + kTile <= n) {
+            T va[ROUNDS][VN], vb[ROUNDS][VN], vc[ROUNDS][VN];
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+                const uint64_t o = base + (uint64_t)r * (kThreads * VN) + (uint64_t)threadIdx.x * VN;
+                vload(a + o, va[r]);
+                vload(b + o, vb[r]);
+                vload(c + o, vc[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+                T vd[VN];
+#pragma unroll
+                for (int k = 0; k < VN; ++k) {
+                    vd[k] = cfg0_expr(va[r][k], vb[r][k], vc[r][k]);
+                    if constexpr (SUM) acc::accumulate<M, false>(p, vd[k]);
+                }
+                const uint64_t o = base + (uint64_t)r * (kThreads * VN) + (uint64_t)threadIdx.x * VN;
+                vstore(d + o, vd);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+#pragma unroll
+                for (int k = 0; k < VN; ++k) {
+                    const uint32_t i = s0 + r * (kThreads * VN) + threadIdx.x * VN + k;
+                    if (i < n) {
+                        const uint64_t o = (uint64_t)comp * pitch + i;
+                        const T v = cfg0_expr(a[o], b[o], c[o]);
+                        d[o] = v;
+                        if constexpr (SUM) acc::accumulate<M, false>(p, v);
+                    }
+                }
+            }
+        }
+        if constexpr (SUM) {
+            const acc::Pair<T> r = acc::block_combine<M>(p, lds);
+            if (threadIdx.x == 0) part[u] = r;
+        }
+    }
+}
+
+// Sum (or sum of squares) of an SoA array in the tile shape.
+template<class T, int M, bool SQUARE, bool VECTOR>
+__global__ __launch_bounds__(kThreads) void tile_reduce_kernel(const T* __restrict__ x, uint32_t n, uint64_t pitch,
+                                                               uint32_t tiles, uint32_t units,
+                                                               acc::Pair<T>* __restrict__ part)
+{
+    constexpr int VN = V16<T>::n;
+    constexpr int ROUNDS = kTile / (kThreads * VN);
+    __shared__ acc::Pair<T> lds[4];
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const uint32_t comp = u / tiles;
+        const uint32_t tile = u - comp * tiles;
+        const uint32_t s0 = tile * kTile;
+        const uint64_t base = (uint64_t)comp * pitch + s0;
+        acc::Pair<T> p{T(0), T(0)};
+        if (VECTOR && s0 + kTile <= n) {
+            T v[ROUNDS][VN];
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r)
+                vload(x + base + (uint64_t)r * (kThreads * VN) + (uint64_t)threadIdx.x * VN, v[r]);
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r)
+#pragma unroll
+                for (int k = 0; k < VN; ++k) acc::accumulate<M, SQUARE>(p, v[r][k]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r)
+#pragma unroll
+                for (int k = 0; k < VN; ++k) {
+                    const uint32_t i = s0 + r * (kThreads * VN) + threadIdx.x * VN + k;
+                    if (i < n) acc::accumulate<M, SQUARE>(p, x[(uint64_t)comp * pitch + i]);
+                }
+        }
+        const acc::Pair<T> r = acc::block_combine<M>(p, lds);
+        if (threadIdx.x == 0) part[u] = r;
+    }
+}
+
+// One level of the group tree: block (g, comp) reduces partials
+// [g*1024, min(T, (g+1)*1024)) of component `comp` by the halving tree.
+template<class T, int M>
+__global__ __launch_bounds__(512) void group_tree_kernel(const acc::Pair<T>* __restrict__ in,
+                                                         acc::Pair<T>* __restrict__ out, uint32_t count,
+                                                         uint32_t in_stride, uint32_t out_stride)
+{
+    __shared__ acc::Pair<T> s[kGroup];
+    const uint32_t g = blockIdx.x, comp = blockIdx.y, t = threadIdx.x;
+    const uint32_t first = g * kGroup;
+    const uint32_t m = min((uint32_t)kGroup, count - first);
+    const acc::Pair<T>* src = in + (uint64_t)comp * in_stride + first;
+    if (t < m) s[t] = src[t];
+    if (t + 512 < m) s[t + 512] = src[t + 512];
+    for (uint32_t stride = kGroup / 2; stride >= 1; stride >>= 1) {
+        __syncthreads();
+        if (t < stride && t + stride < m) s[t] = acc::combine<M>(s[t], s[t + stride]);
+    }
+    if (t == 0) out[(uint64_t)comp * out_stride + g] = s[0];
+}
+
+template<class T>
+__global__ void store_pairs_kernel(const acc::Pair<T>* __restrict__ in, uint32_t in_stride, T* __restrict__ out,
+                                   int ncomp)
+{
+    const int c = threadIdx.x;
+    if (c < ncomp) {
+        out[2 * c] = in[(uint64_t)c * in_stride].h;
+        out[2 * c + 1] = in[(uint64_t)c * in_stride].c;
+    }
+}
+
+int grid_for(uint32_t units)
+{
+    // Enough blocks to fill 256 CUs several times over; units beyond that
+    // are walked grid-stride by the same blocks.
+    const uint32_t cap = (uint32_t)device_cus() * 16u;
+    return (int)std::max<uint32_t>(1u, std::min(units, cap));
+}
+
+size_t tree_scratch_pairs(uint32_t tiles, int ncomp)
+{
+    size_t total = 0;
+    uint32_t t = tiles;
+    total += (size_t)t * ncomp;
+    while (t > 1) {
+        t = (t + kGroup - 1) / kGroup;
+        total += (size_t)t * ncomp;
+    }
+    return total;
+}
+
+// Reduce `ncomp` rows of `tiles` partials (in scratch) to (hi, comp) pairs in `out`.
+template<class T, int M>
+void finish_tree(acc::Pair<T>* scratch, uint32_t tiles, int ncomp, T* out, hipStream_t st)
+{
+    acc::Pair<T>* cur = scratch;
+    uint32_t count = tiles, stride = tiles;
+    acc::Pair<T>* next = scratch + (size_t)tiles * ncomp;
+    while (count > 1) {
+        const uint32_t groups = (count + kGroup - 1) / kGroup;
+        hipLaunchKernelGGL((group_tree_kernel<T, M>), dim3(groups, ncomp), dim3(512), 0, st, cur, next, count, stride,
+                           groups);
+        cur = next;
+        next = next + (size_t)groups * ncomp;
+        stride = groups;
+        count = groups;
+    }
+    hipLaunchKernelGGL((store_pairs_kernel<T>), dim3(1), dim3(64), 0, st, cur, stride, out, ncomp);
+    check(hipGetLastError(), "group tree launch");
+}
+
+template<class T>
+void check_sizes(int ncomp, uint64_t n, uint64_t pitch, const char* op)
+{
+    if (ncomp < 1 || ncomp > 64) fail(op, "ncomp must be in [1, 64]");
+    if (pitch < n) fail(op, "pitch < n");
+    if ((uint64_t)ncomp * pitch > 0xFFFFFFFFull) fail(op, "ncomp*pitch exceeds the 32-bit offset_t span");
+}
+
+template<class T, int M, bool SUM>
+void launch_cfg0(const T* a, const T* b, const T* c, T* d, int ncomp, uint64_t n, uint64_t pitch, T* sums,
+                 hipStream_t st)
+{
+    const uint32_t tiles = n ? (uint32_t)((n + kTile - 1) / kTile) : 1u;
+    const uint32_t units = tiles * (uint32_t)ncomp;
+    const bool vec = aligned16(a) && aligned16(b) && aligned16(c) && aligned16(d) && (pitch * sizeof(T)) % 16 == 0;
+    StreamScratch scratch(SUM ? tree_scratch_pairs(tiles, ncomp) * sizeof(acc::Pair<T>) : 0, st);
+    auto* part = static_cast<acc::Pair<T>*>(scratch.get());
+    const int grid = grid_for(units);
+    if (vec)
+        hipLaunchKernelGGL((cfg0_kernel<T, M, SUM, true>), dim3(grid), dim3(kThreads), 0, st, a, b, c, d, (uint32_t)n,
+                           pitch, tiles, units, part);
+    else
+        hipLaunchKernelGGL((cfg0_kernel<T, M, SUM, false>), dim3(grid), dim3(kThreads), 0, st, a, b, c, d,
+                           (uint32_t)n, pitch, tiles, units, part);
+    check(hipGetLastError(), "cfg0_kernel launch");
+    if constexpr (SUM) finish_tree<T, M>(part, tiles, ncomp, sums, st);
+}
+
+template<class T>
+void dispatch_cfg0(const void* a, const void* b, const void* c, void* d, int ncomp, uint64_t n, uint64_t pitch,
+                   int mode, void* sums, hipStream_t st)
+{
+    auto A = static_cast<const T*>(a), B = static_cast<const T*>(b), Cc = static_cast<const T*>(c);
+    auto D = static_cast<T*>(d);
+    auto S = static_cast<T*>(sums);
+    if (mode != AETH_SUM_NONE && !sums) fail("aeth_eval_cfg0", "sums is NULL");
+    switch (mode) {
+    case AETH_SUM_NONE: launch_cfg0<T, acc::kNeumaier, false>(A, B, Cc, D, ncomp, n, pitch, S, st); break;
+    case AETH_SUM_NEUMAIER: launch_cfg0<T, acc::kNeumaier, true>(A, B, Cc, D, ncomp, n, pitch, S, st); break;
+    case AETH_SUM_PLAIN: launch_cfg0<T, acc::kPlain, true>(A, B, Cc, D, ncomp, n, pitch, S, st); break;
+    case AETH_SUM_FLOATFLOAT:
+        if constexpr (sizeof(T) == 4) {
+            launch_cfg0<T, acc::kFloatFloat, true>(A, B, Cc, D, ncomp, n, pitch, S, st);
+            break;
+        } else {
+            fail("aeth_eval_cfg0", "float-float accumulation needs dtype F32", 0, 2);
+        }
+    default: fail("aeth_eval_cfg0", "unknown sum mode");
+    }
+}
+
+template<class T, int M, bool SQ>
+void launch_reduce(const T* x, int ncomp, uint64_t n, uint64_t pitch, T* out, hipStream_t st)
+{
+    const uint32_t tiles = n ? (uint32_t)((n + kTile - 1) / kTile) : 1u;
+    const uint32_t units = tiles * (uint32_t)ncomp;
+    const bool vec = aligned16(x) && (pitch * sizeof(T)) % 16 == 0;
+    StreamScratch scratch(tree_scratch_pairs(tiles, ncomp) * sizeof(acc::Pair<T>), st);
+    auto* part = static_cast<acc::Pair<T>*>(scratch.get());
+    const int grid = grid_for(units);
+    if (vec)
+        hipLaunchKernelGGL((tile_reduce_kernel<T, M, SQ, true>), dim3(grid), dim3(kThreads), 0, st, x, (uint32_t)n,
+                           pitch, tiles, units, part);
+    else
+        hipLaunchKernelGGL((tile_reduce_kernel<T, M, SQ, false>), dim3(grid), dim3(kThreads), 0, st, x, (uint32_t)n,
+                           pitch, tiles, units, part);
+    check(hipGetLastError(), "tile_reduce_kernel launch");
+    finish_tree<T, M>(part, tiles, ncomp, out, st);
+}
+
+template<class T, int M>
+void dispatch_reduce_sq(const T* x, int ncomp, uint64_t n, uint64_t pitch, int square, T* out, hipStream_t st)
+{
+    if (square)
+        launch_reduce<T, M, true>(x, ncomp, n, pitch, out, st);
+    else
+        launch_reduce<T, M, false>(x, ncomp, n, pitch, out, st);
+}
+
+template<class T>
+void dispatch_reduce(int mode, int square, const void* x, int ncomp, uint64_t n, uint64_t pitch, void* out,
+                     hipStream_t st)
+{
+    auto X = static_cast<const T*>(x);
+    auto O = static_cast<T*>(out);
+    switch (mode) {
+    case AETH_SUM_NEUMAIER: dispatch_reduce_sq<T, acc::kNeumaier>(X, ncomp, n, pitch, square, O, st); break;
+    case AETH_SUM_PLAIN: dispatch_reduce_sq<T, acc::kPlain>(X, ncomp, n, pitch, square, O, st); break;
+    case AETH_SUM_FLOATFLOAT:
+        if constexpr (sizeof(T) == 4) {
+            dispatch_reduce_sq<T, acc::kFloatFloat>(X, ncomp, n, pitch, square, O, st);
+            break;
+        } else {
+            fail("aeth_reduce", "float-float accumulation needs dtype F32", 0, 2);
+        }
+    default: fail("aeth_reduce", "unknown mode");
+    }
+}
+
+// e = cross(a,b).dot(c) + sqrt(a.dot(a) + 1), two samples per lane per step.
+template<bool VECTOR>
+__global__ __launch_bounds__(kThreads) void vec3geo_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                           const double* __restrict__ c, double* __restrict__ e,
+                                                           uint32_t n, uint64_t pitch)
+{
+    const uint32_t pairs = (n + 1) / 2;
+    for (uint32_t p = blockIdx.x * kThreads + threadIdx.x; p < pairs; p += gridDim.x * kThreads) {
+        const uint32_t i0 = 2 * p;
+        double A[3][2], B[3][2], Cv[3][2];
+        const bool full = VECTOR && i0 + 1 < n;
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                vload(a + (uint64_t)k * pitch + i0, A[k]);
+                vload(b + (uint64_t)k * pitch + i0, B[k]);
+                vload(c + (uint64_t)k * pitch + i0, Cv[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const uint32_t i = i0 + j;
+                    const bool ok = i < n;
+                    A[k][j] = ok ? a[(uint64_t)k * pitch + i] : 0.0;
+                    B[k][j] = ok ? b[(uint64_t)k * pitch + i] : 0.0;
+                    Cv[k][j] = ok ? c[(uint64_t)k * pitch + i] : 0.0;
+                }
+        }
+        double out[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const double x0 = A[1][j] * B[2][j] - A[2][j] * B[1][j];
+            const double x1 = A[2][j] * B[0][j] - A[0][j] * B[2][j];
+            const double x2 = A[0][j] * B[1][j] - A[1][j] * B[0][j];
+            const double dxc = (x2 * Cv[2][j]) + ((x1 * Cv[1][j]) + (x0 * Cv[0][j]));
+            const double daa = (A[2][j] * A[2][j]) + ((A[1][j] * A[1][j]) + (A[0][j] * A[0][j]));
+            out[j] = dxc + __builtin_sqrt(daa + 1.0);
+        }
+        if (full) {
+            vstore(e + i0, out);
+        } else {
+            if (i0 < n) e[i0] = out[0];
+            if (i0 + 1 < n) e[i0 + 1] = out[1];
+        }
+    }
+}
+
+} // namespace
+} // namespace aeth
+
+using namespace aeth;
+
+extern "C" int aeth_eval_cfg0(int dtype, const void* a, const void* b, const void* c, void* d, int ncomp, uint64_t n,
+                              uint64_t pitch, int sum_mode, void* sums, void* stream)
+{
+    return guarded([&] {
+        if (dtype == AETH_F64) {
+            check_sizes<double>(ncomp, n, pitch, "aeth_eval_cfg0");
+            dispatch_cfg0<double>(a, b, c, d, ncomp, n, pitch, sum_mode, sums, as_stream(stream));
+        } else if (dtype == AETH_F32) {
+            check_sizes<float>(ncomp, n, pitch, "aeth_eval_cfg0");
+            dispatch_cfg0<float>(a, b, c, d, ncomp, n, pitch, sum_mode, sums, as_stream(stream));
+        } else {
+            fail("aeth_eval_cfg0", "unknown dtype");
+        }
+    });
+}
+
+extern "C" int aeth_reduce(int dtype, int mode, int square, const void* x, int ncomp, uint64_t n, uint64_t pitch,
+                           void* out, void* stream)
+{
+    return guarded([&] {
+        if (!out) fail("aeth_reduce", "out is NULL");
+        if (dtype == AETH_F64) {
+            check_sizes<double>(ncomp, n, pitch, "aeth_reduce");
+            dispatch_reduce<double>(mode, square, x, ncomp, n, pitch, out, as_stream(stream));
+        } else if (dtype == AETH_F32) {
+            check_sizes<float>(ncomp, n, pitch, "aeth_reduce");
+            dispatch_reduce<float>(mode, square, x, ncomp, n, pitch, out, as_stream(stream));
+        } else {
+            fail("aeth_reduce", "unknown dtype");
+        }
+    });
+}
+
+extern "C" int aeth_eval_vec3geo(const void* a, const void* b, const void* c, void* e, uint64_t n, uint64_t pitch,
+                                 void* stream)
+{
+    return guarded([&] {
+        check_sizes<double>(3, n, pitch, "aeth_eval_vec3geo");
+        if (n == 0) return;
+        auto A = static_cast<const double*>(a), B = static_cast<const double*>(b), C = static_cast<const double*>(c);
+        auto E = static_cast<double*>(e);
+        const bool vec = aligned16(a) && aligned16(b) && aligned16(c) && aligned16(e) && pitch % 2 == 0;
+        const uint32_t pairs = (uint32_t)((n + 1) / 2);
+        const int grid = (int)std::min<uint32_t>((pairs + kThreads - 1) / kThreads, (uint32_t)device_cus() * 32u);
+        hipStream_t st = as_stream(stream);
+        if (vec)
+            hipLaunchKernelGGL((vec3geo_kernel<true>), dim3(grid), dim3(kThreads), 0, st, A, B, C, E, (uint32_t)n,
+                               pitch);
+        else
+            hipLaunchKernelGGL((vec3geo_kernel<false>), dim3(grid), dim3(kThreads), 0, st, A, B, C, E, (uint32_t)n,
+                               pitch);
+        check(hipGetLastError(), "vec3geo_kernel launch");
+    });
+}

@@ -1,0 +1,98 @@
+/*
+ * aether_hip.h -- the C ABI of libaether_hip.so, the MI355X (gfx950) engine
+ * behind aether's data-parallel hot path.
+ *
+ * Conventions (the reference's C seam, dsc/src/seam.h:1-57 and
+ * dsc/src/plugin.cpp:22-39): every entry returns 0 on success, 1 on failure
+ * with a per-thread message in aeth_last_error(), 2 when the request is valid
+ * but this build does not support it. `stream` is a raw hipStream_t (NULL =
+ * the null stream); every launch is stream-ordered and returns without
+ * synchronising. Device pointers are plain pointers; SoA arrays are `ncomp`
+ * planes of `pitch` elements, element (c, i) at c*pitch + i
+ * (array/Array.h:80,100-110,279-295). Sample counts are limited by the
+ * reference's 32-bit offset_t (index/Offset.h:73-91): ncomp*pitch <= 2^32-1.
+ * No torch types; callable from C, C++, ctypes.
+ */
+#ifndef AETHER_HIP_H
+#define AETHER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AETH_ABI_VERSION 1
+
+enum { AETH_F64 = 0, AETH_F32 = 1 };
+
+/* ------------------------------------------------------------------ errors */
+/* dsc/src/plugin.cpp:22-39 (thread-local last_error) */
+const char* aeth_last_error(void);
+int aeth_abi_version(void);
+/* Number of HIP devices; 0 only if a driver answers and >= 1 device exists
+ * (dsc/src/seam.h:24 probe). */
+int aeth_probe(int* device_count);
+
+/* ------------------------------------------------------------------ RNG */
+/* Philox4x32-10 streams (random/detail/Backend.h:144-236). Element k of the
+ * fill is element first+k of the stream:
+ *   BITS32      : word (j & 1) of block(seed, j >> 1, 0)  -- randomBits64's lo then hi
+ *   UNIFORM_F64 : uniform01<double>(seed, j, 0)            (Backend.h:184-195)
+ *   UNIFORM_F32 : uniform01<float>(seed, j, 0)
+ *   NORMAL_F64  : standardNormal<double>(seed, j, 0)       (Backend.h:209-236)
+ *   NORMAL_F32  : standardNormal<float>(seed, j, 0)
+ * j must stay below 2^32 (BITS32: 2^33). `out` must be 16-byte aligned for
+ * the vector path (any alignment works, slower). */
+enum { AETH_RNG_BITS32 = 0, AETH_RNG_UNIFORM_F64 = 1, AETH_RNG_UNIFORM_F32 = 2, AETH_RNG_NORMAL_F64 = 3,
+       AETH_RNG_NORMAL_F32 = 4 };
+int aeth_rng_fill(int kind, uint64_t seed, uint64_t first, uint64_t n, void* out, void* stream);
+
+/* Array fill, the lowering of `view[i] = leaf` for aether's distribution
+ * leaves (random/detail/DistributionLeaf.h:111-236): plane c, sample i gets
+ *   UNIFORM: p0 + (p1 - p0) * uniform01(seed, i, c)
+ *   NORMAL : p0 + p1 * standardNormal(seed, i, c)
+ * `seed` is the leaf's seed after salting: Generator(s).uniform(...) draws
+ * with seed = key ^ 0x243F6A8885A308D3, normal with key ^ 0x13198A2E03707344
+ * (random/Generator.h:64-67, DistributionLeaf.h:117,184); the raw backend
+ * stream ("key = seed") passes the seed itself. */
+enum { AETH_DIST_UNIFORM = 0, AETH_DIST_NORMAL = 1 };
+int aeth_array_fill(int dist, int dtype, uint64_t seed, double p0, double p1, int ncomp, uint64_t n,
+                    uint64_t pitch, void* out, void* stream);
+
+/* ------------------------------------------------------------------ fused evaluation */
+/* Compensated accumulation modes (accum/atomic.h:142-188, banded/Ff2.h:132-145). */
+enum { AETH_SUM_NONE = -1, AETH_SUM_NEUMAIER = 0, AETH_SUM_PLAIN = 1, AETH_SUM_FLOATFLOAT = 2 };
+
+/* d = a + b*c + sin(a)*exp(-b*b) per component, one pass, fused with an
+ * optional per-component sum of d in `sum_mode`. The lowering of
+ *   dv[i] = av + bv*cv + cwiseSin(av)*cwiseExp(-1.0*(bv*bv))
+ * (expr/Assign.h:124-160,264-271; expr/Operations.h:25-71;
+ * expr/nodes/Elementwise.h:291-357,513-533) over whole arrays.
+ * sums (device memory, may be NULL when sum_mode is NONE): 2*ncomp values of
+ * dtype, (hi, comp) per component; the sum is hi + comp. */
+int aeth_eval_cfg0(int dtype, const void* a, const void* b, const void* c, void* d, int ncomp, uint64_t n,
+                   uint64_t pitch, int sum_mode, void* sums, void* stream);
+
+/* e = cross(a, b).dot(c) + sqrt(a.dot(a) + 1) over Array<double,3> a, b, c
+ * (pitch `pitch`) into a scalar plane e (expr/nodes/Geometric.h:32-73,
+ * expr/Expression.h:162-173, expr/Reduce.h:150-190, math/math.h:132). */
+int aeth_eval_vec3geo(const void* a, const void* b, const void* c, void* e, uint64_t n, uint64_t pitch,
+                      void* stream);
+
+/* Per-component sum (square = 0) or sum of squares (square = 1) of an SoA
+ * array in the fixed reduction shape; out: 2*ncomp values (hi, comp|lo). */
+int aeth_reduce(int dtype, int mode, int square, const void* x, int ncomp, uint64_t n, uint64_t pitch, void* out,
+                void* stream);
+
+/* Elementwise math (math/math.h:228-287, math/explog.h:39,61).
+ * fn: 0 sin, 1 cos, 2 exp, 3 log: y[i] = fn(x[i]).
+ * fn 4 (fused stress form): y holds 4 planes of n: sin, cos, exp, log of x. */
+enum { AETH_FN_SIN = 0, AETH_FN_COS = 1, AETH_FN_EXP = 2, AETH_FN_LOG = 3, AETH_FN_ALL4 = 4 };
+int aeth_math(int fn, int dtype, const void* x, void* y, uint64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

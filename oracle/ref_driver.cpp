@@ -1,0 +1,222 @@
+// ref_driver.cpp -- drives the REFERENCE's own header-only CPU path.
+// TEST INFRASTRUCTURE ONLY. Compiled by oracle/Makefile (target `ref`) against
+// the reference headers where they lie (-I <aether source tree>,
+// -DAETHER_CPP_MODE=1); the binary goes to oracle/_ref/ and is used to mint
+// small golden vectors (tests/golden/mint_ref_vectors.py) and, on the GPU
+// host, as the timed CPU baseline of the fused FP64 expression.
+//
+// Modes (all output is raw little-endian binary on stdout unless noted):
+//   cfg0   <n> <seedA> <seedB> <seedC>  : a,b,c,d (3 planes x n each, pitch = n
+//                                         compacted) then 3 tree sums of d
+//   cfg0g  <n> <seed>                   : same, inputs from Generator(seed).uniform
+//   cfg0f  <n> <seedA> <seedB> <seedC>  : the float variant
+//   vec3   <n> <seedA> <seedB> <seedC>  : e (n doubles)
+//   keys   <seed> <streamId> <k>        : generator key, substream key (text)
+//   rows   <seed> <n> <counter>         : per global i<n: bits64, u01d bits,
+//                                         u01f bits, normal d, normal f
+//   fft    <n> <seed> <fwd>             : pocketfft c2c of a length-n signal
+//                                         (entries u01-derived), in then out
+//   bench  <n> <reps>                   : times the OpenMP fused loop (text)
+#include <aether/aether.h>
+#include <aether/fft/Fft.h>
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <omp.h>
+
+using aether::SampleIndex;
+namespace rnd = aether::random::detail;
+namespace at = aether::accum::atomic::detail;
+
+namespace {
+
+// The raw-seed uniform leaf: key = seed ^ salt so the leaf draws u01(seed, i, c).
+template<class T>
+auto rawUniform(std::uint64_t seed)
+{
+    using Leaf = rnd::UniformLeaf<T, 3, T, T>;
+    return Leaf(seed ^ Leaf::salt, T(-1), T(1));
+}
+
+template<class T>
+void writePlanes(const aether::Array<T, 3>& a, std::size_t n)
+{
+    auto v = a.hostView();
+    for (std::size_t c = 0; c < 3; ++c)
+        for (std::size_t i = 0; i < n; ++i) {
+            const T x = v(c, i);
+            std::fwrite(&x, sizeof(T), 1, stdout);
+        }
+}
+
+template<class T>
+T treeSum(const aether::Array<T, 3>& d, std::size_t n, std::size_t c)
+{
+    auto v = d.hostView();
+    T h[32] = {}, cc[32] = {};
+    for (std::size_t i = 0; i < n; ++i)
+        at::neumaierAdd_(h[i % 32], cc[i % 32], static_cast<T>(v(c, i)));
+    T oh, oc;
+    at::reduce32Host_(h, cc, oh, oc);
+    return oh + oc;
+}
+
+template<class T, class LA, class LB, class LC>
+void cfg0(std::size_t n, const LA& la, const LB& lb, const LC& lc)
+{
+    aether::Array<T, 3> a(n), b(n), c(n), d(n);
+    auto av = a.hostView();
+    auto bv = b.hostView();
+    auto cv = c.hostView();
+    auto dv = d.hostView();
+#pragma omp parallel for schedule(static)
+    for (std::ptrdiff_t k = 0; k < static_cast<std::ptrdiff_t>(n); ++k) {
+        const auto i = SampleIndex::make(static_cast<std::size_t>(k));
+        av[i] = la;
+        bv[i] = lb;
+        cv[i] = lc;
+    }
+#pragma omp parallel for schedule(static)
+    for (std::ptrdiff_t k = 0; k < static_cast<std::ptrdiff_t>(n); ++k) {
+        const auto i = SampleIndex::make(static_cast<std::size_t>(k));
+        dv[i] = av + bv * cv + aether::cwiseSin(av) * aether::cwiseExp(T(-1) * (bv * bv));
+    }
+    writePlanes(a, n);
+    writePlanes(b, n);
+    writePlanes(c, n);
+    writePlanes(d, n);
+    for (std::size_t k = 0; k < 3; ++k) {
+        const T s = treeSum(d, n, k);
+        std::fwrite(&s, sizeof(T), 1, stdout);
+    }
+}
+
+void vec3(std::size_t n, std::uint64_t sa, std::uint64_t sb, std::uint64_t sc)
+{
+    aether::Array<double, 3> a(n), b(n), c(n);
+    aether::Array<double> e(n);
+    auto av = a.hostView();
+    auto bv = b.hostView();
+    auto cv = c.hostView();
+    auto ev = e.hostView();
+    for (std::size_t k = 0; k < n; ++k) {
+        const auto i = SampleIndex::make(k);
+        av[i] = rawUniform<double>(sa);
+        bv[i] = rawUniform<double>(sb);
+        cv[i] = rawUniform<double>(sc);
+    }
+    for (std::size_t k = 0; k < n; ++k) {
+        const auto i = SampleIndex::make(k);
+        const auto ai = av[i].get();
+        const auto bi = bv[i].get();
+        const auto ci = cv[i].get();
+        const double r = ai.cross(bi).dot(ci) + aether::math::sqrt(ai.dot(ai) + 1.0);
+        ev(k) = r;
+        std::fwrite(&r, sizeof(double), 1, stdout);
+    }
+}
+
+void rows(std::uint64_t seed, std::size_t n, std::uint64_t counter)
+{
+    for (std::size_t g = 0; g < n; ++g) {
+        const auto gi = static_cast<aether::offset_t>(g);
+        const std::uint64_t b = rnd::randomBits64(seed, gi, counter);
+        const double ud = rnd::uniform01<double>(seed, gi, counter);
+        const float uf = rnd::uniform01<float>(seed, gi, counter);
+        const double nd = rnd::standardNormal<double>(seed, gi, counter);
+        const float nf = rnd::standardNormal<float>(seed, gi, counter);
+        std::fwrite(&b, 8, 1, stdout);
+        std::fwrite(&ud, 8, 1, stdout);
+        std::fwrite(&uf, 4, 1, stdout);
+        std::fwrite(&nf, 4, 1, stdout);
+        std::fwrite(&nd, 8, 1, stdout);
+    }
+}
+
+void fft(std::size_t n, std::uint64_t seed, bool forward)
+{
+    std::vector<double> in(2 * n), out(2 * n);
+    for (std::size_t i = 0; i < n; ++i) {
+        in[2 * i] = rnd::standardNormal<double>(seed, static_cast<aether::offset_t>(i), 0);
+        in[2 * i + 1] = rnd::standardNormal<double>(seed, static_cast<aether::offset_t>(i), 1);
+    }
+    const std::size_t shape[1] = { n };
+    const std::size_t axes[1] = { 0 };
+    aether::fft::c2c<double>(in.data(), out.data(), shape, axes, forward, aether::fft::Norm::Backward);
+    std::fwrite(in.data(), 8, 2 * n, stdout);
+    std::fwrite(out.data(), 8, 2 * n, stdout);
+}
+
+void bench(std::size_t n, int reps)
+{
+    aether::Array<double, 3> a(n), b(n), c(n), d(n);
+    auto av = a.hostView();
+    auto bv = b.hostView();
+    auto cv = c.hostView();
+    auto dv = d.hostView();
+    const auto la = rawUniform<double>(42), lb = rawUniform<double>(43), lc = rawUniform<double>(44);
+#pragma omp parallel for schedule(static)
+    for (std::ptrdiff_t k = 0; k < static_cast<std::ptrdiff_t>(n); ++k) {
+        const auto i = SampleIndex::make(static_cast<std::size_t>(k));
+        av[i] = la;
+        bv[i] = lb;
+        cv[i] = lc;
+        dv[i] = la; // first touch of d by the same thread layout
+    }
+    std::vector<double> ms;
+    double sums[3] = {0, 0, 0};
+    for (int r = 0; r <= reps; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(static)
+        for (std::ptrdiff_t k = 0; k < static_cast<std::ptrdiff_t>(n); ++k) {
+            const auto i = SampleIndex::make(static_cast<std::size_t>(k));
+            dv[i] = av + bv * cv + aether::cwiseSin(av) * aether::cwiseExp(-1.0 * (bv * bv));
+        }
+        for (std::size_t k = 0; k < 3; ++k) sums[k] = treeSum(d, n, k);
+        const auto t1 = std::chrono::steady_clock::now();
+        if (r > 0) ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+    }
+    std::sort(ms.begin(), ms.end());
+    std::printf("{\"n\": %zu, \"threads\": %d, \"median_ms\": %.6f, \"min_ms\": %.6f, \"sum0\": %.17g}\n", n,
+        omp_get_max_threads(), ms[ms.size() / 2], ms[0], sums[0]);
+}
+
+} // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) return 2;
+    const std::string mode = argv[1];
+    auto u = [&](int k) { return static_cast<std::uint64_t>(std::strtoull(argv[k], nullptr, 0)); };
+    if (mode == "cfg0" && argc == 6) {
+        cfg0<double>(u(2), rawUniform<double>(u(3)), rawUniform<double>(u(4)), rawUniform<double>(u(5)));
+    } else if (mode == "cfg0f" && argc == 6) {
+        cfg0<float>(u(2), rawUniform<float>(u(3)), rawUniform<float>(u(4)), rawUniform<float>(u(5)));
+    } else if (mode == "cfg0g" && argc == 4) {
+        const aether::random::Generator g(u(3));
+        const auto leaf = g.uniform<double, 3>(-1.0, 1.0);
+        cfg0<double>(u(2), leaf, g.substream(1).uniform<double, 3>(-1.0, 1.0),
+            g.substream(2).uniform<double, 3>(-1.0, 1.0));
+    } else if (mode == "vec3" && argc == 6) {
+        vec3(u(2), u(3), u(4), u(5));
+    } else if (mode == "keys" && argc == 5) {
+        const aether::random::Generator g(u(2), u(3));
+        std::printf("%llu %llu\n", static_cast<unsigned long long>(g.key()),
+            static_cast<unsigned long long>(g.substream(u(4)).key()));
+    } else if (mode == "rows" && argc == 5) {
+        rows(u(2), u(3), u(4));
+    } else if (mode == "fft" && argc == 5) {
+        fft(u(2), u(3), u(4) != 0);
+    } else if (mode == "bench" && argc == 4) {
+        bench(u(2), static_cast<int>(u(3)));
+    } else {
+        return 2;
+    }
+    return 0;
+}
