@@ -1,0 +1,138 @@
+"""bench.py -- fused SoA evaluation throughput on one MI355X.
+
+Workload (one step): d = a + b*c + sin(a)*exp(-b*b) over three
+Array<double,3> a, b, c of 2^27 samples (Philox4x32-10 uniforms on (-1, 1),
+seeds 42/43/44, counter = component), fused with a compensated (Neumaier)
+sum of d per component -- one pass, one kernel plus a tiny tree.
+Inputs are resident in HBM before the timed region. Algorithmic bytes per
+step: 3 components x 2^27 samples x 32 B (3 loads + 1 store of a double).
+
+Prints ONE JSON line. `roofline.achieved` is the algorithmic bytes of one
+launch over its average duration, timed with HIP events on the launch
+stream; `cpu_baseline` runs the reference's own OpenMP CPU path (built from
+its sources into oracle/_ref) -- or the oracle's restatement when that build
+is absent -- on a bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fused SoA eval GB/s (% HBM peak); served jobs/s + p50/p99 at stated err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--log2n", type=int, default=27)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-log2n", type=int, default=24)
+    return p.parse_args()
+
+
+def cpu_baseline(log2n: int) -> dict:
+    """The reference's fused loop + its canonical compensated sums, on the host cores."""
+    n = 1 << log2n
+    cores = min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    ref = os.path.join(ROOT, "oracle", "_ref", "aether_ref")
+    if os.path.exists(ref):
+        out = subprocess.run([ref, "bench", str(n), "5"], capture_output=True, text=True, env=env, timeout=600,
+                             check=True).stdout
+        r = json.loads(out)
+        ms = r["median_ms"]
+        return {"value": round(3 * n * 32 / (ms * 1e-3) / 1e9, 3), "unit": "GB/s", "cores": r["threads"],
+                "kind": "reference",
+                "sample": f"2^{log2n} samples x 3 components, median of 5 after 1 warm-up; eval + 3 tree sums "
+                          f"(eval alone {r['eval_median_ms']:.1f} ms of {ms:.1f} ms)"}
+    import numpy as np
+
+    from oracle import pyoracle as O
+
+    a = O.fill_array_uniform(42, 3, n, n)
+    b = O.fill_array_uniform(43, 3, n, n)
+    c = O.fill_array_uniform(44, 3, n, n)
+    ts = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        d = O.eval_cfg0(a, b, c, 3, n, n, mathmode=0, nthreads=cores)
+        for k in range(3):
+            O.tree_sum32(np.ascontiguousarray(d[k * n:(k + 1) * n]))
+        ts.append(time.perf_counter() - t0)
+    ms = sorted(ts[1:])[1] * 1e3
+    return {"value": round(3 * n * 32 / (ms * 1e-3) / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"2^{log2n} samples x 3 components, median of 3 after 1 warm-up"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(the sample axis shards per GPU with no exchange except 3 sum pairs)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import aether_amd as ae
+
+    n = 1 << args.log2n
+    a, b, c, d = (ae.Array(n, 3, torch.float64) for _ in range(4))
+    for arr, s in ((a, 42), (b, 43), (c, 44)):
+        ae.array_fill_uniform(arr, s, -1.0, 1.0)
+    sums = torch.empty((3, 2), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        ae.eval_cfg0(a, b, c, d, ae.SUM_NEUMAIER, sums)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        e0.record()
+        ae.eval_cfg0(a, b, c, d, ae.SUM_NEUMAIER, sums)
+        e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = sorted(e0.elapsed_time(e1) for e0, e1 in ev)
+    avg_launch_ms = sum(launch_ms) / len(launch_ms)
+    bytes_step = 3 * n * 32
+    ms_per_step = wall * 1e3 / args.steps
+    value = bytes_step / (wall / args.steps) / 1e9
+    achieved = bytes_step / (avg_launch_ms * 1e-3) / 1e9
+    s = sums.cpu()
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: Philox4x32-10 uniforms on (-1,1), seeds 42/43/44, counter = component",
+        "config": {"workload": "fused FP64 d = a + b*c + sin(a)*exp(-b*b) + Neumaier sums of d",
+                   "samples": n, "components": 3, "arrays": 4, "bytes_per_step": bytes_step},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "launch_ms": {"min": round(launch_ms[0], 4), "median": round(launch_ms[len(launch_ms) // 2], 4),
+                      "avg": round(avg_launch_ms, 4)},
+        "sums": [s[k, 0].item() + s[k, 1].item() for k in range(3)],
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_log2n)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

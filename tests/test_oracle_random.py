@@ -1,0 +1,113 @@
+"""Pins the oracle's RNG against the reference's golden data.
+
+Fixtures: tests/golden/random_core_rows.json and random_key_rows.json (the
+rows of the reference's tests/random/golden_random.h), ref_rows_*.npz and
+ref_keys.npz (minted from the reference's own CPU path), and the published
+Random123 known answers for philox4x32-10.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+def _core():
+    return json.load(open(os.path.join(GOLDEN, "random_core_rows.json")))
+
+
+def test_philox_known_answers(orc):
+    # Random123 kat_vectors, philox4x32-10 (Salmon et al., SC'11)
+    assert orc.philox((0, 0, 0, 0), 0, 0) == (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)
+    assert orc.philox((0xFFFFFFFF,) * 4, 0xFFFFFFFF, 0xFFFFFFFF) == (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)
+    assert orc.philox((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), 0xA4093822, 0x299F31D0) == (
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)
+
+
+def test_golden_core_rows_exact(orc):
+    # tests/test_Random.cpp:356-388 (Golden_BitsUniformExactBothModes)
+    data = _core()
+    assert len(data["rows"]) == 315
+    for r in data["rows"]:
+        s, g, c = r["seed"], r["global"], r["counter"]
+        assert orc.bits64(s, g, c) == r["bits64"]
+        assert int(np.float64(orc.u01_f64(s, g, c)).view(np.uint64)) == r["u01d_bits"]
+        assert int(np.float32(orc.u01_f32(s, g, c)).view(np.uint32)) == r["u01f_bits"]
+
+
+@pytest.mark.parametrize("mathmode", [0, 1])
+def test_golden_normals_within_tolerance(orc, mathmode):
+    # tests/test_Random.cpp:390-407: abs AND rel band, 1e-12 (double) / 1e-5 (float)
+    data = _core()
+    tol = data["tolerance"]
+    for r in data["rows"]:
+        s, g, c = r["seed"], r["global"], r["counter"]
+        nd = orc.normal_f64(s, g, c, mathmode)
+        nf = orc.normal_f32(s, g, c, mathmode)
+        assert abs(nd - r["normal_d"]) <= tol["normal_d"] * max(1.0, abs(r["normal_d"]))
+        assert abs(nf - r["normal_f"]) <= tol["normal_f"] * max(1.0, abs(r["normal_f"]))
+
+
+def test_golden_key_rows(orc):
+    rows = json.load(open(os.path.join(GOLDEN, "random_key_rows.json")))["rows"]
+    assert len(rows) == 45
+    for r in rows:
+        k = orc.generator_key(r["seed"], r["stream"])
+        assert k == r["key"]
+        assert orc.substream_key(k, r["k"]) == r["sub_key"]
+
+
+def test_reference_minted_keys(orc):
+    rows = np.load(os.path.join(GOLDEN, "ref_keys.npz"))["rows"]
+    for seed, stream, k, key, sub in rows.tolist():
+        got = orc.generator_key(seed, stream)
+        assert got == key
+        assert orc.substream_key(got, k) == sub
+
+
+@pytest.mark.parametrize("seed,counter", [(42, 0), (42, 3), (7, 0), (7, 3)])
+def test_reference_minted_rows(orc, seed, counter):
+    z = np.load(os.path.join(GOLDEN, f"ref_rows_s{seed}_c{counter}.npz"))
+    n = len(z["bits"])
+    bits = np.array([orc.bits64(seed, g, counter) for g in range(n)], np.uint64)
+    assert np.array_equal(bits, z["bits"])
+    ud = np.array([orc.u01_f64(seed, g, counter) for g in range(n)])
+    assert np.array_equal(ud.view(np.uint64), z["ud"].view(np.uint64))
+    uf = np.array([orc.u01_f32(seed, g, counter) for g in range(n)], np.float32)
+    assert np.array_equal(uf.view(np.uint32), z["uf"].view(np.uint32))
+    # the libm route reproduces the reference's host normals bit for bit (same glibc)
+    nd0 = np.array([orc.normal_f64(seed, g, counter, 0) for g in range(n)])
+    assert np.max(np.abs(nd0 - z["nd"]) / np.maximum(1, np.abs(z["nd"]))) <= 1e-15
+    nd1 = np.array([orc.normal_f64(seed, g, counter, 1) for g in range(n)])
+    assert np.max(np.abs(nd1 - z["nd"]) / np.maximum(1, np.abs(z["nd"]))) <= 1e-12
+    nf1 = np.array([orc.normal_f32(seed, g, counter, 1) for g in range(n)], np.float32)
+    assert np.max(np.abs(nf1.astype(np.float64) - z["nf"]) / np.maximum(1, np.abs(z["nf"]))) <= 1e-5
+
+
+def test_streams_and_word_layout(orc):
+    # BITS32 word j is word (j & 1) of block(seed, j >> 1, 0): lo then hi of randomBits64
+    w = orc.fill_bits32(42, 64)
+    for j in range(0, 64, 2):
+        b = orc.bits64(42, j // 2, 0)
+        assert int(w[j]) == b & 0xFFFFFFFF and int(w[j + 1]) == b >> 32
+    u = orc.fill_uniform_f64(42, 16, first=5)
+    assert all(u[k] == orc.u01_f64(42, 5 + k, 0) for k in range(16))
+
+
+def test_uniform_open_interval_extremes():
+    # Backend.h:123-134: w*2^-32 + 2^-33 and (2(w>>9)+1)*2^-24 never reach 0 or 1
+    for w in (0, 1, 0xFFFFFFFF, 0xFFFFFE00):
+        d = w * 2.0**-32 + 2.0**-33
+        f = np.float32(((w >> 9) << 1) | 1) * np.float32(2.0**-24)
+        assert 0.0 < d < 1.0 and np.float32(0) < f < np.float32(1)
+
+
+def test_bits_moments(orc):
+    # a brute statistical check on a bounded sample (tests/test_Random.cpp moments style)
+    u = orc.fill_uniform_f64(42, 1 << 16)
+    assert abs(u.mean() - 0.5) < 6 * (1 / np.sqrt(12)) / np.sqrt(u.size)
+    z = orc.fill_normal_f64(42, 1 << 16)
+    assert abs(z.mean()) < 6 / np.sqrt(z.size)
+    assert abs(z.var() - 1) < 6 * np.sqrt(2 / z.size)
