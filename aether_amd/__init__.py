@@ -19,7 +19,7 @@ from ._lib import AetherError, call, lib
 __all__ = [
     "AetherError", "Array", "Generator", "rng_fill", "array_fill_uniform", "array_fill_normal", "eval_cfg0",
     "eval_vec3geo", "reduce", "math", "stream_handle", "F64", "F32", "SUM_NONE", "SUM_NEUMAIER", "SUM_PLAIN",
-    "SUM_FLOATFLOAT", "RNG_BITS32", "RNG_UNIFORM_F64", "RNG_UNIFORM_F32", "RNG_NORMAL_F64", "RNG_NORMAL_F32",
+    "SUM_FLOATFLOAT", "FftBatch", "RNG_BITS32", "RNG_UNIFORM_F64", "RNG_UNIFORM_F32", "RNG_NORMAL_F64", "RNG_NORMAL_F32",
 ]
 
 F64, F32 = 0, 1
@@ -159,3 +159,42 @@ def math(fn: str, x: torch.Tensor, y: torch.Tensor = None, stream=None) -> torch
         y = torch.empty((4 * n,) if fn == "all4" else (n,), dtype=x.dtype, device=x.device)
     call("aeth_math", FN[fn], _dtype_code(x.dtype), x.data_ptr(), y.data_ptr(), n, stream_handle(stream))
     return y
+
+
+class FftBatch:
+    """A batch of uneven power-of-two complex FFTs run as one launch (aeth_fft_batch_*).
+
+    prec: 0 complex128, 1 complex64, 2 complex64 with float-float arithmetic.
+    lengths / offsets are in complex elements of the flat input/output buffers.
+    """
+
+    def __init__(self, lengths, in_offsets=None, out_offsets=None, prec: int = 0):
+        import ctypes as C
+
+        import numpy as np
+
+        L = np.ascontiguousarray(lengths, np.uint64)
+        if in_offsets is None:
+            in_offsets = np.concatenate([[0], np.cumsum(L)[:-1]]).astype(np.uint64) if len(L) else L
+        I = np.ascontiguousarray(in_offsets, np.uint64)
+        O = np.ascontiguousarray(out_offsets if out_offsets is not None else I, np.uint64)
+        self.prec = prec
+        self.total = int(L.sum())
+        h = C.c_void_p()
+        call("aeth_fft_batch_plan", prec, len(L), L.ctypes.data, I.ctypes.data, O.ctypes.data, C.byref(h))
+        self._h = h
+
+    def __call__(self, x: torch.Tensor, out: torch.Tensor, forward: bool = True, norm: int = 0, stream=None):
+        call("aeth_fft_batch_exec", self._h, x.data_ptr(), out.data_ptr(), int(forward), norm, stream_handle(stream))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            call("aeth_fft_batch_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

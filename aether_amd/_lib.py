@@ -29,6 +29,9 @@ SIGNATURES = {
     "aeth_eval_vec3geo": (_i32, [_vp, _vp, _vp, _vp, _u64, _u64, _vp]),
     "aeth_reduce": (_i32, [_i32, _i32, _i32, _vp, _i32, _u64, _u64, _vp, _vp]),
     "aeth_math": (_i32, [_i32, _i32, _vp, _vp, _u64, _vp]),
+    "aeth_fft_batch_plan": (_i32, [_i32, _u64, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "aeth_fft_batch_exec": (_i32, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    "aeth_fft_batch_destroy": (_i32, [_vp]),
 }
 
 

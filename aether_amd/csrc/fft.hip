@@ -1,0 +1,445 @@
+// fft.hip -- batched power-of-two complex FFTs, LDS-resident, one launch.
+//
+// The reference's host FFT is pocketfft (fft/Fft.h:98-236) and its GPU FFT is
+// cuFFT behind the plugin seam (dsc/src/cufft_plan.h:122-320). This is a
+// from-scratch gfx950 design: every transform of up to 8192 points lives in
+// LDS for its whole life -- read from HBM once, written once. A work
+// descriptor names a chunk of signals of one length (as many as fit in 8192
+// points); one workgroup of 1024 threads runs the Stockham autosort passes
+// (radix 4, a final radix 2 when log2 N is odd) on all signals of its chunk
+// at once, so small and large transforms are handled by the same launch and
+// every pass has 2048 (radix 4) or 4096 (radix 2) butterflies in flight.
+// LDS rows are padded by one element every 16 so the stride-4 writes of the
+// first passes do not collide in the banks.
+//
+// Precisions: complex128; complex64; complex64 with float-float (Ff2-style,
+// banded/Ff2.h:132-145,246-321) accumulation, stored as complex64. Twiddles
+// come from one FP64 table exp(-2 pi i m / 8192) generated on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/aether_hip.h"
+#include "accum.hpp"
+#include "common.hpp"
+#include "dmath.hpp"
+
+namespace aeth {
+namespace fft {
+
+constexpr int kMaxLog2 = 13;
+constexpr int kMaxN = 1 << kMaxLog2;
+constexpr int kThreads = 1024;
+
+__host__ __device__ constexpr uint32_t pad(uint32_t e) { return e + (e >> 4); }
+constexpr uint32_t kLdsElems = pad(kMaxN);
+
+struct Desc {
+    uint32_t first_sig; // index into the signal table
+    uint16_t count;     // signals in this chunk
+    uint16_t log2n;
+};
+
+struct Sig {
+    uint64_t in_off;  // complex elements
+    uint64_t out_off; // complex elements
+};
+
+// ---- working types --------------------------------------------------------
+struct C128 {
+    double re, im;
+};
+struct C64 {
+    float re, im;
+};
+struct CFF {
+    float rh, rl, ih, il;
+};
+
+__device__ __forceinline__ C128 add(C128 a, C128 b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ C128 sub(C128 a, C128 b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ C64 add(C64 a, C64 b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ C64 sub(C64 a, C64 b) { return {a.re - b.re, a.im - b.im}; }
+
+__device__ __forceinline__ acc::Pair<float> ffadd(float ah, float al, float bh, float bl)
+{
+    return acc::ff_add(acc::Pair<float>{ah, al}, acc::Pair<float>{bh, bl});
+}
+__device__ __forceinline__ CFF add(CFF a, CFF b)
+{
+    const auto r = ffadd(a.rh, a.rl, b.rh, b.rl), i = ffadd(a.ih, a.il, b.ih, b.il);
+    return {r.h, r.c, i.h, i.c};
+}
+__device__ __forceinline__ CFF sub(CFF a, CFF b)
+{
+    const auto r = ffadd(a.rh, a.rl, -b.rh, -b.rl), i = ffadd(a.ih, a.il, -b.ih, -b.il);
+    return {r.h, r.c, i.h, i.c};
+}
+
+// -i * x (forward) or +i * x (inverse): exact
+template<class C>
+__device__ __forceinline__ C rot(C x, bool fwd);
+template<>
+__device__ __forceinline__ C128 rot(C128 x, bool fwd) { return fwd ? C128{x.im, -x.re} : C128{-x.im, x.re}; }
+template<>
+__device__ __forceinline__ C64 rot(C64 x, bool fwd) { return fwd ? C64{x.im, -x.re} : C64{-x.im, x.re}; }
+template<>
+__device__ __forceinline__ CFF rot(CFF x, bool fwd)
+{
+    return fwd ? CFF{x.ih, x.il, -x.rh, -x.rl} : CFF{-x.ih, -x.il, x.rh, x.rl};
+}
+
+// float-float product of (ah + al)(bh + bl), 2Prod + cross terms, fast2Sum renormalised
+__device__ __forceinline__ acc::Pair<float> ffmul(float ah, float al, float bh, float bl)
+{
+    const float p = ah * bh;
+    float e = __builtin_fmaf(ah, bh, -p);
+    e = __builtin_fmaf(ah, bl, e);
+    e = __builtin_fmaf(al, bh, e);
+    const float hi = p + e;
+    return acc::Pair<float>{hi, e - (hi - p)};
+}
+
+// twiddle tables: W[m] = exp(-2 pi i m / 8192)
+struct TwTables {
+    const double2* d = nullptr;
+    const float2* f = nullptr;
+    const float4* ff = nullptr; // (re_hi, re_lo, im_hi, im_lo)
+};
+
+__device__ __forceinline__ C128 twiddle(const TwTables& t, uint32_t m, bool fwd, C128*)
+{
+    const double2 w = t.d[m];
+    return {w.x, fwd ? w.y : -w.y};
+}
+__device__ __forceinline__ C64 twiddle(const TwTables& t, uint32_t m, bool fwd, C64*)
+{
+    const float2 w = t.f[m];
+    return {w.x, fwd ? w.y : -w.y};
+}
+__device__ __forceinline__ CFF twiddle(const TwTables& t, uint32_t m, bool fwd, CFF*)
+{
+    const float4 w = t.ff[m];
+    return fwd ? CFF{w.x, w.y, w.z, w.w} : CFF{w.x, w.y, -w.z, -w.w};
+}
+
+__device__ __forceinline__ C128 cmul(C128 a, C128 w)
+{
+    return {a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re};
+}
+__device__ __forceinline__ C64 cmul(C64 a, C64 w) { return {a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re}; }
+__device__ __forceinline__ CFF cmul(CFF a, CFF w)
+{
+    const auto rr = ffmul(a.rh, a.rl, w.rh, w.rl), ii = ffmul(a.ih, a.il, w.ih, w.il);
+    const auto ri = ffmul(a.rh, a.rl, w.ih, w.il), ir = ffmul(a.ih, a.il, w.rh, w.rl);
+    const auto re = ffadd(rr.h, rr.c, -ii.h, -ii.c), im = ffadd(ri.h, ri.c, ir.h, ir.c);
+    return {re.h, re.c, im.h, im.c};
+}
+
+// global I/O: complex128 as double2, complex64 (both float modes) as float2
+__device__ __forceinline__ C128 gload(const void* p, uint64_t i, C128*)
+{
+    const double2 v = static_cast<const double2*>(p)[i];
+    return {v.x, v.y};
+}
+__device__ __forceinline__ C64 gload(const void* p, uint64_t i, C64*)
+{
+    const float2 v = static_cast<const float2*>(p)[i];
+    return {v.x, v.y};
+}
+__device__ __forceinline__ CFF gload(const void* p, uint64_t i, CFF*)
+{
+    const float2 v = static_cast<const float2*>(p)[i];
+    return {v.x, 0.0f, v.y, 0.0f};
+}
+__device__ __forceinline__ void gstore(void* p, uint64_t i, C128 v, double s)
+{
+    static_cast<double2*>(p)[i] = make_double2(v.re * s, v.im * s);
+}
+__device__ __forceinline__ void gstore(void* p, uint64_t i, C64 v, double s)
+{
+    const float fs = (float)s;
+    static_cast<float2*>(p)[i] = make_float2(v.re * fs, v.im * fs);
+}
+__device__ __forceinline__ void gstore(void* p, uint64_t i, CFF v, double s)
+{
+    // scale in float-float (s as a float-float pair), then round once to float
+    const float sh = (float)s, sl = (float)(s - (double)sh);
+    const auto r = ffmul(v.rh, v.rl, sh, sl), im = ffmul(v.ih, v.il, sh, sl);
+    static_cast<float2*>(p)[i] = make_float2(r.h + r.c, im.h + im.c);
+}
+
+// One Stockham pass of radix R over all signals of the chunk (P points).
+template<int R, class C>
+__device__ __forceinline__ void pass(C* buf, uint32_t log2n, uint32_t P, uint32_t Ns, const TwTables& tw, bool fwd)
+{
+    constexpr int MAXB = kMaxN / (R * kThreads);
+    const uint32_t N = 1u << log2n;
+    const uint32_t NR = N / R;
+    const uint32_t nb = P / R;
+    C v[MAXB][R];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const uint32_t b = threadIdx.x + q * kThreads;
+        if (b < nb) {
+            const uint32_t s = b / NR, j = b - s * NR;
+            const uint32_t base = s * N;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = buf[pad(base + j + r * NR)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const uint32_t b = threadIdx.x + q * kThreads;
+        if (b < nb) {
+            const uint32_t s = b / NR, j = b - s * NR;
+            const uint32_t base = s * N;
+            const uint32_t k = j & (Ns - 1);
+            const uint32_t m = k * (kMaxN / (R * Ns));
+            C* x = v[q];
+#pragma unroll
+            for (int r = 1; r < R; ++r) x[r] = cmul(x[r], twiddle(tw, (r * m) & (kMaxN - 1), fwd, (C*)nullptr));
+            C y[R];
+            if constexpr (R == 4) {
+                const C s02 = add(x[0], x[2]), d02 = sub(x[0], x[2]);
+                const C s13 = add(x[1], x[3]), d13 = rot(sub(x[1], x[3]), fwd);
+                y[0] = add(s02, s13);
+                y[1] = add(d02, d13);
+                y[2] = sub(s02, s13);
+                y[3] = sub(d02, d13);
+            } else {
+                y[0] = add(x[0], x[1]);
+                y[1] = sub(x[0], x[1]);
+            }
+            const uint32_t idx = base + (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[pad(idx + r * Ns)] = y[r];
+        }
+    }
+    __syncthreads();
+}
+
+template<class C>
+__global__ __launch_bounds__(kThreads) void fft_kernel(const Desc* __restrict__ descs, const Sig* __restrict__ sigs,
+                                                       const void* __restrict__ in, void* __restrict__ out,
+                                                       TwTables tw, int fwd_flag, int norm)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C* buf = reinterpret_cast<C*>(lds_raw);
+    const bool fwd = fwd_flag != 0;
+    const Desc d = descs[blockIdx.x];
+    const uint32_t log2n = d.log2n, N = 1u << log2n, P = (uint32_t)d.count << log2n;
+    // numpy norms (fft/Fft.h:73-76): Ortho 1/sqrt(N); Backward scales the inverse, Forward the forward
+    const double scale = norm == 2 ? 1.0 / __builtin_sqrt((double)N)
+                                   : ((fwd == (norm == 1)) ? 1.0 / (double)N : 1.0);
+    for (uint32_t e = threadIdx.x; e < P; e += kThreads) {
+        const uint32_t s = e >> log2n, j = e & (N - 1);
+        buf[pad(e)] = gload(in, sigs[d.first_sig + s].in_off + j, (C*)nullptr);
+    }
+    __syncthreads();
+    uint32_t Ns = 1;
+    for (; Ns * 4 <= N; Ns *= 4) pass<4>(buf, log2n, P, Ns, tw, fwd);
+    if (Ns < N) pass<2>(buf, log2n, P, Ns, tw, fwd);
+    for (uint32_t e = threadIdx.x; e < P; e += kThreads) {
+        const uint32_t s = e >> log2n, j = e & (N - 1);
+        gstore(out, sigs[d.first_sig + s].out_off + j, buf[pad(e)], scale);
+    }
+}
+
+__global__ void twiddle_kernel(double2* d, float2* f, float4* ff)
+{
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (uint32_t)kMaxN) return;
+    // angle -2 pi m / 8192; m/8192 is exact, sincos of the product is within an ulp
+    double s, c;
+    dm::sincos(-2.0 * 3.141592653589793238 * ((double)m / kMaxN), &s, &c);
+    // snap the exact cases (quarter turns) so symmetric twiddles are exact
+    if ((m & (kMaxN / 4 - 1)) == 0) {
+        const uint32_t q = m / (kMaxN / 4);
+        c = q == 0 ? 1.0 : q == 2 ? -1.0 : 0.0;
+        s = q == 1 ? -1.0 : q == 3 ? 1.0 : 0.0;
+    }
+    d[m] = make_double2(c, s);
+    f[m] = make_float2((float)c, (float)s);
+    const float ch = (float)c, sh = (float)s;
+    ff[m] = make_float4(ch, (float)(c - (double)ch), sh, (float)(s - (double)sh));
+}
+
+// Per-device twiddle tables, created once.
+struct DeviceTables {
+    double2* d = nullptr;
+    float2* f = nullptr;
+    float4* ff = nullptr;
+};
+std::mutex g_tw_mutex;
+std::map<int, DeviceTables> g_tw;
+
+TwTables tables(hipStream_t st)
+{
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    auto it = g_tw.find(dev);
+    if (it == g_tw.end()) {
+        DeviceTables t;
+        check(hipMalloc(&t.d, sizeof(double2) * kMaxN), "hipMalloc(twiddles)", sizeof(double2) * kMaxN);
+        check(hipMalloc(&t.f, sizeof(float2) * kMaxN), "hipMalloc(twiddles)", sizeof(float2) * kMaxN);
+        check(hipMalloc(&t.ff, sizeof(float4) * kMaxN), "hipMalloc(twiddles)", sizeof(float4) * kMaxN);
+        hipLaunchKernelGGL(twiddle_kernel, dim3(kMaxN / 256), dim3(256), 0, st, t.d, t.f, t.ff);
+        check(hipGetLastError(), "twiddle_kernel launch");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize(twiddles)");
+        for (auto fn : {(const void*)fft_kernel<C128>, (const void*)fft_kernel<C64>, (const void*)fft_kernel<CFF>}) {
+            check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsElems * 16),
+                  "hipFuncSetAttribute(LDS)");
+        }
+        it = g_tw.emplace(dev, t).first;
+    }
+    return TwTables{it->second.d, it->second.f, it->second.ff};
+}
+
+// A batch plan: descriptors and signal table resident on the device.
+struct BatchPlan {
+    int prec = 0; // 0 c128, 1 c64, 2 c64 float-float
+    int dev = 0;
+    Desc* descs = nullptr;
+    Sig* sigs = nullptr;
+    uint32_t ndesc = 0;
+    uint64_t nsig = 0;
+};
+
+size_t lds_bytes(int prec) { return (size_t)kLdsElems * (prec == 1 ? sizeof(C64) : 16); }
+
+void build_tables(const uint32_t* log2n, const uint64_t* in_off, const uint64_t* out_off, uint64_t count,
+                  std::vector<Desc>& descs, std::vector<Sig>& sigs)
+{
+    // group by length, largest first; a chunk holds up to 8192 points of one length
+    std::vector<uint64_t> order(count);
+    for (uint64_t i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return log2n[a] > log2n[b]; });
+    sigs.resize(count);
+    uint64_t i = 0;
+    while (i < count) {
+        const uint32_t L = log2n[order[i]];
+        const uint32_t per = (uint32_t)kMaxN >> L;
+        Desc d;
+        d.first_sig = (uint32_t)i;
+        d.log2n = (uint16_t)L;
+        uint32_t c = 0;
+        while (i < count && c < per && log2n[order[i]] == L) {
+            sigs[i] = Sig{in_off[order[i]], out_off[order[i]]};
+            ++i;
+            ++c;
+        }
+        d.count = (uint16_t)c;
+        descs.push_back(d);
+    }
+}
+
+void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm, hipStream_t st)
+{
+    const TwTables tw = tables(st);
+    const size_t lds = lds_bytes(p.prec);
+    if (p.ndesc == 0) return;
+    switch (p.prec) {
+    case 0:
+        hipLaunchKernelGGL(fft_kernel<C128>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
+                           forward, norm);
+        break;
+    case 1:
+        hipLaunchKernelGGL(fft_kernel<C64>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
+                           forward, norm);
+        break;
+    default:
+        hipLaunchKernelGGL(fft_kernel<CFF>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
+                           forward, norm);
+        break;
+    }
+    check(hipGetLastError(), "fft_kernel launch");
+}
+
+} // namespace fft
+} // namespace aeth
+
+using namespace aeth;
+using namespace aeth::fft;
+
+namespace {
+uint32_t log2_exact(uint64_t n)
+{
+    if (n < 1 || (n & (n - 1)) != 0) return 0xFFFFFFFFu;
+    uint32_t l = 0;
+    while ((1ull << l) < n) ++l;
+    return l;
+}
+} // namespace
+
+extern "C" int aeth_fft_batch_plan(int prec, uint64_t count, const uint64_t* lengths, const uint64_t* in_offsets,
+                                   const uint64_t* out_offsets, void** plan)
+{
+    return guarded([&] {
+        if (!plan) fail("aeth_fft_batch_plan", "plan is NULL");
+        *plan = nullptr;
+        if (prec < 0 || prec > 2) fail("aeth_fft_batch_plan", "unknown precision");
+        std::vector<uint32_t> l2(count);
+        for (uint64_t i = 0; i < count; ++i) {
+            const uint32_t l = log2_exact(lengths[i]);
+            if (l == 0xFFFFFFFFu || l > (uint32_t)kMaxLog2)
+                fail("aeth_fft_batch_plan", "lengths must be powers of two in [1, 8192]", 0, 2);
+            l2[i] = l;
+        }
+        std::vector<Desc> descs;
+        std::vector<Sig> sigs;
+        build_tables(l2.data(), in_offsets, out_offsets, count, descs, sigs);
+        auto* p = new BatchPlan;
+        p->prec = prec;
+        check(hipGetDevice(&p->dev), "hipGetDevice");
+        p->ndesc = (uint32_t)descs.size();
+        p->nsig = sigs.size();
+        try {
+            if (!descs.empty()) {
+                check(hipMalloc(&p->descs, sizeof(Desc) * descs.size()), "hipMalloc(descriptors)");
+                check(hipMalloc(&p->sigs, sizeof(Sig) * sigs.size()), "hipMalloc(signals)");
+                check(hipMemcpy(p->descs, descs.data(), sizeof(Desc) * descs.size(), hipMemcpyHostToDevice),
+                      "hipMemcpy(descriptors)");
+                check(hipMemcpy(p->sigs, sigs.data(), sizeof(Sig) * sigs.size(), hipMemcpyHostToDevice),
+                      "hipMemcpy(signals)");
+            }
+            (void)tables(nullptr);
+        } catch (...) {
+            (void)hipFree(p->descs);
+            (void)hipFree(p->sigs);
+            delete p;
+            throw;
+        }
+        *plan = p;
+    });
+}
+
+extern "C" int aeth_fft_batch_exec(void* plan, const void* in, void* out, int forward, int norm, void* stream)
+{
+    return guarded([&] {
+        if (!plan) fail("aeth_fft_batch_exec", "plan is NULL");
+        auto* p = static_cast<BatchPlan*>(plan);
+        if (norm < 0 || norm > 2) fail("aeth_fft_batch_exec", "unknown norm");
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        if (dev != p->dev) fail("aeth_fft_batch_exec", "plan belongs to another device");
+        launch(*p, in, out, forward, norm, as_stream(stream));
+    });
+}
+
+extern "C" int aeth_fft_batch_destroy(void* plan)
+{
+    return guarded([&] {
+        auto* p = static_cast<BatchPlan*>(plan);
+        if (!p) return;
+        (void)hipFree(p->descs);
+        (void)hipFree(p->sigs);
+        delete p;
+    });
+}

@@ -92,6 +92,28 @@ int aeth_reduce(int dtype, int mode, int square, const void* x, int ncomp, uint6
 enum { AETH_FN_SIN = 0, AETH_FN_COS = 1, AETH_FN_EXP = 2, AETH_FN_LOG = 3, AETH_FN_ALL4 = 4 };
 int aeth_math(int fn, int dtype, const void* x, void* y, uint64_t n, void* stream);
 
+/* ------------------------------------------------------------------ batched FFT */
+/* Many complex transforms of uneven power-of-two lengths (1..8192) in ONE
+ * launch driven by a device-resident work-descriptor table. Replaces, for
+ * power-of-two lines, fft::c2c (fft/Fft.h:186-197) on the host and the
+ * seam's fft_plan/fft_exec/fft_destroy (dsc/src/seam.h:41-50,
+ * dsc/src/cufft_plan.h:122-320) on the device. Data are interleaved
+ * (re, im) pairs; signal i occupies `lengths[i]` consecutive complex elements
+ * starting at complex offset in_offsets[i] (input) / out_offsets[i] (output).
+ * prec: C128 (double pairs), C64 (float pairs), C64_FF (float pairs in
+ * memory, float-float arithmetic inside). norm follows numpy / fft::Norm
+ * (fft/Fft.h:58,73-76): BACKWARD scales the inverse by 1/N, FORWARD the
+ * forward by 1/N, ORTHO both by 1/sqrt(N). Plan creation allocates and
+ * copies (like the seam's); exec allocates nothing and may be captured.
+ * In-place (in == out with equal offsets) is allowed. Lengths that are not
+ * powers of two, or above 8192, return 2. */
+enum { AETH_FFT_C128 = 0, AETH_FFT_C64 = 1, AETH_FFT_C64_FF = 2 };
+enum { AETH_FFT_BACKWARD = 0, AETH_FFT_FORWARD = 1, AETH_FFT_ORTHO = 2 };
+int aeth_fft_batch_plan(int prec, uint64_t count, const uint64_t* lengths, const uint64_t* in_offsets,
+                        const uint64_t* out_offsets, void** plan);
+int aeth_fft_batch_exec(void* plan, const void* in, void* out, int forward, int norm, void* stream);
+int aeth_fft_batch_destroy(void* plan);
+
 #ifdef __cplusplus
 }
 #endif
