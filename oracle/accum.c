@@ -162,6 +162,15 @@ static pd_t combine_d(pd_t a, pd_t b)
     return r;
 }
 
+static pd_t combine_dm(pd_t a, pd_t b, int mode)
+{
+    if (mode == MODE_PLAIN) {
+        pd_t r = {a.h + b.h, 0.0};
+        return r;
+    }
+    return combine_d(a, b);
+}
+
 static pf_t combine_f(pf_t a, pf_t b, int mode)
 {
     pf_t r;
@@ -179,7 +188,7 @@ static pf_t combine_f(pf_t a, pf_t b, int mode)
 }
 
 /* In-group halving tree (stride 512, 256, ..., 1), then recursively over group results. */
-static pd_t tree_d(pd_t* p, uint64_t T)
+static pd_t tree_d(pd_t* p, uint64_t T, int mode)
 {
     while (T > 1) {
         const uint64_t G = (T + GROUP - 1) / GROUP;
@@ -188,7 +197,7 @@ static pd_t tree_d(pd_t* p, uint64_t T)
             const uint64_t m = (T - g * GROUP) < GROUP ? (T - g * GROUP) : GROUP;
             for (uint64_t stride = GROUP / 2; stride >= 1; stride >>= 1)
                 for (uint64_t t = 0; t < stride; ++t)
-                    if (t + stride < m) q[t] = combine_d(q[t], q[t + stride]);
+                    if (t + stride < m) q[t] = combine_dm(q[t], q[t + stride], mode);
             p[g] = q[0];
         }
         T = G;
@@ -215,7 +224,7 @@ static pf_t tree_f(pf_t* p, uint64_t T, int mode)
 
 /* lanes -> one pair: butterfly within each 64-lane wave (a balanced tree over
  * adjacent lanes), then ((w0, w1), (w2, w3)). */
-static pd_t tile_combine_d(pd_t lane[LANES])
+static pd_t tile_combine_d(pd_t lane[LANES], int mode)
 {
     pd_t w[4];
     for (int wv = 0; wv < 4; ++wv) {
@@ -223,12 +232,12 @@ static pd_t tile_combine_d(pd_t lane[LANES])
         memcpy(v, lane + wv * WAVE, sizeof v);
         for (int off = 1; off < WAVE; off <<= 1) {
             pd_t nv[WAVE];
-            for (int i = 0; i < WAVE; ++i) nv[i] = combine_d(v[i], v[i ^ off]);
+            for (int i = 0; i < WAVE; ++i) nv[i] = combine_dm(v[i], v[i ^ off], mode);
             memcpy(v, nv, sizeof v);
         }
         w[wv] = v[0];
     }
-    return combine_d(combine_d(w[0], w[1]), combine_d(w[2], w[3]));
+    return combine_dm(combine_dm(w[0], w[1], mode), combine_dm(w[2], w[3], mode), mode);
 }
 
 static pf_t tile_combine_f(pf_t lane[LANES], int mode)
@@ -247,7 +256,7 @@ static pf_t tile_combine_f(pf_t lane[LANES], int mode)
     return combine_f(combine_f(w[0], w[1], mode), combine_f(w[2], w[3], mode), mode);
 }
 
-static void device_sum_d(const double* x, uint64_t n, int square, double* hi, double* comp)
+static void device_sum_d(const double* x, uint64_t n, int square, int mode, double* hi, double* comp)
 {
     const uint64_t T = n ? (n + TILE - 1) / TILE : 1;
     pd_t* part = (pd_t*)calloc(T, sizeof(pd_t));
@@ -261,15 +270,16 @@ static void device_sum_d(const double* x, uint64_t n, int square, double* hi, do
                     const uint64_t i = t * TILE + (uint64_t)r * 512 + 2 * (uint64_t)l + k;
                     if (i < n) {
                         const double v = x[i];
-                        orc_neumaier_add_f64(&h, &c, square ? v * v : v);
+                        if (mode == MODE_PLAIN) h = h + (square ? v * v : v);
+                        else orc_neumaier_add_f64(&h, &c, square ? v * v : v);
                     }
                 }
             lane[l].h = h;
             lane[l].c = c;
         }
-        part[t] = tile_combine_d(lane);
+        part[t] = tile_combine_d(lane, mode);
     }
-    const pd_t r = tree_d(part, T);
+    const pd_t r = tree_d(part, T, mode);
     *hi = r.h;
     *comp = r.c;
     free(part);
@@ -319,8 +329,24 @@ static void device_sum_f(const float* x, uint64_t n, int square, int mode, float
     free(part);
 }
 
-void orc_device_sum_f64(const double* x, uint64_t n, double* hi, double* comp) { device_sum_d(x, n, 0, hi, comp); }
-void orc_device_sumsq_f64(const double* x, uint64_t n, double* hi, double* comp) { device_sum_d(x, n, 1, hi, comp); }
+void orc_device_sum_f64(const double* x, uint64_t n, double* hi, double* comp)
+{
+    device_sum_d(x, n, 0, MODE_NEUMAIER, hi, comp);
+}
+void orc_device_sumsq_f64(const double* x, uint64_t n, double* hi, double* comp)
+{
+    device_sum_d(x, n, 1, MODE_NEUMAIER, hi, comp);
+}
+void orc_device_sum_plain_f64(const double* x, uint64_t n, double* hi)
+{
+    double c;
+    device_sum_d(x, n, 0, MODE_PLAIN, hi, &c);
+}
+void orc_device_sumsq_plain_f64(const double* x, uint64_t n, double* hi)
+{
+    double c;
+    device_sum_d(x, n, 1, MODE_PLAIN, hi, &c);
+}
 void orc_device_sum_f32(const float* x, uint64_t n, float* hi, float* comp)
 {
     device_sum_f(x, n, 0, MODE_NEUMAIER, hi, comp);

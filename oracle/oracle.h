@@ -126,7 +126,9 @@ void orc_device_sumsq_f32(const float* x, uint64_t n, float* hi, float* comp);
 /* float-float (Ff2-style, banded/Ff2.h:71-380) accumulation in the same shape. */
 void orc_device_sum_ff_f32(const float* x, uint64_t n, float* hi, float* lo);
 void orc_device_sumsq_ff_f32(const float* x, uint64_t n, float* hi, float* lo);
-/* plain FP32 (no compensation) in the same shape. */
+/* plain (no compensation) in the same shape. */
+void orc_device_sum_plain_f64(const double* x, uint64_t n, double* hi);
+void orc_device_sumsq_plain_f64(const double* x, uint64_t n, double* hi);
 void orc_device_sum_plain_f32(const float* x, uint64_t n, float* hi);
 void orc_device_sumsq_plain_f32(const float* x, uint64_t n, float* hi);
 /* Exact-enough reference: long double Neumaier walk (tests compare with math.fsum). */

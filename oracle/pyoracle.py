@@ -67,6 +67,8 @@ def lib():
             "orc_device_sum_ff_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
             "orc_device_sumsq_ff_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
             "orc_device_sum_plain_f32": (None, [_pf, _u64, C.POINTER(_f32)]),
+            "orc_device_sum_plain_f64": (None, [_pd, _u64, C.POINTER(_f64)]),
+            "orc_device_sumsq_plain_f64": (None, [_pd, _u64, C.POINTER(_f64)]),
             "orc_device_sumsq_plain_f32": (None, [_pf, _u64, C.POINTER(_f32)]),
             "orc_neumaier_add_f64": (None, [C.POINTER(_f64), C.POINTER(_f64), _f64]),
             "orc_pair_combine_f64": (None, [_f64, _f64, _f64, _f64, C.POINTER(_f64), C.POINTER(_f64)]),
@@ -210,7 +212,12 @@ def device_sum(x, square=False, mode="neumaier"):
     x = np.ascontiguousarray(x)
     if x.dtype == np.float64:
         h, c = _f64(), _f64()
-        (lib().orc_device_sumsq_f64 if square else lib().orc_device_sum_f64)(x, x.size, C.byref(h), C.byref(c))
+        if mode == "plain":
+            (lib().orc_device_sumsq_plain_f64 if square else lib().orc_device_sum_plain_f64)(x, x.size, C.byref(h))
+        elif mode == "neumaier":
+            (lib().orc_device_sumsq_f64 if square else lib().orc_device_sum_f64)(x, x.size, C.byref(h), C.byref(c))
+        else:
+            raise ValueError("float-float accumulation is for float32")
         return h.value, c.value
     h, c = _f32(), _f32()
     if mode == "neumaier":
