@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -42,6 +43,7 @@ struct Desc {
     uint32_t first_sig; // index into the signal table
     uint16_t count;     // signals in this chunk
     uint16_t log2n;
+    uint32_t stride;    // elements between consecutive points of a signal (1 = contiguous)
 };
 
 struct Sig {
@@ -227,7 +229,7 @@ __device__ __forceinline__ void pass(C* buf, uint32_t log2n, uint32_t P, uint32_
 template<class C>
 __global__ __launch_bounds__(kThreads) void fft_kernel(const Desc* __restrict__ descs, const Sig* __restrict__ sigs,
                                                        const void* __restrict__ in, void* __restrict__ out,
-                                                       TwTables tw, int fwd_flag, int norm)
+                                                       TwTables tw, int fwd_flag, int norm, double fixed_scale)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C* buf = reinterpret_cast<C*>(lds_raw);
@@ -235,19 +237,39 @@ __global__ __launch_bounds__(kThreads) void fft_kernel(const Desc* __restrict__ 
     const Desc d = descs[blockIdx.x];
     const uint32_t log2n = d.log2n, N = 1u << log2n, P = (uint32_t)d.count << log2n;
     // numpy norms (fft/Fft.h:73-76): Ortho 1/sqrt(N); Backward scales the inverse, Forward the forward
-    const double scale = norm == 2 ? 1.0 / __builtin_sqrt((double)N)
-                                   : ((fwd == (norm == 1)) ? 1.0 / (double)N : 1.0);
-    for (uint32_t e = threadIdx.x; e < P; e += kThreads) {
-        const uint32_t s = e >> log2n, j = e & (N - 1);
-        buf[pad(e)] = gload(in, sigs[d.first_sig + s].in_off + j, (C*)nullptr);
+    // norm 3: a caller-fixed scale (multi-axis plans scale once, by the product of the lengths)
+    const double scale = norm == 3 ? fixed_scale
+                         : norm == 2 ? 1.0 / __builtin_sqrt((double)N)
+                                     : ((fwd == (norm == 1)) ? 1.0 / (double)N : 1.0);
+    const uint32_t stride = d.stride;
+    if (stride == 1) {
+        for (uint32_t e = threadIdx.x; e < P; e += kThreads) {
+            const uint32_t s = e >> log2n, j = e & (N - 1);
+            buf[pad(e)] = gload(in, sigs[d.first_sig + s].in_off + j, (C*)nullptr);
+        }
+    } else {
+        // strided lines: neighbouring lanes take neighbouring signals at the same point
+        const uint32_t cnt = d.count;
+        for (uint32_t e = threadIdx.x; e < P; e += kThreads) {
+            const uint32_t j = e / cnt, s = e - j * cnt;
+            buf[pad(s * N + j)] = gload(in, sigs[d.first_sig + s].in_off + (uint64_t)j * stride, (C*)nullptr);
+        }
     }
     __syncthreads();
     uint32_t Ns = 1;
     for (; Ns * 4 <= N; Ns *= 4) pass<4>(buf, log2n, P, Ns, tw, fwd);
     if (Ns < N) pass<2>(buf, log2n, P, Ns, tw, fwd);
-    for (uint32_t e = threadIdx.x; e < P; e += kThreads) {
-        const uint32_t s = e >> log2n, j = e & (N - 1);
-        gstore(out, sigs[d.first_sig + s].out_off + j, buf[pad(e)], scale);
+    if (stride == 1) {
+        for (uint32_t e = threadIdx.x; e < P; e += kThreads) {
+            const uint32_t s = e >> log2n, j = e & (N - 1);
+            gstore(out, sigs[d.first_sig + s].out_off + j, buf[pad(e)], scale);
+        }
+    } else {
+        const uint32_t cnt = d.count;
+        for (uint32_t e = threadIdx.x; e < P; e += kThreads) {
+            const uint32_t j = e / cnt, s = e - j * cnt;
+            gstore(out, sigs[d.first_sig + s].out_off + (uint64_t)j * stride, buf[pad(s * N + j)], scale);
+        }
     }
 }
 
@@ -312,10 +334,18 @@ struct BatchPlan {
     uint64_t nsig = 0;
 };
 
+uint32_t log2_exact(uint64_t n)
+{
+    if (n < 1 || (n & (n - 1)) != 0) return 0xFFFFFFFFu;
+    uint32_t l = 0;
+    while ((1ull << l) < n) ++l;
+    return l;
+}
+
 size_t lds_bytes(int prec) { return (size_t)kLdsElems * (prec == 1 ? sizeof(C64) : 16); }
 
 void build_tables(const uint32_t* log2n, const uint64_t* in_off, const uint64_t* out_off, uint64_t count,
-                  std::vector<Desc>& descs, std::vector<Sig>& sigs)
+                  std::vector<Desc>& descs, std::vector<Sig>& sigs, uint32_t stride = 1)
 {
     // group by length, largest first; a chunk holds up to 8192 points of one length
     std::vector<uint64_t> order(count);
@@ -329,6 +359,7 @@ void build_tables(const uint32_t* log2n, const uint64_t* in_off, const uint64_t*
         Desc d;
         d.first_sig = (uint32_t)i;
         d.log2n = (uint16_t)L;
+        d.stride = stride;
         uint32_t c = 0;
         while (i < count && c < per && log2n[order[i]] == L) {
             sigs[i] = Sig{in_off[order[i]], out_off[order[i]]};
@@ -340,7 +371,8 @@ void build_tables(const uint32_t* log2n, const uint64_t* in_off, const uint64_t*
     }
 }
 
-void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm, hipStream_t st)
+void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm, hipStream_t st,
+            double fixed_scale = 1.0)
 {
     const TwTables tw = tables(st);
     const size_t lds = lds_bytes(p.prec);
@@ -348,15 +380,15 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
     switch (p.prec) {
     case 0:
         hipLaunchKernelGGL(fft_kernel<C128>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
-                           forward, norm);
+                           forward, norm, fixed_scale);
         break;
     case 1:
         hipLaunchKernelGGL(fft_kernel<C64>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
-                           forward, norm);
+                           forward, norm, fixed_scale);
         break;
     default:
         hipLaunchKernelGGL(fft_kernel<CFF>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
-                           forward, norm);
+                           forward, norm, fixed_scale);
         break;
     }
     check(hipGetLastError(), "fft_kernel launch");
@@ -368,15 +400,6 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
 using namespace aeth;
 using namespace aeth::fft;
 
-namespace {
-uint32_t log2_exact(uint64_t n)
-{
-    if (n < 1 || (n & (n - 1)) != 0) return 0xFFFFFFFFu;
-    uint32_t l = 0;
-    while ((1ull << l) < n) ++l;
-    return l;
-}
-} // namespace
 
 extern "C" int aeth_fft_batch_plan(int prec, uint64_t count, const uint64_t* lengths, const uint64_t* in_offsets,
                                    const uint64_t* out_offsets, void** plan)
@@ -443,3 +466,116 @@ extern "C" int aeth_fft_batch_destroy(void* plan)
         delete p;
     });
 }
+
+// ---------------------------------------------------------------------------
+// Plans for the memory seam's fft_plan/fft_exec/fft_destroy
+// (dsc/src/seam.h:41-50, dsc/src/cufft_plan.h:125-222): c2c over any set of
+// axes of a dense row-major array, one strided-line pass per axis.
+namespace aeth {
+namespace fft {
+
+struct SeamPlan {
+    int dev = 0;
+    int forward = 1;
+    std::vector<BatchPlan*> passes;
+    std::vector<double> scales;
+    std::mutex mu; // cufft_plan.h:187 -- one execution at a time per plan
+    ~SeamPlan()
+    {
+        for (auto* p : passes) {
+            (void)hipFree(p->descs);
+            (void)hipFree(p->sigs);
+            delete p;
+        }
+    }
+};
+
+int seam_plan(int dev, int kind, int is_double, int norm, int forward, int ndim, const int64_t* shape, int naxes,
+              const int32_t* axes, void* stream, void** plan)
+{
+    return guarded([&] {
+        if (!plan) fail("fft_plan", "plan is NULL");
+        *plan = nullptr;
+        if (kind != 0) fail("fft_plan", "only AETHER_FFT_C2C is implemented on this device", 0, 2);
+        if (ndim < 1 || naxes < 1) fail("fft_plan", "no axes to transform");
+        std::vector<bool> seen(ndim, false);
+        for (int a = 0; a < naxes; ++a) {
+            if (axes[a] < 0 || axes[a] >= ndim) fail("fft_plan", "axis out of range");
+            if (seen[axes[a]]) fail("fft_plan", "repeated axis");
+            seen[axes[a]] = true;
+        }
+        if (norm < 0 || norm > 2) fail("fft_plan", "unknown norm");
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        check(hipStreamIsCapturing(as_stream(stream), &cs), "hipStreamIsCapturing");
+        if (cs != hipStreamCaptureStatusNone) fail("fft_plan", "refused while the stream is capturing");
+        int prev = 0;
+        check(hipGetDevice(&prev), "hipGetDevice");
+        if (prev != dev) check(hipSetDevice(dev), "hipSetDevice");
+        auto sp = std::make_unique<SeamPlan>();
+        sp->dev = dev;
+        sp->forward = forward;
+        uint64_t total = 1, points = 1;
+        for (int d = 0; d < ndim; ++d) total *= (uint64_t)shape[d];
+        for (int a = 0; a < naxes; ++a) points *= (uint64_t)shape[axes[a]];
+        for (int a = 0; a < naxes; ++a) {
+            const int ax = axes[a];
+            const uint64_t n = (uint64_t)shape[ax];
+            const uint32_t l2 = log2_exact(n);
+            if (l2 == 0xFFFFFFFFu || l2 > (uint32_t)kMaxLog2)
+                fail("fft_plan", "axis lengths must be powers of two in [1, 8192]", 0, 2);
+            uint64_t inner = 1;
+            for (int d = ax + 1; d < ndim; ++d) inner *= (uint64_t)shape[d];
+            if (inner > 0xFFFFFFFFull) fail("fft_plan", "inner extent too large");
+            const uint64_t lines = n ? total / n : 0;
+            std::vector<uint32_t> lg(lines, l2);
+            std::vector<uint64_t> off(lines);
+            for (uint64_t li = 0; li < lines; ++li) {
+                const uint64_t o = li / inner, in = li % inner;
+                off[li] = o * n * inner + in;
+            }
+            std::vector<Desc> descs;
+            std::vector<Sig> sigs;
+            build_tables(lg.data(), off.data(), off.data(), lines, descs, sigs, (uint32_t)inner);
+            auto* bp = new BatchPlan;
+            bp->prec = is_double ? 0 : 1;
+            bp->dev = dev;
+            bp->ndesc = (uint32_t)descs.size();
+            bp->nsig = sigs.size();
+            sp->passes.push_back(bp);
+            if (!descs.empty()) {
+                check(hipMalloc(&bp->descs, sizeof(Desc) * descs.size()), "hipMalloc(descriptors)");
+                check(hipMalloc(&bp->sigs, sizeof(Sig) * sigs.size()), "hipMalloc(signals)");
+                check(hipMemcpy(bp->descs, descs.data(), sizeof(Desc) * descs.size(), hipMemcpyHostToDevice),
+                      "hipMemcpy(descriptors)");
+                check(hipMemcpy(bp->sigs, sigs.data(), sizeof(Sig) * sigs.size(), hipMemcpyHostToDevice),
+                      "hipMemcpy(signals)");
+            }
+            // fft/Fft.h:73-76 scale of the whole transform, applied by the last pass
+            double sc = 1.0;
+            if (a + 1 == naxes) {
+                if (norm == 2) sc = 1.0 / std::sqrt((double)points);
+                else if ((forward != 0) == (norm == 1)) sc = 1.0 / (double)points;
+            }
+            sp->scales.push_back(sc);
+        }
+        (void)tables(as_stream(stream));
+        if (prev != dev) check(hipSetDevice(prev), "hipSetDevice");
+        *plan = sp.release();
+    });
+}
+
+int seam_exec(void* plan, const void* in, void* out, void* stream)
+{
+    return guarded([&] {
+        auto* sp = static_cast<SeamPlan*>(plan);
+        if (!sp) fail("fft_exec", "plan is NULL");
+        std::lock_guard<std::mutex> lock(sp->mu);
+        for (size_t i = 0; i < sp->passes.size(); ++i)
+            launch(*sp->passes[i], i == 0 ? in : out, out, sp->forward, 3, as_stream(stream), sp->scales[i]);
+    });
+}
+
+void seam_destroy(void* plan) { delete static_cast<SeamPlan*>(plan); }
+
+} // namespace fft
+} // namespace aeth

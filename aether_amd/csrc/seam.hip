@@ -1,0 +1,280 @@
+// seam.hip -- the `aether-memory/1` C seam (dsc/src/seam.h:22-51) over HIP.
+//
+// Same table shape, field order and return conventions as the reference's
+// CUDA plugin (dsc/src/plugin.cpp:20-156), exported as aether_memory_v1().
+// Memory kinds CPU / DEVICE / PINNED; device chunks are stream-ordered
+// (hipMallocAsync from the device's default pool); a chunk released while its
+// stream is being captured is queued and freed by drain() after the capture
+// ends (chunk/StreamOrdered.h:160-286). fill and device-to-device copy are
+// hand-written kernels (16-byte lanes), legal inside a graph capture and free
+// of memset nodes; other copies go through hipMemcpyAsync.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/aether_hip.h"
+#include "common.hpp"
+
+namespace aeth {
+namespace fft {
+int seam_plan(int dev, int kind, int is_double, int norm, int forward, int ndim, const int64_t* shape, int naxes,
+              const int32_t* axes, void* stream, void** plan);
+int seam_exec(void* plan, const void* in, void* out, void* stream);
+void seam_destroy(void* plan);
+} // namespace fft
+
+namespace {
+
+struct Chunk {
+    int kind;
+    int dev;
+    void* ptr;
+    size_t bytes;
+    hipStream_t stream;
+};
+
+std::mutex g_deferred_mu;
+std::vector<Chunk*> g_deferred;
+
+struct OnDevice {
+    int prev = 0;
+    bool switched = false;
+    explicit OnDevice(int dev)
+    {
+        check(hipGetDevice(&prev), "hipGetDevice");
+        switched = prev != dev;
+        if (switched) check(hipSetDevice(dev), "hipSetDevice");
+    }
+    ~OnDevice()
+    {
+        if (switched) (void)hipSetDevice(prev);
+    }
+};
+
+void free_now(Chunk* c)
+{
+    if (c->kind == AETHER_MEM_DEVICE) {
+        OnDevice d(c->dev);
+        (void)hipFreeAsync(c->ptr, c->stream);
+    } else if (c->kind == AETHER_MEM_PINNED) {
+        (void)hipHostFree(c->ptr);
+    } else {
+        std::free(c->ptr);
+    }
+    delete c;
+}
+
+bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+    return st != hipStreamCaptureStatusNone;
+}
+
+constexpr int kThreads = 256;
+
+// Fill `bytes` with a repeating `width`-byte pattern (width divides 16 or is
+// 1, 2, 4, 8, 16): the 16-byte lane pattern is pre-expanded on the host.
+__global__ __launch_bounds__(kThreads) void fill_kernel(unsigned char* __restrict__ p, size_t bytes, uint4 pat16)
+{
+    const size_t vecs = bytes / 16;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < vecs; i += (size_t)gridDim.x * kThreads)
+        reinterpret_cast<uint4*>(p)[i] = pat16;
+    const size_t tail = bytes - vecs * 16;
+    if (blockIdx.x == 0 && threadIdx.x < tail) {
+        const unsigned char* pb = reinterpret_cast<const unsigned char*>(&pat16);
+        p[vecs * 16 + threadIdx.x] = pb[threadIdx.x];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void copy_kernel(unsigned char* __restrict__ dst,
+                                                        const unsigned char* __restrict__ src, size_t bytes,
+                                                        bool vec)
+{
+    if (vec) {
+        const size_t vecs = bytes / 16;
+        for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < vecs; i += (size_t)gridDim.x * kThreads)
+            reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
+        const size_t tail = bytes - vecs * 16;
+        if (blockIdx.x == 0 && threadIdx.x < tail) dst[vecs * 16 + threadIdx.x] = src[vecs * 16 + threadIdx.x];
+    } else {
+        for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < bytes; i += (size_t)gridDim.x * kThreads)
+            dst[i] = src[i];
+    }
+}
+
+int grid_for(size_t work)
+{
+    const size_t blocks = (work + kThreads - 1) / kThreads;
+    return (int)std::max<size_t>(1, std::min<size_t>(blocks, (size_t)device_cus() * 32));
+}
+
+int s_probe(int* count) { return aeth_probe(count); }
+
+int s_runtime_version(int* v)
+{
+    return guarded([&] { check(hipRuntimeGetVersion(v), "hipRuntimeGetVersion"); });
+}
+
+int s_current_device(int* d)
+{
+    return guarded([&] { check(hipGetDevice(d), "hipGetDevice"); });
+}
+
+const char* s_last_error() { return last_error(); }
+
+int s_chunk_alloc(int kind, int dev, size_t bytes, void* stream, void** chunk, void** ptr)
+{
+    return guarded([&] {
+        if (!chunk || !ptr) fail("chunk_alloc", "NULL out-parameter");
+        auto* c = new Chunk{kind, dev, nullptr, bytes, as_stream(stream)};
+        try {
+            if (kind == AETHER_MEM_DEVICE) {
+                OnDevice d(dev);
+                if (bytes) check(hipMallocAsync(&c->ptr, bytes, c->stream), "hipMallocAsync", bytes);
+            } else if (kind == AETHER_MEM_PINNED) {
+                if (bytes) check(hipHostMalloc(&c->ptr, bytes, hipHostMallocDefault), "hipHostMalloc", bytes);
+            } else if (kind == AETHER_MEM_CPU) {
+                c->ptr = bytes ? std::aligned_alloc(256, (bytes + 255) / 256 * 256) : nullptr;
+                if (bytes && !c->ptr) fail("chunk_alloc", "host allocation failed", bytes);
+            } else {
+                fail("chunk_alloc", "unknown memory kind");
+            }
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *chunk = c;
+        *ptr = c->ptr;
+    });
+}
+
+void s_chunk_release(void* chunk)
+{
+    auto* c = static_cast<Chunk*>(chunk);
+    if (!c) return;
+    if (c->kind == AETHER_MEM_DEVICE && capturing(c->stream)) {
+        std::lock_guard<std::mutex> lock(g_deferred_mu);
+        g_deferred.push_back(c);
+        return;
+    }
+    free_now(c);
+}
+
+void s_drain()
+{
+    std::vector<Chunk*> keep, now;
+    {
+        std::lock_guard<std::mutex> lock(g_deferred_mu);
+        for (Chunk* c : g_deferred) (capturing(c->stream) ? keep : now).push_back(c);
+        g_deferred.swap(keep);
+    }
+    for (Chunk* c : now) free_now(c);
+}
+
+int s_fill(int kind, int dev, void* ptr, size_t bytes, const void* value, size_t width, void* stream)
+{
+    return guarded([&] {
+        if (bytes == 0) return;
+        if (!ptr || !value) fail("fill", "NULL pointer");
+        if (width == 0 || width > 16 || (16 % width) != 0 || bytes % width != 0)
+            fail("fill", "width must divide 16 and the byte count", bytes);
+        unsigned char pat[16];
+        for (size_t i = 0; i < 16; ++i) pat[i] = static_cast<const unsigned char*>(value)[i % width];
+        if (kind == AETHER_MEM_DEVICE) {
+            OnDevice d(dev);
+            uint4 p16;
+            std::memcpy(&p16, pat, 16);
+            if (reinterpret_cast<uintptr_t>(ptr) % 16 != 0) fail("fill", "device pointer must be 16-byte aligned", 0, 2);
+            hipLaunchKernelGGL(fill_kernel, dim3(grid_for(bytes / 16 + 1)), dim3(kThreads), 0, as_stream(stream),
+                               static_cast<unsigned char*>(ptr), bytes, p16);
+            check(hipGetLastError(), "fill_kernel launch", bytes);
+        } else {
+            // host memory: ordered after the stream's queued work
+            check(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize", bytes);
+            unsigned char* p = static_cast<unsigned char*>(ptr);
+            for (size_t i = 0; i < bytes; ++i) p[i] = pat[i % 16];
+        }
+    });
+}
+
+int s_copy(int dst_kind, int dst_dev, void* dst, int src_kind, int src_dev, const void* src, size_t bytes,
+           void* stream)
+{
+    return guarded([&] {
+        if (bytes == 0) return;
+        hipStream_t st = as_stream(stream);
+        if (dst_kind == AETHER_MEM_DEVICE && src_kind == AETHER_MEM_DEVICE && dst_dev == src_dev) {
+            OnDevice d(dst_dev);
+            const bool vec = aligned16(dst) && aligned16(src);
+            hipLaunchKernelGGL(copy_kernel, dim3(grid_for(vec ? bytes / 16 + 1 : bytes)), dim3(kThreads), 0, st,
+                               static_cast<unsigned char*>(dst), static_cast<const unsigned char*>(src), bytes, vec);
+            check(hipGetLastError(), "copy_kernel launch", bytes);
+            return;
+        }
+        if (dst_kind == AETHER_MEM_DEVICE && src_kind == AETHER_MEM_DEVICE) {
+            check(hipMemcpyPeerAsync(dst, dst_dev, src, src_dev, bytes, st), "hipMemcpyPeerAsync", bytes);
+            return;
+        }
+        const int dev = dst_kind == AETHER_MEM_DEVICE ? dst_dev : src_dev;
+        OnDevice d(dev);
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, st), "hipMemcpyAsync", bytes);
+        // pageable host memory: the copy must be complete before the caller may touch it
+        if (dst_kind == AETHER_MEM_CPU || src_kind == AETHER_MEM_CPU)
+            check(hipStreamSynchronize(st), "hipStreamSynchronize", bytes);
+    });
+}
+
+int s_stream_wait(int dev, void* producer, void* consumer)
+{
+    return guarded([&] {
+        OnDevice d(dev);
+        hipEvent_t ev;
+        check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreateWithFlags");
+        hipError_t e = hipEventRecord(ev, as_stream(producer));
+        if (e == hipSuccess) e = hipStreamWaitEvent(as_stream(consumer), ev, 0);
+        (void)hipEventDestroy(ev);
+        check(e, "stream_wait");
+    });
+}
+
+int s_stream_sync(int dev, void* stream)
+{
+    return guarded([&] {
+        OnDevice d(dev);
+        check(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
+    });
+}
+
+int s_pointer_device(const void* p, int* dev)
+{
+    return guarded([&] {
+        hipPointerAttribute_t a;
+        check(hipPointerGetAttributes(&a, p), "hipPointerGetAttributes");
+        if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) fail("pointer_device", "not a device pointer");
+        *dev = a.device;
+    });
+}
+
+int s_fft_plan(int dev, int kind, int is_double, int norm, int forward, int ndim, const int64_t* shape, int naxes,
+               const int32_t* axes, void* stream, void** plan)
+{
+    return fft::seam_plan(dev, kind, is_double, norm, forward, ndim, shape, naxes, axes, stream, plan);
+}
+
+int s_fft_exec(void* plan, const void* in, void* out, void* stream) { return fft::seam_exec(plan, in, out, stream); }
+
+void s_fft_destroy(void* plan) { fft::seam_destroy(plan); }
+
+const AetherMemoryV1 kTable = {AETHER_MEMORY_ABI, s_probe,        s_runtime_version, s_current_device, s_last_error,
+                               s_chunk_alloc,     s_chunk_release, s_drain,           s_fill,           s_copy,
+                               s_stream_wait,     s_stream_sync,  s_pointer_device,  s_fft_plan,       s_fft_exec,
+                               s_fft_destroy};
+
+} // namespace
+} // namespace aeth
+
+extern "C" __attribute__((visibility("default"))) const AetherMemoryV1* aether_memory_v1(void) { return &aeth::kTable; }

@@ -37,7 +37,35 @@ def parse():
     p.add_argument("--log2n", type=int, default=27)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-log2n", type=int, default=24)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run the seam's device copy kernel over 3 GiB (known bytes; PMC calibration)")
     return p.parse_args()
+
+
+def copy_calibration(steps: int) -> None:
+    """Copy 3 GiB device-to-device with the seam's copy kernel: reads and writes exactly 3 GiB per launch."""
+    import ctypes as C
+
+    import torch
+
+    from aether_amd import _lib
+
+    L = _lib.lib()
+    L.aether_memory_v1.restype = C.c_void_p
+    table = L.aether_memory_v1()
+    # field 9 of AetherMemoryV1 is `copy` (after abi padded to 8 bytes and 8 function pointers)
+    fp = C.cast(table + 8 + 8 * 8, C.POINTER(C.c_void_p)).contents.value
+    copy = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t,
+                       C.c_void_p)(fp)
+    nbytes = 3 << 30
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    s = torch.cuda.current_stream().cuda_stream
+    for _ in range(steps):
+        assert copy(1, 0, b.data_ptr(), 1, 0, a.data_ptr(), nbytes, s) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(a[:16], b[:16])
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": steps}))
 
 
 def cpu_baseline(log2n: int) -> dict:
@@ -80,6 +108,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(the sample axis shards per GPU with no exchange except 3 sum pairs)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.steps)
+        return
     import torch
 
     import aether_amd as ae

@@ -27,6 +27,43 @@ extern "C" {
 
 enum { AETH_F64 = 0, AETH_F32 = 1 };
 
+/* ------------------------------------------------------------------ the memory seam */
+/* `aether-memory/1`: the reference's C seam between aether._core and its
+ * device plugin, same table layout, field order, kinds and return codes
+ * (dsc/src/seam.h:16-51), so the reference's loader (dsc/src/aether_core.cpp:
+ * 95-113: dlopen, abi check, probe) binds this library unchanged. Kinds:
+ * CPU, DEVICE (HIP device memory), PINNED (hipHostMalloc). FFT plans here
+ * cover C2C over power-of-two axes (<= 8192); R2C/C2R and other lengths
+ * return 2. */
+#define AETHER_MEMORY_ABI 2
+enum { AETHER_MEM_CPU = 0, AETHER_MEM_DEVICE = 1, AETHER_MEM_PINNED = 2 };
+enum { AETHER_FFT_C2C = 0, AETHER_FFT_R2C = 1, AETHER_FFT_C2R = 2 };
+enum { AETHER_FFT_BACKWARD = 0, AETHER_FFT_FORWARD = 1, AETHER_FFT_ORTHO = 2 };
+
+typedef struct AetherMemoryV1 {
+    uint32_t abi;
+    int (*probe)(int* device_count);
+    int (*runtime_version)(int* version);
+    int (*current_device)(int* dev);
+    const char* (*last_error)(void);
+    int (*chunk_alloc)(int kind, int dev, size_t bytes, void* stream, void** chunk, void** ptr);
+    void (*chunk_release)(void* chunk);
+    void (*drain)(void);
+    int (*fill)(int kind, int dev, void* ptr, size_t bytes, const void* value, size_t width, void* stream);
+    int (*copy)(int dst_kind, int dst_dev, void* dst, int src_kind, int src_dev, const void* src, size_t bytes,
+                void* stream);
+    int (*stream_wait)(int dev, void* producer, void* consumer);
+    int (*stream_sync)(int dev, void* stream);
+    int (*pointer_device)(const void* ptr, int* dev);
+    int (*fft_plan)(int dev, int kind, int is_double, int norm, int forward, int ndim, const int64_t* shape, int naxes,
+                    const int32_t* axes, void* stream, void** plan);
+    int (*fft_exec)(void* plan, const void* in, void* out, void* stream);
+    void (*fft_destroy)(void* plan);
+} AetherMemoryV1;
+
+/* dsc/src/plugin.cpp:156 (the plugin's only export) */
+const AetherMemoryV1* aether_memory_v1(void);
+
 /* ------------------------------------------------------------------ errors */
 /* dsc/src/plugin.cpp:22-39 (thread-local last_error) */
 const char* aeth_last_error(void);

@@ -1,0 +1,175 @@
+"""GPU: the `aether-memory/1` seam (include/aether_hip.h, mirroring the
+reference's dsc/src/seam.h) behaves as the reference's plugin does
+(dsc/src/plugin.cpp): stream-ordered chunks, fill with a pattern, ordered
+copies between kinds, deferred frees under capture, FFT plans over axes
+(vs the long-double DFT oracle with the reference's bound, and numpy.fft).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from aether_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+CPU, DEVICE, PINNED = 0, 1, 2
+_vp, _i, _sz = C.c_void_p, C.c_int, C.c_size_t
+
+
+class Seam(C.Structure):
+    _fields_ = [
+        ("abi", C.c_uint32),
+        ("probe", C.CFUNCTYPE(_i, C.POINTER(_i))),
+        ("runtime_version", C.CFUNCTYPE(_i, C.POINTER(_i))),
+        ("current_device", C.CFUNCTYPE(_i, C.POINTER(_i))),
+        ("last_error", C.CFUNCTYPE(C.c_char_p)),
+        ("chunk_alloc", C.CFUNCTYPE(_i, _i, _i, _sz, _vp, C.POINTER(_vp), C.POINTER(_vp))),
+        ("chunk_release", C.CFUNCTYPE(None, _vp)),
+        ("drain", C.CFUNCTYPE(None)),
+        ("fill", C.CFUNCTYPE(_i, _i, _i, _vp, _sz, _vp, _sz, _vp)),
+        ("copy", C.CFUNCTYPE(_i, _i, _i, _vp, _i, _i, _vp, _sz, _vp)),
+        ("stream_wait", C.CFUNCTYPE(_i, _i, _vp, _vp)),
+        ("stream_sync", C.CFUNCTYPE(_i, _i, _vp)),
+        ("pointer_device", C.CFUNCTYPE(_i, _vp, C.POINTER(_i))),
+        ("fft_plan", C.CFUNCTYPE(_i, _i, _i, _i, _i, _i, _i, C.POINTER(C.c_int64), _i, C.POINTER(C.c_int32), _vp,
+                                 C.POINTER(_vp))),
+        ("fft_exec", C.CFUNCTYPE(_i, _vp, _vp, _vp, _vp)),
+        ("fft_destroy", C.CFUNCTYPE(None, _vp)),
+    ]
+
+
+@pytest.fixture(scope="module")
+def seam(dev):
+    L = _lib.lib()
+    fn = L.aether_memory_v1
+    fn.restype = C.POINTER(Seam)
+    t = fn().contents
+    assert t.abi == 2
+    n = _i()
+    assert t.probe(C.byref(n)) == 0 and n.value >= 1
+    return t
+
+
+def alloc(seam, kind, nbytes, stream=None):
+    ch, p = _vp(), _vp()
+    assert seam.chunk_alloc(kind, 0, nbytes, stream, C.byref(ch), C.byref(p)) == 0, seam.last_error()
+    return ch, p.value
+
+
+def test_fill_copy_roundtrip(seam):
+    n = 1 << 20
+    s = torch.cuda.current_stream().cuda_stream
+    dch, d = alloc(seam, DEVICE, n * 8, s)
+    hch, h = alloc(seam, PINNED, n * 8)
+    val = C.c_double(3.25)
+    assert seam.fill(DEVICE, 0, d, n * 8, C.byref(val), 8, s) == 0
+    assert seam.copy(PINNED, 0, h, DEVICE, 0, d, n * 8, s) == 0
+    assert seam.stream_sync(0, s) == 0
+    arr = np.ctypeslib.as_array(C.cast(h, C.POINTER(C.c_double)), shape=(n,))
+    assert np.all(arr == 3.25)
+    # host -> device -> device -> host with a pattern, unaligned tail
+    src = np.arange(n, dtype=np.float64)
+    d2ch, d2 = alloc(seam, DEVICE, n * 8, s)
+    assert seam.copy(DEVICE, 0, d, CPU, 0, src.ctypes.data, n * 8 - 8, s) == 0
+    assert seam.copy(DEVICE, 0, d2, DEVICE, 0, d, n * 8 - 8, s) == 0
+    out = np.zeros(n)
+    assert seam.copy(CPU, 0, out.ctypes.data, DEVICE, 0, d2, n * 8, s) == 0
+    assert np.array_equal(out[:-1], src[:-1]) and out[-1] == 3.25
+    dv = _i(-1)
+    assert seam.pointer_device(d, C.byref(dv)) == 0 and dv.value == 0
+    for ch in (dch, hch, d2ch):
+        seam.chunk_release(ch)
+
+
+def test_fill_patterns_and_errors(seam):
+    s = torch.cuda.current_stream().cuda_stream
+    ch, d = alloc(seam, DEVICE, 4096, s)
+    pat = (C.c_uint8 * 4)(1, 2, 3, 4)
+    assert seam.fill(DEVICE, 0, d, 4096, pat, 4, s) == 0
+    out = np.zeros(4096, np.uint8)
+    assert seam.copy(CPU, 0, out.ctypes.data, DEVICE, 0, d, 4096, s) == 0
+    assert np.array_equal(out, np.tile([1, 2, 3, 4], 1024))
+    assert seam.fill(DEVICE, 0, d, 4095, pat, 4, s) == 1  # byte count not a multiple of the width
+    assert b"width" in seam.last_error()
+    seam.chunk_release(ch)
+
+
+def test_fill_and_copy_in_graph_replays(seam):
+    # capture-legal: replay several times with eager work between, every replay checked
+    n = 1 << 16
+    a = torch.zeros(n, dtype=torch.float64, device="cuda")
+    b = torch.zeros(n, dtype=torch.float64, device="cuda")
+    g = torch.cuda.CUDAGraph()
+    v = C.c_double(0.0)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        s = torch.cuda.current_stream().cuda_stream
+        v.value = 7.5
+        assert seam.fill(DEVICE, 0, a.data_ptr(), n * 8, C.byref(v), 8, s) == 0
+        assert seam.copy(DEVICE, 0, b.data_ptr(), DEVICE, 0, a.data_ptr(), n * 8, s) == 0
+    for k in range(4):
+        a.fill_(float(k))
+        b.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.all(a == 7.5) and torch.all(b == 7.5)
+
+
+def test_release_under_capture_is_deferred(seam):
+    side = torch.cuda.Stream()
+    ch, _ = alloc(seam, DEVICE, 1 << 20, side.cuda_stream)
+    x = torch.zeros(16, device="cuda")
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        x.add_(1.0)
+        seam.chunk_release(ch)  # queued, not freed inside the capture
+    seam.drain()
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.all(x == 1.0)
+
+
+@pytest.mark.parametrize("shape,axes", [((8, 64), (1,)), ((64, 8), (0,)), ((16, 32), (0, 1)), ((4, 8, 16), (1,)),
+                                        ((4, 8, 16), (0, 2)), ((1024,), (0,))])
+@pytest.mark.parametrize("norm,forward", [(0, 1), (0, 0), (2, 1)])
+def test_fft_plan_axes(seam, orc, shape, axes, norm, forward):
+    rng = np.random.default_rng(len(shape) * 10 + len(axes))
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    xin = torch.from_numpy(x).to("cuda")
+    out = torch.zeros_like(xin)
+    s = torch.cuda.current_stream().cuda_stream
+    shp = (C.c_int64 * len(shape))(*shape)
+    ax = (C.c_int32 * len(axes))(*axes)
+    plan = _vp()
+    assert seam.fft_plan(0, 0, 1, norm, forward, len(shape), shp, len(axes), ax, s, C.byref(plan)) == 0, \
+        seam.last_error()
+    assert seam.fft_exec(plan, xin.data_ptr(), out.data_ptr(), s) == 0
+    torch.cuda.synchronize()
+    seam.fft_destroy(plan)
+    y = out.cpu().numpy()
+    npn = {0: "backward", 1: "forward", 2: "ortho"}[norm]
+    ref = np.fft.fftn(x, axes=axes, norm=npn) if forward else np.fft.ifftn(x, axes=axes, norm=npn)
+    points = int(np.prod([shape[a] for a in axes]))
+    bound = (8 + 4 * np.log2(points + 1)) * 2.0**-52 * np.linalg.norm(x) * max(1.0, np.abs(ref).max() /
+                                                                                max(np.abs(x).max(), 1e-300))
+    assert np.max(np.abs(y - ref)) <= bound
+    if len(shape) == 1 and forward and norm == 0:
+        assert np.max(np.abs(y - orc.dft(x))) <= (8 + 4 * np.log2(points + 1)) * 2.0**-52 * np.linalg.norm(x)
+
+
+def test_fft_plan_unsupported(seam):
+    s = torch.cuda.current_stream().cuda_stream
+    shp = (C.c_int64 * 1)(12)
+    ax = (C.c_int32 * 1)(0)
+    plan = _vp()
+    assert seam.fft_plan(0, 0, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 2
+    shp = (C.c_int64 * 1)(16)
+    assert seam.fft_plan(0, 1, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 2  # R2C
+    ax2 = (C.c_int32 * 2)(0, 0)
+    shp2 = (C.c_int64 * 2)(4, 4)
+    assert seam.fft_plan(0, 0, 1, 0, 1, 2, shp2, 2, ax2, s, C.byref(plan)) == 1  # repeated axis
