@@ -73,7 +73,7 @@ def test_fill_copy_roundtrip(seam):
     src = np.arange(n, dtype=np.float64)
     d2ch, d2 = alloc(seam, DEVICE, n * 8, s)
     assert seam.copy(DEVICE, 0, d, CPU, 0, src.ctypes.data, n * 8 - 8, s) == 0
-    assert seam.copy(DEVICE, 0, d2, DEVICE, 0, d, n * 8 - 8, s) == 0
+    assert seam.copy(DEVICE, 0, d2, DEVICE, 0, d, n * 8, s) == 0  # carries d's filled last element
     out = np.zeros(n)
     assert seam.copy(CPU, 0, out.ctypes.data, DEVICE, 0, d2, n * 8, s) == 0
     assert np.array_equal(out[:-1], src[:-1]) and out[-1] == 3.25
