@@ -25,6 +25,7 @@ __all__ = [
 F64, F32 = 0, 1
 SUM_NONE, SUM_NEUMAIER, SUM_PLAIN, SUM_FLOATFLOAT = -1, 0, 1, 2
 RNG_BITS32, RNG_UNIFORM_F64, RNG_UNIFORM_F32, RNG_NORMAL_F64, RNG_NORMAL_F32 = 0, 1, 2, 3, 4
+AETH_RNG_DIST_UNIFORM, AETH_RNG_DIST_NORMAL = 0, 1
 FN = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "all4": 4}
 
 CAPACITY_ALIGNMENT = 32  # array/Array.h:80 kCapacityAlignment
@@ -106,6 +107,23 @@ class Array:
     def host(self):
         """(ncomp, n) host copy (download, array/Array.h:258-268)."""
         return self.data[:, : self.n].cpu()
+
+    # an Array in an expression is a leaf (expr/Operations.h:25-71)
+    def _leaf(self):
+        from .expr import Leaf
+
+        return Leaf(self)
+
+    def __add__(self, o): return self._leaf() + o
+    def __radd__(self, o): return o + self._leaf()
+    def __sub__(self, o): return self._leaf() - o
+    def __rsub__(self, o): return o - self._leaf()
+    def __mul__(self, o): return self._leaf() * o
+    def __rmul__(self, o): return o * self._leaf()
+    def __truediv__(self, o): return self._leaf() / o
+    def __neg__(self): return -self._leaf()
+    def dot(self, o): return self._leaf().dot(o)
+    def cross(self, o): return self._leaf().cross(o)
 
 
 def rng_fill(kind: int, seed: int, out: torch.Tensor, first: int = 0, stream=None) -> torch.Tensor:

@@ -27,6 +27,7 @@ SIGNATURES = {
     "aeth_array_fill": (_i32, [_i32, _i32, _u64, _f64, _f64, _i32, _u64, _u64, _vp, _vp]),
     "aeth_eval_cfg0": (_i32, [_i32, _vp, _vp, _vp, _vp, _i32, _u64, _u64, _i32, _vp, _vp]),
     "aeth_eval_vec3geo": (_i32, [_vp, _vp, _vp, _vp, _u64, _u64, _vp]),
+    "aeth_eval_tape": (_i32, [_i32, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _u64, _i32, _vp, _vp]),
     "aeth_reduce": (_i32, [_i32, _i32, _i32, _vp, _i32, _u64, _u64, _vp, _vp]),
     "aeth_math": (_i32, [_i32, _i32, _vp, _vp, _u64, _vp]),
     "aeth_fft_batch_plan": (_i32, [_i32, _u64, _vp, _vp, _vp, C.POINTER(_vp)]),

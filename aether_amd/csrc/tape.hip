@@ -1,0 +1,427 @@
+// tape.hip -- generic lowering of `dest = expr` to one gfx950 kernel.
+//
+// aether evaluates an assignment per sample with the component loop unrolled
+// at compile time (expr/Assign.h:124-160) and its runtime evaluator applies
+// one of six fixed ops (eval/Runtime.h:73-167). Here any expression tree the
+// host lowers becomes one PROGRAM per output component: the planes it reads
+// and a postfix tape. A block owns one (program, 2048-sample tile) unit; all
+// planes of the program are loaded first with 16-byte lanes (issued together,
+// so their latencies overlap), then the tape runs on a register stack. The
+// tape is wave-uniform (it lives in the kernel arguments), so dispatch is
+// scalar branching only; each transcendental is instantiated once and fed
+// through a register. The tile and sum shape are those of eval.hip, so a tape
+// and the specialised kernel for the same tree agree bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "../../include/aether_hip.h"
+#include "accum.hpp"
+#include "common.hpp"
+#include "dmath.hpp"
+#include "rng.hpp"
+
+namespace aeth {
+namespace {
+
+constexpr int kTile = 2048;
+constexpr int kThreads = 256;
+constexpr int kGroup = 1024;
+constexpr int kDepth = AETH_TAPE_MAX_DEPTH;
+
+struct TapeArgs {
+    aeth_program prog[AETH_TAPE_MAX_PROGRAMS];
+    const void* leaf[AETH_TAPE_MAX_LEAVES];
+    uint64_t pitch[AETH_TAPE_MAX_LEAVES];
+    double consts[AETH_TAPE_MAX_CONSTS];
+    aeth_rng_leaf rng[AETH_TAPE_MAX_RNGS];
+    uint32_t n, tiles, units;
+};
+static_assert(sizeof(TapeArgs) < 4000, "kernel arguments must fit the 4 KiB kernarg segment");
+
+template<class T>
+using ext16 = T __attribute__((ext_vector_type(16 / sizeof(T))));
+
+template<bool NT, class T, int VN>
+__device__ __forceinline__ void vload(const T* p, T (&v)[VN])
+{
+    using VT = ext16<T>;
+    VT x;
+    if constexpr (NT) x = __builtin_nontemporal_load(reinterpret_cast<const VT*>(p));
+    else x = *reinterpret_cast<const VT*>(p);
+#pragma unroll
+    for (int k = 0; k < VN; ++k) v[k] = x[k];
+}
+
+template<bool NT, class T, int VN>
+__device__ __forceinline__ void vstore(T* p, const T (&v)[VN])
+{
+    using VT = ext16<T>;
+    VT x;
+#pragma unroll
+    for (int k = 0; k < VN; ++k) x[k] = v[k];
+    if constexpr (NT) __builtin_nontemporal_store(x, reinterpret_cast<VT*>(p));
+    else *reinterpret_cast<VT*>(p) = x;
+}
+
+template<class T, int VN>
+__device__ __forceinline__ void get(const T (&S)[kDepth][VN], int d, T (&x)[VN])
+{
+    switch (d) {
+#define AETH_GET(D)                                                                                                   \
+    case D:                                                                                                           \
+        for (int k = 0; k < VN; ++k) x[k] = S[D][k];                                                                  \
+        break;
+        AETH_GET(0) AETH_GET(1) AETH_GET(2) AETH_GET(3) AETH_GET(4) AETH_GET(5) AETH_GET(6) AETH_GET(7)
+#undef AETH_GET
+    default: break;
+    }
+}
+
+template<class T, int VN>
+__device__ __forceinline__ void put(T (&S)[kDepth][VN], int d, const T (&x)[VN])
+{
+    switch (d) {
+#define AETH_PUT(D)                                                                                                   \
+    case D:                                                                                                           \
+        for (int k = 0; k < VN; ++k) S[D][k] = x[k];                                                                  \
+        break;
+        AETH_PUT(0) AETH_PUT(1) AETH_PUT(2) AETH_PUT(3) AETH_PUT(4) AETH_PUT(5) AETH_PUT(6) AETH_PUT(7)
+#undef AETH_PUT
+    default: break;
+    }
+}
+
+template<class T>
+__device__ __forceinline__ T unary(int op, T a)
+{
+    if constexpr (sizeof(T) == 8) {
+        switch (op) {
+        case AETH_OP_NEG: return -a;
+        case AETH_OP_SIN: return dm::sin(a);
+        case AETH_OP_COS: return dm::cos(a);
+        case AETH_OP_EXP: return dm::exp(a);
+        case AETH_OP_LOG: return dm::log(a);
+        case AETH_OP_SQRT: return __builtin_sqrt(a);
+        default: return __builtin_fabs(a);
+        }
+    } else {
+        switch (op) {
+        case AETH_OP_NEG: return -a;
+        case AETH_OP_SIN: return dm::sinf(a);
+        case AETH_OP_COS: return dm::cosf(a);
+        case AETH_OP_EXP: return dm::expf(a);
+        case AETH_OP_LOG: return dm::logf(a);
+        case AETH_OP_SQRT: return __builtin_sqrtf(a);
+        default: return __builtin_fabsf(a);
+        }
+    }
+}
+
+template<class T>
+__device__ __forceinline__ T binary(int op, T a, T b)
+{
+    switch (op) {
+    case AETH_OP_ADD: return a + b;
+    case AETH_OP_SUB: return a - b;
+    case AETH_OP_MUL: return a * b;
+    default: return a / b;
+    }
+}
+
+template<class T>
+__device__ __forceinline__ T draw(const aeth_rng_leaf& r, uint32_t i)
+{
+    const rng::U4 b = rng::block(r.seed, i, r.comp);
+    if constexpr (sizeof(T) == 8) {
+        if (r.dist == AETH_DIST_UNIFORM) return r.p0 + (r.p1 - r.p0) * rng::u01_from_word_d(b.x);
+        return r.p0 + r.p1 * rng::normal_from_block_d(b);
+    } else {
+        const float p0 = (float)r.p0, p1 = (float)r.p1;
+        if (r.dist == AETH_DIST_UNIFORM) return p0 + (p1 - p0) * rng::u01_from_word_f(b.x);
+        return p0 + p1 * rng::normal_from_block_f(b);
+    }
+}
+
+// Run program P on VN samples. L: the program's planes for these samples.
+template<class T, int VN, int NL>
+__device__ __forceinline__ void run(const aeth_program& P, const TapeArgs& A, const T (&L)[NL][VN],
+                                    uint32_t first_sample, T (&out)[VN])
+{
+    T S[kDepth][VN];
+    int d = 0;
+    const int ni = P.n_insns;
+    for (int pc = 0; pc < ni; ++pc) {
+        const aeth_insn w = P.insns[pc];
+        const int op = w.op;
+        T x[VN];
+        if (op == AETH_OP_PUSH) {
+            const int j = w.arg;
+#pragma unroll
+            for (int q = 0; q < NL; ++q)
+                if (q == j)
+#pragma unroll
+                    for (int k = 0; k < VN; ++k) x[k] = L[q][k];
+            put(S, d, x);
+            ++d;
+        } else if (op == AETH_OP_CONST) {
+            const T c = (T)A.consts[w.arg];
+#pragma unroll
+            for (int k = 0; k < VN; ++k) x[k] = c;
+            put(S, d, x);
+            ++d;
+        } else if (op == AETH_OP_RNG) {
+            const aeth_rng_leaf& r = A.rng[w.arg];
+#pragma unroll
+            for (int k = 0; k < VN; ++k) x[k] = draw<T>(r, first_sample + k);
+            put(S, d, x);
+            ++d;
+        } else if (op >= AETH_OP_NEG) {
+            get(S, d - 1, x);
+#pragma unroll
+            for (int k = 0; k < VN; ++k) x[k] = unary<T>(op, x[k]);
+            put(S, d - 1, x);
+        } else {
+            T y[VN];
+            get(S, d - 2, x);
+            get(S, d - 1, y);
+#pragma unroll
+            for (int k = 0; k < VN; ++k) x[k] = binary<T>(op, x[k], y[k]);
+            put(S, d - 2, x);
+            --d;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < VN; ++k) out[k] = S[0][k];
+}
+
+template<class T, int M, bool SUM, int NL, bool VECTOR, bool NT>
+__global__ __launch_bounds__(kThreads) void tape_kernel(const TapeArgs A, acc::Pair<T>* __restrict__ part)
+{
+    constexpr int VN = 16 / sizeof(T);
+    constexpr int ROUNDS = kTile / (kThreads * VN);
+    __shared__ acc::Pair<T> lds[4];
+    const uint32_t n = A.n;
+    for (uint32_t u = blockIdx.x; u < A.units; u += gridDim.x) {
+        const uint32_t pi = u / A.tiles;
+        const uint32_t tile = u - pi * A.tiles;
+        const uint32_t s0 = tile * kTile;
+        const aeth_program& P = A.prog[pi];
+        const int nl = P.n_loads;
+        T* outp = static_cast<T*>(const_cast<void*>(A.leaf[P.out_leaf])) + (uint64_t)P.out_comp * A.pitch[P.out_leaf];
+        acc::Pair<T> acc{T(0), T(0)};
+        const bool full = VECTOR && s0 + kTile <= n;
+        for (int r = 0; r < ROUNDS; ++r) {
+            const uint32_t i0 = s0 + r * (kThreads * VN) + threadIdx.x * VN;
+            T L[NL][VN];
+#pragma unroll
+            for (int q = 0; q < NL; ++q) {
+                if (q < nl) {
+                    const T* src = static_cast<const T*>(A.leaf[P.load_leaf[q]]) +
+                                   (uint64_t)P.load_comp[q] * A.pitch[P.load_leaf[q]];
+                    if (full) {
+                        vload<NT>(src + i0, L[q]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < VN; ++k) L[q][k] = (i0 + k < n) ? src[i0 + k] : T(0);
+                    }
+                }
+            }
+            T v[VN];
+            run<T, VN, NL>(P, A, L, i0, v);
+            if (full) {
+                vstore<NT>(outp + i0, v);
+                if constexpr (SUM)
+#pragma unroll
+                    for (int k = 0; k < VN; ++k) acc::accumulate<M, false>(acc, v[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < VN; ++k)
+                    if (i0 + k < n) {
+                        outp[i0 + k] = v[k];
+                        if constexpr (SUM) acc::accumulate<M, false>(acc, v[k]);
+                    }
+            }
+        }
+        if constexpr (SUM) {
+            const acc::Pair<T> rr = acc::block_combine<M>(acc, lds);
+            if (threadIdx.x == 0) part[u] = rr;
+        }
+    }
+}
+
+// Group tree of eval.hip (same shape), repeated here to keep the unit self-contained.
+template<class T, int M>
+__global__ __launch_bounds__(512) void tape_tree_kernel(const acc::Pair<T>* __restrict__ in,
+                                                        acc::Pair<T>* __restrict__ out, uint32_t count,
+                                                        uint32_t in_stride, uint32_t out_stride)
+{
+    __shared__ acc::Pair<T> s[kGroup];
+    const uint32_t g = blockIdx.x, comp = blockIdx.y, t = threadIdx.x;
+    const uint32_t first = g * kGroup;
+    const uint32_t m = min((uint32_t)kGroup, count - first);
+    const acc::Pair<T>* src = in + (uint64_t)comp * in_stride + first;
+    if (t < m) s[t] = src[t];
+    if (t + 512 < m) s[t + 512] = src[t + 512];
+    for (uint32_t stride = kGroup / 2; stride >= 1; stride >>= 1) {
+        __syncthreads();
+        if (t < stride && t + stride < m) s[t] = acc::combine<M>(s[t], s[t + stride]);
+    }
+    if (t == 0) out[(uint64_t)comp * out_stride + g] = s[0];
+}
+
+template<class T>
+__global__ void tape_store_pairs(const acc::Pair<T>* __restrict__ in, uint32_t in_stride, T* __restrict__ out, int np)
+{
+    const int c = threadIdx.x;
+    if (c < np) {
+        out[2 * c] = in[(uint64_t)c * in_stride].h;
+        out[2 * c + 1] = in[(uint64_t)c * in_stride].c;
+    }
+}
+
+// Host-side validation: stack effects, ranges, one value left.
+void validate(const aeth_program& p, int n_leaves, int n_consts, int n_rngs)
+{
+    if (p.n_loads > AETH_TAPE_MAX_LOADS || p.n_insns > AETH_TAPE_MAX_INSNS || p.n_insns == 0)
+        fail("aeth_eval_tape", "program size out of range");
+    if (p.out_leaf >= n_leaves) fail("aeth_eval_tape", "output leaf out of range");
+    for (int k = 0; k < p.n_loads; ++k)
+        if (p.load_leaf[k] >= n_leaves) fail("aeth_eval_tape", "load leaf out of range");
+    int d = 0;
+    for (int i = 0; i < p.n_insns; ++i) {
+        const int op = p.insns[i].op, arg = p.insns[i].arg;
+        if (op == AETH_OP_PUSH) {
+            if (arg >= p.n_loads) fail("aeth_eval_tape", "PUSH index out of range");
+            ++d;
+        } else if (op == AETH_OP_CONST) {
+            if (arg >= n_consts) fail("aeth_eval_tape", "CONST index out of range");
+            ++d;
+        } else if (op == AETH_OP_RNG) {
+            if (arg >= n_rngs) fail("aeth_eval_tape", "RNG index out of range");
+            ++d;
+        } else if (op >= AETH_OP_ADD && op <= AETH_OP_DIV) {
+            if (d < 2) fail("aeth_eval_tape", "stack underflow");
+            --d;
+        } else if (op >= AETH_OP_NEG && op <= AETH_OP_ABS) {
+            if (d < 1) fail("aeth_eval_tape", "stack underflow");
+        } else {
+            fail("aeth_eval_tape", "unknown opcode");
+        }
+        if (d > kDepth) fail("aeth_eval_tape", "stack deeper than 8");
+    }
+    if (d != 1) fail("aeth_eval_tape", "a program must leave exactly one value");
+}
+
+template<class T, int M, bool SUM, int NL>
+void launch_nl(const TapeArgs& A, bool vec, bool nt, acc::Pair<T>* part, int grid, hipStream_t st)
+{
+    if (vec && nt)
+        hipLaunchKernelGGL((tape_kernel<T, M, SUM, NL, true, true>), dim3(grid), dim3(kThreads), 0, st, A, part);
+    else if (vec)
+        hipLaunchKernelGGL((tape_kernel<T, M, SUM, NL, true, false>), dim3(grid), dim3(kThreads), 0, st, A, part);
+    else
+        hipLaunchKernelGGL((tape_kernel<T, M, SUM, NL, false, false>), dim3(grid), dim3(kThreads), 0, st, A, part);
+}
+
+template<class T, int M, bool SUM>
+void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums, hipStream_t st)
+{
+    size_t pairs = (size_t)A.tiles * np;
+    for (uint32_t t = A.tiles; t > 1;) {
+        t = (t + kGroup - 1) / kGroup;
+        pairs += (size_t)t * np;
+    }
+    StreamScratch scratch(SUM ? pairs * sizeof(acc::Pair<T>) : 0, st);
+    auto* part = static_cast<acc::Pair<T>*>(scratch.get());
+    const int grid = (int)std::max<uint32_t>(1u, A.units);
+    if (max_loads <= 4) launch_nl<T, M, SUM, 4>(A, vec, nt, part, grid, st);
+    else if (max_loads <= 9) launch_nl<T, M, SUM, 9>(A, vec, nt, part, grid, st);
+    else launch_nl<T, M, SUM, 16>(A, vec, nt, part, grid, st);
+    check(hipGetLastError(), "tape_kernel launch");
+    if constexpr (SUM) {
+        acc::Pair<T>* cur = part;
+        acc::Pair<T>* next = part + (size_t)A.tiles * np;
+        uint32_t count = A.tiles, stride = A.tiles;
+        while (count > 1) {
+            const uint32_t groups = (count + kGroup - 1) / kGroup;
+            hipLaunchKernelGGL((tape_tree_kernel<T, M>), dim3(groups, np), dim3(512), 0, st, cur, next, count, stride,
+                               groups);
+            cur = next;
+            next += (size_t)groups * np;
+            stride = groups;
+            count = groups;
+        }
+        hipLaunchKernelGGL((tape_store_pairs<T>), dim3(1), dim3(64), 0, st, cur, stride, sums, np);
+        check(hipGetLastError(), "tape tree launch");
+    }
+}
+
+template<class T>
+void dispatch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, int mode, T* sums, hipStream_t st)
+{
+    switch (mode) {
+    case AETH_SUM_NONE: launch<T, acc::kNeumaier, false>(A, max_loads, np, vec, nt, sums, st); break;
+    case AETH_SUM_NEUMAIER: launch<T, acc::kNeumaier, true>(A, max_loads, np, vec, nt, sums, st); break;
+    case AETH_SUM_PLAIN: launch<T, acc::kPlain, true>(A, max_loads, np, vec, nt, sums, st); break;
+    case AETH_SUM_FLOATFLOAT:
+        if constexpr (sizeof(T) == 4) {
+            launch<T, acc::kFloatFloat, true>(A, max_loads, np, vec, nt, sums, st);
+            break;
+        } else {
+            fail("aeth_eval_tape", "float-float accumulation needs dtype F32", 0, 2);
+        }
+    default: fail("aeth_eval_tape", "unknown sum mode");
+    }
+}
+
+} // namespace
+} // namespace aeth
+
+using namespace aeth;
+
+extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_leaves,
+                              void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts,
+                              int n_rngs, const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums,
+                              void* stream)
+{
+    return guarded([&] {
+        if (dtype != AETH_F64 && dtype != AETH_F32) fail("aeth_eval_tape", "unknown dtype");
+        if (n_programs < 1 || n_programs > AETH_TAPE_MAX_PROGRAMS) fail("aeth_eval_tape", "1..8 programs");
+        if (n_leaves < 1 || n_leaves > AETH_TAPE_MAX_LEAVES) fail("aeth_eval_tape", "1..16 leaves");
+        if (n_consts < 0 || n_consts > AETH_TAPE_MAX_CONSTS) fail("aeth_eval_tape", "0..16 constants");
+        if (n_rngs < 0 || n_rngs > AETH_TAPE_MAX_RNGS) fail("aeth_eval_tape", "0..8 rng leaves");
+        if (n > 0xFFFFFFFFull) fail("aeth_eval_tape", "n beyond the 32-bit offset_t");
+        if (sum_mode != AETH_SUM_NONE && !sums) fail("aeth_eval_tape", "sums is NULL");
+        TapeArgs A;
+        std::memset(&A, 0, sizeof A);
+        const size_t es = dtype == AETH_F64 ? 8 : 4;
+        bool vec = true;
+        int max_loads = 0;
+        for (int p = 0; p < n_programs; ++p) {
+            validate(programs[p], n_leaves, n_consts, n_rngs);
+            A.prog[p] = programs[p];
+            max_loads = std::max<int>(max_loads, programs[p].n_loads);
+        }
+        uint64_t span_bytes = 0;
+        for (int l = 0; l < n_leaves; ++l) {
+            A.leaf[l] = leaves[l];
+            A.pitch[l] = leaf_pitch[l];
+            if (leaf_pitch[l] < n) fail("aeth_eval_tape", "leaf pitch < n");
+            vec = vec && aligned16(leaves[l]) && (leaf_pitch[l] * es) % 16 == 0;
+        }
+        for (int p = 0; p < n_programs; ++p) span_bytes += (uint64_t)(programs[p].n_loads + 1) * n * es;
+        for (int c = 0; c < n_consts; ++c) A.consts[c] = consts[c];
+        for (int r = 0; r < n_rngs; ++r) A.rng[r] = rngs[r];
+        A.n = (uint32_t)n;
+        A.tiles = n ? (uint32_t)((n + kTile - 1) / kTile) : 1u;
+        A.units = A.tiles * (uint32_t)n_programs;
+        const bool nt = span_bytes >= (512ull << 20);
+        hipStream_t st = as_stream(stream);
+        if (dtype == AETH_F64)
+            dispatch<double>(A, max_loads, n_programs, vec, nt, sum_mode, static_cast<double*>(sums), st);
+        else
+            dispatch<float>(A, max_loads, n_programs, vec, nt, sum_mode, static_cast<float*>(sums), st);
+    });
+}

@@ -1,0 +1,281 @@
+"""Expression trees over device Arrays, lowered to one HIP kernel per assignment.
+
+The Python spelling of aether's lazy assignment (expr/Operations.h:25-71,
+expr/nodes/Arithmetic.h:33-143, expr/nodes/Elementwise.h:245-533,
+expr/nodes/Geometric.h:32-73, expr/Reduce.h:150-190):
+
+    d = a + b * c + sin(a) * exp(-1.0 * (b * b))
+    assign(D, d)                       # one kernel, optional fused sums
+
+Each output component becomes one program (aeth_eval_tape): the planes it
+reads and a postfix tape in aether's evaluation order. `-x` is `x * -1`
+(CWiseScale), `x / s` for a scalar s is `x * (1/s)` with the reciprocal
+rounded in the element type (expr/Operations.h:62-71), folds are
+right-nested `e[N-1] + (... + e[0])` (docs/content/numerics_contract.rst:12-24).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import AETH_RNG_DIST_NORMAL, AETH_RNG_DIST_UNIFORM, Array, _dtype_code, stream_handle
+from ._lib import call
+
+OP = {"push": 0, "const": 1, "rng": 2, "+": 3, "-": 4, "*": 5, "/": 6, "neg": 7, "sin": 8, "cos": 9, "exp": 10,
+      "log": 11, "sqrt": 12, "abs": 13}
+MAX_LOADS, MAX_INSNS, MAX_PROGRAMS, MAX_LEAVES, MAX_CONSTS, MAX_RNGS = 16, 64, 8, 16, 16, 8
+
+
+class aeth_insn(C.Structure):
+    _fields_ = [("op", C.c_uint8), ("arg", C.c_uint8)]
+
+
+class aeth_program(C.Structure):
+    _fields_ = [("n_loads", C.c_uint8), ("n_insns", C.c_uint8), ("out_leaf", C.c_uint8), ("out_comp", C.c_uint8),
+                ("load_leaf", C.c_uint8 * MAX_LOADS), ("load_comp", C.c_uint8 * MAX_LOADS),
+                ("insns", aeth_insn * MAX_INSNS)]
+
+
+class aeth_rng_leaf(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("p0", C.c_double), ("p1", C.c_double), ("comp", C.c_uint32),
+                ("dist", C.c_uint32)]
+
+
+def _wrap(x) -> "Expr":
+    if isinstance(x, Expr):
+        return x
+    if isinstance(x, Array):
+        return Leaf(x)
+    if isinstance(x, (int, float)):
+        return Const(float(x))
+    raise TypeError(f"cannot use {type(x)} in an expression")
+
+
+class Expr:
+    ncomp = 1
+
+    def __add__(self, o): return Bin("+", self, _wrap(o))
+    def __radd__(self, o): return Bin("+", _wrap(o), self)
+    def __sub__(self, o): return Bin("-", self, _wrap(o))
+    def __rsub__(self, o): return Bin("-", _wrap(o), self)
+    def __mul__(self, o): return Bin("*", self, _wrap(o))
+    def __rmul__(self, o): return Bin("*", _wrap(o), self)
+
+    def __truediv__(self, o):
+        if isinstance(o, (int, float)):
+            return Recip(self, float(o))  # x * (1/s): expr/Operations.h:62-71
+        return Bin("/", self, _wrap(o))
+
+    def __neg__(self): return Bin("*", self, Const(-1.0))  # numerics_contract.rst:66
+
+    def dot(self, o): return Dot(self, _wrap(o))
+    def cross(self, o): return Cross(self, _wrap(o))
+    def squared_norm(self): return Dot(self, self)
+    def norm(self): return Un("sqrt", Dot(self, self))
+
+    def emit(self, c: int, ctx: "_Ctx") -> None:
+        raise NotImplementedError
+
+
+class Leaf(Expr):
+    def __init__(self, arr: Array):
+        self.arr = arr
+        self.ncomp = arr.ncomp
+
+    def emit(self, c, ctx):
+        ctx.push(self.arr, c if self.ncomp > 1 else 0)
+
+
+class Const(Expr):
+    def __init__(self, v: float):
+        self.v = v
+
+    def emit(self, c, ctx):
+        ctx.const(self.v)
+
+
+class Recip(Expr):
+    def __init__(self, e: Expr, s: float):
+        self.e, self.s = e, s
+        self.ncomp = e.ncomp
+
+    def emit(self, c, ctx):
+        self.e.emit(c, ctx)
+        r = (np.float32(1) / np.float32(self.s)) if ctx.dtype == torch.float32 else 1.0 / self.s
+        ctx.const(float(r))
+        ctx.op("*")
+
+
+class Random(Expr):
+    """A distribution leaf (random/detail/DistributionLeaf.h:111-236)."""
+
+    def __init__(self, seed: int, ncomp: int, dist: int, p0: float, p1: float):
+        self.seed, self.ncomp, self.dist, self.p0, self.p1 = seed, ncomp, dist, p0, p1
+
+    def emit(self, c, ctx):
+        ctx.rng(self.seed, c, self.dist, self.p0, self.p1)
+
+
+def uniform(generator, ncomp: int, lo: float, hi: float) -> Random:
+    """Generator(s).uniform<Real, ncomp>(lo, hi) (random/Generator.h:86-90)."""
+    return Random(generator.uniform_seed(), ncomp, AETH_RNG_DIST_UNIFORM, lo, hi)
+
+
+def normal(generator, ncomp: int, mean: float, sd: float) -> Random:
+    """Generator(s).normal<Real, ncomp>(mean, sd) (random/Generator.h:94-104)."""
+    return Random(generator.normal_seed(), ncomp, AETH_RNG_DIST_NORMAL, mean, sd)
+
+
+class Bin(Expr):
+    def __init__(self, op, l: Expr, r: Expr):
+        self.op, self.l, self.r = op, l, r
+        self.ncomp = max(l.ncomp, r.ncomp)
+        if l.ncomp > 1 and r.ncomp > 1 and l.ncomp != r.ncomp:
+            raise ValueError("component counts differ")
+
+    def emit(self, c, ctx):
+        self.l.emit(c if self.l.ncomp > 1 else 0, ctx)
+        self.r.emit(c if self.r.ncomp > 1 else 0, ctx)
+        ctx.op(self.op)
+
+
+class Un(Expr):
+    def __init__(self, fn, e: Expr):
+        self.fn, self.e = fn, e
+        self.ncomp = e.ncomp
+
+    def emit(self, c, ctx):
+        self.e.emit(c, ctx)
+        ctx.op(self.fn)
+
+
+class Dot(Expr):
+    """Right-nested fold (x_{N-1}*y_{N-1}) + (... + (x_0*y_0)) (expr/Reduce.h:172-188)."""
+
+    def __init__(self, a: Expr, b: Expr):
+        if a.ncomp != b.ncomp:
+            raise ValueError("dot: component counts differ")
+        self.a, self.b = a, b
+
+    def emit(self, c, ctx):
+        n = self.a.ncomp
+        # postfix of (t_{n-1} + (t_{n-2} + (... + t_0)))
+        for k in range(n - 1, -1, -1):
+            self.a.emit(k, ctx)
+            self.b.emit(k, ctx)
+            ctx.op("*")
+        for _ in range(n - 1):
+            ctx.op("+")
+
+
+class Cross(Expr):
+    """Geometric.h:55-71: (l1 r2 - l2 r1, l2 r0 - l0 r2, l0 r1 - l1 r0)."""
+
+    ncomp = 3
+
+    def __init__(self, l: Expr, r: Expr):
+        if l.ncomp != 3 or r.ncomp != 3:
+            raise ValueError("cross needs 3-vectors")
+        self.l, self.r = l, r
+
+    def emit(self, c, ctx):
+        i, j = [(1, 2), (2, 0), (0, 1)][c]
+        self.l.emit(i, ctx)
+        self.r.emit(j, ctx)
+        ctx.op("*")
+        self.l.emit(j, ctx)
+        self.r.emit(i, ctx)
+        ctx.op("*")
+        ctx.op("-")
+
+
+def sin(e): return Un("sin", _wrap(e))
+def cos(e): return Un("cos", _wrap(e))
+def exp(e): return Un("exp", _wrap(e))
+def log(e): return Un("log", _wrap(e))
+def sqrt(e): return Un("sqrt", _wrap(e))
+def abs_(e): return Un("abs", _wrap(e))
+
+
+# aether's names (expr/nodes/Elementwise.h:500-533)
+cwiseSin, cwiseCos, cwiseExp, cwiseLog, cwiseSqrt, cwiseAbs = sin, cos, exp, log, sqrt, abs_
+
+
+class _Ctx:
+    def __init__(self, dtype, leaves, consts, rngs):
+        self.dtype, self.leaves, self.consts, self.rngs = dtype, leaves, consts, rngs
+        self.loads, self.insns = [], []
+
+    def _leaf(self, arr: Array) -> int:
+        for k, x in enumerate(self.leaves):
+            if x is arr:
+                return k
+        if len(self.leaves) >= MAX_LEAVES:
+            raise ValueError("too many leaves")
+        self.leaves.append(arr)
+        return len(self.leaves) - 1
+
+    def push(self, arr, comp):
+        key = (self._leaf(arr), comp)
+        if key not in self.loads:
+            self.loads.append(key)
+        self.insns.append((OP["push"], self.loads.index(key)))
+
+    def const(self, v):
+        if v not in self.consts:
+            self.consts.append(v)
+        self.insns.append((OP["const"], self.consts.index(v)))
+
+    def rng(self, seed, comp, dist, p0, p1):
+        key = (seed & ((1 << 64) - 1), float(p0), float(p1), comp, dist)
+        if key not in self.rngs:
+            self.rngs.append(key)
+        self.insns.append((OP["rng"], self.rngs.index(key)))
+
+    def op(self, name):
+        self.insns.append((OP[name], 0))
+
+
+def lower(dest: Array, expr: Expr):
+    """Lower `dest = expr` to (programs, leaves, consts, rngs)."""
+    expr = _wrap(expr)
+    if expr.ncomp not in (1, dest.ncomp):
+        raise ValueError("expression and destination component counts differ")
+    leaves, consts, rngs = [], [], []
+    progs = (aeth_program * dest.ncomp)()
+    for c in range(dest.ncomp):
+        ctx = _Ctx(dest.dtype, leaves, consts, rngs)
+        expr.emit(c if expr.ncomp > 1 else 0, ctx)
+        out_leaf = ctx._leaf(dest)
+        p = progs[c]
+        if len(ctx.loads) > MAX_LOADS or len(ctx.insns) > MAX_INSNS:
+            raise ValueError("expression too large for one program")
+        p.n_loads, p.n_insns, p.out_leaf, p.out_comp = len(ctx.loads), len(ctx.insns), out_leaf, c
+        for k, (lf, cp) in enumerate(ctx.loads):
+            p.load_leaf[k], p.load_comp[k] = lf, cp
+        for k, (o, a) in enumerate(ctx.insns):
+            p.insns[k].op, p.insns[k].arg = o, a
+    if len(consts) > MAX_CONSTS or len(rngs) > MAX_RNGS:
+        raise ValueError("too many constants or random leaves")
+    return progs, leaves, consts, rngs
+
+
+def assign(dest: Array, expr, sum_mode: int = -1, sums: torch.Tensor = None, stream=None):
+    """`dest = expr` over all samples as ONE kernel; returns (ncomp, 2) sums when sum_mode >= 0."""
+    progs, leaves, consts, rngs = lower(dest, _wrap(expr))
+    for lf in leaves:
+        if lf.n != dest.n or lf.dtype != dest.dtype:
+            raise ValueError("all arrays of an assignment share sample count and dtype")
+    ptrs = (C.c_void_p * len(leaves))(*[lf.ptr() for lf in leaves])
+    pitch = (C.c_uint64 * len(leaves))(*[lf.capacity for lf in leaves])
+    cst = (C.c_double * max(1, len(consts)))(*consts)
+    rg = (aeth_rng_leaf * max(1, len(rngs)))()
+    for k, (seed, p0, p1, comp, dist) in enumerate(rngs):
+        rg[k].seed, rg[k].p0, rg[k].p1, rg[k].comp, rg[k].dist = seed, p0, p1, comp, dist
+    if sum_mode >= 0 and sums is None:
+        sums = torch.empty((dest.ncomp, 2), dtype=dest.dtype, device=dest.data.device)
+    call("aeth_eval_tape", _dtype_code(dest.dtype), dest.ncomp, progs, len(leaves), ptrs, pitch, len(consts), cst,
+         len(rngs), rg, dest.n, sum_mode, sums.data_ptr() if sums is not None else None, stream_handle(stream))
+    return sums

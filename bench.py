@@ -36,6 +36,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--log2n", type=int, default=27)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-extra", action="store_true", help="only the headline workload")
     p.add_argument("--cpu-log2n", type=int, default=24)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run the seam's device copy kernel over 3 GiB (known bytes; PMC calibration)")
@@ -102,6 +103,195 @@ def cpu_baseline(log2n: int) -> dict:
             "sample": f"2^{log2n} samples x 3 components, median of 3 after 1 warm-up"}
 
 
+def _time(fn, reps: int = 10, warm: int = 2) -> float:
+    """Average milliseconds of fn() over reps launches, HIP events on the launch stream."""
+    import torch
+
+    for _ in range(warm):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _gbs(nbytes: float, ms: float) -> float:
+    return round(nbytes / (ms * 1e-3) / 1e9, 1)
+
+
+def wl_tape(a, b, c, d, n) -> dict:
+    """The same expression through the generic tape lowering (aeth_eval_tape)."""
+    import aether_amd as ae
+    from aether_amd import expr as E
+
+    x = a + b * c + E.sin(a) * E.exp(-1.0 * (b * b))
+    sums = None
+
+    def run():
+        E.assign(d, x, ae.SUM_NEUMAIER, sums)
+
+    ms = _time(run)
+    return {"ms": round(ms, 4), "GB/s": _gbs(3 * n * 32, ms), "frac": round(3 * n * 32 / (ms * 1e-3) / 1e9 / 8000, 4)}
+
+
+def wl_vec3(a, b, c, n) -> dict:
+    """e = cross(a,b).dot(c) + sqrt(a.dot(a) + 1): 9 planes read, 1 written (80 B/sample)."""
+    import torch
+
+    import aether_amd as ae
+
+    e = torch.empty(n, dtype=torch.float64, device="cuda")
+    ms = _time(lambda: ae.eval_vec3geo(a, b, c, e))
+    del e
+    return {"samples": n, "ms": round(ms, 4), "GB/s": _gbs(80 * n, ms),
+            "frac": round(80 * n / (ms * 1e-3) / 1e9 / 8000, 4)}
+
+
+def wl_philox() -> dict:
+    """Seed 42: 2^30 uint32 words, 2^29 uniform doubles, 2^28 Box-Muller doubles (bytes written only)."""
+    import torch
+
+    import aether_amd as ae
+
+    out = {}
+    buf = torch.empty(1 << 30, dtype=torch.int32, device="cuda")  # 4 GiB, reused
+    ms = _time(lambda: ae.rng_fill(ae.RNG_BITS32, 42, buf), reps=5)
+    out["bits32_2^30"] = {"ms": round(ms, 4), "GB/s": _gbs(4 << 30, ms), "Gsamples/s": round((1 << 30) / ms / 1e6, 2)}
+    u = buf.view(torch.float64)[: 1 << 29]
+    ms = _time(lambda: ae.rng_fill(ae.RNG_UNIFORM_F64, 42, u), reps=5)
+    out["uniform_f64_2^29"] = {"ms": round(ms, 4), "GB/s": _gbs(8 << 29, ms),
+                               "Gsamples/s": round((1 << 29) / ms / 1e6, 2)}
+    z = buf.view(torch.float64)[: 1 << 28]
+    ms = _time(lambda: ae.rng_fill(ae.RNG_NORMAL_F64, 42, z), reps=5)
+    out["normal_f64_2^28"] = {"ms": round(ms, 4), "GB/s": _gbs(8 << 28, ms),
+                              "Gsamples/s": round((1 << 28) / ms / 1e6, 2)}
+    del buf, u, z
+    return out
+
+
+def wl_fp32(log2n: int = 28) -> dict:
+    """Config-0 expression in FP32 at 2^28 samples; sums and L2 norms in plain FP32 and float-float,
+    each error stated against an FP64 compensated reduction of the same float values."""
+    import torch
+
+    import aether_amd as ae
+
+    n = 1 << log2n
+    a, b, c, d = (ae.Array(n, 3, torch.float32) for _ in range(4))
+    for arr, s in ((a, 42), (b, 43), (c, 44)):
+        ae.array_fill_uniform(arr, s, -1.0, 1.0)
+    res = {"samples": n}
+    for name, mode in (("plain", ae.SUM_PLAIN), ("floatfloat", ae.SUM_FLOATFLOAT)):
+        sums = torch.empty((3, 2), dtype=torch.float32, device="cuda")
+        ms = _time(lambda: ae.eval_cfg0(a, b, c, d, mode, sums), reps=5)
+        res[f"eval+sum_{name}"] = {"ms": round(ms, 4), "GB/s": _gbs(3 * n * 16, ms),
+                                   "frac": round(3 * n * 16 / (ms * 1e-3) / 1e9 / 8000, 4)}
+    # FP64 references: the float values of d widened exactly, summed with the FP64 compensated kernel
+    d64 = ae.Array(n, 3, torch.float64)
+    d64.data[:, :n].copy_(d.data[:, :n])
+    ref_sum = ae.reduce(d64, ae.SUM_NEUMAIER).sum(1).cpu()
+    ref_sq = ae.reduce(d64, ae.SUM_NEUMAIER, square=True).sum(1).cpu()
+    del d64
+    errs = {}
+    for name, mode in (("plain", ae.SUM_PLAIN), ("floatfloat", ae.SUM_FLOATFLOAT)):
+        s = ae.reduce(d, mode).double().sum(1).cpu()
+        q = ae.reduce(d, mode, square=True).double().sum(1).cpu()
+        errs[name] = {"sum_rel_err_max": float(((s - ref_sum).abs() / ref_sum.abs()).max()),
+                      "l2norm_rel_err_max": float(((q.sqrt() - ref_sq.sqrt()).abs() / ref_sq.sqrt()).max())}
+    res["error_vs_fp64"] = errs
+    del a, b, c, d
+    return res
+
+
+def wl_trig(log2n: int = 26) -> dict:
+    """sin/cos/exp/log of 2^26 floats, |x| log-uniform on [1e-6, 1e5], random sign (Philox-drawn);
+    one fused pass (4 B in, 16 B out per input); ulp error vs the FP64 device routines."""
+    import torch
+
+    import aether_amd as ae
+    from aether_amd import expr as E
+
+    n = 1 << log2n
+    g = ae.Generator.from_seed(42)
+    u = ae.Array(n, 2, torch.float64)
+    E.assign(u, E.uniform(g, 2, 0.0, 1.0))
+    x64 = (10.0 ** (u.data[0, :n] * 11.0 - 6.0)) * torch.where(u.data[1, :n] < 0.5, -1.0, 1.0)
+    x = x64.float().contiguous()
+    del u, x64
+    y = torch.empty(4 * n, dtype=torch.float32, device="cuda")
+    ms = _time(lambda: ae.math("all4", x, y), reps=5)
+    xd = x.double()
+    res = {"inputs": n, "ms": round(ms, 4), "GB/s": _gbs(20 * n, ms), "Gcalls/s": round(4 * n / ms / 1e6, 2)}
+    errs = {}
+    inf = torch.tensor(float("inf"), device="cuda")
+    for k, fn in enumerate(["sin", "cos", "exp", "log"]):
+        yk = y[k * n:(k + 1) * n]
+        ref = ae.math(fn, xd)
+        rf = ref.float()
+        ok = torch.isfinite(yk) & torch.isfinite(rf)
+        spacing = (torch.nextafter(rf, inf) - rf).double().abs().clamp_min(1e-45)
+        err = torch.where(ok, (yk.double() - ref).abs() / spacing, torch.zeros_like(ref))
+        errs[fn] = {"max_ulp_vs_fp64": round(float(err.max()), 3), "finite_outputs": int(ok.sum())}
+    res["error_vs_fp64"] = errs
+    return res
+
+
+def wl_fft() -> dict:
+    """4096 complex signals, lengths uniform over 2^6..2^13 (Philox seed 7), entries N(0,1):
+    forward then inverse, one launch for the whole batch vs one launch per length."""
+    import numpy as np
+    import torch
+
+    import aether_amd as ae
+
+    g = ae.Generator.from_seed(7)
+    rng = np.random.default_rng(7)
+    lengths = (1 << rng.integers(6, 14, 4096)).astype(np.uint64)
+    total = int(lengths.sum())
+    z = ae.Array(total, 2, torch.float64)
+    from aether_amd import expr as E
+
+    E.assign(z, E.normal(g, 2, 0.0, 1.0))
+    x128 = torch.complex(z.data[0, :total], z.data[1, :total]).contiguous()
+    del z
+    res = {"signals": 4096, "points": total}
+    for name, prec, dt in (("c128", 0, torch.complex128), ("c64_floatfloat", 2, torch.complex64),
+                           ("c64_plain", 1, torch.complex64)):
+        x = x128.to(dt)
+        y = torch.empty_like(x)
+        w = torch.empty_like(x)
+        plan = ae.FftBatch(lengths, prec=prec)
+        xr, yr, wr = torch.view_as_real(x), torch.view_as_real(y), torch.view_as_real(w)
+        ms_f = _time(lambda: plan(xr, yr, True, 0), reps=10)
+        ms_b = _time(lambda: plan(yr, wr, False, 0), reps=10)
+        bytes_pass = 2 * total * x.element_size()
+        entry = {"fwd_ms": round(ms_f, 4), "inv_ms": round(ms_b, 4), "fwd_GB/s": _gbs(bytes_pass, ms_f),
+                 "round_trip_max_abs_err": float((w.to(torch.complex128) - x128).abs().max())}
+        if prec != 0:
+            ref = torch.empty_like(x128)
+            p128 = ae.FftBatch(lengths, prec=0)
+            p128(torch.view_as_real(x.to(torch.complex128)), torch.view_as_real(ref), True, 0)
+            torch.cuda.synchronize()
+            entry["fwd_max_rel_err_vs_c128"] = float(((y.to(torch.complex128) - ref).abs().max() /
+                                                     ref.abs().max()))
+            p128.close()
+        # the comparison: one launch per length (same kernel, 8 launches)
+        offs = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
+        per = [ae.FftBatch(lengths[lengths == L], offs[lengths == L], offs[lengths == L], prec=prec)
+               for L in np.unique(lengths)]
+        ms_per = _time(lambda: [p(xr, yr, True, 0) for p in per], reps=10)
+        entry["per_length_launches_fwd_ms"] = round(ms_per, 4)
+        for p in per:
+            p.close()
+        plan.close()
+        res[name] = entry
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -160,6 +350,18 @@ def main():
                       "avg": round(avg_launch_ms, 4)},
         "sums": [s[k, 0].item() + s[k, 1].item() for k in range(3)],
     }
+    if not args.no_extra:
+        extra = {"fused_fp64_via_tape": wl_tape(a, b, c, d, n), "vec3_geometric_f64": wl_vec3(a, b, c, n)}
+        del a, b, c, d
+        torch.cuda.empty_cache()
+        extra["philox_seed42"] = wl_philox()
+        torch.cuda.empty_cache()
+        extra["fp32_cfg0_2^28"] = wl_fp32()
+        torch.cuda.empty_cache()
+        extra["trig_stress_f32_2^26"] = wl_trig()
+        torch.cuda.empty_cache()
+        extra["fft_batch_4096"] = wl_fft()
+        line["workloads"] = extra
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_log2n)
     print(json.dumps(line), flush=True)

@@ -112,6 +112,58 @@ enum { AETH_SUM_NONE = -1, AETH_SUM_NEUMAIER = 0, AETH_SUM_PLAIN = 1, AETH_SUM_F
 int aeth_eval_cfg0(int dtype, const void* a, const void* b, const void* c, void* d, int ncomp, uint64_t n,
                    uint64_t pitch, int sum_mode, void* sums, void* stream);
 
+/* Generic lowering of `dest = expr` (expr/Assign.h:124-160,264-271): each
+ * output component is a PROGRAM -- a postfix tape over the planes it reads.
+ * The tape is evaluated in registers for every sample; all planes a program
+ * reads are loaded first (16-byte lanes), so one assignment is one kernel
+ * whose HBM traffic is exactly the planes read plus the planes written.
+ * Node order follows aether's evaluation order; nothing is contracted, so a
+ * tape gives the same bits as the specialised kernels above for the same
+ * tree (e.g. the cfg0 expression). Operations (stack effect in brackets):
+ *   PUSH   arg = index into the program's `loads` [-> x]
+ *   CONST  arg = index into `consts`                 [-> c]
+ *   RNG    arg = index into `rngs`: a distribution leaf drawing at (seed,
+ *          sample, comp) as aeth_array_fill does     [-> x]
+ *   ADD SUB MUL DIV                                  [a b -> a op b]
+ *   NEG SIN COS EXP LOG SQRT ABS                     [a -> f(a)]
+ * Limits: 16 loads, 64 instructions, stack depth 8, 8 programs, 16 leaves,
+ * 16 constants, 8 rng leaves per call. */
+enum { AETH_OP_PUSH = 0, AETH_OP_CONST = 1, AETH_OP_RNG = 2, AETH_OP_ADD = 3, AETH_OP_SUB = 4, AETH_OP_MUL = 5,
+       AETH_OP_DIV = 6, AETH_OP_NEG = 7, AETH_OP_SIN = 8, AETH_OP_COS = 9, AETH_OP_EXP = 10, AETH_OP_LOG = 11,
+       AETH_OP_SQRT = 12, AETH_OP_ABS = 13 };
+#define AETH_TAPE_MAX_LOADS 16
+#define AETH_TAPE_MAX_INSNS 64
+#define AETH_TAPE_MAX_DEPTH 8
+#define AETH_TAPE_MAX_PROGRAMS 8
+#define AETH_TAPE_MAX_LEAVES 16
+#define AETH_TAPE_MAX_CONSTS 16
+#define AETH_TAPE_MAX_RNGS 8
+typedef struct {
+    uint8_t op;
+    uint8_t arg;
+} aeth_insn;
+typedef struct {
+    uint8_t n_loads;
+    uint8_t n_insns;
+    uint8_t out_leaf;   /* destination leaf slot */
+    uint8_t out_comp;   /* destination component (plane) */
+    uint8_t load_leaf[AETH_TAPE_MAX_LOADS];
+    uint8_t load_comp[AETH_TAPE_MAX_LOADS];
+    aeth_insn insns[AETH_TAPE_MAX_INSNS];
+} aeth_program;
+typedef struct {
+    uint64_t seed;  /* salted leaf seed, as aeth_array_fill */
+    double p0, p1;  /* (lo, hi) or (mean, sd) */
+    uint32_t comp;  /* Philox counter = component */
+    uint32_t dist;  /* AETH_DIST_UNIFORM / AETH_DIST_NORMAL */
+} aeth_rng_leaf;
+/* leaves: device base pointers (planes at leaf_pitch elements apart); all
+ * leaves share dtype and the sample count n. sums as in aeth_eval_cfg0, one
+ * (hi, comp) pair per program in program order. */
+int aeth_eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_leaves, void* const* leaves,
+                   const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs,
+                   const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums, void* stream);
+
 /* e = cross(a, b).dot(c) + sqrt(a.dot(a) + 1) over Array<double,3> a, b, c
  * (pitch `pitch`) into a scalar plane e (expr/nodes/Geometric.h:32-73,
  * expr/Expression.h:162-173, expr/Reduce.h:150-190, math/math.h:132). */
