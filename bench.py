@@ -27,6 +27,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fused SoA eval GB/s (% HBM peak); served jobs/s + p50/p99 at stated err vs FP64"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fused kernel at 3 x 2^27 samples, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# scaled by the same counters over a known-size copy (profiles/cfg0_f64_2p27_hbm_traffic.md).
+MEASURED_TRAFFIC_BYTES = {27: 12_890_958_603}
 
 
 def parse():
@@ -345,7 +348,8 @@ def main():
         "config": {"workload": "fused FP64 d = a + b*c + sin(a)*exp(-b*b) + Neumaier sums of d",
                    "samples": n, "components": 3, "arrays": 4, "bytes_per_step": bytes_step},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC_BYTES.get(args.log2n)},
         "launch_ms": {"min": round(launch_ms[0], 4), "median": round(launch_ms[len(launch_ms) // 2], 4),
                       "avg": round(avg_launch_ms, 4)},
         "sums": [s[k, 0].item() + s[k, 1].item() for k in range(3)],
