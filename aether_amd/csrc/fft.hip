@@ -28,6 +28,7 @@
 #include "accum.hpp"
 #include "common.hpp"
 #include "dmath.hpp"
+#include "fft.hpp"
 
 namespace aeth {
 namespace fft {
@@ -38,18 +39,6 @@ constexpr int kThreads = 1024;
 
 __host__ __device__ constexpr uint32_t pad(uint32_t e) { return e + (e >> 4); }
 constexpr uint32_t kLdsElems = pad(kMaxN);
-
-struct Desc {
-    uint32_t first_sig; // index into the signal table
-    uint16_t count;     // signals in this chunk
-    uint16_t log2n;
-    uint32_t stride;    // elements between consecutive points of a signal (1 = contiguous)
-};
-
-struct Sig {
-    uint64_t in_off;  // complex elements
-    uint64_t out_off; // complex elements
-};
 
 // ---- working types --------------------------------------------------------
 struct C128 {
@@ -324,15 +313,6 @@ TwTables tables(hipStream_t st)
     return TwTables{it->second.d, it->second.f, it->second.ff};
 }
 
-// A batch plan: descriptors and signal table resident on the device.
-struct BatchPlan {
-    int prec = 0; // 0 c128, 1 c64, 2 c64 float-float
-    int dev = 0;
-    Desc* descs = nullptr;
-    Sig* sigs = nullptr;
-    uint32_t ndesc = 0;
-    uint64_t nsig = 0;
-};
 
 uint32_t log2_exact(uint64_t n)
 {
@@ -345,7 +325,7 @@ uint32_t log2_exact(uint64_t n)
 size_t lds_bytes(int prec) { return (size_t)kLdsElems * (prec == 1 ? sizeof(C64) : 16); }
 
 void build_tables(const uint32_t* log2n, const uint64_t* in_off, const uint64_t* out_off, uint64_t count,
-                  std::vector<Desc>& descs, std::vector<Sig>& sigs, uint32_t stride = 1)
+                  std::vector<Desc>& descs, std::vector<Sig>& sigs, uint32_t stride)
 {
     // group by length, largest first; a chunk holds up to 8192 points of one length
     std::vector<uint64_t> order(count);
@@ -372,7 +352,7 @@ void build_tables(const uint32_t* log2n, const uint64_t* in_off, const uint64_t*
 }
 
 void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm, hipStream_t st,
-            double fixed_scale = 1.0)
+            double fixed_scale)
 {
     const TwTables tw = tables(st);
     const size_t lds = lds_bytes(p.prec);

@@ -203,6 +203,66 @@ int aeth_fft_batch_plan(int prec, uint64_t count, const uint64_t* lengths, const
 int aeth_fft_batch_exec(void* plan, const void* in, void* out, int forward, int norm, void* stream);
 int aeth_fft_batch_destroy(void* plan);
 
+/* ------------------------------------------------------------------ serving */
+/* A continuous-batching job server on one GPU (no counterpart in the
+ * reference, which has no serving loop). Jobs:
+ *   EVAL  : a, b, c = Array<T,3>(n) of uniforms on (-1,1) drawn from seeds
+ *           seed, seed+1, seed+2 (counter = component); d = a + b*c +
+ *           sin(a)*exp(-b*b); result: per-component compensated sums of d
+ *           (Neumaier for F64, float-float for F32) as 3 (hi, comp) pairs.
+ *   SAMPLE: n standard normals standardNormal<T>(seed, i, 0); result: (hi,
+ *           comp) of their sum and of their sum of squares.
+ *   FFT   : n (1..64) signals of lengths 2^fft_log2n[k], interleaved entries
+ *           standardNormal<T>(seed, j, 0); forward transform (complex128 for
+ *           F64, complex64 with float-float arithmetic for F32); result: (hi,
+ *           comp) of the sum of every output real and imaginary part.
+ * Each job's result is bit for bit the result of aeth_run_job_plain on the
+ * same job: batching never changes a value. Arrays live in a device pool of
+ * hbm_budget_bytes; a job is admitted (in arrival order) when all its buffers
+ * fit and released when its batch completes. fp32_compensated runs every job
+ * in FP32 storage with compensated accumulation. */
+enum { AETH_JOB_EVAL = 0, AETH_JOB_SAMPLE = 1, AETH_JOB_FFT = 2 };
+typedef struct {
+    uint32_t kind;
+    uint32_t dtype;          /* AETH_F64 / AETH_F32 */
+    uint64_t n;              /* EVAL / SAMPLE: samples; FFT: number of signals */
+    uint64_t seed;
+    double arrival;          /* seconds after the start of aeth_server_replay */
+    uint8_t fft_log2n[64];
+} aeth_job;
+typedef struct {
+    uint64_t id;
+    uint32_t kind;
+    int32_t status;          /* 0 ok; 1 failed; 2 cannot fit the budget */
+    double value[8];         /* (hi, comp) pairs as described above, widened to double */
+    double submit_time;      /* seconds (server clock; replay: since its start) */
+    double done_time;
+} aeth_job_result;
+typedef struct {
+    uint64_t hbm_budget_bytes;
+    uint32_t streams;        /* batches in flight, 1..4 */
+    uint32_t max_batch_jobs; /* coalescing cap per batch */
+    uint32_t fp32_compensated;
+    uint32_t reserved;
+} aeth_server_config;
+typedef struct {
+    uint64_t submitted, completed, failed, batches;
+    uint64_t pool_bytes_in_use, pool_bytes_budget;
+} aeth_server_stats;
+int aeth_server_create(const aeth_server_config* cfg, void** server);
+int aeth_server_submit(void* server, const aeth_job* job, uint64_t* id);
+int aeth_server_poll(void* server, aeth_job_result* out, uint32_t max, uint32_t* count);
+int aeth_server_wait_idle(void* server);
+int aeth_server_get_stats(void* server, aeth_server_stats* stats);
+int aeth_server_destroy(void* server);
+/* Open-loop replay: submits jobs[i] at jobs[i].arrival seconds after the call
+ * starts, waits for all, results[i] is job i's result. */
+int aeth_server_replay(void* server, const aeth_job* jobs, uint64_t count, aeth_job_result* results);
+/* The plain per-call form of one job: its own hipMalloc/hipFree, the single-
+ * call entries above, synchronous (how a caller of the reference serves a
+ * request). */
+int aeth_run_job_plain(const aeth_job* job, aeth_job_result* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,116 @@
+"""GPU: the job server. Each result is bit for bit the same alone (plain
+per-call path) and inside a served mixed batch; results match the oracle;
+admission respects the HBM budget.
+"""
+import numpy as np
+import pytest
+import torch
+
+import aether_amd as ae
+from aether_amd import serve as S
+
+pytestmark = pytest.mark.gpu
+
+
+def jobs_small_mix(count=60, seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        k = i % 3
+        if k == 0:
+            out.append(S.make_job(S.JOB_EVAL, int(2 ** rng.uniform(5, 18)), 1000 + 8 * i,
+                                  dtype=S.F32 if i % 2 else S.F64))
+        elif k == 1:
+            out.append(S.make_job(S.JOB_SAMPLE, int(2 ** rng.uniform(5, 19)), 5000 + 8 * i,
+                                  dtype=S.F32 if i % 4 == 1 else S.F64))
+        else:
+            lens = [int(v) for v in rng.integers(0, 14, int(rng.integers(1, 9)))]
+            out.append(S.make_job(S.JOB_FFT, len(lens), 9000 + 8 * i, dtype=S.F64 if i % 2 else S.F32,
+                                  fft_log2n=lens))
+    return out
+
+
+def test_alone_equals_in_the_mix(dev):
+    jobs = jobs_small_mix()
+    alone = [S.run_plain(j) for j in jobs]
+    with S.Server(budget_bytes=4 << 30, streams=4, max_batch_jobs=64) as srv:
+        ids = [srv.submit(j) for j in jobs]
+        srv.wait_idle()
+        got = {r.id: r for r in srv.poll()}
+        st = srv.stats()
+    assert st.completed == len(jobs) and st.failed == 0
+    assert st.batches < len(jobs)  # jobs were coalesced
+    for j, a, i in zip(jobs, alone, ids):
+        r = got[i]
+        assert r.status == 0
+        assert list(r.value) == list(a.value), (j.kind, j.dtype, j.n, list(r.value), list(a.value))
+
+
+def test_eval_job_matches_oracle(dev, orc):
+    n, seed = 10_000, 77
+    r = S.run_plain(S.make_job(S.JOB_EVAL, n, seed, S.F64))
+    pitch = -(-n // 32) * 32
+    A, B, C = (orc.fill_array_uniform(seed + k, 3, n, pitch) for k in range(3))
+    d = orc.eval_cfg0(A, B, C, 3, n, pitch, mathmode=1).reshape(3, pitch)[:, :n]
+    for k in range(3):
+        h, c = orc.device_sum(np.ascontiguousarray(d[k]))
+        assert (r.value[2 * k], r.value[2 * k + 1]) == (h, c)
+
+
+def test_sample_job_matches_oracle(dev, orc):
+    n, seed = 50_001, 99
+    r = S.run_plain(S.make_job(S.JOB_SAMPLE, n, seed, S.F64))
+    z = orc.fill_normal_f64(seed, n)
+    assert (r.value[0], r.value[1]) == orc.device_sum(z)
+    assert (r.value[2], r.value[3]) == orc.device_sum(z, square=True)
+
+
+def test_fft_job_matches_oracle_within_bound(dev, orc):
+    lens = [6, 10, 3]
+    seed = 1234
+    r = S.run_plain(S.make_job(S.JOB_FFT, 3, seed, S.F64, fft_log2n=lens))
+    total = sum(1 << l for l in lens)
+    x = orc.fill_normal_f64(seed, 2 * total).view(np.complex128)
+    pos, acc = 0, 0.0
+    for l in lens:
+        acc += orc.dft(x[pos:pos + (1 << l)]).view(np.float64).sum()
+        pos += 1 << l
+    got = r.value[0] + r.value[1]
+    assert abs(got - acc) <= 1e-9 * np.abs(x).sum()
+
+
+def test_budget_admission_and_oversize(dev):
+    jobs = [S.make_job(S.JOB_EVAL, 1 << 16, 8 * i + 1, S.F64) for i in range(20)]
+    big = S.make_job(S.JOB_EVAL, 1 << 22, 7, S.F64)  # needs ~400 MB > budget
+    with S.Server(budget_bytes=32 << 20, streams=2, max_batch_jobs=8) as srv:
+        ids = [srv.submit(j) for j in jobs]
+        bid = srv.submit(big)
+        srv.wait_idle()
+        res = {r.id: r for r in srv.poll()}
+    assert all(res[i].status == 0 for i in ids)
+    assert res[bid].status == 2
+
+
+def test_fp32_compensated_mode_error(dev):
+    jobs = [S.make_job(S.JOB_EVAL, 1 << 15, 8 * i + 3, S.F64) for i in range(6)]
+    ref = [S.run_plain(j) for j in jobs]
+    with S.Server(budget_bytes=1 << 30, fp32_compensated=True) as srv:
+        ids = [srv.submit(j) for j in jobs]
+        srv.wait_idle()
+        res = {r.id: r for r in srv.poll()}
+    for j, i, a in zip(jobs, ids, ref):
+        for k in range(3):
+            s32 = res[i].value[2 * k] + res[i].value[2 * k + 1]
+            s64 = a.value[2 * k] + a.value[2 * k + 1]
+            # FP32 inputs are the FP64 draws rounded to float; the sum error is that input rounding,
+            # not accumulation: relative to sum |d| it stays near 2^-24
+            assert abs(s32 - s64) <= 4 * 2.0**-24 * (1 << 15) * 2.5
+
+
+def test_replay_open_loop(dev):
+    jobs = S.make_mix(rate=2000.0, duration=0.2, seed=11)
+    with S.Server(budget_bytes=16 << 30) as srv:
+        res = srv.replay(jobs)
+    assert all(r.status == 0 for r in res)
+    lat = np.array([r.done_time - r.submit_time for r in res])
+    assert np.all(lat > 0)
