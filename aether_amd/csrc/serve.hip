@@ -577,7 +577,8 @@ struct Server {
             r.id = J.id;
             r.kind = J.spec.kind;
             r.status = 0;
-            for (int k = 0; k < 8; ++k) r.value[k] = S.host_res[8 * j + k];
+            const int used = 2 * result_pairs(J.spec.kind);
+            for (int k = 0; k < 8; ++k) r.value[k] = k < used ? S.host_res[8 * j + k] : 0.0;
             r.submit_time = J.submit_t;
             r.done_time = t;
             out.push_back(r);

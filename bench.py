@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--log2n", type=int, default=27)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    p.add_argument("--serve-seconds", type=float, default=3.0,
+                   help="length of the Poisson job stream (20k jobs/s) replayed into the server")
     p.add_argument("--cpu-log2n", type=int, default=24)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run the seam's device copy kernel over 3 GiB (known bytes; PMC calibration)")
@@ -295,6 +297,134 @@ def wl_fft() -> dict:
     return res
 
 
+def fp32_error_vs_fp64(jobs, r32, per_kind: int = 12) -> dict:
+    """Error of FP32-compensated served results against an FP64 oracle on the SAME FP32 inputs:
+    the job's float inputs are regenerated, widened exactly to double and pushed through the FP64
+    path (Neumaier sums, complex128 FFT). Stated relative to sum |terms| (the conditioning scale of a
+    sum) and to |result|."""
+    import numpy as np
+    import torch
+
+    import aether_amd as ae
+    from aether_amd import serve as S
+
+    errs = {"eval": [], "sample": [], "fft": []}
+    seen = {S.JOB_EVAL: 0, S.JOB_SAMPLE: 0, S.JOB_FFT: 0}
+    for j, r in zip(jobs, r32):
+        if seen[j.kind] >= per_kind or r.status != 0:
+            continue
+        seen[j.kind] += 1
+        if j.kind == S.JOB_EVAL:
+            ins = []
+            for k in range(3):
+                a32 = ae.Array(j.n, 3, torch.float32)
+                ae.array_fill_uniform(a32, j.seed + k, -1.0, 1.0)
+                a64 = ae.Array(j.n, 3, torch.float64)
+                a64.data.copy_(a32.data.double())
+                ins.append(a64)
+            d = ae.Array(j.n, 3, torch.float64)
+            ref = ae.eval_cfg0(ins[0], ins[1], ins[2], d, ae.SUM_NEUMAIER).sum(1).cpu()
+            absum = d.data[:, : j.n].abs().sum(1).cpu()
+            for k in range(3):
+                got = r.value[2 * k] + r.value[2 * k + 1]
+                errs["eval"].append((abs(got - ref[k].item()) / absum[k].item(),
+                                     abs(got - ref[k].item()) / max(abs(ref[k].item()), 1e-300)))
+        elif j.kind == S.JOB_SAMPLE:
+            z = torch.empty(j.n, dtype=torch.float32, device="cuda")
+            ae.rng_fill(ae.RNG_NORMAL_F32, j.seed, z)
+            x = ae.Array(j.n, 1, torch.float64)
+            x.data[0, : j.n].copy_(z.double())
+            ref = ae.reduce(x, ae.SUM_NEUMAIER).sum(1).cpu()[0].item()
+            ref2 = ae.reduce(x, ae.SUM_NEUMAIER, square=True).sum(1).cpu()[0].item()
+            absum = x.data[0, : j.n].abs().sum().item()
+            got, got2 = r.value[0] + r.value[1], r.value[2] + r.value[3]
+            errs["sample"].append((abs(got - ref) / absum, abs(got - ref) / max(abs(ref), 1e-300)))
+            errs["sample"].append((abs(got2 - ref2) / ref2, abs(got2 - ref2) / ref2))
+        else:
+            lens = np.array([1 << j.fft_log2n[k] for k in range(j.n)], np.uint64)
+            total = int(lens.sum())
+            z = torch.empty(2 * total, dtype=torch.float32, device="cuda")
+            ae.rng_fill(ae.RNG_NORMAL_F32, j.seed, z)
+            xc = torch.view_as_complex(z.double().view(total, 2)).contiguous()
+            yc = torch.empty_like(xc)
+            plan = ae.FftBatch(lens, prec=0)
+            plan(torch.view_as_real(xc), torch.view_as_real(yc), True, 0)
+            plan.close()
+            y = ae.Array(2 * total, 1, torch.float64)
+            y.data[0, : 2 * total].copy_(torch.view_as_real(yc).reshape(-1))
+            ref = ae.reduce(y, ae.SUM_NEUMAIER).sum(1).cpu()[0].item()
+            absum = y.data[0, : 2 * total].abs().sum().item()
+            got = r.value[0] + r.value[1]
+            errs["fft"].append((abs(got - ref) / absum, abs(got - ref) / max(abs(ref), 1e-300)))
+    torch.cuda.synchronize()
+    return {k: {"jobs_checked": seen[{"eval": S.JOB_EVAL, "sample": S.JOB_SAMPLE, "fft": S.JOB_FFT}[k]],
+                "max_err_rel_to_sum_abs": float(max((e[0] for e in v), default=0.0)),
+                "max_err_rel_to_result": float(max((e[1] for e in v), default=0.0))}
+            for k, v in errs.items()}
+
+
+def wl_serve(duration: float, rate: float = 20000.0, plain_seconds: float = 6.0) -> dict:
+    """Served mixed job stream (Poisson arrivals): continuous batching vs the plain per-call form.
+
+    Served: open-loop replay of the stream into aeth_server (64 GB budget, 4 streams), once with each job's
+    requested precision and once in the FP32-compensated form. Plain: the same jobs, one at a time through
+    aeth_run_job_plain (own allocations, synchronous), closed loop for `plain_seconds`."""
+    import time as _t
+
+    import numpy as np
+
+    from aether_amd import serve as S
+
+    jobs = S.make_mix(rate, duration, seed=5)
+    nbytes = sum(S.job_bytes(j) for j in jobs)
+    out = {"offered_jobs_per_s": rate, "duration_s": duration, "jobs": len(jobs),
+           "offered_GB_per_s": round(nbytes / duration / 1e9, 1)}
+
+    def summarize(res, label):
+        sub = np.array([r.submit_time for r in res])
+        done = np.array([r.done_time for r in res])
+        lat = (done - sub) * 1e3
+        span = done.max() - sub.min()
+        out[label] = {"jobs_per_s": round(len(res) / span, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 3), "makespan_s": round(span, 3),
+                      "algorithmic_GB_per_s": round(nbytes / span / 1e9, 1),
+                      "failed": int(sum(r.status != 0 for r in res))}
+        return res
+
+    with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256) as srv:
+        r64 = summarize(srv.replay(jobs), "served")
+        out["served"]["batches"] = int(srv.stats().batches)
+    with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True) as srv:
+        r32 = summarize(srv.replay(jobs), "served_fp32_compensated")
+        out["served_fp32_compensated"]["batches"] = int(srv.stats().batches)
+    out["fp32_compensated_error_vs_fp64"] = fp32_error_vs_fp64(jobs, r32)
+    # capacity: the same kind of stream submitted all at once (closed loop), FP32-compensated form
+    cap_jobs = S.make_mix(rate, min(duration, 1.0), seed=6)
+    for j in cap_jobs:
+        j.arrival = 0.0
+    with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True) as srv:
+        summarize(srv.replay(cap_jobs), "capacity_fp32_compensated_closed_loop")
+    with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256) as srv:
+        summarize(srv.replay(cap_jobs), "capacity_closed_loop")
+    # plain per-call form, closed loop over the head of the same stream
+    t0 = _t.perf_counter()
+    done = 0
+    lat = []
+    for j in jobs:
+        r = S.run_plain(j)
+        lat.append((r.done_time - r.submit_time) * 1e3)
+        done += 1
+        if _t.perf_counter() - t0 > plain_seconds:
+            break
+    el = _t.perf_counter() - t0
+    out["plain_per_call"] = {"jobs_per_s": round(done / el, 1), "jobs_run": done,
+                             "p50_ms_service": round(float(np.percentile(lat, 50)), 3)}
+    out["served_fp32_compensated_vs_plain"] = round(out["served_fp32_compensated"]["jobs_per_s"] /
+                                                    out["plain_per_call"]["jobs_per_s"], 2)
+    out["served_vs_plain"] = round(out["served"]["jobs_per_s"] / out["plain_per_call"]["jobs_per_s"], 2)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -365,6 +495,8 @@ def main():
         extra["trig_stress_f32_2^26"] = wl_trig()
         torch.cuda.empty_cache()
         extra["fft_batch_4096"] = wl_fft()
+        torch.cuda.empty_cache()
+        extra["serving_mixed_stream"] = wl_serve(args.serve_seconds)
         line["workloads"] = extra
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_log2n)

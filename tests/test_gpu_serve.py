@@ -44,6 +44,8 @@ def test_alone_equals_in_the_mix(dev):
         r = got[i]
         assert r.status == 0
         assert list(r.value) == list(a.value), (j.kind, j.dtype, j.n, list(r.value), list(a.value))
+        used = {S.JOB_EVAL: 6, S.JOB_SAMPLE: 4, S.JOB_FFT: 2}[j.kind]
+        assert all(v == 0.0 for v in list(r.value)[used:])
 
 
 def test_eval_job_matches_oracle(dev, orc):
