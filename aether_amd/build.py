@@ -27,15 +27,44 @@ FLAGS = [
 ]
 
 
+JIT_HEADERS = ["dmath.hpp", "rng.hpp", "accum.hpp", "units.hpp"]
+
+
+def write_jit_prelude() -> str:
+    """Embed the device headers (and the C ABI's tape structs) as one source string for hipRTC.
+
+    The runtime compiler lowers each distinct expression tape to its own kernel; it includes
+    exactly the device code the ahead-of-time kernels use, so both produce the same bits."""
+    import re
+
+    head = ("typedef unsigned char uint8_t; typedef unsigned short uint16_t; typedef unsigned int uint32_t;\n"
+            "typedef unsigned long long uint64_t; typedef int int32_t; typedef long long int64_t;\n"
+            "typedef decltype(sizeof(0)) size_t;\n")
+    parts = [head]
+    hdr = open(os.path.join(HERE, "..", "include", "aether_hip.h")).read()
+    parts.append(re.sub(r"^#include .*$", "", hdr, flags=re.M))
+    for f in JIT_HEADERS:
+        text = open(os.path.join(CSRC, f)).read()
+        text = re.sub(r"^#pragma once\s*$", "", text, flags=re.M)
+        text = re.sub(r"^#include .*$", "", text, flags=re.M)
+        parts.append(text)
+    body = "\n".join(parts)
+    assert ")AETHJIT" not in body
+    out = os.path.join(BUILD, "jit_prelude.inc")
+    with open(out, "w") as fh:
+        fh.write('static const char kJitPrelude[] = R"AETHJIT(' + body + ')AETHJIT";\n')
+    return out
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, "-I", BUILD, "-c", src, "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", src, "-o", obj]
+        cmd = [HIPCC, *FLAGS, "-I", BUILD, "-x", "hip", "-c", src, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr}")
@@ -45,11 +74,12 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIB_DIR, exist_ok=True)
+    write_jit_prelude()
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", "-lhiprtc"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

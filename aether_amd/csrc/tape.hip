@@ -13,8 +13,18 @@
 // and the specialised kernel for the same tree agree bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <hip/hiprtc.h>
+
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "jit_prelude.inc"
 
 #include "../../include/aether_hip.h"
 #include "accum.hpp"
@@ -326,7 +336,7 @@ void launch_nl(const TapeArgs& A, bool vec, bool nt, acc::Pair<T>* part, int gri
 }
 
 template<class T, int M, bool SUM>
-void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums, hipStream_t st)
+void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums, hipStream_t st, hipFunction_t jit)
 {
     size_t pairs = (size_t)A.tiles * np;
     for (uint32_t t = A.tiles; t > 1;) {
@@ -336,9 +346,20 @@ void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums
     StreamScratch scratch(SUM ? pairs * sizeof(acc::Pair<T>) : 0, st);
     auto* part = static_cast<acc::Pair<T>*>(scratch.get());
     const int grid = (int)std::max<uint32_t>(1u, A.units);
-    if (max_loads <= 4) launch_nl<T, M, SUM, 4>(A, vec, nt, part, grid, st);
-    else if (max_loads <= 9) launch_nl<T, M, SUM, 9>(A, vec, nt, part, grid, st);
-    else launch_nl<T, M, SUM, 16>(A, vec, nt, part, grid, st);
+    if (jit) {
+        TapeArgs args = A;
+        void* part_v = part;
+        int vec_i = vec ? 1 : 0, nt_i = nt ? 1 : 0;
+        void* params[] = {&args, &part_v, &vec_i, &nt_i};
+        check(hipModuleLaunchKernel(jit, (unsigned)grid, 1, 1, kThreads, 1, 1, 0, st, params, nullptr),
+              "hipModuleLaunchKernel(tape)");
+    } else if (max_loads <= 4) {
+        launch_nl<T, M, SUM, 4>(A, vec, nt, part, grid, st);
+    } else if (max_loads <= 9) {
+        launch_nl<T, M, SUM, 9>(A, vec, nt, part, grid, st);
+    } else {
+        launch_nl<T, M, SUM, 16>(A, vec, nt, part, grid, st);
+    }
     check(hipGetLastError(), "tape_kernel launch");
     if constexpr (SUM) {
         acc::Pair<T>* cur = part;
@@ -359,15 +380,16 @@ void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums
 }
 
 template<class T>
-void dispatch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, int mode, T* sums, hipStream_t st)
+void dispatch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, int mode, T* sums, hipStream_t st,
+              hipFunction_t jit)
 {
     switch (mode) {
-    case AETH_SUM_NONE: launch<T, acc::kNeumaier, false>(A, max_loads, np, vec, nt, sums, st); break;
-    case AETH_SUM_NEUMAIER: launch<T, acc::kNeumaier, true>(A, max_loads, np, vec, nt, sums, st); break;
-    case AETH_SUM_PLAIN: launch<T, acc::kPlain, true>(A, max_loads, np, vec, nt, sums, st); break;
+    case AETH_SUM_NONE: launch<T, acc::kNeumaier, false>(A, max_loads, np, vec, nt, sums, st, jit); break;
+    case AETH_SUM_NEUMAIER: launch<T, acc::kNeumaier, true>(A, max_loads, np, vec, nt, sums, st, jit); break;
+    case AETH_SUM_PLAIN: launch<T, acc::kPlain, true>(A, max_loads, np, vec, nt, sums, st, jit); break;
     case AETH_SUM_FLOATFLOAT:
         if constexpr (sizeof(T) == 4) {
-            launch<T, acc::kFloatFloat, true>(A, max_loads, np, vec, nt, sums, st);
+            launch<T, acc::kFloatFloat, true>(A, max_loads, np, vec, nt, sums, st, jit);
             break;
         } else {
             fail("aeth_eval_tape", "float-float accumulation needs dtype F32", 0, 2);
@@ -376,6 +398,215 @@ void dispatch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, int m
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Runtime lowering: each distinct tape structure becomes one straight-line
+// kernel compiled by hipRTC from the same device headers (kJitPrelude) and
+// cached for the life of the process. Generated bodies follow the tape's
+// order exactly, so they give the interpreter's bits.
+struct JitEntry {
+    hipModule_t module = nullptr;
+    hipFunction_t fn = nullptr;
+};
+std::mutex g_jit_mu;
+std::map<std::string, JitEntry> g_jit;
+std::atomic<uint64_t> g_jit_compiled{0}, g_jit_hits{0}, g_jit_fallbacks{0};
+
+std::string tape_key(int dtype, int mode, int np, const aeth_program* P, int dev)
+{
+    std::string k = std::to_string(dev) + ":" + std::to_string(dtype) + ":" + std::to_string(mode) + ":" +
+                    std::to_string(np);
+    for (int p = 0; p < np; ++p) {
+        k += "|" + std::to_string(P[p].out_leaf) + "," + std::to_string(P[p].out_comp) + ";";
+        for (int i = 0; i < P[p].n_loads; ++i)
+            k += std::to_string(P[p].load_leaf[i]) + "." + std::to_string(P[p].load_comp[i]) + " ";
+        k += ";";
+        for (int i = 0; i < P[p].n_insns; ++i)
+            k += std::to_string(P[p].insns[i].op) + "." + std::to_string(P[p].insns[i].arg) + " ";
+    }
+    return k;
+}
+
+std::string gen_program(const aeth_program& P, bool dbl)
+{
+    const char* T = dbl ? "double" : "float";
+    std::string b;
+    b += "      " + std::string(T) + "* outp = (" + T + "*)A.leaf[" + std::to_string(P.out_leaf) + "] + (uint64_t)" +
+         std::to_string(P.out_comp) + " * A.pitch[" + std::to_string(P.out_leaf) + "];\n";
+    for (int q = 0; q < P.n_loads; ++q)
+        b += "      const " + std::string(T) + "* L" + std::to_string(q) + " = (const " + T + "*)A.leaf[" +
+             std::to_string(P.load_leaf[q]) + "] + (uint64_t)" + std::to_string(P.load_comp[q]) + " * A.pitch[" +
+             std::to_string(P.load_leaf[q]) + "];\n";
+    b += "      for (int r = 0; r < ROUNDS; ++r) {\n";
+    b += "        const uint32_t i0 = s0 + r * (256 * VN) + threadIdx.x * VN;\n";
+    for (int q = 0; q < P.n_loads; ++q) b += "        T x" + std::to_string(q) + "[VN];\n";
+    b += "        if (full) {\n";
+    for (int q = 0; q < P.n_loads; ++q)
+        b += "          if (nt) aeth::units::vload<true>(L" + std::to_string(q) + " + i0, x" + std::to_string(q) +
+             "); else aeth::units::vload<false>(L" + std::to_string(q) + " + i0, x" + std::to_string(q) + ");\n";
+    b += "        } else {\n";
+    for (int q = 0; q < P.n_loads; ++q)
+        b += "          for (int k = 0; k < VN; ++k) x" + std::to_string(q) + "[k] = (i0 + k < n) ? L" +
+             std::to_string(q) + "[i0 + k] : T(0);\n";
+    b += "        }\n        T v[VN];\n        for (int k = 0; k < VN; ++k) {\n";
+    std::vector<std::string> st;
+    int tmp = 0;
+    auto fresh = [&] { return "t" + std::to_string(tmp++); };
+    for (int i = 0; i < P.n_insns; ++i) {
+        const int op = P.insns[i].op, arg = P.insns[i].arg;
+        if (op == AETH_OP_PUSH) {
+            st.push_back("x" + std::to_string(arg) + "[k]");
+        } else if (op == AETH_OP_CONST) {
+            st.push_back("((T)A.consts[" + std::to_string(arg) + "])");
+        } else if (op == AETH_OP_RNG) {
+            const std::string t = fresh();
+            b += "          const T " + t + " = jit_draw<T>(A.rng[" + std::to_string(arg) + "], i0 + k);\n";
+            st.push_back(t);
+        } else if (op >= AETH_OP_ADD && op <= AETH_OP_DIV) {
+            const std::string y = st.back();
+            st.pop_back();
+            const std::string x = st.back();
+            st.pop_back();
+            const char* sym = op == AETH_OP_ADD ? "+" : op == AETH_OP_SUB ? "-" : op == AETH_OP_MUL ? "*" : "/";
+            const std::string t = fresh();
+            b += "          const T " + t + " = " + x + " " + sym + " " + y + ";\n";
+            st.push_back(t);
+        } else {
+            const std::string x = st.back();
+            st.pop_back();
+            std::string f;
+            switch (op) {
+            case AETH_OP_NEG: f = "-(" + x + ")"; break;
+            case AETH_OP_SIN: f = std::string(dbl ? "aeth::dm::sin(" : "aeth::dm::sinf(") + x + ")"; break;
+            case AETH_OP_COS: f = std::string(dbl ? "aeth::dm::cos(" : "aeth::dm::cosf(") + x + ")"; break;
+            case AETH_OP_EXP: f = std::string(dbl ? "aeth::dm::exp(" : "aeth::dm::expf(") + x + ")"; break;
+            case AETH_OP_LOG: f = std::string(dbl ? "aeth::dm::log(" : "aeth::dm::logf(") + x + ")"; break;
+            case AETH_OP_SQRT: f = std::string(dbl ? "__builtin_sqrt(" : "__builtin_sqrtf(") + x + ")"; break;
+            default: f = std::string(dbl ? "__builtin_fabs(" : "__builtin_fabsf(") + x + ")"; break;
+            }
+            const std::string t = fresh();
+            b += "          const T " + t + " = " + f + ";\n";
+            st.push_back(t);
+        }
+    }
+    b += "          v[k] = " + st.back() + ";\n        }\n";
+    b += "        if (full) {\n          if (nt) aeth::units::vstore<true>(outp + i0, v); else aeth::units::vstore<false>(outp + i0, v);\n";
+    b += "          if (SUM) for (int k = 0; k < VN; ++k) aeth::acc::accumulate<M, false>(acc, v[k]);\n";
+    b += "        } else {\n          for (int k = 0; k < VN; ++k) if (i0 + k < n) { outp[i0 + k] = v[k]; "
+         "if (SUM) aeth::acc::accumulate<M, false>(acc, v[k]); }\n        }\n      }\n";
+    return b;
+}
+
+std::string gen_kernel(int dtype, int mode, int np, const aeth_program* P)
+{
+    const bool dbl = dtype == AETH_F64;
+    const int M = mode == AETH_SUM_NONE ? 0 : mode;
+    std::string s = kJitPrelude;
+    s += R"K(
+struct TapeArgsJ {
+    aeth_program prog[AETH_TAPE_MAX_PROGRAMS];
+    const void* leaf[AETH_TAPE_MAX_LEAVES];
+    uint64_t pitch[AETH_TAPE_MAX_LEAVES];
+    double consts[AETH_TAPE_MAX_CONSTS];
+    aeth_rng_leaf rng[AETH_TAPE_MAX_RNGS];
+    uint32_t n, tiles, units;
+};
+template<class T>
+__device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
+{
+    const aeth::rng::U4 b = aeth::rng::block(r.seed, i, r.comp);
+    if constexpr (sizeof(T) == 8) {
+        if (r.dist == AETH_DIST_UNIFORM) return r.p0 + (r.p1 - r.p0) * aeth::rng::u01_from_word_d(b.x);
+        return r.p0 + r.p1 * aeth::rng::normal_from_block_d(b);
+    } else {
+        const float p0 = (float)r.p0, p1 = (float)r.p1;
+        if (r.dist == AETH_DIST_UNIFORM) return p0 + (p1 - p0) * aeth::rng::u01_from_word_f(b.x);
+        return p0 + p1 * aeth::rng::normal_from_block_f(b);
+    }
+}
+)K";
+    s += "extern \"C\" __global__ __launch_bounds__(256) void aeth_tape_jit(const TapeArgsJ A, void* part_, int vec, "
+         "int nt)\n{\n";
+    s += std::string("  typedef ") + (dbl ? "double" : "float") + " T;\n";
+    s += "  constexpr int M = " + std::to_string(M) + ";\n";
+    s += std::string("  constexpr bool SUM = ") + (mode == AETH_SUM_NONE ? "false" : "true") + ";\n";
+    s += "  constexpr int VN = 16 / sizeof(T);\n  constexpr int ROUNDS = 2048 / (256 * VN);\n";
+    s += "  __shared__ aeth::acc::Pair<T> lds[4];\n";
+    s += "  aeth::acc::Pair<T>* part = (aeth::acc::Pair<T>*)part_;\n  const uint32_t n = A.n;\n";
+    s += "  for (uint32_t u = blockIdx.x; u < A.units; u += gridDim.x) {\n";
+    s += "    const uint32_t pi = u / A.tiles;\n    const uint32_t tile = u - pi * A.tiles;\n";
+    s += "    const uint32_t s0 = tile * 2048;\n    const bool full = vec && s0 + 2048 <= n;\n";
+    s += "    aeth::acc::Pair<T> acc{T(0), T(0)};\n    switch (pi) {\n";
+    for (int p = 0; p < np; ++p) {
+        s += "    case " + std::to_string(p) + ": {\n" + gen_program(P[p], dbl) + "      break;\n    }\n";
+    }
+    s += "    default: break;\n    }\n";
+    s += "    if (SUM) {\n      const aeth::acc::Pair<T> rr = aeth::acc::block_combine<M>(acc, lds);\n"
+         "      if (threadIdx.x == 0) part[u] = rr;\n    }\n  }\n}\n";
+    return s;
+}
+
+// Compile a tape's kernel for `arch_opt` ("--offload-arch=gfx950"); false (log in last error) on failure.
+bool jit_compile(int dtype, int mode, int np, const aeth_program* P, const std::string& arch_opt,
+                 std::vector<char>& code)
+{
+    const std::string src = gen_kernel(dtype, mode, np, P);
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "aeth_tape_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        set_error("hiprtcCreateProgram failed");
+        return false;
+    }
+    const char* opts[] = {arch_opt.c_str(), "-O3", "-ffp-contract=off", "-std=c++20", "-Xclang", "-target-feature",
+                          "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult rc = hiprtcCompileProgram(prog, 8, opts);
+    if (rc != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls, '\0');
+        if (ls) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        set_error("hiprtc: " + log.substr(0, 600));
+        return false;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return true;
+}
+
+// Returns nullptr (and records why) when the runtime compiler is unavailable.
+hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P)
+{
+    static const bool disabled = std::getenv("AETHER_TAPE_INTERPRET") != nullptr;
+    if (disabled) return nullptr;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    const std::string key = tape_key(dtype, mode, np, P, dev);
+    std::lock_guard<std::mutex> lock(g_jit_mu);
+    auto it = g_jit.find(key);
+    if (it != g_jit.end()) {
+        ++g_jit_hits;
+        return it->second.fn;
+    }
+    hipDeviceProp_t prop;
+    check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);
+    std::vector<char> code;
+    if (!jit_compile(dtype, mode, np, P, arch, code)) {
+        ++g_jit_fallbacks;
+        return nullptr;
+    }
+    JitEntry e;
+    check(hipModuleLoadData(&e.module, code.data()), "hipModuleLoadData");
+    check(hipModuleGetFunction(&e.fn, e.module, "aeth_tape_jit"), "hipModuleGetFunction");
+    g_jit.emplace(key, e);
+    ++g_jit_compiled;
+    return e.fn;
+}
 } // namespace
 } // namespace aeth
 
@@ -419,9 +650,35 @@ extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* pro
         A.units = A.tiles * (uint32_t)n_programs;
         const bool nt = span_bytes >= (512ull << 20);
         hipStream_t st = as_stream(stream);
+        if (sum_mode == AETH_SUM_FLOATFLOAT && dtype == AETH_F64)
+            fail("aeth_eval_tape", "float-float accumulation needs dtype F32", 0, 2);
+        hipFunction_t jit = jit_function(dtype, sum_mode, n_programs, programs);
         if (dtype == AETH_F64)
-            dispatch<double>(A, max_loads, n_programs, vec, nt, sum_mode, static_cast<double*>(sums), st);
+            dispatch<double>(A, max_loads, n_programs, vec, nt, sum_mode, static_cast<double*>(sums), st, jit);
         else
-            dispatch<float>(A, max_loads, n_programs, vec, nt, sum_mode, static_cast<float*>(sums), st);
+            dispatch<float>(A, max_loads, n_programs, vec, nt, sum_mode, static_cast<float*>(sums), st, jit);
+    });
+}
+
+extern "C" int aeth_tape_jit_stats(uint64_t* compiled, uint64_t* cache_hits, uint64_t* fallbacks)
+{
+    return guarded([&] {
+        if (compiled) *compiled = g_jit_compiled.load();
+        if (cache_hits) *cache_hits = g_jit_hits.load();
+        if (fallbacks) *fallbacks = g_jit_fallbacks.load();
+    });
+}
+
+extern "C" int aeth_tape_jit_build(int dtype, int sum_mode, int n_programs, const aeth_program* programs,
+                                   const char* arch, uint64_t* code_bytes)
+{
+    return guarded([&] {
+        if (n_programs < 1 || n_programs > AETH_TAPE_MAX_PROGRAMS) fail("aeth_tape_jit_build", "1..8 programs");
+        for (int p = 0; p < n_programs; ++p)
+            validate(programs[p], AETH_TAPE_MAX_LEAVES, AETH_TAPE_MAX_CONSTS, AETH_TAPE_MAX_RNGS);
+        std::vector<char> code;
+        if (!jit_compile(dtype, sum_mode, n_programs, programs, std::string("--offload-arch=") + arch, code))
+            throw Failure{1};
+        if (code_bytes) *code_bytes = code.size();
     });
 }

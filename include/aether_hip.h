@@ -164,6 +164,16 @@ int aeth_eval_tape(int dtype, int n_programs, const aeth_program* programs, int 
                    const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs,
                    const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums, void* stream);
 
+/* Each distinct tape structure is lowered at run time (hipRTC) to its own
+ * straight-line kernel, compiled once per process and cached; the resident
+ * interpreter kernel runs a tape when the runtime compiler is unavailable or
+ * AETHER_TAPE_INTERPRET is set. Both give the same bits. */
+int aeth_tape_jit_stats(uint64_t* compiled, uint64_t* cache_hits, uint64_t* fallbacks);
+/* Compile (without loading or running) the kernel of a tape for `arch`
+ * ("gfx950"); works without a GPU. code_bytes: size of the code object. */
+int aeth_tape_jit_build(int dtype, int sum_mode, int n_programs, const aeth_program* programs, const char* arch,
+                        uint64_t* code_bytes);
+
 /* e = cross(a, b).dot(c) + sqrt(a.dot(a) + 1) over Array<double,3> a, b, c
  * (pitch `pitch`) into a scalar plane e (expr/nodes/Geometric.h:32-73,
  * expr/Expression.h:162-173, expr/Reduce.h:150-190, math/math.h:132). */

@@ -71,3 +71,19 @@ def test_limits_are_enforced():
         x = x * 1.0001
     with pytest.raises(ValueError):
         E.lower(d, x)
+
+
+@pytest.mark.parametrize("dt,mode", [(torch.float64, 0), (torch.float32, 2), (torch.float64, -1)])
+def test_tape_kernels_compile_without_a_gpu(dt, mode):
+    """The runtime lowering of a tape produces a gfx950 code object (hipRTC needs no device)."""
+    import ctypes as C
+
+    from aether_amd import _lib
+
+    a, b, c, d = arr(3, dt), arr(3, dt), arr(3, dt), arr(3, dt)
+    g = ae.Generator.from_seed(1)
+    x = a + b * c + E.sin(a) * E.exp(-1.0 * (b * b)) + E.log(E.abs_(a) + 1.0) / 3.0 - E.uniform(g, 3, 0.0, 1.0)
+    progs, _, _, _ = E.lower(d, x)
+    size = C.c_uint64()
+    _lib.call("aeth_tape_jit_build", 0 if dt == torch.float64 else 1, mode, 3, progs, b"gfx950", C.byref(size))
+    assert size.value > 1000

@@ -96,3 +96,40 @@ def test_invalid_tape_is_rejected(dev):
     with pytest.raises(ae.AetherError):
         ae._lib.call("aeth_eval_tape", ae.F64, 1, progs, len(leaves), ptrs, pitch, 1, cst, 0, None, n, -1, None,
                      ae.stream_handle())
+
+
+def test_runtime_lowering_was_used(dev):
+    import ctypes as C
+
+    c, h, f = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    ae._lib.call("aeth_tape_jit_stats", C.byref(c), C.byref(h), C.byref(f))
+    assert c.value >= 1 and f.value == 0
+
+
+INTERP_SCRIPT = r"""
+import torch, aether_amd as ae
+from aether_amd import expr as E
+n = 30001
+arrs = []
+for s in (42, 43, 44):
+    a = ae.Array(n, 3, torch.float64); ae.array_fill_uniform(a, s); arrs.append(a)
+a, b, c = arrs
+d1, d2 = ae.Array(n, 3, torch.float64), ae.Array(n, 3, torch.float64)
+s1 = ae.eval_cfg0(a, b, c, d1, ae.SUM_NEUMAIER)
+s2 = E.assign(d2, a + b * c + E.sin(a) * E.exp(-1.0 * (b * b)), ae.SUM_NEUMAIER)
+torch.cuda.synchronize()
+assert torch.equal(d1.data[:, :n], d2.data[:, :n]) and torch.equal(s1, s2)
+print("interpreter ok")
+"""
+
+
+def test_interpreter_path_same_bits(dev):
+    import os
+    import subprocess
+    import sys
+
+    env = dict(os.environ, AETHER_TAPE_INTERPRET="1")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", INTERP_SCRIPT], cwd=root, env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0 and "interpreter ok" in r.stdout, r.stderr[-2000:]
