@@ -17,6 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libaether_hip.so")
+API_CHECK = os.path.join(LIB_DIR, "api_check")
+ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AETH_OFFLOAD_ARCH", "gfx950")
 
@@ -84,9 +86,24 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_api_check()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources")
+        print(f"built {LIB} from {len(srcs)} sources, and {API_CHECK}")
     return LIB
+
+
+def build_api_check() -> str:
+    """The C++ front end's test driver (tests/cpp/api_check.cpp): a host-only g++ build
+    against include/aether/aether.hpp and libaether_hip.so, no HIP headers involved."""
+    src = os.path.join(ROOT, "tests", "cpp", "api_check.cpp")
+    cmd = ["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-Wextra", "-Werror",
+           "-I", os.path.join(ROOT, "include"), src, "-o", API_CHECK + ".tmp", "-L", LIB_DIR, "-laether_hip",
+           "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{res.stderr}")
+    os.replace(API_CHECK + ".tmp", API_CHECK)
+    return API_CHECK
 
 
 if __name__ == "__main__":

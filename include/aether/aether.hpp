@@ -1,0 +1,840 @@
+// aether/aether.hpp -- the C++ front end of libaether_hip.so, in aether's spelling.
+//
+// A host translation unit (g++ or clang, C++20, no HIP headers) keeps the
+// reference's API for the hot path and gets one gfx950 kernel per assignment:
+//
+//   aether::Array<double, 3> a(n), b(n), c(n), d(n);
+//   const aether::random::Generator g(42);
+//   a.deviceView() = g.uniform<double, 3>(-1.0, 1.0);                  // Philox leaf, on the device
+//   auto dv = d.deviceView();
+//   dv = a.deviceView() + b.deviceView() * c.deviceView()
+//        + aether::cwiseSin(a.deviceView()) * aether::cwiseExp(-1.0 * (b.deviceView() * b.deviceView()));
+//   d.download();
+//
+// References: Array (array/Array.h:82-364), views and SampleIndex
+// (view/View.h:150-424, index/SampleIndex.h:36-61), lazy assignment
+// (expr/Assign.h:124-160,251-346), operators and nodes (expr/Operations.h:
+// 25-71, expr/nodes/Arithmetic.h:33-143, expr/nodes/Elementwise.h:245-533,
+// expr/nodes/Geometric.h:32-73, expr/Reduce.h:150-190), Generator and leaves
+// (random/Generator.h:56-195, random/detail/DistributionLeaf.h:111-236,
+// random/free.h:41-44), FFT (fft/Fft.h:58-236).
+//
+// Device assignment lowers the tree to the tape of aeth_eval_tape (one program
+// per component; node semantics: -x is x*-1, x/s is x*(1/s) with the
+// reciprocal rounded in T, folds right-nested). The per-sample host spelling
+// `hv[SampleIndex::make(i)] = expr` evaluates on the host with std:: math,
+// like the reference's host route.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../aether_hip.h"
+
+namespace aether {
+
+class Error : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
+};
+
+using Stream = void*; // a hipStream_t; nullptr = the null stream
+using offset_t = std::uint32_t;
+inline constexpr std::size_t kCapacityAlignment = 32;
+
+namespace detail {
+
+inline void check(int rc, const char* op)
+{
+    if (rc != 0) throw Error(std::string(op) + ": " + aeth_last_error());
+}
+
+inline const AetherMemoryV1& seam()
+{
+    static const AetherMemoryV1* t = aether_memory_v1();
+    return *t;
+}
+
+inline int current_device()
+{
+    int d = 0;
+    check(seam().current_device(&d), "current_device");
+    return d;
+}
+
+template<class T>
+constexpr int dtype_code()
+{
+    static_assert(std::is_same_v<T, double> || std::is_same_v<T, float>, "aether: T must be float or double");
+    return std::is_same_v<T, double> ? AETH_F64 : AETH_F32;
+}
+
+// random/detail/Backend.h:75-81
+constexpr std::uint64_t splitmix64(std::uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+// Philox4x32-10 block at (seed, global, counter) (random/detail/Philox.h:95-133, Backend.h:144-150)
+inline std::array<std::uint32_t, 4> philox_block(std::uint64_t seed, std::uint32_t global, std::uint64_t counter)
+{
+    std::uint32_t x = (std::uint32_t)counter, y = (std::uint32_t)(counter >> 32), z = global, w = 0;
+    std::uint32_t k0 = (std::uint32_t)seed, k1 = (std::uint32_t)(seed >> 32);
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const std::uint64_t p0 = (std::uint64_t)0xD2511F53u * x, p1 = (std::uint64_t)0xCD9E8D57u * z;
+        const std::uint32_t nx = (std::uint32_t)(p1 >> 32) ^ y ^ k0, nz = (std::uint32_t)(p0 >> 32) ^ w ^ k1;
+        x = nx;
+        y = (std::uint32_t)p1;
+        z = nz;
+        w = (std::uint32_t)p0;
+    }
+    return {x, y, z, w};
+}
+
+// ---- lowering context for aeth_eval_tape
+struct TapeCtx {
+    int dtype = AETH_F64;
+    std::vector<const void*> leaves;
+    std::vector<std::uint64_t> pitch;
+    std::vector<double> consts;
+    std::vector<aeth_rng_leaf> rngs;
+    std::vector<std::pair<std::uint8_t, std::uint8_t>> loads; // (leaf, comp) of the current program
+    std::vector<aeth_insn> insns;
+
+    int leaf(const void* p, std::uint64_t pt)
+    {
+        for (std::size_t k = 0; k < leaves.size(); ++k)
+            if (leaves[k] == p) return (int)k;
+        if (leaves.size() >= AETH_TAPE_MAX_LEAVES) throw Error("aether: too many arrays in one assignment");
+        leaves.push_back(p);
+        pitch.push_back(pt);
+        return (int)leaves.size() - 1;
+    }
+    void push(const void* p, std::uint64_t pt, std::size_t comp)
+    {
+        const std::pair<std::uint8_t, std::uint8_t> key{(std::uint8_t)leaf(p, pt), (std::uint8_t)comp};
+        std::size_t j = 0;
+        while (j < loads.size() && loads[j] != key) ++j;
+        if (j == loads.size()) loads.push_back(key);
+        insns.push_back(aeth_insn{AETH_OP_PUSH, (std::uint8_t)j});
+    }
+    void constant(double v)
+    {
+        std::size_t j = 0;
+        while (j < consts.size() && std::memcmp(&consts[j], &v, sizeof v) != 0) ++j;
+        if (j == consts.size()) consts.push_back(v);
+        insns.push_back(aeth_insn{AETH_OP_CONST, (std::uint8_t)j});
+    }
+    void rng(std::uint64_t seed, double p0, double p1, std::uint32_t comp, std::uint32_t dist)
+    {
+        std::size_t j = 0;
+        for (; j < rngs.size(); ++j) {
+            const auto& r = rngs[j];
+            if (r.seed == seed && r.p0 == p0 && r.p1 == p1 && r.comp == comp && r.dist == dist) break;
+        }
+        if (j == rngs.size()) rngs.push_back(aeth_rng_leaf{seed, p0, p1, comp, dist});
+        insns.push_back(aeth_insn{AETH_OP_RNG, (std::uint8_t)j});
+    }
+    void op(int o) { insns.push_back(aeth_insn{(std::uint8_t)o, 0}); }
+
+    std::vector<aeth_program> programs; // one per destination component
+};
+
+/**
+ * Lower `dest = e` (dest: N planes `pitch` elements apart at `dest`) to one
+ * program per component. Leaves are numbered in order of first use, the
+ * destination after the first program's operands; constants and random
+ * leaves are shared across programs.
+ */
+template<class T, std::size_t N, class E>
+TapeCtx lower(const void* dest, std::size_t pitch, const E& e)
+{
+    static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
+    static_assert(N <= AETH_TAPE_MAX_PROGRAMS, "aether: too many components for one assignment");
+    TapeCtx x;
+    x.dtype = dtype_code<T>();
+    x.programs.resize(N);
+    for (std::size_t c = 0; c < N; ++c) {
+        x.loads.clear();
+        x.insns.clear();
+        e.emit(E::ncomp > 1 ? c : 0, x);
+        const int out = x.leaf(dest, pitch);
+        if (x.loads.size() > AETH_TAPE_MAX_LOADS || x.insns.size() > AETH_TAPE_MAX_INSNS)
+            throw Error("aether: expression too large for one program");
+        aeth_program& p = x.programs[c];
+        std::memset(&p, 0, sizeof p);
+        p.n_loads = (std::uint8_t)x.loads.size();
+        p.n_insns = (std::uint8_t)x.insns.size();
+        p.out_leaf = (std::uint8_t)out;
+        p.out_comp = (std::uint8_t)c;
+        for (std::size_t k = 0; k < x.loads.size(); ++k) {
+            p.load_leaf[k] = x.loads[k].first;
+            p.load_comp[k] = x.loads[k].second;
+        }
+        for (std::size_t k = 0; k < x.insns.size(); ++k) p.insns[k] = x.insns[k];
+    }
+    if (x.consts.size() > AETH_TAPE_MAX_CONSTS || x.rngs.size() > AETH_TAPE_MAX_RNGS)
+        throw Error("aether: too many constants or random leaves");
+    return x;
+}
+
+} // namespace detail
+
+// ---------------------------------------------------------------- index
+/** index/SampleIndex.h:36-61 */
+struct SampleIndex {
+    offset_t g = 0, w = 0;
+    static constexpr SampleIndex make(std::size_t index) { return SampleIndex{(offset_t)index, (offset_t)index}; }
+    constexpr offset_t global() const { return g; }
+    constexpr offset_t work() const { return w; }
+};
+
+// ---------------------------------------------------------------- expressions
+template<class D>
+struct Expr;
+
+template<class E>
+concept aether_expression = std::is_base_of_v<Expr<E>, E>;
+
+template<class E>
+using elem_t = typename E::element_type;
+
+template<class L, class R, int OP>
+struct BinaryNode;
+template<class E, int FN>
+struct UnaryNode;
+template<class L, class R>
+struct DotNode;
+template<class L, class R>
+struct CrossNode;
+template<class T>
+struct ScalarNode;
+
+/** CRTP base (expr/Expression.h:145-486): members dot/cross/squaredNorm/norm. */
+template<class D>
+struct Expr {
+    const D& self() const { return static_cast<const D&>(*this); }
+    template<class E>
+    auto dot(const E& o) const
+    {
+        return DotNode<D, E>(self(), o);
+    }
+    template<class E>
+    auto cross(const E& o) const
+    {
+        return CrossNode<D, E>(self(), o);
+    }
+    auto squaredNorm() const { return DotNode<D, D>(self(), self()); }
+    auto norm() const { return UnaryNode<DotNode<D, D>, AETH_OP_SQRT>(DotNode<D, D>(self(), self())); }
+};
+
+template<class T>
+struct ScalarNode : Expr<ScalarNode<T>> {
+    using element_type = T;
+    static constexpr std::size_t ncomp = 1;
+    T v;
+    explicit ScalarNode(T x) : v(x) {}
+    void emit(std::size_t, detail::TapeCtx& c) const { c.constant((double)v); }
+    T eval(std::size_t, std::size_t) const { return v; }
+};
+
+template<class L, class R, int OP>
+struct BinaryNode : Expr<BinaryNode<L, R, OP>> {
+    using element_type = elem_t<L>;
+    static constexpr std::size_t ncomp = L::ncomp > R::ncomp ? L::ncomp : R::ncomp;
+    static_assert(L::ncomp == R::ncomp || L::ncomp == 1 || R::ncomp == 1, "aether: component counts differ");
+    static_assert(std::is_same_v<elem_t<L>, elem_t<R>>, "aether: element types differ");
+    L l;
+    R r;
+    BinaryNode(const L& a, const R& b) : l(a), r(b) {}
+    void emit(std::size_t c, detail::TapeCtx& x) const
+    {
+        l.emit(L::ncomp > 1 ? c : 0, x);
+        r.emit(R::ncomp > 1 ? c : 0, x);
+        x.op(OP);
+    }
+    element_type eval(std::size_t c, std::size_t i) const
+    {
+        const element_type a = l.eval(L::ncomp > 1 ? c : 0, i), b = r.eval(R::ncomp > 1 ? c : 0, i);
+        if constexpr (OP == AETH_OP_ADD) return a + b;
+        else if constexpr (OP == AETH_OP_SUB) return a - b;
+        else if constexpr (OP == AETH_OP_MUL) return a * b;
+        else return a / b;
+    }
+};
+
+template<class E, int FN>
+struct UnaryNode : Expr<UnaryNode<E, FN>> {
+    using element_type = elem_t<E>;
+    static constexpr std::size_t ncomp = E::ncomp;
+    E e;
+    explicit UnaryNode(const E& x) : e(x) {}
+    void emit(std::size_t c, detail::TapeCtx& x) const
+    {
+        e.emit(c, x);
+        x.op(FN);
+    }
+    element_type eval(std::size_t c, std::size_t i) const
+    {
+        const element_type a = e.eval(c, i);
+        if constexpr (FN == AETH_OP_SIN) return std::sin(a);
+        else if constexpr (FN == AETH_OP_COS) return std::cos(a);
+        else if constexpr (FN == AETH_OP_EXP) return std::exp(a);
+        else if constexpr (FN == AETH_OP_LOG) return std::log(a);
+        else if constexpr (FN == AETH_OP_SQRT) return std::sqrt(a);
+        else if constexpr (FN == AETH_OP_ABS) return std::fabs(a);
+        else return -a;
+    }
+};
+
+/** expr/Reduce.h:172-188: (x_{N-1} y_{N-1}) + (... + (x_0 y_0)) */
+template<class L, class R>
+struct DotNode : Expr<DotNode<L, R>> {
+    using element_type = elem_t<L>;
+    static constexpr std::size_t ncomp = 1;
+    static_assert(L::ncomp == R::ncomp, "aether: dot operands differ in size");
+    L l;
+    R r;
+    DotNode(const L& a, const R& b) : l(a), r(b) {}
+    void emit(std::size_t, detail::TapeCtx& x) const
+    {
+        for (std::size_t k = L::ncomp; k-- > 0;) {
+            l.emit(k, x);
+            r.emit(k, x);
+            x.op(AETH_OP_MUL);
+        }
+        for (std::size_t k = 1; k < L::ncomp; ++k) x.op(AETH_OP_ADD);
+    }
+    element_type eval(std::size_t, std::size_t i) const
+    {
+        element_type acc = l.eval(0, i) * r.eval(0, i);
+        for (std::size_t k = 1; k < L::ncomp; ++k) acc = l.eval(k, i) * r.eval(k, i) + acc;
+        return acc;
+    }
+};
+
+/** expr/nodes/Geometric.h:55-71 */
+template<class L, class R>
+struct CrossNode : Expr<CrossNode<L, R>> {
+    using element_type = elem_t<L>;
+    static constexpr std::size_t ncomp = 3;
+    static_assert(L::ncomp == 3 && R::ncomp == 3, "aether: cross needs 3-vectors");
+    L l;
+    R r;
+    CrossNode(const L& a, const R& b) : l(a), r(b) {}
+    static constexpr std::size_t I[3] = {1, 2, 0}, J[3] = {2, 0, 1};
+    void emit(std::size_t c, detail::TapeCtx& x) const
+    {
+        l.emit(I[c], x);
+        r.emit(J[c], x);
+        x.op(AETH_OP_MUL);
+        l.emit(J[c], x);
+        r.emit(I[c], x);
+        x.op(AETH_OP_MUL);
+        x.op(AETH_OP_SUB);
+    }
+    element_type eval(std::size_t c, std::size_t i) const
+    {
+        return l.eval(I[c], i) * r.eval(J[c], i) - l.eval(J[c], i) * r.eval(I[c], i);
+    }
+};
+
+// Scalars wrap into ScalarNode of the other operand's element type.
+template<class E, class S>
+auto wrap_scalar(const S& s)
+{
+    return ScalarNode<elem_t<E>>((elem_t<E>)s);
+}
+
+template<class L, class R>
+    requires aether_expression<L> && aether_expression<R>
+auto operator+(const L& l, const R& r)
+{
+    return BinaryNode<L, R, AETH_OP_ADD>(l, r);
+}
+template<class L, class R>
+    requires aether_expression<L> && aether_expression<R>
+auto operator-(const L& l, const R& r)
+{
+    return BinaryNode<L, R, AETH_OP_SUB>(l, r);
+}
+template<class L, class R>
+    requires aether_expression<L> && aether_expression<R>
+auto operator*(const L& l, const R& r)
+{
+    return BinaryNode<L, R, AETH_OP_MUL>(l, r);
+}
+template<class L, class R>
+    requires aether_expression<L> && aether_expression<R>
+auto operator/(const L& l, const R& r)
+{
+    return BinaryNode<L, R, AETH_OP_DIV>(l, r);
+}
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto operator+(const E& e, S s)
+{
+    return e + wrap_scalar<E>(s);
+}
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto operator+(S s, const E& e)
+{
+    return wrap_scalar<E>(s) + e;
+}
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto operator-(const E& e, S s)
+{
+    return e - wrap_scalar<E>(s);
+}
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto operator-(S s, const E& e)
+{
+    return wrap_scalar<E>(s) - e;
+}
+/** CWiseScale (expr/Operations.h:49-59): s * e */
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto operator*(S s, const E& e)
+{
+    return wrap_scalar<E>(s) * e;
+}
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto operator*(const E& e, S s)
+{
+    return e * wrap_scalar<E>(s);
+}
+/** expr/Operations.h:62-71: e / s is e * (1 / s), the reciprocal rounded in T */
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto operator/(const E& e, S s)
+{
+    return e * wrap_scalar<E>(elem_t<E>(1) / (elem_t<E>)s);
+}
+/** -x is x * -1 (docs/content/numerics_contract.rst:66) */
+template<class E>
+    requires aether_expression<E>
+auto operator-(const E& e)
+{
+    return e * wrap_scalar<E>(-1);
+}
+
+template<class E>
+    requires aether_expression<E>
+auto cwiseSin(const E& e) { return UnaryNode<E, AETH_OP_SIN>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseCos(const E& e) { return UnaryNode<E, AETH_OP_COS>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseExp(const E& e) { return UnaryNode<E, AETH_OP_EXP>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseLog(const E& e) { return UnaryNode<E, AETH_OP_LOG>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseSqrt(const E& e) { return UnaryNode<E, AETH_OP_SQRT>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseAbs(const E& e) { return UnaryNode<E, AETH_OP_ABS>(e); }
+template<class E>
+    requires aether_expression<E>
+auto sqrt(const E& e) { return UnaryNode<E, AETH_OP_SQRT>(e); }
+
+// ---------------------------------------------------------------- views
+template<class T, std::size_t N>
+class View;
+
+/** A view as an expression leaf (view/View.h:150-424). */
+template<class T, std::size_t N>
+struct ViewLeaf : Expr<ViewLeaf<T, N>> {
+    using element_type = std::remove_const_t<T>;
+    static constexpr std::size_t ncomp = N;
+    T* p;
+    std::size_t pitch;
+    void emit(std::size_t c, detail::TapeCtx& x) const { x.push(p, pitch, c); }
+    element_type eval(std::size_t c, std::size_t i) const { return p[c * pitch + i]; }
+};
+
+enum class SumMode : int { None = AETH_SUM_NONE, Neumaier = AETH_SUM_NEUMAIER, Plain = AETH_SUM_PLAIN,
+                           FloatFloat = AETH_SUM_FLOATFLOAT };
+
+/** SampleRef (expr/Assign.h:251-346): host per-sample assignment. */
+template<class T, std::size_t N>
+class SampleRef {
+public:
+    SampleRef(T* p, std::size_t pitch, std::size_t i) : p_(p), pitch_(pitch), i_(i) {}
+    template<class E>
+    SampleRef& operator=(const E& e)
+    {
+        for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] = (T)e.eval(E::ncomp > 1 ? c : 0, i_);
+        return *this;
+    }
+    template<class E>
+    SampleRef& operator+=(const E& e)
+    {
+        for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] += (T)e.eval(E::ncomp > 1 ? c : 0, i_);
+        return *this;
+    }
+    std::array<T, N> get() const
+    {
+        std::array<T, N> r{};
+        for (std::size_t c = 0; c < N; ++c) r[c] = p_[c * pitch_ + i_];
+        return r;
+    }
+
+private:
+    T* p_;
+    std::size_t pitch_, i_;
+};
+
+/**
+ * View<T, N>: `samples` items of N components, plane pitch `pitch`
+ * (layout_stride with strides {pitch, 1}). A device view assigned an
+ * expression runs one kernel over all samples; a host view supports the
+ * per-sample spelling and element access.
+ */
+template<class T, std::size_t N>
+class View : public Expr<View<T, N>> {
+public:
+    using element_type = T;
+    static constexpr std::size_t ncomp = N;
+
+    View(T* p, std::size_t samples, std::size_t pitch, bool device)
+        : p_(p), samples_(samples), pitch_(pitch), device_(device)
+    {
+    }
+    View(const View&) = default;
+    T* data() const { return p_; }
+    std::size_t samples() const { return samples_; }
+    std::size_t pitch() const { return pitch_; }
+    bool on_device() const { return device_; }
+    T& operator()(std::size_t c, std::size_t i) const { return p_[c * pitch_ + i]; }
+    T& operator()(std::size_t i) const { return p_[i]; }
+    SampleRef<T, N> operator[](SampleIndex i) const { return SampleRef<T, N>(p_, pitch_, i.global()); }
+
+    void emit(std::size_t c, detail::TapeCtx& x) const { x.push(p_, pitch_, c); }
+    T eval(std::size_t c, std::size_t i) const { return p_[c * pitch_ + i]; }
+
+    /** Whole-array lazy assignment: one kernel (device views only). */
+    template<class E>
+        requires aether_expression<E>
+    View& operator=(const E& e)
+    {
+        assign(e, SumMode::None, nullptr, nullptr);
+        return *this;
+    }
+    View& operator=(const View& o)
+    {
+        if (this != &o) assign(o, SumMode::None, nullptr, nullptr);
+        return *this;
+    }
+
+    /**
+     * dest = e, fused with per-component sums of dest (sums_host: N values
+     * hi + comp when mode != None). Synchronises the stream when sums are asked.
+     */
+    template<class E>
+    void assign(const E& e, SumMode mode, double* sums_host, Stream stream)
+    {
+        static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
+        if (!device_) throw Error("aether: whole-array assignment needs a device view");
+        const detail::TapeCtx x = detail::lower<T, N>(p_, pitch_, e);
+        void* sums_dev = nullptr;
+        void* chunk = nullptr;
+        const int dev = detail::current_device();
+        if (mode != SumMode::None)
+            detail::check(detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, 2 * N * sizeof(T), stream, &chunk,
+                                                     &sums_dev),
+                          "chunk_alloc");
+        const int rc = aeth_eval_tape(x.dtype, (int)N, x.programs.data(), (int)x.leaves.size(),
+                                      const_cast<void* const*>(x.leaves.data()), x.pitch.data(), (int)x.consts.size(),
+                                      x.consts.data(), (int)x.rngs.size(), x.rngs.data(), samples_, (int)mode,
+                                      sums_dev, stream);
+        if (rc == 0 && mode != SumMode::None) {
+            T h[2 * N];
+            detail::check(detail::seam().copy(AETHER_MEM_CPU, dev, h, AETHER_MEM_DEVICE, dev, sums_dev,
+                                              sizeof h, stream),
+                          "copy");
+            for (std::size_t c = 0; c < N; ++c) sums_host[c] = (double)h[2 * c] + (double)h[2 * c + 1];
+        }
+        if (chunk) detail::seam().chunk_release(chunk);
+        detail::check(rc, "aeth_eval_tape");
+    }
+
+private:
+    T* p_;
+    std::size_t samples_, pitch_;
+    bool device_;
+};
+
+// ---------------------------------------------------------------- Array
+/**
+ * Array<T, N> (array/Array.h:82-364): N components per sample, SoA planes of
+ * capacity() elements (samples rounded up to 32), one pinned host copy and
+ * one device copy, moved with upload()/download().
+ */
+template<class T, std::size_t N = 1>
+class Array {
+public:
+    explicit Array(std::size_t n_samples) : samples_(n_samples), capacity_(quantize(n_samples)) { allocate(); }
+    Array(const Array&) = delete;
+    Array& operator=(const Array&) = delete;
+    Array(Array&& o) noexcept { swap(o); }
+    Array& operator=(Array&& o) noexcept
+    {
+        swap(o);
+        return *this;
+    }
+    ~Array() { release(); }
+
+    std::size_t samples() const { return samples_; }
+    std::size_t size() const { return N * samples_; }
+    std::size_t capacity() const { return capacity_; }
+    View<T, N> hostView() const { return View<T, N>(static_cast<T*>(host_), samples_, capacity_, false); }
+    View<T, N> deviceView() const { return View<T, N>(static_cast<T*>(dev_), samples_, capacity_, true); }
+    void upload(Stream s = nullptr)
+    {
+        const int dev = detail::current_device();
+        detail::check(detail::seam().copy(AETHER_MEM_DEVICE, dev, dev_, AETHER_MEM_PINNED, dev, host_, bytes(),
+                                          s),
+                      "upload");
+        detail::check(detail::seam().stream_sync(dev, s), "upload");
+    }
+    void download(Stream s = nullptr)
+    {
+        const int dev = detail::current_device();
+        detail::check(detail::seam().copy(AETHER_MEM_PINNED, dev, host_, AETHER_MEM_DEVICE, dev, dev_, bytes(),
+                                          s),
+                      "download");
+        detail::check(detail::seam().stream_sync(dev, s), "download");
+    }
+    /** Grow capacity (array/Array.h:120-135): pitched copy of the samples on both sides. */
+    void reserve(std::size_t n)
+    {
+        if (n <= capacity_) return;
+        Array next(samples_);
+        next.release();
+        next.capacity_ = quantize(n);
+        next.allocate();
+        const int dev = detail::current_device();
+        for (std::size_t c = 0; c < N; ++c) {
+            detail::check(detail::seam().copy(AETHER_MEM_DEVICE, dev, static_cast<T*>(next.dev_) + c * next.capacity_,
+                                              AETHER_MEM_DEVICE, dev, static_cast<T*>(dev_) + c * capacity_,
+                                              samples_ * sizeof(T), nullptr),
+                          "reserve");
+            std::memcpy(static_cast<T*>(next.host_) + c * next.capacity_, static_cast<T*>(host_) + c * capacity_,
+                        samples_ * sizeof(T));
+        }
+        detail::check(detail::seam().stream_sync(dev, nullptr), "reserve");
+        swap(next);
+    }
+    void resize(std::size_t n)
+    {
+        reserve(n);
+        samples_ = n;
+    }
+
+private:
+    static std::size_t quantize(std::size_t n)
+    {
+        return (n + kCapacityAlignment - 1) / kCapacityAlignment * kCapacityAlignment;
+    }
+    std::size_t bytes() const { return N * capacity_ * sizeof(T); }
+    void allocate()
+    {
+        const int dev = detail::current_device();
+        const std::size_t b = bytes() ? bytes() : sizeof(T);
+        detail::check(detail::seam().chunk_alloc(AETHER_MEM_PINNED, dev, b, nullptr, &host_chunk_, &host_),
+                      "Array host allocation");
+        detail::check(detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, b, nullptr, &dev_chunk_, &dev_),
+                      "Array device allocation");
+        detail::check(detail::seam().stream_sync(dev, nullptr), "Array allocation");
+    }
+    void release()
+    {
+        if (dev_chunk_) detail::seam().chunk_release(dev_chunk_);
+        if (host_chunk_) detail::seam().chunk_release(host_chunk_);
+        dev_chunk_ = host_chunk_ = nullptr;
+        dev_ = host_ = nullptr;
+    }
+    void swap(Array& o)
+    {
+        std::swap(samples_, o.samples_);
+        std::swap(capacity_, o.capacity_);
+        std::swap(host_chunk_, o.host_chunk_);
+        std::swap(dev_chunk_, o.dev_chunk_);
+        std::swap(host_, o.host_);
+        std::swap(dev_, o.dev_);
+    }
+    std::size_t samples_ = 0, capacity_ = 0;
+    void* host_chunk_ = nullptr;
+    void* dev_chunk_ = nullptr;
+    void* host_ = nullptr;
+    void* dev_ = nullptr;
+};
+
+template<class T, std::size_t N>
+using VecView = View<T, N>;
+using Vec3dView = View<double, 3>;
+
+/** dest = e with fused per-component sums (hi + comp per component) copied to the host. */
+template<class T, std::size_t N, class E>
+std::array<double, N> evaluate(View<T, N> dest, const E& e, SumMode mode = SumMode::Neumaier, Stream s = nullptr)
+{
+    std::array<double, N> sums{};
+    dest.assign(e, mode, sums.data(), s);
+    return sums;
+}
+
+// ---------------------------------------------------------------- random
+namespace random {
+
+inline constexpr std::uint64_t DEFAULT_SEED = 0xFE7A5EED0C0FFEE5ULL; // random/free.h:44
+
+/** A distribution leaf (random/detail/DistributionLeaf.h:111-236). */
+template<class Real, std::size_t VD>
+struct DistLeaf : Expr<DistLeaf<Real, VD>> {
+    using element_type = Real;
+    static constexpr std::size_t ncomp = VD;
+    std::uint64_t seed; // salted
+    double p0, p1;
+    std::uint32_t dist;
+    void emit(std::size_t c, detail::TapeCtx& x) const { x.rng(seed, p0, p1, (std::uint32_t)c, dist); }
+    Real eval(std::size_t c, std::size_t i) const
+    {
+        const auto b = detail::philox_block(seed, (std::uint32_t)i, c);
+        if (dist == AETH_DIST_UNIFORM) {
+            if constexpr (std::is_same_v<Real, double>) return p0 + (p1 - p0) * ((double)b[0] * 0x1p-32 + 0x1p-33);
+            else {
+                const float u = (float)(((b[0] >> 9) << 1) | 1u) * 0x1p-24f;
+                return (float)p0 + ((float)p1 - (float)p0) * u;
+            }
+        }
+        if constexpr (std::is_same_v<Real, double>) {
+            const std::uint64_t zx = (std::uint64_t)b[0] ^ ((std::uint64_t)b[1] << 21);
+            const std::uint64_t zy = (std::uint64_t)b[2] ^ ((std::uint64_t)b[3] << 21);
+            const double u = (double)zx * 0x1p-53 + 0x1p-54, v = (double)zy * (0x1p-53 * 2.0) + 0x1p-53;
+            return p0 + p1 * (std::sqrt(-2.0 * std::log(u)) * std::sin(v * 3.1415926535897932));
+        } else {
+            const float k = 2.3283064e-10f * 6.2831855f;
+            const float u = (float)(((b[0] >> 9) << 1) | 1u) * 0x1p-24f, v = (float)b[1] * k + (k / 2.0f);
+            return (float)p0 + (float)p1 * (std::sqrt(-2.0f * std::log(u)) * std::sin(v));
+        }
+    }
+};
+
+/** random::Generator (random/Generator.h:56-195). */
+class Generator {
+public:
+    constexpr explicit Generator(std::uint64_t seed, std::uint64_t stream_id = 0)
+        : key_(detail::splitmix64(seed ^ detail::splitmix64(stream_id)))
+    {
+    }
+    constexpr Generator substream(std::uint64_t k) const
+    {
+        Generator g(0);
+        g.key_ = detail::splitmix64(key_ ^ detail::splitmix64(k ^ 0x2545F4914F6CDD1DULL));
+        return g;
+    }
+    template<class Real, std::size_t VD>
+    DistLeaf<Real, VD> uniform(Real lo, Real hi) const
+    {
+        DistLeaf<Real, VD> l;
+        l.seed = key_ ^ 0x243F6A8885A308D3ULL;
+        l.p0 = lo;
+        l.p1 = hi;
+        l.dist = AETH_DIST_UNIFORM;
+        return l;
+    }
+    template<class Real, std::size_t VD>
+    DistLeaf<Real, VD> normal(Real mean, Real sd) const
+    {
+        DistLeaf<Real, VD> l;
+        l.seed = key_ ^ 0x13198A2E03707344ULL;
+        l.p0 = mean;
+        l.p1 = sd;
+        l.dist = AETH_DIST_NORMAL;
+        return l;
+    }
+    constexpr std::uint64_t key() const { return key_; }
+
+private:
+    std::uint64_t key_;
+};
+
+inline Generator seed(std::uint64_t s) { return Generator(s); }
+
+} // namespace random
+
+// ---------------------------------------------------------------- fft
+namespace fft {
+
+enum class Norm { Backward = AETHER_FFT_BACKWARD, Forward = AETHER_FFT_FORWARD, Ortho = AETHER_FFT_ORTHO };
+
+/**
+ * Complex to complex over `axes` of a dense row-major array of `shape`
+ * (fft/Fft.h:186-197). `in`/`out` hold prod(shape) interleaved (re, im)
+ * pairs, in device memory (or host memory: staged through the device).
+ */
+template<class T>
+void c2c(const T* in, T* out, std::span<const std::size_t> shape, std::span<const std::size_t> axes, bool forward,
+         Norm norm = Norm::Backward, Stream s = nullptr)
+{
+    static_assert(std::is_same_v<T, double> || std::is_same_v<T, float>);
+    const auto& t = detail::seam();
+    const int dev = detail::current_device();
+    std::vector<std::int64_t> sh(shape.begin(), shape.end());
+    std::vector<std::int32_t> ax(axes.begin(), axes.end());
+    std::size_t total = 1;
+    for (auto v : shape) total *= v;
+    const std::size_t bytes = 2 * total * sizeof(T);
+    int pdev = -1;
+    const bool on_device = t.pointer_device(in, &pdev) == 0;
+    const void* din = in;
+    void* dout = out;
+    void *cin = nullptr, *cout = nullptr, *bin = nullptr, *bout = nullptr;
+    if (!on_device) {
+        detail::check(t.chunk_alloc(AETHER_MEM_DEVICE, dev, bytes, s, &cin, &bin), "chunk_alloc");
+        detail::check(t.chunk_alloc(AETHER_MEM_DEVICE, dev, bytes, s, &cout, &bout), "chunk_alloc");
+        detail::check(t.copy(AETHER_MEM_DEVICE, dev, bin, AETHER_MEM_CPU, dev, in, bytes, s), "copy");
+        din = bin;
+        dout = bout;
+    }
+    void* plan = nullptr;
+    const int rc = t.fft_plan(dev, AETHER_FFT_C2C, std::is_same_v<T, double> ? 1 : 0, (int)norm, forward ? 1 : 0,
+                              (int)sh.size(), sh.data(), (int)ax.size(), ax.data(), s, &plan);
+    if (rc == 0) {
+        detail::check(t.fft_exec(plan, din, dout, s), "fft_exec");
+        t.fft_destroy(plan);
+    }
+    if (!on_device) {
+        if (rc == 0) detail::check(t.copy(AETHER_MEM_CPU, dev, out, AETHER_MEM_DEVICE, dev, bout, bytes, s), "copy");
+        t.chunk_release(cin);
+        t.chunk_release(cout);
+    }
+    detail::check(rc, "fft_plan");
+    detail::check(t.stream_sync(dev, s), "stream_sync");
+}
+
+} // namespace fft
+} // namespace aether

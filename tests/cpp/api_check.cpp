@@ -1,0 +1,222 @@
+// api_check.cpp -- drives the C++ front end (include/aether/aether.hpp) the way
+// a user of aether's headers would, for the tests in tests/test_cpp_api.py.
+// Built by __graft_entry__.build() with g++ against libaether_hip.so.
+//
+// Modes (binary little-endian on stdout unless noted):
+//   tape                       : lowered programs of fixed expressions (text; no GPU)
+//   cfg0g  <n> <seed>          : a, b, c from Generator(seed) (substreams 1, 2) on the
+//                                device, d = cfg0 with fused Neumaier sums; writes
+//                                a, b, c, d (3 x n each) then 3 x (hi, comp)
+//   host   <n> <seed>          : the same inputs and d through the per-sample host
+//                                spelling (std:: math); writes a, b, c, d
+//   vec3   <n> <sA> <sB> <sC>  : raw-seed uniform inputs, e = (a x b).c + sqrt(a.a + 1)
+//                                on the device; writes e (n)
+//   fft    <n> <fwd>           : reads n complex (re, im) doubles on stdin, writes c2c
+//   float  <n> <seed>          : cfg0 in float with float-float sums; writes d, sums
+//   resize <n>                 : upload/reserve/resize/download round trip (text)
+#include <aether/aether.hpp>
+
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+using aether::Array;
+using aether::SampleIndex;
+
+namespace {
+
+template<class T, std::size_t N>
+void write_planes(const Array<T, N>& a)
+{
+    auto v = a.hostView();
+    for (std::size_t c = 0; c < N; ++c) std::fwrite(&v(c, 0), sizeof(T), a.samples(), stdout);
+}
+
+template<class V>
+auto cfg0(const V& a, const V& b, const V& c)
+{
+    using T = typename V::element_type;
+    return a + b * c + aether::cwiseSin(a) * aether::cwiseExp(T(-1) * (b * b));
+}
+
+void print_tape(const char* name, const aether::detail::TapeCtx& x, const void* const* ids, int nids)
+{
+    std::printf("expr %s programs %zu leaves %zu consts %zu rngs %zu\n", name, x.programs.size(), x.leaves.size(),
+                x.consts.size(), x.rngs.size());
+    std::printf("leaves");
+    for (auto* p : x.leaves) {
+        int k = 0;
+        while (k < nids && ids[k] != p) ++k;
+        std::printf(" %d", k);
+    }
+    std::printf("\n");
+    for (const auto& p : x.programs) {
+        std::printf("program %d %d %d %d :", p.n_loads, p.n_insns, p.out_leaf, p.out_comp);
+        for (int k = 0; k < p.n_loads; ++k) std::printf(" %d.%d", p.load_leaf[k], p.load_comp[k]);
+        std::printf(" :");
+        for (int k = 0; k < p.n_insns; ++k) std::printf(" %d.%d", p.insns[k].op, p.insns[k].arg);
+        std::printf("\n");
+    }
+    for (double v : x.consts) std::printf("const %.17g\n", v);
+    for (const auto& r : x.rngs)
+        std::printf("rng %llu %.17g %.17g %u %u\n", (unsigned long long)r.seed, r.p0, r.p1, r.comp, r.dist);
+}
+
+int tape()
+{
+    // fake device planes: lowering reads no memory
+    static double buf[8][64];
+    const void* ids[8];
+    for (int k = 0; k < 8; ++k) ids[k] = buf[k];
+    aether::View<double, 3> a(buf[0], 10, 32, true), b(buf[1], 10, 32, true), c(buf[2], 10, 32, true),
+        d(buf[3], 10, 32, true);
+    aether::View<double, 1> e(buf[4], 10, 32, true), s(buf[5], 10, 32, true);
+    namespace ad = aether::detail;
+    print_tape("cfg0", ad::lower<double, 3>(buf[3], 32, cfg0(a, b, c)), ids, 8);
+    print_tape("vec3", ad::lower<double, 1>(buf[4], 32, a.cross(b).dot(c) + aether::sqrt(a.dot(a) + 1.0)), ids, 8);
+    print_tape("negdiv", ad::lower<double, 1>(buf[5], 32, -e / 4.0), ids, 8);
+    print_tape("ops",
+               ad::lower<double, 1>(buf[5], 32,
+                                    aether::cwiseLog(aether::cwiseAbs(e) + 0.5) -
+                                        aether::cwiseCos(e) * aether::cwiseSqrt(e * e + 1.0) / (e + 2.0)),
+               ids, 8);
+    const aether::random::Generator g(7);
+    print_tape("random",
+               ad::lower<double, 3>(buf[3], 32,
+                                    g.uniform<double, 3>(-2.0, 5.0) + g.substream(3).normal<double, 3>(1.0, 0.5) * a),
+               ids, 8);
+    print_tape("norm", ad::lower<double, 1>(buf[4], 32, a.norm() - b.squaredNorm()), ids, 8);
+    print_tape("inplace", ad::lower<double, 3>(buf[0], 32, a * 2.0 + 1.0), ids, 8);
+    return 0;
+}
+
+template<class T>
+void fill_generator(Array<T, 3>& a, Array<T, 3>& b, Array<T, 3>& c, std::uint64_t seed)
+{
+    const aether::random::Generator g(seed);
+    a.deviceView() = g.uniform<T, 3>(T(-1), T(1));
+    b.deviceView() = g.substream(1).uniform<T, 3>(T(-1), T(1));
+    c.deviceView() = g.substream(2).uniform<T, 3>(T(-1), T(1));
+}
+
+int cfg0g(std::size_t n, std::uint64_t seed)
+{
+    Array<double, 3> a(n), b(n), c(n), d(n);
+    fill_generator(a, b, c, seed);
+    const auto sums = aether::evaluate(d.deviceView(), cfg0(a.deviceView(), b.deviceView(), c.deviceView()),
+                                       aether::SumMode::Neumaier);
+    a.download();
+    b.download();
+    c.download();
+    d.download();
+    write_planes(a);
+    write_planes(b);
+    write_planes(c);
+    write_planes(d);
+    std::fwrite(sums.data(), sizeof(double), 3, stdout);
+    return 0;
+}
+
+int host(std::size_t n, std::uint64_t seed)
+{
+    Array<double, 3> a(n), b(n), c(n), d(n);
+    const aether::random::Generator g(seed);
+    const auto la = g.uniform<double, 3>(-1.0, 1.0);
+    const auto lb = g.substream(1).uniform<double, 3>(-1.0, 1.0);
+    const auto lc = g.substream(2).uniform<double, 3>(-1.0, 1.0);
+    auto av = a.hostView(), bv = b.hostView(), cv = c.hostView(), dv = d.hostView();
+    for (std::size_t k = 0; k < n; ++k) {
+        const auto i = SampleIndex::make(k);
+        av[i] = la;
+        bv[i] = lb;
+        cv[i] = lc;
+    }
+    for (std::size_t k = 0; k < n; ++k) dv[SampleIndex::make(k)] = cfg0(av, bv, cv);
+    write_planes(a);
+    write_planes(b);
+    write_planes(c);
+    write_planes(d);
+    return 0;
+}
+
+int vec3(std::size_t n, std::uint64_t sa, std::uint64_t sb, std::uint64_t sc)
+{
+    Array<double, 3> a(n), b(n), c(n);
+    Array<double> e(n);
+    const std::uint64_t seeds[3] = {sa, sb, sc};
+    Array<double, 3>* arrs[3] = {&a, &b, &c};
+    for (int k = 0; k < 3; ++k)
+        aether::detail::check(aeth_array_fill(AETH_DIST_UNIFORM, AETH_F64, seeds[k], -1.0, 1.0, 3, n,
+                                              arrs[k]->capacity(), arrs[k]->deviceView().data(), nullptr),
+                              "aeth_array_fill");
+    auto av = a.deviceView(), bv = b.deviceView(), cv = c.deviceView();
+    e.deviceView() = av.cross(bv).dot(cv) + aether::sqrt(av.dot(av) + 1.0);
+    e.download();
+    write_planes(e);
+    return 0;
+}
+
+int fft(std::size_t n, bool forward)
+{
+    std::vector<double> in(2 * n), out(2 * n);
+    if (std::fread(in.data(), sizeof(double), 2 * n, stdin) != 2 * n) return 3;
+    const std::size_t shape[1] = {n}, axes[1] = {0};
+    aether::fft::c2c<double>(in.data(), out.data(), shape, axes, forward);
+    std::fwrite(out.data(), sizeof(double), 2 * n, stdout);
+    return 0;
+}
+
+int flt(std::size_t n, std::uint64_t seed)
+{
+    Array<float, 3> a(n), b(n), c(n), d(n);
+    fill_generator(a, b, c, seed);
+    const auto sums = aether::evaluate(d.deviceView(), cfg0(a.deviceView(), b.deviceView(), c.deviceView()),
+                                       aether::SumMode::FloatFloat);
+    d.download();
+    write_planes(d);
+    std::fwrite(sums.data(), sizeof(double), 3, stdout);
+    return 0;
+}
+
+int resize(std::size_t n)
+{
+    Array<double, 2> x(n);
+    auto hv = x.hostView();
+    for (std::size_t i = 0; i < n; ++i) {
+        hv(0, i) = double(i);
+        hv(1, i) = -double(i);
+    }
+    x.upload();
+    x.deviceView() = x.deviceView() * 3.0;
+    const std::size_t cap0 = x.capacity();
+    x.resize(n + 100);
+    x.download();
+    auto v = x.hostView();
+    bool ok = x.samples() == n + 100 && x.capacity() >= n + 100 && x.capacity() % 32 == 0 && cap0 % 32 == 0;
+    for (std::size_t i = 0; i < n; ++i) ok = ok && v(0, i) == 3.0 * double(i) && v(1, i) == -3.0 * double(i);
+    std::printf("resize %s samples %zu capacity %zu\n", ok ? "ok" : "BAD", x.samples(), x.capacity());
+    return ok ? 0 : 1;
+}
+
+} // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) return 2;
+    const std::string mode = argv[1];
+    auto u = [&](int k) { return (std::uint64_t)std::strtoull(argv[k], nullptr, 0); };
+    try {
+        if (mode == "tape") return tape();
+        if (mode == "cfg0g" && argc == 4) return cfg0g(u(2), u(3));
+        if (mode == "host" && argc == 4) return host(u(2), u(3));
+        if (mode == "vec3" && argc == 6) return vec3(u(2), u(3), u(4), u(5));
+        if (mode == "fft" && argc == 4) return fft(u(2), u(3) != 0);
+        if (mode == "float" && argc == 4) return flt(u(2), u(3));
+        if (mode == "resize" && argc == 3) return resize(u(2));
+    } catch (const aether::Error& e) {
+        std::fprintf(stderr, "aether error: %s\n", e.what());
+        return 1;
+    }
+    return 2;
+}
