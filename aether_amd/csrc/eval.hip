@@ -240,52 +240,67 @@ void dispatch_reduce(int mode, int square, const void* x, int ncomp, uint64_t n,
     }
 }
 
-// e = cross(a,b).dot(c) + sqrt(a.dot(a) + 1), two samples per lane per step.
-template<bool VECTOR>
+// e = cross(a,b).dot(c) + sqrt(a.dot(a) + 1) (expr/nodes/Geometric.h:55-71, expr/Reduce.h:172-188:
+// right-nested folds, nothing contracted).
+__device__ __forceinline__ double vec3geo_expr(const double (&A)[3], const double (&B)[3], const double (&C)[3])
+{
+    const double x0 = A[1] * B[2] - A[2] * B[1];
+    const double x1 = A[2] * B[0] - A[0] * B[2];
+    const double x2 = A[0] * B[1] - A[1] * B[0];
+    const double dxc = (x2 * C[2]) + ((x1 * C[1]) + (x0 * C[0]));
+    const double daa = (A[2] * A[2]) + ((A[1] * A[1]) + (A[0] * A[0]));
+    return dxc + __builtin_sqrt(daa + 1.0);
+}
+
+// One block per tile of kV3Tile samples: a full tile issues all 9 x kV3Rounds
+// 16-byte loads of a lane before any arithmetic (each lane holds 2 samples per
+// round; consecutive lanes take consecutive vectors of a plane), then writes
+// its kV3Rounds 16-byte results. The last, partial tile goes element-wise.
+constexpr int kV3Rounds = 2;
+constexpr uint32_t kV3Tile = kThreads * 2 * kV3Rounds;
+
+template<bool VECTOR, bool NT>
 __global__ __launch_bounds__(kThreads) void vec3geo_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                                            const double* __restrict__ c, double* __restrict__ e,
                                                            uint32_t n, uint64_t pitch)
 {
-    const uint32_t pairs = (n + 1) / 2;
-    for (uint32_t p = blockIdx.x * kThreads + threadIdx.x; p < pairs; p += gridDim.x * kThreads) {
-        const uint32_t i0 = 2 * p;
-        double A[3][2], B[3][2], Cv[3][2];
-        const bool full = VECTOR && i0 + 1 < n;
-        if (full) {
+    const uint64_t base = (uint64_t)blockIdx.x * kV3Tile;
+    if (VECTOR && base + kV3Tile <= n) {
+        double A[kV3Rounds][3][2], B[kV3Rounds][3][2], C[kV3Rounds][3][2];
+#pragma unroll
+        for (int r = 0; r < kV3Rounds; ++r) {
+            const uint64_t i = base + 2 * (threadIdx.x + (uint64_t)r * kThreads);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                vload(a + (uint64_t)k * pitch + i0, A[k]);
-                vload(b + (uint64_t)k * pitch + i0, B[k]);
-                vload(c + (uint64_t)k * pitch + i0, Cv[k]);
+                vload<NT>(a + (uint64_t)k * pitch + i, A[r][k]);
+                vload<NT>(b + (uint64_t)k * pitch + i, B[r][k]);
+                vload<NT>(c + (uint64_t)k * pitch + i, C[r][k]);
             }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const uint32_t i = i0 + j;
-                    const bool ok = i < n;
-                    A[k][j] = ok ? a[(uint64_t)k * pitch + i] : 0.0;
-                    B[k][j] = ok ? b[(uint64_t)k * pitch + i] : 0.0;
-                    Cv[k][j] = ok ? c[(uint64_t)k * pitch + i] : 0.0;
-                }
         }
-        double out[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const double x0 = A[1][j] * B[2][j] - A[2][j] * B[1][j];
-            const double x1 = A[2][j] * B[0][j] - A[0][j] * B[2][j];
-            const double x2 = A[0][j] * B[1][j] - A[1][j] * B[0][j];
-            const double dxc = (x2 * Cv[2][j]) + ((x1 * Cv[1][j]) + (x0 * Cv[0][j]));
-            const double daa = (A[2][j] * A[2][j]) + ((A[1][j] * A[1][j]) + (A[0][j] * A[0][j]));
-            out[j] = dxc + __builtin_sqrt(daa + 1.0);
+        for (int r = 0; r < kV3Rounds; ++r) {
+            double out[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const double As[3] = {A[r][0][j], A[r][1][j], A[r][2][j]};
+                const double Bs[3] = {B[r][0][j], B[r][1][j], B[r][2][j]};
+                const double Cs[3] = {C[r][0][j], C[r][1][j], C[r][2][j]};
+                out[j] = vec3geo_expr(As, Bs, Cs);
+            }
+            vstore<NT>(e + base + 2 * (threadIdx.x + (uint64_t)r * kThreads), out);
         }
-        if (full) {
-            vstore(e + i0, out);
-        } else {
-            if (i0 < n) e[i0] = out[0];
-            if (i0 + 1 < n) e[i0 + 1] = out[1];
+        return;
+    }
+    const uint64_t end = base + kV3Tile < n ? base + kV3Tile : n;
+    for (uint64_t i = base + threadIdx.x; i < end; i += kThreads) {
+        double As[3], Bs[3], Cs[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            As[k] = a[(uint64_t)k * pitch + i];
+            Bs[k] = b[(uint64_t)k * pitch + i];
+            Cs[k] = c[(uint64_t)k * pitch + i];
         }
+        e[i] = vec3geo_expr(As, Bs, Cs);
     }
 }
 
@@ -336,14 +351,15 @@ extern "C" int aeth_eval_vec3geo(const void* a, const void* b, const void* c, vo
         auto A = static_cast<const double*>(a), B = static_cast<const double*>(b), C = static_cast<const double*>(c);
         auto E = static_cast<double*>(e);
         const bool vec = aligned16(a) && aligned16(b) && aligned16(c) && aligned16(e) && pitch % 2 == 0;
-        const uint32_t pairs = (uint32_t)((n + 1) / 2);
-        const int grid = (int)std::min<uint32_t>((pairs + kThreads - 1) / kThreads, (uint32_t)device_cus() * 32u);
+        const bool nt = streaming(80ull * n);
+        const dim3 grid((unsigned)((n + kV3Tile - 1) / kV3Tile));
         hipStream_t st = as_stream(stream);
-        if (vec)
-            hipLaunchKernelGGL((vec3geo_kernel<true>), dim3(grid), dim3(kThreads), 0, st, A, B, C, E, (uint32_t)n,
-                               pitch);
+        if (vec && nt)
+            hipLaunchKernelGGL((vec3geo_kernel<true, true>), grid, dim3(kThreads), 0, st, A, B, C, E, (uint32_t)n, pitch);
+        else if (vec)
+            hipLaunchKernelGGL((vec3geo_kernel<true, false>), grid, dim3(kThreads), 0, st, A, B, C, E, (uint32_t)n, pitch);
         else
-            hipLaunchKernelGGL((vec3geo_kernel<false>), dim3(grid), dim3(kThreads), 0, st, A, B, C, E, (uint32_t)n,
+            hipLaunchKernelGGL((vec3geo_kernel<false, false>), grid, dim3(kThreads), 0, st, A, B, C, E, (uint32_t)n,
                                pitch);
         check(hipGetLastError(), "vec3geo_kernel launch");
     });

@@ -450,25 +450,159 @@ extern "C" int aeth_fft_batch_destroy(void* plan)
 // ---------------------------------------------------------------------------
 // Plans for the memory seam's fft_plan/fft_exec/fft_destroy
 // (dsc/src/seam.h:41-50, dsc/src/cufft_plan.h:125-222): c2c over any set of
-// axes of a dense row-major array, one strided-line pass per axis.
+// axes of a dense row-major array, one strided-line pass per axis. Real
+// transforms (cufft_plan.h:9-12: `shape` is the real side's, the last axis
+// is halved to n/2 + 1 on the complex side) run the halved axis as a complex
+// transform of the whole line in a plan-owned work array: r2c widens the
+// reals, transforms, keeps bins 0..n/2, then runs the other axes in place on
+// the output; c2r runs the other axes first (into a plan-owned copy, the
+// input is left unchanged), extends each line by Hermitian symmetry
+// (X[n-k] = conj X[k]), transforms and keeps the real parts -- which ignores
+// the imaginary parts of the DC and Nyquist bins, as a real inverse does.
 namespace aeth {
 namespace fft {
 
+namespace {
+
+constexpr int kAuxThreads = 256;
+
+inline dim3 aux_grid(uint64_t items)
+{
+    const uint64_t b = (items + kAuxThreads - 1) / kAuxThreads;
+    return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b, 65536)));
+}
+
+// reals -> complex (x, 0)
+template<class T>
+__global__ __launch_bounds__(kAuxThreads) void widen_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                            uint64_t count)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * kAuxThreads) {
+        out[2 * i] = in[i];
+        out[2 * i + 1] = T(0);
+    }
+}
+
+// bins 0..n/2 of every line along the halved axis: full (outer, n, inner) -> half (outer, n/2+1, inner)
+template<class T>
+__global__ __launch_bounds__(kAuxThreads) void keep_half_kernel(const T* __restrict__ full, T* __restrict__ half,
+                                                                uint64_t outer, uint64_t n, uint64_t inner)
+{
+    const uint64_t h = n / 2 + 1, count = outer * h * inner;
+    for (uint64_t t = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; t < count;
+         t += (uint64_t)gridDim.x * kAuxThreads) {
+        const uint64_t j = t % inner, r = t / inner, k = r % h, o = r / h;
+        const uint64_t src = (o * n + k) * inner + j;
+        half[2 * t] = full[2 * src];
+        half[2 * t + 1] = full[2 * src + 1];
+    }
+}
+
+// Hermitian extension: half (outer, n/2+1, inner) -> full (outer, n, inner), X[k] = conj X[n-k] for k > n/2
+template<class T>
+__global__ __launch_bounds__(kAuxThreads) void hermitian_kernel(const T* __restrict__ half, T* __restrict__ full,
+                                                                uint64_t outer, uint64_t n, uint64_t inner)
+{
+    const uint64_t h = n / 2 + 1, count = outer * n * inner;
+    for (uint64_t t = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; t < count;
+         t += (uint64_t)gridDim.x * kAuxThreads) {
+        const uint64_t j = t % inner, r = t / inner, k = r % n, o = r / n;
+        const bool mirror = k >= h;
+        const uint64_t src = (o * h + (mirror ? n - k : k)) * inner + j;
+        full[2 * t] = half[2 * src];
+        full[2 * t + 1] = mirror ? -half[2 * src + 1] : half[2 * src + 1];
+    }
+}
+
+template<class T>
+__global__ __launch_bounds__(kAuxThreads) void real_part_kernel(const T* __restrict__ full, T* __restrict__ out,
+                                                                uint64_t count)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * kAuxThreads)
+        out[i] = full[2 * i];
+}
+
+} // namespace
+
 struct SeamPlan {
     int dev = 0;
+    int kind = AETHER_FFT_C2C;
     int forward = 1;
-    std::vector<BatchPlan*> passes;
+    bool dbl = true;
+    std::vector<BatchPlan*> passes; // complex passes on the complex-side shape
     std::vector<double> scales;
+    BatchPlan* line = nullptr;       // real transforms: the halved axis over the full complex shape
+    double line_scale = 1.0;
+    void* full = nullptr;            // real transforms: (outer, n, inner) complex work
+    void* half = nullptr;            // c2r with other axes: (outer, n/2+1, inner) complex work
+    uint64_t outer = 1, n = 1, inner = 1;
     std::mutex mu; // cufft_plan.h:187 -- one execution at a time per plan
+    static void free_pass(BatchPlan* p)
+    {
+        if (!p) return;
+        (void)hipFree(p->descs);
+        (void)hipFree(p->sigs);
+        delete p;
+    }
     ~SeamPlan()
     {
-        for (auto* p : passes) {
-            (void)hipFree(p->descs);
-            (void)hipFree(p->sigs);
-            delete p;
-        }
+        for (auto* p : passes) free_pass(p);
+        free_pass(line);
+        (void)hipFree(full);
+        (void)hipFree(half);
     }
 };
+
+namespace {
+
+// One strided-line pass along `ax` of a dense row-major complex `shape`.
+BatchPlan* make_pass(const std::vector<int64_t>& shape, int ax, int dev, bool dbl)
+{
+    const int ndim = (int)shape.size();
+    uint64_t total = 1;
+    for (int d = 0; d < ndim; ++d) total *= (uint64_t)shape[d];
+    const uint64_t n = (uint64_t)shape[ax];
+    const uint32_t l2 = log2_exact(n);
+    if (l2 == 0xFFFFFFFFu || l2 > (uint32_t)kMaxLog2)
+        fail("fft_plan", "axis lengths must be powers of two in [1, 8192]", 0, 2);
+    uint64_t inner = 1;
+    for (int d = ax + 1; d < ndim; ++d) inner *= (uint64_t)shape[d];
+    if (inner > 0xFFFFFFFFull) fail("fft_plan", "inner extent too large");
+    const uint64_t lines = n ? total / n : 0;
+    std::vector<uint32_t> lg(lines, l2);
+    std::vector<uint64_t> off(lines);
+    for (uint64_t li = 0; li < lines; ++li) {
+        const uint64_t o = li / inner, in = li % inner;
+        off[li] = o * n * inner + in;
+    }
+    std::vector<Desc> descs;
+    std::vector<Sig> sigs;
+    build_tables(lg.data(), off.data(), off.data(), lines, descs, sigs, (uint32_t)inner);
+    auto bp = std::make_unique<BatchPlan>();
+    bp->prec = dbl ? 0 : 1;
+    bp->dev = dev;
+    bp->ndesc = (uint32_t)descs.size();
+    bp->nsig = sigs.size();
+    if (!descs.empty()) {
+        check(hipMalloc(&bp->descs, sizeof(Desc) * descs.size()), "hipMalloc(descriptors)");
+        if (hipMalloc(&bp->sigs, sizeof(Sig) * sigs.size()) != hipSuccess) {
+            (void)hipFree(bp->descs);
+            fail("fft_plan", "hipMalloc(signals) failed", sizeof(Sig) * sigs.size());
+        }
+        const bool ok =
+            hipMemcpy(bp->descs, descs.data(), sizeof(Desc) * descs.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(bp->sigs, sigs.data(), sizeof(Sig) * sigs.size(), hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) {
+            SeamPlan::free_pass(bp.release());
+            fail("fft_plan", "hipMemcpy(descriptor tables) failed");
+        }
+    }
+    return bp.release();
+}
+
+} // namespace
 
 int seam_plan(int dev, int kind, int is_double, int norm, int forward, int ndim, const int64_t* shape, int naxes,
               const int32_t* axes, void* stream, void** plan)
@@ -476,7 +610,7 @@ int seam_plan(int dev, int kind, int is_double, int norm, int forward, int ndim,
     return guarded([&] {
         if (!plan) fail("fft_plan", "plan is NULL");
         *plan = nullptr;
-        if (kind != 0) fail("fft_plan", "only AETHER_FFT_C2C is implemented on this device", 0, 2);
+        if (kind < AETHER_FFT_C2C || kind > AETHER_FFT_C2R) fail("fft_plan", "unknown transform kind");
         if (ndim < 1 || naxes < 1) fail("fft_plan", "no axes to transform");
         std::vector<bool> seen(ndim, false);
         for (int a = 0; a < naxes; ++a) {
@@ -484,6 +618,8 @@ int seam_plan(int dev, int kind, int is_double, int norm, int forward, int ndim,
             if (seen[axes[a]]) fail("fft_plan", "repeated axis");
             seen[axes[a]] = true;
         }
+        for (int d = 0; d < ndim; ++d)
+            if (shape[d] < 0) fail("fft_plan", "negative extent");
         if (norm < 0 || norm > 2) fail("fft_plan", "unknown norm");
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         check(hipStreamIsCapturing(as_stream(stream), &cs), "hipStreamIsCapturing");
@@ -491,58 +627,94 @@ int seam_plan(int dev, int kind, int is_double, int norm, int forward, int ndim,
         int prev = 0;
         check(hipGetDevice(&prev), "hipGetDevice");
         if (prev != dev) check(hipSetDevice(dev), "hipSetDevice");
+        struct Restore {
+            int prev, dev;
+            ~Restore()
+            {
+                if (prev != dev) (void)hipSetDevice(prev);
+            }
+        } restore{prev, dev};
         auto sp = std::make_unique<SeamPlan>();
         sp->dev = dev;
-        sp->forward = forward;
-        uint64_t total = 1, points = 1;
-        for (int d = 0; d < ndim; ++d) total *= (uint64_t)shape[d];
+        sp->kind = kind;
+        // cufft_plan.h:127 -- r2c is forward and c2r inverse whatever `forward` says
+        sp->forward = kind == AETHER_FFT_R2C ? 1 : kind == AETHER_FFT_C2R ? 0 : forward;
+        sp->dbl = is_double != 0;
+        const std::vector<int64_t> real_shape(shape, shape + ndim);
+        uint64_t points = 1;
         for (int a = 0; a < naxes; ++a) points *= (uint64_t)shape[axes[a]];
-        for (int a = 0; a < naxes; ++a) {
-            const int ax = axes[a];
-            const uint64_t n = (uint64_t)shape[ax];
-            const uint32_t l2 = log2_exact(n);
-            if (l2 == 0xFFFFFFFFu || l2 > (uint32_t)kMaxLog2)
-                fail("fft_plan", "axis lengths must be powers of two in [1, 8192]", 0, 2);
-            uint64_t inner = 1;
-            for (int d = ax + 1; d < ndim; ++d) inner *= (uint64_t)shape[d];
-            if (inner > 0xFFFFFFFFull) fail("fft_plan", "inner extent too large");
-            const uint64_t lines = n ? total / n : 0;
-            std::vector<uint32_t> lg(lines, l2);
-            std::vector<uint64_t> off(lines);
-            for (uint64_t li = 0; li < lines; ++li) {
-                const uint64_t o = li / inner, in = li % inner;
-                off[li] = o * n * inner + in;
+        // fft/Fft.h:73-76 scale of the whole transform, applied by the last complex step
+        double sc = 1.0;
+        if (norm == 2) sc = 1.0 / std::sqrt((double)points);
+        else if ((sp->forward != 0) == (norm == 1)) sc = 1.0 / (double)points;
+        if (kind == AETHER_FFT_C2C) {
+            for (int a = 0; a < naxes; ++a) {
+                sp->passes.push_back(make_pass(real_shape, axes[a], dev, sp->dbl));
+                sp->scales.push_back(a + 1 == naxes ? sc : 1.0);
             }
-            std::vector<Desc> descs;
-            std::vector<Sig> sigs;
-            build_tables(lg.data(), off.data(), off.data(), lines, descs, sigs, (uint32_t)inner);
-            auto* bp = new BatchPlan;
-            bp->prec = is_double ? 0 : 1;
-            bp->dev = dev;
-            bp->ndesc = (uint32_t)descs.size();
-            bp->nsig = sigs.size();
-            sp->passes.push_back(bp);
-            if (!descs.empty()) {
-                check(hipMalloc(&bp->descs, sizeof(Desc) * descs.size()), "hipMalloc(descriptors)");
-                check(hipMalloc(&bp->sigs, sizeof(Sig) * sigs.size()), "hipMalloc(signals)");
-                check(hipMemcpy(bp->descs, descs.data(), sizeof(Desc) * descs.size(), hipMemcpyHostToDevice),
-                      "hipMemcpy(descriptors)");
-                check(hipMemcpy(bp->sigs, sigs.data(), sizeof(Sig) * sigs.size(), hipMemcpyHostToDevice),
-                      "hipMemcpy(signals)");
+        } else {
+            const int ax = axes[naxes - 1];
+            std::vector<int64_t> half = real_shape;
+            half[ax] = real_shape[ax] / 2 + 1;
+            sp->n = (uint64_t)real_shape[ax];
+            for (int d = 0; d < ax; ++d) sp->outer *= (uint64_t)real_shape[d];
+            for (int d = ax + 1; d < ndim; ++d) sp->inner *= (uint64_t)real_shape[d];
+            // r2c: halved axis first, then the others (the last carries the scale);
+            // c2r: the others first, then the halved axis with the scale
+            sp->line = make_pass(real_shape, ax, dev, sp->dbl);
+            for (int a = 0; a + 1 < naxes; ++a) {
+                sp->passes.push_back(make_pass(half, axes[a], dev, sp->dbl));
+                sp->scales.push_back(kind == AETHER_FFT_R2C && a + 2 == naxes ? sc : 1.0);
             }
-            // fft/Fft.h:73-76 scale of the whole transform, applied by the last pass
-            double sc = 1.0;
-            if (a + 1 == naxes) {
-                if (norm == 2) sc = 1.0 / std::sqrt((double)points);
-                else if ((forward != 0) == (norm == 1)) sc = 1.0 / (double)points;
-            }
-            sp->scales.push_back(sc);
+            sp->line_scale = (kind == AETHER_FFT_C2R || naxes == 1) ? sc : 1.0;
+            const size_t elem = sp->dbl ? sizeof(double) : sizeof(float);
+            const uint64_t total_full = sp->outer * sp->n * sp->inner;
+            const uint64_t total_half = sp->outer * (sp->n / 2 + 1) * sp->inner;
+            if (total_full) check(hipMalloc(&sp->full, 2 * elem * total_full), "hipMalloc(fft work)");
+            if (kind == AETHER_FFT_C2R && naxes > 1 && total_half)
+                check(hipMalloc(&sp->half, 2 * elem * total_half), "hipMalloc(fft work)");
         }
         (void)tables(as_stream(stream));
-        if (prev != dev) check(hipSetDevice(prev), "hipSetDevice");
+        check(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
         *plan = sp.release();
     });
 }
+
+namespace {
+
+template<class T>
+void run_real(SeamPlan& sp, const void* in, void* out, hipStream_t st)
+{
+    const uint64_t h = sp.n / 2 + 1;
+    const uint64_t total_full = sp.outer * sp.n * sp.inner, total_half = sp.outer * h * sp.inner;
+    if (total_full == 0) return;
+    T* full = static_cast<T*>(sp.full);
+    if (sp.kind == AETHER_FFT_R2C) {
+        hipLaunchKernelGGL(widen_kernel<T>, aux_grid(total_full), dim3(kAuxThreads), 0, st,
+                           static_cast<const T*>(in), full, total_full);
+        check(hipGetLastError(), "widen_kernel launch");
+        launch(*sp.line, full, full, 1, 3, st, sp.line_scale);
+        hipLaunchKernelGGL(keep_half_kernel<T>, aux_grid(total_half), dim3(kAuxThreads), 0, st, full,
+                           static_cast<T*>(out), sp.outer, sp.n, sp.inner);
+        check(hipGetLastError(), "keep_half_kernel launch");
+        for (size_t i = 0; i < sp.passes.size(); ++i) launch(*sp.passes[i], out, out, 1, 3, st, sp.scales[i]);
+        return;
+    }
+    const void* src = in;
+    for (size_t i = 0; i < sp.passes.size(); ++i) {
+        launch(*sp.passes[i], i == 0 ? in : sp.half, sp.half, 0, 3, st, sp.scales[i]);
+        src = sp.half;
+    }
+    hipLaunchKernelGGL(hermitian_kernel<T>, aux_grid(total_full), dim3(kAuxThreads), 0, st,
+                       static_cast<const T*>(src), full, sp.outer, sp.n, sp.inner);
+    check(hipGetLastError(), "hermitian_kernel launch");
+    launch(*sp.line, full, full, 0, 3, st, sp.line_scale);
+    hipLaunchKernelGGL(real_part_kernel<T>, aux_grid(total_full), dim3(kAuxThreads), 0, st, full,
+                       static_cast<T*>(out), total_full);
+    check(hipGetLastError(), "real_part_kernel launch");
+}
+
+} // namespace
 
 int seam_exec(void* plan, const void* in, void* out, void* stream)
 {
@@ -550,8 +722,15 @@ int seam_exec(void* plan, const void* in, void* out, void* stream)
         auto* sp = static_cast<SeamPlan*>(plan);
         if (!sp) fail("fft_exec", "plan is NULL");
         std::lock_guard<std::mutex> lock(sp->mu);
-        for (size_t i = 0; i < sp->passes.size(); ++i)
-            launch(*sp->passes[i], i == 0 ? in : out, out, sp->forward, 3, as_stream(stream), sp->scales[i]);
+        const hipStream_t st = as_stream(stream);
+        if (sp->kind == AETHER_FFT_C2C) {
+            for (size_t i = 0; i < sp->passes.size(); ++i)
+                launch(*sp->passes[i], i == 0 ? in : out, out, sp->forward, 3, st, sp->scales[i]);
+            return;
+        }
+        if (in == out) fail("fft_exec", "in == out is allowed for c2c only");
+        if (sp->dbl) run_real<double>(*sp, in, out, st);
+        else run_real<float>(*sp, in, out, st);
     });
 }
 

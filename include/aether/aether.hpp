@@ -495,6 +495,12 @@ public:
         for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] += (T)e.eval(E::ncomp > 1 ? c : 0, i_);
         return *this;
     }
+    template<class E>
+    SampleRef& operator-=(const E& e)
+    {
+        for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] -= (T)e.eval(E::ncomp > 1 ? c : 0, i_);
+        return *this;
+    }
     std::array<T, N> get() const
     {
         std::array<T, N> r{};
@@ -632,12 +638,9 @@ public:
     void reserve(std::size_t n)
     {
         if (n <= capacity_) return;
-        Array next(samples_);
-        next.release();
-        next.capacity_ = quantize(n);
-        next.allocate();
+        Array next(samples_, quantize(n));
         const int dev = detail::current_device();
-        for (std::size_t c = 0; c < N; ++c) {
+        for (std::size_t c = 0; c < N && samples_ > 0; ++c) {
             detail::check(detail::seam().copy(AETHER_MEM_DEVICE, dev, static_cast<T*>(next.dev_) + c * next.capacity_,
                                               AETHER_MEM_DEVICE, dev, static_cast<T*>(dev_) + c * capacity_,
                                               samples_ * sizeof(T), nullptr),
@@ -653,8 +656,35 @@ public:
         reserve(n);
         samples_ = n;
     }
+    /** Append one sample on the host copy, doubling capacity when full (array/Array.h:155-163);
+     *  upload() afterwards to reach the device copy. */
+    void push_back(const std::array<T, N>& item)
+    {
+        if (samples_ == capacity_) reserve(capacity_ == 0 ? kCapacityAlignment : capacity_ * 2);
+        for (std::size_t c = 0; c < N; ++c) static_cast<T*>(host_)[c * capacity_ + samples_] = item[c];
+        ++samples_;
+    }
+    void clear() { samples_ = 0; }
+    /** The reserved tail [samples(), capacity()) (array/Array.h:196,221). */
+    View<T, N> hostSpare() const
+    {
+        return View<T, N>(static_cast<T*>(host_) + samples_, capacity_ - samples_, capacity_, false);
+    }
+    View<T, N> deviceSpare() const
+    {
+        return View<T, N>(static_cast<T*>(dev_) + samples_, capacity_ - samples_, capacity_, true);
+    }
+    /** Adopt `count` samples written into the spare tail (array/Array.h:238-247). */
+    void commit(std::size_t count)
+    {
+        if (count > capacity_ - samples_)
+            throw Error("Array::commit: count=" + std::to_string(count) + " exceeds spare capacity (" +
+                        std::to_string(capacity_ - samples_) + ")");
+        samples_ += count;
+    }
 
 private:
+    Array(std::size_t n_samples, std::size_t capacity) : samples_(n_samples), capacity_(capacity) { allocate(); }
     static std::size_t quantize(std::size_t n)
     {
         return (n + kCapacityAlignment - 1) / kCapacityAlignment * kCapacityAlignment;
@@ -695,6 +725,8 @@ private:
 
 template<class T, std::size_t N>
 using VecView = View<T, N>;
+using Vec3d = Array<double, 3>; // aether.h:234-260
+using Vec3f = Array<float, 3>;
 using Vec3dView = View<double, 3>;
 
 /** dest = e with fused per-component sums (hi + comp per component) copied to the host. */
@@ -791,49 +823,109 @@ namespace fft {
 
 enum class Norm { Backward = AETHER_FFT_BACKWARD, Forward = AETHER_FFT_FORWARD, Ortho = AETHER_FFT_ORTHO };
 
+namespace detail {
+
+// One transform through the seam's plans. `in_values`/`out_values`: T values on each side. Host buffers are
+// staged through device chunks; the call returns when the result is in `out`.
+template<class T>
+void run(int kind, const T* in, std::size_t in_values, T* out, std::size_t out_values,
+         std::span<const std::size_t> shape, std::span<const std::size_t> axes, bool forward, Norm norm, Stream s)
+{
+    static_assert(std::is_same_v<T, double> || std::is_same_v<T, float>);
+    const auto& t = aether::detail::seam();
+    const int dev = aether::detail::current_device();
+    const std::vector<std::int64_t> sh(shape.begin(), shape.end());
+    const std::vector<std::int32_t> ax(axes.begin(), axes.end());
+    int pdev = -1;
+    const bool host = t.pointer_device(in, &pdev) != 0;
+    struct Staging {
+        const AetherMemoryV1& t;
+        void* chunk[2] = {nullptr, nullptr};
+        void* ptr[2] = {nullptr, nullptr};
+        void* plan = nullptr;
+        ~Staging()
+        {
+            if (plan) t.fft_destroy(plan);
+            for (void* c : chunk)
+                if (c) t.chunk_release(c);
+        }
+    } st{t};
+    const void* din = in;
+    void* dout = out;
+    if (host) {
+        const std::size_t bytes[2] = {in_values * sizeof(T), out_values * sizeof(T)};
+        for (int k = 0; k < 2; ++k)
+            aether::detail::check(t.chunk_alloc(AETHER_MEM_DEVICE, dev, bytes[k] ? bytes[k] : sizeof(T), s,
+                                                &st.chunk[k], &st.ptr[k]),
+                                  "fft staging");
+        aether::detail::check(t.copy(AETHER_MEM_DEVICE, dev, st.ptr[0], AETHER_MEM_CPU, dev, in, bytes[0], s),
+                              "fft upload");
+        din = st.ptr[0];
+        dout = st.ptr[1];
+    }
+    aether::detail::check(t.fft_plan(dev, kind, std::is_same_v<T, double> ? 1 : 0, (int)norm, forward ? 1 : 0,
+                                     (int)sh.size(), sh.data(), (int)ax.size(), ax.data(), s, &st.plan),
+                          "fft_plan");
+    aether::detail::check(t.fft_exec(st.plan, din, dout, s), "fft_exec");
+    if (host)
+        aether::detail::check(
+            t.copy(AETHER_MEM_CPU, dev, out, AETHER_MEM_DEVICE, dev, st.ptr[1], out_values * sizeof(T), s),
+            "fft download");
+    aether::detail::check(t.stream_sync(dev, s), "stream_sync");
+}
+
+inline std::size_t count(std::span<const std::size_t> shape)
+{
+    std::size_t total = 1;
+    for (auto v : shape) total *= v;
+    return total;
+}
+
+inline std::size_t half_count(std::span<const std::size_t> shape, std::span<const std::size_t> axes)
+{
+    if (axes.empty() || axes.back() >= shape.size()) throw Error("aether::fft: bad axes");
+    const std::size_t n = shape[axes.back()];
+    return n ? count(shape) / n * (n / 2 + 1) : 0;
+}
+
+} // namespace detail
+
 /**
  * Complex to complex over `axes` of a dense row-major array of `shape`
  * (fft/Fft.h:186-197). `in`/`out` hold prod(shape) interleaved (re, im)
- * pairs, in device memory (or host memory: staged through the device).
+ * pairs, in device memory or in host memory (staged through the device).
  */
 template<class T>
 void c2c(const T* in, T* out, std::span<const std::size_t> shape, std::span<const std::size_t> axes, bool forward,
          Norm norm = Norm::Backward, Stream s = nullptr)
 {
-    static_assert(std::is_same_v<T, double> || std::is_same_v<T, float>);
-    const auto& t = detail::seam();
-    const int dev = detail::current_device();
-    std::vector<std::int64_t> sh(shape.begin(), shape.end());
-    std::vector<std::int32_t> ax(axes.begin(), axes.end());
-    std::size_t total = 1;
-    for (auto v : shape) total *= v;
-    const std::size_t bytes = 2 * total * sizeof(T);
-    int pdev = -1;
-    const bool on_device = t.pointer_device(in, &pdev) == 0;
-    const void* din = in;
-    void* dout = out;
-    void *cin = nullptr, *cout = nullptr, *bin = nullptr, *bout = nullptr;
-    if (!on_device) {
-        detail::check(t.chunk_alloc(AETHER_MEM_DEVICE, dev, bytes, s, &cin, &bin), "chunk_alloc");
-        detail::check(t.chunk_alloc(AETHER_MEM_DEVICE, dev, bytes, s, &cout, &bout), "chunk_alloc");
-        detail::check(t.copy(AETHER_MEM_DEVICE, dev, bin, AETHER_MEM_CPU, dev, in, bytes, s), "copy");
-        din = bin;
-        dout = bout;
-    }
-    void* plan = nullptr;
-    const int rc = t.fft_plan(dev, AETHER_FFT_C2C, std::is_same_v<T, double> ? 1 : 0, (int)norm, forward ? 1 : 0,
-                              (int)sh.size(), sh.data(), (int)ax.size(), ax.data(), s, &plan);
-    if (rc == 0) {
-        detail::check(t.fft_exec(plan, din, dout, s), "fft_exec");
-        t.fft_destroy(plan);
-    }
-    if (!on_device) {
-        if (rc == 0) detail::check(t.copy(AETHER_MEM_CPU, dev, out, AETHER_MEM_DEVICE, dev, bout, bytes, s), "copy");
-        t.chunk_release(cin);
-        t.chunk_release(cout);
-    }
-    detail::check(rc, "fft_plan");
-    detail::check(t.stream_sync(dev, s), "stream_sync");
+    const std::size_t v = 2 * detail::count(shape);
+    detail::run<T>(AETHER_FFT_C2C, in, v, out, v, shape, axes, forward, norm, s);
+}
+
+/**
+ * Real to complex over `axes` (fft/Fft.h:198-213): `shape` is the real
+ * input's; `out` holds that shape with the last axis of `axes` cut to
+ * n/2 + 1, as (re, im) pairs.
+ */
+template<class T>
+void r2c(const T* in, T* out, std::span<const std::size_t> shape, std::span<const std::size_t> axes,
+         Norm norm = Norm::Backward, Stream s = nullptr)
+{
+    detail::run<T>(AETHER_FFT_R2C, in, detail::count(shape), out, 2 * detail::half_count(shape, axes), shape, axes,
+                   true, norm, s);
+}
+
+/**
+ * Complex to real over `axes` (fft/Fft.h:215-236): `shape` is the real
+ * output's; `in` holds the half spectrum and is left unchanged.
+ */
+template<class T>
+void c2r(const T* in, T* out, std::span<const std::size_t> shape, std::span<const std::size_t> axes,
+         Norm norm = Norm::Backward, Stream s = nullptr)
+{
+    detail::run<T>(AETHER_FFT_C2R, in, 2 * detail::half_count(shape, axes), out, detail::count(shape), shape, axes,
+                   false, norm, s);
 }
 
 } // namespace fft

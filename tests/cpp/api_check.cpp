@@ -12,6 +12,9 @@
 //   vec3   <n> <sA> <sB> <sC>  : raw-seed uniform inputs, e = (a x b).c + sqrt(a.a + 1)
 //                                on the device; writes e (n)
 //   fft    <n> <fwd>           : reads n complex (re, im) doubles on stdin, writes c2c
+//   rfft   <rows> <n>          : reads rows x n doubles on stdin; r2c over the rows' axis,
+//                                writes the half spectrum, then c2r back, writes the reals
+//   append <n>                 : push_back / commit / clear on the host (text)
 //   float  <n> <seed>          : cfg0 in float with float-float sums; writes d, sums
 //   resize <n>                 : upload/reserve/resize/download round trip (text)
 #include <aether/aether.hpp>
@@ -167,6 +170,49 @@ int fft(std::size_t n, bool forward)
     return 0;
 }
 
+int rfft(std::size_t rows, std::size_t n)
+{
+    std::vector<double> in(rows * n), spec(rows * (n / 2 + 1) * 2), back(rows * n);
+    if (std::fread(in.data(), sizeof(double), in.size(), stdin) != in.size()) return 3;
+    const std::size_t shape[2] = {rows, n}, axes[1] = {1};
+    aether::fft::r2c<double>(in.data(), spec.data(), shape, axes);
+    aether::fft::c2r<double>(spec.data(), back.data(), shape, axes);
+    std::fwrite(spec.data(), sizeof(double), spec.size(), stdout);
+    std::fwrite(back.data(), sizeof(double), back.size(), stdout);
+    return 0;
+}
+
+int append(std::size_t n)
+{
+    aether::Vec3d x(0);
+    bool ok = x.samples() == 0 && x.capacity() == 0;
+    for (std::size_t i = 0; i < n; ++i) x.push_back({double(i), 2.0 * double(i), -double(i)});
+    ok = ok && x.samples() == n && x.capacity() >= n && x.capacity() % 32 == 0;
+    auto v = x.hostView();
+    for (std::size_t i = 0; i < n; ++i) ok = ok && v(0, i) == double(i) && v(1, i) == 2.0 * double(i) && v(2, i) == -double(i);
+    // the host spelling on a SampleRef: += and -=
+    auto r = v[SampleIndex::make(n / 2)];
+    r += v;         // x -> 2x
+    r -= 0.25 * v;  // reads the updated sample: 2x - x/2
+    const auto g = r.get();
+    ok = ok && g[0] == 1.5 * double(n / 2) && g[1] == 3.0 * double(n / 2);
+    const std::size_t spare = x.capacity() - x.samples();
+    auto sp = x.hostSpare();
+    if (spare > 0) sp(0, 0) = 42.0;
+    x.commit(spare > 0 ? 1 : 0);
+    if (spare > 0) ok = ok && x.hostView()(0, n) == 42.0;
+    bool threw = false;
+    try {
+        x.commit(x.capacity() - x.samples() + 1);
+    } catch (const aether::Error&) {
+        threw = true;
+    }
+    x.clear();
+    ok = ok && threw && x.samples() == 0 && x.capacity() >= n;
+    std::printf("append %s samples %zu capacity %zu\n", ok ? "ok" : "BAD", x.samples(), x.capacity());
+    return ok ? 0 : 1;
+}
+
 int flt(std::size_t n, std::uint64_t seed)
 {
     Array<float, 3> a(n), b(n), c(n), d(n);
@@ -213,6 +259,8 @@ int main(int argc, char** argv)
         if (mode == "vec3" && argc == 6) return vec3(u(2), u(3), u(4), u(5));
         if (mode == "fft" && argc == 4) return fft(u(2), u(3) != 0);
         if (mode == "float" && argc == 4) return flt(u(2), u(3));
+        if (mode == "rfft" && argc == 4) return rfft(u(2), u(3));
+        if (mode == "append" && argc == 3) return append(u(2));
         if (mode == "resize" && argc == 3) return resize(u(2));
     } catch (const aether::Error& e) {
         std::fprintf(stderr, "aether error: %s\n", e.what());

@@ -150,3 +150,21 @@ def test_float_assignment_with_float_float_sums(dev, orc):
 @pytest.mark.gpu
 def test_upload_resize_download(dev):
     assert run("resize", 5000).decode().startswith("resize ok")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows,n", [(3, 64), (1, 1024), (5, 2)])
+def test_rfft_round_trip(dev, rows, n):
+    x = np.random.default_rng(n).standard_normal((rows, n))
+    out = run("rfft", rows, n, stdin=x.tobytes())
+    h = n // 2 + 1
+    spec = np.frombuffer(out, dtype=np.complex128, count=rows * h).reshape(rows, h)
+    back = np.frombuffer(out[spec.nbytes:], dtype=np.float64).reshape(rows, n)
+    c = (8 + 4 * np.log2(n + 1)) * 2.0**-52 * np.linalg.norm(x)
+    assert np.max(np.abs(spec - np.fft.rfft(x, axis=1))) <= c * np.sqrt(n)
+    assert np.max(np.abs(back - x)) <= 2 * c
+
+
+@pytest.mark.gpu
+def test_append_commit_clear(dev):
+    assert run("append", 100).decode().startswith("append ok")

@@ -102,7 +102,7 @@ def test_cfg0_full_size_properties(dev):
     assert torch.allclose(d.data[0, i].cpu(), ref, rtol=0, atol=1e-15)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 999, 2000, 65537])
+@pytest.mark.parametrize("n", [1, 2, 3, 999, 1024, 1025, 2000, 65537, 8_000_001])
 def test_vec3geo_parity(dev, orc, n):
     a, b, c = filled(n, torch.float64)
     e = torch.empty(n, dtype=torch.float64, device="cuda")
@@ -111,6 +111,18 @@ def test_vec3geo_parity(dev, orc, n):
     A, B, C = (x.data.cpu().numpy().reshape(-1) for x in (a, b, c))
     exp = orc.eval_vec3geo(A, B, C, n, a.capacity)
     assert np.array_equal(e.cpu().numpy().view(np.uint64), exp.view(np.uint64))
+
+
+def test_vec3geo_unaligned_output(dev, orc):
+    # an output one element off 16-byte alignment takes the element-wise path
+    n = 5001
+    a, b, c = filled(n, torch.float64)
+    buf = torch.empty(n + 1, dtype=torch.float64, device="cuda")
+    ae.eval_vec3geo(a, b, c, buf[1:])
+    torch.cuda.synchronize()
+    A, B, C = (x.data.cpu().numpy().reshape(-1) for x in (a, b, c))
+    exp = orc.eval_vec3geo(A, B, C, n, a.capacity)
+    assert np.array_equal(buf[1:].cpu().numpy().view(np.uint64), exp.view(np.uint64))
 
 
 @pytest.mark.parametrize("dt", [torch.float64, torch.float32])

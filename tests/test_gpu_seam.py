@@ -168,8 +168,56 @@ def test_fft_plan_unsupported(seam):
     ax = (C.c_int32 * 1)(0)
     plan = _vp()
     assert seam.fft_plan(0, 0, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 2
-    shp = (C.c_int64 * 1)(16)
-    assert seam.fft_plan(0, 1, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 2  # R2C
+    assert seam.fft_plan(0, 1, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 2  # R2C of length 12
+    assert seam.fft_plan(0, 3, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 1  # unknown kind
     ax2 = (C.c_int32 * 2)(0, 0)
     shp2 = (C.c_int64 * 2)(4, 4)
     assert seam.fft_plan(0, 0, 1, 0, 1, 2, shp2, 2, ax2, s, C.byref(plan)) == 1  # repeated axis
+
+
+@pytest.mark.parametrize("shape,axes", [((8, 64), (1,)), ((64, 8), (0,)), ((16, 32), (0, 1)), ((16, 32), (1, 0)),
+                                        ((4, 8, 16), (1,)), ((4, 8, 16), (0, 2)), ((2048,), (0,)), ((3, 2), (1,)),
+                                        ((5, 1), (1,))])
+@pytest.mark.parametrize("norm", [0, 1, 2])
+@pytest.mark.parametrize("dbl", [1, 0])
+def test_fft_plan_real(seam, shape, axes, norm, dbl):
+    """R2C then C2R vs numpy's rfftn / irfftn (cufft_plan.h:9-12 layout; fft/Fft.h:198-236)."""
+    rng = np.random.default_rng(sum(shape) + len(axes))
+    x = rng.standard_normal(shape)
+    half = list(shape)
+    half[axes[-1]] = shape[axes[-1]] // 2 + 1
+    ft = np.float64 if dbl else np.float32
+    ct = np.complex128 if dbl else np.complex64
+    xin = torch.from_numpy(x.astype(ft)).to("cuda")
+    spec = torch.zeros(tuple(half), dtype=torch.complex128 if dbl else torch.complex64, device="cuda")
+    back = torch.zeros_like(xin)
+    s = torch.cuda.current_stream().cuda_stream
+    shp = (C.c_int64 * len(shape))(*shape)
+    ax = (C.c_int32 * len(axes))(*axes)
+    npn = {0: "backward", 1: "forward", 2: "ortho"}[norm]
+    p1, p2 = _vp(), _vp()
+    assert seam.fft_plan(0, 1, dbl, norm, 0, len(shape), shp, len(axes), ax, s, C.byref(p1)) == 0, seam.last_error()
+    assert seam.fft_plan(0, 2, dbl, norm, 1, len(shape), shp, len(axes), ax, s, C.byref(p2)) == 0, seam.last_error()
+    assert seam.fft_exec(p1, xin.data_ptr(), spec.data_ptr(), s) == 0, seam.last_error()
+    spec_in = spec.clone()
+    assert seam.fft_exec(p2, spec.data_ptr(), back.data_ptr(), s) == 0, seam.last_error()
+    torch.cuda.synchronize()
+    seam.fft_destroy(p1)
+    seam.fft_destroy(p2)
+    eps = 2.0**-52 if dbl else 2.0**-23
+    points = int(np.prod([shape[a] for a in axes]))
+    xr = x.astype(ft).astype(np.float64)
+    # a stable FFT: ||dy||_2 <= c log2(N) eps ||y||_2 with ||y||_2 = sqrt(N) s ||x||_2 (Parseval); the forward
+    # and inverse scales multiply to 1/N under every norm, so the inverse's bound is c log2(N) eps ||x||_2
+    s_fwd = {0: 1.0, 1: 1.0 / points, 2: points**-0.5}[norm]
+    c = (8 + 4 * np.log2(points + 1)) * eps * np.linalg.norm(xr)
+    ref = np.fft.rfftn(xr, axes=axes, norm=npn)
+    got = spec.cpu().numpy().astype(np.complex128)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) <= c * np.sqrt(points) * s_fwd
+    assert torch.equal(spec, spec_in)  # c2r leaves its input unchanged
+    sizes = [shape[a] for a in axes]
+    ref_back = np.fft.irfftn(got, s=sizes, axes=axes, norm=npn)
+    out = back.cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(out - ref_back)) <= c
+    assert np.max(np.abs(out - xr)) <= 2 * c
