@@ -4,8 +4,8 @@
 // BITS32), several elements per lane so each lane issues one 16-byte store
 // (global_store_dwordx4) per step. Streams and conversions are aether's
 // (random/detail/Backend.h:123-236), element j of a fill is the draw at
-// global sample j, counter 0 (the BASELINE "key = seed, counter = sample
-// index" streams).
+// global sample j, counter 0 ("key = seed, counter = sample index"
+// streams).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

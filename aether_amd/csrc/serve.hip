@@ -1,8 +1,8 @@
 // serve.hip -- continuous-batching job server over the hot path (one GPU).
 //
-// The reference has no serving loop; this is the serving
-// extension. Jobs (evaluate / sample / FFT) arrive from any thread. One
-// scheduler thread keeps up to `streams` batches in flight: whenever a stream
+// The reference has no serving loop; this is the serving extension. Jobs
+// (evaluate / sample / FFT) arrive from any thread. One scheduler thread
+// keeps up to `streams` batches in flight: whenever a stream
 // slot is free it takes the pending jobs in arrival order, admits each one
 // whose buffers fit in the device pool (an HBM arena carved by a first-fit
 // allocator: the budget), and launches the whole batch as a handful of

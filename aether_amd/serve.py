@@ -93,7 +93,7 @@ def make_mix(rate: float, duration: float, seed: int = 5):
 
 
 def job_bytes(j: Job) -> float:
-    """Algorithmic bytes of a job's hot-path work (SURVEY.md 8(d)): eval 3 loads + 1 store per
+    """Algorithmic bytes of a job's hot-path work: eval 3 loads + 1 store per
     (component, sample); sampling: bytes written; FFT: read and write once per point."""
     es = 8 if j.dtype == F64 else 4
     if j.kind == JOB_EVAL:

@@ -216,7 +216,7 @@ int aeth_fft_batch_destroy(void* plan);
 
 /* ------------------------------------------------------------------ serving */
 /* A continuous-batching job server on one GPU (no counterpart in the
- * reference, which has no serving loop; SURVEY.md fact 10). Jobs:
+ * reference, which has no serving loop). Jobs:
  *   EVAL  : a, b, c = Array<T,3>(n) of uniforms on (-1,1) drawn from seeds
  *           seed, seed+1, seed+2 (counter = component); d = a + b*c +
  *           sin(a)*exp(-b*b); result: per-component compensated sums of d
