@@ -14,8 +14,9 @@
 //             r^13, scale by ldexp.
 //   log     : x = 2^k m, m in [sqrt(2)/2, sqrt(2)); s = f/(2+f);
 //             log(1+f) = f - (hfsq - s(hfsq + R(s^2))), R from 2 atanh(s).
-//   float   : through the double routines (FP64 runs at the FP32 rate on
-//             CDNA4 with packed FP32 disabled), one final rounding.
+//   float   : the argument widened to double, the same reductions, shorter
+//             polynomials (Taylor sin to r^11, cos to r^10, exp to r^8, log's
+//             R to s^10), one final rounding.
 // Every multiply-add that is fused is an explicit __builtin_fma; the library
 // is compiled with -ffp-contract=off so nothing else is fused.
 #pragma once
@@ -290,10 +291,133 @@ __device__ __forceinline__ double log(double x)
     return y;
 }
 
-__device__ __forceinline__ float sinf(float x) { return (float)sin((double)x); }
-__device__ __forceinline__ float cosf(float x) { return (float)cos((double)x); }
-__device__ __forceinline__ float expf(float x) { return (float)exp((double)x); }
-__device__ __forceinline__ float logf(float x) { return (float)log((double)x); }
+// ---- float routines. The argument is widened to double (exact), reduced
+// once, and evaluated with polynomials just long enough for float (error
+// below 2^-30 relative before the one rounding to float): a third of the
+// FP64 operations of the double routines.
+
+// |x| < 2^20: r = x - n P1 - n P2 (two FMAs; P1 + P2 is pi/2 to 2^-107, so
+// r is within 2^-80 absolute); above, the Payne-Hanek path, out of line.
+__device__ __forceinline__ int reduce_pio2_f(double x, double* r)
+{
+    const double n = __builtin_rint(x * kTwoOverPi);
+    double rr = __builtin_fma(-n, kPio2_1, x);
+    rr = __builtin_fma(-n, kPio2_2, rr);
+    int q = ((int)(long long)n) & 3;
+    const double ax = __builtin_fabs(x);
+    if (ax >= 0x1p20 && ax <= 0x1.fffffffffffffp+1023) {
+        double lh, ll;
+        int ql = reduce_large(ax, &lh, &ll);
+        if (x < 0) {
+            lh = -lh;
+            ll = -ll;
+            ql = (-ql) & 3;
+        }
+        rr = lh + ll;
+        q = ql;
+    }
+    *r = rr;
+    return q;
+}
+
+// sin r on |r| <= pi/4, Taylor to r^11 (truncation < 2^-37 relative)
+__device__ __forceinline__ double ksinf(double r)
+{
+    const double z = r * r;
+    double p = -0x1.ae64567f544e4p-26;
+    p = __builtin_fma(p, z, 0x1.71de3a556c734p-19);
+    p = __builtin_fma(p, z, -0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, z, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, z, -0x1.5555555555555p-3);
+    return __builtin_fma(r * z, p, r);
+}
+
+// cos r on |r| <= pi/4, Taylor to r^10 (truncation < 2^-33)
+__device__ __forceinline__ double kcosf(double r)
+{
+    const double z = r * r;
+    double p = -0x1.27e4fb7789f5cp-22;
+    p = __builtin_fma(p, z, 0x1.a01a01a01a01ap-16);
+    p = __builtin_fma(p, z, -0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, z, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, z, -0.5);
+    return __builtin_fma(z, p, 1.0);
+}
+
+__device__ __forceinline__ float sinf(float x)
+{
+    double r;
+    const int q = reduce_pio2_f((double)x, &r);
+    const double sn = ksinf(r), cs = kcosf(r);
+    double v = (q & 1) ? cs : sn;
+    v = (q & 2) ? -v : v;
+    float y = (float)v;
+    const float ax = __builtin_fabsf(x);
+    y = ax < 0x1p-12f ? x : y; // sin x rounds to x; keeps -0
+    y = !(ax <= 0x1.fffffep+127f) ? x - x : y;
+    return y;
+}
+
+__device__ __forceinline__ float cosf(float x)
+{
+    double r;
+    const int q = reduce_pio2_f((double)x, &r);
+    const double sn = ksinf(r), cs = kcosf(r);
+    double v = (q & 1) ? sn : cs;
+    v = ((q + 1) & 2) ? -v : v;
+    float y = (float)v;
+    y = !(__builtin_fabsf(x) <= 0x1.fffffep+127f) ? x - x : y;
+    return y;
+}
+
+// e^x: clamp to [-104, 89] (beyond: 0 and +inf after rounding), n = rint(x log2 e),
+// r = x - n ln2 in two parts (|r| <= 0.347), e^r = 1 + r + r^2 p(r), Taylor to r^8
+__device__ __forceinline__ float expf(float x)
+{
+    const double xc = __builtin_fmin(__builtin_fmax((double)x, -104.0), 89.0);
+    const double n = __builtin_rint(xc * kLog2e);
+    double r = __builtin_fma(-n, kLn2Hi, xc);
+    r = __builtin_fma(-n, kLn2Lo, r);
+    double p = 0x1.a01a01a01a01ap-16;
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0.5);
+    const double u = __builtin_fma(r * r, p, r);
+    float y = (float)__builtin_ldexp(1.0 + u, (int)n);
+    y = (x != x) ? x + x : y;
+    return y;
+}
+
+// log x: the double arrangement above with R to s^10 (truncation < 2^-34)
+__device__ __forceinline__ float logf(float x)
+{
+    const uint64_t u = bits_of((double)x);
+    int k = (int)(u >> 52) - 1023;
+    uint64_t mb = (u & 0xFFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL;
+    const bool big = mb > 0x3FF6A09E667F3BCCULL;
+    mb = big ? mb - 0x0010000000000000ULL : mb;
+    k += big ? 1 : 0;
+    const double f = from_bits(mb) - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double R = 0x1.745d1745d1746p-3;
+    R = __builtin_fma(R, z, 0x1.c71c71c71c71cp-3);
+    R = __builtin_fma(R, z, 0x1.2492492492492p-2);
+    R = __builtin_fma(R, z, 0x1.999999999999ap-2);
+    R = __builtin_fma(R, z, 0x1.5555555555555p-1);
+    R = R * z;
+    const double hfsq = 0.5 * f * f;
+    const double dk = (double)k;
+    float y = (float)(dk * kLn2Hi42 - ((hfsq - (s * (hfsq + R) + dk * kLn2Lo42)) - f));
+    y = (x == 0.0f) ? -__builtin_inff() : y;
+    y = (x < 0.0f) ? __builtin_nanf("") : y;
+    y = (x == __builtin_inff()) ? x : y;
+    y = (x != x) ? x + x : y;
+    return y;
+}
 
 } // namespace dm
 } // namespace aeth

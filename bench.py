@@ -238,7 +238,8 @@ def wl_trig(log2n: int = 26) -> dict:
         ref = ae.math(fn, xd)
         rf = ref.float()
         ok = torch.isfinite(yk) & torch.isfinite(rf)
-        spacing = (torch.nextafter(rf, inf) - rf).double().abs().clamp_min(1e-45)
+        arf = rf.abs()  # the ulp at the rounded value, measured away from zero
+        spacing = (torch.nextafter(arf, inf) - arf).double().clamp_min(1e-45)
         err = torch.where(ok, (yk.double() - ref).abs() / spacing, torch.zeros_like(ref))
         errs[fn] = {"max_ulp_vs_fp64": round(float(err.max()), 3), "finite_outputs": int(ok.sum())}
     res["error_vs_fp64"] = errs

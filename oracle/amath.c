@@ -18,7 +18,9 @@
  *   exp     : n = rint(x/ln2), two-part ln2 reduction with tail, Taylor to r^13.
  *   log     : x = 2^k m, m in [sqrt(2)/2, sqrt(2)), s = f/(2+f),
  *             log(1+f) = f - (hfsq - s(hfsq + R(s^2))), R from 2atanh(s).
- *   float   : evaluated through the double routines, rounded once to float.
+ *   float   : the argument widened to double, the same reductions (two-part
+ *             pi/2 below 2^20), Taylor sin to r^11, cos to r^10, exp to r^8,
+ *             log's R to s^10, one rounding to float.
  */
 #include <math.h>
 #include <stdint.h>
@@ -284,10 +286,129 @@ double orc_log(double x)
     return dk * LN2_HI42 - ((hfsq - (s * (hfsq + R) + dk * LN2_LO42)) - f);
 }
 
-float orc_sinf(float x) { return (float)orc_sin((double)x); }
-float orc_cosf(float x) { return (float)orc_cos((double)x); }
-float orc_expf(float x) { return (float)orc_exp((double)x); }
-float orc_logf(float x) { return (float)orc_log((double)x); }
+/* ---- float routines: argument widened to double, reduced once, polynomials
+ * long enough for float, one rounding (same operations as the device). */
+
+/* |x| < 2^20: r = x - n P1 - n P2; above: Payne-Hanek */
+static int reduce_f(double x, double* r)
+{
+    const double n = rint(x * TWO_OVER_PI);
+    double rr = fma(-n, PIO2_1, x);
+    rr = fma(-n, PIO2_2, rr);
+    int q = isfinite(n) ? ((int)(long long)n) & 3 : 0;
+    const double ax = fabs(x);
+    if (ax >= 0x1p20 && ax <= 0x1.fffffffffffffp+1023) {
+        double lh, ll;
+        int ql = reduce_payne_hanek(ax, &lh, &ll);
+        if (x < 0) {
+            lh = -lh;
+            ll = -ll;
+            ql = (-ql) & 3;
+        }
+        rr = lh + ll;
+        q = ql;
+    }
+    *r = rr;
+    return q;
+}
+
+/* Taylor sin to r^11 on |r| <= pi/4 */
+static double ksinf_(double r)
+{
+    const double z = r * r;
+    double p = -0x1.ae64567f544e4p-26;      /* -1/11! */
+    p = fma(p, z, 0x1.71de3a556c734p-19);   /* 1/9! */
+    p = fma(p, z, -0x1.a01a01a01a01ap-13);  /* -1/7! */
+    p = fma(p, z, 0x1.1111111111111p-7);    /* 1/5! */
+    p = fma(p, z, -0x1.5555555555555p-3);   /* -1/3! */
+    return fma(r * z, p, r);
+}
+
+/* Taylor cos to r^10 on |r| <= pi/4 */
+static double kcosf_(double r)
+{
+    const double z = r * r;
+    double p = -0x1.27e4fb7789f5cp-22;      /* -1/10! */
+    p = fma(p, z, 0x1.a01a01a01a01ap-16);   /* 1/8! */
+    p = fma(p, z, -0x1.6c16c16c16c17p-10);  /* -1/6! */
+    p = fma(p, z, 0x1.5555555555555p-5);    /* 1/4! */
+    p = fma(p, z, -0.5);
+    return fma(z, p, 1.0);
+}
+
+float orc_sinf(float x)
+{
+    double r;
+    const int q = reduce_f((double)x, &r);
+    const double sn = ksinf_(r), cs = kcosf_(r);
+    double v = (q & 1) ? cs : sn;
+    v = (q & 2) ? -v : v;
+    float y = (float)v;
+    const float ax = fabsf(x);
+    y = ax < 0x1p-12f ? x : y;
+    y = !(ax <= 0x1.fffffep+127f) ? x - x : y;
+    return y;
+}
+
+float orc_cosf(float x)
+{
+    double r;
+    const int q = reduce_f((double)x, &r);
+    const double sn = ksinf_(r), cs = kcosf_(r);
+    double v = (q & 1) ? sn : cs;
+    v = ((q + 1) & 2) ? -v : v;
+    float y = (float)v;
+    y = !(fabsf(x) <= 0x1.fffffep+127f) ? x - x : y;
+    return y;
+}
+
+/* e^x on [-104, 89]: n = rint(x log2 e), two-part ln2, Taylor to r^8 */
+float orc_expf(float x)
+{
+    const double xc = fmin(fmax((double)x, -104.0), 89.0);
+    const double n = rint(xc * LOG2E);
+    double r = fma(-n, LN2_HI, xc);
+    r = fma(-n, LN2_LO, r);
+    double p = 0x1.a01a01a01a01ap-16;       /* 1/8! */
+    p = fma(p, r, 0x1.a01a01a01a01ap-13);   /* 1/7! */
+    p = fma(p, r, 0x1.6c16c16c16c17p-10);   /* 1/6! */
+    p = fma(p, r, 0x1.1111111111111p-7);    /* 1/5! */
+    p = fma(p, r, 0x1.5555555555555p-5);    /* 1/4! */
+    p = fma(p, r, 0x1.5555555555555p-3);    /* 1/3! */
+    p = fma(p, r, 0.5);
+    const double u = fma(r * r, p, r);
+    float y = (float)ldexp(1.0 + u, (int)n);
+    y = (x != x) ? x + x : y;
+    return y;
+}
+
+/* log x: orc_log's arrangement with R to s^10 */
+float orc_logf(float x)
+{
+    const uint64_t u = dbits((double)x);
+    int k = (int)(u >> 52) - 1023;
+    uint64_t mb = (u & 0xFFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL;
+    const int big = mb > 0x3FF6A09E667F3BCCULL;
+    mb = big ? mb - 0x0010000000000000ULL : mb;
+    k += big ? 1 : 0;
+    const double f = bitsd(mb) - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double R = 0x1.745d1745d1746p-3;        /* 2/11 */
+    R = fma(R, z, 0x1.c71c71c71c71cp-3);    /* 2/9 */
+    R = fma(R, z, 0x1.2492492492492p-2);    /* 2/7 */
+    R = fma(R, z, 0x1.999999999999ap-2);    /* 2/5 */
+    R = fma(R, z, 0x1.5555555555555p-1);    /* 2/3 */
+    R = R * z;
+    const double hfsq = 0.5 * f * f;
+    const double dk = (double)k;
+    float y = (float)(dk * LN2_HI42 - ((hfsq - (s * (hfsq + R) + dk * LN2_LO42)) - f));
+    y = (x == 0.0f) ? -INFINITY : y;
+    y = (x < 0.0f) ? NAN : y;
+    y = (x == INFINITY) ? x : y;
+    y = (x != x) ? x + x : y;
+    return y;
+}
 
 void orc_math_f64(int fn, int mode, uint64_t n, const double* x, double* y)
 {

@@ -163,8 +163,9 @@ def test_math_bit_exact(dev, orc, fn, dt):
     if fn == "log":
         x = np.abs(x)
     if fn == "exp":
-        x = np.clip(x, -700, 700) if dt == torch.float64 else np.clip(x, -87, 88)
-    specials = [0.0, -0.0, np.inf, -np.inf, np.nan, 1e-300, 1e300, 2.0**60, -2.0**40, 5e-324]
+        x = np.clip(x, -700, 700) if dt == torch.float64 else np.clip(x, -104, 89)  # float: subnormal results too
+    specials = [0.0, -0.0, np.inf, -np.inf, np.nan, 1e-300, 1e300, 2.0**60, -2.0**40, 5e-324, 1e-40, -1e-40,
+                3e38, -103.5, 88.8]
     x = np.concatenate([x, specials]).astype(np_dt(dt))
     y = ae.math(fn, torch.from_numpy(x).to("cuda"))
     torch.cuda.synchronize()
