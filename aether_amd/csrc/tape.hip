@@ -437,25 +437,47 @@ std::string gen_program(const aeth_program& P, bool dbl)
         b += "      const " + std::string(T) + "* L" + std::to_string(q) + " = (const " + T + "*)A.leaf[" +
              std::to_string(P.load_leaf[q]) + "] + (uint64_t)" + std::to_string(P.load_comp[q]) + " * A.pitch[" +
              std::to_string(P.load_leaf[q]) + "];\n";
+    // Programs with few loads issue every load of the tile first (all rounds), then the arithmetic and
+    // the stores: as many 16-byte loads in flight per lane as the ahead-of-time kernels. Programs with
+    // many loads go round by round (one round's planes in registers).
+    const bool upfront = P.n_loads <= 6;
+    auto load_round = [&](const char* ind, const char* slot) {
+        std::string t;
+        t += std::string(ind) + "if (full) {\n" + ind + "  if (nt) {\n";
+        for (int q = 0; q < P.n_loads; ++q)
+            t += std::string(ind) + "    aeth::units::vload<true>(L" + std::to_string(q) + " + i0, x" +
+                 std::to_string(q) + "[" + slot + "]);\n";
+        t += std::string(ind) + "  } else {\n";
+        for (int q = 0; q < P.n_loads; ++q)
+            t += std::string(ind) + "    aeth::units::vload<false>(L" + std::to_string(q) + " + i0, x" +
+                 std::to_string(q) + "[" + slot + "]);\n";
+        t += std::string(ind) + "  }\n" + ind + "} else {\n";
+        for (int q = 0; q < P.n_loads; ++q)
+            t += std::string(ind) + "  for (int k = 0; k < VN; ++k) x" + std::to_string(q) + "[" + slot +
+                 "][k] = (i0 + k < n) ? L" + std::to_string(q) + "[i0 + k] : T(0);\n";
+        t += std::string(ind) + "}\n";
+        return t;
+    };
+    const std::string slot = upfront ? "r" : "0";
+    for (int q = 0; q < P.n_loads; ++q)
+        b += "      T x" + std::to_string(q) + (upfront ? "[ROUNDS][VN];\n" : "[1][VN];\n");
+    if (upfront) {
+        b += "      for (int r = 0; r < ROUNDS; ++r) {\n"
+             "        const uint32_t i0 = s0 + r * (256 * VN) + threadIdx.x * VN;\n";
+        b += load_round("        ", "r");
+        b += "      }\n";
+    }
     b += "      for (int r = 0; r < ROUNDS; ++r) {\n";
     b += "        const uint32_t i0 = s0 + r * (256 * VN) + threadIdx.x * VN;\n";
-    for (int q = 0; q < P.n_loads; ++q) b += "        T x" + std::to_string(q) + "[VN];\n";
-    b += "        if (full) {\n";
-    for (int q = 0; q < P.n_loads; ++q)
-        b += "          if (nt) aeth::units::vload<true>(L" + std::to_string(q) + " + i0, x" + std::to_string(q) +
-             "); else aeth::units::vload<false>(L" + std::to_string(q) + " + i0, x" + std::to_string(q) + ");\n";
-    b += "        } else {\n";
-    for (int q = 0; q < P.n_loads; ++q)
-        b += "          for (int k = 0; k < VN; ++k) x" + std::to_string(q) + "[k] = (i0 + k < n) ? L" +
-             std::to_string(q) + "[i0 + k] : T(0);\n";
-    b += "        }\n        T v[VN];\n        for (int k = 0; k < VN; ++k) {\n";
+    if (!upfront) b += load_round("        ", "0");
+    b += "        T v[VN];\n        for (int k = 0; k < VN; ++k) {\n";
     std::vector<std::string> st;
     int tmp = 0;
     auto fresh = [&] { return "t" + std::to_string(tmp++); };
     for (int i = 0; i < P.n_insns; ++i) {
         const int op = P.insns[i].op, arg = P.insns[i].arg;
         if (op == AETH_OP_PUSH) {
-            st.push_back("x" + std::to_string(arg) + "[k]");
+            st.push_back("x" + std::to_string(arg) + "[" + slot + "][k]");
         } else if (op == AETH_OP_CONST) {
             st.push_back("((T)A.consts[" + std::to_string(arg) + "])");
         } else if (op == AETH_OP_RNG) {
@@ -625,38 +647,66 @@ extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* pro
         if (n_rngs < 0 || n_rngs > AETH_TAPE_MAX_RNGS) fail("aeth_eval_tape", "0..8 rng leaves");
         if (n > 0xFFFFFFFFull) fail("aeth_eval_tape", "n beyond the 32-bit offset_t");
         if (sum_mode != AETH_SUM_NONE && !sums) fail("aeth_eval_tape", "sums is NULL");
-        TapeArgs A;
-        std::memset(&A, 0, sizeof A);
         const size_t es = dtype == AETH_F64 ? 8 : 4;
+        for (int p = 0; p < n_programs; ++p) validate(programs[p], n_leaves, n_consts, n_rngs);
         bool vec = true;
-        int max_loads = 0;
-        for (int p = 0; p < n_programs; ++p) {
-            validate(programs[p], n_leaves, n_consts, n_rngs);
-            A.prog[p] = programs[p];
-            max_loads = std::max<int>(max_loads, programs[p].n_loads);
-        }
-        uint64_t span_bytes = 0;
         for (int l = 0; l < n_leaves; ++l) {
-            A.leaf[l] = leaves[l];
-            A.pitch[l] = leaf_pitch[l];
             if (leaf_pitch[l] < n) fail("aeth_eval_tape", "leaf pitch < n");
             vec = vec && aligned16(leaves[l]) && (leaf_pitch[l] * es) % 16 == 0;
         }
-        for (int p = 0; p < n_programs; ++p) span_bytes += (uint64_t)(programs[p].n_loads + 1) * n * es;
-        for (int c = 0; c < n_consts; ++c) A.consts[c] = consts[c];
-        for (int r = 0; r < n_rngs; ++r) A.rng[r] = rngs[r];
-        A.n = (uint32_t)n;
-        A.tiles = n ? (uint32_t)((n + kTile - 1) / kTile) : 1u;
-        A.units = A.tiles * (uint32_t)n_programs;
-        const bool nt = span_bytes >= (512ull << 20);
-        hipStream_t st = as_stream(stream);
         if (sum_mode == AETH_SUM_FLOATFLOAT && dtype == AETH_F64)
             fail("aeth_eval_tape", "float-float accumulation needs dtype F32", 0, 2);
-        hipFunction_t jit = jit_function(dtype, sum_mode, n_programs, programs);
-        if (dtype == AETH_F64)
-            dispatch<double>(A, max_loads, n_programs, vec, nt, sum_mode, static_cast<double*>(sums), st, jit);
-        else
-            dispatch<float>(A, max_loads, n_programs, vec, nt, sum_mode, static_cast<float*>(sums), st, jit);
+        // A program that reads a plane another program writes (a = a.cross(b)): aether stores component
+        // 0 of a sample before it evaluates component 1 (expr/Assign.h:124-160), so such programs run as
+        // one launch each, in component order -- the same values, sample by sample.
+        auto plane = [&](int leaf, int comp) {
+            return static_cast<const char*>(leaves[leaf]) + (uint64_t)comp * leaf_pitch[leaf] * es;
+        };
+        for (int p = 0; p < n_programs; ++p)
+            for (int q = p + 1; q < n_programs; ++q)
+                if (plane(programs[p].out_leaf, programs[p].out_comp) == plane(programs[q].out_leaf, programs[q].out_comp))
+                    fail("aeth_eval_tape", "two programs write the same plane");
+        bool serial = false;
+        for (int p = 0; p < n_programs && !serial; ++p)
+            for (int q = 0; q < n_programs && !serial; ++q) {
+                if (q == p) continue;
+                const char* out = plane(programs[q].out_leaf, programs[q].out_comp);
+                for (int k = 0; k < programs[p].n_loads; ++k)
+                    if (plane(programs[p].load_leaf[k], programs[p].load_comp[k]) == out) serial = true;
+            }
+        hipStream_t st = as_stream(stream);
+        auto run = [&](const aeth_program* P, int np, void* out_sums) {
+            TapeArgs A;
+            std::memset(&A, 0, sizeof A);
+            int max_loads = 0;
+            uint64_t span_bytes = 0;
+            for (int p = 0; p < np; ++p) {
+                A.prog[p] = P[p];
+                max_loads = std::max<int>(max_loads, P[p].n_loads);
+                span_bytes += (uint64_t)(P[p].n_loads + 1) * n * es;
+            }
+            for (int l = 0; l < n_leaves; ++l) {
+                A.leaf[l] = leaves[l];
+                A.pitch[l] = leaf_pitch[l];
+            }
+            for (int c = 0; c < n_consts; ++c) A.consts[c] = consts[c];
+            for (int r = 0; r < n_rngs; ++r) A.rng[r] = rngs[r];
+            A.n = (uint32_t)n;
+            A.tiles = n ? (uint32_t)((n + kTile - 1) / kTile) : 1u;
+            A.units = A.tiles * (uint32_t)np;
+            const bool nt = span_bytes >= (512ull << 20);
+            hipFunction_t jit = jit_function(dtype, sum_mode, np, P);
+            if (dtype == AETH_F64)
+                dispatch<double>(A, max_loads, np, vec, nt, sum_mode, static_cast<double*>(out_sums), st, jit);
+            else
+                dispatch<float>(A, max_loads, np, vec, nt, sum_mode, static_cast<float*>(out_sums), st, jit);
+        };
+        if (!serial) {
+            run(programs, n_programs, sums);
+        } else {
+            for (int p = 0; p < n_programs; ++p)
+                run(&programs[p], 1, sums ? static_cast<char*>(sums) + 2 * es * (size_t)p : nullptr);
+        }
     });
 }
 

@@ -87,3 +87,27 @@ def test_tape_kernels_compile_without_a_gpu(dt, mode):
     size = C.c_uint64()
     _lib.call("aeth_tape_jit_build", 0 if dt == torch.float64 else 1, mode, 3, progs, b"gfx950", C.byref(size))
     assert size.value > 1000
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_wide_tapes_compile_without_a_gpu(dt):
+    """Programs with many loads (round-by-round loading) lower and compile too: a 9-plane Vec3 tree and a
+    16-plane sum."""
+    import ctypes as C
+
+    from aether_amd import _lib
+
+    a, b, c, e = arr(3, dt), arr(3, dt), arr(3, dt), arr(1, dt)
+    progs, _, _, _ = E.lower(e, a.cross(b).dot(c) + E.sqrt(a.dot(a) + 1.0))
+    assert progs[0].n_loads == 9
+    size = C.c_uint64()
+    _lib.call("aeth_tape_jit_build", 0 if dt == torch.float64 else 1, 0, 1, progs, b"gfx950", C.byref(size))
+    assert size.value > 1000
+    many = [arr(1, dt) for _ in range(16)]
+    x = many[0]
+    for m in many[1:]:
+        x = x + m
+    progs, _, _, _ = E.lower(many[0], x)
+    assert progs[0].n_loads == 16
+    _lib.call("aeth_tape_jit_build", 0 if dt == torch.float64 else 1, -1, 1, progs, b"gfx950", C.byref(size))
+    assert size.value > 1000

@@ -98,6 +98,37 @@ def test_invalid_tape_is_rejected(dev):
                      ae.stream_handle())
 
 
+@pytest.mark.parametrize("n", [1000, 70001])
+def test_cross_component_aliasing_is_sequential(dev, n):
+    # a = a x b: aether stores component 0 of a sample before it evaluates component 1
+    # (expr/Assign.h:124-160), so components 1 and 2 read the updated components
+    a, b = filled(n, torch.float64, seeds=(3, 4))
+    A = a.data[:, :n].cpu().numpy().copy()
+    B = b.data[:, :n].cpu().numpy()
+    sums = E.assign(a, a.cross(b), ae.SUM_NEUMAIER)
+    torch.cuda.synchronize()
+    A[0] = A[1] * B[2] - A[2] * B[1]
+    A[1] = A[2] * B[0] - A[0] * B[2]
+    A[2] = A[0] * B[1] - A[1] * B[0]
+    assert np.array_equal(a.data[:, :n].cpu().numpy(), A)
+    assert torch.equal(sums, ae.reduce(a, ae.SUM_NEUMAIER))
+
+
+def test_two_programs_one_plane_rejected(dev):
+    n = 100
+    a, b = filled(n, torch.float64, seeds=(3, 4))
+    progs, leaves, consts, rngs = E.lower(b, a * 2.0)
+    progs[1].out_comp = 0  # programs 0 and 1 both write plane 0 of b
+    import ctypes as C
+
+    ptrs = (C.c_void_p * len(leaves))(*[lf.ptr() for lf in leaves])
+    pitch = (C.c_uint64 * len(leaves))(*[lf.capacity for lf in leaves])
+    cst = (C.c_double * len(consts))(*consts)
+    with pytest.raises(ae.AetherError):
+        ae._lib.call("aeth_eval_tape", ae.F64, 3, progs, len(leaves), ptrs, pitch, len(consts), cst, 0, None, n, -1,
+                     None, ae.stream_handle())
+
+
 def test_runtime_lowering_was_used(dev):
     import ctypes as C
 
@@ -119,6 +150,11 @@ s1 = ae.eval_cfg0(a, b, c, d1, ae.SUM_NEUMAIER)
 s2 = E.assign(d2, a + b * c + E.sin(a) * E.exp(-1.0 * (b * b)), ae.SUM_NEUMAIER)
 torch.cuda.synchronize()
 assert torch.equal(d1.data[:, :n], d2.data[:, :n]) and torch.equal(s1, s2)
+A = a.data[:, :n].cpu().numpy().copy(); B = b.data[:, :n].cpu().numpy()
+E.assign(a, a.cross(b)); torch.cuda.synchronize()
+A[0] = A[1] * B[2] - A[2] * B[1]; A[1] = A[2] * B[0] - A[0] * B[2]; A[2] = A[0] * B[1] - A[1] * B[0]
+import numpy as np
+assert np.array_equal(a.data[:, :n].cpu().numpy(), A)
 print("interpreter ok")
 """
 
