@@ -50,6 +50,18 @@ struct TapeArgs {
 };
 static_assert(sizeof(TapeArgs) < 4000, "kernel arguments must fit the 4 KiB kernarg segment");
 
+// Arguments of a runtime-compiled tape kernel: every plane a program touches resolved on the host to
+// its address (no dependent reads of descriptors before a block's first load), the constants, the
+// random leaves and, per program, the leaf index of each RNG instruction.
+struct JitArgs {
+    const void* plane[AETH_TAPE_MAX_PROGRAMS][1 + AETH_TAPE_MAX_LOADS]; // [p][0] output, [p][1 + q] load q
+    double consts[AETH_TAPE_MAX_CONSTS];
+    aeth_rng_leaf rng[AETH_TAPE_MAX_RNGS];
+    uint8_t rng_arg[AETH_TAPE_MAX_PROGRAMS][AETH_TAPE_MAX_INSNS];
+    uint32_t n, tiles, units;
+};
+static_assert(sizeof(JitArgs) < 4000, "kernel arguments must fit the 4 KiB kernarg segment");
+
 template<class T>
 using ext16 = T __attribute__((ext_vector_type(16 / sizeof(T))));
 
@@ -347,10 +359,26 @@ void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums
     auto* part = static_cast<acc::Pair<T>*>(scratch.get());
     const int grid = (int)std::max<uint32_t>(1u, A.units);
     if (jit) {
-        TapeArgs args = A;
+        JitArgs args;
+        std::memset(&args, 0, sizeof args);
+        for (int p = 0; p < np; ++p) {
+            const aeth_program& P = A.prog[p];
+            auto plane = [&](int leaf, int comp) {
+                return static_cast<const void*>(static_cast<const T*>(A.leaf[leaf]) + (uint64_t)comp * A.pitch[leaf]);
+            };
+            args.plane[p][0] = plane(P.out_leaf, P.out_comp);
+            for (int q = 0; q < P.n_loads; ++q) args.plane[p][1 + q] = plane(P.load_leaf[q], P.load_comp[q]);
+            for (int i = 0; i < P.n_insns; ++i)
+                if (P.insns[i].op == AETH_OP_RNG) args.rng_arg[p][i] = P.insns[i].arg;
+        }
+        std::memcpy(args.consts, A.consts, sizeof args.consts);
+        std::memcpy(args.rng, A.rng, sizeof args.rng);
+        args.n = A.n;
+        args.tiles = A.tiles;
+        args.units = A.units;
         void* part_v = part;
-        int vec_i = vec ? 1 : 0, nt_i = nt ? 1 : 0;
-        void* params[] = {&args, &part_v, &vec_i, &nt_i};
+        int vec_i = vec ? 1 : 0;
+        void* params[] = {&args, &part_v, &vec_i};
         check(hipModuleLaunchKernel(jit, (unsigned)grid, 1, 1, kThreads, 1, 1, 0, st, params, nullptr),
               "hipModuleLaunchKernel(tape)");
     } else if (max_loads <= 4) {
@@ -412,18 +440,29 @@ std::mutex g_jit_mu;
 std::map<std::string, JitEntry> g_jit;
 std::atomic<uint64_t> g_jit_compiled{0}, g_jit_hits{0}, g_jit_fallbacks{0};
 
-std::string tape_key(int dtype, int mode, int np, const aeth_program* P, int dev)
+// Programs that differ only in which planes they read and write, and in which random leaves they
+// draw from, share one code path: the dataflow (op codes, which loaded plane each PUSH names, which
+// constant each CONST names) is compiled in; plane addresses and random leaves come from the kernel
+// arguments per program at run time. So the components of a vector expression run the same
+// instructions, as the ahead-of-time kernels do.
+std::string structure_key(const aeth_program& P)
+{
+    std::string k = std::to_string(P.n_loads) + ":";
+    for (int i = 0; i < P.n_insns; ++i) {
+        const int op = P.insns[i].op;
+        k += std::to_string(op);
+        if (op == AETH_OP_PUSH || op == AETH_OP_CONST) k += "." + std::to_string(P.insns[i].arg);
+        k += " ";
+    }
+    return k;
+}
+
+std::string tape_key(int dtype, int mode, int np, const aeth_program* P, bool nt, int dev)
 {
     std::string k = std::to_string(dev) + ":" + std::to_string(dtype) + ":" + std::to_string(mode) + ":" +
-                    std::to_string(np);
-    for (int p = 0; p < np; ++p) {
-        k += "|" + std::to_string(P[p].out_leaf) + "," + std::to_string(P[p].out_comp) + ";";
-        for (int i = 0; i < P[p].n_loads; ++i)
-            k += std::to_string(P[p].load_leaf[i]) + "." + std::to_string(P[p].load_comp[i]) + " ";
-        k += ";";
-        for (int i = 0; i < P[p].n_insns; ++i)
-            k += std::to_string(P[p].insns[i].op) + "." + std::to_string(P[p].insns[i].arg) + " ";
-    }
+                    std::to_string(np) + (nt ? ":nt" : ":t");
+    // the generated code depends on each program's structure only (and on which programs share one)
+    for (int p = 0; p < np; ++p) k += "|" + structure_key(P[p]);
     return k;
 }
 
@@ -431,27 +470,23 @@ std::string gen_program(const aeth_program& P, bool dbl)
 {
     const char* T = dbl ? "double" : "float";
     std::string b;
-    b += "      " + std::string(T) + "* outp = (" + T + "*)A.leaf[" + std::to_string(P.out_leaf) + "] + (uint64_t)" +
-         std::to_string(P.out_comp) + " * A.pitch[" + std::to_string(P.out_leaf) + "];\n";
-    for (int q = 0; q < P.n_loads; ++q)
-        b += "      const " + std::string(T) + "* L" + std::to_string(q) + " = (const " + T + "*)A.leaf[" +
-             std::to_string(P.load_leaf[q]) + "] + (uint64_t)" + std::to_string(P.load_comp[q]) + " * A.pitch[" +
-             std::to_string(P.load_leaf[q]) + "];\n";
+    b += "      " + std::string(T) + "* outp = (" + T + "*)A.plane[pi][0];\n";
+    for (int q = 0; q < P.n_loads; ++q) {
+        const std::string Q = std::to_string(q);
+        b += "      const " + std::string(T) + "* L" + Q + " = (const " + T + "*)A.plane[pi][" +
+             std::to_string(1 + q) + "];\n";
+    }
     // Programs with few loads issue every load of the tile first (all rounds), then the arithmetic and
     // the stores: as many 16-byte loads in flight per lane as the ahead-of-time kernels. Programs with
     // many loads go round by round (one round's planes in registers).
     const bool upfront = P.n_loads <= 6;
     auto load_round = [&](const char* ind, const char* slot) {
         std::string t;
-        t += std::string(ind) + "if (full) {\n" + ind + "  if (nt) {\n";
+        t += std::string(ind) + "if (full) {\n";
         for (int q = 0; q < P.n_loads; ++q)
-            t += std::string(ind) + "    aeth::units::vload<true>(L" + std::to_string(q) + " + i0, x" +
+            t += std::string(ind) + "  aeth::units::vload<NT>(L" + std::to_string(q) + " + i0, x" +
                  std::to_string(q) + "[" + slot + "]);\n";
-        t += std::string(ind) + "  } else {\n";
-        for (int q = 0; q < P.n_loads; ++q)
-            t += std::string(ind) + "    aeth::units::vload<false>(L" + std::to_string(q) + " + i0, x" +
-                 std::to_string(q) + "[" + slot + "]);\n";
-        t += std::string(ind) + "  }\n" + ind + "} else {\n";
+        t += std::string(ind) + "} else {\n";
         for (int q = 0; q < P.n_loads; ++q)
             t += std::string(ind) + "  for (int k = 0; k < VN; ++k) x" + std::to_string(q) + "[" + slot +
                  "][k] = (i0 + k < n) ? L" + std::to_string(q) + "[i0 + k] : T(0);\n";
@@ -482,7 +517,7 @@ std::string gen_program(const aeth_program& P, bool dbl)
             st.push_back("((T)A.consts[" + std::to_string(arg) + "])");
         } else if (op == AETH_OP_RNG) {
             const std::string t = fresh();
-            b += "          const T " + t + " = jit_draw<T>(A.rng[" + std::to_string(arg) + "], i0 + k);\n";
+            b += "          const T " + t + " = jit_draw<T>(A.rng[A.rng_arg[pi][" + std::to_string(i) + "]], i0 + k);\n";
             st.push_back(t);
         } else if (op >= AETH_OP_ADD && op <= AETH_OP_DIV) {
             const std::string y = st.back();
@@ -512,25 +547,24 @@ std::string gen_program(const aeth_program& P, bool dbl)
         }
     }
     b += "          v[k] = " + st.back() + ";\n        }\n";
-    b += "        if (full) {\n          if (nt) aeth::units::vstore<true>(outp + i0, v); else aeth::units::vstore<false>(outp + i0, v);\n";
+    b += "        if (full) {\n          aeth::units::vstore<NT>(outp + i0, v);\n";
     b += "          if (SUM) for (int k = 0; k < VN; ++k) aeth::acc::accumulate<M, false>(acc, v[k]);\n";
     b += "        } else {\n          for (int k = 0; k < VN; ++k) if (i0 + k < n) { outp[i0 + k] = v[k]; "
          "if (SUM) aeth::acc::accumulate<M, false>(acc, v[k]); }\n        }\n      }\n";
     return b;
 }
 
-std::string gen_kernel(int dtype, int mode, int np, const aeth_program* P)
+std::string gen_kernel(int dtype, int mode, int np, const aeth_program* P, bool nt)
 {
     const bool dbl = dtype == AETH_F64;
     const int M = mode == AETH_SUM_NONE ? 0 : mode;
     std::string s = kJitPrelude;
     s += R"K(
-struct TapeArgsJ {
-    aeth_program prog[AETH_TAPE_MAX_PROGRAMS];
-    const void* leaf[AETH_TAPE_MAX_LEAVES];
-    uint64_t pitch[AETH_TAPE_MAX_LEAVES];
+struct JitArgs {
+    const void* plane[AETH_TAPE_MAX_PROGRAMS][1 + AETH_TAPE_MAX_LOADS];
     double consts[AETH_TAPE_MAX_CONSTS];
     aeth_rng_leaf rng[AETH_TAPE_MAX_RNGS];
+    uint8_t rng_arg[AETH_TAPE_MAX_PROGRAMS][AETH_TAPE_MAX_INSNS];
     uint32_t n, tiles, units;
 };
 template<class T>
@@ -547,9 +581,12 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
     }
 }
 )K";
-    s += "extern \"C\" __global__ __launch_bounds__(256) void aeth_tape_jit(const TapeArgsJ A, void* part_, int vec, "
-         "int nt)\n{\n";
+    s += "extern \"C\" __global__ __launch_bounds__(256) void aeth_tape_jit(const JitArgs A, void* part_, int vec)\n"
+         "{\n";
     s += std::string("  typedef ") + (dbl ? "double" : "float") + " T;\n";
+    // nontemporal is compiled in: a run-time choice between two otherwise identical loads is merged
+    // by the optimiser into one plain load
+    s += std::string("  constexpr bool NT = ") + (nt ? "true" : "false") + ";\n";
     s += "  constexpr int M = " + std::to_string(M) + ";\n";
     s += std::string("  constexpr bool SUM = ") + (mode == AETH_SUM_NONE ? "false" : "true") + ";\n";
     s += "  constexpr int VN = 16 / sizeof(T);\n  constexpr int ROUNDS = 2048 / (256 * VN);\n";
@@ -559,8 +596,16 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
     s += "    const uint32_t pi = u / A.tiles;\n    const uint32_t tile = u - pi * A.tiles;\n";
     s += "    const uint32_t s0 = tile * 2048;\n    const bool full = vec && s0 + 2048 <= n;\n";
     s += "    aeth::acc::Pair<T> acc{T(0), T(0)};\n    switch (pi) {\n";
+    std::vector<bool> done(np, false);
     for (int p = 0; p < np; ++p) {
-        s += "    case " + std::to_string(p) + ": {\n" + gen_program(P[p], dbl) + "      break;\n    }\n";
+        if (done[p]) continue;
+        const std::string key = structure_key(P[p]);
+        for (int q = p; q < np; ++q)
+            if (!done[q] && structure_key(P[q]) == key) {
+                s += "    case " + std::to_string(q) + ":\n";
+                done[q] = true;
+            }
+        s += "    {\n" + gen_program(P[p], dbl) + "      break;\n    }\n";
     }
     s += "    default: break;\n    }\n";
     s += "    if (SUM) {\n      const aeth::acc::Pair<T> rr = aeth::acc::block_combine<M>(acc, lds);\n"
@@ -569,10 +614,10 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
 }
 
 // Compile a tape's kernel for `arch_opt` ("--offload-arch=gfx950"); false (log in last error) on failure.
-bool jit_compile(int dtype, int mode, int np, const aeth_program* P, const std::string& arch_opt,
+bool jit_compile(int dtype, int mode, int np, const aeth_program* P, bool nt, const std::string& arch_opt,
                  std::vector<char>& code)
 {
-    const std::string src = gen_kernel(dtype, mode, np, P);
+    const std::string src = gen_kernel(dtype, mode, np, P, nt);
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "aeth_tape_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
         set_error("hiprtcCreateProgram failed");
@@ -599,13 +644,13 @@ bool jit_compile(int dtype, int mode, int np, const aeth_program* P, const std::
 }
 
 // Returns nullptr (and records why) when the runtime compiler is unavailable.
-hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P)
+hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, bool nt)
 {
     static const bool disabled = std::getenv("AETHER_TAPE_INTERPRET") != nullptr;
     if (disabled) return nullptr;
     int dev = 0;
     check(hipGetDevice(&dev), "hipGetDevice");
-    const std::string key = tape_key(dtype, mode, np, P, dev);
+    const std::string key = tape_key(dtype, mode, np, P, nt, dev);
     std::lock_guard<std::mutex> lock(g_jit_mu);
     auto it = g_jit.find(key);
     if (it != g_jit.end()) {
@@ -618,7 +663,7 @@ hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P)
     const size_t colon = arch.find(':');
     if (colon != std::string::npos) arch.resize(colon);
     std::vector<char> code;
-    if (!jit_compile(dtype, mode, np, P, arch, code)) {
+    if (!jit_compile(dtype, mode, np, P, nt, arch, code)) {
         ++g_jit_fallbacks;
         return nullptr;
     }
@@ -695,7 +740,7 @@ extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* pro
             A.tiles = n ? (uint32_t)((n + kTile - 1) / kTile) : 1u;
             A.units = A.tiles * (uint32_t)np;
             const bool nt = span_bytes >= (512ull << 20);
-            hipFunction_t jit = jit_function(dtype, sum_mode, np, P);
+            hipFunction_t jit = jit_function(dtype, sum_mode, np, P, nt);
             if (dtype == AETH_F64)
                 dispatch<double>(A, max_loads, np, vec, nt, sum_mode, static_cast<double*>(out_sums), st, jit);
             else
@@ -727,7 +772,7 @@ extern "C" int aeth_tape_jit_build(int dtype, int sum_mode, int n_programs, cons
         for (int p = 0; p < n_programs; ++p)
             validate(programs[p], AETH_TAPE_MAX_LEAVES, AETH_TAPE_MAX_CONSTS, AETH_TAPE_MAX_RNGS);
         std::vector<char> code;
-        if (!jit_compile(dtype, sum_mode, n_programs, programs, std::string("--offload-arch=") + arch, code))
+        if (!jit_compile(dtype, sum_mode, n_programs, programs, true, std::string("--offload-arch=") + arch, code))
             throw Failure{1};
         if (code_bytes) *code_bytes = code.size();
     });
