@@ -216,3 +216,5 @@ class FftBatch:
             self.close()
         except Exception:
             pass
+
+from . import fft  # noqa: E402,F401  (batched FFTs with the reference's Python API)

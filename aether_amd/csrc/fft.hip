@@ -524,16 +524,125 @@ __global__ __launch_bounds__(kAuxThreads) void real_part_kernel(const T* __restr
         out[i] = full[2 * i];
 }
 
+// ---- Bluestein (chirp-z) passes for lengths that are not powers of two.
+// X_k = c_k * sum_j (x_j c_j) conj(c_{k-j}) with c_j = exp(-+ i pi j^2 / n) (forward / inverse):
+// the sum is a circular convolution of length m (a power of two >= 2n - 1) done with two of the
+// batched LDS transforms and a pointwise product with the transform of the chirp, precomputed per
+// plan with the 1/m of the inverse folded in. The chirp angle is pi (j^2 mod 2n) / n, reduced
+// exactly in integers before the one rounding. The convolution runs in complex128 for both widths
+// (a complex64 plan reads and writes float, rounding once at the end), so a float transform of any
+// length is as accurate as the double one rounded to float.
+struct BluesteinPass {
+    uint64_t n = 0, m = 0, lines = 0, inner = 1;
+    int prec = 0;              // of the data: 0 complex128, 1 complex64
+    BatchPlan* conv = nullptr; // lines x m contiguous complex128 signals
+    double* work = nullptr;    // lines x m complex128
+    double* chirp = nullptr;   // n complex: exp(i pi j^2 / n) (sign applied per direction)
+    double* bhat = nullptr;    // m complex: FFT_m(b) / m for the plan's direction
+};
+
+__device__ __forceinline__ void chirp_at(uint64_t j, uint64_t n, double* re, double* im)
+{
+    const uint64_t r = (j * j) % (2 * n);
+    dm::sincos(3.141592653589793238 * ((double)r / (double)n), im, re);
+}
+
+__global__ __launch_bounds__(kAuxThreads) void chirp_kernel(double* __restrict__ chirp, uint64_t n)
+{
+    for (uint64_t j = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; j < n; j += (uint64_t)gridDim.x * kAuxThreads)
+        chirp_at(j, n, &chirp[2 * j], &chirp[2 * j + 1]);
+}
+
+// b_l = conj^dir(c_l) for l < n and b_{m-l} = b_l, zero between; sign = -1 makes it conj
+__global__ __launch_bounds__(kAuxThreads) void chirp_b_kernel(const double* __restrict__ chirp, double* __restrict__ b,
+                                                              uint64_t n, uint64_t m, double sign)
+{
+    for (uint64_t l = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; l < m; l += (uint64_t)gridDim.x * kAuxThreads) {
+        const uint64_t k = l < n ? l : (m - l < n ? m - l : m);
+        b[2 * l] = k < m ? chirp[2 * k] : 0.0;
+        b[2 * l + 1] = k < m ? sign * chirp[2 * k + 1] : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kAuxThreads) void scale_kernel(double* __restrict__ x, uint64_t count, double s)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kAuxThreads)
+        x[i] *= s;
+}
+
+// a_j = x_j * conj^dir(c_j), zero-padded to m; x read along a strided line
+template<class T>
+__global__ __launch_bounds__(kAuxThreads) void blue_pre_kernel(const T* __restrict__ in, double* __restrict__ work,
+                                                               const double* __restrict__ chirp, uint64_t lines,
+                                                               uint64_t n, uint64_t m, uint64_t inner, double sign)
+{
+    const uint64_t count = lines * m;
+    for (uint64_t t = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; t < count; t += (uint64_t)gridDim.x * kAuxThreads) {
+        const uint64_t li = t / m, j = t - li * m;
+        double re = 0.0, im = 0.0;
+        if (j < n) {
+            const uint64_t o = li / inner, q = li - o * inner;
+            const uint64_t e = (o * n + j) * inner + q;
+            const double xr = in[2 * e], xi = in[2 * e + 1];
+            const double cr = chirp[2 * j], ci = sign * chirp[2 * j + 1];
+            re = xr * cr - xi * ci;
+            im = xr * ci + xi * cr;
+        }
+        work[2 * t] = re;
+        work[2 * t + 1] = im;
+    }
+}
+
+__global__ __launch_bounds__(kAuxThreads) void blue_mul_kernel(double* __restrict__ work,
+                                                               const double* __restrict__ bhat, uint64_t lines,
+                                                               uint64_t m)
+{
+    const uint64_t count = lines * m;
+    for (uint64_t t = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; t < count; t += (uint64_t)gridDim.x * kAuxThreads) {
+        const uint64_t j = t % m;
+        const double ar = work[2 * t], ai = work[2 * t + 1];
+        const double br = bhat[2 * j], bi = bhat[2 * j + 1];
+        work[2 * t] = ar * br - ai * bi;
+        work[2 * t + 1] = ar * bi + ai * br;
+    }
+}
+
+// X_k = conj^dir(c_k) * conv_k * scale, written along the strided line
+template<class T>
+__global__ __launch_bounds__(kAuxThreads) void blue_post_kernel(const double* __restrict__ work, T* __restrict__ out,
+                                                                const double* __restrict__ chirp, uint64_t lines,
+                                                                uint64_t n, uint64_t m, uint64_t inner, double sign,
+                                                                double scale)
+{
+    const uint64_t count = lines * n;
+    for (uint64_t t = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; t < count; t += (uint64_t)gridDim.x * kAuxThreads) {
+        const uint64_t li = t / n, k = t - li * n;
+        const uint64_t w = li * m + k;
+        const double vr = work[2 * w], vi = work[2 * w + 1];
+        const double cr = chirp[2 * k], ci = sign * chirp[2 * k + 1];
+        const uint64_t o = li / inner, q = li - o * inner;
+        const uint64_t e = (o * n + k) * inner + q;
+        out[2 * e] = (T)((vr * cr - vi * ci) * scale);
+        out[2 * e + 1] = (T)((vr * ci + vi * cr) * scale);
+    }
+}
+
 } // namespace
+
+// One axis of a plan: a Stockham pass (power-of-two lengths up to 8192) or a Bluestein pass.
+struct Pass {
+    BatchPlan* stockham = nullptr;
+    BluesteinPass* blue = nullptr;
+};
 
 struct SeamPlan {
     int dev = 0;
     int kind = AETHER_FFT_C2C;
     int forward = 1;
     bool dbl = true;
-    std::vector<BatchPlan*> passes; // complex passes on the complex-side shape
+    std::vector<Pass> passes; // complex passes on the complex-side shape
     std::vector<double> scales;
-    BatchPlan* line = nullptr;       // real transforms: the halved axis over the full complex shape
+    Pass line;                // real transforms: the halved axis over the full complex shape
     double line_scale = 1.0;
     void* full = nullptr;            // real transforms: (outer, n, inner) complex work
     void* half = nullptr;            // c2r with other axes: (outer, n/2+1, inner) complex work
@@ -546,9 +655,21 @@ struct SeamPlan {
         (void)hipFree(p->sigs);
         delete p;
     }
+    static void free_pass(Pass& p)
+    {
+        free_pass(p.stockham);
+        if (p.blue) {
+            free_pass(p.blue->conv);
+            (void)hipFree(p.blue->work);
+            (void)hipFree(p.blue->chirp);
+            (void)hipFree(p.blue->bhat);
+            delete p.blue;
+        }
+        p = Pass{};
+    }
     ~SeamPlan()
     {
-        for (auto* p : passes) free_pass(p);
+        for (auto& p : passes) free_pass(p);
         free_pass(line);
         (void)hipFree(full);
         (void)hipFree(half);
@@ -557,29 +678,14 @@ struct SeamPlan {
 
 namespace {
 
-// One strided-line pass along `ax` of a dense row-major complex `shape`.
-BatchPlan* make_pass(const std::vector<int64_t>& shape, int ax, int dev, bool dbl)
+// The descriptor tables of `lines` signals of 2^l2 points at `off[li]`, `stride` apart, on the device.
+BatchPlan* make_batch(const std::vector<uint64_t>& off, uint32_t l2, uint64_t stride, int dev, bool dbl)
 {
-    const int ndim = (int)shape.size();
-    uint64_t total = 1;
-    for (int d = 0; d < ndim; ++d) total *= (uint64_t)shape[d];
-    const uint64_t n = (uint64_t)shape[ax];
-    const uint32_t l2 = log2_exact(n);
-    if (l2 == 0xFFFFFFFFu || l2 > (uint32_t)kMaxLog2)
-        fail("fft_plan", "axis lengths must be powers of two in [1, 8192]", 0, 2);
-    uint64_t inner = 1;
-    for (int d = ax + 1; d < ndim; ++d) inner *= (uint64_t)shape[d];
-    if (inner > 0xFFFFFFFFull) fail("fft_plan", "inner extent too large");
-    const uint64_t lines = n ? total / n : 0;
+    const uint64_t lines = off.size();
     std::vector<uint32_t> lg(lines, l2);
-    std::vector<uint64_t> off(lines);
-    for (uint64_t li = 0; li < lines; ++li) {
-        const uint64_t o = li / inner, in = li % inner;
-        off[li] = o * n * inner + in;
-    }
     std::vector<Desc> descs;
     std::vector<Sig> sigs;
-    build_tables(lg.data(), off.data(), off.data(), lines, descs, sigs, (uint32_t)inner);
+    build_tables(lg.data(), off.data(), off.data(), lines, descs, sigs, (uint32_t)stride);
     auto bp = std::make_unique<BatchPlan>();
     bp->prec = dbl ? 0 : 1;
     bp->dev = dev;
@@ -600,6 +706,109 @@ BatchPlan* make_pass(const std::vector<int64_t>& shape, int ax, int dev, bool db
         }
     }
     return bp.release();
+}
+
+constexpr uint64_t kMaxBluestein = (uint64_t)kMaxN / 2; // 2n - 1 <= 8192
+
+BluesteinPass* make_bluestein(uint64_t n, uint64_t lines, uint64_t inner, int dev, bool dbl, bool forward,
+                              hipStream_t st)
+{
+    auto bp = std::make_unique<BluesteinPass>();
+    struct Guard {
+        std::unique_ptr<BluesteinPass>& p;
+        ~Guard()
+        {
+            if (!p) return;
+            SeamPlan::free_pass(p->conv);
+            (void)hipFree(p->work);
+            (void)hipFree(p->chirp);
+            (void)hipFree(p->bhat);
+        }
+    } guard{bp};
+    bp->n = n;
+    bp->lines = lines;
+    bp->inner = inner;
+    bp->prec = dbl ? 0 : 1;
+    uint64_t m = 1;
+    uint32_t l2 = 0;
+    while (m < 2 * n - 1) {
+        m <<= 1;
+        ++l2;
+    }
+    bp->m = m;
+    check(hipMalloc(&bp->work, 2 * sizeof(double) * lines * m), "hipMalloc(fft work)");
+    check(hipMalloc(&bp->chirp, 2 * sizeof(double) * n), "hipMalloc(chirp)");
+    check(hipMalloc(&bp->bhat, 2 * sizeof(double) * m), "hipMalloc(chirp transform)");
+    std::vector<uint64_t> off(lines);
+    for (uint64_t li = 0; li < lines; ++li) off[li] = li * m;
+    bp->conv = make_batch(off, l2, 1, dev, true);
+    // the chirp, then b and its transform in complex128 (one signal of m points), scaled by 1/m
+    hipLaunchKernelGGL(chirp_kernel, aux_grid(n), dim3(kAuxThreads), 0, st, bp->chirp, n);
+    hipLaunchKernelGGL(chirp_b_kernel, aux_grid(m), dim3(kAuxThreads), 0, st, (const double*)bp->chirp, bp->bhat, n, m,
+                       forward ? 1.0 : -1.0);
+    std::unique_ptr<BatchPlan> one(make_batch(std::vector<uint64_t>{0}, l2, 1, dev, true));
+    launch(*one, bp->bhat, bp->bhat, 1, 3, st, 1.0);
+    hipLaunchKernelGGL(scale_kernel, aux_grid(2 * m), dim3(kAuxThreads), 0, st, bp->bhat, 2 * m, 1.0 / (double)m);
+    check(hipGetLastError(), "chirp kernels");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    SeamPlan::free_pass(one.release());
+    return bp.release();
+}
+
+// One strided-line pass along `ax` of a dense row-major complex `shape`: Stockham for powers of two up
+// to 8192, Bluestein for other lengths up to 4096; anything else is unsupported (2).
+Pass make_pass(const std::vector<int64_t>& shape, int ax, int dev, bool dbl, bool forward, hipStream_t st)
+{
+    const int ndim = (int)shape.size();
+    uint64_t total = 1;
+    for (int d = 0; d < ndim; ++d) total *= (uint64_t)shape[d];
+    const uint64_t n = (uint64_t)shape[ax];
+    uint64_t inner = 1;
+    for (int d = ax + 1; d < ndim; ++d) inner *= (uint64_t)shape[d];
+    if (inner > 0xFFFFFFFFull) fail("fft_plan", "inner extent too large");
+    const uint64_t lines = n ? total / n : 0;
+    Pass p;
+    const uint32_t l2 = log2_exact(n);
+    if (l2 != 0xFFFFFFFFu && l2 <= (uint32_t)kMaxLog2) {
+        std::vector<uint64_t> off(lines);
+        for (uint64_t li = 0; li < lines; ++li) {
+            const uint64_t o = li / inner, in = li % inner;
+            off[li] = o * n * inner + in;
+        }
+        p.stockham = make_batch(off, l2, inner, dev, dbl);
+        return p;
+    }
+    if (n == 0 || n > kMaxBluestein)
+        fail("fft_plan", "axis lengths must be powers of two up to 8192 or other lengths up to 4096", 0, 2);
+    if (lines) p.blue = make_bluestein(n, lines, inner, dev, dbl, forward, st);
+    return p;
+}
+
+template<class T>
+void run_bluestein(const BluesteinPass& b, const void* in, void* out, hipStream_t st, double scale, bool forward)
+{
+    const double sign = forward ? -1.0 : 1.0; // conj(c) forward
+    double* work = b.work;
+    hipLaunchKernelGGL(blue_pre_kernel<T>, aux_grid(b.lines * b.m), dim3(kAuxThreads), 0, st,
+                       static_cast<const T*>(in), work, (const double*)b.chirp, b.lines, b.n, b.m, b.inner, sign);
+    launch(*b.conv, work, work, 1, 3, st, 1.0);
+    hipLaunchKernelGGL(blue_mul_kernel, aux_grid(b.lines * b.m), dim3(kAuxThreads), 0, st, work,
+                       (const double*)b.bhat, b.lines, b.m);
+    launch(*b.conv, work, work, 0, 3, st, 1.0);
+    hipLaunchKernelGGL(blue_post_kernel<T>, aux_grid(b.lines * b.n), dim3(kAuxThreads), 0, st, (const double*)work,
+                       static_cast<T*>(out), (const double*)b.chirp, b.lines, b.n, b.m, b.inner, sign, scale);
+    check(hipGetLastError(), "bluestein kernels");
+}
+
+// One pass from `in` to `out` (in == out allowed) with the transform's direction and scale.
+void run_pass(const Pass& p, const void* in, void* out, int forward, double scale, hipStream_t st)
+{
+    if (p.stockham) {
+        launch(*p.stockham, in, out, forward, 3, st, scale);
+    } else if (p.blue) {
+        if (p.blue->prec == 0) run_bluestein<double>(*p.blue, in, out, st, scale, forward != 0);
+        else run_bluestein<float>(*p.blue, in, out, st, scale, forward != 0);
+    }
 }
 
 } // namespace
@@ -649,7 +858,7 @@ int seam_plan(int dev, int kind, int is_double, int norm, int forward, int ndim,
         else if ((sp->forward != 0) == (norm == 1)) sc = 1.0 / (double)points;
         if (kind == AETHER_FFT_C2C) {
             for (int a = 0; a < naxes; ++a) {
-                sp->passes.push_back(make_pass(real_shape, axes[a], dev, sp->dbl));
+                sp->passes.push_back(make_pass(real_shape, axes[a], dev, sp->dbl, sp->forward != 0, as_stream(stream)));
                 sp->scales.push_back(a + 1 == naxes ? sc : 1.0);
             }
         } else {
@@ -661,9 +870,9 @@ int seam_plan(int dev, int kind, int is_double, int norm, int forward, int ndim,
             for (int d = ax + 1; d < ndim; ++d) sp->inner *= (uint64_t)real_shape[d];
             // r2c: halved axis first, then the others (the last carries the scale);
             // c2r: the others first, then the halved axis with the scale
-            sp->line = make_pass(real_shape, ax, dev, sp->dbl);
+            sp->line = make_pass(real_shape, ax, dev, sp->dbl, sp->forward != 0, as_stream(stream));
             for (int a = 0; a + 1 < naxes; ++a) {
-                sp->passes.push_back(make_pass(half, axes[a], dev, sp->dbl));
+                sp->passes.push_back(make_pass(half, axes[a], dev, sp->dbl, sp->forward != 0, as_stream(stream)));
                 sp->scales.push_back(kind == AETHER_FFT_R2C && a + 2 == naxes ? sc : 1.0);
             }
             sp->line_scale = (kind == AETHER_FFT_C2R || naxes == 1) ? sc : 1.0;
@@ -693,22 +902,22 @@ void run_real(SeamPlan& sp, const void* in, void* out, hipStream_t st)
         hipLaunchKernelGGL(widen_kernel<T>, aux_grid(total_full), dim3(kAuxThreads), 0, st,
                            static_cast<const T*>(in), full, total_full);
         check(hipGetLastError(), "widen_kernel launch");
-        launch(*sp.line, full, full, 1, 3, st, sp.line_scale);
+        run_pass(sp.line, full, full, 1, sp.line_scale, st);
         hipLaunchKernelGGL(keep_half_kernel<T>, aux_grid(total_half), dim3(kAuxThreads), 0, st, full,
                            static_cast<T*>(out), sp.outer, sp.n, sp.inner);
         check(hipGetLastError(), "keep_half_kernel launch");
-        for (size_t i = 0; i < sp.passes.size(); ++i) launch(*sp.passes[i], out, out, 1, 3, st, sp.scales[i]);
+        for (size_t i = 0; i < sp.passes.size(); ++i) run_pass(sp.passes[i], out, out, 1, sp.scales[i], st);
         return;
     }
     const void* src = in;
     for (size_t i = 0; i < sp.passes.size(); ++i) {
-        launch(*sp.passes[i], i == 0 ? in : sp.half, sp.half, 0, 3, st, sp.scales[i]);
+        run_pass(sp.passes[i], i == 0 ? in : sp.half, sp.half, 0, sp.scales[i], st);
         src = sp.half;
     }
     hipLaunchKernelGGL(hermitian_kernel<T>, aux_grid(total_full), dim3(kAuxThreads), 0, st,
                        static_cast<const T*>(src), full, sp.outer, sp.n, sp.inner);
     check(hipGetLastError(), "hermitian_kernel launch");
-    launch(*sp.line, full, full, 0, 3, st, sp.line_scale);
+    run_pass(sp.line, full, full, 0, sp.line_scale, st);
     hipLaunchKernelGGL(real_part_kernel<T>, aux_grid(total_full), dim3(kAuxThreads), 0, st, full,
                        static_cast<T*>(out), total_full);
     check(hipGetLastError(), "real_part_kernel launch");
@@ -725,7 +934,7 @@ int seam_exec(void* plan, const void* in, void* out, void* stream)
         const hipStream_t st = as_stream(stream);
         if (sp->kind == AETHER_FFT_C2C) {
             for (size_t i = 0; i < sp->passes.size(); ++i)
-                launch(*sp->passes[i], i == 0 ? in : out, out, sp->forward, 3, st, sp->scales[i]);
+                run_pass(sp->passes[i], i == 0 ? in : out, out, sp->forward, sp->scales[i], st);
             return;
         }
         if (in == out) fail("fft_exec", "in == out is allowed for c2c only");
