@@ -33,9 +33,10 @@ enum { AETH_F64 = 0, AETH_F32 = 1 };
  * (dsc/src/seam.h:16-51), so the reference's loader (dsc/src/aether_core.cpp:
  * 95-113: dlopen, abi check, probe) binds this library unchanged. Kinds:
  * CPU, DEVICE (HIP device memory), PINNED (hipHostMalloc). FFT plans cover
- * C2C, R2C and C2R over axes of power-of-two length <= 8192 (other lengths
- * return 2); for R2C/C2R `shape` is the real side's and the last axis is
- * n/2 + 1 long on the complex side (dsc/src/cufft_plan.h:9-12). */
+ * C2C, R2C and C2R over axes of power-of-two length <= 8192 or any other
+ * length <= 4096 (Bluestein); longer axes return 2. For R2C/C2R `shape` is
+ * the real side's and the last axis is n/2 + 1 long on the complex side
+ * (dsc/src/cufft_plan.h:9-12). */
 #define AETHER_MEMORY_ABI 2
 enum { AETHER_MEM_CPU = 0, AETHER_MEM_DEVICE = 1, AETHER_MEM_PINNED = 2 };
 enum { AETHER_FFT_C2C = 0, AETHER_FFT_R2C = 1, AETHER_FFT_C2R = 2 };
