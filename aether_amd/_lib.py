@@ -35,6 +35,9 @@ SIGNATURES = {
     "aeth_fft_batch_plan": (_i32, [_i32, _u64, _vp, _vp, _vp, C.POINTER(_vp)]),
     "aeth_fft_batch_exec": (_i32, [_vp, _vp, _vp, _i32, _i32, _vp]),
     "aeth_fft_batch_destroy": (_i32, [_vp]),
+    "aeth_keyed_sum": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _u64, C.c_int64, _u64, _vp, _vp, _vp, _vp]),
+    "aeth_plane_add": (_i32, [_i32, _i32, _vp, _u64, _u64, _vp, _vp, _u64, _vp]),
+    "aeth_append_where": (_i32, [_i32, _vp, _u64, _f64, _vp, _u64, _vp, _vp, _i32, _vp]),
 }
 
 

@@ -275,6 +275,45 @@ int aeth_server_replay(void* server, const aeth_job* jobs, uint64_t count, aeth_
  * request). */
 int aeth_run_job_plain(const aeth_job* job, aeth_job_result* out, void* stream);
 
+/* ------------------------------------------------------------------ accumulation */
+/* The device primitives are in include/aether/device/accum.hpp for HIP kernels of a caller's own;
+ * these entries run them over whole arrays, one lane per element. */
+
+/* Keyed sum: target[keys[l]] += terms[l] for l < n, with ONE atomic per run of lanes of a wavefront
+ * that share a slot (accum/run.h:1-461; wave64 forms):
+ *   AETH_RUN_ALIGNED : the slot is equal over every aligned group of `width` lanes (2..64), given
+ *                      that lane l's own sample is l + own; the vote checks 64 consecutive own samples
+ *                      starting on a multiple of 64;
+ *   AETH_RUN_TESTED  : one vote checks that the full wave shares one slot;
+ *   AETH_RUN_SORTED  : keys are non-decreasing in lane order; each maximal run commits once.
+ * A wave that fails its vote, or is not full, commits lane by lane: correct for any keys. With
+ * `compensated`, each commit adds its two-sum residual into comp[key] (delivered: target + comp).
+ * Keys outside [0, slots) take no part. commits (device, may be NULL): (run, lane) commit counters,
+ * incremented atomically (diagnostics). dtype: AETH_F64 / AETH_F32 (terms, target, comp). */
+enum { AETH_RUN_ALIGNED = 0, AETH_RUN_TESTED = 1, AETH_RUN_SORTED = 2 };
+int aeth_keyed_sum(int dtype, int form, int width, int compensated, const int64_t* keys, const void* terms,
+                   uint64_t n, int64_t own, uint64_t slots, void* target, void* comp, unsigned long long* commits,
+                   void* stream);
+
+/* Accumulation plane (accum/AccumPlane.h:36-470): value[d*pitch + i] (+ comp) += terms[d*term_pitch + i]
+ * for i < n and d < vd (1, 2, 3, 4, 6 or 8 lanes), components in ascending order. COMPENSATED is one
+ * hardware atomic add plus one of its two-sum residual into comp (accum/atomic.h:120-137): safe for
+ * concurrent producers on any streams, the delivered value + comp independent of their order to within
+ * the residuals' own rounding. SERIALIZED is a plain read-add-write (comp unused): the caller orders
+ * the producers of a sample. */
+enum { AETH_PLANE_COMPENSATED = 0, AETH_PLANE_SERIALIZED = 1 };
+int aeth_plane_add(int policy, int vd, const double* terms, uint64_t term_pitch, uint64_t n, double* value,
+                   double* comp, uint64_t pitch, void* stream);
+
+/* Append counter (accum/AppendCounter.h:30-192): every i < n with x[i] > threshold claims a slot of
+ * `out` (capacity places) from the device counter and writes i there; a claim past capacity sets
+ * *overflow = 1 and writes nothing (the counter itself keeps the raw count). per_wave: one atomic per
+ * wavefront (its claimants take consecutive slots in lane order) instead of one per claim. counter
+ * and overflow are device memory the caller zeroes between steps; which slot an element gets is not
+ * deterministic across waves. */
+int aeth_append_where(int dtype, const void* x, uint64_t n, double threshold, uint32_t* out, uint64_t capacity,
+                      uint32_t* counter, uint32_t* overflow, int per_wave, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
