@@ -26,6 +26,7 @@
 // like the reference's host route.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstddef>
@@ -737,6 +738,167 @@ std::array<double, N> evaluate(View<T, N> dest, const E& e, SumMode mode = SumMo
     dest.assign(e, mode, sums.data(), s);
     return sums;
 }
+
+// ---------------------------------------------------------------- accumulation
+namespace accum {
+
+inline constexpr std::uint32_t npos = 0xFFFFFFFFu; // accum/AppendCounter.h:26-29
+
+enum class Policy { CompensatedAtomic = AETH_PLANE_COMPENSATED, Serialized = AETH_PLANE_SERIALIZED };
+
+namespace detail {
+inline void zero_device(void* p, std::size_t bytes, Stream s)
+{
+    if (!bytes) return;
+    const std::uint32_t z = 0;
+    aether::detail::check(aether::detail::seam().fill(AETHER_MEM_DEVICE, aether::detail::current_device(), p, bytes,
+                                                      &z, sizeof z, s),
+                          "fill");
+}
+} // namespace detail
+
+/**
+ * Plane<VD> (accum/AccumPlane.h:405-470): per-sample accumulator of VD lanes of double with a
+ * companion residual lane. add() folds a term array into every sample with the compensated atomic
+ * (concurrent producers on any streams) or a plain add (Serialized); delivered() writes value + comp.
+ * Device kernels of a caller's own use aether::accum::PlaneView (include/aether/device/accum.hpp) on
+ * valueView()/compView().
+ */
+template<std::size_t VD = 3, Policy P = Policy::CompensatedAtomic>
+class Plane {
+public:
+    explicit Plane(std::size_t n) : value_(n), comp_(P == Policy::CompensatedAtomic ? n : 0) { zeroDeviceAsync(); }
+    std::size_t samples() const { return value_.samples(); }
+    View<double, VD> valueView() const { return value_.deviceView(); }
+    View<double, VD> compView() const { return comp_.deviceView(); }
+    /** Clear both lanes of the whole allocation, stream-ordered. */
+    void zeroDeviceAsync(Stream s = nullptr)
+    {
+        detail::zero_device(value_.deviceView().data(), VD * value_.capacity() * sizeof(double), s);
+        detail::zero_device(comp_.deviceView().data(), VD * comp_.capacity() * sizeof(double), s);
+    }
+    /** plane[i] += terms[i] for every sample, components in ascending order. */
+    void add(const Array<double, VD>& terms, Stream s = nullptr)
+    {
+        if (terms.samples() != samples()) throw Error("AccumPlane::add: terms and plane differ in samples");
+        aether::detail::check(aeth_plane_add((int)P, (int)VD, terms.deviceView().data(), terms.capacity(), samples(),
+                                             value_.deviceView().data(),
+                                             P == Policy::CompensatedAtomic ? comp_.deviceView().data() : nullptr,
+                                             value_.capacity(), s),
+                              "aeth_plane_add");
+    }
+    /** out = value + comp (one kernel). */
+    void delivered(Array<double, VD>& out, Stream s = nullptr) const
+    {
+        if (out.samples() != samples()) throw Error("AccumPlane::delivered: output and plane differ in samples");
+        if constexpr (P == Policy::CompensatedAtomic) out.deviceView().assign(valueView() + compView(), SumMode::None, nullptr, s);
+        else out.deviceView().assign(valueView() * 1.0, SumMode::None, nullptr, s);
+    }
+
+private:
+    Array<double, VD> value_;
+    Array<double, VD> comp_;
+};
+
+/** A device slot counter over `capacity` places with an overflow flag (accum/AppendCounter.h:82-192). */
+class AppendCounter {
+public:
+    explicit AppendCounter(std::size_t capacity) : capacity_(capacity)
+    {
+        const int dev = aether::detail::current_device();
+        aether::detail::check(aether::detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, 32, nullptr, &chunk_, &mem_),
+                              "AppendCounter allocation");
+        reset(true);
+        aether::detail::check(aether::detail::seam().stream_sync(dev, nullptr), "AppendCounter");
+    }
+    AppendCounter(const AppendCounter&) = delete;
+    AppendCounter& operator=(const AppendCounter&) = delete;
+    ~AppendCounter()
+    {
+        if (chunk_) aether::detail::seam().chunk_release(chunk_);
+    }
+    std::size_t capacity() const { return capacity_; }
+    std::uint32_t* counter() const { return static_cast<std::uint32_t*>(mem_); }
+    std::uint32_t* overflowFlag() const { return static_cast<std::uint32_t*>(mem_) + 4; }
+    /** The raw count (may exceed capacity: every overflowing claim still incremented it). */
+    std::size_t rawCount() const { return read(0); }
+    bool overflowed() const { return read(4) != 0; }
+    /** Zero the counter (and the overflow flag when asked), stream-ordered. */
+    void reset(bool clearOverflow = false, Stream s = nullptr)
+    {
+        detail::zero_device(mem_, clearOverflow ? 32 : 16, s);
+    }
+    /** Throw if the flag is set, and clear it. */
+    void checkOverflow(const std::string& context = "AppendCounter::checkOverflow")
+    {
+        if (!overflowed()) return;
+        detail::zero_device(overflowFlag(), 16, nullptr);
+        throw Error(context + ": more claims than capacity " + std::to_string(capacity_));
+    }
+    /** Adopt the claimed slots into `arr` (clamped to its spare capacity) and reset for the next step. */
+    template<class T, std::size_t N>
+    std::size_t commit(Array<T, N>& arr)
+    {
+        const std::size_t k = std::min(std::min(rawCount(), capacity_), arr.capacity() - arr.samples());
+        arr.commit(k);
+        reset();
+        return k;
+    }
+
+private:
+    std::size_t read(int word) const
+    {
+        std::uint32_t v = 0;
+        const int dev = aether::detail::current_device();
+        aether::detail::check(aether::detail::seam().copy(AETHER_MEM_CPU, dev, &v, AETHER_MEM_DEVICE, dev,
+                                                          static_cast<std::uint32_t*>(mem_) + word, sizeof v, nullptr),
+                              "AppendCounter read");
+        aether::detail::check(aether::detail::seam().stream_sync(dev, nullptr), "AppendCounter read");
+        return v;
+    }
+    std::size_t capacity_;
+    void* chunk_ = nullptr;
+    void* mem_ = nullptr;
+};
+
+/**
+ * Append the index of every sample of `x` above `threshold` into the spare tail of `out` (its device
+ * copy), claiming slots from `counter` (whose capacity must not exceed out's spare capacity); one
+ * atomic per wavefront. Commit with counter.commit(out) after download or device use.
+ */
+template<class T>
+void appendWhere(View<T, 1> x, double threshold, AppendCounter& counter, Array<std::uint32_t>& out,
+                 Stream s = nullptr)
+{
+    if (counter.capacity() > out.capacity() - out.samples())
+        throw Error("appendWhere: the counter's capacity exceeds the array's spare capacity");
+    aether::detail::check(aeth_append_where(aether::detail::dtype_code<T>(), x.data(), x.samples(), threshold,
+                                            out.deviceSpare().data(), counter.capacity(), counter.counter(),
+                                            counter.overflowFlag(), 1, s),
+                          "aeth_append_where");
+}
+
+/** run forms of keyedSum (accum/run.h:1-40). */
+enum class Run { Aligned = AETH_RUN_ALIGNED, Tested = AETH_RUN_TESTED, Sorted = AETH_RUN_SORTED };
+
+/**
+ * target[keys[l]] += terms[l] for every lane with ONE atomic per run of a wavefront's lanes that
+ * share a slot; with `comp`, each commit is compensated (delivered: target + comp). All pointers are
+ * device memory.
+ */
+template<class T>
+void keyedSum(Run form, const std::int64_t* keys, const T* terms, std::size_t n, T* target, std::size_t slots,
+              T* comp = nullptr, int width = 64, std::int64_t own = 0, Stream s = nullptr)
+{
+    aether::detail::check(aeth_keyed_sum(aether::detail::dtype_code<T>(), (int)form, width, comp != nullptr ? 1 : 0,
+                                         keys, terms, n, own, slots, target, comp, nullptr, s),
+                          "aeth_keyed_sum");
+}
+
+} // namespace accum
+
+template<std::size_t VD = 3, accum::Policy P = accum::Policy::CompensatedAtomic>
+using AccumPlane = accum::Plane<VD, P>;
 
 // ---------------------------------------------------------------- random
 namespace random {

@@ -17,8 +17,15 @@
 //   append <n>                 : push_back / commit / clear on the host (text)
 //   float  <n> <seed>          : cfg0 in float with float-float sums; writes d, sums
 //   resize <n>                 : upload/reserve/resize/download round trip (text)
+//   plane  <n>                 : three producers into an AccumPlane<3>; max error of the delivered
+//                                sum in ulps of the exact one (text)
+//   where  <n> <threshold>     : appendWhere over Generator(5) uniforms into an Array<uint32_t>'s
+//                                spare capacity; checks the committed set on the host (text)
+//   keyed  <n> <run>           : sorted keyed sum of runs of `run` lanes, compensated; max error over
+//                                eps * sum|t| per slot (text)
 #include <aether/aether.hpp>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -213,6 +220,89 @@ int append(std::size_t n)
     return ok ? 0 : 1;
 }
 
+int plane(std::size_t n)
+{
+    aether::AccumPlane<3> p(n);
+    std::vector<Array<double, 3>> terms;
+    const aether::random::Generator g(11);
+    for (int k = 0; k < 3; ++k) {
+        terms.emplace_back(n);
+        terms.back().deviceView() = g.substream(k).normal<double, 3>(0.0, std::pow(10.0, k));
+        p.add(terms.back());
+    }
+    Array<double, 3> out(n);
+    p.delivered(out);
+    out.download();
+    for (auto& t : terms) t.download();
+    double worst = 0.0;
+    for (std::size_t c = 0; c < 3; ++c)
+        for (std::size_t i = 0; i < n; ++i) {
+            long double e = 0.0L;
+            for (auto& t : terms) e += (long double)t.hostView()(c, i);
+            const double ex = (double)e;
+            const double ulp = std::nextafter(std::fabs(ex), INFINITY) - std::fabs(ex);
+            worst = std::max(worst, std::fabs(out.hostView()(c, i) - ex) / ulp);
+        }
+    std::printf("plane %s worst_ulp %.3f\n", worst <= 1.0 ? "ok" : "BAD", worst);
+    return worst <= 1.0 ? 0 : 1;
+}
+
+int where(std::size_t n, double threshold)
+{
+    Array<double> x(n);
+    x.deviceView() = aether::random::Generator(5).uniform<double, 1>(0.0, 1.0);
+    Array<std::uint32_t> out(0);
+    out.reserve(n);
+    aether::accum::AppendCounter ctr(out.capacity());
+    aether::accum::appendWhere(x.deviceView(), threshold, ctr, out);
+    const std::size_t raw = ctr.rawCount();
+    const std::size_t k = ctr.commit(out);
+    out.download();
+    x.download();
+    std::vector<char> seen(n, 0);
+    bool ok = k == raw && out.samples() == k && !ctr.overflowed();
+    for (std::size_t j = 0; j < k && ok; ++j) {
+        const std::uint32_t i = out.hostView()(0, j);
+        ok = i < n && !seen[i] && x.hostView()(0, i) > threshold;
+        if (ok) seen[i] = 1;
+    }
+    std::size_t expect = 0;
+    for (std::size_t i = 0; i < n; ++i) expect += x.hostView()(0, i) > threshold;
+    ok = ok && expect == k;
+    std::printf("where %s count %zu\n", ok ? "ok" : "BAD", k);
+    return ok ? 0 : 1;
+}
+
+int keyed(std::size_t n, std::size_t run)
+{
+    Array<double> terms(n), target(n / run + 1), comp(n / run + 1);
+    Array<std::int64_t> keys(n);
+    terms.deviceView() = aether::random::Generator(9).uniform<double, 1>(-1.0, 1.0);
+    for (std::size_t i = 0; i < n; ++i) keys.hostView()(0, i) = (std::int64_t)(i / run);
+    keys.upload();
+    const std::size_t slots = n / run + 1;
+    aether::accum::detail::zero_device(target.deviceView().data(), target.capacity() * 8, nullptr);
+    aether::accum::detail::zero_device(comp.deviceView().data(), comp.capacity() * 8, nullptr);
+    aether::accum::keyedSum(aether::accum::Run::Sorted, keys.deviceView().data(), terms.deviceView().data(), n,
+                            target.deviceView().data(), slots, comp.deviceView().data());
+    target.download();
+    comp.download();
+    terms.download();
+    double worst = 0.0;
+    for (std::size_t k = 0; k < slots; ++k) {
+        long double e = 0.0L, mass = 0.0L;
+        for (std::size_t i = k * run; i < std::min(n, (k + 1) * run); ++i) {
+            e += (long double)terms.hostView()(0, i);
+            mass += std::fabs((long double)terms.hostView()(0, i));
+        }
+        if (mass == 0.0L) continue;
+        const long double got = (long double)target.hostView()(0, k) + (long double)comp.hostView()(0, k);
+        worst = std::max(worst, (double)(std::fabs(got - e) / (mass * 0x1p-52L)));
+    }
+    std::printf("keyed %s worst %.3f\n", worst <= 1.0 ? "ok" : "BAD", worst);
+    return worst <= 1.0 ? 0 : 1;
+}
+
 int flt(std::size_t n, std::uint64_t seed)
 {
     Array<float, 3> a(n), b(n), c(n), d(n);
@@ -261,6 +351,9 @@ int main(int argc, char** argv)
         if (mode == "float" && argc == 4) return flt(u(2), u(3));
         if (mode == "rfft" && argc == 4) return rfft(u(2), u(3));
         if (mode == "append" && argc == 3) return append(u(2));
+        if (mode == "plane" && argc == 3) return plane(u(2));
+        if (mode == "where" && argc == 4) return where(u(2), std::strtod(argv[3], nullptr));
+        if (mode == "keyed" && argc == 4) return keyed(u(2), u(3));
         if (mode == "resize" && argc == 3) return resize(u(2));
     } catch (const aether::Error& e) {
         std::fprintf(stderr, "aether error: %s\n", e.what());

@@ -168,3 +168,18 @@ def test_rfft_round_trip(dev, rows, n):
 @pytest.mark.gpu
 def test_append_commit_clear(dev):
     assert run("append", 100).decode().startswith("append ok")
+
+
+@pytest.mark.gpu
+def test_accum_plane_delivers_the_exact_sum(dev):
+    assert run("plane", 100_003).decode().startswith("plane ok")
+
+
+@pytest.mark.gpu
+def test_append_where_commits_the_selected_samples(dev):
+    assert run("where", 300_007, 0.75).decode().startswith("where ok")
+
+
+@pytest.mark.gpu
+def test_keyed_sum_compensated_within_one_eps(dev):
+    assert run("keyed", 1 << 18, 777).decode().startswith("keyed ok")
