@@ -148,11 +148,15 @@ class AppendCounter:
         return k
 
 
+CLAIMS = {"lane": 0, "wave": 1, "tile": 2}
+
+
 def append_where(x: torch.Tensor, threshold: float, counter: AppendCounter, out: torch.Tensor,
-                 per_wave: bool = True, stream=None) -> None:
-    """Append the index of every element of x above `threshold` to `out` (int32, counter.capacity places)."""
+                 claim: str = "tile", stream=None) -> None:
+    """Append the index of every element of x above `threshold` to `out` (int32, counter.capacity places).
+    claim: one atomic per "lane" (claim), per "wave" or per "tile" of 16384 elements."""
     if out.dtype != torch.int32 or out.numel() < counter.capacity or not out.is_cuda:
         raise AetherError(1, "out is an int32 device tensor with the counter's capacity")
     x = x.contiguous()
     call("aeth_append_where", _code(x), x.data_ptr(), x.numel(), float(threshold), out.data_ptr(), counter.capacity,
-         counter.counter_ptr, counter.overflow_ptr, int(bool(per_wave)), stream_handle(stream))
+         counter.counter_ptr, counter.overflow_ptr, CLAIMS[claim], stream_handle(stream))

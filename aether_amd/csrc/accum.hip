@@ -124,6 +124,67 @@ __global__ __launch_bounds__(kLanes) void append_where_kernel(const T* __restric
     if (slot != aether::accum::npos) out[slot] = (uint32_t)i;
 }
 
+// One atomic per tile of 16384 elements: every wave ballots its 64 rounds into LDS, wave 0 scans the
+// 256 (round, wave) counts and claims the tile's total with one atomic, and each claimant writes at
+// base + (claims before it in round-major, wave, lane order): a tile's elements keep their order.
+constexpr int kTileRounds = 64;
+constexpr int kWaves = kLanes / 64;
+
+template<class T>
+__global__ __launch_bounds__(kLanes) void append_tile_kernel(const T* __restrict__ x, uint64_t n, double threshold,
+                                                             uint32_t* __restrict__ out,
+                                                             aether::accum::AppendCounterView v)
+{
+    __shared__ uint64_t ball[kTileRounds * kWaves];
+    __shared__ uint32_t before[kTileRounds * kWaves];
+    __shared__ uint32_t base_s;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+    const uint64_t tile0 = (uint64_t)blockIdx.x * (kLanes * kTileRounds);
+#pragma unroll 8
+    for (int r = 0; r < kTileRounds; ++r) {
+        const uint64_t i = tile0 + (uint64_t)r * kLanes + threadIdx.x;
+        const bool want = i < n && (double)x[i] > threshold;
+        const uint64_t b = __ballot(want);
+        if (lane == 0) ball[r * kWaves + wave] = b;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // lane l owns entries 4l .. 4l+3 (entry = round * kWaves + wave)
+        uint32_t c[kWaves], own = 0;
+#pragma unroll
+        for (int k = 0; k < kWaves; ++k) {
+            c[k] = (uint32_t)__popcll(ball[lane * kWaves + k]);
+            own += c[k];
+        }
+        uint32_t incl = own; // inclusive scan of the lanes' totals
+#pragma unroll
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        uint32_t run = incl - own;
+#pragma unroll
+        for (int k = 0; k < kWaves; ++k) {
+            before[lane * kWaves + k] = run;
+            run += c[k];
+        }
+        if (lane == 63) base_s = incl ? atomicAdd(v.counter, incl) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = base_s;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    bool over = false;
+    for (int r = 0; r < kTileRounds; ++r) {
+        const uint64_t b = ball[r * kWaves + wave];
+        if ((b >> lane) & 1ull) {
+            const uint32_t slot = base + before[r * kWaves + wave] + (uint32_t)__popcll(b & below);
+            if ((uint64_t)slot < v.capacity && slot >= base) out[slot] = (uint32_t)(tile0 + (uint64_t)r * kLanes + threadIdx.x);
+            else over = true;
+        }
+    }
+    if (over) *v.overflow = 1u;
+}
+
 } // namespace
 } // namespace aeth
 
@@ -167,26 +228,30 @@ extern "C" int aeth_plane_add(int policy, int vd, const double* terms, uint64_t 
 }
 
 extern "C" int aeth_append_where(int dtype, const void* x, uint64_t n, double threshold, uint32_t* out,
-                                 uint64_t capacity, uint32_t* counter, uint32_t* overflow, int per_wave, void* stream)
+                                 uint64_t capacity, uint32_t* counter, uint32_t* overflow, int claim, void* stream)
 {
     return guarded([&] {
         if (!counter || !overflow) fail("aeth_append_where", "counter and overflow flag are required");
         if (n == 0) return;
         if (n > 0xFFFFFFFFull) fail("aeth_append_where", "indices are 32-bit");
         const aether::accum::AppendCounterView v{counter, capacity, overflow};
+        if (claim < AETH_CLAIM_LANE || claim > AETH_CLAIM_TILE) fail("aeth_append_where", "unknown claim granularity");
         hipStream_t st = as_stream(stream);
         const dim3 g = lanes_grid(n);
-        if (dtype == AETH_F64) {
-            auto X = static_cast<const double*>(x);
-            if (per_wave) hipLaunchKernelGGL((append_where_kernel<double, true>), g, dim3(kLanes), 0, st, X, n, threshold, out, v);
-            else hipLaunchKernelGGL((append_where_kernel<double, false>), g, dim3(kLanes), 0, st, X, n, threshold, out, v);
-        } else if (dtype == AETH_F32) {
-            auto X = static_cast<const float*>(x);
-            if (per_wave) hipLaunchKernelGGL((append_where_kernel<float, true>), g, dim3(kLanes), 0, st, X, n, threshold, out, v);
-            else hipLaunchKernelGGL((append_where_kernel<float, false>), g, dim3(kLanes), 0, st, X, n, threshold, out, v);
-        } else {
-            fail("aeth_append_where", "unknown dtype");
-        }
+        const dim3 gt((unsigned)((n + kLanes * kTileRounds - 1) / (kLanes * kTileRounds)));
+        auto go = [&](auto tag) {
+            using T = decltype(tag);
+            auto X = static_cast<const T*>(x);
+            if (claim == AETH_CLAIM_TILE)
+                hipLaunchKernelGGL((append_tile_kernel<T>), gt, dim3(kLanes), 0, st, X, n, threshold, out, v);
+            else if (claim == AETH_CLAIM_WAVE)
+                hipLaunchKernelGGL((append_where_kernel<T, true>), g, dim3(kLanes), 0, st, X, n, threshold, out, v);
+            else
+                hipLaunchKernelGGL((append_where_kernel<T, false>), g, dim3(kLanes), 0, st, X, n, threshold, out, v);
+        };
+        if (dtype == AETH_F64) go(double{});
+        else if (dtype == AETH_F32) go(float{});
+        else fail("aeth_append_where", "unknown dtype");
         check(hipGetLastError(), "append_where_kernel launch");
     });
 }

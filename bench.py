@@ -246,6 +246,67 @@ def wl_trig(log2n: int = 26) -> dict:
     return res
 
 
+def wl_accum() -> dict:
+    """Accumulation: a sorted keyed sum (one atomic per run of a wave's lanes), a compensated plane fed by
+    four concurrent producers, and append-where with per-wave and per-claim atomics."""
+    import torch
+
+    import aether_amd as ae
+    from aether_amd import accum as A
+
+    out = {}
+    n, run = 1 << 26, 256
+    keys = torch.arange(n, dtype=torch.int64, device="cuda") // run
+    terms = torch.empty(n, dtype=torch.float64, device="cuda")
+    ae.rng_fill(ae.RNG_UNIFORM_F64, 17, terms)
+    slots = n // run
+    tg, cp, _ = A.keyed_sum(keys, terms, slots, "sorted", compensated=True)
+    for comp in (False, True):
+        def go():
+            A.keyed_sum(keys, terms, slots, "sorted", compensated=comp, target=tg, comp=cp)
+        ms = _time(go)
+        out[f"keyed_sum_sorted_{'compensated' if comp else 'plain'}_2^26"] = {
+            "ms": round(ms, 4), "GB/s": _gbs(16 * n, ms), "Glanes/s": round(n / ms / 1e6, 2)}
+    del keys, terms, tg, cp
+    torch.cuda.empty_cache()
+    m, vd = 1 << 24, 3
+    plane = A.Plane(m, vd)
+    prods = []
+    for k in range(4):
+        x = ae.Array(m, vd, torch.float64)
+        ae.array_fill_uniform(x, 300 + k, -1.0, 1.0)
+        prods.append(x)
+    streams = [torch.cuda.Stream() for _ in range(4)]
+    torch.cuda.synchronize()
+
+    def feed():
+        for k, x in enumerate(prods):
+            streams[k].wait_stream(torch.cuda.current_stream())
+            plane.add(x, stream=streams[k])
+        for st in streams:
+            torch.cuda.current_stream().wait_stream(st)
+
+    ms = _time(feed, reps=5)
+    # per producer and sample: the term (24 B) read, value and comp (2 x 24 B) read and written
+    out["plane_compensated_4_producers_2^24x3"] = {"ms": round(ms, 4), "GB/s": _gbs(4 * m * 120, ms)}
+    del plane, prods
+    torch.cuda.empty_cache()
+    n = 1 << 27
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    ae.rng_fill(ae.RNG_UNIFORM_F64, 23, x)
+    ctr = A.AppendCounter(n // 5)
+    dst = torch.empty(ctr.capacity, dtype=torch.int32, device="cuda")
+    for claim in ("tile", "wave", "lane"):
+        def go():
+            ctr.reset()
+            A.append_where(x, 0.9, ctr, dst, claim)
+        ms = _time(go)
+        kept = ctr.raw_count()
+        out[f"append_where_per_{claim}_2^27"] = {
+            "ms": round(ms, 4), "GB/s": _gbs(8 * n + 4 * kept, ms), "appended": kept}
+    return out
+
+
 def wl_fft() -> dict:
     """4096 complex signals, lengths uniform over 2^6..2^13 (Philox seed 7), entries N(0,1):
     forward then inverse, one launch for the whole batch vs one launch per length."""
@@ -496,6 +557,8 @@ def main():
         extra["trig_stress_f32_2^26"] = wl_trig()
         torch.cuda.empty_cache()
         extra["fft_batch_4096"] = wl_fft()
+        torch.cuda.empty_cache()
+        extra["accumulation"] = wl_accum()
         torch.cuda.empty_cache()
         extra["serving_mixed_stream"] = wl_serve(args.serve_seconds)
         line["workloads"] = extra

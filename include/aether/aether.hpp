@@ -864,7 +864,7 @@ private:
 /**
  * Append the index of every sample of `x` above `threshold` into the spare tail of `out` (its device
  * copy), claiming slots from `counter` (whose capacity must not exceed out's spare capacity); one
- * atomic per wavefront. Commit with counter.commit(out) after download or device use.
+ * atomic per tile of 16384 samples, which keep their order. Commit with counter.commit(out).
  */
 template<class T>
 void appendWhere(View<T, 1> x, double threshold, AppendCounter& counter, Array<std::uint32_t>& out,
@@ -874,7 +874,7 @@ void appendWhere(View<T, 1> x, double threshold, AppendCounter& counter, Array<s
         throw Error("appendWhere: the counter's capacity exceeds the array's spare capacity");
     aether::detail::check(aeth_append_where(aether::detail::dtype_code<T>(), x.data(), x.samples(), threshold,
                                             out.deviceSpare().data(), counter.capacity(), counter.counter(),
-                                            counter.overflowFlag(), 1, s),
+                                            counter.overflowFlag(), AETH_CLAIM_TILE, s),
                           "aeth_append_where");
 }
 

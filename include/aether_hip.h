@@ -307,12 +307,14 @@ int aeth_plane_add(int policy, int vd, const double* terms, uint64_t term_pitch,
 
 /* Append counter (accum/AppendCounter.h:30-192): every i < n with x[i] > threshold claims a slot of
  * `out` (capacity places) from the device counter and writes i there; a claim past capacity sets
- * *overflow = 1 and writes nothing (the counter itself keeps the raw count). per_wave: one atomic per
- * wavefront (its claimants take consecutive slots in lane order) instead of one per claim. counter
- * and overflow are device memory the caller zeroes between steps; which slot an element gets is not
- * deterministic across waves. */
+ * *overflow = 1 and writes nothing (the counter itself keeps the raw count). claim: the granularity
+ * of the atomics on the counter -- LANE (one per claim, the reference's tryAppend), WAVE (one per
+ * wavefront; its claimants take consecutive slots in lane order) or TILE (one per block of 16384
+ * elements, which keep their order). counter and overflow are device memory the caller zeroes
+ * between steps; which slots a wave or tile gets depends on the order the atomics land. */
+enum { AETH_CLAIM_LANE = 0, AETH_CLAIM_WAVE = 1, AETH_CLAIM_TILE = 2 };
 int aeth_append_where(int dtype, const void* x, uint64_t n, double threshold, uint32_t* out, uint64_t capacity,
-                      uint32_t* counter, uint32_t* overflow, int per_wave, void* stream);
+                      uint32_t* counter, uint32_t* overflow, int claim, void* stream);
 
 #ifdef __cplusplus
 }

@@ -171,15 +171,15 @@ def test_serialized_plane_is_the_sequential_sum(dev):
     assert torch.count_nonzero(plane.value.data[:, :n]).item() == 0
 
 
-@pytest.mark.parametrize("per_wave", [True, False])
+@pytest.mark.parametrize("claim", ["tile", "wave", "lane"])
 @pytest.mark.parametrize("dt", [torch.float64, torch.float32])
-def test_append_where_hands_out_every_slot_once(dev, per_wave, dt):
+def test_append_where_hands_out_every_slot_once(dev, claim, dt):
     n = 1_000_003
     x = torch.rand(n, dtype=torch.float64, generator=torch.Generator().manual_seed(3)).to(dt).cuda()
     want = np.nonzero(x.cpu().numpy().astype(np.float64) > 0.9)[0]
     ctr = A.AppendCounter(200_000)
     out = torch.full((ctr.capacity,), -1, dtype=torch.int32, device="cuda")
-    A.append_where(x, 0.9, ctr, out, per_wave)
+    A.append_where(x, 0.9, ctr, out, claim)
     torch.cuda.synchronize()
     assert ctr.raw_count() == len(want) and not ctr.overflowed()
     k = ctr.commit()
@@ -188,13 +188,13 @@ def test_append_where_hands_out_every_slot_once(dev, per_wave, dt):
     assert ctr.raw_count() == 0  # commit reset the counter
 
 
-@pytest.mark.parametrize("per_wave", [True, False])
-def test_append_overflow_is_flagged_and_clamped(dev, per_wave):
+@pytest.mark.parametrize("claim", ["tile", "wave", "lane"])
+def test_append_overflow_is_flagged_and_clamped(dev, claim):
     n = 100_000
     x = torch.linspace(0.0, 1.0, n, dtype=torch.float64, device="cuda")
     ctr = A.AppendCounter(1000)
     out = torch.full((1000,), -1, dtype=torch.int32, device="cuda")
-    A.append_where(x, 0.5, ctr, out, per_wave)
+    A.append_where(x, 0.5, ctr, out, claim)
     torch.cuda.synchronize()
     total = int((x > 0.5).sum().item())
     assert ctr.raw_count() == total and ctr.overflowed()
@@ -204,3 +204,14 @@ def test_append_overflow_is_flagged_and_clamped(dev, per_wave):
     with pytest.raises(ae.AetherError, match="capacity"):
         ctr.check_overflow()
     assert not ctr.overflowed()
+
+
+def test_append_tile_keeps_order_within_a_tile(dev):
+    # one tile's claimants take consecutive slots in element order
+    x = torch.zeros(16384, dtype=torch.float64, device="cuda")
+    x[::3] = 1.0
+    ctr = A.AppendCounter(16384)
+    out = torch.full((16384,), -1, dtype=torch.int32, device="cuda")
+    A.append_where(x, 0.5, ctr, out, "tile")
+    k = ctr.commit()
+    assert np.array_equal(out[:k].cpu().numpy(), np.arange(0, 16384, 3))
