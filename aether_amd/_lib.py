@@ -38,6 +38,7 @@ SIGNATURES = {
     "aeth_keyed_sum": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _u64, C.c_int64, _u64, _vp, _vp, _vp, _vp]),
     "aeth_plane_add": (_i32, [_i32, _i32, _vp, _u64, _u64, _vp, _vp, _u64, _vp]),
     "aeth_append_where": (_i32, [_i32, _vp, _u64, _f64, _vp, _u64, _vp, _vp, _i32, _vp]),
+    "aeth_runtime_eval": (_i32, [_i32, _vp, _vp, _vp, _f64, _vp]),
 }
 
 

@@ -900,6 +900,96 @@ void keyedSum(Run form, const std::int64_t* keys, const T* terms, std::size_t n,
 template<std::size_t VD = 3, accum::Policy P = accum::Policy::CompensatedAtomic>
 using AccumPlane = accum::Plane<VD, P>;
 
+// ---------------------------------------------------------------- runtime views
+inline constexpr std::size_t RuntimeRankCap = AETH_RUNTIME_RANK_CAP; // view/RuntimeView.h:71
+
+/** RuntimeView (view/RuntimeView.h:77-120): untyped data, dtype, rank, extents, element strides. */
+struct RuntimeView {
+    aeth_runtime_view v{};
+    std::size_t size() const
+    {
+        std::size_t t = 1;
+        for (int k = 0; k < v.rank; ++k) t *= v.extents[k];
+        return t;
+    }
+    int rank() const { return v.rank; }
+};
+
+/** A RuntimeView of one component plane of a View: rank 1 over the samples. */
+template<class T, std::size_t N>
+RuntimeView fromView(const View<T, N>& view, std::size_t component = 0)
+{
+    RuntimeView r;
+    r.v.data = view.data() + component * view.pitch();
+    r.v.dtype = detail::dtype_code<T>();
+    r.v.rank = 1;
+    r.v.extents[0] = (std::uint32_t)view.samples();
+    r.v.strides[0] = 1;
+    return r;
+}
+
+/** A RuntimeView over `shape` (row-major) with explicit element strides; strides empty = dense. */
+template<class T>
+RuntimeView makeRuntimeView(T* data, std::span<const std::size_t> shape, std::span<const std::int64_t> strides = {})
+{
+    if (shape.size() > RuntimeRankCap) throw Error("RuntimeView: rank exceeds " + std::to_string(RuntimeRankCap));
+    RuntimeView r;
+    r.v.data = const_cast<std::remove_const_t<T>*>(data);
+    r.v.dtype = detail::dtype_code<std::remove_const_t<T>>();
+    r.v.rank = (int)shape.size();
+    std::int64_t dense = 1;
+    for (std::size_t k = shape.size(); k-- > 0;) {
+        r.v.extents[k] = (std::uint32_t)shape[k];
+        r.v.strides[k] = strides.empty() ? dense : strides[k];
+        dense *= (std::int64_t)shape[k];
+    }
+    return r;
+}
+
+namespace interop {
+/**
+ * fromDLPack (interop/DLPack.h:221-260) for any DLTensor-shaped struct (the DLPack field names: data,
+ * device.device_type, ndim, dtype.{code,bits,lanes}, shape, strides, byte_offset): float32/float64,
+ * one lane, zero byte_offset, device memory (ROCm or CUDA device types); strides NULL = dense.
+ */
+template<class DLT>
+RuntimeView fromDLPack(const DLT& t)
+{
+    if (t.dtype.code != 2 || t.dtype.lanes != 1 || (t.dtype.bits != 32 && t.dtype.bits != 64))
+        throw Error("aether::interop::fromDLPack: float32/float64 single-lane tensors only");
+    if (t.byte_offset != 0) throw Error("aether::interop::fromDLPack: byte_offset != 0 is not supported");
+    if (t.ndim < 0 || (std::size_t)t.ndim > RuntimeRankCap) throw Error("aether::interop::fromDLPack: rank exceeds the cap");
+    const int dt = (int)t.device.device_type;
+    if (dt != 10 && dt != 2 && dt != 13) // kDLROCM, kDLCUDA, kDLCUDAManaged
+        throw Error("aether::interop::fromDLPack: not device memory");
+    RuntimeView r;
+    r.v.data = t.data;
+    r.v.dtype = t.dtype.bits == 64 ? AETH_F64 : AETH_F32;
+    r.v.rank = t.ndim;
+    std::int64_t dense = 1;
+    for (int k = t.ndim; k-- > 0;) {
+        r.v.extents[k] = (std::uint32_t)t.shape[k];
+        r.v.strides[k] = t.strides ? (std::int64_t)t.strides[k] : dense;
+        dense *= (std::int64_t)t.shape[k];
+    }
+    return r;
+}
+} // namespace interop
+
+namespace eval {
+
+enum class RuntimeOp : int { Assign = AETH_RT_ASSIGN, Add = AETH_RT_ADD, Sub = AETH_RT_SUB, Scale = AETH_RT_SCALE,
+                             AddScaled = AETH_RT_ADD_SCALED, SubScaled = AETH_RT_SUB_SCALED };
+
+/** runtimeEval (eval/Runtime.h:250-300): validates shapes and dtypes, then one kernel on the device. */
+inline void runtimeEval(RuntimeOp op, const RuntimeView& a, const RuntimeView& b, const RuntimeView& out,
+                        double scalar = 0.0, Stream s = nullptr)
+{
+    aether::detail::check(aeth_runtime_eval((int)op, &a.v, &b.v, &out.v, scalar, s), "aether::eval::runtimeEval");
+}
+
+} // namespace eval
+
 // ---------------------------------------------------------------- random
 namespace random {
 

@@ -316,6 +316,30 @@ enum { AETH_CLAIM_LANE = 0, AETH_CLAIM_WAVE = 1, AETH_CLAIM_TILE = 2 };
 int aeth_append_where(int dtype, const void* x, uint64_t n, double threshold, uint32_t* out, uint64_t capacity,
                       uint32_t* counter, uint32_t* overflow, int claim, void* stream);
 
+/* ------------------------------------------------------------------ runtime evaluator */
+/* eval::runtimeEval over RuntimeViews (eval/Runtime.h:1-342, view/RuntimeView.h:71-120): elementwise
+ * ops over strided N-d views of device memory, each operand addressed through its own element
+ * strides over the output's logical row-major shape (rank 0..6; a rank-0 view is one element):
+ *   ASSIGN     out = a          (the one op that may cast double <-> float)
+ *   ADD / SUB  out = a +- b
+ *   SCALE      out = s * a
+ *   ADD_SCALED / SUB_SCALED   out = a +- s * b   (s rounded to the operands' type; not contracted)
+ * Shapes must agree; dtypes (AETH_F64 / AETH_F32) must agree except for ASSIGN. Dense row-major
+ * operands run 16-byte lanes; other layouts the general strided kernel. Strides are in elements,
+ * non-negative. */
+#define AETH_RUNTIME_RANK_CAP 6
+typedef struct {
+    void* data;
+    int32_t dtype;
+    int32_t rank;
+    uint32_t extents[AETH_RUNTIME_RANK_CAP];
+    int64_t strides[AETH_RUNTIME_RANK_CAP];
+} aeth_runtime_view;
+enum { AETH_RT_ASSIGN = 0, AETH_RT_ADD = 1, AETH_RT_SUB = 2, AETH_RT_SCALE = 3, AETH_RT_ADD_SCALED = 4,
+       AETH_RT_SUB_SCALED = 5 };
+int aeth_runtime_eval(int op, const aeth_runtime_view* a, const aeth_runtime_view* b, const aeth_runtime_view* out,
+                      double scalar, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,8 @@
 //                                spare capacity; checks the committed set on the host (text)
 //   keyed  <n> <run>           : sorted keyed sum of runs of `run` lanes, compensated; max error over
 //                                eps * sum|t| per slot (text)
+//   runtime <rows> <cols>      : runtimeEval AddScaled of a transposed and a dense view, and an
+//                                Assign cast to float, checked on the host (text)
 #include <aether/aether.hpp>
 
 #include <cmath>
@@ -303,6 +305,43 @@ int keyed(std::size_t n, std::size_t run)
     return worst <= 1.0 ? 0 : 1;
 }
 
+int runtime(std::size_t rows, std::size_t cols)
+{
+    const std::size_t n = rows * cols;
+    Array<double> a(n), b(n), out(n);
+    Array<float> f(n);
+    a.deviceView() = aether::random::Generator(3).uniform<double, 1>(-1.0, 1.0);
+    b.deviceView() = aether::random::Generator(4).uniform<double, 1>(-1.0, 1.0);
+    // a read as the transpose of a (cols x rows) matrix: shape (rows, cols), strides (1, rows)
+    const std::size_t shape[2] = {rows, cols};
+    const std::int64_t tstr[2] = {1, (std::int64_t)rows};
+    const auto va = aether::makeRuntimeView<double>(a.deviceView().data(), shape, tstr);
+    const auto vb = aether::makeRuntimeView<double>(b.deviceView().data(), shape);
+    const auto vo = aether::makeRuntimeView<double>(out.deviceView().data(), shape);
+    aether::eval::runtimeEval(aether::eval::RuntimeOp::AddScaled, va, vb, vo, 0.25);
+    aether::eval::runtimeEval(aether::eval::RuntimeOp::Assign, vo, aether::RuntimeView{},
+                              aether::makeRuntimeView<float>(f.deviceView().data(), shape));
+    a.download();
+    b.download();
+    out.download();
+    f.download();
+    bool ok = true;
+    for (std::size_t i = 0; i < rows && ok; ++i)
+        for (std::size_t j = 0; j < cols && ok; ++j) {
+            const double want = a.hostView()(0, j * rows + i) + 0.25 * b.hostView()(0, i * cols + j);
+            ok = out.hostView()(0, i * cols + j) == want && f.hostView()(0, i * cols + j) == (float)want;
+        }
+    bool threw = false;
+    try {
+        aether::eval::runtimeEval(aether::eval::RuntimeOp::Add, va, vb,
+                                  aether::makeRuntimeView<float>(f.deviceView().data(), shape));
+    } catch (const aether::Error&) {
+        threw = true;
+    }
+    std::printf("runtime %s\n", ok && threw ? "ok" : "BAD");
+    return ok && threw ? 0 : 1;
+}
+
 int flt(std::size_t n, std::uint64_t seed)
 {
     Array<float, 3> a(n), b(n), c(n), d(n);
@@ -354,6 +393,7 @@ int main(int argc, char** argv)
         if (mode == "plane" && argc == 3) return plane(u(2));
         if (mode == "where" && argc == 4) return where(u(2), std::strtod(argv[3], nullptr));
         if (mode == "keyed" && argc == 4) return keyed(u(2), u(3));
+        if (mode == "runtime" && argc == 4) return runtime(u(2), u(3));
         if (mode == "resize" && argc == 3) return resize(u(2));
     } catch (const aether::Error& e) {
         std::fprintf(stderr, "aether error: %s\n", e.what());

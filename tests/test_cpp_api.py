@@ -183,3 +183,8 @@ def test_append_where_commits_the_selected_samples(dev):
 @pytest.mark.gpu
 def test_keyed_sum_compensated_within_one_eps(dev):
     assert run("keyed", 1 << 18, 777).decode().startswith("keyed ok")
+
+
+@pytest.mark.gpu
+def test_runtime_eval_strided_views(dev):
+    assert run("runtime", 123, 77).decode().startswith("runtime ok")
