@@ -532,14 +532,117 @@ __global__ __launch_bounds__(kAuxThreads) void real_part_kernel(const T* __restr
 // exactly in integers before the one rounding. The convolution runs in complex128 for both widths
 // (a complex64 plan reads and writes float, rounding once at the end), so a float transform of any
 // length is as accurate as the double one rounded to float.
+// ---- four-step passes for powers of two above 8192 (up to 2^26 points per line): a line of
+// n = n1 n2 points is an n1 x n2 matrix (element (j1, j2) at n2 j1 + j2); transform its columns
+// (n1 points, stride n2), multiply by W_n^(j2 k1), transform its rows (n2 points), and read the result
+// out transposed: X[k1 + n1 k2]. All in complex128 in a plan-owned copy of the lines; W_n^m is the
+// product of two table entries, W_n^(8192 h) W_n^l with m = 8192 h + l.
+struct FourStepPass {
+    uint64_t n = 0, n1 = 0, n2 = 0, lines = 0, inner = 1;
+    int prec = 0;              // of the data
+    BatchPlan* cols = nullptr; // lines x n2 signals of n1 points, stride n2, in `work`
+    BatchPlan* rows = nullptr; // lines x n1 signals of n2 points, contiguous, in `work`
+    double* work = nullptr;    // lines x n complex128
+    double* thi = nullptr;     // W_n^(8192 h), h < n / 8192
+    double* tlo = nullptr;     // W_n^l, l < 8192
+};
+
+struct BluesteinPass;
+
+// One axis of a plan: a Stockham pass (powers of two up to 8192), a four-step pass (powers of two up
+// to 2^26) or a Bluestein pass (other lengths up to 2^25).
+struct Pass {
+    BatchPlan* stockham = nullptr;
+    FourStepPass* four = nullptr;
+    BluesteinPass* blue = nullptr;
+};
+
 struct BluesteinPass {
     uint64_t n = 0, m = 0, lines = 0, inner = 1;
-    int prec = 0;              // of the data: 0 complex128, 1 complex64
-    BatchPlan* conv = nullptr; // lines x m contiguous complex128 signals
-    double* work = nullptr;    // lines x m complex128
-    double* chirp = nullptr;   // n complex: exp(i pi j^2 / n) (sign applied per direction)
-    double* bhat = nullptr;    // m complex: FFT_m(b) / m for the plan's direction
+    int prec = 0;           // of the data: 0 complex128, 1 complex64
+    Pass conv;              // lines x m contiguous complex128 signals
+    double* work = nullptr; // lines x m complex128
+    double* chirp = nullptr; // n complex: exp(i pi j^2 / n) (sign applied per direction)
+    double* bhat = nullptr;  // m complex: FFT_m(b) / m for the plan's direction
 };
+
+constexpr uint32_t kTwLo = 8192;
+
+// W_n^m for m < n (n a power of two): angle -2 pi m / n, m / n exact; quarter turns exact
+__device__ __forceinline__ void unit_root(uint64_t m, uint64_t n, double* re, double* im)
+{
+    double s, c;
+    dm::sincos(-2.0 * 3.141592653589793238 * ((double)m / (double)n), &s, &c);
+    if (n >= 4 && (m % (n / 4)) == 0) {
+        const uint64_t q = m / (n / 4);
+        c = q == 0 ? 1.0 : q == 2 ? -1.0 : 0.0;
+        s = q == 1 ? -1.0 : q == 3 ? 1.0 : 0.0;
+    }
+    *re = c;
+    *im = s;
+}
+
+__global__ __launch_bounds__(kAuxThreads) void fs_table_kernel(double* __restrict__ thi, double* __restrict__ tlo,
+                                                               uint64_t n)
+{
+    const uint64_t nhi = n / kTwLo;
+    for (uint64_t t = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; t < kTwLo + nhi;
+         t += (uint64_t)gridDim.x * kAuxThreads) {
+        if (t < kTwLo) unit_root(t, n, &tlo[2 * t], &tlo[2 * t + 1]);
+        else unit_root((t - kTwLo) * kTwLo, n, &thi[2 * (t - kTwLo)], &thi[2 * (t - kTwLo) + 1]);
+    }
+}
+
+// lines of `in` (n points, stride inner) into dense complex128 lines
+template<class T>
+__global__ __launch_bounds__(kAuxThreads) void fs_gather_kernel(const T* __restrict__ in, double* __restrict__ work,
+                                                                uint64_t lines, uint64_t n, uint64_t inner)
+{
+    const uint64_t count = lines * n;
+    for (uint64_t t = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; t < count; t += (uint64_t)gridDim.x * kAuxThreads) {
+        const uint64_t li = t / n, e = t - li * n;
+        const uint64_t o = li / inner, q = li - o * inner;
+        const uint64_t src = (o * n + e) * inner + q;
+        work[2 * t] = (double)in[2 * src];
+        work[2 * t + 1] = (double)in[2 * src + 1];
+    }
+}
+
+// after the column transforms: element (k1, j2) of every line times W_n^(j2 k1) (conjugated for the inverse)
+__global__ __launch_bounds__(kAuxThreads) void fs_twiddle_kernel(double* __restrict__ work,
+                                                                 const double* __restrict__ thi,
+                                                                 const double* __restrict__ tlo, uint64_t lines,
+                                                                 uint64_t n2, uint64_t n, double sign)
+{
+    const uint64_t count = lines * n;
+    for (uint64_t t = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; t < count; t += (uint64_t)gridDim.x * kAuxThreads) {
+        const uint64_t e = t % n, k1 = e / n2, j2 = e - k1 * n2;
+        const uint64_t m = k1 * j2; // < n
+        const uint64_t h = m / kTwLo, l = m - h * kTwLo;
+        const double ar = thi[2 * h], ai = thi[2 * h + 1], br = tlo[2 * l], bi = tlo[2 * l + 1];
+        const double wr = ar * br - ai * bi, wi = sign * (ar * bi + ai * br);
+        const double xr = work[2 * t], xi = work[2 * t + 1];
+        work[2 * t] = xr * wr - xi * wi;
+        work[2 * t + 1] = xr * wi + xi * wr;
+    }
+}
+
+// read out transposed: X[k1 + n1 k2] = work[k1 n2 + k2], scaled, along the strided output line
+template<class T>
+__global__ __launch_bounds__(kAuxThreads) void fs_scatter_kernel(const double* __restrict__ work, T* __restrict__ out,
+                                                                 uint64_t lines, uint64_t n1, uint64_t n2,
+                                                                 uint64_t inner, double scale)
+{
+    const uint64_t n = n1 * n2, count = lines * n;
+    for (uint64_t t = (uint64_t)blockIdx.x * kAuxThreads + threadIdx.x; t < count; t += (uint64_t)gridDim.x * kAuxThreads) {
+        const uint64_t li = t / n, e = t - li * n;
+        const uint64_t k1 = e / n2, k2 = e - k1 * n2;
+        const uint64_t o = li / inner, q = li - o * inner;
+        const uint64_t dst = (o * n + k1 + n1 * k2) * inner + q;
+        out[2 * dst] = (T)(work[2 * t] * scale);
+        out[2 * dst + 1] = (T)(work[2 * t + 1] * scale);
+    }
+}
 
 __device__ __forceinline__ void chirp_at(uint64_t j, uint64_t n, double* re, double* im)
 {
@@ -629,12 +732,6 @@ __global__ __launch_bounds__(kAuxThreads) void blue_post_kernel(const double* __
 
 } // namespace
 
-// One axis of a plan: a Stockham pass (power-of-two lengths up to 8192) or a Bluestein pass.
-struct Pass {
-    BatchPlan* stockham = nullptr;
-    BluesteinPass* blue = nullptr;
-};
-
 struct SeamPlan {
     int dev = 0;
     int kind = AETHER_FFT_C2C;
@@ -655,9 +752,20 @@ struct SeamPlan {
         (void)hipFree(p->sigs);
         delete p;
     }
+    static void free_four(FourStepPass* f)
+    {
+        if (!f) return;
+        free_pass(f->cols);
+        free_pass(f->rows);
+        (void)hipFree(f->work);
+        (void)hipFree(f->thi);
+        (void)hipFree(f->tlo);
+        delete f;
+    }
     static void free_pass(Pass& p)
     {
         free_pass(p.stockham);
+        free_four(p.four);
         if (p.blue) {
             free_pass(p.blue->conv);
             (void)hipFree(p.blue->work);
@@ -708,7 +816,81 @@ BatchPlan* make_batch(const std::vector<uint64_t>& off, uint32_t l2, uint64_t st
     return bp.release();
 }
 
-constexpr uint64_t kMaxBluestein = (uint64_t)kMaxN / 2; // 2n - 1 <= 8192
+constexpr uint32_t kMaxFourLog2 = 2 * kMaxLog2;          // four-step: up to 2^26 points
+constexpr uint64_t kMaxBluestein = 1ull << (kMaxFourLog2 - 1); // 2n - 1 <= 2^26
+
+FourStepPass* make_four(uint64_t n, uint64_t lines, uint64_t inner, int dev, bool dbl, hipStream_t st)
+{
+    auto f = std::make_unique<FourStepPass>();
+    struct Guard {
+        std::unique_ptr<FourStepPass>& p;
+        ~Guard()
+        {
+            if (!p) return;
+            SeamPlan::free_pass(p->cols);
+            SeamPlan::free_pass(p->rows);
+            (void)hipFree(p->work);
+            (void)hipFree(p->thi);
+            (void)hipFree(p->tlo);
+        }
+    } guard{f};
+    const uint32_t L = log2_exact(n);
+    const uint32_t L1 = L / 2, L2 = L - L1; // both <= 13
+    f->n = n;
+    f->n1 = 1ull << L1;
+    f->n2 = 1ull << L2;
+    f->lines = lines;
+    f->inner = inner;
+    f->prec = dbl ? 0 : 1;
+    check(hipMalloc(&f->work, 2 * sizeof(double) * lines * n), "hipMalloc(fft work)", 2 * sizeof(double) * lines * n);
+    check(hipMalloc(&f->thi, 2 * sizeof(double) * (n / kTwLo)), "hipMalloc(four-step twiddles)");
+    check(hipMalloc(&f->tlo, 2 * sizeof(double) * kTwLo), "hipMalloc(four-step twiddles)");
+    std::vector<uint64_t> off;
+    off.reserve(lines * f->n2);
+    for (uint64_t li = 0; li < lines; ++li)
+        for (uint64_t j2 = 0; j2 < f->n2; ++j2) off.push_back(li * n + j2);
+    f->cols = make_batch(off, L1, f->n2, dev, true);
+    off.clear();
+    for (uint64_t li = 0; li < lines; ++li)
+        for (uint64_t k1 = 0; k1 < f->n1; ++k1) off.push_back(li * n + k1 * f->n2);
+    f->rows = make_batch(off, L2, 1, dev, true);
+    hipLaunchKernelGGL(fs_table_kernel, aux_grid(kTwLo + n / kTwLo), dim3(kAuxThreads), 0, st, f->thi, f->tlo, n);
+    check(hipGetLastError(), "four-step table kernel");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return f.release();
+}
+
+template<class T>
+void run_four(const FourStepPass& f, const void* in, void* out, int forward, double scale, hipStream_t st)
+{
+    const uint64_t count = f.lines * f.n;
+    hipLaunchKernelGGL(fs_gather_kernel<T>, aux_grid(count), dim3(kAuxThreads), 0, st, static_cast<const T*>(in), f.work,
+                       f.lines, f.n, f.inner);
+    launch(*f.cols, f.work, f.work, forward, 3, st, 1.0);
+    hipLaunchKernelGGL(fs_twiddle_kernel, aux_grid(count), dim3(kAuxThreads), 0, st, f.work, (const double*)f.thi,
+                       (const double*)f.tlo, f.lines, f.n2, f.n, forward ? 1.0 : -1.0);
+    launch(*f.rows, f.work, f.work, forward, 3, st, 1.0);
+    hipLaunchKernelGGL(fs_scatter_kernel<T>, aux_grid(count), dim3(kAuxThreads), 0, st, (const double*)f.work,
+                       static_cast<T*>(out), f.lines, f.n1, f.n2, f.inner, scale);
+    check(hipGetLastError(), "four-step kernels");
+}
+
+// A power-of-two transform over `lines` dense complex128 lines of m points (Bluestein's convolution).
+Pass make_dense_pow2(uint64_t m, uint64_t lines, int dev, hipStream_t st)
+{
+    Pass p;
+    const uint32_t l2 = log2_exact(m);
+    if (l2 <= (uint32_t)kMaxLog2) {
+        std::vector<uint64_t> off(lines);
+        for (uint64_t li = 0; li < lines; ++li) off[li] = li * m;
+        p.stockham = make_batch(off, l2, 1, dev, true);
+    } else {
+        p.four = make_four(m, lines, 1, dev, true, st);
+    }
+    return p;
+}
+
+void run_pass(const Pass& p, const void* in, void* out, int forward, double scale, hipStream_t st);
 
 BluesteinPass* make_bluestein(uint64_t n, uint64_t lines, uint64_t inner, int dev, bool dbl, bool forward,
                               hipStream_t st)
@@ -736,27 +918,27 @@ BluesteinPass* make_bluestein(uint64_t n, uint64_t lines, uint64_t inner, int de
         ++l2;
     }
     bp->m = m;
-    check(hipMalloc(&bp->work, 2 * sizeof(double) * lines * m), "hipMalloc(fft work)");
+    (void)l2;
+    check(hipMalloc(&bp->work, 2 * sizeof(double) * lines * m), "hipMalloc(fft work)", 2 * sizeof(double) * lines * m);
     check(hipMalloc(&bp->chirp, 2 * sizeof(double) * n), "hipMalloc(chirp)");
     check(hipMalloc(&bp->bhat, 2 * sizeof(double) * m), "hipMalloc(chirp transform)");
-    std::vector<uint64_t> off(lines);
-    for (uint64_t li = 0; li < lines; ++li) off[li] = li * m;
-    bp->conv = make_batch(off, l2, 1, dev, true);
+    bp->conv = make_dense_pow2(m, lines, dev, st);
     // the chirp, then b and its transform in complex128 (one signal of m points), scaled by 1/m
     hipLaunchKernelGGL(chirp_kernel, aux_grid(n), dim3(kAuxThreads), 0, st, bp->chirp, n);
     hipLaunchKernelGGL(chirp_b_kernel, aux_grid(m), dim3(kAuxThreads), 0, st, (const double*)bp->chirp, bp->bhat, n, m,
                        forward ? 1.0 : -1.0);
-    std::unique_ptr<BatchPlan> one(make_batch(std::vector<uint64_t>{0}, l2, 1, dev, true));
-    launch(*one, bp->bhat, bp->bhat, 1, 3, st, 1.0);
+    Pass one = make_dense_pow2(m, 1, dev, st);
+    run_pass(one, bp->bhat, bp->bhat, 1, 1.0, st);
     hipLaunchKernelGGL(scale_kernel, aux_grid(2 * m), dim3(kAuxThreads), 0, st, bp->bhat, 2 * m, 1.0 / (double)m);
     check(hipGetLastError(), "chirp kernels");
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    SeamPlan::free_pass(one.release());
+    SeamPlan::free_pass(one);
     return bp.release();
 }
 
 // One strided-line pass along `ax` of a dense row-major complex `shape`: Stockham for powers of two up
-// to 8192, Bluestein for other lengths up to 4096; anything else is unsupported (2).
+// to 8192, four-step for powers of two up to 2^26, Bluestein for other lengths up to 2^25; anything else
+// is unsupported (2).
 Pass make_pass(const std::vector<int64_t>& shape, int ax, int dev, bool dbl, bool forward, hipStream_t st)
 {
     const int ndim = (int)shape.size();
@@ -778,8 +960,12 @@ Pass make_pass(const std::vector<int64_t>& shape, int ax, int dev, bool dbl, boo
         p.stockham = make_batch(off, l2, inner, dev, dbl);
         return p;
     }
+    if (l2 != 0xFFFFFFFFu && l2 <= kMaxFourLog2) {
+        if (lines) p.four = make_four(n, lines, inner, dev, dbl, st);
+        return p;
+    }
     if (n == 0 || n > kMaxBluestein)
-        fail("fft_plan", "axis lengths must be powers of two up to 8192 or other lengths up to 4096", 0, 2);
+        fail("fft_plan", "axis lengths must be powers of two up to 2^26 or other lengths up to 2^25", 0, 2);
     if (lines) p.blue = make_bluestein(n, lines, inner, dev, dbl, forward, st);
     return p;
 }
@@ -791,10 +977,10 @@ void run_bluestein(const BluesteinPass& b, const void* in, void* out, hipStream_
     double* work = b.work;
     hipLaunchKernelGGL(blue_pre_kernel<T>, aux_grid(b.lines * b.m), dim3(kAuxThreads), 0, st,
                        static_cast<const T*>(in), work, (const double*)b.chirp, b.lines, b.n, b.m, b.inner, sign);
-    launch(*b.conv, work, work, 1, 3, st, 1.0);
+    run_pass(b.conv, work, work, 1, 1.0, st);
     hipLaunchKernelGGL(blue_mul_kernel, aux_grid(b.lines * b.m), dim3(kAuxThreads), 0, st, work,
                        (const double*)b.bhat, b.lines, b.m);
-    launch(*b.conv, work, work, 0, 3, st, 1.0);
+    run_pass(b.conv, work, work, 0, 1.0, st);
     hipLaunchKernelGGL(blue_post_kernel<T>, aux_grid(b.lines * b.n), dim3(kAuxThreads), 0, st, (const double*)work,
                        static_cast<T*>(out), (const double*)b.chirp, b.lines, b.n, b.m, b.inner, sign, scale);
     check(hipGetLastError(), "bluestein kernels");
@@ -805,6 +991,10 @@ void run_pass(const Pass& p, const void* in, void* out, int forward, double scal
 {
     if (p.stockham) {
         launch(*p.stockham, in, out, forward, 3, st, scale);
+    } else if (p.four) {
+        // the convolution passes of Bluestein run on complex128 work whatever the data width
+        if (p.four->prec == 0) run_four<double>(*p.four, in, out, forward, scale, st);
+        else run_four<float>(*p.four, in, out, forward, scale, st);
     } else if (p.blue) {
         if (p.blue->prec == 0) run_bluestein<double>(*p.blue, in, out, st, scale, forward != 0);
         else run_bluestein<float>(*p.blue, in, out, st, scale, forward != 0);

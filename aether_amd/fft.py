@@ -6,9 +6,10 @@ inverse scaled by 1/n), "forward" or "ortho". Arrays are torch tensors (or anyth
 DLPack, or numpy arrays); the transform always runs on the GPU through this library's seam plans
 (``aether_memory_v1().fft_plan``): a host array is copied to the device and the result copied back.
 
-Lengths: powers of two up to 8192 run the LDS-resident Stockham transform; other lengths up to 4096
-run Bluestein's chirp-z through the same transforms; anything longer raises ``FftUnavailable`` (code 2,
-as the reference's plugin does when its FFT library is missing).
+Lengths: powers of two up to 8192 run the LDS-resident Stockham transform, powers of two up to 2^26 a
+four-step transform over it; other lengths up to 2^25 run Bluestein's chirp-z through the same
+transforms; anything longer raises ``FftUnavailable`` (code 2, as the reference's plugin does when its FFT
+library is missing).
 
 Plans are built eagerly, outside a stream capture; ``plan(...)`` (and the first call of a transform on a
 new shape) builds one, later calls reuse it. ``Plan.execute(input, output, stream=None)`` allocates

@@ -63,6 +63,43 @@ def test_c2c_lengths_match_numpy(dev, dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+def test_long_lengths_match_numpy(dev, dtype):
+    """Powers of two above 8192 (four-step: columns, twiddle, rows, transposed read-out) and other lengths
+    whose Bluestein convolution is longer than 8192 (its convolution then runs four-step), both directions,
+    held to the same ULP-scaled bound."""
+    for n, lines in ((16384, 3), (32768, 2), (1 << 16, 1), (1 << 20, 1), (5000, 2), (10007, 1), (65537, 1),
+                     (300000, 1)):
+        x = rand_c((lines, n), dtype, n)
+        for forward in (True, False):
+            fn = F.fft if forward else F.ifft
+            got = host(fn(on(x)))
+            ref = (np.fft.fft if forward else np.fft.ifft)(cplx(x), axis=-1)
+            err = np.abs(cplx(got) - ref).max()
+            assert err <= tol(dtype, n, x, scale_of("backward", forward, n)), (n, forward, err)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_long_axis_strided_and_inside_nd(dev, dtype):
+    """A long axis that is not the last one (its lines are strided by the inner extent), a long axis in a
+    multi-axis transform, and real transforms over long axes."""
+    x = rand_c((16384, 3), dtype, 21)
+    got = host(F.fft(on(x), axis=0))
+    ref = np.fft.fft(cplx(x), axis=0)
+    assert np.abs(cplx(got) - ref).max() <= tol(dtype, 16384, x, 1.0)
+    x = rand_c((4, 20000), dtype, 22)
+    got = host(F.fftn(on(x), axes=(0, 1)))
+    ref = np.fft.fftn(cplx(x), axes=(0, 1))
+    assert np.abs(cplx(got) - ref).max() <= tol(dtype, 80000, x, 1.0)
+    for n in (1 << 15, 50000):
+        r = np.random.default_rng(n).uniform(-1, 1, (2, n)).astype(NP[dtype])
+        spec = host(F.rfft(on(r)))
+        ref = np.fft.rfft(r.astype(np.float64), axis=-1)
+        assert np.abs(cplx(spec) - ref).max() <= tol(dtype, n, r, 1.0), n
+        back = host(F.irfft(on(spec), n=n))
+        assert np.abs(back - r).max() <= 2 * tol(dtype, n, spec, 1.0 / n), n
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_multi_axis_and_batched_match_numpy(dev, dtype):
     x = rand_c((4, 7, 6), dtype, 5)
     xc = cplx(x)
@@ -170,9 +207,9 @@ def test_misuse_is_a_named_error(dev):
         p.execute(torch.zeros((7, 2), dtype=torch.float32, device="cuda"),
                   torch.zeros((8, 2), dtype=torch.float32, device="cuda"))
     with pytest.raises(F.FftUnavailable):
-        F.plan("c2c", "float32", (5000,), (0,))  # not a power of two and beyond 4096
+        F.plan("c2c", "float32", ((1 << 25) + 1,), (0,))  # not a power of two and beyond 2^25
     with pytest.raises(F.FftUnavailable):
-        F.plan("c2c", "float64", (16384,), (0,))
+        F.plan("c2c", "float64", (1 << 27,), (0,))  # a power of two beyond 2^26
 
 
 def test_a_plan_built_before_a_capture_runs_inside_it_and_replays(dev):
