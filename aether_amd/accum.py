@@ -113,7 +113,7 @@ class AppendCounter:
 
     def __init__(self, capacity: int, device="cuda"):
         self.capacity = int(capacity)
-        # [counter, pad, overflow, pad]: each word on its own 16 bytes (the seam's fill alignment)
+        # [counter, pad, overflow, pad]: each word on its own 16-byte slot, so each is reset by one fill
         self._mem = torch.empty(8, dtype=torch.int32, device=device)
         self.reset(clear_overflow=True)
 

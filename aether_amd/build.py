@@ -87,8 +87,9 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, LIB)
     build_api_check()
+    core = build_dsc_core()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources, and {API_CHECK}")
+        print(f"built {LIB} from {len(srcs)} sources, {API_CHECK} and {core}")
     return LIB
 
 
@@ -104,6 +105,24 @@ def build_api_check() -> str:
         raise RuntimeError(f"g++ failed for {src}:\n{res.stderr}")
     os.replace(API_CHECK + ".tmp", API_CHECK)
     return API_CHECK
+
+
+def build_dsc_core() -> str:
+    """aether_amd.dsc._core (aether_amd/dsc/_core.cpp): the CPython extension of the array package, a
+    host-only g++ build against Python's headers; it opens libaether_hip.so itself on the first device
+    call, so it links neither the HIP runtime nor torch."""
+    import sysconfig
+
+    src = os.path.join(HERE, "dsc", "_core.cpp")
+    out = os.path.join(HERE, "dsc", "_core" + sysconfig.get_config_var("EXT_SUFFIX"))
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", "-Wno-missing-field-initializers",
+           "-Werror", "-fvisibility=hidden", "-I", sysconfig.get_paths()["include"], "-I", os.path.join(ROOT, "include"),
+           src, "-o", out + ".tmp", "-ldl"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{res.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -1020,9 +1020,12 @@ int seam_plan(int dev, int kind, int is_double, int norm, int forward, int ndim,
         for (int d = 0; d < ndim; ++d)
             if (shape[d] < 0) fail("fft_plan", "negative extent");
         if (norm < 0 || norm > 2) fail("fft_plan", "unknown norm");
+        // dsc/src/cufft_plan.h:129-131: a plan allocates, so it is built before a capture begins
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         check(hipStreamIsCapturing(as_stream(stream), &cs), "hipStreamIsCapturing");
-        if (cs != hipStreamCaptureStatusNone) fail("fft_plan", "refused while the stream is capturing");
+        if (cs != hipStreamCaptureStatusNone)
+            fail("fft_plan", "fft plan creation is not allowed inside a stream capture: build the plan before the "
+                             "capture begins");
         int prev = 0;
         check(hipGetDevice(&prev), "hipGetDevice");
         if (prev != dev) check(hipSetDevice(dev), "hipSetDevice");

@@ -54,39 +54,72 @@ struct OnDevice {
     }
 };
 
+// Queries and releases made while another thread or stream captures in global mode: the thread's
+// capture mode is relaxed for the call's duration, so they are legal and do not invalidate the capture.
+struct RelaxedCapture {
+    hipStreamCaptureMode prev = hipStreamCaptureModeRelaxed;
+    RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&prev); }
+    ~RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&prev); }
+};
+
+// Never throws: it runs from chunk_release, which returns nothing.
 void free_now(Chunk* c)
 {
-    if (c->kind == AETHER_MEM_DEVICE) {
-        OnDevice d(c->dev);
-        (void)hipFreeAsync(c->ptr, c->stream);
-    } else if (c->kind == AETHER_MEM_PINNED) {
-        (void)hipHostFree(c->ptr);
-    } else {
-        std::free(c->ptr);
+    RelaxedCapture relaxed;
+    try {
+        if (c->kind == AETHER_MEM_DEVICE) {
+            OnDevice d(c->dev);
+            (void)hipFreeAsync(c->ptr, c->stream);
+        } else if (c->kind == AETHER_MEM_PINNED) {
+            (void)hipHostFree(c->ptr);
+        } else {
+            std::free(c->ptr);
+        }
+    } catch (...) {
     }
+    (void)hipGetLastError();
     delete c;
 }
 
+// Whether `s` is being captured, or a capture elsewhere makes work on it illegal (a failed query, such
+// as the implicit capture of the legacy stream under a blocking capture, answers yes).
 bool capturing(hipStream_t s)
 {
+    RelaxedCapture relaxed;
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
     return st != hipStreamCaptureStatusNone;
+}
+
+// A release a capture makes illegal now: a device free is ordered on its own stream, a pinned free
+// synchronises the device (the legacy stream).
+bool blocked(const Chunk* c)
+{
+    if (c->kind == AETHER_MEM_CPU) return false;
+    return capturing(nullptr) || (c->kind == AETHER_MEM_DEVICE && capturing(c->stream));
 }
 
 constexpr int kThreads = 256;
 
 // Fill `bytes` with a repeating `width`-byte pattern (width divides 16 or is
 // 1, 2, 4, 8, 16): the 16-byte lane pattern is pre-expanded on the host.
-__global__ __launch_bounds__(kThreads) void fill_kernel(unsigned char* __restrict__ p, size_t bytes, uint4 pat16)
+// Byte k of the region takes pat16[k % 16]. The `head` bytes before the first 16-byte boundary and the
+// tail after the last one are stored bytewise; the aligned body takes 16-byte stores of body16, which is
+// pat16 rotated by `head` (so body byte j is pat16[(head + j) % 16]).
+__global__ __launch_bounds__(kThreads) void fill_kernel(unsigned char* __restrict__ p, size_t bytes, uint4 pat16,
+                                                        uint4 body16, uint32_t head)
 {
-    const size_t vecs = bytes / 16;
+    unsigned char* body = p + head;
+    const size_t vecs = (bytes - head) / 16;
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < vecs; i += (size_t)gridDim.x * kThreads)
-        reinterpret_cast<uint4*>(p)[i] = pat16;
-    const size_t tail = bytes - vecs * 16;
-    if (blockIdx.x == 0 && threadIdx.x < tail) {
-        const unsigned char* pb = reinterpret_cast<const unsigned char*>(&pat16);
-        p[vecs * 16 + threadIdx.x] = pb[threadIdx.x];
+        reinterpret_cast<uint4*>(body)[i] = body16;
+    const size_t tail = bytes - head - vecs * 16;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < head) p[threadIdx.x] = reinterpret_cast<const unsigned char*>(&pat16)[threadIdx.x];
+        if (threadIdx.x < tail) body[vecs * 16 + threadIdx.x] = reinterpret_cast<const unsigned char*>(&body16)[threadIdx.x];
     }
 }
 
@@ -152,27 +185,30 @@ int s_chunk_alloc(int kind, int dev, size_t bytes, void* stream, void** chunk, v
     });
 }
 
-void s_chunk_release(void* chunk)
-{
-    auto* c = static_cast<Chunk*>(chunk);
-    if (!c) return;
-    if (c->kind == AETHER_MEM_DEVICE && capturing(c->stream)) {
-        std::lock_guard<std::mutex> lock(g_deferred_mu);
-        g_deferred.push_back(c);
-        return;
-    }
-    free_now(c);
-}
-
+// Release every deferred chunk whose capture has ended; those still blocked stay queued.
 void s_drain()
 {
     std::vector<Chunk*> keep, now;
     {
         std::lock_guard<std::mutex> lock(g_deferred_mu);
-        for (Chunk* c : g_deferred) (capturing(c->stream) ? keep : now).push_back(c);
+        if (g_deferred.empty()) return;
+        for (Chunk* c : g_deferred) (blocked(c) ? keep : now).push_back(c);
         g_deferred.swap(keep);
     }
     for (Chunk* c : now) free_now(c);
+}
+
+void s_chunk_release(void* chunk)
+{
+    auto* c = static_cast<Chunk*>(chunk);
+    if (!c) return;
+    s_drain();
+    if (blocked(c)) {
+        std::lock_guard<std::mutex> lock(g_deferred_mu);
+        g_deferred.push_back(c);
+        return;
+    }
+    free_now(c);
 }
 
 int s_fill(int kind, int dev, void* ptr, size_t bytes, const void* value, size_t width, void* stream)
@@ -186,11 +222,16 @@ int s_fill(int kind, int dev, void* ptr, size_t bytes, const void* value, size_t
         for (size_t i = 0; i < 16; ++i) pat[i] = static_cast<const unsigned char*>(value)[i % width];
         if (kind == AETHER_MEM_DEVICE) {
             OnDevice d(dev);
-            uint4 p16;
+            if (reinterpret_cast<uintptr_t>(ptr) % width != 0) fail("fill", "device pointer is not aligned to the width", bytes);
+            const uint32_t head =
+                (uint32_t)std::min<size_t>(bytes, (16 - reinterpret_cast<uintptr_t>(ptr) % 16) % 16);
+            unsigned char rot[16];
+            for (size_t i = 0; i < 16; ++i) rot[i] = pat[(head + i) % 16];
+            uint4 p16, b16;
             std::memcpy(&p16, pat, 16);
-            if (reinterpret_cast<uintptr_t>(ptr) % 16 != 0) fail("fill", "device pointer must be 16-byte aligned", 0, 2);
+            std::memcpy(&b16, rot, 16);
             hipLaunchKernelGGL(fill_kernel, dim3(grid_for(bytes / 16 + 1)), dim3(kThreads), 0, as_stream(stream),
-                               static_cast<unsigned char*>(ptr), bytes, p16);
+                               static_cast<unsigned char*>(ptr), bytes, p16, b16, head);
             check(hipGetLastError(), "fill_kernel launch", bytes);
         } else {
             // host memory: ordered after the stream's queued work
