@@ -118,11 +118,15 @@ __device__ __forceinline__ CFF twiddle(const TwTables& t, uint32_t m, bool fwd, 
     return fwd ? CFF{w.x, w.y, w.z, w.w} : CFF{w.x, w.y, -w.z, -w.w};
 }
 
+// complex products with one rounding less per part (fused multiply-add)
 __device__ __forceinline__ C128 cmul(C128 a, C128 w)
 {
-    return {a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re};
+    return {__builtin_fma(a.re, w.re, -(a.im * w.im)), __builtin_fma(a.re, w.im, a.im * w.re)};
 }
-__device__ __forceinline__ C64 cmul(C64 a, C64 w) { return {a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re}; }
+__device__ __forceinline__ C64 cmul(C64 a, C64 w)
+{
+    return {__builtin_fmaf(a.re, w.re, -(a.im * w.im)), __builtin_fmaf(a.re, w.im, a.im * w.re)};
+}
 __device__ __forceinline__ CFF cmul(CFF a, CFF w)
 {
     const auto rr = ffmul(a.rh, a.rl, w.rh, w.rl), ii = ffmul(a.ih, a.il, w.ih, w.il);
@@ -262,6 +266,346 @@ __global__ __launch_bounds__(kThreads) void fft_kernel(const Desc* __restrict__ 
     }
 }
 
+// ---- register-resident transforms (complex128 and complex64 on contiguous signals) -------------------
+// One workgroup per chunk (up to 8192 points of one length); 512 threads hold 16 points each. A transform
+// of 2^L points is L/4 Stockham passes of radix 16, then one of radix 2, 4 or 8 when 4 does not divide L;
+// each pass is a DFT in registers (radix 16 as 4 x 4, radix 8 as 2 x 4, constant twiddles inside). The
+// first pass reads its inputs straight from HBM and the last writes its outputs straight to HBM, both
+// coalesced (consecutive lanes take consecutive butterflies), so a point crosses LDS only between passes:
+// one 8-byte word at a time (real parts, then imaginary parts of complex128; complex64 as one word),
+// through pad(8192) words. The twiddles of a pass, W^(r m) for r < R, are products of the table entries
+// W^m, W^2m, W^4m and W^8m (at most four factors), formed where they are used: a radix-16 butterfly loads
+// four entries instead of fifteen. Complex products use fused multiply-adds.
+constexpr int kRT = 512; // threads
+constexpr int kRP = 16;  // points per thread
+
+template<class C>
+struct Words;
+template<>
+struct Words<C128> {
+    static constexpr int n = 2;
+    using W = double;
+    __device__ static W get(const C128& v, int k) { return k ? v.im : v.re; }
+    __device__ static void put(C128& v, int k, W w)
+    {
+        if (k) v.im = w;
+        else v.re = w;
+    }
+};
+template<>
+struct Words<C64> {
+    static constexpr int n = 1;
+    using W = float2;
+    __device__ static W get(const C64& v, int) { return make_float2(v.re, v.im); }
+    __device__ static void put(C64& v, int, W w) { v = C64{w.x, w.y}; }
+};
+
+// cos(2 pi k / 16)
+__device__ constexpr double kCos16[16] = {1.0,
+                                          0.92387953251128675613,
+                                          0.70710678118654752440,
+                                          0.38268343236508977173,
+                                          0.0,
+                                          -0.38268343236508977173,
+                                          -0.70710678118654752440,
+                                          -0.92387953251128675613,
+                                          -1.0,
+                                          -0.92387953251128675613,
+                                          -0.70710678118654752440,
+                                          -0.38268343236508977173,
+                                          0.0,
+                                          0.38268343236508977173,
+                                          0.70710678118654752440,
+                                          0.92387953251128675613};
+
+// x * W16^E (forward W16 = exp(-2 pi i / 16); the inverse conjugates); E is a compile-time constant
+template<int E, class C>
+__device__ __forceinline__ C w16(C x, bool fwd)
+{
+    if constexpr (E % 16 == 0) {
+        return x;
+    } else if constexpr (E % 16 == 4) {
+        return rot(x, fwd);
+    } else {
+        using R = decltype(x.re);
+        const R c = (R)kCos16[E % 16], s = (R)kCos16[(E + 12) % 16]; // sin(t) = cos(t - pi/2)
+        return cmul(x, C{c, fwd ? -s : s});
+    }
+}
+
+template<class C>
+__device__ __forceinline__ void dft4(C& a, C& b, C& c, C& d, bool fwd)
+{
+    const C s02 = add(a, c), d02 = sub(a, c);
+    const C s13 = add(b, d), d13 = rot(sub(b, d), fwd);
+    a = add(s02, s13);
+    b = add(d02, d13);
+    c = sub(s02, s13);
+    d = sub(d02, d13);
+}
+
+// In-place DFT of R points, natural order in and out. N = A B: y[c + B d] = DFT_A over a of
+// W_N^(a c) DFT_B over b of x[a + A b].
+template<int R, class C>
+__device__ __forceinline__ void dft(C* x, bool fwd)
+{
+    if constexpr (R == 2) {
+        const C t = x[0];
+        x[0] = add(t, x[1]);
+        x[1] = sub(t, x[1]);
+    } else if constexpr (R == 4) {
+        dft4(x[0], x[1], x[2], x[3], fwd);
+    } else if constexpr (R == 8) {
+        C z0[4] = {x[0], x[2], x[4], x[6]}, z1[4] = {x[1], x[3], x[5], x[7]};
+        dft4(z0[0], z0[1], z0[2], z0[3], fwd);
+        dft4(z1[0], z1[1], z1[2], z1[3], fwd);
+        z1[1] = w16<2>(z1[1], fwd);
+        z1[2] = w16<4>(z1[2], fwd);
+        z1[3] = w16<6>(z1[3], fwd);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            x[c] = add(z0[c], z1[c]);
+            x[c + 4] = sub(z0[c], z1[c]);
+        }
+    } else {
+        static_assert(R == 16, "radix 2, 4, 8 or 16");
+#pragma unroll
+        for (int a = 0; a < 4; ++a) dft4(x[a], x[a + 4], x[a + 8], x[a + 12], fwd);
+        // z[a][c] sits in x[a + 4c]; times W16^(a c)
+        x[5] = w16<1>(x[5], fwd);
+        x[9] = w16<2>(x[9], fwd);
+        x[13] = w16<3>(x[13], fwd);
+        x[6] = w16<2>(x[6], fwd);
+        x[10] = w16<4>(x[10], fwd);
+        x[14] = w16<6>(x[14], fwd);
+        x[7] = w16<3>(x[7], fwd);
+        x[11] = w16<6>(x[11], fwd);
+        x[15] = w16<9>(x[15], fwd);
+        // y[c + 4 d] = DFT4 over a of z[a][c]: the four points of column c are x[4c .. 4c+3]
+        C y[16];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            C p0 = x[4 * c], p1 = x[4 * c + 1], p2 = x[4 * c + 2], p3 = x[4 * c + 3];
+            dft4(p0, p1, p2, p3, fwd);
+            y[c] = p0;
+            y[c + 4] = p1;
+            y[c + 8] = p2;
+            y[c + 12] = p3;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) x[r] = y[r];
+    }
+}
+
+// The twiddle source: the device table W^m = exp(-2 pi i m / 8192) (conjugated for the inverse).
+struct TwGlobal {
+    TwTables t;
+    template<class C>
+    __device__ __forceinline__ C at(uint32_t m, bool fwd) const
+    {
+        return twiddle(t, m, fwd, (C*)nullptr);
+    }
+};
+
+// x[r] *= W^(r m) for r = 1..R-1, from W^m, W^2m, W^4m, W^8m; each power is formed where it is used
+template<int R, class C, class TW>
+__device__ __forceinline__ void twiddle_run(C* x, uint32_t m, const TW& q, bool fwd)
+{
+    const C w1 = q.template at<C>(m, fwd);
+    x[1] = cmul(x[1], w1);
+    if constexpr (R > 2) {
+        const C w2 = q.template at<C>(2 * m, fwd);
+        x[2] = cmul(x[2], w2);
+        const C w3 = cmul(w1, w2);
+        x[3] = cmul(x[3], w3);
+        if constexpr (R > 4) {
+            const C w4 = q.template at<C>(4 * m, fwd);
+            x[4] = cmul(x[4], w4);
+            const C w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
+            x[5] = cmul(x[5], w5);
+            x[6] = cmul(x[6], w6);
+            x[7] = cmul(x[7], w7);
+            if constexpr (R > 8) {
+                const C w8 = q.template at<C>(8 * m, fwd);
+                x[8] = cmul(x[8], w8);
+                x[9] = cmul(x[9], cmul(w1, w8));
+                x[10] = cmul(x[10], cmul(w2, w8));
+                x[11] = cmul(x[11], cmul(w3, w8));
+                x[12] = cmul(x[12], cmul(w4, w8));
+                x[13] = cmul(x[13], cmul(w5, w8));
+                x[14] = cmul(x[14], cmul(w6, w8));
+                x[15] = cmul(x[15], cmul(w7, w8));
+            }
+        }
+    }
+}
+
+// One pass of radix R (stride Ns = product of the earlier radices) over the thread's 16 / R butterflies
+// b = threadIdx.x + u * kRT, b < nb; butterfly u holds v[u R .. u R + R - 1]; 2^l2nr butterflies a signal.
+template<int R, class C, class TW>
+__device__ __forceinline__ void rpass(C (&v)[kRP], uint32_t l2nr, uint32_t nb, uint32_t Ns, const TW& q, bool fwd)
+{
+    constexpr int U = kRP / R;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t b = threadIdx.x + u * kRT;
+        if (b < nb) {
+            if (Ns > 1) {
+                const uint32_t j = b & ((1u << l2nr) - 1u);
+                const uint32_t k = j & (Ns - 1u);
+                twiddle_run<R>(&v[u * R], k * (kMaxN / (R * Ns)), q, fwd);
+            }
+            dft<R>(&v[u * R], fwd);
+        }
+    }
+}
+
+// The first pass's inputs: butterfly (s, j) reads x[j + r N / R] of signal s, for r in [R0, R1).
+template<int R, class C, int R0 = 0, int R1 = R, int NV>
+__device__ __forceinline__ void rload(C (&v)[NV], const Sig* __restrict__ sigs, const void* __restrict__ in,
+                                      uint32_t l2nr, uint32_t nb)
+{
+    constexpr int U = kRP / R;
+    static_assert(U == 1 || (R0 == 0 && R1 == R), "a point range is for radix-16 loads");
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t b = threadIdx.x + u * kRT;
+        if (b < nb) {
+            const uint32_t s = b >> l2nr, j = b & ((1u << l2nr) - 1u);
+            const uint64_t base = sigs[s].in_off + j;
+#pragma unroll
+            for (int r = R0; r < R1; ++r) v[u * R + r - R0] = gload(in, base + ((uint64_t)r << l2nr), (C*)nullptr);
+        }
+    }
+}
+
+// The last pass's outputs (Ns = N / R): butterfly (s, j) writes X[j + r Ns] of signal s.
+template<int R, class C>
+__device__ __forceinline__ void rstore(const C (&v)[kRP], const Sig* __restrict__ sigs, void* __restrict__ out,
+                                       uint32_t l2nr, uint32_t nb, double scale)
+{
+    constexpr int U = kRP / R;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t b = threadIdx.x + u * kRT;
+        if (b < nb) {
+            const uint32_t s = b >> l2nr, j = b & ((1u << l2nr) - 1u);
+            const uint64_t base = sigs[s].out_off + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) gstore(out, base + ((uint64_t)r << l2nr), v[u * R + r], scale);
+        }
+    }
+}
+
+// Pass (R, Ns) outputs to LDS at their Stockham places, then the next pass's (R2) inputs back, one word
+// of every point at a time.
+template<int R, int R2, class C>
+__device__ __forceinline__ void rexchange(C (&v)[kRP], typename Words<C>::W* lds, uint32_t log2n, uint32_t P,
+                                          uint32_t Ns)
+{
+    constexpr int U = kRP / R, U2 = kRP / R2;
+    const uint32_t N = 1u << log2n;
+    const uint32_t nr = N / R, nr2 = N / R2, nb = P / R, nb2 = P / R2;
+#pragma unroll
+    for (int w = 0; w < Words<C>::n; ++w) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t b = threadIdx.x + u * kRT;
+            if (b < nb) {
+                const uint32_t s = b / nr, j = b - s * nr;
+                const uint32_t k = j & (Ns - 1u);
+                const uint32_t idx = s * N + (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[pad(idx + r * Ns)] = Words<C>::get(v[u * R + r], w);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U2; ++u) {
+            const uint32_t b = threadIdx.x + u * kRT;
+            if (b < nb2) {
+                const uint32_t s = b / nr2, j = b - s * nr2;
+                const uint32_t base = s * N + j;
+#pragma unroll
+                for (int r = 0; r < R2; ++r) Words<C>::put(v[u * R2 + r], w, lds[pad(base + r * nr2)]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template<int R, class C, class TW>
+__device__ __forceinline__ void rsingle(C (&v)[kRP], const Sig* sigs, const void* in, void* out, uint32_t P,
+                                        double scale, const TW& q, bool fwd)
+{
+    rload<R>(v, sigs, in, 0, P / R);
+    rpass<R>(v, 0, P / R, 1, q, fwd);
+    rstore<R>(v, sigs, out, 0, P / R, scale);
+}
+
+// the tail pass of radix R after the passes of 16, then its outputs
+template<int R, class C, class TW>
+__device__ __forceinline__ void rtail(C (&v)[kRP], const Sig* sigs, void* out, uint32_t log2n, uint32_t P,
+                                      uint32_t Ns, double scale, const TW& q, bool fwd)
+{
+    constexpr uint32_t lr = R == 2 ? 1 : R == 4 ? 2 : 3;
+    rpass<R>(v, log2n - lr, P / R, Ns, q, fwd);
+    rstore<R>(v, sigs, out, log2n - lr, P / R, scale);
+}
+
+// Transform one chunk: read, passes, write.
+template<class C, class TW>
+__device__ __forceinline__ void rchunk(C (&v)[kRP], const Desc d, const Sig* s, const void* in, void* out,
+                                       typename Words<C>::W* xch, const TW& q, bool fwd, int norm, double fixed_scale)
+{
+    const uint32_t L = d.log2n, N = 1u << L, P = (uint32_t)d.count << L;
+    // numpy norms (fft/Fft.h:73-76); norm 3: a caller-fixed scale
+    const double scale = norm == 3 ? fixed_scale
+                         : norm == 2 ? 1.0 / __builtin_sqrt((double)N)
+                                     : ((fwd == (norm == 1)) ? 1.0 / (double)N : 1.0);
+    if (L == 0) {
+        for (uint32_t e = threadIdx.x; e < P; e += kRT) gstore(out, s[e].out_off, gload(in, s[e].in_off, (C*)nullptr), scale);
+        return;
+    }
+    const uint32_t nq = L / 4, rem = L % 4;
+    if (nq == 0) {
+        if (rem == 1) rsingle<2>(v, s, in, out, P, scale, q, fwd);
+        else if (rem == 2) rsingle<4>(v, s, in, out, P, scale, q, fwd);
+        else rsingle<8>(v, s, in, out, P, scale, q, fwd);
+        return;
+    }
+    rload<16>(v, s, in, L - 4, P / 16);
+    uint32_t Ns = 1;
+    for (uint32_t p = 0; p < nq; ++p) {
+        rpass<16>(v, L - 4, P / 16, Ns, q, fwd);
+        if (p + 1 == nq && rem == 0) {
+            rstore<16>(v, s, out, L - 4, P / 16, scale);
+            return;
+        }
+        if (p + 1 < nq) rexchange<16, 16>(v, xch, L, P, Ns);
+        else if (rem == 1) rexchange<16, 2>(v, xch, L, P, Ns);
+        else if (rem == 2) rexchange<16, 4>(v, xch, L, P, Ns);
+        else rexchange<16, 8>(v, xch, L, P, Ns);
+        Ns *= 16;
+    }
+    if (rem == 1) rtail<2>(v, s, out, L, P, Ns, scale, q, fwd);
+    else if (rem == 2) rtail<4>(v, s, out, L, P, Ns, scale, q, fwd);
+    else rtail<8>(v, s, out, L, P, Ns, scale, q, fwd);
+}
+
+// complex64 fits two workgroups a CU (4 waves a SIMD, 128 registers); complex128's passes need more
+template<class C>
+__global__ __launch_bounds__(kRT, sizeof(C) == 8 ? 4 : 2) void fft_reg_kernel(const Desc* __restrict__ descs, const Sig* __restrict__ sigs,
+                                                         const void* __restrict__ in, void* __restrict__ out,
+                                                         TwTables tw, int fwd_flag, int norm, double fixed_scale)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    auto* xch = reinterpret_cast<typename Words<C>::W*>(lds_raw);
+    const Desc d = descs[blockIdx.x];
+    C v[kRP];
+    rchunk(v, d, sigs + d.first_sig, in, out, xch, TwGlobal{tw}, fwd_flag != 0, norm, fixed_scale);
+}
+
 __global__ void twiddle_kernel(double2* d, float2* f, float4* ff)
 {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,6 +652,13 @@ TwTables tables(hipStream_t st)
             check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsElems * 16),
                   "hipFuncSetAttribute(LDS)");
         }
+        check(hipFuncSetAttribute((const void*)fft_reg_kernel<C128>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kLdsElems * 8),
+              "hipFuncSetAttribute(LDS)");
+        check(hipFuncSetAttribute((const void*)fft_reg_kernel<C64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kLdsElems * 8),
+              "hipFuncSetAttribute(LDS)");
+
         it = g_tw.emplace(dev, t).first;
     }
     return TwTables{it->second.d, it->second.f, it->second.ff};
@@ -357,6 +708,16 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
     const TwTables tw = tables(st);
     const size_t lds = lds_bytes(p.prec);
     if (p.ndesc == 0) return;
+    if (p.stride == 1 && p.prec != 2) {
+        if (p.prec == 0)
+            hipLaunchKernelGGL(fft_reg_kernel<C128>, dim3(p.ndesc), dim3(kRT), kLdsElems * 8, st, p.descs, p.sigs, in,
+                               out, tw, forward, norm, fixed_scale);
+        else
+            hipLaunchKernelGGL(fft_reg_kernel<C64>, dim3(p.ndesc), dim3(kRT), kLdsElems * 8, st, p.descs, p.sigs, in,
+                               out, tw, forward, norm, fixed_scale);
+        check(hipGetLastError(), "fft_reg_kernel launch");
+        return;
+    }
     switch (p.prec) {
     case 0:
         hipLaunchKernelGGL(fft_kernel<C128>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
@@ -797,6 +1158,7 @@ BatchPlan* make_batch(const std::vector<uint64_t>& off, uint32_t l2, uint64_t st
     auto bp = std::make_unique<BatchPlan>();
     bp->prec = dbl ? 0 : 1;
     bp->dev = dev;
+    bp->stride = (uint32_t)stride;
     bp->ndesc = (uint32_t)descs.size();
     bp->nsig = sigs.size();
     if (!descs.empty()) {

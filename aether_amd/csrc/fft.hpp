@@ -30,6 +30,7 @@ struct BatchPlan {
     Sig* sigs = nullptr;
     uint32_t ndesc = 0;
     uint64_t nsig = 0;
+    uint32_t stride = 1; // of every descriptor (1: contiguous signals)
 };
 
 uint32_t log2_exact(uint64_t n);

@@ -32,6 +32,11 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC_BYTES = {27: 12_890_958_603}
 
 
+# secondary workloads that run on their own (no arguments)
+SINGLE_WORKLOADS = {"philox": "wl_philox", "fp32": "wl_fp32", "trig": "wl_trig", "fft": "wl_fft",
+                    "accum": "wl_accum"}
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -45,6 +50,8 @@ def parse():
     p.add_argument("--cpu-log2n", type=int, default=24)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run the seam's device copy kernel over 3 GiB (known bytes; PMC calibration)")
+    p.add_argument("--workload", choices=sorted(SINGLE_WORKLOADS),
+                   help="only run one secondary workload and print its JSON (profiling)")
     return p.parse_args()
 
 
@@ -495,6 +502,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.steps)
+        return
+    if args.workload:
+        print(json.dumps({args.workload: globals()[SINGLE_WORKLOADS[args.workload]]()}), flush=True)
         return
     import torch
 
