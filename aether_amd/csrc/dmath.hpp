@@ -419,5 +419,404 @@ __device__ __forceinline__ float logf(float x)
     return y;
 }
 
+// ---- the rest of the reference's elementwise set (expr/nodes/Elementwise.h:200-851): tan, atan,
+// asin, acos, tanh, pow, and the exact rsqrt / sign / fmin / fmax / ordered max. Double-double
+// (unevaluated sum) steps wherever a plain evaluation would lose more than half an ulp; constants from
+// gen_math_constants.py. Each is <= 1 ulp from the correctly rounded value on the reference's MPFR rows.
+
+__device__ __forceinline__ void two_sum(double a, double b, double* s, double* e)
+{
+    const double ss = a + b;
+    const double bb = ss - a;
+    *e = (a - (ss - bb)) + (b - bb);
+    *s = ss;
+}
+
+// |a| >= |b| (or a == 0)
+__device__ __forceinline__ void fast_two_sum(double a, double b, double* s, double* e)
+{
+    const double ss = a + b;
+    *e = b - (ss - a);
+    *s = ss;
+}
+
+// atan(k/16) and pi/2 - atan(k/16), k = 0..16, as (hi, lo)
+__device__ __constant__ static const double kAtanK[17][2] = {
+    {0x0.0p+0, 0x0.0p+0},
+    {0x1.ff55bb72cfdeap-5, -0x1.c934d86d23f1dp-60},
+    {0x1.fd5ba9aac2f6ep-4, -0x1.cd37686760c17p-59},
+    {0x1.7b97b4bce5b02p-3, 0x1.347b0b4f881cap-58},
+    {0x1.f5b75f92c80ddp-3, 0x1.8ab6e3cf7afbdp-57},
+    {0x1.362773707ebccp-2, -0x1.963a544b672d8p-57},
+    {0x1.6f61941e4def1p-2, -0x1.c63aae6f6e918p-56},
+    {0x1.a64eec3cc23fdp-2, -0x1.24dec1b50b7ffp-56},
+    {0x1.dac670561bb4fp-2, 0x1.a2b7f222f65e2p-56},
+    {0x1.0657e94db30d0p-1, -0x1.d5b495f6349e6p-56},
+    {0x1.1e00babdefeb4p-1, -0x1.928df287a668fp-58},
+    {0x1.345f01cce37bbp-1, 0x1.1021137c71102p-55},
+    {0x1.4978fa3269ee1p-1, 0x1.2419a87f2a458p-56},
+    {0x1.5d58987169b18p-1, 0x1.0028e4bc5e7cap-57},
+    {0x1.700a7c5784634p-1, -0x1.8c34d25aadef6p-56},
+    {0x1.819d0b7158a4dp-1, -0x1.bf76229d3b917p-56},
+    {0x1.921fb54442d18p-1, 0x1.1a62633145c07p-55},
+};
+__device__ __constant__ static const double kAtanCoK[17][2] = {
+    {0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54},
+    {0x1.82250768ac529p+0, -0x1.e78c96d05afcbp-58},
+    {0x1.7249faa996a21p+0, 0x1.a8cc1e7480c68p-54},
+    {0x1.62acbeaca61b8p+0, 0x1.c6ac9f134fa91p-60},
+    {0x1.5368c951e9cfdp+0, -0x1.96f47948a99f1p-54},
+    {0x1.4495d86823225p+0, 0x1.4d29adbab2a62p-54},
+    {0x1.3647503caf55cp+0, 0x1.17e21d9a42c9ap-55},
+    {0x1.288bfa3512419p+0, 0x1.8e684e7a2281bp-56},
+    {0x1.1b6e192ebbe44p+0, 0x1.b1b466a88828ep-54},
+    {0x1.0ef3c09d694b0p+0, 0x1.8fcf88aed2e80p-54},
+    {0x1.031f57e54adbep+0, 0x1.338b4259c0270p-54},
+    {0x1.efe068bba2275p-1, 0x1.24a3b2e61a70bp-55},
+    {0x1.dac670561bb4fp-1, 0x1.a2b7f222f65e2p-55},
+    {0x1.c6e6d2171bf18p-1, 0x1.f4ba8d3373e1bp-55},
+    {0x1.b434ee31013fdp-1, -0x1.0520d0701d877p-55},
+    {0x1.a2a25f172cfe4p-1, -0x1.d700509dad6cep-56},
+    {0x1.921fb54442d18p-1, 0x1.1a62633145c07p-55},
+};
+constexpr double kPio2H = 0x1.921fb54442d18p+0, kPio2L = 0x1.1a62633145c07p-54;
+constexpr double kPiH = 0x1.921fb54442d18p+1, kPiL = 0x1.1a62633145c07p-53;
+
+// tan: sin and cos of the reduced argument as unevaluated sums (ksin / kcos before their last
+// addition), then one double-double quotient; odd quadrants give -cos/sin.
+__device__ __forceinline__ double tan(double x)
+{
+    const double ax = __builtin_fabs(x);
+    if (!(ax <= 0x1.fffffffffffffp+1023)) return x - x;
+    if (ax < 0x1p-27) return x;
+    double rh, rl;
+    const int q = reduce_pio2(x, &rh, &rl);
+    const double z = rh * rh;
+    double ps = 0x1.952c77030ad4ap-49;
+    ps = __builtin_fma(ps, z, -0x1.ae7f3e733b81fp-41);
+    ps = __builtin_fma(ps, z, 0x1.6124613a86d09p-33);
+    ps = __builtin_fma(ps, z, -0x1.ae64567f544e4p-26);
+    ps = __builtin_fma(ps, z, 0x1.71de3a556c734p-19);
+    ps = __builtin_fma(ps, z, -0x1.a01a01a01a01ap-13);
+    ps = __builtin_fma(ps, z, 0x1.1111111111111p-7);
+    ps = __builtin_fma(ps, z, -0x1.5555555555555p-3);
+    const double stail = __builtin_fma(-0.5 * z, rl, rl);
+    double sh, sl;
+    fast_two_sum(rh, __builtin_fma(rh * z, ps, stail), &sh, &sl);
+    double pc = 0x1.ae7f3e733b81fp-45;
+    pc = __builtin_fma(pc, z, -0x1.93974a8c07c9dp-37);
+    pc = __builtin_fma(pc, z, 0x1.1eed8eff8d898p-29);
+    pc = __builtin_fma(pc, z, -0x1.27e4fb7789f5cp-22);
+    pc = __builtin_fma(pc, z, 0x1.a01a01a01a01ap-16);
+    pc = __builtin_fma(pc, z, -0x1.6c16c16c16c17p-10);
+    pc = __builtin_fma(pc, z, 0x1.5555555555555p-5);
+    const double hz = 0.5 * z;
+    const double w = 1.0 - hz;
+    double ch, cl;
+    fast_two_sum(w, ((1.0 - w) - hz) + __builtin_fma(z * z, pc, -(rh * rl)), &ch, &cl);
+    const bool odd = (q & 1) != 0;
+    const double nh = odd ? -ch : sh, nl = odd ? -cl : sl;
+    const double dh = odd ? sh : ch, dl = odd ? sl : cl;
+    const double th = nh / dh;
+    const double tl = (__builtin_fma(-th, dh, nh) + nl - th * dl) / dh;
+    return th + tl;
+}
+
+// atan(zh + zl) for 0 <= zh <= 1 (zl a tail) as hi + lo, or pi/2 minus it (co): atan(c) from the table
+// plus atan(t), t = (z - c) / (1 + z c), c = k/16 the nearest sixteenth, |t| <= 1/32; atan(t) to t^13.
+__device__ __forceinline__ void atan_core(double zh, double zl, bool co, double* hi, double* lo)
+{
+    double kd = __builtin_rint(zh * 16.0);
+    kd = (kd >= 0.0 && kd <= 16.0) ? kd : 0.0;
+    const int k = (int)kd;
+    const double c = kd * 0.0625;
+    const double num = zh - c; // exact
+    double nh, nl;
+    two_sum(num, zl, &nh, &nl);
+    const double ph = zh * c;
+    const double pe = __builtin_fma(zh, c, -ph);
+    double dh, dl;
+    fast_two_sum(1.0, ph, &dh, &dl);
+    dl += pe;
+    const double th = nh / dh;
+    const double tl = (__builtin_fma(-th, dh, nh) + nl - th * dl) / dh;
+    const double w = th * th;
+    double P = 1.0 / 13.0;
+    P = __builtin_fma(P, w, -1.0 / 11.0);
+    P = __builtin_fma(P, w, 1.0 / 9.0);
+    P = __builtin_fma(P, w, -1.0 / 7.0);
+    P = __builtin_fma(P, w, 1.0 / 5.0);
+    P = __builtin_fma(P, w, -1.0 / 3.0);
+    const double tail = __builtin_fma(th * w, P, tl); // atan(t) - th
+    const double bh = co ? kAtanCoK[k][0] : kAtanK[k][0];
+    const double bl = co ? kAtanCoK[k][1] : kAtanK[k][1];
+    double s, e;
+    two_sum(bh, co ? -th : th, &s, &e);
+    fast_two_sum(s, e + (bl + (co ? -tail : tail)), hi, lo);
+}
+
+__device__ __forceinline__ double atan(double x)
+{
+    if (x != x) return x + x;
+    const double a = __builtin_fabs(x);
+    const bool big = a > 1.0;
+    double zh = a, zl = 0.0;
+    if (big) {
+        zh = 1.0 / a;
+        zl = (a <= 0x1.fffffffffffffp+1023) ? __builtin_fma(-zh, a, 1.0) / a : 0.0;
+    }
+    double h, l;
+    atan_core(zh, zl, big, &h, &l);
+    const double r = h + l;
+    return copysign(r, x);
+}
+
+// asin(a) for 0 <= a <= 1/2 as hi + lo: atan(a / sqrt(1 - a^2)), every step an unevaluated sum
+__device__ __forceinline__ void asin_small(double a, double* hi, double* lo)
+{
+    const double p = a * a;
+    const double pe = __builtin_fma(a, a, -p);
+    double wh, wl;
+    fast_two_sum(1.0, -p, &wh, &wl);
+    wl -= pe;
+    const double sh = __builtin_sqrt(wh);
+    const double sl = (__builtin_fma(-sh, sh, wh) + wl) / (2.0 * sh);
+    const double zh = a / sh;
+    const double zl = (__builtin_fma(-zh, sh, a) - zh * sl) / sh;
+    atan_core(zh, zl, false, hi, lo);
+}
+
+// asin(s) for s = sqrt(v), 0 <= v <= 1/4 (v exact), as hi + lo
+__device__ __forceinline__ void asin_of_root(double v, double* hi, double* lo)
+{
+    const double sh = __builtin_sqrt(v);
+    const double sl = sh > 0.0 ? __builtin_fma(-sh, sh, v) / (2.0 * sh) : 0.0;
+    double wh, wl;
+    fast_two_sum(1.0, -v, &wh, &wl); // 1 - s^2 = 1 - v
+    const double qh = __builtin_sqrt(wh);
+    const double ql = (__builtin_fma(-qh, qh, wh) + wl) / (2.0 * qh);
+    const double zh = sh / qh;
+    const double zl = (__builtin_fma(-zh, qh, sh) + sl - zh * ql) / qh;
+    atan_core(zh, zl, false, hi, lo);
+}
+
+__device__ __forceinline__ double asin(double x)
+{
+    const double a = __builtin_fabs(x);
+    if (!(a <= 1.0)) return (x - x) / (x - x);
+    double h, l, r;
+    if (a <= 0.5) {
+        asin_small(a, &h, &l);
+        r = h + l;
+    } else {
+        // asin(a) = pi/2 - 2 asin(sqrt((1 - a) / 2))
+        asin_of_root((1.0 - a) * 0.5, &h, &l);
+        double s, e;
+        two_sum(kPio2H, -2.0 * h, &s, &e);
+        r = s + (e + (kPio2L - 2.0 * l));
+    }
+    return copysign(r, x);
+}
+
+__device__ __forceinline__ double acos(double x)
+{
+    const double a = __builtin_fabs(x);
+    if (!(a <= 1.0)) return (x - x) / (x - x);
+    double h, l, s, e;
+    if (a <= 0.5) {
+        asin_small(a, &h, &l); // pi/2 - asin(x)
+        if (x < 0) {
+            h = -h;
+            l = -l;
+        }
+        two_sum(kPio2H, -h, &s, &e);
+        return s + (e + (kPio2L - l));
+    }
+    asin_of_root((1.0 - a) * 0.5, &h, &l);
+    if (x > 0) return 2.0 * h + 2.0 * l; // 2 asin(sqrt((1 - x) / 2))
+    two_sum(kPiH, -2.0 * h, &s, &e);     // pi - 2 asin(sqrt((1 + x) / 2))
+    return s + (e + (kPiL - 2.0 * l));
+}
+
+// tanh: |x| < 0.625 its Taylor series to x^43; above 1 - 2 / (exp(2|x|) + 1)
+__device__ __forceinline__ double tanh(double x)
+{
+    const double a = __builtin_fabs(x);
+    double r;
+    if (a < 0.625) {
+        const double w = a * a;
+        double P = -0x1.42ba1a349b490p-28;
+        P = __builtin_fma(P, w, 0x1.8e25ff9327e2cp-27);
+        P = __builtin_fma(P, w, -0x1.eb3229047434cp-26);
+        P = __builtin_fma(P, w, 0x1.2efe8db3aff1fp-24);
+        P = __builtin_fma(P, w, -0x1.75cde6563fed9p-23);
+        P = __builtin_fma(P, w, 0x1.cd299de4ae6bbp-22);
+        P = __builtin_fma(P, w, -0x1.1c77df95c1c0dp-20);
+        P = __builtin_fma(P, w, 0x1.5ef2da474e5b7p-19);
+        P = __builtin_fma(P, w, -0x1.b0f72d3ee24e9p-18);
+        P = __builtin_fma(P, w, 0x1.0b132d39a6050p-16);
+        P = __builtin_fma(P, w, -0x1.497d8eea25259p-15);
+        P = __builtin_fma(P, w, 0x1.967e18afcafadp-14);
+        P = __builtin_fma(P, w, -0x1.f57d7734d1664p-13);
+        P = __builtin_fma(P, w, 0x1.3558248036744p-11);
+        P = __builtin_fma(P, w, -0x1.7da36452b75e3p-10);
+        P = __builtin_fma(P, w, 0x1.d6d3d0e157de0p-9);
+        P = __builtin_fma(P, w, -0x1.226e355e6c23dp-7);
+        P = __builtin_fma(P, w, 0x1.664f4882c10fap-6);
+        P = __builtin_fma(P, w, -0x1.ba1ba1ba1ba1cp-5);
+        P = __builtin_fma(P, w, 0x1.1111111111111p-3);
+        P = __builtin_fma(P, w, -0x1.5555555555555p-2);
+        r = __builtin_fma(a * w, P, a);
+    } else {
+        const double e = exp(2.0 * a);
+        r = 1.0 - 2.0 / (e + 1.0);
+    }
+    return copysign(r, x);
+}
+
+// log(a) for finite a > 0 as hi + lo to about 2^-66 relative: a = 2^k m, m in [sqrt(2)/2, sqrt(2)),
+// s = f / (2 + f) as an unevaluated sum, log m = 2s + 2s^3/3 + 2s^5 Q(s^2) with the cubic term as an
+// unevaluated product, k ln 2 in two parts.
+__device__ __forceinline__ void log_dd(double a, double* hi, double* lo)
+{
+    const bool sub = a < 0x1p-1022;
+    const double as = sub ? a * 0x1p54 : a;
+    const uint64_t u = bits_of(as);
+    int k = (int)(u >> 52) - 1023 - (sub ? 54 : 0);
+    uint64_t mb = (u & 0xFFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL;
+    const bool half = mb > 0x3FF6A09E667F3BCCULL;
+    mb = half ? mb - 0x0010000000000000ULL : mb;
+    k += half ? 1 : 0;
+    const double f = from_bits(mb) - 1.0; // exact
+    double dh, dl;
+    fast_two_sum(2.0, f, &dh, &dl);
+    const double sh = f / dh;
+    const double sl = (__builtin_fma(-sh, dh, f) - sh * dl) / dh;
+    const double w = sh * sh;
+    const double wl = __builtin_fma(sh, sh, -w);
+    const double c3 = w * sh;
+    const double c3l = __builtin_fma(w, sh, -c3) + wl * sh;
+    const double m3 = c3 * 0x1.5555555555555p-1; // s^3 2/3
+    const double m3l = __builtin_fma(c3, 0x1.5555555555555p-1, -m3) +
+                       (c3l * 0x1.5555555555555p-1 + c3 * 0x1.5555555555555p-55);
+    double Q = 2.0 / 25.0;
+    Q = __builtin_fma(Q, w, 2.0 / 23.0);
+    Q = __builtin_fma(Q, w, 2.0 / 21.0);
+    Q = __builtin_fma(Q, w, 2.0 / 19.0);
+    Q = __builtin_fma(Q, w, 2.0 / 17.0);
+    Q = __builtin_fma(Q, w, 2.0 / 15.0);
+    Q = __builtin_fma(Q, w, 2.0 / 13.0);
+    Q = __builtin_fma(Q, w, 2.0 / 11.0);
+    Q = __builtin_fma(Q, w, 2.0 / 9.0);
+    Q = __builtin_fma(Q, w, 2.0 / 7.0);
+    Q = __builtin_fma(Q, w, 2.0 / 5.0);
+    const double tail = c3 * w * Q; // 2 s^5 / 5 + ...
+    double g, ge;
+    two_sum(2.0 * sh, m3, &g, &ge);
+    // 2 sl (1 + s^2): the tail of s through the linear and cubic terms
+    const double low = ge + (__builtin_fma(2.0 * w, sl, 2.0 * sl) + (m3l + tail));
+    const double dk = (double)k;
+    double s1, e1;
+    two_sum(dk * kLn2Hi42, g, &s1, &e1);
+    fast_two_sum(s1, e1 + (low + dk * kLn2Lo42), hi, lo);
+}
+
+// exp(xh + xl), |xl| tiny next to xh, xh in [-746, 710]: exp's reduction with the tail added
+__device__ __forceinline__ double exp_dd(double xh, double xl)
+{
+    const double n = __builtin_rint(xh * kLog2e);
+    const double r1 = __builtin_fma(-n, kLn2Hi, xh);
+    const double r = __builtin_fma(-n, kLn2Lo, r1);
+    const double rl = __builtin_fma(-n, kLn2Lo, r1 - r) + xl;
+    double q = 0x1.6124613a86d09p-33;
+    q = __builtin_fma(q, r, 0x1.1eed8eff8d898p-29);
+    q = __builtin_fma(q, r, 0x1.ae64567f544e4p-26);
+    q = __builtin_fma(q, r, 0x1.27e4fb7789f5cp-22);
+    q = __builtin_fma(q, r, 0x1.71de3a556c734p-19);
+    q = __builtin_fma(q, r, 0x1.a01a01a01a01ap-16);
+    q = __builtin_fma(q, r, 0x1.a01a01a01a01ap-13);
+    q = __builtin_fma(q, r, 0x1.6c16c16c16c17p-10);
+    q = __builtin_fma(q, r, 0x1.1111111111111p-7);
+    q = __builtin_fma(q, r, 0x1.5555555555555p-5);
+    q = __builtin_fma(q, r, 0x1.5555555555555p-3);
+    q = __builtin_fma(q, r, 0.5);
+    const double uu = __builtin_fma(r * r, q, r);
+    // exp(r + rl) = (1 + uu)(1 + rl) to first order in rl: the tail carries the argument's low part (up to
+    // 2^-53 |xh|), so its product with uu is kept
+    const double p = 1.0 + (uu + __builtin_fma(rl, uu, rl));
+    return __builtin_ldexp(p, (int)n);
+}
+
+// 0: not an integer (or not finite); 1: an even integer; 2: an odd integer
+__device__ __forceinline__ int pow_int_kind(double y)
+{
+    if (!(__builtin_fabs(y) <= 0x1.fffffffffffffp+1023)) return 0;
+    if (__builtin_rint(y) != y) return 0;
+    if (__builtin_fabs(y) >= 0x1p53) return 1;
+    const double h = y * 0.5;
+    return __builtin_rint(h) != h ? 2 : 1;
+}
+
+// pow with C99 Annex F special cases; |x|^y = exp(y log|x|) with both steps as unevaluated sums
+__device__ __forceinline__ double pow(double x, double y)
+{
+    if (y == 0.0) return 1.0;
+    if (x == 1.0) return 1.0;
+    if (x != x || y != y) return x + y;
+    const int ik = pow_int_kind(y);
+    const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+    if (ay > 0x1.fffffffffffffp+1023) {
+        if (ax == 1.0) return 1.0;
+        return ((ax < 1.0) == (y < 0)) ? __builtin_inf() : 0.0;
+    }
+    const bool neg = x < 0 || (x == 0.0 && __builtin_signbit(x));
+    const double sgn = (neg && ik == 2) ? -1.0 : 1.0;
+    if (x == 0.0) return y < 0 ? sgn / 0.0 : sgn * 0.0;
+    if (ax > 0x1.fffffffffffffp+1023) return y < 0 ? sgn * 0.0 : sgn / 0.0;
+    if (x < 0 && ik == 0) return (x - x) / (x - x);
+    double lh, ll;
+    log_dd(ax, &lh, &ll);
+    const double ph = y * lh;
+    const double pl = __builtin_fma(y, lh, -ph) + y * ll;
+    if (ph > 710.0) return sgn / 0.0;
+    if (ph < -746.0) return sgn * 0.0;
+    return sgn * exp_dd(ph, pl);
+}
+
+// 1 / sqrt(x), two correctly rounded operations (the reference's host rsqrt, math/root.h:74)
+__device__ __forceinline__ double rsqrt(double x) { return 1.0 / __builtin_sqrt(x); }
+__device__ __forceinline__ float rsqrtf(float x) { return 1.0f / __builtin_sqrtf(x); }
+// math/math.h:161-171: 1, -1, or 0 (zeros and NaN)
+template<class T>
+__device__ __forceinline__ T sign(T x)
+{
+    return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0));
+}
+// fmin / fmax as glibc's generic form: the other operand when one is NaN; on equal values (+-0) the second
+template<class T>
+__device__ __forceinline__ T fmin(T a, T b)
+{
+    return (a < b || b != b) ? a : b;
+}
+template<class T>
+__device__ __forceinline__ T fmax(T a, T b)
+{
+    return (a > b || b != b) ? a : b;
+}
+// the reductions' ordered max (expr/Reduce.h MaxOp): a > b ? a : b
+template<class T>
+__device__ __forceinline__ T selgt(T a, T b)
+{
+    return a > b ? a : b;
+}
+
+// float forms: the double routine on the widened argument, one rounding
+__device__ __forceinline__ float tanf(float x) { return (float)tan((double)x); }
+__device__ __forceinline__ float atanf(float x) { return (float)atan((double)x); }
+__device__ __forceinline__ float asinf(float x) { return (float)asin((double)x); }
+__device__ __forceinline__ float acosf(float x) { return (float)acos((double)x); }
+__device__ __forceinline__ float tanhf(float x) { return (float)tanh((double)x); }
+__device__ __forceinline__ float powf(float x, float y) { return (float)pow((double)x, (double)y); }
+
 } // namespace dm
 } // namespace aeth

@@ -115,6 +115,15 @@ __device__ __forceinline__ void put(T (&S)[kDepth][VN], int d, const T (&x)[VN])
     }
 }
 
+__host__ __device__ constexpr bool is_unary(int op)
+{
+    return (op >= AETH_OP_NEG && op <= AETH_OP_ABS) || (op >= AETH_OP_SIGN && op <= AETH_OP_TANH);
+}
+__host__ __device__ constexpr bool is_binary(int op)
+{
+    return (op >= AETH_OP_ADD && op <= AETH_OP_DIV) || (op >= AETH_OP_FMIN && op <= AETH_OP_POW);
+}
+
 template<class T>
 __device__ __forceinline__ T unary(int op, T a)
 {
@@ -126,6 +135,12 @@ __device__ __forceinline__ T unary(int op, T a)
         case AETH_OP_EXP: return dm::exp(a);
         case AETH_OP_LOG: return dm::log(a);
         case AETH_OP_SQRT: return __builtin_sqrt(a);
+        case AETH_OP_SIGN: return dm::sign(a);
+        case AETH_OP_TAN: return dm::tan(a);
+        case AETH_OP_ATAN: return dm::atan(a);
+        case AETH_OP_ASIN: return dm::asin(a);
+        case AETH_OP_ACOS: return dm::acos(a);
+        case AETH_OP_TANH: return dm::tanh(a);
         default: return __builtin_fabs(a);
         }
     } else {
@@ -136,6 +151,12 @@ __device__ __forceinline__ T unary(int op, T a)
         case AETH_OP_EXP: return dm::expf(a);
         case AETH_OP_LOG: return dm::logf(a);
         case AETH_OP_SQRT: return __builtin_sqrtf(a);
+        case AETH_OP_SIGN: return dm::sign(a);
+        case AETH_OP_TAN: return dm::tanf(a);
+        case AETH_OP_ATAN: return dm::atanf(a);
+        case AETH_OP_ASIN: return dm::asinf(a);
+        case AETH_OP_ACOS: return dm::acosf(a);
+        case AETH_OP_TANH: return dm::tanhf(a);
         default: return __builtin_fabsf(a);
         }
     }
@@ -148,6 +169,12 @@ __device__ __forceinline__ T binary(int op, T a, T b)
     case AETH_OP_ADD: return a + b;
     case AETH_OP_SUB: return a - b;
     case AETH_OP_MUL: return a * b;
+    case AETH_OP_FMIN: return dm::fmin(a, b);
+    case AETH_OP_FMAX: return dm::fmax(a, b);
+    case AETH_OP_SELGT: return dm::selgt(a, b);
+    case AETH_OP_POW:
+        if constexpr (sizeof(T) == 8) return dm::pow(a, b);
+        else return dm::powf(a, b);
     default: return a / b;
     }
 }
@@ -199,7 +226,7 @@ __device__ __forceinline__ void run(const aeth_program& P, const TapeArgs& A, co
             for (int k = 0; k < VN; ++k) x[k] = draw<T>(r, first_sample + k);
             put(S, d, x);
             ++d;
-        } else if (op >= AETH_OP_NEG) {
+        } else if (is_unary(op)) {
             get(S, d - 1, x);
 #pragma unroll
             for (int k = 0; k < VN; ++k) x[k] = unary<T>(op, x[k]);
@@ -323,10 +350,10 @@ void validate(const aeth_program& p, int n_leaves, int n_consts, int n_rngs)
         } else if (op == AETH_OP_RNG) {
             if (arg >= n_rngs) fail("aeth_eval_tape", "RNG index out of range");
             ++d;
-        } else if (op >= AETH_OP_ADD && op <= AETH_OP_DIV) {
+        } else if (is_binary(op)) {
             if (d < 2) fail("aeth_eval_tape", "stack underflow");
             --d;
-        } else if (op >= AETH_OP_NEG && op <= AETH_OP_ABS) {
+        } else if (is_unary(op)) {
             if (d < 1) fail("aeth_eval_tape", "stack underflow");
         } else {
             fail("aeth_eval_tape", "unknown opcode");
@@ -519,14 +546,23 @@ std::string gen_program(const aeth_program& P, bool dbl)
             const std::string t = fresh();
             b += "          const T " + t + " = jit_draw<T>(A.rng[A.rng_arg[pi][" + std::to_string(i) + "]], i0 + k);\n";
             st.push_back(t);
-        } else if (op >= AETH_OP_ADD && op <= AETH_OP_DIV) {
+        } else if (is_binary(op)) {
             const std::string y = st.back();
             st.pop_back();
             const std::string x = st.back();
             st.pop_back();
-            const char* sym = op == AETH_OP_ADD ? "+" : op == AETH_OP_SUB ? "-" : op == AETH_OP_MUL ? "*" : "/";
             const std::string t = fresh();
-            b += "          const T " + t + " = " + x + " " + sym + " " + y + ";\n";
+            if (op >= AETH_OP_ADD && op <= AETH_OP_DIV) {
+                const char* sym = op == AETH_OP_ADD ? "+" : op == AETH_OP_SUB ? "-" : op == AETH_OP_MUL ? "*" : "/";
+                b += "          const T " + t + " = " + x + " " + sym + " " + y + ";\n";
+            } else {
+                const char* fn = op == AETH_OP_FMIN    ? "aeth::dm::fmin("
+                                 : op == AETH_OP_FMAX  ? "aeth::dm::fmax("
+                                 : op == AETH_OP_SELGT ? "aeth::dm::selgt("
+                                 : dbl                 ? "aeth::dm::pow("
+                                                       : "aeth::dm::powf(";
+                b += "          const T " + t + " = " + fn + x + ", " + y + ");\n";
+            }
             st.push_back(t);
         } else {
             const std::string x = st.back();
@@ -539,6 +575,12 @@ std::string gen_program(const aeth_program& P, bool dbl)
             case AETH_OP_EXP: f = std::string(dbl ? "aeth::dm::exp(" : "aeth::dm::expf(") + x + ")"; break;
             case AETH_OP_LOG: f = std::string(dbl ? "aeth::dm::log(" : "aeth::dm::logf(") + x + ")"; break;
             case AETH_OP_SQRT: f = std::string(dbl ? "__builtin_sqrt(" : "__builtin_sqrtf(") + x + ")"; break;
+            case AETH_OP_SIGN: f = "aeth::dm::sign(" + x + ")"; break;
+            case AETH_OP_TAN: f = std::string(dbl ? "aeth::dm::tan(" : "aeth::dm::tanf(") + x + ")"; break;
+            case AETH_OP_ATAN: f = std::string(dbl ? "aeth::dm::atan(" : "aeth::dm::atanf(") + x + ")"; break;
+            case AETH_OP_ASIN: f = std::string(dbl ? "aeth::dm::asin(" : "aeth::dm::asinf(") + x + ")"; break;
+            case AETH_OP_ACOS: f = std::string(dbl ? "aeth::dm::acos(" : "aeth::dm::acosf(") + x + ")"; break;
+            case AETH_OP_TANH: f = std::string(dbl ? "aeth::dm::tanh(" : "aeth::dm::tanhf(") + x + ")"; break;
             default: f = std::string(dbl ? "__builtin_fabs(" : "__builtin_fabsf(") + x + ")"; break;
             }
             const std::string t = fresh();

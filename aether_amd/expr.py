@@ -24,7 +24,8 @@ from . import AETH_RNG_DIST_NORMAL, AETH_RNG_DIST_UNIFORM, Array, _dtype_code, s
 from ._lib import call
 
 OP = {"push": 0, "const": 1, "rng": 2, "+": 3, "-": 4, "*": 5, "/": 6, "neg": 7, "sin": 8, "cos": 9, "exp": 10,
-      "log": 11, "sqrt": 12, "abs": 13}
+      "log": 11, "sqrt": 12, "abs": 13, "sign": 14, "tan": 15, "atan": 16, "asin": 17, "acos": 18, "tanh": 19,
+      "fmin": 20, "fmax": 21, "selgt": 22, "pow": 23}
 MAX_LOADS, MAX_INSNS, MAX_PROGRAMS, MAX_LEAVES, MAX_CONSTS, MAX_RNGS = 16, 64, 8, 16, 16, 8
 
 
@@ -74,6 +75,30 @@ class Expr:
     def cross(self, o): return Cross(self, _wrap(o))
     def squared_norm(self): return Dot(self, self)
     def norm(self): return Un("sqrt", Dot(self, self))
+    # expr/Expression.h:196-242: the reciprocal / cubed norms, the ordered max-abs and the sum
+    def r_norm(self): return Bin("/", Const(1.0), self.norm())
+    def r_squared_norm(self): return Bin("/", Const(1.0), Dot(self, self))
+    def cubed_norm(self): return Bin("*", Dot(self, self), self.norm())
+    def r_cubed_norm(self): return Bin("/", Const(1.0), self.cubed_norm())
+    def max_norm(self): return Fold(self, True)
+    def sum(self): return Fold(self, False)
+    def unit_vector(self): return Bin("*", self, self.r_norm())
+    def cwise_mul(self, o): return Bin("*", self, _wrap(o))
+    # expr/nodes/Quaternion.h, expr/nodes/Structural.h
+    def quat_mul(self, o): return QuatMul(self, _wrap(o))
+    def quat_conj(self): return QuatConj(self)
+    def quat_reciprocal(self): return Bin("*", QuatConj(self), self.r_squared_norm())
+    def quat_rotate(self, v): return QuatRotate(self, _wrap(v))
+    def as_pure_quaternion(self): return PureQuat(self)
+    def segment(self, off, length): return Segment(self, off, length)
+    def head(self, n): return Segment(self, 0, n)
+    def tail(self, n): return Segment(self, self.ncomp - n, n)
+    def as_back_3d_vector(self): return Segment(self, self.ncomp - 3, 3)
+    # the reference's spellings
+    rNorm, rSquaredNorm, cubedNorm, rCubedNorm, maxNorm = r_norm, r_squared_norm, cubed_norm, r_cubed_norm, max_norm
+    squaredNorm, unitVector, cwiseMul, quatMul, quatConj = squared_norm, unit_vector, cwise_mul, quat_mul, quat_conj
+    quatReciprocal, quatRotate, asPureQuaternion, asBack3DVector = (quat_reciprocal, quat_rotate, as_pure_quaternion,
+                                                                     as_back_3d_vector)
 
     def emit(self, c: int, ctx: "_Ctx") -> None:
         raise NotImplementedError
@@ -191,6 +216,134 @@ class Cross(Expr):
         ctx.op("-")
 
 
+class Fold(Expr):
+    """expr/Reduce.h:150-164: acc = T(x_0); acc = op(T(x_{k+1}), acc). sum: T = x, op = +; max-abs:
+    T(x) = x < 0 ? -x : x (as selgt(-x, x)), op = the ordered max selgt(a, b) = a > b ? a : b. All
+    terms are pushed first (highest component first) so each op's left operand is the new term."""
+
+    ncomp = 1
+
+    def __init__(self, e: Expr, is_max: bool):
+        self.e, self.is_max = e, is_max
+
+    def emit(self, c, ctx):
+        n = self.e.ncomp
+        for k in range(n - 1, -1, -1):
+            self.e.emit(k, ctx)
+            if self.is_max:
+                ctx.op("neg")
+                self.e.emit(k, ctx)
+                ctx.op("selgt")
+        for _ in range(n - 1):
+            ctx.op("selgt" if self.is_max else "+")
+
+
+class QuatMul(Expr):
+    """Quaternion.h:35-80, each component's sum left to right as written there."""
+
+    ncomp = 4
+    _B = [(0, 1, 2, 3), (1, 0, 3, 2), (2, 3, 0, 1), (3, 2, 1, 0)]
+    _S = [("-", "-", "-"), ("+", "+", "-"), ("-", "+", "+"), ("+", "-", "+")]
+
+    def __init__(self, l: Expr, r: Expr):
+        if l.ncomp != 4 or r.ncomp != 4:
+            raise ValueError("quat_mul needs quaternions")
+        self.l, self.r = l, r
+
+    def emit(self, c, ctx):
+        for t in range(4):
+            self.l.emit(t, ctx)
+            self.r.emit(self._B[c][t], ctx)
+            ctx.op("*")
+            if t:
+                ctx.op(self._S[c][t - 1])
+
+
+class QuatConj(Expr):
+    """Quaternion.h:83-110: (x0, -x1, -x2, -x3), true negation."""
+
+    ncomp = 4
+
+    def __init__(self, e: Expr):
+        if e.ncomp != 4:
+            raise ValueError("quat_conj needs a quaternion")
+        self.e = e
+
+    def emit(self, c, ctx):
+        self.e.emit(c, ctx)
+        if c:
+            ctx.op("neg")
+
+
+class QuatRotate(Expr):
+    """Quaternion.h:117-170: t = 2 (u x v), v' = v + w t + u x t (u the vector part), sums as written."""
+
+    ncomp = 3
+    _I, _J = (1, 2, 0), (2, 0, 1)
+
+    def __init__(self, q: Expr, v: Expr):
+        if q.ncomp != 4 or v.ncomp != 3:
+            raise ValueError("quat_rotate needs a quaternion and a 3-vector")
+        self.q, self.v = q, v
+
+    def _t(self, c, ctx):
+        i, j = self._I[c], self._J[c]
+        ctx.const(2.0)
+        self.q.emit(1 + i, ctx)
+        self.v.emit(j, ctx)
+        ctx.op("*")
+        self.q.emit(1 + j, ctx)
+        self.v.emit(i, ctx)
+        ctx.op("*")
+        ctx.op("-")
+        ctx.op("*")
+
+    def emit(self, c, ctx):
+        i, j = self._I[c], self._J[c]
+        self.v.emit(c, ctx)
+        self.q.emit(0, ctx)
+        self._t(c, ctx)
+        ctx.op("*")
+        ctx.op("+")
+        self.q.emit(1 + i, ctx)
+        self._t(j, ctx)
+        ctx.op("*")
+        self.q.emit(1 + j, ctx)
+        self._t(i, ctx)
+        ctx.op("*")
+        ctx.op("-")
+        ctx.op("+")
+
+
+class PureQuat(Expr):
+    """Quaternion.h:173-205: (0, x0, x1, x2)."""
+
+    ncomp = 4
+
+    def __init__(self, e: Expr):
+        if e.ncomp != 3:
+            raise ValueError("as_pure_quaternion needs a 3-vector")
+        self.e = e
+
+    def emit(self, c, ctx):
+        if c == 0:
+            ctx.const(0.0)
+        else:
+            self.e.emit(c - 1, ctx)
+
+
+class Segment(Expr):
+    """Structural.h Segment: components off .. off + length - 1."""
+
+    def __init__(self, e: Expr, off: int, length: int):
+        if off < 0 or length < 1 or off + length > e.ncomp:
+            raise ValueError("segment out of range")
+        self.e, self.off, self.ncomp = e, off, length
+
+    def emit(self, c, ctx):
+        self.e.emit(self.off + c, ctx)
+
+
 def sin(e): return Un("sin", _wrap(e))
 def cos(e): return Un("cos", _wrap(e))
 def exp(e): return Un("exp", _wrap(e))
@@ -199,8 +352,24 @@ def sqrt(e): return Un("sqrt", _wrap(e))
 def abs_(e): return Un("abs", _wrap(e))
 
 
-# aether's names (expr/nodes/Elementwise.h:500-533)
+def sign(e): return Un("sign", _wrap(e))
+def tan(e): return Un("tan", _wrap(e))
+def atan(e): return Un("atan", _wrap(e))
+def asin(e): return Un("asin", _wrap(e))
+def acos(e): return Un("acos", _wrap(e))
+def tanh(e): return Un("tanh", _wrap(e))
+def rsqrt(e): return Bin("/", Const(1.0), Un("sqrt", _wrap(e)))  # math/root.h:74: 1 / sqrt(x)
+def minimum(a, b): return Bin("fmin", _wrap(a), _wrap(b))
+def maximum(a, b): return Bin("fmax", _wrap(a), _wrap(b))
+def clamp(e, lo, hi): return minimum(maximum(e, lo), hi)  # Elementwise.h:219-224
+def div(a, b): return Bin("/", _wrap(a), _wrap(b))
+def pow_(a, b): return Bin("pow", _wrap(a), _wrap(b))
+
+
+# aether's names (expr/nodes/Elementwise.h:194-851)
 cwiseSin, cwiseCos, cwiseExp, cwiseLog, cwiseSqrt, cwiseAbs = sin, cos, exp, log, sqrt, abs_
+cwiseSign, cwiseTan, cwiseAtan, cwiseAsin, cwiseAcos, cwiseTanh, cwiseRsqrt = sign, tan, atan, asin, acos, tanh, rsqrt
+cwiseMin, cwiseMax, cwiseDiv, cwisePow = minimum, maximum, div, pow_
 
 
 class _Ctx:

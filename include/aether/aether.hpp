@@ -226,8 +226,22 @@ template<class L, class R>
 struct CrossNode;
 template<class T>
 struct ScalarNode;
+template<class E, bool MAX>
+struct FoldNode;
+template<class L, class R>
+struct QuatMulNode;
+template<class E>
+struct QuatConjNode;
+template<class Q, class V>
+struct QuatRotateNode;
+template<class E>
+struct PureQuatNode;
+template<class E, std::size_t OFF, std::size_t LEN>
+struct SegmentNode;
 
-/** CRTP base (expr/Expression.h:145-486): members dot/cross/squaredNorm/norm. */
+/** CRTP base (expr/Expression.h:145-354): the per-sample reductions, geometric, quaternion and slicing
+ *  members. Each is a lazy node evaluated per sample (on the device: one tape program per component);
+ *  the folds keep the reference's order (expr/Reduce.h:150-164: acc = op(T(x_{k+1}), acc)). */
 template<class D>
 struct Expr {
     const D& self() const { return static_cast<const D&>(*this); }
@@ -242,7 +256,73 @@ struct Expr {
         return CrossNode<D, E>(self(), o);
     }
     auto squaredNorm() const { return DotNode<D, D>(self(), self()); }
-    auto norm() const { return UnaryNode<DotNode<D, D>, AETH_OP_SQRT>(DotNode<D, D>(self(), self())); }
+    auto norm() const { return UnaryNode<DotNode<D, D>, AETH_OP_SQRT>(squaredNorm()); }
+    /** 1 / sqrt(|v|^2) (Expression.h:202-206) */
+    auto rNorm() const { return BinaryNode<ScalarNode<typename D::element_type>, decltype(norm()), AETH_OP_DIV>(one(), norm()); }
+    /** 1 / |v|^2 */
+    auto rSquaredNorm() const
+    {
+        return BinaryNode<ScalarNode<typename D::element_type>, DotNode<D, D>, AETH_OP_DIV>(one(), squaredNorm());
+    }
+    /** |v|^2 sqrt(|v|^2) */
+    auto cubedNorm() const
+    {
+        return BinaryNode<DotNode<D, D>, decltype(norm()), AETH_OP_MUL>(squaredNorm(), norm());
+    }
+    /** 1 / (|v|^2 sqrt(|v|^2)) */
+    auto rCubedNorm() const
+    {
+        return BinaryNode<ScalarNode<typename D::element_type>, decltype(cubedNorm()), AETH_OP_DIV>(one(), cubedNorm());
+    }
+    /** max over components of (x < 0 ? -x : x), the ordered max (expr/Reduce.h:74-130) */
+    auto maxNorm() const { return FoldNode<D, true>(self()); }
+    /** sum of the components */
+    auto sum() const { return FoldNode<D, false>(self()); }
+    /** v * rNorm() (expr/nodes/Geometric.h UnitVector) */
+    auto unitVector() const { return BinaryNode<D, decltype(rNorm()), AETH_OP_MUL>(self(), rNorm()); }
+    /** the Hadamard product (expr/nodes/Product.h) */
+    template<class E>
+    auto cwiseMul(const E& o) const
+    {
+        return BinaryNode<D, E, AETH_OP_MUL>(self(), o);
+    }
+    /** Hamilton product (expr/nodes/Quaternion.h:35-80) */
+    template<class E>
+    auto quatMul(const E& o) const
+    {
+        return QuatMulNode<D, E>(self(), o);
+    }
+    auto quatConj() const { return QuatConjNode<D>(self()); }
+    auto quatReciprocal() const
+    {
+        return BinaryNode<QuatConjNode<D>, decltype(rSquaredNorm()), AETH_OP_MUL>(quatConj(), rSquaredNorm());
+    }
+    /** rotation of the 3-vector v by this unit quaternion (Quaternion.h:117-170) */
+    template<class V>
+    auto quatRotate(const V& v) const
+    {
+        return QuatRotateNode<D, V>(self(), v);
+    }
+    auto asPureQuaternion() const { return PureQuatNode<D>(self()); }
+    template<std::size_t OFF, std::size_t LEN>
+    auto segment() const
+    {
+        return SegmentNode<D, OFF, LEN>(self());
+    }
+    template<std::size_t N>
+    auto head() const
+    {
+        return segment<0, N>();
+    }
+    template<std::size_t N>
+    auto tail() const
+    {
+        return segment<D::ncomp - N, N>();
+    }
+    auto asBack3DVector() const { return tail<3>(); }
+
+private:
+    static auto one() { return ScalarNode<typename D::element_type>(1); }
 };
 
 template<class T>
@@ -276,6 +356,10 @@ struct BinaryNode : Expr<BinaryNode<L, R, OP>> {
         if constexpr (OP == AETH_OP_ADD) return a + b;
         else if constexpr (OP == AETH_OP_SUB) return a - b;
         else if constexpr (OP == AETH_OP_MUL) return a * b;
+        else if constexpr (OP == AETH_OP_FMIN) return std::fmin(a, b);
+        else if constexpr (OP == AETH_OP_FMAX) return std::fmax(a, b);
+        else if constexpr (OP == AETH_OP_SELGT) return a > b ? a : b;
+        else if constexpr (OP == AETH_OP_POW) return std::pow(a, b);
         else return a / b;
     }
 };
@@ -300,8 +384,182 @@ struct UnaryNode : Expr<UnaryNode<E, FN>> {
         else if constexpr (FN == AETH_OP_LOG) return std::log(a);
         else if constexpr (FN == AETH_OP_SQRT) return std::sqrt(a);
         else if constexpr (FN == AETH_OP_ABS) return std::fabs(a);
+        else if constexpr (FN == AETH_OP_SIGN) return a > 0 ? element_type(1) : (a < 0 ? element_type(-1) : element_type(0));
+        else if constexpr (FN == AETH_OP_TAN) return std::tan(a);
+        else if constexpr (FN == AETH_OP_ATAN) return std::atan(a);
+        else if constexpr (FN == AETH_OP_ASIN) return std::asin(a);
+        else if constexpr (FN == AETH_OP_ACOS) return std::acos(a);
+        else if constexpr (FN == AETH_OP_TANH) return std::tanh(a);
         else return -a;
     }
+};
+
+/** expr/Reduce.h:150-164: acc = T(x_0), then acc = op(T(x_{k+1}), acc); sum: T = x, op = +; maxNorm:
+ *  T(x) = x < 0 ? -x : x (as SELGT(-x, x)), op = the ordered max SELGT. All N terms are pushed first
+ *  (highest component first), so each op's left operand is the new term. */
+template<class E, bool MAX>
+struct FoldNode : Expr<FoldNode<E, MAX>> {
+    using element_type = elem_t<E>;
+    static constexpr std::size_t ncomp = 1;
+    E e;
+    explicit FoldNode(const E& x) : e(x) {}
+    void emit(std::size_t, detail::TapeCtx& x) const
+    {
+        for (std::size_t k = E::ncomp; k-- > 0;) {
+            e.emit(k, x);
+            if constexpr (MAX) {
+                x.op(AETH_OP_NEG);
+                e.emit(k, x);
+                x.op(AETH_OP_SELGT);
+            }
+        }
+        for (std::size_t k = 1; k < E::ncomp; ++k) x.op(MAX ? AETH_OP_SELGT : AETH_OP_ADD);
+    }
+    element_type term(std::size_t k, std::size_t i) const
+    {
+        const element_type a = e.eval(k, i);
+        if constexpr (MAX) return -a > a ? -a : a;
+        else return a;
+    }
+    element_type eval(std::size_t, std::size_t i) const
+    {
+        element_type acc = term(0, i);
+        for (std::size_t k = 1; k < E::ncomp; ++k) {
+            const element_type t = term(k, i);
+            if constexpr (MAX) acc = t > acc ? t : acc;
+            else acc = t + acc;
+        }
+        return acc;
+    }
+};
+
+/** expr/nodes/Quaternion.h:35-80: Hamilton product, left to right as written there */
+template<class L, class R>
+struct QuatMulNode : Expr<QuatMulNode<L, R>> {
+    using element_type = elem_t<L>;
+    static constexpr std::size_t ncomp = 4;
+    static_assert(L::ncomp == 4 && R::ncomp == 4, "aether: quatMul needs quaternions");
+    L l;
+    R r;
+    QuatMulNode(const L& a, const R& b) : l(a), r(b) {}
+    // component c = l[A0] r[B0] s1 l[A1] r[B1] s2 l[A2] r[B2] s3 l[A3] r[B3], s = +1 / -1
+    static constexpr int A[4][4] = {{0, 1, 2, 3}, {0, 1, 2, 3}, {0, 1, 2, 3}, {0, 1, 2, 3}};
+    static constexpr int B[4][4] = {{0, 1, 2, 3}, {1, 0, 3, 2}, {2, 3, 0, 1}, {3, 2, 1, 0}};
+    static constexpr int S[4][3] = {{-1, -1, -1}, {1, 1, -1}, {-1, 1, 1}, {1, -1, 1}};
+    void emit(std::size_t c, detail::TapeCtx& x) const
+    {
+        l.emit(A[c][0], x);
+        r.emit(B[c][0], x);
+        x.op(AETH_OP_MUL);
+        for (int t = 1; t < 4; ++t) {
+            l.emit(A[c][t], x);
+            r.emit(B[c][t], x);
+            x.op(AETH_OP_MUL);
+            x.op(S[c][t - 1] > 0 ? AETH_OP_ADD : AETH_OP_SUB);
+        }
+    }
+    element_type eval(std::size_t c, std::size_t i) const
+    {
+        element_type acc = l.eval(A[c][0], i) * r.eval(B[c][0], i);
+        for (int t = 1; t < 4; ++t) {
+            const element_type p = l.eval(A[c][t], i) * r.eval(B[c][t], i);
+            acc = S[c][t - 1] > 0 ? acc + p : acc - p;
+        }
+        return acc;
+    }
+};
+
+/** Quaternion.h:83-110: (x0, -x1, -x2, -x3), true negation */
+template<class E>
+struct QuatConjNode : Expr<QuatConjNode<E>> {
+    using element_type = elem_t<E>;
+    static constexpr std::size_t ncomp = 4;
+    static_assert(E::ncomp == 4, "aether: quatConj needs a quaternion");
+    E e;
+    explicit QuatConjNode(const E& x) : e(x) {}
+    void emit(std::size_t c, detail::TapeCtx& x) const
+    {
+        e.emit(c, x);
+        if (c > 0) x.op(AETH_OP_NEG);
+    }
+    element_type eval(std::size_t c, std::size_t i) const { return c > 0 ? -e.eval(c, i) : e.eval(c, i); }
+};
+
+/** Quaternion.h:117-170: t = 2 (u x v); v' = v + w t + u x t, each sum as written there */
+template<class Q, class V>
+struct QuatRotateNode : Expr<QuatRotateNode<Q, V>> {
+    using element_type = elem_t<Q>;
+    static constexpr std::size_t ncomp = 3;
+    static_assert(Q::ncomp == 4 && V::ncomp == 3, "aether: quatRotate needs a quaternion and a 3-vector");
+    Q q;
+    V v;
+    QuatRotateNode(const Q& a, const V& b) : q(a), v(b) {}
+    // t_c = 2 * (u[I] v[J] - u[J] v[I]) with u = q[1..3]
+    static constexpr std::size_t I[3] = {1, 2, 0}, J[3] = {2, 0, 1};
+    void emit_t(std::size_t c, detail::TapeCtx& x) const
+    {
+        x.constant(2.0);
+        q.emit(1 + I[c], x);
+        v.emit(J[c], x);
+        x.op(AETH_OP_MUL);
+        q.emit(1 + J[c], x);
+        v.emit(I[c], x);
+        x.op(AETH_OP_MUL);
+        x.op(AETH_OP_SUB);
+        x.op(AETH_OP_MUL);
+    }
+    void emit(std::size_t c, detail::TapeCtx& x) const
+    {
+        v.emit(c, x);
+        q.emit(0, x);
+        emit_t(c, x);
+        x.op(AETH_OP_MUL);
+        x.op(AETH_OP_ADD);
+        q.emit(1 + I[c], x);
+        emit_t(J[c], x);
+        x.op(AETH_OP_MUL);
+        q.emit(1 + J[c], x);
+        emit_t(I[c], x);
+        x.op(AETH_OP_MUL);
+        x.op(AETH_OP_SUB);
+        x.op(AETH_OP_ADD);
+    }
+    element_type t(std::size_t c, std::size_t i) const
+    {
+        return element_type(2) * (q.eval(1 + I[c], i) * v.eval(J[c], i) - q.eval(1 + J[c], i) * v.eval(I[c], i));
+    }
+    element_type eval(std::size_t c, std::size_t i) const
+    {
+        return v.eval(c, i) + q.eval(0, i) * t(c, i) + (q.eval(1 + I[c], i) * t(J[c], i) - q.eval(1 + J[c], i) * t(I[c], i));
+    }
+};
+
+/** Quaternion.h:173-205: (0, x0, x1, x2) */
+template<class E>
+struct PureQuatNode : Expr<PureQuatNode<E>> {
+    using element_type = elem_t<E>;
+    static constexpr std::size_t ncomp = 4;
+    static_assert(E::ncomp == 3, "aether: asPureQuaternion needs a 3-vector");
+    E e;
+    explicit PureQuatNode(const E& x) : e(x) {}
+    void emit(std::size_t c, detail::TapeCtx& x) const
+    {
+        if (c == 0) x.constant(0.0);
+        else e.emit(c - 1, x);
+    }
+    element_type eval(std::size_t c, std::size_t i) const { return c == 0 ? element_type(0) : e.eval(c - 1, i); }
+};
+
+/** expr/nodes/Structural.h Segment: components OFF .. OFF + LEN - 1 */
+template<class E, std::size_t OFF, std::size_t LEN>
+struct SegmentNode : Expr<SegmentNode<E, OFF, LEN>> {
+    using element_type = elem_t<E>;
+    static constexpr std::size_t ncomp = LEN;
+    static_assert(OFF + LEN <= E::ncomp && LEN > 0, "aether: segment out of range");
+    E e;
+    explicit SegmentNode(const E& x) : e(x) {}
+    void emit(std::size_t c, detail::TapeCtx& x) const { e.emit(OFF + c, x); }
+    element_type eval(std::size_t c, std::size_t i) const { return e.eval(OFF + c, i); }
 };
 
 /** expr/Reduce.h:172-188: (x_{N-1} y_{N-1}) + (... + (x_0 y_0)) */
@@ -460,6 +718,81 @@ auto cwiseAbs(const E& e) { return UnaryNode<E, AETH_OP_ABS>(e); }
 template<class E>
     requires aether_expression<E>
 auto sqrt(const E& e) { return UnaryNode<E, AETH_OP_SQRT>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseSign(const E& e) { return UnaryNode<E, AETH_OP_SIGN>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseTan(const E& e) { return UnaryNode<E, AETH_OP_TAN>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseAtan(const E& e) { return UnaryNode<E, AETH_OP_ATAN>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseAsin(const E& e) { return UnaryNode<E, AETH_OP_ASIN>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseAcos(const E& e) { return UnaryNode<E, AETH_OP_ACOS>(e); }
+template<class E>
+    requires aether_expression<E>
+auto cwiseTanh(const E& e) { return UnaryNode<E, AETH_OP_TANH>(e); }
+/** math/root.h:74 (host): T(1) / sqrt(x), two correctly rounded operations */
+template<class E>
+    requires aether_expression<E>
+auto cwiseRsqrt(const E& e) { return wrap_scalar<E>(1) / UnaryNode<E, AETH_OP_SQRT>(e); }
+/** Elementwise.h:198-210: min / max against a broadcast bound */
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto cwiseMin(const E& e, S bound)
+{
+    return BinaryNode<E, ScalarNode<elem_t<E>>, AETH_OP_FMIN>(e, wrap_scalar<E>(bound));
+}
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto cwiseMax(const E& e, S bound)
+{
+    return BinaryNode<E, ScalarNode<elem_t<E>>, AETH_OP_FMAX>(e, wrap_scalar<E>(bound));
+}
+/** Elementwise.h:761-775: min / max of two expressions */
+template<class L, class R>
+    requires aether_expression<L> && aether_expression<R>
+auto cwiseMin(const L& l, const R& r)
+{
+    return BinaryNode<L, R, AETH_OP_FMIN>(l, r);
+}
+template<class L, class R>
+    requires aether_expression<L> && aether_expression<R>
+auto cwiseMax(const L& l, const R& r)
+{
+    return BinaryNode<L, R, AETH_OP_FMAX>(l, r);
+}
+/** Elementwise.h:219-224: fmin(fmax(e, lo), hi) */
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto clamp(const E& e, S lo, S hi)
+{
+    return cwiseMin(cwiseMax(e, lo), hi);
+}
+/** Elementwise.h:742-746: l / r, both expressions */
+template<class L, class R>
+    requires aether_expression<L> && aether_expression<R>
+auto cwiseDiv(const L& l, const R& r)
+{
+    return BinaryNode<L, R, AETH_OP_DIV>(l, r);
+}
+/** Elementwise.h:749-760: e ^ exponent (a broadcast scalar) or e ^ r */
+template<class E, class S>
+    requires aether_expression<E> && std::is_arithmetic_v<S>
+auto cwisePow(const E& e, S exponent)
+{
+    return BinaryNode<E, ScalarNode<elem_t<E>>, AETH_OP_POW>(e, wrap_scalar<E>(exponent));
+}
+template<class L, class R>
+    requires aether_expression<L> && aether_expression<R>
+auto cwisePow(const L& l, const R& r)
+{
+    return BinaryNode<L, R, AETH_OP_POW>(l, r);
+}
 
 // ---------------------------------------------------------------- views
 template<class T, std::size_t N>

@@ -129,11 +129,18 @@ int aeth_eval_cfg0(int dtype, const void* a, const void* b, const void* c, void*
  *          sample, comp) as aeth_array_fill does     [-> x]
  *   ADD SUB MUL DIV                                  [a b -> a op b]
  *   NEG SIN COS EXP LOG SQRT ABS                     [a -> f(a)]
+ *   SIGN TAN ATAN ASIN ACOS TANH                     [a -> f(a)]
+ *   FMIN FMAX                                        [a b -> fmin/fmax(a, b)]
+ *   SELGT  (a > b ? a : b, the reductions' max)      [a b -> .]
+ *   POW                                              [a b -> a^b]
+ *   (expr/nodes/Elementwise.h:200-851, expr/Reduce.h:74-130, math/math.h:161-171)
  * Limits: 16 loads, 64 instructions, stack depth 8, 8 programs, 16 leaves,
  * 16 constants, 8 rng leaves per call. */
 enum { AETH_OP_PUSH = 0, AETH_OP_CONST = 1, AETH_OP_RNG = 2, AETH_OP_ADD = 3, AETH_OP_SUB = 4, AETH_OP_MUL = 5,
        AETH_OP_DIV = 6, AETH_OP_NEG = 7, AETH_OP_SIN = 8, AETH_OP_COS = 9, AETH_OP_EXP = 10, AETH_OP_LOG = 11,
-       AETH_OP_SQRT = 12, AETH_OP_ABS = 13 };
+       AETH_OP_SQRT = 12, AETH_OP_ABS = 13, AETH_OP_SIGN = 14, AETH_OP_TAN = 15, AETH_OP_ATAN = 16,
+       AETH_OP_ASIN = 17, AETH_OP_ACOS = 18, AETH_OP_TANH = 19, AETH_OP_FMIN = 20, AETH_OP_FMAX = 21,
+       AETH_OP_SELGT = 22, AETH_OP_POW = 23 };
 #define AETH_TAPE_MAX_LOADS 16
 #define AETH_TAPE_MAX_INSNS 64
 #define AETH_TAPE_MAX_DEPTH 8

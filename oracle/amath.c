@@ -410,6 +410,387 @@ float orc_logf(float x)
     return y;
 }
 
+/* ---- the rest of the reference's elementwise set (expr/nodes/Elementwise.h:200-851): tan, atan,
+ * asin, acos, tanh, pow, and the exact rsqrt / sign / fmin / fmax / ordered-max. Same style: every
+ * operation as the device states it; double-double (unevaluated sum) corrections where a plain
+ * evaluation would lose more than half an ulp. Constants: aether_amd/csrc/gen_math_constants.py. */
+
+static inline void two_sum(double a, double b, double* s, double* e)
+{
+    const double ss = a + b;
+    const double bb = ss - a;
+    *e = (a - (ss - bb)) + (b - bb);
+    *s = ss;
+}
+
+/* |a| >= |b| (or a == 0) */
+static inline void fast_two_sum(double a, double b, double* s, double* e)
+{
+    const double ss = a + b;
+    *e = b - (ss - a);
+    *s = ss;
+}
+
+/* atan(k/16) and pi/2 - atan(k/16), k = 0..16, as (hi, lo) */
+static const double kAtanK[17][2] = {
+    {0x0.0p+0, 0x0.0p+0},
+    {0x1.ff55bb72cfdeap-5, -0x1.c934d86d23f1dp-60},
+    {0x1.fd5ba9aac2f6ep-4, -0x1.cd37686760c17p-59},
+    {0x1.7b97b4bce5b02p-3, 0x1.347b0b4f881cap-58},
+    {0x1.f5b75f92c80ddp-3, 0x1.8ab6e3cf7afbdp-57},
+    {0x1.362773707ebccp-2, -0x1.963a544b672d8p-57},
+    {0x1.6f61941e4def1p-2, -0x1.c63aae6f6e918p-56},
+    {0x1.a64eec3cc23fdp-2, -0x1.24dec1b50b7ffp-56},
+    {0x1.dac670561bb4fp-2, 0x1.a2b7f222f65e2p-56},
+    {0x1.0657e94db30d0p-1, -0x1.d5b495f6349e6p-56},
+    {0x1.1e00babdefeb4p-1, -0x1.928df287a668fp-58},
+    {0x1.345f01cce37bbp-1, 0x1.1021137c71102p-55},
+    {0x1.4978fa3269ee1p-1, 0x1.2419a87f2a458p-56},
+    {0x1.5d58987169b18p-1, 0x1.0028e4bc5e7cap-57},
+    {0x1.700a7c5784634p-1, -0x1.8c34d25aadef6p-56},
+    {0x1.819d0b7158a4dp-1, -0x1.bf76229d3b917p-56},
+    {0x1.921fb54442d18p-1, 0x1.1a62633145c07p-55},
+};
+static const double kAtanCoK[17][2] = {
+    {0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54},
+    {0x1.82250768ac529p+0, -0x1.e78c96d05afcbp-58},
+    {0x1.7249faa996a21p+0, 0x1.a8cc1e7480c68p-54},
+    {0x1.62acbeaca61b8p+0, 0x1.c6ac9f134fa91p-60},
+    {0x1.5368c951e9cfdp+0, -0x1.96f47948a99f1p-54},
+    {0x1.4495d86823225p+0, 0x1.4d29adbab2a62p-54},
+    {0x1.3647503caf55cp+0, 0x1.17e21d9a42c9ap-55},
+    {0x1.288bfa3512419p+0, 0x1.8e684e7a2281bp-56},
+    {0x1.1b6e192ebbe44p+0, 0x1.b1b466a88828ep-54},
+    {0x1.0ef3c09d694b0p+0, 0x1.8fcf88aed2e80p-54},
+    {0x1.031f57e54adbep+0, 0x1.338b4259c0270p-54},
+    {0x1.efe068bba2275p-1, 0x1.24a3b2e61a70bp-55},
+    {0x1.dac670561bb4fp-1, 0x1.a2b7f222f65e2p-55},
+    {0x1.c6e6d2171bf18p-1, 0x1.f4ba8d3373e1bp-55},
+    {0x1.b434ee31013fdp-1, -0x1.0520d0701d877p-55},
+    {0x1.a2a25f172cfe4p-1, -0x1.d700509dad6cep-56},
+    {0x1.921fb54442d18p-1, 0x1.1a62633145c07p-55},
+};
+#define PIO2_H 0x1.921fb54442d18p+0
+#define PIO2_L 0x1.1a62633145c07p-54
+#define PI_H 0x1.921fb54442d18p+1
+#define PI_L 0x1.1a62633145c07p-53
+
+/* tan: sin and cos of the reduced argument as unevaluated sums (the kernels above before their last
+ * addition), then one double-double quotient; odd quadrants give -cos/sin. */
+double orc_tan(double x)
+{
+    const double ax = fabs(x);
+    if (!(ax <= 0x1.fffffffffffffp+1023)) return x - x;
+    if (ax < 0x1p-27) return x;
+    double rh, rl;
+    const int q = reduce(x, &rh, &rl);
+    const double z = rh * rh;
+    double ps = 0x1.952c77030ad4ap-49;
+    ps = fma(ps, z, -0x1.ae7f3e733b81fp-41);
+    ps = fma(ps, z, 0x1.6124613a86d09p-33);
+    ps = fma(ps, z, -0x1.ae64567f544e4p-26);
+    ps = fma(ps, z, 0x1.71de3a556c734p-19);
+    ps = fma(ps, z, -0x1.a01a01a01a01ap-13);
+    ps = fma(ps, z, 0x1.1111111111111p-7);
+    ps = fma(ps, z, -0x1.5555555555555p-3);
+    const double stail = fma(-0.5 * z, rl, rl);
+    double sh, sl;
+    fast_two_sum(rh, fma(rh * z, ps, stail), &sh, &sl);
+    double pc = 0x1.ae7f3e733b81fp-45;
+    pc = fma(pc, z, -0x1.93974a8c07c9dp-37);
+    pc = fma(pc, z, 0x1.1eed8eff8d898p-29);
+    pc = fma(pc, z, -0x1.27e4fb7789f5cp-22);
+    pc = fma(pc, z, 0x1.a01a01a01a01ap-16);
+    pc = fma(pc, z, -0x1.6c16c16c16c17p-10);
+    pc = fma(pc, z, 0x1.5555555555555p-5);
+    const double hz = 0.5 * z;
+    const double w = 1.0 - hz;
+    double ch, cl;
+    fast_two_sum(w, ((1.0 - w) - hz) + fma(z * z, pc, -(rh * rl)), &ch, &cl);
+    /* even q: (sh + sl) / (ch + cl); odd q: -(ch + cl) / (sh + sl) */
+    const int odd = q & 1;
+    const double nh = odd ? -ch : sh, nl = odd ? -cl : sl;
+    const double dh = odd ? sh : ch, dl = odd ? sl : cl;
+    const double th = nh / dh;
+    const double tl = (fma(-th, dh, nh) + nl - th * dl) / dh;
+    return th + tl;
+}
+
+/* atan(zh + zl) for 0 <= zh <= 1 (zl a tail), as hi + lo: atan(c) from the table plus atan(t),
+ * t = (z - c) / (1 + z c), c = k/16 the nearest sixteenth, |t| <= 1/32; atan(t) to t^13. */
+static void atan_core(double zh, double zl, int co, double* hi, double* lo)
+{
+    double kd = rint(zh * 16.0);
+    kd = (kd >= 0.0 && kd <= 16.0) ? kd : 0.0;
+    const int k = (int)kd;
+    const double c = kd * 0.0625;
+    const double num = zh - c; /* exact */
+    double nh, nl;
+    two_sum(num, zl, &nh, &nl);
+    const double ph = zh * c;
+    const double pe = fma(zh, c, -ph);
+    double dh, dl;
+    fast_two_sum(1.0, ph, &dh, &dl);
+    dl += pe;
+    const double th = nh / dh;
+    const double tl = (fma(-th, dh, nh) + nl - th * dl) / dh;
+    const double w = th * th;
+    double P = 1.0 / 13.0;
+    P = fma(P, w, -1.0 / 11.0);
+    P = fma(P, w, 1.0 / 9.0);
+    P = fma(P, w, -1.0 / 7.0);
+    P = fma(P, w, 1.0 / 5.0);
+    P = fma(P, w, -1.0 / 3.0);
+    const double tail = fma(th * w, P, tl); /* atan(t) - th */
+    const double bh = co ? kAtanCoK[k][0] : kAtanK[k][0];
+    const double bl = co ? kAtanCoK[k][1] : kAtanK[k][1];
+    double s, e;
+    two_sum(bh, co ? -th : th, &s, &e);
+    fast_two_sum(s, e + (bl + (co ? -tail : tail)), hi, lo);
+}
+
+double orc_atan(double x)
+{
+    if (x != x) return x + x;
+    const double a = fabs(x);
+    const int big = a > 1.0;
+    double zh = a, zl = 0.0;
+    if (big) {
+        zh = 1.0 / a;
+        zl = (a <= 0x1.fffffffffffffp+1023) ? fma(-zh, a, 1.0) / a : 0.0;
+    }
+    double h, l;
+    atan_core(zh, zl, big, &h, &l);
+    const double r = h + l;
+    return copysign(r, x);
+}
+
+/* asin(a) for 0 <= a <= 1/2 as hi + lo: atan(a / sqrt(1 - a^2)), every step an unevaluated sum */
+static void asin_small(double a, double* hi, double* lo)
+{
+    const double p = a * a;
+    const double pe = fma(a, a, -p);
+    double wh, wl;
+    fast_two_sum(1.0, -p, &wh, &wl);
+    wl -= pe;
+    const double sh = sqrt(wh);
+    const double sl = (fma(-sh, sh, wh) + wl) / (2.0 * sh);
+    const double zh = a / sh;
+    const double zl = (fma(-zh, sh, a) - zh * sl) / sh;
+    atan_core(zh, zl, 0, hi, lo);
+}
+
+/* asin(s) for s = sqrt(v), 0 <= v <= 1/4 (v exact), as hi + lo */
+static void asin_of_root(double v, double* hi, double* lo)
+{
+    const double sh = sqrt(v);
+    const double sl = sh > 0.0 ? fma(-sh, sh, v) / (2.0 * sh) : 0.0;
+    double wh, wl;
+    fast_two_sum(1.0, -v, &wh, &wl); /* 1 - s^2 = 1 - v */
+    const double qh = sqrt(wh);
+    const double ql = (fma(-qh, qh, wh) + wl) / (2.0 * qh);
+    const double zh = sh / qh;
+    const double zl = (fma(-zh, qh, sh) + sl - zh * ql) / qh;
+    atan_core(zh, zl, 0, hi, lo);
+}
+
+double orc_asin(double x)
+{
+    const double a = fabs(x);
+    if (!(a <= 1.0)) return (x - x) / (x - x); /* NaN: |x| > 1 or NaN */
+    double h, l, r;
+    if (a <= 0.5) {
+        asin_small(a, &h, &l);
+        r = h + l;
+    } else {
+        /* asin(a) = pi/2 - 2 asin(sqrt((1 - a) / 2)) */
+        asin_of_root((1.0 - a) * 0.5, &h, &l);
+        double s, e;
+        two_sum(PIO2_H, -2.0 * h, &s, &e);
+        r = s + (e + (PIO2_L - 2.0 * l));
+    }
+    return copysign(r, x);
+}
+
+double orc_acos(double x)
+{
+    const double a = fabs(x);
+    if (!(a <= 1.0)) return (x - x) / (x - x);
+    double h, l, s, e;
+    if (a <= 0.5) {
+        /* pi/2 - asin(x) */
+        asin_small(a, &h, &l);
+        if (x < 0) {
+            h = -h;
+            l = -l;
+        }
+        two_sum(PIO2_H, -h, &s, &e);
+        return s + (e + (PIO2_L - l));
+    }
+    asin_of_root((1.0 - a) * 0.5, &h, &l);
+    if (x > 0) return 2.0 * h + 2.0 * l; /* 2 asin(sqrt((1 - x) / 2)) */
+    two_sum(PI_H, -2.0 * h, &s, &e);     /* pi - 2 asin(sqrt((1 + x) / 2)) */
+    return s + (e + (PI_L - 2.0 * l));
+}
+
+/* tanh: |x| < 0.625 its Taylor series to x^43; above 1 - 2 / (exp(2|x|) + 1) */
+double orc_tanh(double x)
+{
+    const double a = fabs(x);
+    double r;
+    if (a < 0.625) {
+        const double w = a * a;
+        double P = -0x1.42ba1a349b490p-28;
+        P = fma(P, w, 0x1.8e25ff9327e2cp-27);
+        P = fma(P, w, -0x1.eb3229047434cp-26);
+        P = fma(P, w, 0x1.2efe8db3aff1fp-24);
+        P = fma(P, w, -0x1.75cde6563fed9p-23);
+        P = fma(P, w, 0x1.cd299de4ae6bbp-22);
+        P = fma(P, w, -0x1.1c77df95c1c0dp-20);
+        P = fma(P, w, 0x1.5ef2da474e5b7p-19);
+        P = fma(P, w, -0x1.b0f72d3ee24e9p-18);
+        P = fma(P, w, 0x1.0b132d39a6050p-16);
+        P = fma(P, w, -0x1.497d8eea25259p-15);
+        P = fma(P, w, 0x1.967e18afcafadp-14);
+        P = fma(P, w, -0x1.f57d7734d1664p-13);
+        P = fma(P, w, 0x1.3558248036744p-11);
+        P = fma(P, w, -0x1.7da36452b75e3p-10);
+        P = fma(P, w, 0x1.d6d3d0e157de0p-9);
+        P = fma(P, w, -0x1.226e355e6c23dp-7);
+        P = fma(P, w, 0x1.664f4882c10fap-6);
+        P = fma(P, w, -0x1.ba1ba1ba1ba1cp-5);
+        P = fma(P, w, 0x1.1111111111111p-3);
+        P = fma(P, w, -0x1.5555555555555p-2);
+        r = fma(a * w, P, a);
+    } else {
+        const double e = orc_exp(2.0 * a);
+        r = 1.0 - 2.0 / (e + 1.0);
+    }
+    return copysign(r, x);
+}
+
+/* log(a) for finite a > 0 as hi + lo to about 2^-66 relative: a = 2^k m, m in [sqrt(2)/2, sqrt(2)),
+ * s = f / (2 + f) as an unevaluated sum, log m = 2s + 2s^3/3 + 2s^5 Q(s^2) with the cubic term as an
+ * unevaluated product, k ln 2 in two parts. */
+static void log_dd(double a, double* hi, double* lo)
+{
+    uint64_t u = dbits(a);
+    int k = 0;
+    if ((u >> 52) == 0) {
+        a *= 0x1p54;
+        u = dbits(a);
+        k -= 54;
+    }
+    k += (int)(u >> 52) - 1023;
+    uint64_t mb = (u & 0xFFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL;
+    if (mb > 0x3FF6A09E667F3BCCULL) {
+        mb -= 0x0010000000000000ULL;
+        k += 1;
+    }
+    const double f = bitsd(mb) - 1.0; /* exact */
+    double dh, dl;
+    fast_two_sum(2.0, f, &dh, &dl);
+    const double sh = f / dh;
+    const double sl = (fma(-sh, dh, f) - sh * dl) / dh;
+    const double w = sh * sh;
+    const double wl = fma(sh, sh, -w);
+    const double c3 = w * sh;
+    const double c3l = fma(w, sh, -c3) + wl * sh;
+    const double m3 = c3 * 0x1.5555555555555p-1; /* s^3 2/3 */
+    const double m3l = fma(c3, 0x1.5555555555555p-1, -m3) + (c3l * 0x1.5555555555555p-1 + c3 * 0x1.5555555555555p-55);
+    double Q = 2.0 / 25.0;
+    Q = fma(Q, w, 2.0 / 23.0);
+    Q = fma(Q, w, 2.0 / 21.0);
+    Q = fma(Q, w, 2.0 / 19.0);
+    Q = fma(Q, w, 2.0 / 17.0);
+    Q = fma(Q, w, 2.0 / 15.0);
+    Q = fma(Q, w, 2.0 / 13.0);
+    Q = fma(Q, w, 2.0 / 11.0);
+    Q = fma(Q, w, 2.0 / 9.0);
+    Q = fma(Q, w, 2.0 / 7.0);
+    Q = fma(Q, w, 2.0 / 5.0);
+    const double tail = c3 * w * Q; /* 2 s^5 / 5 + ... */
+    double g, ge;
+    two_sum(2.0 * sh, m3, &g, &ge);
+    /* 2 sl (1 + s^2): the tail of s through the linear and cubic terms */
+    const double low = ge + (fma(2.0 * w, sl, 2.0 * sl) + (m3l + tail));
+    const double dk = (double)k;
+    double s1, e1;
+    two_sum(dk * LN2_HI42, g, &s1, &e1);
+    fast_two_sum(s1, e1 + (low + dk * LN2_LO42), hi, lo);
+}
+
+/* exp(xh + xl), |xl| tiny next to xh, xh in [-746, 710]: the reduction above with the tail added */
+static double exp_dd(double xh, double xl)
+{
+    const double n = rint(xh * LOG2E);
+    const double r1 = fma(-n, LN2_HI, xh);
+    const double r = fma(-n, LN2_LO, r1);
+    const double rl = fma(-n, LN2_LO, r1 - r) + xl;
+    double q = 0x1.6124613a86d09p-33;
+    q = fma(q, r, 0x1.1eed8eff8d898p-29);
+    q = fma(q, r, 0x1.ae64567f544e4p-26);
+    q = fma(q, r, 0x1.27e4fb7789f5cp-22);
+    q = fma(q, r, 0x1.71de3a556c734p-19);
+    q = fma(q, r, 0x1.a01a01a01a01ap-16);
+    q = fma(q, r, 0x1.a01a01a01a01ap-13);
+    q = fma(q, r, 0x1.6c16c16c16c17p-10);
+    q = fma(q, r, 0x1.1111111111111p-7);
+    q = fma(q, r, 0x1.5555555555555p-5);
+    q = fma(q, r, 0x1.5555555555555p-3);
+    q = fma(q, r, 0.5);
+    const double uu = fma(r * r, q, r);
+    /* exp(r + rl) = (1 + uu)(1 + rl) to first order in rl: the tail carries the argument's low part
+     * (up to 2^-53 |xh|), so its product with uu is kept */
+    const double p = 1.0 + (uu + fma(rl, uu, rl));
+    return ldexp(p, (int)n);
+}
+
+/* y is an integer (finite); odd: also odd */
+static int pow_int_kind(double y)
+{
+    if (!(fabs(y) <= 0x1.fffffffffffffp+1023)) return 0; /* inf/nan: not an integer */
+    if (rint(y) != y) return 0;
+    if (fabs(y) >= 0x1p53) return 1; /* even */
+    const double h = y * 0.5;
+    return rint(h) != h ? 2 : 1;      /* 2: odd */
+}
+
+/* pow with C99 Annex F special cases; |x|^y = exp(y log|x|) with both steps as unevaluated sums */
+double orc_pow(double x, double y)
+{
+    if (y == 0.0) return 1.0;
+    if (x == 1.0) return 1.0;
+    if (x != x || y != y) return x + y;
+    const int ik = pow_int_kind(y);
+    const double ax = fabs(x), ay = fabs(y);
+    if (ay > 0x1.fffffffffffffp+1023) { /* y = +-inf */
+        if (ax == 1.0) return 1.0;
+        return ((ax < 1.0) == (y < 0)) ? 1.0 / 0.0 : 0.0;
+    }
+    const int neg = x < 0 || (x == 0.0 && signbit(x));
+    const double sgn = (neg && ik == 2) ? -1.0 : 1.0;
+    if (x == 0.0) return y < 0 ? sgn / 0.0 : sgn * 0.0;
+    if (ax > 0x1.fffffffffffffp+1023) return y < 0 ? sgn * 0.0 : sgn / 0.0;
+    if (x < 0 && ik == 0) return (x - x) / (x - x);
+    double lh, ll;
+    log_dd(ax, &lh, &ll);
+    const double ph = y * lh;
+    const double pl = fma(y, lh, -ph) + y * ll;
+    if (ph > 710.0) return sgn / 0.0;
+    if (ph < -746.0) return sgn * 0.0;
+    return sgn * exp_dd(ph, pl);
+}
+
+double orc_rsqrt(double x) { return 1.0 / sqrt(x); }
+double orc_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+/* fmin / fmax as glibc's generic form: the other operand when one is NaN; on equal values (+-0) the
+ * second */
+double orc_fmin(double a, double b) { return (a < b || b != b) ? a : b; }
+double orc_fmax(double a, double b) { return (a > b || b != b) ? a : b; }
+/* the reductions' ordered max (expr/Reduce.h MaxOp): a > b ? a : b */
+double orc_selgt(double a, double b) { return a > b ? a : b; }
+
 void orc_math_f64(int fn, int mode, uint64_t n, const double* x, double* y)
 {
 #pragma omp parallel for schedule(static)
@@ -419,9 +800,41 @@ void orc_math_f64(int fn, int mode, uint64_t n, const double* x, double* y)
         if (mode == 0) {
             r = fn == 0 ? sin(v) : fn == 1 ? cos(v) : fn == 2 ? exp(v) : log(v);
         } else {
-            r = fn == 0 ? orc_sin(v) : fn == 1 ? orc_cos(v) : fn == 2 ? orc_exp(v) : orc_log(v);
+            switch (fn) {
+            case 0: r = orc_sin(v); break;
+            case 1: r = orc_cos(v); break;
+            case 2: r = orc_exp(v); break;
+            case 3: r = orc_log(v); break;
+            case 4: r = orc_tan(v); break;
+            case 5: r = orc_atan(v); break;
+            case 6: r = orc_asin(v); break;
+            case 7: r = orc_acos(v); break;
+            case 8: r = orc_tanh(v); break;
+            case 9: r = orc_rsqrt(v); break;
+            default: r = orc_sign(v); break;
+            }
         }
         y[i] = r;
+    }
+}
+
+/* binary: 0 pow, 1 fmin, 2 fmax, 3 ordered max */
+void orc_math2_f64(int fn, uint64_t n, const double* x, const double* y, double* z)
+{
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; ++i) {
+        const double a = x[i], b = y[i];
+        z[i] = fn == 0 ? orc_pow(a, b) : fn == 1 ? orc_fmin(a, b) : fn == 2 ? orc_fmax(a, b) : orc_selgt(a, b);
+    }
+}
+
+/* the float forms: the double routine on the widened argument, one rounding */
+void orc_math2_f32(int fn, uint64_t n, const float* x, const float* y, float* z)
+{
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; ++i) {
+        const double a = x[i], b = y[i];
+        z[i] = (float)(fn == 0 ? orc_pow(a, b) : fn == 1 ? orc_fmin(a, b) : fn == 2 ? orc_fmax(a, b) : orc_selgt(a, b));
     }
 }
 
@@ -434,7 +847,19 @@ void orc_math_f32(int fn, int mode, uint64_t n, const float* x, float* y)
         if (mode == 0) {
             r = fn == 0 ? sinf(v) : fn == 1 ? cosf(v) : fn == 2 ? expf(v) : logf(v);
         } else {
-            r = fn == 0 ? orc_sinf(v) : fn == 1 ? orc_cosf(v) : fn == 2 ? orc_expf(v) : orc_logf(v);
+            switch (fn) {
+            case 0: r = orc_sinf(v); break;
+            case 1: r = orc_cosf(v); break;
+            case 2: r = orc_expf(v); break;
+            case 3: r = orc_logf(v); break;
+            case 4: r = (float)orc_tan(v); break;
+            case 5: r = (float)orc_atan(v); break;
+            case 6: r = (float)orc_asin(v); break;
+            case 7: r = (float)orc_acos(v); break;
+            case 8: r = (float)orc_tanh(v); break;
+            case 9: r = 1.0f / sqrtf(v); break;
+            default: r = (float)orc_sign(v); break;
+            }
         }
         y[i] = r;
     }

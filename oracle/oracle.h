@@ -80,7 +80,22 @@ float orc_cosf(float x);
 float orc_expf(float x);
 float orc_logf(float x);
 /* vector forms for the tests: fn 0=sin 1=cos 2=exp 3=log; mode 0 = libm, 1 = own */
+double orc_tan(double x);
+double orc_atan(double x);
+double orc_asin(double x);
+double orc_acos(double x);
+double orc_tanh(double x);
+double orc_pow(double x, double y);
+double orc_rsqrt(double x);
+double orc_sign(double x);
+double orc_fmin(double a, double b);
+double orc_fmax(double a, double b);
+double orc_selgt(double a, double b);
+/* fn: 0 sin 1 cos 2 exp 3 log 4 tan 5 atan 6 asin 7 acos 8 tanh 9 rsqrt 10 sign (mode 0: libm for 0-3) */
 void orc_math_f64(int fn, int mode, uint64_t n, const double* x, double* y);
+/* fn: 0 pow 1 fmin 2 fmax 3 ordered max (a > b ? a : b) */
+void orc_math2_f64(int fn, uint64_t n, const double* x, const double* y, double* z);
+void orc_math2_f32(int fn, uint64_t n, const float* x, const float* y, float* z);
 void orc_math_f32(int fn, int mode, uint64_t n, const float* x, float* y);
 
 /* ---------------- fused SoA expressions ---------------- */

@@ -55,6 +55,8 @@ def lib():
             "orc_fill_array_normal_f64": (None, [_u64, _f64, _f64, _i32, _u64, _u64, _i32, _pd]),
             "orc_math_f64": (None, [_i32, _i32, _u64, _pd, _pd]),
             "orc_math_f32": (None, [_i32, _i32, _u64, _pf, _pf]),
+            "orc_math2_f64": (None, [_i32, _u64, _pd, _pd, _pd]),
+            "orc_math2_f32": (None, [_i32, _u64, _pf, _pf, _pf]),
             "orc_eval_cfg0_f64": (None, [_pd, _pd, _pd, _pd, _i32, _u64, _u64, _i32, _i32]),
             "orc_eval_cfg0_f32": (None, [_pf, _pf, _pf, _pf, _i32, _u64, _u64, _i32, _i32]),
             "orc_eval_vec3geo_f64": (None, [_pd, _pd, _pd, _pd, _u64, _u64, _i32]),
@@ -166,7 +168,9 @@ def fill_array_normal_f64(seed, ncomp, n, pitch, mean=0.0, sd=1.0, mathmode=1):
 
 
 # ---------------------------------------------------------------- math
-_FN = {"sin": 0, "cos": 1, "exp": 2, "log": 3}
+_FN = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "tan": 4, "atan": 5, "asin": 6, "acos": 7, "tanh": 8, "rsqrt": 9,
+       "sign": 10}
+_FN2 = {"pow": 0, "fmin": 1, "fmax": 2, "selgt": 3}
 
 
 def math64(fn, x, mode=1):
@@ -174,6 +178,23 @@ def math64(fn, x, mode=1):
     y = np.empty_like(x)
     lib().orc_math_f64(_FN[fn], mode, x.size, x, y)
     return y
+
+
+def math2_64(fn, x, y):
+    """pow / fmin / fmax / selgt (a > b ? a : b), elementwise over broadcast x, y."""
+    x, y = np.broadcast_arrays(np.asarray(x, np.float64), np.asarray(y, np.float64))
+    x, y = np.ascontiguousarray(x), np.ascontiguousarray(y)
+    z = np.empty_like(x)
+    lib().orc_math2_f64(_FN2[fn], x.size, x, y, z)
+    return z
+
+
+def math2_32(fn, x, y):
+    x, y = np.broadcast_arrays(np.asarray(x, np.float32), np.asarray(y, np.float32))
+    x, y = np.ascontiguousarray(x), np.ascontiguousarray(y)
+    z = np.empty_like(x)
+    lib().orc_math2_f32(_FN2[fn], x.size, x, y, z)
+    return z
 
 
 def math32(fn, x, mode=1):

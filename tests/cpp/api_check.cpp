@@ -25,6 +25,11 @@
 //                                eps * sum|t| per slot (text)
 //   runtime <rows> <cols>      : runtimeEval AddScaled of a transposed and a dense view, and an
 //                                Assign cast to float, checked on the host (text)
+//   ext    <n> <seed>          : q, r (quaternions) and a (3-vector) from Generator(seed) on the
+//                                device; quaternion, fold, norm, min/max/clamp and slice members
+//                                on the device, then the same per sample on the host; writes q, r,
+//                                a, the four device results, the four host results, then
+//                                tan(a) + |a|^1.5 from the device
 #include <aether/aether.hpp>
 
 #include <cmath>
@@ -78,12 +83,13 @@ void print_tape(const char* name, const aether::detail::TapeCtx& x, const void* 
 int tape()
 {
     // fake device planes: lowering reads no memory
-    static double buf[8][64];
+    static double buf[8][160];
     const void* ids[8];
     for (int k = 0; k < 8; ++k) ids[k] = buf[k];
     aether::View<double, 3> a(buf[0], 10, 32, true), b(buf[1], 10, 32, true), c(buf[2], 10, 32, true),
         d(buf[3], 10, 32, true);
     aether::View<double, 1> e(buf[4], 10, 32, true), s(buf[5], 10, 32, true);
+    aether::View<double, 4> q(buf[6], 10, 32, true), r(buf[7], 10, 32, true);
     namespace ad = aether::detail;
     print_tape("cfg0", ad::lower<double, 3>(buf[3], 32, cfg0(a, b, c)), ids, 8);
     print_tape("vec3", ad::lower<double, 1>(buf[4], 32, a.cross(b).dot(c) + aether::sqrt(a.dot(a) + 1.0)), ids, 8);
@@ -100,6 +106,27 @@ int tape()
                ids, 8);
     print_tape("norm", ad::lower<double, 1>(buf[4], 32, a.norm() - b.squaredNorm()), ids, 8);
     print_tape("inplace", ad::lower<double, 3>(buf[0], 32, a * 2.0 + 1.0), ids, 8);
+    using namespace aether;
+    print_tape("cwise",
+               ad::lower<double, 3>(buf[3], 32,
+                                    cwiseTan(a) + cwiseAtan(b) - cwiseAsin(c) * cwiseAcos(a) + cwiseTanh(b) +
+                                        cwiseSign(c) + cwiseRsqrt(a)),
+               ids, 8);
+    print_tape("minmax",
+               ad::lower<double, 3>(buf[3], 32,
+                                    clamp(a, -0.5, 0.5) + cwiseMin(a, b) * cwiseMax(b, 0.25) + cwiseDiv(a, c) +
+                                        cwisePow(cwiseAbs(a), 1.5) + cwisePow(cwiseAbs(b), c)),
+               ids, 8);
+    print_tape("norms1", ad::lower<double, 1>(buf[4], 32, a.rNorm() + a.rSquaredNorm()), ids, 8);
+    print_tape("norms2", ad::lower<double, 1>(buf[4], 32, a.cubedNorm() - a.rCubedNorm()), ids, 8);
+    print_tape("norms3", ad::lower<double, 1>(buf[4], 32, q.maxNorm() + a.sum()), ids, 8);
+    print_tape("unit", ad::lower<double, 3>(buf[3], 32, a.unitVector() + a.cwiseMul(b)), ids, 8);
+    print_tape("quat", ad::lower<double, 4>(buf[6], 32, q.quatMul(r) + q.quatConj()), ids, 8);
+    print_tape("qrecip", ad::lower<double, 4>(buf[7], 32, q.quatReciprocal()), ids, 8);
+    print_tape("qrot", ad::lower<double, 3>(buf[3], 32, q.quatRotate(a)), ids, 8);
+    print_tape("slices",
+               ad::lower<double, 3>(buf[3], 32, q.asBack3DVector() + q.head<3>() - a.asPureQuaternion().tail<3>()),
+               ids, 8);
     return 0;
 }
 
@@ -376,6 +403,78 @@ int resize(std::size_t n)
 
 } // namespace
 
+template<class Q, class V>
+auto ext1(const Q& q, const Q& r)
+{
+    return q.quatMul(r) + q.quatConj() - aether::cwiseMax(q.quatReciprocal(), -2.0);
+}
+
+template<class Q, class V>
+auto ext2(const Q& q, const V& a)
+{
+    return q.quatRotate(a) + q.template head<3>() * a.sum() - a.asPureQuaternion().template tail<3>();
+}
+
+template<class Q, class V>
+auto ext3(const Q& q, const V& a)
+{
+    return q.maxNorm() + a.sum() - a.rCubedNorm();
+}
+
+template<class Q, class V>
+auto ext4(const Q& q, const V& a)
+{
+    return a.unitVector() + aether::cwiseMin(a, 0.25) - aether::clamp(a, -0.5, 0.5) +
+           aether::cwiseDiv(a, q.asBack3DVector()) + aether::cwiseSign(a);
+}
+
+int ext(std::size_t n, std::uint64_t seed)
+{
+    Array<double, 4> q(n), r(n), o1(n), h1(n);
+    Array<double, 3> a(n), o2(n), o4(n), o5(n), h2(n), h4(n);
+    Array<double> o3(n), h3(n);
+    const aether::random::Generator g(seed);
+    q.deviceView() = g.uniform<double, 4>(-1.0, 1.0);
+    r.deviceView() = g.substream(1).uniform<double, 4>(-1.0, 1.0);
+    a.deviceView() = g.substream(2).uniform<double, 3>(-1.0, 1.0);
+    const auto qd = q.deviceView(), rd = r.deviceView();
+    const auto ad = a.deviceView();
+    o1.deviceView() = ext1<decltype(qd), decltype(ad)>(qd, rd);
+    o2.deviceView() = ext2(qd, ad);
+    o3.deviceView() = ext3(qd, ad);
+    o4.deviceView() = ext4(qd, ad);
+    o5.deviceView() = aether::cwiseTan(ad) + aether::cwisePow(aether::cwiseAbs(ad), 1.5);
+    for (auto* x : {&q, &r, &o1}) x->download();
+    for (auto* x : {&a, &o2, &o4, &o5}) x->download();
+    o3.download();
+    const auto qh = q.hostView(), rh = r.hostView();
+    const auto ah = a.hostView();
+    auto v1 = h1.hostView();
+    auto v2 = h2.hostView();
+    auto v3 = h3.hostView();
+    auto v4 = h4.hostView();
+    for (std::size_t k = 0; k < n; ++k) {
+        const auto i = SampleIndex::make(k);
+        v1[i] = ext1<decltype(qh), decltype(ah)>(qh, rh);
+        v2[i] = ext2(qh, ah);
+        v3[i] = ext3(qh, ah);
+        v4[i] = ext4(qh, ah);
+    }
+    write_planes(q);
+    write_planes(r);
+    write_planes(a);
+    write_planes(o1);
+    write_planes(o2);
+    write_planes(o3);
+    write_planes(o4);
+    write_planes(h1);
+    write_planes(h2);
+    write_planes(h3);
+    write_planes(h4);
+    write_planes(o5);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) return 2;
@@ -395,6 +494,7 @@ int main(int argc, char** argv)
         if (mode == "keyed" && argc == 4) return keyed(u(2), u(3));
         if (mode == "runtime" && argc == 4) return runtime(u(2), u(3));
         if (mode == "resize" && argc == 3) return resize(u(2));
+        if (mode == "ext" && argc == 4) return ext(u(2), u(3));
     } catch (const aether::Error& e) {
         std::fprintf(stderr, "aether error: %s\n", e.what());
         return 1;

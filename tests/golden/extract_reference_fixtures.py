@@ -2,7 +2,7 @@
 
 Reads (as text, never compiles or imports) the row tables of
   tests/random/golden_random.h          -> random_core_rows.json, random_key_rows.json
-  tests/bandmath/golden_{sin,cos,exp,log,sqrt}.h -> bandmath_<fn>.npz
+  tests/bandmath/golden_{sin,cos,exp,log,sqrt,tan,tanh,atan,asin,acos,rsqrt,pow}.h -> bandmath_<fn>.npz
 of the reference source tree and writes them next to this script. Only the
 numbers are kept (inputs and expected outputs); no source text is copied.
 Run: python tests/golden/extract_reference_fixtures.py [/path/to/reference]
@@ -68,11 +68,28 @@ def extract_bandmath(ref, fn):
     print(f"bandmath {fn}: {len(rows)} rows")
 
 
+def extract_bandmath2(ref, fn):
+    """Two-argument rows ({ {x, y}, ref, label }): x_bits, y_bits, ref_bits."""
+    path = os.path.join(ref, "tests", "bandmath", f"golden_{fn}.h")
+    text = open(path).read()
+    block = text.split("kRows[] = {", 1)[1].split("};", 1)[0]
+    rows = re.findall(r"\{\s*\{\s*0x([0-9A-Fa-f]+)ULL\s*,\s*0x([0-9A-Fa-f]+)ULL\s*\}\s*,\s*0x([0-9A-Fa-f]+)ULL"
+                      r"\s*,\s*(\d)\s*\}", block)
+    x = np.array([int(r[0], 16) for r in rows], np.uint64)
+    y = np.array([int(r[1], 16) for r in rows], np.uint64)
+    z = np.array([int(r[2], 16) for r in rows], np.uint64)
+    lab = np.array([int(r[3]) for r in rows], np.uint8)
+    np.savez_compressed(os.path.join(HERE, f"bandmath_{fn}.npz"), x_bits=x, y_bits=y, ref_bits=z, label=lab,
+                        source=np.array(f"tests/bandmath/golden_{fn}.h"))
+    print(f"bandmath {fn}: {len(rows)} rows")
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     extract_random(ref)
-    for fn in ("sin", "cos", "exp", "log", "sqrt"):
+    for fn in ("sin", "cos", "exp", "log", "sqrt", "tan", "tanh", "atan", "asin", "acos", "rsqrt"):
         extract_bandmath(ref, fn)
+    extract_bandmath2(ref, "pow")
 
 
 if __name__ == "__main__":

@@ -50,8 +50,8 @@ def render(name, lowered, ids):
 def test_cpp_lowering_equals_python_lowering():
     got = run("tape").decode().splitlines()
     arr = lambda k: ae.Array(10, k, torch.float64, device="cpu")  # noqa: E731
-    a, b, c, d, e, s = arr(3), arr(3), arr(3), arr(3), arr(1), arr(1)
-    ids = [a, b, c, d, e, s]
+    a, b, c, d, e, s, q, r = arr(3), arr(3), arr(3), arr(3), arr(1), arr(1), arr(4), arr(4)
+    ids = [a, b, c, d, e, s, q, r]
     g = ae.Generator.from_seed(7)
     want = []
     want += render("cfg0", E.lower(d, a + b * c + E.sin(a) * E.exp(-1.0 * (b * b))), ids)
@@ -62,6 +62,19 @@ def test_cpp_lowering_equals_python_lowering():
                    ids)
     want += render("norm", E.lower(e, E.Leaf(a).norm() - E.Leaf(b).squared_norm()), ids)
     want += render("inplace", E.lower(a, a * 2.0 + 1.0), ids)
+    L = E.Leaf
+    want += render("cwise", E.lower(d, E.tan(a) + E.atan(b) - E.asin(c) * E.acos(a) + E.tanh(b) + E.sign(c) +
+                                    E.rsqrt(a)), ids)
+    want += render("minmax", E.lower(d, E.clamp(a, -0.5, 0.5) + E.minimum(a, b) * E.maximum(b, 0.25) + E.div(a, c) +
+                                     E.pow_(E.abs_(a), 1.5) + E.pow_(E.abs_(b), c)), ids)
+    want += render("norms1", E.lower(e, L(a).rNorm() + L(a).rSquaredNorm()), ids)
+    want += render("norms2", E.lower(e, L(a).cubedNorm() - L(a).rCubedNorm()), ids)
+    want += render("norms3", E.lower(e, L(q).maxNorm() + L(a).sum()), ids)
+    want += render("unit", E.lower(d, L(a).unitVector() + L(a).cwiseMul(b)), ids)
+    want += render("quat", E.lower(q, L(q).quatMul(r) + L(q).quatConj()), ids)
+    want += render("qrecip", E.lower(r, L(q).quatReciprocal()), ids)
+    want += render("qrot", E.lower(d, L(q).quatRotate(a)), ids)
+    want += render("slices", E.lower(d, L(q).asBack3DVector() + L(q).head(3) - L(a).asPureQuaternion().tail(3)), ids)
     assert got == want
 
 

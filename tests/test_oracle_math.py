@@ -136,3 +136,126 @@ def test_float_specials(orc):
     l = orc.math32("log", f32([0.0, -0.0, -1.0, np.inf, np.nan, 1.0, 1e-45]))
     assert l[0] == -np.inf and l[1] == -np.inf and np.isnan(l[2]) and l[3] == np.inf and np.isnan(l[4])
     assert l[5] == 0 and l[6] == np.float32(np.log(np.float64(np.float32(1e-45))))
+
+
+# ---- the rest of the elementwise set: tan, atan, asin, acos, tanh, rsqrt, pow, sign, fmin, fmax, the ordered max
+EXT = ["tan", "tanh", "atan", "asin", "acos", "rsqrt"]
+
+
+@pytest.mark.parametrize("fn", EXT)
+def test_mpfr_golden_rows_extended(orc, fn):
+    """<= 1 ulp of the correctly rounded value on every row of the reference's MPFR corpus (in-domain,
+    degraded and rejected rows alike); non-finite references exactly."""
+    z = np.load(os.path.join(GOLDEN, f"bandmath_{fn}.npz"))
+    x = z["x_bits"].view(np.float64)
+    ref = z["ref_bits"].view(np.float64)
+    y = orc.math64(fn, x)
+    fin = np.isfinite(ref)
+    assert np.max(ulp_err(y[fin], ref[fin])) <= ULP_BOUND_F64
+    nf = ~fin
+    assert np.all((np.isnan(ref[nf]) & np.isnan(y[nf])) | (ref[nf] == y[nf]))
+
+
+def test_pow_mpfr_golden_rows(orc):
+    z = np.load(os.path.join(GOLDEN, "bandmath_pow.npz"))
+    x, yy, ref = z["x_bits"].view(np.float64), z["y_bits"].view(np.float64), z["ref_bits"].view(np.float64)
+    got = orc.math2_64("pow", x, yy)
+    fin = np.isfinite(ref)
+    assert np.max(ulp_err(got[fin], ref[fin])) <= ULP_BOUND_F64
+    nf = ~fin
+    assert np.all((np.isnan(ref[nf]) & np.isnan(got[nf])) | (ref[nf] == got[nf]))
+
+
+def _cr(f, *args):
+    mp.mp.prec = 200
+    return float(f(*[mp.mpf(float(a)) for a in args]))
+
+
+@pytest.mark.parametrize("fn,mf,xs", [
+    ("tan", mp.tan, lambda r: np.concatenate([r.uniform(-2, 2, 300), 10 ** r.uniform(-9, 7, 300) * r.choice([-1, 1], 300)])),
+    ("atan", mp.atan, lambda r: np.concatenate([r.uniform(-2, 2, 300), 10 ** r.uniform(-12, 12, 300) * r.choice([-1, 1], 300)])),
+    ("asin", mp.asin, lambda r: np.concatenate([r.uniform(-1, 1, 300), 1 - 10 ** r.uniform(-16, -1, 150),
+                                                 -1 + 10 ** r.uniform(-16, -1, 150)])),
+    ("acos", mp.acos, lambda r: np.concatenate([r.uniform(-1, 1, 300), 1 - 10 ** r.uniform(-16, -1, 150),
+                                                 -1 + 10 ** r.uniform(-16, -1, 150)])),
+    ("tanh", mp.tanh, lambda r: np.concatenate([r.uniform(-3, 3, 300), 10 ** r.uniform(-12, 1.4, 300) * r.choice([-1, 1], 300)])),
+])
+def test_extended_random_vs_mpmath(orc, fn, mf, xs):
+    """Seeded random inputs over each function's whole range: within 1 ulp of the correctly rounded value
+    (mpmath at 200 bits)."""
+    x = xs(np.random.default_rng(11))
+    y = orc.math64(fn, x)
+    ref = np.array([_cr(mf, v) for v in x])
+    assert np.max(ulp_err(y, ref)) <= 1.0
+
+
+def test_pow_random_vs_mpmath(orc):
+    """Bases over the whole exponent range, moderate and huge exponents, bases next to 1 with exponents up
+    to 1e8 (where y log x must carry ~2^-63 relative accuracy): within 1 ulp wherever the result is a
+    normal double."""
+    r = np.random.default_rng(12)
+    x = np.concatenate([10 ** r.uniform(-300, 300, 200), r.uniform(0, 4, 200), 1 + r.uniform(-1e-6, 1e-6, 200)])
+    y = np.concatenate([r.uniform(-1, 1, 200), r.uniform(-200, 200, 200), r.uniform(-1e8, 1e8, 200)])
+    got = orc.math2_64("pow", x, y)
+    mp.mp.prec = 200
+    for a, b, g in zip(x, y, got):
+        ref = mp.power(mp.mpf(float(a)), mp.mpf(float(b)))
+        if not (mp.mpf(2) ** -1022 < abs(ref) < mp.mpf(2) ** 1024):
+            continue
+        assert ulp_err(np.array([g]), np.array([float(ref)]))[0] <= 1.0, (a, b)
+
+
+def test_pow_special_cases_follow_c99(orc):
+    """C99 Annex F.9.4.4 (what the reference's std::pow and the device pow give) against numpy's power on
+    every pairing of special bases and exponents."""
+    vals = np.array([0.0, -0.0, 1.0, -1.0, 0.5, -0.5, 2.0, -2.0, 3.0, -3.0, np.inf, -np.inf, np.nan, 1e-310, -1e-310,
+                     2.5, -2.5, 1e308, -1e308, 2.0 ** 53, -(2.0 ** 53) - 2, 2.0 ** 53 + 2])
+    x, y = np.meshgrid(vals, vals)
+    x, y = x.ravel(), y.ravel()
+    got = orc.math2_64("pow", x, y)
+    with np.errstate(all="ignore"):
+        want = np.power(x, y)
+    special = np.isin(x, [0.0, 1.0, -1.0, np.inf, -np.inf]) | np.isin(y, [0.0, 1.0, -1.0, np.inf, -np.inf])
+    special |= np.isnan(x) | np.isnan(y) | ~np.isfinite(want) | (want == 0)
+    same = (np.isnan(got) & np.isnan(want)) | ((got == want) & (np.signbit(got) == np.signbit(want)))
+    assert np.all(same[special]), list(zip(x[~same], y[~same], got[~same], want[~same]))[:8]
+    # the rest (finite, non-trivial) are ordinary values: within 1 ulp, with the sign numpy gives
+    rest = ~special
+    assert np.all(np.signbit(got[rest]) == np.signbit(want[rest]))
+    assert np.max(ulp_err(got[rest], want[rest])) <= 1.0
+
+
+def test_extended_specials(orc):
+    inf, nan = np.inf, np.nan
+    assert np.array_equal(orc.math64("tan", np.array([0.0, -0.0])), [0.0, -0.0])
+    assert np.signbit(orc.math64("tan", np.array([-0.0]))[0])
+    assert np.all(np.isnan(orc.math64("tan", np.array([inf, -inf, nan]))))
+    at = orc.math64("atan", np.array([inf, -inf, 0.0, -0.0, 1.0, nan, 1e-300]))
+    assert at[0] == np.pi / 2 and at[1] == -np.pi / 2 and at[2] == 0 and np.signbit(at[3])
+    assert at[4] == np.pi / 4 and np.isnan(at[5]) and at[6] == 1e-300
+    for fn in ("asin", "acos"):
+        assert np.all(np.isnan(orc.math64(fn, np.array([1.0000000000000002, -1.5, inf, nan]))))
+    assert np.array_equal(orc.math64("asin", np.array([1.0, -1.0, 0.0])), [np.pi / 2, -np.pi / 2, 0.0])
+    assert np.array_equal(orc.math64("acos", np.array([1.0, -1.0, 0.0])), [0.0, np.pi, np.pi / 2])
+    th = orc.math64("tanh", np.array([inf, -inf, 30.0, -0.0, nan]))
+    assert th[0] == 1 and th[1] == -1 and th[2] == 1 and np.signbit(th[3]) and np.isnan(th[4])
+    rs = orc.math64("rsqrt", np.array([0.0, inf, -1.0, 4.0]))
+    assert rs[0] == inf and rs[1] == 0 and np.isnan(rs[2]) and rs[3] == 0.5
+    sg = orc.math64("sign", np.array([3.0, -2.0, 0.0, -0.0, nan, -inf]))
+    assert np.array_equal(sg, [1.0, -1.0, 0.0, 0.0, 0.0, -1.0]) and not np.signbit(sg[3])
+    a = np.array([1.0, nan, 2.0, -0.0, 0.0, nan])
+    b = np.array([2.0, 1.0, nan, 0.0, -0.0, nan])
+    mn, mx, sg2 = orc.math2_64("fmin", a, b), orc.math2_64("fmax", a, b), orc.math2_64("selgt", a, b)
+    assert mn[0] == 1 and mn[1] == 1 and mn[2] == 2 and not np.signbit(mn[3]) and np.signbit(mn[4]) and np.isnan(mn[5])
+    assert mx[0] == 2 and mx[1] == 1 and mx[2] == 2 and not np.signbit(mx[3]) and np.signbit(mx[4]) and np.isnan(mx[5])
+    assert sg2[0] == 2 and sg2[1] == 1 and np.isnan(sg2[2]) and not np.signbit(sg2[3]) and np.signbit(sg2[4])
+
+
+def test_float_forms_round_the_double_routine_once(orc):
+    x = np.random.default_rng(13).uniform(-0.99, 0.99, 2000).astype(np.float32)
+    for fn in EXT[:-1]:
+        got = orc.math32(fn, x)
+        assert np.array_equal(got, orc.math64(fn, x.astype(np.float64)).astype(np.float32)), fn
+    y = np.random.default_rng(14).uniform(0.1, 3, 2000).astype(np.float32)
+    assert np.array_equal(orc.math2_32("pow", np.abs(x) + 0.5, y),
+                          orc.math2_64("pow", (np.abs(x) + 0.5).astype(np.float64), y.astype(np.float64)).astype(np.float32))
