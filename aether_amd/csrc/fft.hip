@@ -12,9 +12,11 @@
 // LDS rows are padded by one element every 16 so the stride-4 writes of the
 // first passes do not collide in the banks.
 //
-// Precisions: complex128; complex64; complex64 with float-float (Ff2-style,
-// banded/Ff2.h:132-145,246-321) accumulation, stored as complex64. Twiddles
-// come from one FP64 table exp(-2 pi i m / 8192) generated on the device.
+// Precisions: complex128; complex64; complex64 in memory with FP64 arithmetic
+// inside (FP64 is full rate on this GPU, so the wide mode costs the double
+// transform's arithmetic and the float transform's memory traffic; only the
+// final rounding to float remains). Twiddles come from one FP64 table
+// exp(-2 pi i m / 8192) generated on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +24,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/aether_hip.h"
@@ -47,29 +50,17 @@ struct C128 {
 struct C64 {
     float re, im;
 };
-struct CFF {
-    float rh, rl, ih, il;
+// complex64 in memory, FP64 arithmetic (the same arithmetic as C128, so equal inputs give equal bits)
+struct CW {
+    double re, im;
 };
 
 __device__ __forceinline__ C128 add(C128 a, C128 b) { return {a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ C128 sub(C128 a, C128 b) { return {a.re - b.re, a.im - b.im}; }
 __device__ __forceinline__ C64 add(C64 a, C64 b) { return {a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ C64 sub(C64 a, C64 b) { return {a.re - b.re, a.im - b.im}; }
-
-__device__ __forceinline__ acc::Pair<float> ffadd(float ah, float al, float bh, float bl)
-{
-    return acc::ff_add(acc::Pair<float>{ah, al}, acc::Pair<float>{bh, bl});
-}
-__device__ __forceinline__ CFF add(CFF a, CFF b)
-{
-    const auto r = ffadd(a.rh, a.rl, b.rh, b.rl), i = ffadd(a.ih, a.il, b.ih, b.il);
-    return {r.h, r.c, i.h, i.c};
-}
-__device__ __forceinline__ CFF sub(CFF a, CFF b)
-{
-    const auto r = ffadd(a.rh, a.rl, -b.rh, -b.rl), i = ffadd(a.ih, a.il, -b.ih, -b.il);
-    return {r.h, r.c, i.h, i.c};
-}
+__device__ __forceinline__ CW add(CW a, CW b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ CW sub(CW a, CW b) { return {a.re - b.re, a.im - b.im}; }
 
 // -i * x (forward) or +i * x (inverse): exact
 template<class C>
@@ -79,27 +70,12 @@ __device__ __forceinline__ C128 rot(C128 x, bool fwd) { return fwd ? C128{x.im, 
 template<>
 __device__ __forceinline__ C64 rot(C64 x, bool fwd) { return fwd ? C64{x.im, -x.re} : C64{-x.im, x.re}; }
 template<>
-__device__ __forceinline__ CFF rot(CFF x, bool fwd)
-{
-    return fwd ? CFF{x.ih, x.il, -x.rh, -x.rl} : CFF{-x.ih, -x.il, x.rh, x.rl};
-}
-
-// float-float product of (ah + al)(bh + bl), 2Prod + cross terms, fast2Sum renormalised
-__device__ __forceinline__ acc::Pair<float> ffmul(float ah, float al, float bh, float bl)
-{
-    const float p = ah * bh;
-    float e = __builtin_fmaf(ah, bh, -p);
-    e = __builtin_fmaf(ah, bl, e);
-    e = __builtin_fmaf(al, bh, e);
-    const float hi = p + e;
-    return acc::Pair<float>{hi, e - (hi - p)};
-}
+__device__ __forceinline__ CW rot(CW x, bool fwd) { return fwd ? CW{x.im, -x.re} : CW{-x.im, x.re}; }
 
 // twiddle tables: W[m] = exp(-2 pi i m / 8192)
 struct TwTables {
     const double2* d = nullptr;
     const float2* f = nullptr;
-    const float4* ff = nullptr; // (re_hi, re_lo, im_hi, im_lo)
 };
 
 __device__ __forceinline__ C128 twiddle(const TwTables& t, uint32_t m, bool fwd, C128*)
@@ -112,10 +88,10 @@ __device__ __forceinline__ C64 twiddle(const TwTables& t, uint32_t m, bool fwd, 
     const float2 w = t.f[m];
     return {w.x, fwd ? w.y : -w.y};
 }
-__device__ __forceinline__ CFF twiddle(const TwTables& t, uint32_t m, bool fwd, CFF*)
+__device__ __forceinline__ CW twiddle(const TwTables& t, uint32_t m, bool fwd, CW*)
 {
-    const float4 w = t.ff[m];
-    return fwd ? CFF{w.x, w.y, w.z, w.w} : CFF{w.x, w.y, -w.z, -w.w};
+    const double2 w = t.d[m];
+    return {w.x, fwd ? w.y : -w.y};
 }
 
 // complex products with one rounding less per part (fused multiply-add)
@@ -127,15 +103,12 @@ __device__ __forceinline__ C64 cmul(C64 a, C64 w)
 {
     return {__builtin_fmaf(a.re, w.re, -(a.im * w.im)), __builtin_fmaf(a.re, w.im, a.im * w.re)};
 }
-__device__ __forceinline__ CFF cmul(CFF a, CFF w)
+__device__ __forceinline__ CW cmul(CW a, CW w)
 {
-    const auto rr = ffmul(a.rh, a.rl, w.rh, w.rl), ii = ffmul(a.ih, a.il, w.ih, w.il);
-    const auto ri = ffmul(a.rh, a.rl, w.ih, w.il), ir = ffmul(a.ih, a.il, w.rh, w.rl);
-    const auto re = ffadd(rr.h, rr.c, -ii.h, -ii.c), im = ffadd(ri.h, ri.c, ir.h, ir.c);
-    return {re.h, re.c, im.h, im.c};
+    return {__builtin_fma(a.re, w.re, -(a.im * w.im)), __builtin_fma(a.re, w.im, a.im * w.re)};
 }
 
-// global I/O: complex128 as double2, complex64 (both float modes) as float2
+// global I/O: complex128 as double2, complex64 (plain and wide) as float2
 __device__ __forceinline__ C128 gload(const void* p, uint64_t i, C128*)
 {
     const double2 v = static_cast<const double2*>(p)[i];
@@ -146,10 +119,10 @@ __device__ __forceinline__ C64 gload(const void* p, uint64_t i, C64*)
     const float2 v = static_cast<const float2*>(p)[i];
     return {v.x, v.y};
 }
-__device__ __forceinline__ CFF gload(const void* p, uint64_t i, CFF*)
+__device__ __forceinline__ CW gload(const void* p, uint64_t i, CW*)
 {
     const float2 v = static_cast<const float2*>(p)[i];
-    return {v.x, 0.0f, v.y, 0.0f};
+    return {(double)v.x, (double)v.y};
 }
 __device__ __forceinline__ void gstore(void* p, uint64_t i, C128 v, double s)
 {
@@ -160,12 +133,10 @@ __device__ __forceinline__ void gstore(void* p, uint64_t i, C64 v, double s)
     const float fs = (float)s;
     static_cast<float2*>(p)[i] = make_float2(v.re * fs, v.im * fs);
 }
-__device__ __forceinline__ void gstore(void* p, uint64_t i, CFF v, double s)
+__device__ __forceinline__ void gstore(void* p, uint64_t i, CW v, double s)
 {
-    // scale in float-float (s as a float-float pair), then round once to float
-    const float sh = (float)s, sl = (float)(s - (double)sh);
-    const auto r = ffmul(v.rh, v.rl, sh, sl), im = ffmul(v.ih, v.il, sh, sl);
-    static_cast<float2*>(p)[i] = make_float2(r.h + r.c, im.h + im.c);
+    // scale in FP64, then round once to float
+    static_cast<float2*>(p)[i] = make_float2((float)(v.re * s), (float)(v.im * s));
 }
 
 // One Stockham pass of radix R over all signals of the chunk (P points).
@@ -283,14 +254,17 @@ template<class C>
 struct Words;
 template<>
 struct Words<C128> {
-    static constexpr int n = 2;
-    using W = double;
-    __device__ static W get(const C128& v, int k) { return k ? v.im : v.re; }
-    __device__ static void put(C128& v, int k, W w)
-    {
-        if (k) v.im = w;
-        else v.re = w;
-    }
+    static constexpr int n = 1;
+    using W = double2;
+    __device__ static W get(const C128& v, int) { return make_double2(v.re, v.im); }
+    __device__ static void put(C128& v, int, W w) { v = C128{w.x, w.y}; }
+};
+template<>
+struct Words<CW> {
+    static constexpr int n = 1;
+    using W = double2;
+    __device__ static W get(const CW& v, int) { return make_double2(v.re, v.im); }
+    __device__ static void put(CW& v, int, W w) { v = CW{w.x, w.y}; }
 };
 template<>
 struct Words<C64> {
@@ -407,26 +381,42 @@ struct TwGlobal {
     }
 };
 
-// x[r] *= W^(r m) for r = 1..R-1, from W^m, W^2m, W^4m, W^8m; each power is formed where it is used
+// The base twiddles of a radix-R butterfly: W^m, W^2m, W^4m, W^8m (as many as R needs)
+template<int R, class C>
+struct TwBase {
+    static constexpr int n = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+    C w[n];
+};
+
 template<int R, class C, class TW>
-__device__ __forceinline__ void twiddle_run(C* x, uint32_t m, const TW& q, bool fwd)
+__device__ __forceinline__ TwBase<R, C> twiddle_fetch(uint32_t m, const TW& q, bool fwd)
 {
-    const C w1 = q.template at<C>(m, fwd);
+    TwBase<R, C> t;
+#pragma unroll
+    for (int e = 0; e < TwBase<R, C>::n; ++e) t.w[e] = q.template at<C>(m << e, fwd);
+    return t;
+}
+
+// x[r] *= W^(r m) for r = 1..R-1, from the base twiddles; each power is formed where it is used
+template<int R, class C>
+__device__ __forceinline__ void twiddle_apply(C* x, const TwBase<R, C>& t)
+{
+    const C w1 = t.w[0];
     x[1] = cmul(x[1], w1);
     if constexpr (R > 2) {
-        const C w2 = q.template at<C>(2 * m, fwd);
+        const C w2 = t.w[1];
         x[2] = cmul(x[2], w2);
         const C w3 = cmul(w1, w2);
         x[3] = cmul(x[3], w3);
         if constexpr (R > 4) {
-            const C w4 = q.template at<C>(4 * m, fwd);
+            const C w4 = t.w[2];
             x[4] = cmul(x[4], w4);
             const C w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
             x[5] = cmul(x[5], w5);
             x[6] = cmul(x[6], w6);
             x[7] = cmul(x[7], w7);
             if constexpr (R > 8) {
-                const C w8 = q.template at<C>(8 * m, fwd);
+                const C w8 = t.w[3];
                 x[8] = cmul(x[8], w8);
                 x[9] = cmul(x[9], cmul(w1, w8));
                 x[10] = cmul(x[10], cmul(w2, w8));
@@ -438,6 +428,12 @@ __device__ __forceinline__ void twiddle_run(C* x, uint32_t m, const TW& q, bool 
             }
         }
     }
+}
+
+template<int R, class C, class TW>
+__device__ __forceinline__ void twiddle_run(C* x, uint32_t m, const TW& q, bool fwd)
+{
+    twiddle_apply<R>(x, twiddle_fetch<R, C>(m, q, fwd));
 }
 
 // One pass of radix R (stride Ns = product of the earlier radices) over the thread's 16 / R butterflies
@@ -457,6 +453,30 @@ __device__ __forceinline__ void rpass(C (&v)[kRP], uint32_t l2nr, uint32_t nb, u
             }
             dft<R>(&v[u * R], fwd);
         }
+    }
+}
+
+// The twiddles of this thread's radix-16 butterfly (one a thread) in the pass of stride Ns, fetched ahead of
+// the exchange before that pass so the loads' latency passes while the exchange runs.
+template<class C, class TW>
+__device__ __forceinline__ TwBase<16, C> rfetch16(uint32_t l2nr, uint32_t nb, uint32_t Ns, const TW& q, bool fwd)
+{
+    static_assert(kRP == 16, "one radix-16 butterfly a thread");
+    TwBase<16, C> t{};
+    const uint32_t b = threadIdx.x;
+    if (b < nb) {
+        const uint32_t k = (b & ((1u << l2nr) - 1u)) & (Ns - 1u);
+        t = twiddle_fetch<16, C>(k * (kMaxN / (16 * Ns)), q, fwd);
+    }
+    return t;
+}
+
+template<class C>
+__device__ __forceinline__ void rpass16(C (&v)[kRP], uint32_t nb, const TwBase<16, C>& t, bool fwd)
+{
+    if (threadIdx.x < nb) {
+        twiddle_apply<16>(v, t);
+        dft<16>(v, fwd);
     }
 }
 
@@ -543,14 +563,58 @@ __device__ __forceinline__ void rsingle(C (&v)[kRP], const Sig* sigs, const void
     rstore<R>(v, sigs, out, 0, P / R, scale);
 }
 
-// the tail pass of radix R after the passes of 16, then its outputs
+// The tail pass of radix R (stride Ns = N / R) after the passes of 16. Butterfly u of a thread sits at
+// j = (threadIdx.x + u kRT) mod Ns. When Ns <= kRT every butterfly of the thread has the same j, so one set of
+// base twiddles serves all of them. When R Ns = kMaxN (one whole 8192-point signal) j = threadIdx.x + u kRT
+// and W^(2^e j) = W^(2^e threadIdx.x) W16^(2^e u), a product by a constant. The base set (of j =
+// threadIdx.x mod Ns) is fetched by the caller, ahead of the exchange before this pass.
 template<int R, class C, class TW>
+__device__ __forceinline__ TwBase<R, C> rfetch_tail(uint32_t Ns, const TW& q, bool fwd)
+{
+    return twiddle_fetch<R, C>((threadIdx.x & (Ns - 1u)) * (kMaxN / (R * Ns)), q, fwd);
+}
+
+template<int R, class C, int u>
+__device__ __forceinline__ TwBase<R, C> tw_shift(const TwBase<R, C>& t0, bool fwd)
+{
+    TwBase<R, C> t;
+    t.w[0] = w16<u>(t0.w[0], fwd);
+    if constexpr (TwBase<R, C>::n > 1) t.w[1] = w16<2 * u>(t0.w[1], fwd);
+    if constexpr (TwBase<R, C>::n > 2) t.w[2] = w16<4 * u>(t0.w[2], fwd);
+    if constexpr (TwBase<R, C>::n > 3) t.w[3] = w16<8 * u>(t0.w[3], fwd);
+    return t;
+}
+
+template<int R, class C, int u = 0>
+__device__ __forceinline__ void rtail_whole(C (&v)[kRP], const TwBase<R, C>& t0, bool fwd)
+{
+    if constexpr (u < kRP / R) {
+        twiddle_apply<R>(&v[u * R], tw_shift<R, C, u>(t0, fwd));
+        dft<R>(&v[u * R], fwd);
+        rtail_whole<R, C, u + 1>(v, t0, fwd);
+    }
+}
+
+template<int R, class C>
 __device__ __forceinline__ void rtail(C (&v)[kRP], const Sig* sigs, void* out, uint32_t log2n, uint32_t P,
-                                      uint32_t Ns, double scale, const TW& q, bool fwd)
+                                      uint32_t Ns, double scale, const TwBase<R, C>& t0, bool fwd)
 {
     constexpr uint32_t lr = R == 2 ? 1 : R == 4 ? 2 : 3;
-    rpass<R>(v, log2n - lr, P / R, Ns, q, fwd);
-    rstore<R>(v, sigs, out, log2n - lr, P / R, scale);
+    constexpr int U = kRP / R;
+    const uint32_t nb = P / R;
+    if (Ns <= (uint32_t)kRT) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (threadIdx.x + u * kRT < nb) {
+                twiddle_apply<R>(&v[u * R], t0);
+                dft<R>(&v[u * R], fwd);
+            }
+        }
+    } else {
+        // Ns > kRT happens only for log2n = 13 (R = 2, Ns = 4096): R Ns = kMaxN and nb = U kRT
+        rtail_whole<R>(v, t0, fwd);
+    }
+    rstore<R>(v, sigs, out, log2n - lr, nb, scale);
 }
 
 // Transform one chunk: read, passes, write.
@@ -576,21 +640,33 @@ __device__ __forceinline__ void rchunk(C (&v)[kRP], const Desc d, const Sig* s, 
     }
     rload<16>(v, s, in, L - 4, P / 16);
     uint32_t Ns = 1;
+    TwBase<16, C> next{};
+    // complex128 fetches each radix-16 pass's twiddles ahead of the exchange before it; complex64 keeps its
+    // registers for two workgroups a CU and fetches them in the pass
+    constexpr bool ahead = sizeof(C) == 16;
     for (uint32_t p = 0; p < nq; ++p) {
-        rpass<16>(v, L - 4, P / 16, Ns, q, fwd);
-        if (p + 1 == nq && rem == 0) {
-            rstore<16>(v, s, out, L - 4, P / 16, scale);
-            return;
-        }
-        if (p + 1 < nq) rexchange<16, 16>(v, xch, L, P, Ns);
-        else if (rem == 1) rexchange<16, 2>(v, xch, L, P, Ns);
-        else if (rem == 2) rexchange<16, 4>(v, xch, L, P, Ns);
-        else rexchange<16, 8>(v, xch, L, P, Ns);
+        if (p == 0 || !ahead) rpass<16>(v, L - 4, P / 16, Ns, q, fwd);
+        else rpass16(v, P / 16, next, fwd);
+        if (p + 1 == nq) break;
+        if constexpr (ahead) next = rfetch16<C>(L - 4, P / 16, Ns * 16, q, fwd);
+        rexchange<16, 16>(v, xch, L, P, Ns);
         Ns *= 16;
     }
-    if (rem == 1) rtail<2>(v, s, out, L, P, Ns, scale, q, fwd);
-    else if (rem == 2) rtail<4>(v, s, out, L, P, Ns, scale, q, fwd);
-    else rtail<8>(v, s, out, L, P, Ns, scale, q, fwd);
+    if (rem == 0) {
+        rstore<16>(v, s, out, L - 4, P / 16, scale);
+    } else if (rem == 1) {
+        const TwBase<2, C> t0 = rfetch_tail<2, C>(Ns * 16, q, fwd);
+        rexchange<16, 2>(v, xch, L, P, Ns);
+        rtail<2>(v, s, out, L, P, Ns * 16, scale, t0, fwd);
+    } else if (rem == 2) {
+        const TwBase<4, C> t0 = rfetch_tail<4, C>(Ns * 16, q, fwd);
+        rexchange<16, 4>(v, xch, L, P, Ns);
+        rtail<4>(v, s, out, L, P, Ns * 16, scale, t0, fwd);
+    } else {
+        const TwBase<8, C> t0 = rfetch_tail<8, C>(Ns * 16, q, fwd);
+        rexchange<16, 8>(v, xch, L, P, Ns);
+        rtail<8>(v, s, out, L, P, Ns * 16, scale, t0, fwd);
+    }
 }
 
 // complex64 fits two workgroups a CU (4 waves a SIMD, 128 registers); complex128's passes need more
@@ -606,7 +682,7 @@ __global__ __launch_bounds__(kRT, sizeof(C) == 8 ? 4 : 2) void fft_reg_kernel(co
     rchunk(v, d, sigs + d.first_sig, in, out, xch, TwGlobal{tw}, fwd_flag != 0, norm, fixed_scale);
 }
 
-__global__ void twiddle_kernel(double2* d, float2* f, float4* ff)
+__global__ void twiddle_kernel(double2* d, float2* f)
 {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= (uint32_t)kMaxN) return;
@@ -621,15 +697,12 @@ __global__ void twiddle_kernel(double2* d, float2* f, float4* ff)
     }
     d[m] = make_double2(c, s);
     f[m] = make_float2((float)c, (float)s);
-    const float ch = (float)c, sh = (float)s;
-    ff[m] = make_float4(ch, (float)(c - (double)ch), sh, (float)(s - (double)sh));
 }
 
 // Per-device twiddle tables, created once.
 struct DeviceTables {
     double2* d = nullptr;
     float2* f = nullptr;
-    float4* ff = nullptr;
 };
 std::mutex g_tw_mutex;
 std::map<int, DeviceTables> g_tw;
@@ -644,24 +717,26 @@ TwTables tables(hipStream_t st)
         DeviceTables t;
         check(hipMalloc(&t.d, sizeof(double2) * kMaxN), "hipMalloc(twiddles)", sizeof(double2) * kMaxN);
         check(hipMalloc(&t.f, sizeof(float2) * kMaxN), "hipMalloc(twiddles)", sizeof(float2) * kMaxN);
-        check(hipMalloc(&t.ff, sizeof(float4) * kMaxN), "hipMalloc(twiddles)", sizeof(float4) * kMaxN);
-        hipLaunchKernelGGL(twiddle_kernel, dim3(kMaxN / 256), dim3(256), 0, st, t.d, t.f, t.ff);
+        hipLaunchKernelGGL(twiddle_kernel, dim3(kMaxN / 256), dim3(256), 0, st, t.d, t.f);
         check(hipGetLastError(), "twiddle_kernel launch");
         check(hipStreamSynchronize(st), "hipStreamSynchronize(twiddles)");
-        for (auto fn : {(const void*)fft_kernel<C128>, (const void*)fft_kernel<C64>, (const void*)fft_kernel<CFF>}) {
+        for (auto fn : {(const void*)fft_kernel<C128>, (const void*)fft_kernel<C64>, (const void*)fft_kernel<CW>}) {
             check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsElems * 16),
                   "hipFuncSetAttribute(LDS)");
         }
         check(hipFuncSetAttribute((const void*)fft_reg_kernel<C128>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kLdsElems * 8),
+                                  kLdsElems * sizeof(Words<C128>::W)),
               "hipFuncSetAttribute(LDS)");
         check(hipFuncSetAttribute((const void*)fft_reg_kernel<C64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kLdsElems * 8),
               "hipFuncSetAttribute(LDS)");
+        check(hipFuncSetAttribute((const void*)fft_reg_kernel<CW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kLdsElems * sizeof(Words<CW>::W)),
+              "hipFuncSetAttribute(LDS)");
 
         it = g_tw.emplace(dev, t).first;
     }
-    return TwTables{it->second.d, it->second.f, it->second.ff};
+    return TwTables{it->second.d, it->second.f};
 }
 
 
@@ -708,9 +783,12 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
     const TwTables tw = tables(st);
     const size_t lds = lds_bytes(p.prec);
     if (p.ndesc == 0) return;
-    if (p.stride == 1 && p.prec != 2) {
-        if (p.prec == 0)
-            hipLaunchKernelGGL(fft_reg_kernel<C128>, dim3(p.ndesc), dim3(kRT), kLdsElems * 8, st, p.descs, p.sigs, in,
+    if (p.stride == 1) {
+        if (p.prec == 2)
+            hipLaunchKernelGGL(fft_reg_kernel<CW>, dim3(p.ndesc), dim3(kRT), kLdsElems * sizeof(Words<CW>::W), st,
+                               p.descs, p.sigs, in, out, tw, forward, norm, fixed_scale);
+        else if (p.prec == 0)
+            hipLaunchKernelGGL(fft_reg_kernel<C128>, dim3(p.ndesc), dim3(kRT), kLdsElems * sizeof(Words<C128>::W), st, p.descs, p.sigs, in,
                                out, tw, forward, norm, fixed_scale);
         else
             hipLaunchKernelGGL(fft_reg_kernel<C64>, dim3(p.ndesc), dim3(kRT), kLdsElems * 8, st, p.descs, p.sigs, in,
@@ -728,7 +806,7 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
                            forward, norm, fixed_scale);
         break;
     default:
-        hipLaunchKernelGGL(fft_kernel<CFF>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
+        hipLaunchKernelGGL(fft_kernel<CW>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
                            forward, norm, fixed_scale);
         break;
     }

@@ -2,9 +2,10 @@
 
 Bound (the reference's tests/test_Fft_common.h:79-83):
     max |out - DFT(x) * scale| <= (8 + 4 log2 N) * eps * ||x||_2 * scale
-with eps of the storage type. The float-float complex64 mode must land far
-inside the complex64 bound (stated: within 4 float ulps of |X|max, i.e. the
-output rounding), and complex64 plain inside its own bound.
+with eps of the storage type. The wide complex64 mode (complex64 in memory,
+FP64 arithmetic inside) must land far inside the complex64 bound (stated:
+within 4 float ulps of |X|max, i.e. the output rounding), and complex64 plain
+inside its own bound.
 """
 import numpy as np
 import pytest
@@ -91,7 +92,7 @@ def test_round_trip_c128(dev):
 
 
 @pytest.mark.parametrize("log2n", [6, 10, 13])
-def test_complex64_plain_and_floatfloat(dev, orc, log2n):
+def test_complex64_plain_and_wide(dev, orc, log2n):
     n = 1 << log2n
     x = signals([n, n], 11).astype(np.complex64)
     ref0 = orc.dft(x[:n].astype(np.complex128))
@@ -100,7 +101,7 @@ def test_complex64_plain_and_floatfloat(dev, orc, log2n):
     err_plain = np.max(np.abs(y_plain[:n] - ref0))
     err_ff = np.max(np.abs(y_ff[:n] - ref0))
     assert err_plain <= bound(n, x[:n], 2.0**-23)
-    # float-float: only the final rounding to complex64 remains
+    # wide: only the final rounding to complex64 remains
     assert err_ff <= 4 * 2.0**-24 * np.max(np.abs(ref0))
     assert err_ff < err_plain
 
