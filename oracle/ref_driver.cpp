@@ -17,6 +17,9 @@
 //   fft    <n> <seed> <fwd>             : pocketfft c2c of a length-n signal
 //                                         (entries u01-derived), in then out
 //   bench  <n> <reps>                   : times the OpenMP fused loop (text)
+//   members <n> <sQ> <sR> <sV>          : q, r (4 planes), v (3 planes) of raw-seed uniforms, then the
+//                                         per-sample members and elementwise functions listed in
+//                                         members() below, each as its planes, in that order
 #include <aether/aether.h>
 #include <aether/fft/Fft.h>
 
@@ -45,11 +48,18 @@ auto rawUniform(std::uint64_t seed)
     return Leaf(seed ^ Leaf::salt, T(-1), T(1));
 }
 
-template<class T>
-void writePlanes(const aether::Array<T, 3>& a, std::size_t n)
+template<class T, std::size_t N>
+auto rawUniformN(std::uint64_t seed)
+{
+    using Leaf = rnd::UniformLeaf<T, N, T, T>;
+    return Leaf(seed ^ Leaf::salt, T(-1), T(1));
+}
+
+template<class T, std::size_t N = 3>
+void writePlanes(const aether::Array<T, N>& a, std::size_t n)
 {
     auto v = a.hostView();
-    for (std::size_t c = 0; c < 3; ++c)
+    for (std::size_t c = 0; c < N; ++c)
         for (std::size_t i = 0; i < n; ++i) {
             const T x = v(c, i);
             std::fwrite(&x, sizeof(T), 1, stdout);
@@ -121,6 +131,93 @@ void vec3(std::size_t n, std::uint64_t sa, std::uint64_t sb, std::uint64_t sc)
         ev(k) = r;
         std::fwrite(&r, sizeof(double), 1, stdout);
     }
+}
+
+// The reference's own members and elementwise functions, per sample through its host spelling:
+// 4-planes: quatMul(q, r), quatConj(q), quatReciprocal(q), asPureQuaternion(v);
+// 3-planes: quatRotate(q, v), unitVector(v), asBack3DVector(q), head<3>(q), cwiseMin(v, 0.25),
+//           cwiseMax(v, -0.25), clamp(v, -0.5, 0.5), cwiseSign(v), cwiseDiv(v, back3(q)), cwiseMul(v, head3(q)),
+//           cwiseMin(v, back3(q)), cwiseMax(v, back3(q)), cwiseTan(v), cwiseAtan(v), cwiseAsin(v),
+//           cwiseAcos(v), cwiseTanh(v), cwiseRsqrt(|v|), cwisePow(|v|, 1.5), cwisePow(|v|, back3(q));
+// 1-planes: maxNorm(q), sum(v), norm(v), squaredNorm(v), rNorm(v), rSquaredNorm(v), cubedNorm(v),
+//           rCubedNorm(v).
+void members(std::size_t n, std::uint64_t sq, std::uint64_t sr, std::uint64_t sv)
+{
+    aether::Array<double, 4> q(n), r(n);
+    aether::Array<double, 3> v(n);
+    auto qv = q.hostView();
+    auto rv = r.hostView();
+    auto vv = v.hostView();
+    for (std::size_t k = 0; k < n; ++k) {
+        const auto i = SampleIndex::make(k);
+        qv[i] = rawUniformN<double, 4>(sq);
+        rv[i] = rawUniformN<double, 4>(sr);
+        vv[i] = rawUniformN<double, 3>(sv);
+    }
+    writePlanes(q, n);
+    writePlanes(r, n);
+    writePlanes(v, n);
+    // members are sample-free: they are called on each sample's materialised Item (Expression.h:155-160;
+    // called on a view, a reduction such as rNorm() would read sample 0 for every sample)
+    auto out4 = [&](auto make) {
+        aether::Array<double, 4> o(n);
+        auto ov = o.hostView();
+        for (std::size_t k = 0; k < n; ++k) {
+            const auto i = SampleIndex::make(k);
+            ov[i] = make(qv[i].get(), rv[i].get(), vv[i].get());
+        }
+        writePlanes(o, n);
+    };
+    auto out3 = [&](auto make) {
+        aether::Array<double, 3> o(n);
+        auto ov = o.hostView();
+        for (std::size_t k = 0; k < n; ++k) {
+            const auto i = SampleIndex::make(k);
+            ov[i] = make(qv[i].get(), rv[i].get(), vv[i].get());
+        }
+        writePlanes(o, n);
+    };
+    out4([](const auto& qi, const auto& ri, const auto&) { return qi.quatMul(ri); });
+    out4([](const auto& qi, const auto&, const auto&) { return qi.quatConj(); });
+    out4([](const auto& qi, const auto&, const auto&) { return qi.quatReciprocal(); });
+    out4([](const auto&, const auto&, const auto& vi) { return vi.asPureQuaternion(); });
+    out3([](const auto& qi, const auto&, const auto& vi) { return qi.quatRotate(vi); });
+    out3([](const auto&, const auto&, const auto& vi) { return vi.unitVector(); });
+    out3([](const auto& qi, const auto&, const auto&) { return qi.asBack3DVector(); });
+    out3([](const auto& qi, const auto&, const auto&) { return qi.template head<3>(); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseMin(vi, 0.25); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseMax(vi, -0.25); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::clamp(vi, -0.5, 0.5); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseSign(vi); });
+    out3([](const auto& qi, const auto&, const auto& vi) { return aether::cwiseDiv(vi, qi.asBack3DVector()); });
+    out3([](const auto& qi, const auto&, const auto& vi) { return vi.cwiseMul(qi.template head<3>()); });
+    out3([](const auto& qi, const auto&, const auto& vi) { return aether::cwiseMin(vi, qi.asBack3DVector()); });
+    out3([](const auto& qi, const auto&, const auto& vi) { return aether::cwiseMax(vi, qi.asBack3DVector()); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseTan(vi); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseAtan(vi); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseAsin(vi); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseAcos(vi); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseTanh(vi); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseRsqrt(aether::cwiseAbs(vi)); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwisePow(aether::cwiseAbs(vi), 1.5); });
+    out3([](const auto& qi, const auto&, const auto& vi) {
+        return aether::cwisePow(aether::cwiseAbs(vi), qi.asBack3DVector());
+    });
+    std::vector<double> s(8 * n);
+    for (std::size_t k = 0; k < n; ++k) {
+        const auto i = SampleIndex::make(k);
+        const auto qi = qv[i].get();
+        const auto vi = vv[i].get();
+        s[0 * n + k] = qi.maxNorm();
+        s[1 * n + k] = vi.sum();
+        s[2 * n + k] = vi.norm();
+        s[3 * n + k] = vi.squaredNorm();
+        s[4 * n + k] = vi.rNorm();
+        s[5 * n + k] = vi.rSquaredNorm();
+        s[6 * n + k] = vi.cubedNorm();
+        s[7 * n + k] = vi.rCubedNorm();
+    }
+    std::fwrite(s.data(), sizeof(double), s.size(), stdout);
 }
 
 void rows(std::uint64_t seed, std::size_t n, std::uint64_t counter)
@@ -220,6 +317,8 @@ int main(int argc, char** argv)
         rows(u(2), u(3), u(4));
     } else if (mode == "fft" && argc == 5) {
         fft(u(2), u(3), u(4) != 0);
+    } else if (mode == "members" && argc == 6) {
+        members(u(2), u(3), u(4), u(5));
     } else if (mode == "bench" && argc == 4) {
         bench(u(2), static_cast<int>(u(3)));
     } else {

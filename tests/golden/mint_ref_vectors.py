@@ -3,7 +3,7 @@
 Runs oracle/_ref/aether_ref (the reference's header-only CPU path compiled
 from its sources by `make -C oracle ref`) and stores small input/output
 vectors next to this script. Run in the build container, where the reference
-tree exists: python tests/golden/mint_ref_vectors.py
+tree exists: python tests/golden/mint_ref_vectors.py [members]
 """
 import hashlib
 import os
@@ -70,9 +70,38 @@ def mint_fft(n, seed):
     print("fft", n)
 
 
-def main():
+MEMBER_PLANES = [  # (name, components), in the order the driver writes them after q, r, v
+    ("quat_mul", 4), ("quat_conj", 4), ("quat_reciprocal", 4), ("as_pure_quaternion", 4),
+    ("quat_rotate", 3), ("unit_vector", 3), ("as_back3", 3), ("head3", 3), ("min_025", 3), ("max_m025", 3),
+    ("clamp_05", 3), ("sign", 3), ("div_back3", 3), ("mul_head3", 3), ("min_back3", 3), ("max_back3", 3),
+    ("tan", 3), ("atan", 3), ("asin", 3), ("acos", 3), ("tanh", 3), ("rsqrt_abs", 3), ("pow_abs_15", 3),
+    ("pow_abs_back3", 3),
+    ("max_norm", 1), ("sum", 1), ("norm", 1), ("squared_norm", 1), ("r_norm", 1), ("r_squared_norm", 1),
+    ("cubed_norm", 1), ("r_cubed_norm", 1)]
+
+
+def mint_members(n, seeds):
+    """Inputs are raw-seed uniforms on (-1, 1) (regenerated by the tests), stored as hashes and heads."""
+    x = np.frombuffer(run("members", n, *seeds), np.float64)
+    out, off = {"n": n, "seeds": np.array(seeds, np.uint64)}, 0
+    for name, k in [("q", 4), ("r", 4), ("v", 3)] + MEMBER_PLANES:
+        plane = x[off: off + k * n].reshape(k, n)
+        if name in ("q", "r", "v"):
+            out["sha_" + name], out["head_" + name] = sha(plane), plane[:, :4].copy()
+        else:
+            out[name] = plane
+        off += k * n
+    assert off == x.size
+    np.savez_compressed(os.path.join(HERE, "ref_members_f64.npz"), **out)
+    print("members", n, len(MEMBER_PLANES))
+
+
+def main(parts):
     if not os.path.exists(REF):
         sys.exit("build the reference driver first: make -C oracle ref")
+    if parts == ["members"]:
+        mint_members(512, (11, 12, 13))
+        return
     mint_cfg0("cfg0", 2000, (42, 43, 44), np.float64, "ref_cfg0_f64.npz")
     mint_cfg0("cfg0f", 2000, (42, 43, 44), np.float32, "ref_cfg0_f32.npz")
     raw = run("cfg0g", 2000, 42)
@@ -88,7 +117,8 @@ def main():
     mint_keys()
     for n in (64, 1024):
         mint_fft(n, 7)
+    mint_members(512, (11, 12, 13))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

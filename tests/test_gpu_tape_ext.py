@@ -16,6 +16,7 @@ import pytest
 import torch
 
 import aether_amd as ae
+import members_ref as M
 from aether_amd import build as B
 from aether_amd import expr as E
 
@@ -129,47 +130,7 @@ def test_scalar_forms_and_clamp(dev, orc):
     assert ok, (x[bad], host(d)[0][bad], want[bad])
 
 
-def hamilton(l, r):
-    """Quaternion.h:66-75, each sum left to right."""
-    return np.stack([l[0] * r[0] - l[1] * r[1] - l[2] * r[2] - l[3] * r[3],
-                     l[0] * r[1] + l[1] * r[0] + l[2] * r[3] - l[3] * r[2],
-                     l[0] * r[2] - l[1] * r[3] + l[2] * r[0] + l[3] * r[1],
-                     l[0] * r[3] + l[1] * r[2] - l[2] * r[1] + l[3] * r[0]])
-
-
-def rotate(q, v):
-    """Quaternion.h:151-166: t = 2 (u x v), v + w t + (u x t)."""
-    w, ux, uy, uz = q
-    vx, vy, vz = v
-    two = q.dtype.type(2)
-    tx, ty, tz = two * (uy * vz - uz * vy), two * (uz * vx - ux * vz), two * (ux * vy - uy * vx)
-    return np.stack([vx + w * tx + (uy * tz - uz * ty), vy + w * ty + (uz * tx - ux * tz),
-                     vz + w * tz + (ux * ty - uy * tx)])
-
-
-def dot(x, y):
-    """Reduce.h:172-188: x_{N-1} y_{N-1} + (... + x_0 y_0)."""
-    acc = x[0] * y[0]
-    for k in range(1, len(x)):
-        acc = x[k] * y[k] + acc
-    return acc
-
-
-def fold_sum(x):
-    acc = x[0]
-    for k in range(1, len(x)):
-        acc = x[k] + acc
-    return acc
-
-
-def fold_max(x):
-    """Reduce.h:74-130,150-164: T(x) = x < 0 ? -x : x written as (-x > x ? -x : x); acc = T(x_{k+1}) > acc ?
-    T(x_{k+1}) : acc."""
-    t = lambda v: np.where(-v > v, -v, v)  # noqa: E731
-    acc = t(x[0])
-    for k in range(1, len(x)):
-        acc = np.where(t(x[k]) > acc, t(x[k]), acc)
-    return acc
+from members_ref import dot, fold_max, fold_sum, hamilton, rotate  # noqa: E402,F401
 
 
 @pytest.mark.parametrize("dt", [torch.float64, torch.float32])
@@ -283,3 +244,42 @@ def test_cpp_members_device_equal_host_and_restatement(dev, orc):
     assert same_bits(o4, want4)[0]
     assert same_bits(o5, orc.math64("tan", a.ravel()).reshape(a.shape) +
                      orc.math2_64("pow", np.abs(a).ravel(), np.full(a.size, 1.5)).reshape(a.shape))[0]
+
+
+def test_members_equal_reference_bits(dev, orc):
+    """The reference's own outputs (ref_members_f64.npz, minted from its CPU path): every arithmetic member
+    bit for bit; the transcendental ones equal to the oracle bit for bit and within the reference's own
+    accuracy of its values."""
+    z = M.load_members()
+    n = int(z["n"])
+    sq, sr, sv = (int(x) for x in z["seeds"])
+    q, r, v = ae.Array(n, 4, torch.float64), ae.Array(n, 4, torch.float64), ae.Array(n, 3, torch.float64)
+    for arr, seed in ((q, sq), (r, sr), (v, sv)):
+        ae.array_fill_uniform(arr, seed, -1.0, 1.0)
+    L = E.Leaf
+    b3 = L(q).asBack3DVector()
+    cases = {
+        "quat_mul": L(q).quatMul(r), "quat_conj": L(q).quatConj(), "quat_reciprocal": L(q).quatReciprocal(),
+        "as_pure_quaternion": L(v).asPureQuaternion(), "quat_rotate": L(q).quatRotate(v),
+        "unit_vector": L(v).unitVector(), "as_back3": b3, "head3": L(q).head(3), "min_025": E.minimum(v, 0.25),
+        "max_m025": E.maximum(v, -0.25), "clamp_05": E.clamp(v, -0.5, 0.5), "sign": E.sign(v),
+        "div_back3": E.div(v, b3), "mul_head3": L(v).cwiseMul(L(q).head(3)), "min_back3": E.minimum(v, b3),
+        "max_back3": E.maximum(v, b3), "tan": E.tan(v), "atan": E.atan(v), "asin": E.asin(v), "acos": E.acos(v),
+        "tanh": E.tanh(v), "rsqrt_abs": E.rsqrt(E.abs_(v)), "pow_abs_15": E.pow_(E.abs_(v), 1.5),
+        "pow_abs_back3": E.pow_(E.abs_(v), b3), "max_norm": L(q).maxNorm(), "sum": L(v).sum(),
+        "norm": L(v).norm(), "squared_norm": L(v).squaredNorm(), "r_norm": L(v).rNorm(),
+        "r_squared_norm": L(v).rSquaredNorm(), "cubed_norm": L(v).cubedNorm(), "r_cubed_norm": L(v).rCubedNorm(),
+    }
+    hq, hv = host(q), host(v)
+    trans = M.transcendental(orc, hq, hv)
+    for name, ex in cases.items():
+        want = z[name]
+        d = ae.Array(n, want.shape[0], torch.float64)
+        E.assign(d, ex)
+        got = host(d)
+        if name in trans:
+            assert same_bits(got, trans[name])[0], name
+            assert np.max(M.ulps(got, want)) <= (2.0 if name == "tanh" else 1.0), name
+        else:
+            ok, bad = same_bits(got, want)
+            assert ok, (name, bad)
