@@ -17,9 +17,10 @@
 //   fft    <n> <seed> <fwd>             : pocketfft c2c of a length-n signal
 //                                         (entries u01-derived), in then out
 //   bench  <n> <reps>                   : times the OpenMP fused loop (text)
-//   members <n> <sQ> <sR> <sV>          : q, r (4 planes), v (3 planes) of raw-seed uniforms, then the
+//   members[f] <n> <sQ> <sR> <sV>       : q, r (4 planes), v (3 planes) of raw-seed uniforms, then the
 //                                         per-sample members and elementwise functions listed in
 //                                         members() below, each as its planes, in that order
+//                                         (membersf: in float)
 #include <aether/aether.h>
 #include <aether/fft/Fft.h>
 
@@ -141,18 +142,19 @@ void vec3(std::size_t n, std::uint64_t sa, std::uint64_t sb, std::uint64_t sc)
 //           cwiseAcos(v), cwiseTanh(v), cwiseRsqrt(|v|), cwisePow(|v|, 1.5), cwisePow(|v|, back3(q));
 // 1-planes: maxNorm(q), sum(v), norm(v), squaredNorm(v), rNorm(v), rSquaredNorm(v), cubedNorm(v),
 //           rCubedNorm(v).
+template<class T>
 void members(std::size_t n, std::uint64_t sq, std::uint64_t sr, std::uint64_t sv)
 {
-    aether::Array<double, 4> q(n), r(n);
-    aether::Array<double, 3> v(n);
+    aether::Array<T, 4> q(n), r(n);
+    aether::Array<T, 3> v(n);
     auto qv = q.hostView();
     auto rv = r.hostView();
     auto vv = v.hostView();
     for (std::size_t k = 0; k < n; ++k) {
         const auto i = SampleIndex::make(k);
-        qv[i] = rawUniformN<double, 4>(sq);
-        rv[i] = rawUniformN<double, 4>(sr);
-        vv[i] = rawUniformN<double, 3>(sv);
+        qv[i] = rawUniformN<T, 4>(sq);
+        rv[i] = rawUniformN<T, 4>(sr);
+        vv[i] = rawUniformN<T, 3>(sv);
     }
     writePlanes(q, n);
     writePlanes(r, n);
@@ -160,7 +162,7 @@ void members(std::size_t n, std::uint64_t sq, std::uint64_t sr, std::uint64_t sv
     // members are sample-free: they are called on each sample's materialised Item (Expression.h:155-160;
     // called on a view, a reduction such as rNorm() would read sample 0 for every sample)
     auto out4 = [&](auto make) {
-        aether::Array<double, 4> o(n);
+        aether::Array<T, 4> o(n);
         auto ov = o.hostView();
         for (std::size_t k = 0; k < n; ++k) {
             const auto i = SampleIndex::make(k);
@@ -169,7 +171,7 @@ void members(std::size_t n, std::uint64_t sq, std::uint64_t sr, std::uint64_t sv
         writePlanes(o, n);
     };
     auto out3 = [&](auto make) {
-        aether::Array<double, 3> o(n);
+        aether::Array<T, 3> o(n);
         auto ov = o.hostView();
         for (std::size_t k = 0; k < n; ++k) {
             const auto i = SampleIndex::make(k);
@@ -185,9 +187,9 @@ void members(std::size_t n, std::uint64_t sq, std::uint64_t sr, std::uint64_t sv
     out3([](const auto&, const auto&, const auto& vi) { return vi.unitVector(); });
     out3([](const auto& qi, const auto&, const auto&) { return qi.asBack3DVector(); });
     out3([](const auto& qi, const auto&, const auto&) { return qi.template head<3>(); });
-    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseMin(vi, 0.25); });
-    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseMax(vi, -0.25); });
-    out3([](const auto&, const auto&, const auto& vi) { return aether::clamp(vi, -0.5, 0.5); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseMin(vi, T(0.25)); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseMax(vi, T(-0.25)); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::clamp(vi, T(-0.5), T(0.5)); });
     out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseSign(vi); });
     out3([](const auto& qi, const auto&, const auto& vi) { return aether::cwiseDiv(vi, qi.asBack3DVector()); });
     out3([](const auto& qi, const auto&, const auto& vi) { return vi.cwiseMul(qi.template head<3>()); });
@@ -199,11 +201,11 @@ void members(std::size_t n, std::uint64_t sq, std::uint64_t sr, std::uint64_t sv
     out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseAcos(vi); });
     out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseTanh(vi); });
     out3([](const auto&, const auto&, const auto& vi) { return aether::cwiseRsqrt(aether::cwiseAbs(vi)); });
-    out3([](const auto&, const auto&, const auto& vi) { return aether::cwisePow(aether::cwiseAbs(vi), 1.5); });
+    out3([](const auto&, const auto&, const auto& vi) { return aether::cwisePow(aether::cwiseAbs(vi), T(1.5)); });
     out3([](const auto& qi, const auto&, const auto& vi) {
         return aether::cwisePow(aether::cwiseAbs(vi), qi.asBack3DVector());
     });
-    std::vector<double> s(8 * n);
+    std::vector<T> s(8 * n);
     for (std::size_t k = 0; k < n; ++k) {
         const auto i = SampleIndex::make(k);
         const auto qi = qv[i].get();
@@ -217,7 +219,7 @@ void members(std::size_t n, std::uint64_t sq, std::uint64_t sr, std::uint64_t sv
         s[6 * n + k] = vi.cubedNorm();
         s[7 * n + k] = vi.rCubedNorm();
     }
-    std::fwrite(s.data(), sizeof(double), s.size(), stdout);
+    std::fwrite(s.data(), sizeof(T), s.size(), stdout);
 }
 
 void rows(std::uint64_t seed, std::size_t n, std::uint64_t counter)
@@ -318,7 +320,9 @@ int main(int argc, char** argv)
     } else if (mode == "fft" && argc == 5) {
         fft(u(2), u(3), u(4) != 0);
     } else if (mode == "members" && argc == 6) {
-        members(u(2), u(3), u(4), u(5));
+        members<double>(u(2), u(3), u(4), u(5));
+    } else if (mode == "membersf" && argc == 6) {
+        members<float>(u(2), u(3), u(4), u(5));
     } else if (mode == "bench" && argc == 4) {
         bench(u(2), static_cast<int>(u(3)));
     } else {

@@ -80,9 +80,10 @@ MEMBER_PLANES = [  # (name, components), in the order the driver writes them aft
     ("cubed_norm", 1), ("r_cubed_norm", 1)]
 
 
-def mint_members(n, seeds):
+def mint_members(n, seeds, dtype=np.float64):
     """Inputs are raw-seed uniforms on (-1, 1) (regenerated by the tests), stored as hashes and heads."""
-    x = np.frombuffer(run("members", n, *seeds), np.float64)
+    mode, fname = ("members", "ref_members_f64.npz") if dtype == np.float64 else ("membersf", "ref_members_f32.npz")
+    x = np.frombuffer(run(mode, n, *seeds), dtype)
     out, off = {"n": n, "seeds": np.array(seeds, np.uint64)}, 0
     for name, k in [("q", 4), ("r", 4), ("v", 3)] + MEMBER_PLANES:
         plane = x[off: off + k * n].reshape(k, n)
@@ -92,8 +93,8 @@ def mint_members(n, seeds):
             out[name] = plane
         off += k * n
     assert off == x.size
-    np.savez_compressed(os.path.join(HERE, "ref_members_f64.npz"), **out)
-    print("members", n, len(MEMBER_PLANES))
+    np.savez_compressed(os.path.join(HERE, fname), **out)
+    print(mode, n, len(MEMBER_PLANES))
 
 
 def main(parts):
@@ -101,6 +102,7 @@ def main(parts):
         sys.exit("build the reference driver first: make -C oracle ref")
     if parts == ["members"]:
         mint_members(512, (11, 12, 13))
+        mint_members(512, (11, 12, 13), np.float32)
         return
     mint_cfg0("cfg0", 2000, (42, 43, 44), np.float64, "ref_cfg0_f64.npz")
     mint_cfg0("cfg0f", 2000, (42, 43, 44), np.float32, "ref_cfg0_f32.npz")
@@ -118,6 +120,7 @@ def main(parts):
     for n in (64, 1024):
         mint_fft(n, 7)
     mint_members(512, (11, 12, 13))
+    mint_members(512, (11, 12, 13), np.float32)
 
 
 if __name__ == "__main__":

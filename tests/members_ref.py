@@ -56,16 +56,16 @@ def fold_max(x):
     return acc
 
 
-def load_members():
-    return np.load(os.path.join(GOLDEN, "ref_members_f64.npz"))
+def load_members(dtype=np.float64):
+    return np.load(os.path.join(GOLDEN, "ref_members_f64.npz" if dtype == np.float64 else "ref_members_f32.npz"))
 
 
-def members_inputs(orc, z):
+def members_inputs(orc, z, dtype=np.float64):
     """q, r (4 planes) and v (3 planes): raw-seed uniforms on (-1, 1) with the fixture's seeds."""
     n = int(z["n"])
     sq, sr, sv = (int(s) for s in z["seeds"])
-    return (orc.fill_array_uniform(sq, 4, n, n).reshape(4, n), orc.fill_array_uniform(sr, 4, n, n).reshape(4, n),
-            orc.fill_array_uniform(sv, 3, n, n).reshape(3, n))
+    f = lambda seed, k: orc.fill_array_uniform(seed, k, n, n, dtype=dtype).reshape(k, n)  # noqa: E731
+    return f(sq, 4), f(sr, 4), f(sv, 3)
 
 
 def restated(q, r, v):
@@ -78,7 +78,8 @@ def restated(q, r, v):
             "quat_mul": hamilton(q, r), "quat_conj": conj(q), "quat_reciprocal": conj(q) * (one / dot(q, q)),
             "as_pure_quaternion": np.stack([np.zeros_like(v[0]), v[0], v[1], v[2]]),
             "quat_rotate": rotate(q, v), "unit_vector": v * (one / np.sqrt(nn)), "as_back3": b3, "head3": q[:3],
-            "min_025": np.fmin(v, 0.25), "max_m025": np.fmax(v, -0.25), "clamp_05": np.fmin(np.fmax(v, -0.5), 0.5),
+            "min_025": np.fmin(v, v.dtype.type(0.25)), "max_m025": np.fmax(v, v.dtype.type(-0.25)),
+            "clamp_05": np.fmin(np.fmax(v, v.dtype.type(-0.5)), v.dtype.type(0.5)),
             "sign": np.sign(v), "div_back3": v / b3, "mul_head3": v * q[:3], "min_back3": np.fmin(v, b3),
             "max_back3": np.fmax(v, b3),
             "max_norm": fold_max(q)[None], "sum": fold_sum(v)[None], "norm": np.sqrt(nn)[None],
@@ -92,14 +93,20 @@ TRANSCENDENTAL = ["tan", "atan", "asin", "acos", "tanh", "rsqrt_abs", "pow_abs_1
 
 
 def transcendental(orc, q, v):
+    """The oracle's values of the transcendental members (double or float, as v is)."""
     av = np.abs(v)
-    f = lambda fn, x: orc.math64(fn, x.ravel()).reshape(x.shape)  # noqa: E731
+    f32 = v.dtype == np.float32
+    m1, m2 = (orc.math32, orc.math2_32) if f32 else (orc.math64, orc.math2_64)
+    f = lambda fn, x: m1(fn, x.ravel()).reshape(x.shape)  # noqa: E731
+    one = v.dtype.type(1)
     return {"tan": f("tan", v), "atan": f("atan", v), "asin": f("asin", v), "acos": f("acos", v),
-            "tanh": f("tanh", v), "rsqrt_abs": 1.0 / np.sqrt(av),
-            "pow_abs_15": orc.math2_64("pow", av.ravel(), np.full(av.size, 1.5)).reshape(v.shape),
-            "pow_abs_back3": orc.math2_64("pow", av.ravel(), q[1:].ravel()).reshape(v.shape)}
+            "tanh": f("tanh", v), "rsqrt_abs": one / np.sqrt(av),
+            "pow_abs_15": m2("pow", av.ravel(), np.full(av.size, 1.5, v.dtype)).reshape(v.shape),
+            "pow_abs_back3": m2("pow", av.ravel(), q[1:].ravel()).reshape(v.shape)}
 
 
 def ulps(a, b):
-    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
-    return np.abs(a - b) / np.spacing(np.abs(b))
+    """|a - b| in units in the last place of b, in b's own precision."""
+    b = np.asarray(b)
+    sp = np.spacing(np.abs(b)).astype(np.float64)
+    return np.abs(np.asarray(a, np.float64) - b.astype(np.float64)) / sp

@@ -246,14 +246,15 @@ def test_cpp_members_device_equal_host_and_restatement(dev, orc):
                      orc.math2_64("pow", np.abs(a).ravel(), np.full(a.size, 1.5)).reshape(a.shape))[0]
 
 
-def test_members_equal_reference_bits(dev, orc):
-    """The reference's own outputs (ref_members_f64.npz, minted from its CPU path): every arithmetic member
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_members_equal_reference_bits(dev, orc, dt):
+    """The reference's own outputs (ref_members_f64/f32.npz, minted from its CPU path): every arithmetic member
     bit for bit; the transcendental ones equal to the oracle bit for bit and within the reference's own
     accuracy of its values."""
-    z = M.load_members()
+    z = M.load_members(np.float64 if dt == torch.float64 else np.float32)
     n = int(z["n"])
     sq, sr, sv = (int(x) for x in z["seeds"])
-    q, r, v = ae.Array(n, 4, torch.float64), ae.Array(n, 4, torch.float64), ae.Array(n, 3, torch.float64)
+    q, r, v = ae.Array(n, 4, dt), ae.Array(n, 4, dt), ae.Array(n, 3, dt)
     for arr, seed in ((q, sq), (r, sr), (v, sv)):
         ae.array_fill_uniform(arr, seed, -1.0, 1.0)
     L = E.Leaf
@@ -274,7 +275,7 @@ def test_members_equal_reference_bits(dev, orc):
     trans = M.transcendental(orc, hq, hv)
     for name, ex in cases.items():
         want = z[name]
-        d = ae.Array(n, want.shape[0], torch.float64)
+        d = ae.Array(n, want.shape[0], dt)
         E.assign(d, ex)
         got = host(d)
         if name in trans:

@@ -1,5 +1,6 @@
-"""The per-sample members and the elementwise functions against the REFERENCE's own outputs
-(tests/golden/ref_members_f64.npz, minted from its CPU path by tests/golden/mint_ref_vectors.py).
+"""The per-sample members and the elementwise functions against the REFERENCE's own outputs, in double and
+in float (tests/golden/ref_members_f64.npz, ref_members_f32.npz, minted from its CPU path by
+tests/golden/mint_ref_vectors.py).
 
 The restatements in members_ref.py (which the GPU tests hold the device to, bit for bit) equal the
 reference's bits on every arithmetic member; the oracle's transcendental functions are within the reference's
@@ -15,10 +16,10 @@ import members_ref as M
 BOUND = {"tanh": 2.0}
 
 
-@pytest.fixture(scope="module")
-def data(orc):
-    z = M.load_members()
-    q, r, v = M.members_inputs(orc, z)
+@pytest.fixture(scope="module", params=[np.float64, np.float32], ids=["f64", "f32"])
+def data(orc, request):
+    z = M.load_members(request.param)
+    q, r, v = M.members_inputs(orc, z, request.param)
     return z, q, r, v
 
 
@@ -34,7 +35,9 @@ def test_arithmetic_members_equal_reference_bits(data):
     for name, want in M.restated(q, r, v).items():
         got = z[name]
         assert got.shape == want.shape, name
-        assert np.array_equal(got.view(np.uint64), np.ascontiguousarray(want).view(np.uint64)), name
+        u = np.uint64 if got.dtype == np.float64 else np.uint32
+        assert want.dtype == got.dtype, name
+        assert np.array_equal(got.view(u), np.ascontiguousarray(want).view(u)), name
 
 
 @pytest.mark.parametrize("name", M.TRANSCENDENTAL)
