@@ -182,7 +182,8 @@ def math(fn: str, x: torch.Tensor, y: torch.Tensor = None, stream=None) -> torch
 class FftBatch:
     """A batch of uneven power-of-two complex FFTs run as one launch (aeth_fft_batch_*).
 
-    prec: 0 complex128, 1 complex64, 2 complex64 in memory with FP64 arithmetic (only the final rounding to float remains).
+    prec: 0 complex128, 1 complex64, 2 complex64 with float-float arithmetic (the compensated FP32 mode), 3 complex64 in memory with FP64
+    arithmetic.
     lengths / offsets are in complex elements of the flat input/output buffers.
     """
 

@@ -12,11 +12,13 @@
 // LDS rows are padded by one element every 16 so the stride-4 writes of the
 // first passes do not collide in the banks.
 //
-// Precisions: complex128; complex64; complex64 in memory with FP64 arithmetic
-// inside (FP64 is full rate on this GPU, so the wide mode costs the double
-// transform's arithmetic and the float transform's memory traffic; only the
-// final rounding to float remains). Twiddles come from one FP64 table
-// exp(-2 pi i m / 8192) generated on the device.
+// Precisions: complex128; complex64; complex64 in memory with float-float
+// (Ff2-style, banded/Ff2.h:132-145,246-321) arithmetic inside, the
+// compensated FP32 mode; and complex64 in memory with FP64 arithmetic inside
+// (FP64 is full rate on this GPU: the double transform's arithmetic at the
+// float transform's memory traffic). Twiddles come from one FP64 table
+// exp(-2 pi i m / 8192) generated on the device (float-float pairs for the
+// compensated mode).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,6 +52,10 @@ struct C128 {
 struct C64 {
     float re, im;
 };
+// complex64 in memory, float-float arithmetic: (re_hi, re_lo, im_hi, im_lo)
+struct CFF {
+    float rh, rl, ih, il;
+};
 // complex64 in memory, FP64 arithmetic (the same arithmetic as C128, so equal inputs give equal bits)
 struct CW {
     double re, im;
@@ -62,6 +68,29 @@ __device__ __forceinline__ C64 sub(C64 a, C64 b) { return {a.re - b.re, a.im - b
 __device__ __forceinline__ CW add(CW a, CW b) { return {a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ CW sub(CW a, CW b) { return {a.re - b.re, a.im - b.im}; }
 
+// float-float sum for the butterflies: TwoSum of the high parts, the low parts added to its error, one
+// fast renormalisation (11 flops). Its error is below 2^-44 (|a| + |b|): a transform's error is measured
+// against the norm of its input, so cancellation in a + b costs nothing there (the accurate Ff2 add of
+// accum.hpp, 23 flops, stays on the reductions).
+__device__ __forceinline__ acc::Pair<float> ffadd(float ah, float al, float bh, float bl)
+{
+    const float s = ah + bh;
+    const float bb = s - ah;
+    const float e = (ah - (s - bb)) + (bh - bb) + (al + bl);
+    const float hi = s + e;
+    return acc::Pair<float>{hi, e - (hi - s)};
+}
+__device__ __forceinline__ CFF add(CFF a, CFF b)
+{
+    const auto r = ffadd(a.rh, a.rl, b.rh, b.rl), i = ffadd(a.ih, a.il, b.ih, b.il);
+    return {r.h, r.c, i.h, i.c};
+}
+__device__ __forceinline__ CFF sub(CFF a, CFF b)
+{
+    const auto r = ffadd(a.rh, a.rl, -b.rh, -b.rl), i = ffadd(a.ih, a.il, -b.ih, -b.il);
+    return {r.h, r.c, i.h, i.c};
+}
+
 // -i * x (forward) or +i * x (inverse): exact
 template<class C>
 __device__ __forceinline__ C rot(C x, bool fwd);
@@ -71,11 +100,28 @@ template<>
 __device__ __forceinline__ C64 rot(C64 x, bool fwd) { return fwd ? C64{x.im, -x.re} : C64{-x.im, x.re}; }
 template<>
 __device__ __forceinline__ CW rot(CW x, bool fwd) { return fwd ? CW{x.im, -x.re} : CW{-x.im, x.re}; }
+template<>
+__device__ __forceinline__ CFF rot(CFF x, bool fwd)
+{
+    return fwd ? CFF{x.ih, x.il, -x.rh, -x.rl} : CFF{-x.ih, -x.il, x.rh, x.rl};
+}
+
+// float-float product of (ah + al)(bh + bl), 2Prod + cross terms, fast2Sum renormalised
+__device__ __forceinline__ acc::Pair<float> ffmul(float ah, float al, float bh, float bl)
+{
+    const float p = ah * bh;
+    float e = __builtin_fmaf(ah, bh, -p);
+    e = __builtin_fmaf(ah, bl, e);
+    e = __builtin_fmaf(al, bh, e);
+    const float hi = p + e;
+    return acc::Pair<float>{hi, e - (hi - p)};
+}
 
 // twiddle tables: W[m] = exp(-2 pi i m / 8192)
 struct TwTables {
     const double2* d = nullptr;
     const float2* f = nullptr;
+    const float4* ff = nullptr; // (re_hi, re_lo, im_hi, im_lo)
 };
 
 __device__ __forceinline__ C128 twiddle(const TwTables& t, uint32_t m, bool fwd, C128*)
@@ -87,6 +133,11 @@ __device__ __forceinline__ C64 twiddle(const TwTables& t, uint32_t m, bool fwd, 
 {
     const float2 w = t.f[m];
     return {w.x, fwd ? w.y : -w.y};
+}
+__device__ __forceinline__ CFF twiddle(const TwTables& t, uint32_t m, bool fwd, CFF*)
+{
+    const float4 w = t.ff[m];
+    return fwd ? CFF{w.x, w.y, w.z, w.w} : CFF{w.x, w.y, -w.z, -w.w};
 }
 __device__ __forceinline__ CW twiddle(const TwTables& t, uint32_t m, bool fwd, CW*)
 {
@@ -103,12 +154,36 @@ __device__ __forceinline__ C64 cmul(C64 a, C64 w)
 {
     return {__builtin_fmaf(a.re, w.re, -(a.im * w.im)), __builtin_fmaf(a.re, w.im, a.im * w.re)};
 }
+// float-float x*y + u*v as one dot product: two exact products (FMA residuals), TwoSum of their high
+// parts, every low-order term (residuals, cross products) gathered into one float, one fast renormalisation
+__device__ __forceinline__ acc::Pair<float> ffdot2(float xh, float xl, float yh, float yl, float uh, float ul,
+                                                   float vh, float vl)
+{
+    const float p = xh * yh, q = uh * vh;
+    const float pe = __builtin_fmaf(xh, yh, -p), qe = __builtin_fmaf(uh, vh, -q);
+    const float s = p + q;
+    const float bb = s - p;
+    float e = (p - (s - bb)) + (q - bb);
+    e += pe + qe;
+    e = __builtin_fmaf(xh, yl, e);
+    e = __builtin_fmaf(xl, yh, e);
+    e = __builtin_fmaf(uh, vl, e);
+    e = __builtin_fmaf(ul, vh, e);
+    const float hi = s + e;
+    return acc::Pair<float>{hi, e - (hi - s)};
+}
+__device__ __forceinline__ CFF cmul(CFF a, CFF w)
+{
+    const auto re = ffdot2(a.rh, a.rl, w.rh, w.rl, -a.ih, -a.il, w.ih, w.il);
+    const auto im = ffdot2(a.rh, a.rl, w.ih, w.il, a.ih, a.il, w.rh, w.rl);
+    return {re.h, re.c, im.h, im.c};
+}
 __device__ __forceinline__ CW cmul(CW a, CW w)
 {
     return {__builtin_fma(a.re, w.re, -(a.im * w.im)), __builtin_fma(a.re, w.im, a.im * w.re)};
 }
 
-// global I/O: complex128 as double2, complex64 (plain and wide) as float2
+// global I/O: complex128 as double2, complex64 (plain, float-float and wide) as float2
 __device__ __forceinline__ C128 gload(const void* p, uint64_t i, C128*)
 {
     const double2 v = static_cast<const double2*>(p)[i];
@@ -118,6 +193,11 @@ __device__ __forceinline__ C64 gload(const void* p, uint64_t i, C64*)
 {
     const float2 v = static_cast<const float2*>(p)[i];
     return {v.x, v.y};
+}
+__device__ __forceinline__ CFF gload(const void* p, uint64_t i, CFF*)
+{
+    const float2 v = static_cast<const float2*>(p)[i];
+    return {v.x, 0.0f, v.y, 0.0f};
 }
 __device__ __forceinline__ CW gload(const void* p, uint64_t i, CW*)
 {
@@ -132,6 +212,13 @@ __device__ __forceinline__ void gstore(void* p, uint64_t i, C64 v, double s)
 {
     const float fs = (float)s;
     static_cast<float2*>(p)[i] = make_float2(v.re * fs, v.im * fs);
+}
+__device__ __forceinline__ void gstore(void* p, uint64_t i, CFF v, double s)
+{
+    // scale in float-float (s as a float-float pair), then round once to float
+    const float sh = (float)s, sl = (float)(s - (double)sh);
+    const auto r = ffmul(v.rh, v.rl, sh, sl), im = ffmul(v.ih, v.il, sh, sl);
+    static_cast<float2*>(p)[i] = make_float2(r.h + r.c, im.h + im.c);
 }
 __device__ __forceinline__ void gstore(void* p, uint64_t i, CW v, double s)
 {
@@ -260,6 +347,13 @@ struct Words<C128> {
     __device__ static void put(C128& v, int, W w) { v = C128{w.x, w.y}; }
 };
 template<>
+struct Words<CFF> {
+    static constexpr int n = 1;
+    using W = float4;
+    __device__ static W get(const CFF& v, int) { return make_float4(v.rh, v.rl, v.ih, v.il); }
+    __device__ static void put(CFF& v, int, W w) { v = CFF{w.x, w.y, w.z, w.w}; }
+};
+template<>
 struct Words<CW> {
     static constexpr int n = 1;
     using W = double2;
@@ -300,6 +394,11 @@ __device__ __forceinline__ C w16(C x, bool fwd)
         return x;
     } else if constexpr (E % 16 == 4) {
         return rot(x, fwd);
+    } else if constexpr (std::is_same_v<C, CFF>) {
+        // the constant as a float-float pair (hi, lo) from its double value
+        const double cd = kCos16[E % 16], sd = kCos16[(E + 12) % 16];
+        const float ch = (float)cd, cl = (float)(cd - (double)ch), sh = (float)sd, sl = (float)(sd - (double)sh);
+        return cmul(x, fwd ? CFF{ch, cl, -sh, -sl} : CFF{ch, cl, sh, sl});
     } else {
         using R = decltype(x.re);
         const R c = (R)kCos16[E % 16], s = (R)kCos16[(E + 12) % 16]; // sin(t) = cos(t - pi/2)
@@ -682,7 +781,7 @@ __global__ __launch_bounds__(kRT, sizeof(C) == 8 ? 4 : 2) void fft_reg_kernel(co
     rchunk(v, d, sigs + d.first_sig, in, out, xch, TwGlobal{tw}, fwd_flag != 0, norm, fixed_scale);
 }
 
-__global__ void twiddle_kernel(double2* d, float2* f)
+__global__ void twiddle_kernel(double2* d, float2* f, float4* ff)
 {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= (uint32_t)kMaxN) return;
@@ -697,12 +796,15 @@ __global__ void twiddle_kernel(double2* d, float2* f)
     }
     d[m] = make_double2(c, s);
     f[m] = make_float2((float)c, (float)s);
+    const float ch = (float)c, sh = (float)s;
+    ff[m] = make_float4(ch, (float)(c - (double)ch), sh, (float)(s - (double)sh));
 }
 
 // Per-device twiddle tables, created once.
 struct DeviceTables {
     double2* d = nullptr;
     float2* f = nullptr;
+    float4* ff = nullptr;
 };
 std::mutex g_tw_mutex;
 std::map<int, DeviceTables> g_tw;
@@ -717,10 +819,11 @@ TwTables tables(hipStream_t st)
         DeviceTables t;
         check(hipMalloc(&t.d, sizeof(double2) * kMaxN), "hipMalloc(twiddles)", sizeof(double2) * kMaxN);
         check(hipMalloc(&t.f, sizeof(float2) * kMaxN), "hipMalloc(twiddles)", sizeof(float2) * kMaxN);
-        hipLaunchKernelGGL(twiddle_kernel, dim3(kMaxN / 256), dim3(256), 0, st, t.d, t.f);
+        check(hipMalloc(&t.ff, sizeof(float4) * kMaxN), "hipMalloc(twiddles)", sizeof(float4) * kMaxN);
+        hipLaunchKernelGGL(twiddle_kernel, dim3(kMaxN / 256), dim3(256), 0, st, t.d, t.f, t.ff);
         check(hipGetLastError(), "twiddle_kernel launch");
         check(hipStreamSynchronize(st), "hipStreamSynchronize(twiddles)");
-        for (auto fn : {(const void*)fft_kernel<C128>, (const void*)fft_kernel<C64>, (const void*)fft_kernel<CW>}) {
+        for (auto fn : {(const void*)fft_kernel<C128>, (const void*)fft_kernel<C64>, (const void*)fft_kernel<CFF>, (const void*)fft_kernel<CW>}) {
             check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsElems * 16),
                   "hipFuncSetAttribute(LDS)");
         }
@@ -730,13 +833,16 @@ TwTables tables(hipStream_t st)
         check(hipFuncSetAttribute((const void*)fft_reg_kernel<C64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kLdsElems * 8),
               "hipFuncSetAttribute(LDS)");
+        check(hipFuncSetAttribute((const void*)fft_reg_kernel<CFF>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kLdsElems * sizeof(Words<CFF>::W)),
+              "hipFuncSetAttribute(LDS)");
         check(hipFuncSetAttribute((const void*)fft_reg_kernel<CW>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kLdsElems * sizeof(Words<CW>::W)),
               "hipFuncSetAttribute(LDS)");
 
         it = g_tw.emplace(dev, t).first;
     }
-    return TwTables{it->second.d, it->second.f};
+    return TwTables{it->second.d, it->second.f, it->second.ff};
 }
 
 
@@ -785,6 +891,9 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
     if (p.ndesc == 0) return;
     if (p.stride == 1) {
         if (p.prec == 2)
+            hipLaunchKernelGGL(fft_reg_kernel<CFF>, dim3(p.ndesc), dim3(kRT), kLdsElems * sizeof(Words<CFF>::W), st,
+                               p.descs, p.sigs, in, out, tw, forward, norm, fixed_scale);
+        else if (p.prec == 3)
             hipLaunchKernelGGL(fft_reg_kernel<CW>, dim3(p.ndesc), dim3(kRT), kLdsElems * sizeof(Words<CW>::W), st,
                                p.descs, p.sigs, in, out, tw, forward, norm, fixed_scale);
         else if (p.prec == 0)
@@ -803,6 +912,10 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
         break;
     case 1:
         hipLaunchKernelGGL(fft_kernel<C64>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
+                           forward, norm, fixed_scale);
+        break;
+    case 2:
+        hipLaunchKernelGGL(fft_kernel<CFF>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
                            forward, norm, fixed_scale);
         break;
     default:
@@ -826,7 +939,7 @@ extern "C" int aeth_fft_batch_plan(int prec, uint64_t count, const uint64_t* len
     return guarded([&] {
         if (!plan) fail("aeth_fft_batch_plan", "plan is NULL");
         *plan = nullptr;
-        if (prec < 0 || prec > 2) fail("aeth_fft_batch_plan", "unknown precision");
+        if (prec < 0 || prec > 3) fail("aeth_fft_batch_plan", "unknown precision");
         std::vector<uint32_t> l2(count);
         for (uint64_t i = 0; i < count; ++i) {
             const uint32_t l = log2_exact(lengths[i]);

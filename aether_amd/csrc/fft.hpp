@@ -24,7 +24,7 @@ struct Sig {
 
 // Descriptors and signal table resident on the device.
 struct BatchPlan {
-    int prec = 0; // 0 complex128, 1 complex64, 2 complex64 in memory with FP64 arithmetic
+    int prec = 0; // 0 complex128, 1 complex64, 2 complex64 with float-float arithmetic, 3 complex64 in memory with FP64 arithmetic
     int dev = 0;
     Desc* descs = nullptr;
     Sig* sigs = nullptr;

@@ -889,7 +889,7 @@ extern "C" int aeth_run_job_plain(const aeth_job* job, aeth_job_result* out, voi
                     pos += len[k];
                 }
                 void* plan = nullptr;
-                if (aeth_fft_batch_plan(s.dtype == AETH_F64 ? AETH_FFT_C128 : AETH_FFT_C64_WIDE, s.n, len.data(),
+                if (aeth_fft_batch_plan(s.dtype == AETH_F64 ? AETH_FFT_C128 : AETH_FFT_C64_FF, s.n, len.data(),
                                         offs.data(), offs.data(), &plan))
                     throw Failure{1};
                 const int rc = aeth_fft_batch_exec(plan, x, y, 1, AETH_FFT_BACKWARD, stream);

@@ -333,8 +333,8 @@ def wl_fft() -> dict:
     x128 = torch.complex(z.data[0, :total], z.data[1, :total]).contiguous()
     del z
     res = {"signals": 4096, "points": total}
-    for name, prec, dt in (("c128", 0, torch.complex128), ("c64_wide", 2, torch.complex64),
-                           ("c64_plain", 1, torch.complex64)):
+    for name, prec, dt in (("c128", 0, torch.complex128), ("c64_floatfloat", 2, torch.complex64),
+                           ("c64_plain", 1, torch.complex64), ("c64_fp64_inside", 3, torch.complex64)):
         x = x128.to(dt)
         y = torch.empty_like(x)
         w = torch.empty_like(x)

@@ -209,15 +209,16 @@ int aeth_math(int fn, int dtype, const void* x, void* y, uint64_t n, void* strea
  * dsc/src/cufft_plan.h:122-320) on the device. Data are interleaved
  * (re, im) pairs; signal i occupies `lengths[i]` consecutive complex elements
  * starting at complex offset in_offsets[i] (input) / out_offsets[i] (output).
- * prec: C128 (double pairs), C64 (float pairs), C64_WIDE (float pairs in
- * memory, FP64 arithmetic inside: only the final rounding to float remains;
- * C64_FF is its former name). norm follows numpy / fft::Norm
+ * prec: C128 (double pairs), C64 (float pairs), C64_FF (float pairs in
+ * memory, float-float arithmetic inside: the compensated FP32 mode),
+ * C64_WIDE (float pairs in memory, FP64 arithmetic inside). norm follows
+ * numpy / fft::Norm
  * (fft/Fft.h:58,73-76): BACKWARD scales the inverse by 1/N, FORWARD the
  * forward by 1/N, ORTHO both by 1/sqrt(N). Plan creation allocates and
  * copies (like the seam's); exec allocates nothing and may be captured.
  * In-place (in == out with equal offsets) is allowed. Lengths that are not
  * powers of two, or above 8192, return 2. */
-enum { AETH_FFT_C128 = 0, AETH_FFT_C64 = 1, AETH_FFT_C64_WIDE = 2, AETH_FFT_C64_FF = AETH_FFT_C64_WIDE };
+enum { AETH_FFT_C128 = 0, AETH_FFT_C64 = 1, AETH_FFT_C64_FF = 2, AETH_FFT_C64_WIDE = 3 };
 enum { AETH_FFT_BACKWARD = 0, AETH_FFT_FORWARD = 1, AETH_FFT_ORTHO = 2 };
 int aeth_fft_batch_plan(int prec, uint64_t count, const uint64_t* lengths, const uint64_t* in_offsets,
                         const uint64_t* out_offsets, void** plan);
@@ -235,7 +236,7 @@ int aeth_fft_batch_destroy(void* plan);
  *           comp) of their sum and of their sum of squares.
  *   FFT   : n (1..64) signals of lengths 2^fft_log2n[k], interleaved entries
  *           standardNormal<T>(seed, j, 0); forward transform (complex128 for
- *           F64, complex64 with FP64 arithmetic inside for F32); result: (hi,
+ *           F64, complex64 with float-float arithmetic for F32); result: (hi,
  *           comp) of the sum of every output real and imaginary part.
  * Each job's result is bit for bit the result of aeth_run_job_plain on the
  * same job: batching never changes a value. Arrays live in a device pool of

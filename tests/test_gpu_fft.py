@@ -2,10 +2,10 @@
 
 Bound (the reference's tests/test_Fft_common.h:79-83):
     max |out - DFT(x) * scale| <= (8 + 4 log2 N) * eps * ||x||_2 * scale
-with eps of the storage type. The wide complex64 mode (complex64 in memory,
-FP64 arithmetic inside) must land far inside the complex64 bound (stated:
-within 4 float ulps of |X|max, i.e. the output rounding), and complex64 plain
-inside its own bound.
+with eps of the storage type. The compensated complex64 modes (complex64 in
+memory; float-float or FP64 arithmetic inside) must land far inside the
+complex64 bound (stated: within 4 float ulps of |X|max, i.e. the output
+rounding), and complex64 plain inside its own bound.
 """
 import numpy as np
 import pytest
@@ -92,16 +92,17 @@ def test_round_trip_c128(dev):
 
 
 @pytest.mark.parametrize("log2n", [6, 10, 13])
-def test_complex64_plain_and_wide(dev, orc, log2n):
+@pytest.mark.parametrize("prec", [2, 3], ids=["floatfloat", "wide"])
+def test_complex64_plain_and_compensated(dev, orc, log2n, prec):
     n = 1 << log2n
     x = signals([n, n], 11).astype(np.complex64)
     ref0 = orc.dft(x[:n].astype(np.complex128))
     y_plain = run([n, n], x, 1)
-    y_ff = run([n, n], x, 2)
+    y_ff = run([n, n], x, prec)
     err_plain = np.max(np.abs(y_plain[:n] - ref0))
     err_ff = np.max(np.abs(y_ff[:n] - ref0))
     assert err_plain <= bound(n, x[:n], 2.0**-23)
-    # wide: only the final rounding to complex64 remains
+    # compensated: only the final rounding to complex64 remains
     assert err_ff <= 4 * 2.0**-24 * np.max(np.abs(ref0))
     assert err_ff < err_plain
 
