@@ -329,9 +329,9 @@ __global__ __launch_bounds__(kThreads) void fft_kernel(const Desc* __restrict__ 
 // of 2^L points is L/4 Stockham passes of radix 16, then one of radix 2, 4 or 8 when 4 does not divide L;
 // each pass is a DFT in registers (radix 16 as 4 x 4, radix 8 as 2 x 4, constant twiddles inside). The
 // first pass reads its inputs straight from HBM and the last writes its outputs straight to HBM, both
-// coalesced (consecutive lanes take consecutive butterflies), so a point crosses LDS only between passes:
-// one 8-byte word at a time (real parts, then imaginary parts of complex128; complex64 as one word),
-// through pad(8192) words. The twiddles of a pass, W^(r m) for r < R, are products of the table entries
+// coalesced (consecutive lanes take consecutive butterflies), so a point crosses LDS only between passes,
+// as one word (16 bytes for complex128 and the compensated types, 8 for complex64) at a swizzled place
+// (xpos). The twiddles of a pass, W^(r m) for r < R, are products of the table entries
 // W^m, W^2m, W^4m and W^8m (at most four factors), formed where they are used: a radix-16 butterfly loads
 // four entries instead of fifteen. Complex products use fused multiply-adds.
 constexpr int kRT = 512; // threads
@@ -616,12 +616,24 @@ __device__ __forceinline__ void rstore(const C (&v)[kRP], const Sig* __restrict_
     }
 }
 
+// LDS place of exchange element e: e with its low bits XORed by the bits above them, so that the writes of
+// the first pass (stride 16 across lanes) spread over all banks while consecutive elements (every read,
+// and the writes of the later passes) stay a permutation inside aligned groups. Measured with ds_*_b128
+// on this GPU: a stride of 16 words conflicts on every access and a plain padding (e + e/16) conflicts on
+// reads of consecutive words, while this placement conflicts on neither. 8-byte words swizzle four bits.
+template<class W>
+__device__ __forceinline__ uint32_t xpos(uint32_t e)
+{
+    return e ^ ((e >> 4) & (sizeof(W) == 16 ? 7u : 15u));
+}
+
 // Pass (R, Ns) outputs to LDS at their Stockham places, then the next pass's (R2) inputs back, one word
 // of every point at a time.
 template<int R, int R2, class C>
 __device__ __forceinline__ void rexchange(C (&v)[kRP], typename Words<C>::W* lds, uint32_t log2n, uint32_t P,
                                           uint32_t Ns)
 {
+    using Word = typename Words<C>::W;
     constexpr int U = kRP / R, U2 = kRP / R2;
     const uint32_t N = 1u << log2n;
     const uint32_t nr = N / R, nr2 = N / R2, nb = P / R, nb2 = P / R2;
@@ -635,7 +647,7 @@ __device__ __forceinline__ void rexchange(C (&v)[kRP], typename Words<C>::W* lds
                 const uint32_t k = j & (Ns - 1u);
                 const uint32_t idx = s * N + (j - k) * R + k;
 #pragma unroll
-                for (int r = 0; r < R; ++r) lds[pad(idx + r * Ns)] = Words<C>::get(v[u * R + r], w);
+                for (int r = 0; r < R; ++r) lds[xpos<Word>(idx + r * Ns)] = Words<C>::get(v[u * R + r], w);
             }
         }
         __syncthreads();
@@ -646,7 +658,7 @@ __device__ __forceinline__ void rexchange(C (&v)[kRP], typename Words<C>::W* lds
                 const uint32_t s = b / nr2, j = b - s * nr2;
                 const uint32_t base = s * N + j;
 #pragma unroll
-                for (int r = 0; r < R2; ++r) Words<C>::put(v[u * R2 + r], w, lds[pad(base + r * nr2)]);
+                for (int r = 0; r < R2; ++r) Words<C>::put(v[u * R2 + r], w, lds[xpos<Word>(base + r * nr2)]);
             }
         }
         __syncthreads();
@@ -828,16 +840,16 @@ TwTables tables(hipStream_t st)
                   "hipFuncSetAttribute(LDS)");
         }
         check(hipFuncSetAttribute((const void*)fft_reg_kernel<C128>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kLdsElems * sizeof(Words<C128>::W)),
+                                  kMaxN * sizeof(Words<C128>::W)),
               "hipFuncSetAttribute(LDS)");
         check(hipFuncSetAttribute((const void*)fft_reg_kernel<C64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kLdsElems * 8),
+                                  kMaxN * 8),
               "hipFuncSetAttribute(LDS)");
         check(hipFuncSetAttribute((const void*)fft_reg_kernel<CFF>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kLdsElems * sizeof(Words<CFF>::W)),
+                                  kMaxN * sizeof(Words<CFF>::W)),
               "hipFuncSetAttribute(LDS)");
         check(hipFuncSetAttribute((const void*)fft_reg_kernel<CW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kLdsElems * sizeof(Words<CW>::W)),
+                                  kMaxN * sizeof(Words<CW>::W)),
               "hipFuncSetAttribute(LDS)");
 
         it = g_tw.emplace(dev, t).first;
@@ -891,16 +903,16 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
     if (p.ndesc == 0) return;
     if (p.stride == 1) {
         if (p.prec == 2)
-            hipLaunchKernelGGL(fft_reg_kernel<CFF>, dim3(p.ndesc), dim3(kRT), kLdsElems * sizeof(Words<CFF>::W), st,
+            hipLaunchKernelGGL(fft_reg_kernel<CFF>, dim3(p.ndesc), dim3(kRT), kMaxN * sizeof(Words<CFF>::W), st,
                                p.descs, p.sigs, in, out, tw, forward, norm, fixed_scale);
         else if (p.prec == 3)
-            hipLaunchKernelGGL(fft_reg_kernel<CW>, dim3(p.ndesc), dim3(kRT), kLdsElems * sizeof(Words<CW>::W), st,
+            hipLaunchKernelGGL(fft_reg_kernel<CW>, dim3(p.ndesc), dim3(kRT), kMaxN * sizeof(Words<CW>::W), st,
                                p.descs, p.sigs, in, out, tw, forward, norm, fixed_scale);
         else if (p.prec == 0)
-            hipLaunchKernelGGL(fft_reg_kernel<C128>, dim3(p.ndesc), dim3(kRT), kLdsElems * sizeof(Words<C128>::W), st, p.descs, p.sigs, in,
+            hipLaunchKernelGGL(fft_reg_kernel<C128>, dim3(p.ndesc), dim3(kRT), kMaxN * sizeof(Words<C128>::W), st, p.descs, p.sigs, in,
                                out, tw, forward, norm, fixed_scale);
         else
-            hipLaunchKernelGGL(fft_reg_kernel<C64>, dim3(p.ndesc), dim3(kRT), kLdsElems * 8, st, p.descs, p.sigs, in,
+            hipLaunchKernelGGL(fft_reg_kernel<C64>, dim3(p.ndesc), dim3(kRT), kMaxN * 8, st, p.descs, p.sigs, in,
                                out, tw, forward, norm, fixed_scale);
         check(hipGetLastError(), "fft_reg_kernel launch");
         return;
