@@ -463,8 +463,19 @@ struct JitEntry {
     hipModule_t module = nullptr;
     hipFunction_t fn = nullptr;
 };
+
+// The launch shape of a runtime-compiled tape kernel: samples per workgroup tile and bytes per vector
+// access. Assignments with sums keep the contract shape (2048-sample tiles of 16-byte accesses: the
+// reduction order is part of the numerics); assignments without sums take the shape the autotuner picked
+// for their tape structure on this device (aeth_tape_tune), or the contract shape.
+struct JitShape {
+    uint32_t tile = 2048;
+    uint32_t vb = 16;
+};
+constexpr JitShape kShapes[] = {{1024, 16}, {2048, 16}, {4096, 16}, {2048, 8}, {4096, 8}};
 std::mutex g_jit_mu;
 std::map<std::string, JitEntry> g_jit;
+std::map<std::string, JitShape> g_tuned; // base tape key -> tuned shape (guarded by g_jit_mu)
 std::atomic<uint64_t> g_jit_compiled{0}, g_jit_hits{0}, g_jit_fallbacks{0};
 
 // Programs that differ only in which planes they read and write, and in which random leaves they
@@ -484,6 +495,12 @@ std::string structure_key(const aeth_program& P)
     return k;
 }
 
+std::string tape_key(int dtype, int mode, int np, const aeth_program* P, bool nt, int dev);
+std::string tape_key(int dtype, int mode, int np, const aeth_program* P, bool nt, int dev, JitShape sh)
+{
+    return tape_key(dtype, mode, np, P, nt, dev) + ":" + std::to_string(sh.tile) + "x" + std::to_string(sh.vb);
+}
+
 std::string tape_key(int dtype, int mode, int np, const aeth_program* P, bool nt, int dev)
 {
     std::string k = std::to_string(dev) + ":" + std::to_string(dtype) + ":" + std::to_string(mode) + ":" +
@@ -493,8 +510,11 @@ std::string tape_key(int dtype, int mode, int np, const aeth_program* P, bool nt
     return k;
 }
 
-std::string gen_program(const aeth_program& P, bool dbl)
+std::string gen_program(const aeth_program& P, bool dbl, uint32_t vb)
 {
+    // 16-byte accesses go through the same vector load/store as the ahead-of-time kernels; narrower ones
+    // are element loops the compiler merges
+    const bool v16 = vb == 16;
     const char* T = dbl ? "double" : "float";
     std::string b;
     b += "      " + std::string(T) + "* outp = (" + T + "*)A.plane[pi][0];\n";
@@ -511,8 +531,10 @@ std::string gen_program(const aeth_program& P, bool dbl)
         std::string t;
         t += std::string(ind) + "if (full) {\n";
         for (int q = 0; q < P.n_loads; ++q)
-            t += std::string(ind) + "  aeth::units::vload<NT>(L" + std::to_string(q) + " + i0, x" +
-                 std::to_string(q) + "[" + slot + "]);\n";
+            t += v16 ? std::string(ind) + "  aeth::units::vload<NT>(L" + std::to_string(q) + " + i0, x" +
+                           std::to_string(q) + "[" + slot + "]);\n"
+                     : std::string(ind) + "  for (int k = 0; k < VN; ++k) x" + std::to_string(q) + "[" + slot +
+                           "][k] = L" + std::to_string(q) + "[i0 + k];\n";
         t += std::string(ind) + "} else {\n";
         for (int q = 0; q < P.n_loads; ++q)
             t += std::string(ind) + "  for (int k = 0; k < VN; ++k) x" + std::to_string(q) + "[" + slot +
@@ -589,14 +611,15 @@ std::string gen_program(const aeth_program& P, bool dbl)
         }
     }
     b += "          v[k] = " + st.back() + ";\n        }\n";
-    b += "        if (full) {\n          aeth::units::vstore<NT>(outp + i0, v);\n";
+    b += v16 ? "        if (full) {\n          aeth::units::vstore<NT>(outp + i0, v);\n"
+             : "        if (full) {\n          for (int k = 0; k < VN; ++k) outp[i0 + k] = v[k];\n";
     b += "          if (SUM) for (int k = 0; k < VN; ++k) aeth::acc::accumulate<M, false>(acc, v[k]);\n";
     b += "        } else {\n          for (int k = 0; k < VN; ++k) if (i0 + k < n) { outp[i0 + k] = v[k]; "
          "if (SUM) aeth::acc::accumulate<M, false>(acc, v[k]); }\n        }\n      }\n";
     return b;
 }
 
-std::string gen_kernel(int dtype, int mode, int np, const aeth_program* P, bool nt)
+std::string gen_kernel(int dtype, int mode, int np, const aeth_program* P, bool nt, JitShape sh)
 {
     const bool dbl = dtype == AETH_F64;
     const int M = mode == AETH_SUM_NONE ? 0 : mode;
@@ -631,12 +654,13 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
     s += std::string("  constexpr bool NT = ") + (nt ? "true" : "false") + ";\n";
     s += "  constexpr int M = " + std::to_string(M) + ";\n";
     s += std::string("  constexpr bool SUM = ") + (mode == AETH_SUM_NONE ? "false" : "true") + ";\n";
-    s += "  constexpr int VN = 16 / sizeof(T);\n  constexpr int ROUNDS = 2048 / (256 * VN);\n";
+    s += "  constexpr uint32_t TILE = " + std::to_string(sh.tile) + ";\n";
+    s += "  constexpr int VN = " + std::to_string(sh.vb) + " / sizeof(T);\n  constexpr int ROUNDS = TILE / (256 * VN);\n";
     s += "  __shared__ aeth::acc::Pair<T> lds[4];\n";
     s += "  aeth::acc::Pair<T>* part = (aeth::acc::Pair<T>*)part_;\n  const uint32_t n = A.n;\n";
     s += "  for (uint32_t u = blockIdx.x; u < A.units; u += gridDim.x) {\n";
     s += "    const uint32_t pi = u / A.tiles;\n    const uint32_t tile = u - pi * A.tiles;\n";
-    s += "    const uint32_t s0 = tile * 2048;\n    const bool full = vec && s0 + 2048 <= n;\n";
+    s += "    const uint32_t s0 = tile * TILE;\n    const bool full = vec && s0 + TILE <= n;\n";
     s += "    aeth::acc::Pair<T> acc{T(0), T(0)};\n    switch (pi) {\n";
     std::vector<bool> done(np, false);
     for (int p = 0; p < np; ++p) {
@@ -647,7 +671,7 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
                 s += "    case " + std::to_string(q) + ":\n";
                 done[q] = true;
             }
-        s += "    {\n" + gen_program(P[p], dbl) + "      break;\n    }\n";
+        s += "    {\n" + gen_program(P[p], dbl, sh.vb) + "      break;\n    }\n";
     }
     s += "    default: break;\n    }\n";
     s += "    if (SUM) {\n      const aeth::acc::Pair<T> rr = aeth::acc::block_combine<M>(acc, lds);\n"
@@ -657,9 +681,9 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
 
 // Compile a tape's kernel for `arch_opt` ("--offload-arch=gfx950"); false (log in last error) on failure.
 bool jit_compile(int dtype, int mode, int np, const aeth_program* P, bool nt, const std::string& arch_opt,
-                 std::vector<char>& code)
+                 std::vector<char>& code, JitShape sh = JitShape{})
 {
-    const std::string src = gen_kernel(dtype, mode, np, P, nt);
+    const std::string src = gen_kernel(dtype, mode, np, P, nt, sh);
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "aeth_tape_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
         set_error("hiprtcCreateProgram failed");
@@ -686,13 +710,13 @@ bool jit_compile(int dtype, int mode, int np, const aeth_program* P, bool nt, co
 }
 
 // Returns nullptr (and records why) when the runtime compiler is unavailable.
-hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, bool nt)
+hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, bool nt, JitShape sh)
 {
     static const bool disabled = std::getenv("AETHER_TAPE_INTERPRET") != nullptr;
     if (disabled) return nullptr;
     int dev = 0;
     check(hipGetDevice(&dev), "hipGetDevice");
-    const std::string key = tape_key(dtype, mode, np, P, nt, dev);
+    const std::string key = tape_key(dtype, mode, np, P, nt, dev, sh);
     std::lock_guard<std::mutex> lock(g_jit_mu);
     auto it = g_jit.find(key);
     if (it != g_jit.end()) {
@@ -705,7 +729,7 @@ hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, b
     const size_t colon = arch.find(':');
     if (colon != std::string::npos) arch.resize(colon);
     std::vector<char> code;
-    if (!jit_compile(dtype, mode, np, P, nt, arch, code)) {
+    if (!jit_compile(dtype, mode, np, P, nt, arch, code, sh)) {
         ++g_jit_fallbacks;
         return nullptr;
     }
@@ -721,12 +745,17 @@ hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, b
 
 using namespace aeth;
 
-extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_leaves,
-                              void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts,
-                              int n_rngs, const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums,
-                              void* stream)
+namespace {
+
+// One tape assignment. `force` (the tuner) runs every launch with that shape; otherwise assignments
+// without sums use the tuned shape of their structure and assignments with sums the contract shape.
+// Returns whether the assignment could run as one launch over all programs (no program reads a plane
+// another one writes).
+bool eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_leaves, void* const* leaves,
+               const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs, const aeth_rng_leaf* rngs,
+               uint64_t n, int sum_mode, void* sums, void* stream, const JitShape* force)
 {
-    return guarded([&] {
+    {
         if (dtype != AETH_F64 && dtype != AETH_F32) fail("aeth_eval_tape", "unknown dtype");
         if (n_programs < 1 || n_programs > AETH_TAPE_MAX_PROGRAMS) fail("aeth_eval_tape", "1..8 programs");
         if (n_leaves < 1 || n_leaves > AETH_TAPE_MAX_LEAVES) fail("aeth_eval_tape", "1..16 leaves");
@@ -779,10 +808,24 @@ extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* pro
             for (int c = 0; c < n_consts; ++c) A.consts[c] = consts[c];
             for (int r = 0; r < n_rngs; ++r) A.rng[r] = rngs[r];
             A.n = (uint32_t)n;
-            A.tiles = n ? (uint32_t)((n + kTile - 1) / kTile) : 1u;
-            A.units = A.tiles * (uint32_t)np;
             const bool nt = span_bytes >= (512ull << 20);
-            hipFunction_t jit = jit_function(dtype, sum_mode, np, P, nt);
+            JitShape sh{};
+            if (force) {
+                sh = *force;
+            } else if (sum_mode == AETH_SUM_NONE) {
+                int dev = 0;
+                check(hipGetDevice(&dev), "hipGetDevice");
+                std::lock_guard<std::mutex> lock(g_jit_mu);
+                const auto it = g_tuned.find(tape_key(dtype, sum_mode, np, P, nt, dev));
+                if (it != g_tuned.end()) sh = it->second;
+            }
+            hipFunction_t jit = jit_function(dtype, sum_mode, np, P, nt, sh);
+            if (!jit) {
+                if (force) fail("aeth_tape_tune", "the runtime compiler is unavailable");
+                sh = JitShape{}; // the resident interpreter runs the contract shape
+            }
+            A.tiles = n ? (uint32_t)((n + sh.tile - 1) / sh.tile) : 1u;
+            A.units = A.tiles * (uint32_t)np;
             if (dtype == AETH_F64)
                 dispatch<double>(A, max_loads, np, vec, nt, sum_mode, static_cast<double*>(out_sums), st, jit);
             else
@@ -793,6 +836,97 @@ extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* pro
         } else {
             for (int p = 0; p < n_programs; ++p)
                 run(&programs[p], 1, sums ? static_cast<char*>(sums) + 2 * es * (size_t)p : nullptr);
+        }
+        return !serial;
+    }
+}
+
+} // namespace
+
+extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_leaves,
+                              void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts,
+                              int n_rngs, const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums,
+                              void* stream)
+{
+    return guarded([&] {
+        eval_tape(dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs, rngs, n,
+                  sum_mode, sums, stream, nullptr);
+    });
+}
+
+extern "C" int aeth_tape_tune(int dtype, int n_programs, const aeth_program* programs, int n_leaves,
+                              void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts,
+                              int n_rngs, const aeth_rng_leaf* rngs, uint64_t n, void* stream, aeth_tape_shape* chosen)
+{
+    return guarded([&] {
+        hipStream_t st = as_stream(stream);
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+            fail("aeth_tape_tune", "tuning is not allowed inside a stream capture");
+        // every candidate writes the same values (the assignment has no sums), so the destination ends
+        // holding the assignment's result; an assignment that reads any plane it writes (a = 2a + b)
+        // would not, and is refused
+        if (n_programs < 1 || n_programs > AETH_TAPE_MAX_PROGRAMS) fail("aeth_tape_tune", "1..8 programs");
+        if (n_leaves < 1 || n_leaves > AETH_TAPE_MAX_LEAVES) fail("aeth_tape_tune", "1..16 leaves");
+        const size_t es = dtype == AETH_F64 ? 8 : 4;
+        for (int p = 0; p < n_programs; ++p) {
+            validate(programs[p], n_leaves, n_consts, n_rngs);
+            for (int q = 0; q < n_programs; ++q) {
+                const auto& Q = programs[q];
+                const char* out = static_cast<const char*>(leaves[Q.out_leaf]) + (uint64_t)Q.out_comp * leaf_pitch[Q.out_leaf] * es;
+                for (int k = 0; k < programs[p].n_loads; ++k) {
+                    const auto& P = programs[p];
+                    const char* in = static_cast<const char*>(leaves[P.load_leaf[k]]) +
+                                     (uint64_t)P.load_comp[k] * leaf_pitch[P.load_leaf[k]] * es;
+                    if (in == out) fail("aeth_tape_tune", "the destination is also read: tune an assignment into another array");
+                }
+            }
+        }
+        hipEvent_t e0, e1;
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        JitShape best{};
+        float best_ms = -1.0f;
+        try {
+            for (const JitShape& sh : kShapes) {
+                auto once = [&] {
+                    eval_tape(dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs,
+                              rngs, n, AETH_SUM_NONE, nullptr, stream, &sh);
+                };
+                once(); // compiles and warms the variant
+                constexpr int kReps = 3;
+                check(hipEventRecord(e0, st), "hipEventRecord");
+                for (int r = 0; r < kReps; ++r) once();
+                check(hipEventRecord(e1, st), "hipEventRecord");
+                check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.0f;
+                check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                ms /= kReps;
+                if (best_ms < 0.0f || ms < best_ms) {
+                    best_ms = ms;
+                    best = sh;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        uint64_t span_bytes = 0;
+        for (int p = 0; p < n_programs; ++p) span_bytes += (uint64_t)(programs[p].n_loads + 1) * n * es;
+        const bool nt = span_bytes >= (512ull << 20);
+        {
+            std::lock_guard<std::mutex> lock(g_jit_mu);
+            g_tuned[tape_key(dtype, AETH_SUM_NONE, n_programs, programs, nt, dev)] = best;
+        }
+        if (chosen) {
+            chosen->tile = best.tile;
+            chosen->vector_bytes = best.vb;
+            chosen->ms = best_ms;
         }
     });
 }

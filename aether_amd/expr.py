@@ -431,8 +431,11 @@ def lower(dest: Array, expr: Expr):
     return progs, leaves, consts, rngs
 
 
-def assign(dest: Array, expr, sum_mode: int = -1, sums: torch.Tensor = None, stream=None):
-    """`dest = expr` over all samples as ONE kernel; returns (ncomp, 2) sums when sum_mode >= 0."""
+class aeth_tape_shape(C.Structure):
+    _fields_ = [("tile", C.c_uint32), ("vector_bytes", C.c_uint32), ("ms", C.c_float)]
+
+
+def _abi_args(dest: Array, expr):
     progs, leaves, consts, rngs = lower(dest, _wrap(expr))
     for lf in leaves:
         if lf.n != dest.n or lf.dtype != dest.dtype:
@@ -443,8 +446,25 @@ def assign(dest: Array, expr, sum_mode: int = -1, sums: torch.Tensor = None, str
     rg = (aeth_rng_leaf * max(1, len(rngs)))()
     for k, (seed, p0, p1, comp, dist) in enumerate(rngs):
         rg[k].seed, rg[k].p0, rg[k].p1, rg[k].comp, rg[k].dist = seed, p0, p1, comp, dist
+    return progs, len(leaves), ptrs, pitch, len(consts), cst, len(rngs), rg
+
+
+def tune(dest: Array, expr, stream=None) -> dict:
+    """Autotune the launch shape of `dest = expr` (aeth_tape_tune): every candidate shape (samples per tile,
+    bytes per vector access) runs on these arrays, the fastest is kept for this expression's structure
+    (assignments without sums), and `dest` holds the result. Returns {"tile", "vector_bytes", "ms"}."""
+    progs, nl, ptrs, pitch, nc, cst, nr, rg = _abi_args(dest, expr)
+    out = aeth_tape_shape()
+    call("aeth_tape_tune", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg, dest.n,
+         stream_handle(stream), C.byref(out))
+    return {"tile": out.tile, "vector_bytes": out.vector_bytes, "ms": out.ms}
+
+
+def assign(dest: Array, expr, sum_mode: int = -1, sums: torch.Tensor = None, stream=None):
+    """`dest = expr` over all samples as ONE kernel; returns (ncomp, 2) sums when sum_mode >= 0."""
+    progs, nl, ptrs, pitch, nc, cst, nr, rg = _abi_args(dest, expr)
     if sum_mode >= 0 and sums is None:
         sums = torch.empty((dest.ncomp, 2), dtype=dest.dtype, device=dest.data.device)
-    call("aeth_eval_tape", _dtype_code(dest.dtype), dest.ncomp, progs, len(leaves), ptrs, pitch, len(consts), cst,
-         len(rngs), rg, dest.n, sum_mode, sums.data_ptr() if sums is not None else None, stream_handle(stream))
+    call("aeth_eval_tape", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg, dest.n,
+         sum_mode, sums.data_ptr() if sums is not None else None, stream_handle(stream))
     return sums

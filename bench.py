@@ -147,7 +147,14 @@ def wl_tape(a, b, c, d, n) -> dict:
         E.assign(d, x, ae.SUM_NEUMAIER, sums)
 
     ms = _time(run)
-    return {"ms": round(ms, 4), "GB/s": _gbs(3 * n * 32, ms), "frac": round(3 * n * 32 / (ms * 1e-3) / 1e9 / 8000, 4)}
+    out = {"ms": round(ms, 4), "GB/s": _gbs(3 * n * 32, ms), "frac": round(3 * n * 32 / (ms * 1e-3) / 1e9 / 8000, 4)}
+    # the same assignment without sums, at the contract shape and at the shape the autotuner keeps
+    ms0 = _time(lambda: E.assign(d, x))
+    shape = E.tune(d, x)
+    ms1 = _time(lambda: E.assign(d, x))
+    out["no_sums"] = {"contract_shape_ms": round(ms0, 4), "tuned_shape": [shape["tile"], shape["vector_bytes"]],
+                      "tuned_ms": round(ms1, 4), "tuned_GB/s": _gbs(3 * n * 32, ms1)}
+    return out
 
 
 def wl_vec3(a, b, c, n) -> dict:

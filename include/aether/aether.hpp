@@ -921,6 +921,21 @@ public:
         detail::check(rc, "aeth_eval_tape");
     }
 
+    /** Autotune the launch shape of dest = e (aeth_tape_tune); this view holds the result afterwards. */
+    template<class E>
+    aeth_tape_shape tune(const E& e, Stream stream = nullptr)
+    {
+        static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
+        if (!device_) throw Error("aether: tuning needs a device view");
+        const detail::TapeCtx x = detail::lower<T, N>(p_, pitch_, e);
+        aeth_tape_shape chosen{};
+        detail::check(aeth_tape_tune(x.dtype, (int)N, x.programs.data(), (int)x.leaves.size(),
+                                     const_cast<void* const*>(x.leaves.data()), x.pitch.data(), (int)x.consts.size(),
+                                     x.consts.data(), (int)x.rngs.size(), x.rngs.data(), samples_, stream, &chosen),
+                      "aeth_tape_tune");
+        return chosen;
+    }
+
 private:
     T* p_;
     std::size_t samples_, pitch_;
@@ -1070,6 +1085,14 @@ std::array<double, N> evaluate(View<T, N> dest, const E& e, SumMode mode = SumMo
     std::array<double, N> sums{};
     dest.assign(e, mode, sums.data(), s);
     return sums;
+}
+
+/** Pick the fastest launch shape for this expression's structure on this device (no counterpart in the
+ *  reference); later assignments of the same structure without sums use it. dest holds the result. */
+template<class T, std::size_t N, class E>
+aeth_tape_shape tune(View<T, N> dest, const E& e, Stream s = nullptr)
+{
+    return dest.tune(e, s);
 }
 
 // ---------------------------------------------------------------- accumulation

@@ -184,6 +184,24 @@ int aeth_tape_jit_stats(uint64_t* compiled, uint64_t* cache_hits, uint64_t* fall
 int aeth_tape_jit_build(int dtype, int sum_mode, int n_programs, const aeth_program* programs, const char* arch,
                         uint64_t* code_bytes);
 
+/* Launch-shape autotuner (no counterpart in the reference, whose kernels take a caller-chosen grid:
+ * expr/Assign.h:124-160). Times the runtime-compiled kernel of this assignment (the arguments of
+ * aeth_eval_tape, without sums) at each candidate shape -- samples per workgroup tile 1024, 2048 or 4096,
+ * 16- or 8-byte vector accesses -- on `stream`, on the caller's arrays, and keeps the fastest for this
+ * tape structure on this device for the rest of the process: later aeth_eval_tape calls of the same
+ * structure without sums launch with it (calls with sums keep the 2048-sample, 16-byte shape, whose
+ * reduction order the sums are defined by). The destination holds the assignment's result afterwards.
+ * Refused inside a stream capture and for an assignment whose destination is also one of its inputs.
+ * Synchronises `stream`. chosen (may be NULL): the shape picked and its average time per launch. */
+typedef struct {
+    uint32_t tile;
+    uint32_t vector_bytes;
+    float ms;
+} aeth_tape_shape;
+int aeth_tape_tune(int dtype, int n_programs, const aeth_program* programs, int n_leaves, void* const* leaves,
+                   const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs,
+                   const aeth_rng_leaf* rngs, uint64_t n, void* stream, aeth_tape_shape* chosen);
+
 /* e = cross(a, b).dot(c) + sqrt(a.dot(a) + 1) over Array<double,3> a, b, c
  * (pitch `pitch`) into a scalar plane e (expr/nodes/Geometric.h:32-73,
  * expr/Expression.h:162-173, expr/Reduce.h:150-190, math/math.h:132). */
