@@ -471,8 +471,10 @@ struct JitEntry {
 struct JitShape {
     uint32_t tile = 2048;
     uint32_t vb = 16;
+    uint32_t waves = 0; // minimum waves per SIMD asked of the register allocator (0: the compiler's choice)
 };
-constexpr JitShape kShapes[] = {{1024, 16}, {2048, 16}, {4096, 16}, {2048, 8}, {4096, 8}};
+constexpr JitShape kShapes[] = {{1024, 16, 0}, {2048, 16, 0}, {4096, 16, 0}, {2048, 8, 0}, {4096, 8, 0},
+                                {2048, 16, 8}};
 std::mutex g_jit_mu;
 std::map<std::string, JitEntry> g_jit;
 std::map<std::string, JitShape> g_tuned; // base tape key -> tuned shape (guarded by g_jit_mu)
@@ -498,7 +500,8 @@ std::string structure_key(const aeth_program& P)
 std::string tape_key(int dtype, int mode, int np, const aeth_program* P, bool nt, int dev);
 std::string tape_key(int dtype, int mode, int np, const aeth_program* P, bool nt, int dev, JitShape sh)
 {
-    return tape_key(dtype, mode, np, P, nt, dev) + ":" + std::to_string(sh.tile) + "x" + std::to_string(sh.vb);
+    return tape_key(dtype, mode, np, P, nt, dev) + ":" + std::to_string(sh.tile) + "x" + std::to_string(sh.vb) + "w" +
+           std::to_string(sh.waves);
 }
 
 std::string tape_key(int dtype, int mode, int np, const aeth_program* P, bool nt, int dev)
@@ -646,8 +649,9 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
     }
 }
 )K";
-    s += "extern \"C\" __global__ __launch_bounds__(256) void aeth_tape_jit(const JitArgs A, void* part_, int vec)\n"
-         "{\n";
+    s += std::string("extern \"C\" __global__ __launch_bounds__(256") +
+         (sh.waves ? ", " + std::to_string(sh.waves) : std::string()) +
+         ") void aeth_tape_jit(const JitArgs A, void* part_, int vec)\n{\n";
     s += std::string("  typedef ") + (dbl ? "double" : "float") + " T;\n";
     // nontemporal is compiled in: a run-time choice between two otherwise identical loads is merged
     // by the optimiser into one plain load
@@ -926,6 +930,7 @@ extern "C" int aeth_tape_tune(int dtype, int n_programs, const aeth_program* pro
         if (chosen) {
             chosen->tile = best.tile;
             chosen->vector_bytes = best.vb;
+            chosen->waves_per_simd = best.waves;
             chosen->ms = best_ms;
         }
     });

@@ -432,7 +432,7 @@ def lower(dest: Array, expr: Expr):
 
 
 class aeth_tape_shape(C.Structure):
-    _fields_ = [("tile", C.c_uint32), ("vector_bytes", C.c_uint32), ("ms", C.c_float)]
+    _fields_ = [("tile", C.c_uint32), ("vector_bytes", C.c_uint32), ("waves_per_simd", C.c_uint32), ("ms", C.c_float)]
 
 
 def _abi_args(dest: Array, expr):
@@ -451,13 +451,14 @@ def _abi_args(dest: Array, expr):
 
 def tune(dest: Array, expr, stream=None) -> dict:
     """Autotune the launch shape of `dest = expr` (aeth_tape_tune): every candidate shape (samples per tile,
-    bytes per vector access) runs on these arrays, the fastest is kept for this expression's structure
-    (assignments without sums), and `dest` holds the result. Returns {"tile", "vector_bytes", "ms"}."""
+    bytes per vector access, minimum waves per SIMD) runs on these arrays, the fastest is kept for this expression's structure
+    (assignments without sums), and `dest` holds the result. Returns {"tile", "vector_bytes",
+    "waves_per_simd", "ms"}."""
     progs, nl, ptrs, pitch, nc, cst, nr, rg = _abi_args(dest, expr)
     out = aeth_tape_shape()
     call("aeth_tape_tune", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg, dest.n,
          stream_handle(stream), C.byref(out))
-    return {"tile": out.tile, "vector_bytes": out.vector_bytes, "ms": out.ms}
+    return {"tile": out.tile, "vector_bytes": out.vector_bytes, "waves_per_simd": out.waves_per_simd, "ms": out.ms}
 
 
 def assign(dest: Array, expr, sum_mode: int = -1, sums: torch.Tensor = None, stream=None):

@@ -152,7 +152,7 @@ def wl_tape(a, b, c, d, n) -> dict:
     ms0 = _time(lambda: E.assign(d, x))
     shape = E.tune(d, x)
     ms1 = _time(lambda: E.assign(d, x))
-    out["no_sums"] = {"contract_shape_ms": round(ms0, 4), "tuned_shape": [shape["tile"], shape["vector_bytes"]],
+    out["no_sums"] = {"contract_shape_ms": round(ms0, 4), "tuned_shape": [shape["tile"], shape["vector_bytes"], shape["waves_per_simd"]],
                       "tuned_ms": round(ms1, 4), "tuned_GB/s": _gbs(3 * n * 32, ms1)}
     return out
 

@@ -187,7 +187,8 @@ int aeth_tape_jit_build(int dtype, int sum_mode, int n_programs, const aeth_prog
 /* Launch-shape autotuner (no counterpart in the reference, whose kernels take a caller-chosen grid:
  * expr/Assign.h:124-160). Times the runtime-compiled kernel of this assignment (the arguments of
  * aeth_eval_tape, without sums) at each candidate shape -- samples per workgroup tile 1024, 2048 or 4096,
- * 16- or 8-byte vector accesses -- on `stream`, on the caller's arrays, and keeps the fastest for this
+ * 16- or 8-byte vector accesses, and a minimum of waves per SIMD asked of the register allocator (none, or
+ * 8) -- on `stream`, on the caller's arrays, and keeps the fastest for this
  * tape structure on this device for the rest of the process: later aeth_eval_tape calls of the same
  * structure without sums launch with it (calls with sums keep the 2048-sample, 16-byte shape, whose
  * reduction order the sums are defined by). The destination holds the assignment's result afterwards.
@@ -196,6 +197,7 @@ int aeth_tape_jit_build(int dtype, int sum_mode, int n_programs, const aeth_prog
 typedef struct {
     uint32_t tile;
     uint32_t vector_bytes;
+    uint32_t waves_per_simd; /* 0: the compiler's choice */
     float ms;
 } aeth_tape_shape;
 int aeth_tape_tune(int dtype, int n_programs, const aeth_program* programs, int n_leaves, void* const* leaves,

@@ -11,7 +11,7 @@ import aether_amd as ae
 from aether_amd import expr as E
 
 pytestmark = pytest.mark.gpu
-SHAPES = {(1024, 16), (2048, 16), (4096, 16), (2048, 8), (4096, 8)}
+SHAPES = {(1024, 16, 0), (2048, 16, 0), (4096, 16, 0), (2048, 8, 0), (4096, 8, 0), (2048, 16, 8)}
 
 
 def filled(n, dt, seeds=(42, 43, 44)):
@@ -31,7 +31,7 @@ def test_tuned_shape_gives_the_same_values(dev, dt, n):
     before, tuned, after = (ae.Array(n, 3, dt) for _ in range(3))
     E.assign(before, x)
     got = E.tune(tuned, x)
-    assert (got["tile"], got["vector_bytes"]) in SHAPES and got["ms"] > 0
+    assert (got["tile"], got["vector_bytes"], got["waves_per_simd"]) in SHAPES and got["ms"] > 0
     E.assign(after, x)
     torch.cuda.synchronize()
     assert torch.equal(before.data[:, :n], tuned.data[:, :n])
