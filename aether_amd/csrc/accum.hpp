@@ -115,22 +115,49 @@ __device__ __forceinline__ Pair<T> combine(Pair<T> a, Pair<T> b)
     }
 }
 
-template<class T>
-__device__ __forceinline__ T shfl_xor(T v, int off)
+// The 32-bit word of lane (lane ^ OFF), OFF a power of two below 64, without a trip through the LDS
+// crossbar's addressed permute: data-parallel-primitive moves inside a quad (OFF 1, 2), the swizzle's
+// XOR mode inside 32 lanes (4, 8, 16) and the half-wave swap (32). Every lane of the wave must be active.
+template<int OFF>
+__device__ __forceinline__ uint32_t xor_lane_u32(uint32_t x)
 {
-    return __shfl_xor(v, off, 64);
+    static_assert(OFF > 0 && OFF < 64 && (OFF & (OFF - 1)) == 0, "a power of two below the wave size");
+    if constexpr (OFF == 1) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false); // quad_perm [1,0,3,2]
+    } else if constexpr (OFF == 2) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false); // quad_perm [2,3,0,1]
+    } else if constexpr (OFF < 32) {
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (OFF << 10) | 0x1F); // and 0x1F, or 0, xor OFF
+    } else {
+        // after the swap the first result holds the low half in both halves, the second the high half
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return (__lane_id() & 32) ? r[0] : r[1];
+    }
 }
 
-// Butterfly over the 64 lanes of a wave: every lane ends with the combined pair.
-template<int M, class T>
+template<int OFF, class T>
+__device__ __forceinline__ T xor_lane(T v)
+{
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t b = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = xor_lane_u32<OFF>((uint32_t)b), hi = xor_lane_u32<OFF>((uint32_t)(b >> 32));
+        return __builtin_bit_cast(T, (hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, xor_lane_u32<OFF>(__builtin_bit_cast(uint32_t, v)));
+    }
+}
+
+// Butterfly over the 64 lanes of a wave (partner lane ^ 1, ^ 2, ..., ^ 32): every lane ends with the
+// combined pair.
+template<int M, class T, int OFF = 1>
 __device__ __forceinline__ Pair<T> wave_combine(Pair<T> p)
 {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        Pair<T> q{shfl_xor(p.h, off), shfl_xor(p.c, off)};
-        p = combine<M>(p, q);
+    if constexpr (OFF < 64) {
+        const Pair<T> q{xor_lane<OFF>(p.h), xor_lane<OFF>(p.c)};
+        return wave_combine<M, T, 2 * OFF>(combine<M>(p, q));
+    } else {
+        return p;
     }
-    return p;
 }
 
 // Combine the 256 lane pairs of a block (4 waves) into one pair on thread 0.

@@ -197,13 +197,45 @@ __device__ __forceinline__ bool wave_aligned(uint64_t own)
     return accum::detail::wave_full() && __all((uint32_t)(own & 63u) == __lane_id());
 }
 
-/** Plain xor butterfly over aligned groups of W lanes. */
+/** The 32-bit word of lane (lane ^ OFF) of a full wave: data-parallel-primitive moves inside a quad
+ *  (OFF 1, 2), the swizzle's XOR mode inside 32 lanes (4, 8, 16), the half-wave swap (32). */
+template<int OFF>
+__device__ __forceinline__ uint32_t xor_lane_u32(uint32_t x)
+{
+    if constexpr (OFF == 1) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (OFF == 2) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (OFF < 32) {
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (OFF << 10) | 0x1F);
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return (__lane_id() & 32) ? r[0] : r[1];
+    }
+}
+
+template<int OFF, class E>
+__device__ __forceinline__ E xor_lane(E v)
+{
+    if constexpr (sizeof(E) == 8) {
+        const uint64_t b = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = xor_lane_u32<OFF>((uint32_t)b), hi = xor_lane_u32<OFF>((uint32_t)(b >> 32));
+        return __builtin_bit_cast(E, (hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(E, xor_lane_u32<OFF>(__builtin_bit_cast(uint32_t, v)));
+    }
+}
+
+/** Plain xor butterfly over aligned groups of W lanes (offsets W/2, ..., 1) of a full wave. */
 template<int W, class E>
 __device__ __forceinline__ E group_sum(E t)
 {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) t += __shfl_xor(t, o);
-    return t;
+    if constexpr (W > 1) {
+        t += xor_lane<W / 2>(t);
+        return group_sum<W / 2>(t);
+    } else {
+        return t;
+    }
 }
 
 /** Segmented sum of a full wave's sorted runs (the run's head lane ends with its sum): lane l adds the
