@@ -835,7 +835,7 @@ TwTables tables(hipStream_t st)
         hipLaunchKernelGGL(twiddle_kernel, dim3(kMaxN / 256), dim3(256), 0, st, t.d, t.f, t.ff);
         check(hipGetLastError(), "twiddle_kernel launch");
         check(hipStreamSynchronize(st), "hipStreamSynchronize(twiddles)");
-        for (auto fn : {(const void*)fft_kernel<C128>, (const void*)fft_kernel<C64>, (const void*)fft_kernel<CFF>, (const void*)fft_kernel<CW>}) {
+        for (auto fn : {(const void*)fft_kernel<C128>, (const void*)fft_kernel<C64>}) {
             check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsElems * 16),
                   "hipFuncSetAttribute(LDS)");
         }
@@ -926,14 +926,9 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
         hipLaunchKernelGGL(fft_kernel<C64>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
                            forward, norm, fixed_scale);
         break;
-    case 2:
-        hipLaunchKernelGGL(fft_kernel<CFF>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
-                           forward, norm, fixed_scale);
-        break;
     default:
-        hipLaunchKernelGGL(fft_kernel<CW>, dim3(p.ndesc), dim3(kThreads), lds, st, p.descs, p.sigs, in, out, tw,
-                           forward, norm, fixed_scale);
-        break;
+        // strided lines come from the seam's plans, which are complex128 or complex64
+        fail("fft launch", "strided lines in a compensated precision");
     }
     check(hipGetLastError(), "fft_kernel launch");
 }
