@@ -291,13 +291,8 @@ __device__ __forceinline__ double log(double x)
     return y;
 }
 
-// ---- float routines. The argument is widened to double (exact), reduced
-// once, and evaluated with polynomials just long enough for float (error
-// below 2^-30 relative before the one rounding to float): a third of the
-// FP64 operations of the double routines.
-
-// |x| < 2^20: r = x - n P1 - n P2 (two FMAs; P1 + P2 is pi/2 to 2^-107, so
-// r is within 2^-80 absolute); above, the Payne-Hanek path, out of line.
+// the double reduction of float arguments from 2^17: below 2^20 r = x - n P1 - n P2 (two FMAs; P1 + P2
+// is pi/2 to 2^-107, so r is within 2^-80 absolute); above, the Payne-Hanek path, out of line.
 __device__ __forceinline__ int reduce_pio2_f(double x, double* r)
 {
     const double n = __builtin_rint(x * kTwoOverPi);
@@ -320,98 +315,187 @@ __device__ __forceinline__ int reduce_pio2_f(double x, double* r)
     return q;
 }
 
-// sin r on |r| <= pi/4, Taylor to r^11 (truncation < 2^-37 relative)
-__device__ __forceinline__ double ksinf(double r)
-{
-    const double z = r * r;
-    double p = -0x1.ae64567f544e4p-26;
-    p = __builtin_fma(p, z, 0x1.71de3a556c734p-19);
-    p = __builtin_fma(p, z, -0x1.a01a01a01a01ap-13);
-    p = __builtin_fma(p, z, 0x1.1111111111111p-7);
-    p = __builtin_fma(p, z, -0x1.5555555555555p-3);
-    return __builtin_fma(r * z, p, r);
+// ---- float routines in float arithmetic. Products of two floats are split exactly with an FMA and
+// float-float (unevaluated sum) steps carry the reduced argument and the leading terms, so each result is
+// one rounding of a value good to about 2^-29: over every float argument at most 0.67 ulp for sin/cos,
+// 0.68 for exp (0.78 where the result is subnormal and rounds twice) and 0.64 for log, more than 99.8%
+// correctly rounded. Double appears only in the reduction of trig arguments of magnitude 2^17 and up.
+
+__device__ __forceinline__ void f_fast2(float a, float b, float& s, float& e)
+{ // |a| >= |b| or a == 0: s + e == a + b exactly
+    const float ss = a + b;
+    e = b - (ss - a);
+    s = ss;
 }
 
-// cos r on |r| <= pi/4, Taylor to r^10 (truncation < 2^-33)
-__device__ __forceinline__ double kcosf(double r)
+__device__ __forceinline__ void f_two2(float a, float b, float& s, float& e)
+{ // any order: s + e == a + b exactly
+    const float ss = a + b;
+    const float bb = ss - a;
+    e = (a - (ss - bb)) + (b - bb);
+    s = ss;
+}
+
+// |x| < 2^17: n = rint(x 2/pi), r = x - n (C1 + C2 + C3), C1 + C2 + C3 = pi/2 to 2^-73 relative. x - n C1
+// is exact (FMA) and n C2 is split exactly, so rh + rl is within 2^-55 of x - n pi/2.
+__device__ __forceinline__ int reduce_ff(float x, float& rh, float& rl)
 {
-    const double z = r * r;
-    double p = -0x1.27e4fb7789f5cp-22;
-    p = __builtin_fma(p, z, 0x1.a01a01a01a01ap-16);
-    p = __builtin_fma(p, z, -0x1.6c16c16c16c17p-10);
-    p = __builtin_fma(p, z, 0x1.5555555555555p-5);
-    p = __builtin_fma(p, z, -0.5);
-    return __builtin_fma(z, p, 1.0);
+    const float n = __builtin_rintf(x * 0x1.45f306p-1f);
+    const float r1 = __builtin_fmaf(-n, 0x1.921fb6p+0f, x);
+    const float p = n * -0x1.777a5cp-25f;
+    const float pe = __builtin_fmaf(n, -0x1.777a5cp-25f, -p);
+    float h, e;
+    f_two2(r1, -p, h, e);
+    const float l = __builtin_fmaf(-n, -0x1.ee59dap-50f, e - pe);
+    f_fast2(h, l, rh, rl);
+    return ((int)n) & 3;
+}
+
+// 2^17 and above (and non-finite): the double reduction, its result split into two floats
+__device__ __noinline__ int reduce_big_f(float x, float& rh, float& rl)
+{
+    double r;
+    const int q = reduce_pio2_f((double)x, &r);
+    rh = (float)r;
+    rl = (float)(r - (double)rh);
+    return q;
+}
+
+__device__ __forceinline__ int reduce_f32(float x, float& rh, float& rl)
+{
+    if (__builtin_fabsf(x) < 0x1p17f) return reduce_ff(x, rh, rl);
+    return reduce_big_f(x, rh, rl);
+}
+
+// sin(rh + rl) on |rh| <= pi/4: rh + (rh^3 P(rh^2) + rl (1 - rh^2/2)), Taylor to r^11; the leading
+// -rh^3/6 also takes the rounding errors of rh^2 and rh^3
+__device__ __forceinline__ float ksinf(float rh, float rl)
+{
+    const float z = rh * rh;
+    float P = -0x1.ae6456p-26f;
+    P = __builtin_fmaf(P, z, 0x1.71de3ap-19f);
+    P = __builtin_fmaf(P, z, -0x1.a01a02p-13f);
+    P = __builtin_fmaf(P, z, 0x1.111112p-7f);
+    P = __builtin_fmaf(P, z, -0x1.555556p-3f);
+    const float zl = __builtin_fmaf(rh, rh, -z);
+    const float c = __builtin_fmaf(-0.5f * z, rl, rl);
+    const float r3 = rh * z;
+    const float r3l = __builtin_fmaf(rh, z, -r3);
+    const float t = __builtin_fmaf(__builtin_fmaf(rh, zl, r3l), -0x1.555556p-3f, c);
+    return rh + __builtin_fmaf(r3, P, t);
+}
+
+// cos(rh + rl) on |rh| <= pi/4: (1 - z/2) split exactly, + z^2 Q(z) - rh rl, Taylor to r^12
+__device__ __forceinline__ float kcosf(float rh, float rl)
+{
+    const float z = rh * rh;
+    const float zl = __builtin_fmaf(rh, rh, -z);
+    float Q = 0x1.1eed8ep-29f;
+    Q = __builtin_fmaf(Q, z, -0x1.27e4fcp-22f);
+    Q = __builtin_fmaf(Q, z, 0x1.a01a02p-16f);
+    Q = __builtin_fmaf(Q, z, -0x1.6c16c2p-10f);
+    Q = __builtin_fmaf(Q, z, 0x1.555556p-5f);
+    const float hz = 0.5f * z;
+    const float w = 1.0f - hz;
+    const float we = (1.0f - w) - hz;
+    return w + __builtin_fmaf(z * z, Q, __builtin_fmaf(-rh, rl, __builtin_fmaf(-0.5f, zl, we)));
 }
 
 __device__ __forceinline__ float sinf(float x)
 {
-    double r;
-    const int q = reduce_pio2_f((double)x, &r);
-    const double sn = ksinf(r), cs = kcosf(r);
-    double v = (q & 1) ? cs : sn;
+    float rh, rl;
+    const int q = reduce_f32(x, rh, rl);
+    float v = (q & 1) ? kcosf(rh, rl) : ksinf(rh, rl);
     v = (q & 2) ? -v : v;
-    float y = (float)v;
     const float ax = __builtin_fabsf(x);
-    y = ax < 0x1p-12f ? x : y; // sin x rounds to x; keeps -0
-    y = !(ax <= 0x1.fffffep+127f) ? x - x : y;
-    return y;
+    v = ax < 0x1p-12f ? x : v; // sin x rounds to x; keeps -0
+    v = !(ax <= 0x1.fffffep+127f) ? x - x : v;
+    return v;
 }
 
 __device__ __forceinline__ float cosf(float x)
 {
-    double r;
-    const int q = reduce_pio2_f((double)x, &r);
-    const double sn = ksinf(r), cs = kcosf(r);
-    double v = (q & 1) ? sn : cs;
+    float rh, rl;
+    const int q = reduce_f32(x, rh, rl);
+    float v = (q & 1) ? ksinf(rh, rl) : kcosf(rh, rl);
     v = ((q + 1) & 2) ? -v : v;
-    float y = (float)v;
-    y = !(__builtin_fabsf(x) <= 0x1.fffffep+127f) ? x - x : y;
-    return y;
+    v = !(__builtin_fabsf(x) <= 0x1.fffffep+127f) ? x - x : v;
+    return v;
 }
 
-// e^x: clamp to [-104, 89] (beyond: 0 and +inf after rounding), n = rint(x log2 e),
-// r = x - n ln2 in two parts (|r| <= 0.347), e^r = 1 + r + r^2 p(r), Taylor to r^8
+// both from one reduction; each equals sinf / cosf
+__device__ __forceinline__ void sincosf(float x, float* sp, float* cp)
+{
+    float rh, rl;
+    const int q = reduce_f32(x, rh, rl);
+    const float sn = ksinf(rh, rl), cs = kcosf(rh, rl);
+    float s = (q & 1) ? cs : sn;
+    float c = (q & 1) ? sn : cs;
+    s = (q & 2) ? -s : s;
+    c = ((q + 1) & 2) ? -c : c;
+    const float ax = __builtin_fabsf(x);
+    s = ax < 0x1p-12f ? x : s;
+    const bool bad = !(ax <= 0x1.fffffep+127f);
+    *sp = bad ? x - x : s;
+    *cp = bad ? x - x : c;
+}
+
+// e^x: clamp to [-104, 89] (beyond: 0 and +inf after rounding), n = rint(x log2 e), r = x - n ln2 as
+// rh + rl (x - n L1 exact by FMA), e^r = (1 + rh) [split exactly] + rh^2 P(rh) + rl (1 + rh), Taylor to
+// r^7, then the scaling by 2^n
 __device__ __forceinline__ float expf(float x)
 {
-    const double xc = __builtin_fmin(__builtin_fmax((double)x, -104.0), 89.0);
-    const double n = __builtin_rint(xc * kLog2e);
-    double r = __builtin_fma(-n, kLn2Hi, xc);
-    r = __builtin_fma(-n, kLn2Lo, r);
-    double p = 0x1.a01a01a01a01ap-16;
-    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
-    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
-    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
-    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
-    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
-    p = __builtin_fma(p, r, 0.5);
-    const double u = __builtin_fma(r * r, p, r);
-    float y = (float)__builtin_ldexp(1.0 + u, (int)n);
+    const float xc = __builtin_fminf(__builtin_fmaxf(x, -104.0f), 89.0f);
+    const float n = __builtin_rintf(xc * 0x1.715476p+0f);
+    const float r1 = __builtin_fmaf(-n, 0x1.62e430p-1f, xc);
+    const float rh = __builtin_fmaf(-n, -0x1.05c610p-29f, r1);
+    const float rl = __builtin_fmaf(-n, -0x1.05c610p-29f, r1 - rh);
+    float P = 0x1.a01a02p-13f;
+    P = __builtin_fmaf(P, rh, 0x1.6c16c2p-10f);
+    P = __builtin_fmaf(P, rh, 0x1.111112p-7f);
+    P = __builtin_fmaf(P, rh, 0x1.555556p-5f);
+    P = __builtin_fmaf(P, rh, 0x1.555556p-3f);
+    P = __builtin_fmaf(P, rh, 0.5f);
+    const float s = 1.0f + rh;
+    const float se = (1.0f - s) + rh;
+    const float tail = __builtin_fmaf(rh * rh, P, __builtin_fmaf(rl, rh, se + rl));
+    float y = __builtin_ldexpf(s + tail, (int)n);
     y = (x != x) ? x + x : y;
     return y;
 }
 
-// log x: the double arrangement above with R to s^10 (truncation < 2^-34)
+// log x = k ln2 + log(1+f), f = m - 1 with m in [sqrt2/2, sqrt2); log(1+f) = f - f^2/2 + s (f^2/2 + R),
+// s = f/(2+f), R to s^10; f - f^2/2 and k K1 + (f - f^2/2) are exact float-float sums (K1 = ln2 to 16
+// bits, so k K1 is exact)
 __device__ __forceinline__ float logf(float x)
 {
-    const uint64_t u = bits_of((double)x);
-    int k = (int)(u >> 52) - 1023;
-    uint64_t mb = (u & 0xFFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL;
-    const bool big = mb > 0x3FF6A09E667F3BCCULL;
-    mb = big ? mb - 0x0010000000000000ULL : mb;
+    const bool sub = __builtin_fabsf(x) < 0x1p-126f;
+    const float xs = sub ? x * 0x1p25f : x;
+    const uint32_t u = __builtin_bit_cast(uint32_t, xs);
+    int k = (int)(u >> 23) - 127 - (sub ? 25 : 0);
+    uint32_t mb = (u & 0x7FFFFFu) | 0x3F800000u;
+    const bool big = mb > 0x3FB504F3u;
+    mb = big ? mb - 0x00800000u : mb;
     k += big ? 1 : 0;
-    const double f = from_bits(mb) - 1.0;
-    const double s = f / (2.0 + f);
-    const double z = s * s;
-    double R = 0x1.745d1745d1746p-3;
-    R = __builtin_fma(R, z, 0x1.c71c71c71c71cp-3);
-    R = __builtin_fma(R, z, 0x1.2492492492492p-2);
-    R = __builtin_fma(R, z, 0x1.999999999999ap-2);
-    R = __builtin_fma(R, z, 0x1.5555555555555p-1);
+    const float f = __builtin_bit_cast(float, mb) - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float R = 0x1.745d18p-3f;
+    R = __builtin_fmaf(R, z, 0x1.c71c72p-3f);
+    R = __builtin_fmaf(R, z, 0x1.24924ap-2f);
+    R = __builtin_fmaf(R, z, 0x1.99999ap-2f);
+    R = __builtin_fmaf(R, z, 0x1.555556p-1f);
     R = R * z;
-    const double hfsq = 0.5 * f * f;
-    const double dk = (double)k;
-    float y = (float)(dk * kLn2Hi42 - ((hfsq - (s * (hfsq + R) + dk * kLn2Lo42)) - f));
+    const float ffh = f * f;
+    const float ffl = __builtin_fmaf(f, f, -ffh);
+    const float hf = 0.5f * ffh;
+    const float d = f - hf;
+    const float de = (f - d) - hf;
+    const float fk = (float)k;
+    float sh, se;
+    f_two2(fk * 0x1.62e400p-1f, d, sh, se);
+    float y = sh + __builtin_fmaf(s, hf + R,
+                                  __builtin_fmaf(fk, 0x1.7f7d1cp-20f, __builtin_fmaf(-0.5f, ffl, de) + se));
     y = (x == 0.0f) ? -__builtin_inff() : y;
     y = (x < 0.0f) ? __builtin_nanf("") : y;
     y = (x == __builtin_inff()) ? x : y;

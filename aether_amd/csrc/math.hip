@@ -44,10 +44,10 @@ __global__ __launch_bounds__(kThreads) void map_kernel(const T* __restrict__ x, 
                 y[i] = s;
                 y[n + i] = c;
             } else {
-                double s, c;
-                dm::sincos((double)v, &s, &c);
-                y[i] = (float)s;
-                y[n + i] = (float)c;
+                float s, c;
+                dm::sincosf(v, &s, &c);
+                y[i] = s;
+                y[n + i] = c;
             }
             y[2 * n + i] = apply<AETH_FN_EXP>(v);
             y[3 * n + i] = apply<AETH_FN_LOG>(v);

@@ -30,6 +30,8 @@
 
 static inline uint64_t dbits(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
 static inline double bitsd(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
+static inline uint32_t fbits(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+static inline float bitsf(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
 
 /* pi/2 = P1 + P2 + P3 (each the rounding of what is left) */
 #define PIO2_1 0x1.921fb54442d18p+0
@@ -286,10 +288,79 @@ double orc_log(double x)
     return dk * LN2_HI42 - ((hfsq - (s * (hfsq + R) + dk * LN2_LO42)) - f);
 }
 
-/* ---- float routines: argument widened to double, reduced once, polynomials
- * long enough for float, one rounding (same operations as the device). */
+/* ---- float routines in float arithmetic (same operations as the device). The product of two floats
+ * is split exactly with an FMA and float-float (unevaluated sum) steps carry the reduced argument and the
+ * leading terms, so each result is one rounding of a value accurate to about 2^-29: measured over every
+ * float argument, at most 0.67 ulp for sin/cos, 0.68 for exp (0.78 where the result is subnormal, which
+ * rounds twice) and 0.64 for log, with more than 99.8% correctly rounded. Double is used only to reduce
+ * trig arguments of magnitude 2^17 and above. */
 
-/* |x| < 2^20: r = x - n P1 - n P2; above: Payne-Hanek */
+static inline void f_fast2(float a, float b, float* s, float* e)
+{   /* |a| >= |b| (or a == 0): s + e == a + b exactly */
+    const float ss = a + b;
+    *e = b - (ss - a);
+    *s = ss;
+}
+
+static inline void f_two2(float a, float b, float* s, float* e)
+{   /* any order: s + e == a + b exactly */
+    const float ss = a + b;
+    const float bb = ss - a;
+    *e = (a - (ss - bb)) + (b - bb);
+    *s = ss;
+}
+
+/* |x| < 2^17: n = rint(x 2/pi), r = x - n (C1 + C2 + C3) with C1 + C2 + C3 = pi/2 to 2^-73 relative;
+ * x - n C1 is exact (FMA), n C2 is split exactly, so rh + rl is within 2^-55 of x - n pi/2. */
+static int reduce_ff(float x, float* rh, float* rl)
+{
+    const float n = rintf(x * 0x1.45f306p-1f);
+    const float r1 = fmaf(-n, 0x1.921fb6p+0f, x);
+    const float p = n * -0x1.777a5cp-25f;
+    const float pe = fmaf(n, -0x1.777a5cp-25f, -p);
+    float h, e;
+    f_two2(r1, -p, &h, &e);
+    const float l = fmaf(-n, -0x1.ee59dap-50f, e - pe);
+    f_fast2(h, l, rh, rl);
+    return ((int)n) & 3;
+}
+
+/* sin(rh + rl) on |rh| <= pi/4: rh + (rh^3 P(rh^2) + rl (1 - rh^2/2)), Taylor to r^11; the leading
+ * -rh^3/6 also takes the rounding errors of rh^2 and rh^3 */
+static float ksinf_(float rh, float rl)
+{
+    const float z = rh * rh;
+    float P = -0x1.ae6456p-26f;              /* -1/11! */
+    P = fmaf(P, z, 0x1.71de3ap-19f);         /* 1/9! */
+    P = fmaf(P, z, -0x1.a01a02p-13f);        /* -1/7! */
+    P = fmaf(P, z, 0x1.111112p-7f);          /* 1/5! */
+    P = fmaf(P, z, -0x1.555556p-3f);         /* -1/3! */
+    const float zl = fmaf(rh, rh, -z);
+    const float c = fmaf(-0.5f * z, rl, rl);
+    const float r3 = rh * z;
+    const float r3l = fmaf(rh, z, -r3);
+    const float t = fmaf(fmaf(rh, zl, r3l), -0x1.555556p-3f, c);
+    return rh + fmaf(r3, P, t);
+}
+
+/* cos(rh + rl) on |rh| <= pi/4: (1 - z/2) split exactly, + z^2 Q(z) - rh rl, Taylor to r^12 */
+static float kcosf_(float rh, float rl)
+{
+    const float z = rh * rh;
+    const float zl = fmaf(rh, rh, -z);
+    float Q = 0x1.1eed8ep-29f;               /* 1/12! */
+    Q = fmaf(Q, z, -0x1.27e4fcp-22f);        /* -1/10! */
+    Q = fmaf(Q, z, 0x1.a01a02p-16f);         /* 1/8! */
+    Q = fmaf(Q, z, -0x1.6c16c2p-10f);        /* -1/6! */
+    Q = fmaf(Q, z, 0x1.555556p-5f);          /* 1/4! */
+    const float hz = 0.5f * z;
+    const float w = 1.0f - hz;
+    const float we = (1.0f - w) - hz;
+    return w + fmaf(z * z, Q, fmaf(-rh, rl, fmaf(-0.5f, zl, we)));
+}
+
+/* the double reduction for float arguments of magnitude 2^17 and above: r = x - n P1 - n P2 below
+ * 2^20, Payne-Hanek above */
 static int reduce_f(double x, double* r)
 {
     const double n = rint(x * TWO_OVER_PI);
@@ -312,97 +383,93 @@ static int reduce_f(double x, double* r)
     return q;
 }
 
-/* Taylor sin to r^11 on |r| <= pi/4 */
-static double ksinf_(double r)
+/* the quadrant and reduced argument: float path below 2^17, the double reduction above */
+static int reduce_f32(float x, float* rh, float* rl)
 {
-    const double z = r * r;
-    double p = -0x1.ae64567f544e4p-26;      /* -1/11! */
-    p = fma(p, z, 0x1.71de3a556c734p-19);   /* 1/9! */
-    p = fma(p, z, -0x1.a01a01a01a01ap-13);  /* -1/7! */
-    p = fma(p, z, 0x1.1111111111111p-7);    /* 1/5! */
-    p = fma(p, z, -0x1.5555555555555p-3);   /* -1/3! */
-    return fma(r * z, p, r);
-}
-
-/* Taylor cos to r^10 on |r| <= pi/4 */
-static double kcosf_(double r)
-{
-    const double z = r * r;
-    double p = -0x1.27e4fb7789f5cp-22;      /* -1/10! */
-    p = fma(p, z, 0x1.a01a01a01a01ap-16);   /* 1/8! */
-    p = fma(p, z, -0x1.6c16c16c16c17p-10);  /* -1/6! */
-    p = fma(p, z, 0x1.5555555555555p-5);    /* 1/4! */
-    p = fma(p, z, -0.5);
-    return fma(z, p, 1.0);
+    if (fabsf(x) < 0x1p17f) return reduce_ff(x, rh, rl);
+    double r;
+    const int q = reduce_f((double)x, &r);
+    *rh = (float)r;
+    *rl = (float)(r - (double)*rh);
+    return q;
 }
 
 float orc_sinf(float x)
 {
-    double r;
-    const int q = reduce_f((double)x, &r);
-    const double sn = ksinf_(r), cs = kcosf_(r);
-    double v = (q & 1) ? cs : sn;
+    float rh, rl;
+    const int q = reduce_f32(x, &rh, &rl);
+    float v = (q & 1) ? kcosf_(rh, rl) : ksinf_(rh, rl);
     v = (q & 2) ? -v : v;
-    float y = (float)v;
     const float ax = fabsf(x);
-    y = ax < 0x1p-12f ? x : y;
-    y = !(ax <= 0x1.fffffep+127f) ? x - x : y;
-    return y;
+    v = ax < 0x1p-12f ? x : v;
+    v = !(ax <= 0x1.fffffep+127f) ? x - x : v;
+    return v;
 }
 
 float orc_cosf(float x)
 {
-    double r;
-    const int q = reduce_f((double)x, &r);
-    const double sn = ksinf_(r), cs = kcosf_(r);
-    double v = (q & 1) ? sn : cs;
+    float rh, rl;
+    const int q = reduce_f32(x, &rh, &rl);
+    float v = (q & 1) ? ksinf_(rh, rl) : kcosf_(rh, rl);
     v = ((q + 1) & 2) ? -v : v;
-    float y = (float)v;
-    y = !(fabsf(x) <= 0x1.fffffep+127f) ? x - x : y;
-    return y;
+    v = !(fabsf(x) <= 0x1.fffffep+127f) ? x - x : v;
+    return v;
 }
 
-/* e^x on [-104, 89]: n = rint(x log2 e), two-part ln2, Taylor to r^8 */
+/* e^x on [-104, 89]: n = rint(x log2 e); r = x - n ln2 as rh + rl (x - n L1 exact by FMA);
+ * e^r = (1 + rh) [split exactly] + rh^2 P(rh) + rl (1 + rh), Taylor to r^7; then the scaling by 2^n */
 float orc_expf(float x)
 {
-    const double xc = fmin(fmax((double)x, -104.0), 89.0);
-    const double n = rint(xc * LOG2E);
-    double r = fma(-n, LN2_HI, xc);
-    r = fma(-n, LN2_LO, r);
-    double p = 0x1.a01a01a01a01ap-16;       /* 1/8! */
-    p = fma(p, r, 0x1.a01a01a01a01ap-13);   /* 1/7! */
-    p = fma(p, r, 0x1.6c16c16c16c17p-10);   /* 1/6! */
-    p = fma(p, r, 0x1.1111111111111p-7);    /* 1/5! */
-    p = fma(p, r, 0x1.5555555555555p-5);    /* 1/4! */
-    p = fma(p, r, 0x1.5555555555555p-3);    /* 1/3! */
-    p = fma(p, r, 0.5);
-    const double u = fma(r * r, p, r);
-    float y = (float)ldexp(1.0 + u, (int)n);
+    const float xc = fminf(fmaxf(x, -104.0f), 89.0f);
+    const float n = rintf(xc * 0x1.715476p+0f);
+    const float r1 = fmaf(-n, 0x1.62e430p-1f, xc);
+    const float rh = fmaf(-n, -0x1.05c610p-29f, r1);
+    const float rl = fmaf(-n, -0x1.05c610p-29f, r1 - rh);
+    float P = 0x1.a01a02p-13f;               /* 1/7! */
+    P = fmaf(P, rh, 0x1.6c16c2p-10f);        /* 1/6! */
+    P = fmaf(P, rh, 0x1.111112p-7f);         /* 1/5! */
+    P = fmaf(P, rh, 0x1.555556p-5f);         /* 1/4! */
+    P = fmaf(P, rh, 0x1.555556p-3f);         /* 1/3! */
+    P = fmaf(P, rh, 0.5f);
+    const float s = 1.0f + rh;
+    const float se = (1.0f - s) + rh;
+    const float tail = fmaf(rh * rh, P, fmaf(rl, rh, se + rl));
+    float y = ldexpf(s + tail, (int)n);
     y = (x != x) ? x + x : y;
     return y;
 }
 
-/* log x: orc_log's arrangement with R to s^10 */
+/* log x = k ln2 + log(1+f), f = m - 1 with m in [sqrt2/2, sqrt2); log(1+f) = f - f^2/2 + s (f^2/2 + R),
+ * s = f/(2+f), R to s^10; f - f^2/2 and k K1 + (f - f^2/2) are carried as exact float-float sums
+ * (K1 = ln2 to 16 bits, so k K1 is exact) */
 float orc_logf(float x)
 {
-    const uint64_t u = dbits((double)x);
-    int k = (int)(u >> 52) - 1023;
-    uint64_t mb = (u & 0xFFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL;
-    const int big = mb > 0x3FF6A09E667F3BCCULL;
-    mb = big ? mb - 0x0010000000000000ULL : mb;
+    const int sub = fabsf(x) < 0x1p-126f;
+    const float xs = sub ? x * 0x1p25f : x;
+    const uint32_t u = fbits(xs);
+    int k = (int)(u >> 23) - 127 - (sub ? 25 : 0);
+    uint32_t mb = (u & 0x7FFFFFu) | 0x3F800000u;
+    const int big = mb > 0x3FB504F3u;
+    mb = big ? mb - 0x00800000u : mb;
     k += big ? 1 : 0;
-    const double f = bitsd(mb) - 1.0;
-    const double s = f / (2.0 + f);
-    const double z = s * s;
-    double R = 0x1.745d1745d1746p-3;        /* 2/11 */
-    R = fma(R, z, 0x1.c71c71c71c71cp-3);    /* 2/9 */
-    R = fma(R, z, 0x1.2492492492492p-2);    /* 2/7 */
-    R = fma(R, z, 0x1.999999999999ap-2);    /* 2/5 */
-    R = fma(R, z, 0x1.5555555555555p-1);    /* 2/3 */
+    const float f = bitsf(mb) - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float R = 0x1.745d18p-3f;                /* 2/11 */
+    R = fmaf(R, z, 0x1.c71c72p-3f);          /* 2/9 */
+    R = fmaf(R, z, 0x1.24924ap-2f);          /* 2/7 */
+    R = fmaf(R, z, 0x1.99999ap-2f);          /* 2/5 */
+    R = fmaf(R, z, 0x1.555556p-1f);          /* 2/3 */
     R = R * z;
-    const double hfsq = 0.5 * f * f;
-    const double dk = (double)k;
-    float y = (float)(dk * LN2_HI42 - ((hfsq - (s * (hfsq + R) + dk * LN2_LO42)) - f));
+    const float ffh = f * f;
+    const float ffl = fmaf(f, f, -ffh);
+    const float hf = 0.5f * ffh;
+    const float d = f - hf;
+    const float de = (f - d) - hf;
+    const float fk = (float)k;
+    float sh, se;
+    f_two2(fk * 0x1.62e400p-1f, d, &sh, &se);
+    float y = sh + fmaf(s, hf + R, fmaf(fk, 0x1.7f7d1cp-20f, fmaf(-0.5f, ffl, de) + se));
     y = (x == 0.0f) ? -INFINITY : y;
     y = (x < 0.0f) ? NAN : y;
     y = (x == INFINITY) ? x : y;
