@@ -368,8 +368,9 @@ __device__ __forceinline__ int reduce_f32(float x, float& rh, float& rl)
 }
 
 // sin(rh + rl) on |rh| <= pi/4: rh + (rh^3 P(rh^2) + rl (1 - rh^2/2)), Taylor to r^11; the leading
-// -rh^3/6 also takes the rounding errors of rh^2 and rh^3
-__device__ __forceinline__ float ksinf(float rh, float rl)
+// -rh^3/6 also takes the rounding errors of rh^2 and rh^3. Returns the rounded value and, in lo, what it
+// left out (the pair feeds tan's quotient).
+__device__ __forceinline__ float ksinf(float rh, float rl, float& lo)
 {
     const float z = rh * rh;
     float P = -0x1.ae6456p-26f;
@@ -382,11 +383,14 @@ __device__ __forceinline__ float ksinf(float rh, float rl)
     const float r3 = rh * z;
     const float r3l = __builtin_fmaf(rh, z, -r3);
     const float t = __builtin_fmaf(__builtin_fmaf(rh, zl, r3l), -0x1.555556p-3f, c);
-    return rh + __builtin_fmaf(r3, P, t);
+    float h;
+    f_fast2(rh, __builtin_fmaf(r3, P, t), h, lo);
+    return h;
 }
 
-// cos(rh + rl) on |rh| <= pi/4: (1 - z/2) split exactly, + z^2 Q(z) - rh rl, Taylor to r^12
-__device__ __forceinline__ float kcosf(float rh, float rl)
+// cos(rh + rl) on |rh| <= pi/4: (1 - z/2) split exactly, + z^2 Q(z) - rh rl, Taylor to r^12; a pair as
+// above
+__device__ __forceinline__ float kcosf(float rh, float rl, float& lo)
 {
     const float z = rh * rh;
     const float zl = __builtin_fmaf(rh, rh, -z);
@@ -398,14 +402,16 @@ __device__ __forceinline__ float kcosf(float rh, float rl)
     const float hz = 0.5f * z;
     const float w = 1.0f - hz;
     const float we = (1.0f - w) - hz;
-    return w + __builtin_fmaf(z * z, Q, __builtin_fmaf(-rh, rl, __builtin_fmaf(-0.5f, zl, we)));
+    float h;
+    f_fast2(w, __builtin_fmaf(z * z, Q, __builtin_fmaf(-rh, rl, __builtin_fmaf(-0.5f, zl, we))), h, lo);
+    return h;
 }
 
 __device__ __forceinline__ float sinf(float x)
 {
-    float rh, rl;
+    float rh, rl, lo;
     const int q = reduce_f32(x, rh, rl);
-    float v = (q & 1) ? kcosf(rh, rl) : ksinf(rh, rl);
+    float v = (q & 1) ? kcosf(rh, rl, lo) : ksinf(rh, rl, lo);
     v = (q & 2) ? -v : v;
     const float ax = __builtin_fabsf(x);
     v = ax < 0x1p-12f ? x : v; // sin x rounds to x; keeps -0
@@ -415,9 +421,9 @@ __device__ __forceinline__ float sinf(float x)
 
 __device__ __forceinline__ float cosf(float x)
 {
-    float rh, rl;
+    float rh, rl, lo;
     const int q = reduce_f32(x, rh, rl);
-    float v = (q & 1) ? ksinf(rh, rl) : kcosf(rh, rl);
+    float v = (q & 1) ? ksinf(rh, rl, lo) : kcosf(rh, rl, lo);
     v = ((q + 1) & 2) ? -v : v;
     v = !(__builtin_fabsf(x) <= 0x1.fffffep+127f) ? x - x : v;
     return v;
@@ -426,9 +432,9 @@ __device__ __forceinline__ float cosf(float x)
 // both from one reduction; each equals sinf / cosf
 __device__ __forceinline__ void sincosf(float x, float* sp, float* cp)
 {
-    float rh, rl;
+    float rh, rl, lo;
     const int q = reduce_f32(x, rh, rl);
-    const float sn = ksinf(rh, rl), cs = kcosf(rh, rl);
+    const float sn = ksinf(rh, rl, lo), cs = kcosf(rh, rl, lo);
     float s = (q & 1) ? cs : sn;
     float c = (q & 1) ? sn : cs;
     s = (q & 2) ? -s : s;
@@ -894,12 +900,186 @@ __device__ __forceinline__ T selgt(T a, T b)
     return a > b ? a : b;
 }
 
-// float forms: the double routine on the widened argument, one rounding
-__device__ __forceinline__ float tanf(float x) { return (float)tan((double)x); }
-__device__ __forceinline__ float atanf(float x) { return (float)atan((double)x); }
-__device__ __forceinline__ float asinf(float x) { return (float)asin((double)x); }
-__device__ __forceinline__ float acosf(float x) { return (float)acos((double)x); }
-__device__ __forceinline__ float tanhf(float x) { return (float)tanh((double)x); }
+// ---- float forms of tan, atan, asin, acos, tanh in float arithmetic, on the float routines' pieces:
+// pair quotients with an exact FMA residual and a table of atan(k/16) as float pairs. Over every float
+// argument (|x| < 2^17 for tan): at most 0.78 ulp for tan, 0.501 for atan/asin/acos, 0.64 for tanh. pow's
+// float form keeps the double routine, rounded once.
+
+// (nh + nl) / (dh + dl) as a pair: q = nh/dh, its residual nh - q dh exact by FMA
+__device__ __forceinline__ void f_div2(float nh, float nl, float dh, float dl, float& h, float& l)
+{
+    const float q = nh / dh;
+    const float r = __builtin_fmaf(-q, dl, __builtin_fmaf(-q, dh, nh) + nl);
+    f_fast2(q, r / dh, h, l);
+}
+
+__device__ __forceinline__ float tanf(float x)
+{
+    float rh, rl, sl, cl, h, l;
+    const int q = reduce_f32(x, rh, rl);
+    const float sh = ksinf(rh, rl, sl), ch = kcosf(rh, rl, cl);
+    if (q & 1) f_div2(-ch, -cl, sh, sl, h, l);
+    else f_div2(sh, sl, ch, cl, h, l);
+    float v = h + l;
+    const float ax = __builtin_fabsf(x);
+    v = ax < 0x1p-12f ? x : v;
+    v = !(ax <= 0x1.fffffep+127f) ? x - x : v;
+    return v;
+}
+
+// atan(k/16), k = 0..16, as float pairs (mpmath)
+__device__ __constant__ static const float kAtan16[17][2] = {
+    {0x0p+0f, 0x0p+0f}, {0x1.ff55bcp-5f, -0x1.1a6042p-30f}, {0x1.fd5baap-4f, -0x1.54f424p-30f},
+    {0x1.7b97b4p-3f, 0x1.79cb60p-28f}, {0x1.f5b760p-3f, -0x1.b4dfc8p-29f}, {0x1.362774p-2f, -0x1.1f0286p-27f},
+    {0x1.6f6194p-2f, 0x1.e4def0p-30f}, {0x1.a64eecp-2f, 0x1.e611fep-29f}, {0x1.dac670p-2f, 0x1.586ed4p-28f},
+    {0x1.0657eap-1f, -0x1.6499e6p-26f}, {0x1.1e00bap-1f, 0x1.7bdfd6p-26f}, {0x1.345f02p-1f, -0x1.98e422p-28f},
+    {0x1.4978fap-1f, 0x1.934f70p-28f}, {0x1.5d5898p-1f, 0x1.c5a6c6p-27f}, {0x1.700a7cp-1f, 0x1.5e118cp-27f},
+    {0x1.819d0cp-1f, -0x1.1d4eb6p-26f}, {0x1.921fb6p-1f, -0x1.777a5cp-26f}};
+
+// atan(th + tl), 0 <= t <= 1, as a pair: c = k/16 nearest t, u = (t - c)/(1 + t c) (|u| <= 1/32, t - c
+// exact), atan(k/16) + u - u^3/3 + u^5/5 - u^7/7. NaN reads entry 0.
+__device__ __forceinline__ void atan01_f(float th, float tl, float& yh, float& yl)
+{
+    const float kf = __builtin_rintf(th * 16.0f);
+    const int k = (kf >= 0.0f && kf <= 16.0f) ? (int)kf : 0;
+    const float c = (float)k * 0.0625f;
+    float nh, nl, uh, ul, s, e;
+    f_fast2(th - c, tl, nh, nl);
+    const float dh = __builtin_fmaf(th, c, 1.0f);
+    const float dl = __builtin_fmaf(tl, c, __builtin_fmaf(th, c, 1.0f - dh));
+    f_div2(nh, nl, dh, dl, uh, ul);
+    const float u2 = uh * uh;
+    float p = __builtin_fmaf(u2, -0x1.24924ap-3f, 0x1.99999ap-3f);
+    p = __builtin_fmaf(p, u2, -0x1.555556p-2f);
+    const float tail = __builtin_fmaf(uh * u2, p, ul) + kAtan16[k][1];
+    f_fast2(kAtan16[k][0], uh, s, e);
+    f_fast2(s, e + tail, yh, yl);
+}
+
+// c_hi + c_lo - (yh + yl), rounded once (c = pi/2 or pi as a float pair)
+__device__ __forceinline__ float f_minus(float chi, float clo, float yh, float yl)
+{
+    float s, e;
+    f_two2(chi, -yh, s, e);
+    return s + ((e + clo) - yl);
+}
+
+__device__ __forceinline__ float atanf(float x)
+{
+    const float ax = __builtin_fabsf(x);
+    const bool inv = ax > 1.0f;
+    const float th = inv ? 1.0f / ax : ax;
+    const float tl = (inv && ax <= 0x1.fffffep+127f) ? __builtin_fmaf(-th, ax, 1.0f) * th : 0.0f;
+    float yh, yl;
+    atan01_f(th, tl, yh, yl);
+    float y = inv ? f_minus(0x1.921fb6p+0f, -0x1.777a5cp-25f, yh, yl) : yh + yl;
+    y = __builtin_copysignf(y, x);
+    y = (x != x) ? x + x : y;
+    return y;
+}
+
+// atan of the smaller of ax / sqrt(1 - x^2) and its inverse (pairs; 1 - x^2 exact as a pair); far for
+// the inverse (ax > sqrt2/2)
+__device__ __forceinline__ void asin_core_f(float ax, float& ah, float& al, bool& far)
+{
+    const float x2h = ax * ax;
+    const float x2l = __builtin_fmaf(ax, ax, -x2h);
+    float mh, me, ml, th, tl;
+    f_two2(1.0f, -x2h, mh, me);
+    f_fast2(mh, me - x2l, mh, ml);
+    const float sh = __builtin_sqrtf(mh);
+    const float sl = mh > 0.0f ? (__builtin_fmaf(-sh, sh, mh) + ml) / (2.0f * sh) : 0.0f;
+    far = ax > 0x1.6a09e6p-1f;
+    if (far) f_div2(sh, sl, ax, 0.0f, th, tl);
+    else f_div2(ax, 0.0f, sh, sl, th, tl);
+    atan01_f(th, tl, ah, al);
+}
+
+__device__ __forceinline__ float asinf(float x)
+{
+    const float ax = __builtin_fabsf(x);
+    float ah, al;
+    bool far;
+    asin_core_f(ax, ah, al, far);
+    float y = far ? f_minus(0x1.921fb6p+0f, -0x1.777a5cp-25f, ah, al) : ah + al;
+    y = __builtin_copysignf(y, x);
+    y = (ax > 1.0f) ? __builtin_nanf("") : y;
+    y = (x != x) ? x + x : y;
+    return y;
+}
+
+__device__ __forceinline__ float acosf(float x)
+{
+    const float ax = __builtin_fabsf(x);
+    float ah, al;
+    bool far;
+    asin_core_f(ax, ah, al, far);
+    float y;
+    if (far) y = x > 0.0f ? ah + al : f_minus(0x1.921fb6p+1f, -0x1.777a5cp-24f, ah, al);
+    else y = x < 0.0f ? f_minus(0x1.921fb6p+0f, -0x1.777a5cp-25f, -ah, -al)
+                      : f_minus(0x1.921fb6p+0f, -0x1.777a5cp-25f, ah, al);
+    y = (ax > 1.0f) ? __builtin_nanf("") : y;
+    y = (x != x) ? x + x : y;
+    return y;
+}
+
+// e^x as a pair for |x| <= 20: expf's steps without the clamp, the pair scaled by 2^n exactly
+__device__ __forceinline__ void expf_pair(float x, float& h, float& l)
+{
+    const float n = __builtin_rintf(x * 0x1.715476p+0f);
+    const float r1 = __builtin_fmaf(-n, 0x1.62e430p-1f, x);
+    const float rh = __builtin_fmaf(-n, -0x1.05c610p-29f, r1);
+    const float rl = __builtin_fmaf(-n, -0x1.05c610p-29f, r1 - rh);
+    float P = 0x1.a01a02p-13f;
+    P = __builtin_fmaf(P, rh, 0x1.6c16c2p-10f);
+    P = __builtin_fmaf(P, rh, 0x1.111112p-7f);
+    P = __builtin_fmaf(P, rh, 0x1.555556p-5f);
+    P = __builtin_fmaf(P, rh, 0x1.555556p-3f);
+    P = __builtin_fmaf(P, rh, 0.5f);
+    const float s = 1.0f + rh;
+    const float se = (1.0f - s) + rh;
+    float eh, el;
+    f_fast2(s, __builtin_fmaf(rh * rh, P, __builtin_fmaf(rl, rh, se + rl)), eh, el);
+    h = __builtin_ldexpf(eh, (int)n);
+    l = __builtin_ldexpf(el, (int)n);
+}
+
+// tanh: below 0.55 the Taylor series to x^21 (x^3/3 with the roundings of x^2 and x^3), above
+// 1 - 2/(e^{2|x|} + 1) in pairs; |x| capped at 10 (tanh rounds to 1 from 9.02)
+__device__ __forceinline__ float tanhf(float x)
+{
+    const float ax = __builtin_fabsf(x);
+    float y;
+    if (ax < 0.55f) {
+        const float z = ax * ax;
+        const float zl = __builtin_fmaf(ax, ax, -z);
+        float P = 0x1.967e18p-14f;
+        P = __builtin_fmaf(P, z, -0x1.f57d78p-13f);
+        P = __builtin_fmaf(P, z, 0x1.355824p-11f);
+        P = __builtin_fmaf(P, z, -0x1.7da364p-10f);
+        P = __builtin_fmaf(P, z, 0x1.d6d3d0p-9f);
+        P = __builtin_fmaf(P, z, -0x1.226e36p-7f);
+        P = __builtin_fmaf(P, z, 0x1.664f48p-6f);
+        P = __builtin_fmaf(P, z, -0x1.ba1ba2p-5f);
+        P = __builtin_fmaf(P, z, 0x1.111112p-3f);
+        P = P * z;
+        const float x3 = ax * z;
+        const float x3l = __builtin_fmaf(ax, z, -x3);
+        const float t = __builtin_fmaf(__builtin_fmaf(ax, zl, x3l), -0x1.555556p-2f, x3 * P);
+        y = ax + __builtin_fmaf(x3, -0x1.555556p-2f, t);
+    } else {
+        float eh, el, dh, de, qh, ql;
+        expf_pair(2.0f * __builtin_fminf(ax, 10.0f), eh, el);
+        f_two2(eh, 1.0f, dh, de);
+        f_fast2(dh, de + el, dh, de);
+        f_div2(2.0f, 0.0f, dh, de, qh, ql);
+        y = f_minus(1.0f, 0.0f, qh, ql);
+    }
+    y = __builtin_copysignf(y, x);
+    y = (x != x) ? x + x : y;
+    return y;
+}
+
 __device__ __forceinline__ float powf(float x, float y) { return (float)pow((double)x, (double)y); }
 
 } // namespace dm

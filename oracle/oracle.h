@@ -79,6 +79,11 @@ float orc_sinf(float x);
 float orc_cosf(float x);
 float orc_expf(float x);
 float orc_logf(float x);
+float orc_tanf(float x);
+float orc_atanf(float x);
+float orc_asinf(float x);
+float orc_acosf(float x);
+float orc_tanhf(float x);
 /* vector forms for the tests: fn 0=sin 1=cos 2=exp 3=log; mode 0 = libm, 1 = own */
 double orc_tan(double x);
 double orc_atan(double x);

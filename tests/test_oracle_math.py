@@ -273,11 +273,58 @@ def test_extended_specials(orc):
     assert sg2[0] == 2 and sg2[1] == 1 and np.isnan(sg2[2]) and not np.signbit(sg2[3]) and np.signbit(sg2[4])
 
 
-def test_float_forms_round_the_double_routine_once(orc):
+FORM_BOUND = {"tan": 0.8, "atan": 0.51, "asin": 0.51, "acos": 0.51, "tanh": 0.65}
+
+
+@pytest.mark.parametrize("fn", sorted(FORM_BOUND))
+def test_float_forms_in_float_arithmetic(orc, fn):
+    """tan/atan/asin/acos/tanh in float arithmetic: 2^20 arguments against double libm (its error is 2^-29
+    of a float ulp) within the stated bound, over magnitudes from 10^-7 to the domain's edge (tan: to 10^6,
+    across the double-reduction boundary 2^17), and 4000 of them against mpmath."""
+    rng = np.random.default_rng(23)
+    n = 1 << 20
+    hi = {"tan": 6.0, "atan": 30.0, "asin": 0.0, "acos": 0.0, "tanh": 1.2}[fn]
+    x = 10.0 ** rng.uniform(-7, hi, n) * np.where(rng.random(n) < 0.5, -1, 1)
+    x = x.astype(np.float32)
+    if fn in ("asin", "acos"):
+        x = np.concatenate([x, (1 - 10.0 ** rng.uniform(-7.3, -0.2, n // 4)).astype(np.float32)])  # near 1
+        x = np.concatenate([x, -x])
+    y = orc.math32(fn, x, mode=1).astype(np.float64)
+    ref = getattr(np, {"atan": "arctan", "asin": "arcsin", "acos": "arccos"}.get(fn, fn))(x.astype(np.float64))
+    sp = np.abs(np.spacing(ref.astype(np.float32))).astype(np.float64)
+    err = np.abs(y - ref) / np.maximum(sp, 2.0 ** -149)
+    assert np.max(err) <= FORM_BOUND[fn], (np.max(err), x[np.argmax(err)])
+    xs = x[:: len(x) // 4000][:4000]
+    ys = orc.math32(fn, xs, mode=1).astype(np.float64)
+    mf = {"tan": mp.tan, "atan": mp.atan, "asin": mp.asin, "acos": mp.acos, "tanh": mp.tanh}[fn]
+    with mp.workdps(40):
+        want = np.array([float(mf(mp.mpf(float(v)))) for v in xs])
+    spm = np.abs(np.spacing(want.astype(np.float32))).astype(np.float64)
+    assert np.max(np.abs(ys - want) / np.maximum(spm, 2.0 ** -149)) <= FORM_BOUND[fn]
+
+
+def test_float_form_specials(orc):
+    f32 = lambda v: np.array(v, np.float32)  # noqa: E731
+    sp = f32([0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -1.0, 1.5, 1e-40, -1e-40])
+    t = orc.math32("tan", sp)
+    assert t[0] == 0 and np.signbit(t[1]) and np.isnan(t[2]) and np.isnan(t[3]) and np.isnan(t[4])
+    assert t[8] == sp[8] and t[9] == sp[9]
+    a = orc.math32("atan", sp)
+    assert a[0] == 0 and np.signbit(a[1]) and a[2] == np.float32(np.pi / 2) and a[3] == -np.float32(np.pi / 2)
+    assert np.isnan(a[4]) and a[5] == np.float32(np.pi / 4) and a[8] == sp[8]
+    s = orc.math32("asin", sp)
+    assert s[0] == 0 and np.signbit(s[1]) and np.isnan(s[2]) and np.isnan(s[7]) and s[5] == np.float32(np.pi / 2)
+    assert s[6] == -np.float32(np.pi / 2) and s[8] == sp[8]
+    c = orc.math32("acos", sp)
+    assert c[5] == 0 and c[6] == np.float32(np.pi) and c[0] == np.float32(np.pi / 2) and np.isnan(c[7])
+    h = orc.math32("tanh", sp)
+    assert h[0] == 0 and np.signbit(h[1]) and h[2] == 1 and h[3] == -1 and np.isnan(h[4]) and h[8] == sp[8]
+    assert orc.math32("tanh", f32([9.02, 20.0, 88.0]))[0] == 1
+
+
+def test_float_pow_rounds_the_double_routine_once(orc):
+    """pow's float form keeps the double routine (the one float form with FP64 arithmetic inside)."""
     x = np.random.default_rng(13).uniform(-0.99, 0.99, 2000).astype(np.float32)
-    for fn in EXT[:-1]:
-        got = orc.math32(fn, x)
-        assert np.array_equal(got, orc.math64(fn, x.astype(np.float64)).astype(np.float32)), fn
     y = np.random.default_rng(14).uniform(0.1, 3, 2000).astype(np.float32)
     assert np.array_equal(orc.math2_32("pow", np.abs(x) + 0.5, y),
                           orc.math2_64("pow", (np.abs(x) + 0.5).astype(np.float64), y.astype(np.float64)).astype(np.float32))
