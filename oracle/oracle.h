@@ -84,6 +84,7 @@ float orc_atanf(float x);
 float orc_asinf(float x);
 float orc_acosf(float x);
 float orc_tanhf(float x);
+float orc_powf(float x, float y);
 /* vector forms for the tests: fn 0=sin 1=cos 2=exp 3=log; mode 0 = libm, 1 = own */
 double orc_tan(double x);
 double orc_atan(double x);
