@@ -318,7 +318,7 @@ __device__ __forceinline__ int reduce_pio2_f(double x, double* r)
 // ---- float routines in float arithmetic. Products of two floats are split exactly with an FMA and
 // float-float (unevaluated sum) steps carry the reduced argument and the leading terms, so each result is
 // one rounding of a value good to about 2^-29: over every float argument at most 0.67 ulp for sin/cos,
-// 0.68 for exp (0.78 where the result is subnormal and rounds twice) and 0.64 for log, more than 99.8%
+// 0.68 for exp (0.78 where the result is subnormal and rounds twice) and 0.643 for log, more than 99.8%
 // correctly rounded. Double appears only in the reduction of trig arguments of magnitude 2^17 and up.
 
 __device__ __forceinline__ void f_fast2(float a, float b, float& s, float& e)
@@ -902,7 +902,7 @@ __device__ __forceinline__ T selgt(T a, T b)
 
 // ---- float forms of tan, atan, asin, acos, tanh in float arithmetic, on the float routines' pieces:
 // pair quotients with an exact FMA residual and a table of atan(k/16) as float pairs. Over every float
-// argument (|x| < 2^17 for tan): at most 0.78 ulp for tan, 0.501 for atan/asin/acos, 0.64 for tanh.
+// argument (|x| < 2^17 for tan): at most 0.783 ulp for tan, 0.501 for atan/asin/acos, 0.640 for tanh.
 
 // (nh + nl) / (dh + dl) as a pair: q = nh/dh, its residual nh - q dh exact by FMA
 __device__ __forceinline__ void f_div2(float nh, float nl, float dh, float dl, float& h, float& l)
@@ -1082,7 +1082,7 @@ __device__ __forceinline__ float tanhf(float x)
 // ---- pow's float form in float arithmetic: ln|x| as a float pair good to 2^-35 relative (so y ln|x|
 // carries an absolute error below 2^-28 wherever the result is finite and nonzero), its product with y split
 // exactly, and e^(ph + pl) with the full e^r factor on the input's low part; at most 0.75 ulp on 1.5*10^8
-// random pairs against long-double pow.
+// random pairs against long-double pow, 0.79 where the result is subnormal.
 
 __device__ __forceinline__ void f_add2(float ah, float al, float bh, float bl, float& h, float& l)
 { // (ah + al) + (bh + bl) as a pair

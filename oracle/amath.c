@@ -292,7 +292,7 @@ double orc_log(double x)
  * is split exactly with an FMA and float-float (unevaluated sum) steps carry the reduced argument and the
  * leading terms, so each result is one rounding of a value accurate to about 2^-29: measured over every
  * float argument, at most 0.67 ulp for sin/cos, 0.68 for exp (0.78 where the result is subnormal, which
- * rounds twice) and 0.64 for log, with more than 99.8% correctly rounded. Double is used only to reduce
+ * rounds twice) and 0.643 for log, with more than 99.8% correctly rounded. Double is used only to reduce
  * trig arguments of magnitude 2^17 and above. */
 
 static inline void f_fast2(float a, float b, float* s, float* e)
@@ -905,7 +905,8 @@ void orc_math2_f64(int fn, uint64_t n, const double* x, const double* y, double*
  * carries an absolute error below 2^-28 wherever the result is finite and nonzero), its product with y
  * split exactly, and e^(ph + pl) with the full e^r factor on the input's low part. Measured on 1.5*10^8
  * random pairs (x over the whole float range with y keeping the result in range, x within 10^-4 of 1
- * with |y| to 10^6, and x in [0, 4) with y in [-10, 10]) against long-double powl: at most 0.75 ulp. */
+ * with |y| to 10^6, and x in [0, 4) with y in [-10, 10]) against long-double powl: at most 0.75 ulp,
+ * 0.79 where the result is subnormal (oracle/float_sweep.c). */
 
 static void f_div2(float nh, float nl, float dh, float dl, float* h, float* l);
 
@@ -1023,8 +1024,8 @@ void orc_math2_f32(int fn, uint64_t n, const float* x, const float* y, float* z)
 
 /* ---- float forms of tan, atan, asin, acos, tanh in float arithmetic, on the float routines' pieces:
  * pair quotients with an exact FMA residual, a table of atan(k/16) as float pairs. Measured over every
- * float argument (|x| < 2^17 for tan): at most 0.78 ulp for tan, 0.501 for atan/asin/acos, 0.64 for
- * tanh. */
+ * float argument (|x| < 2^17 for tan): at most 0.783 ulp for tan, 0.501 for atan/asin/acos, 0.640
+ * for tanh (oracle/float_sweep.c). */
 
 /* (nh + nl) / (dh + dl) as a pair: q = nh/dh, its residual nh - q dh exact by FMA */
 static void f_div2(float nh, float nl, float dh, float dl, float* h, float* l)
