@@ -68,6 +68,8 @@ void orc_fill_array_uniform_f32(uint64_t seed, float lo, float hi, int ncomp, ui
                                 float* out);
 void orc_fill_array_normal_f64(uint64_t seed, double mean, double sd, int ncomp, uint64_t n, uint64_t pitch,
                                int mathmode, double* out);
+void orc_fill_array_normal_f32(uint64_t seed, float mean, float sd, int ncomp, uint64_t n, uint64_t pitch,
+                               int mathmode, float* out);
 
 /* ---------------- own math (shared algorithm with the device) ---------- */
 double orc_sin(double x);

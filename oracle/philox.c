@@ -222,3 +222,15 @@ void orc_fill_array_normal_f64(uint64_t seed, double mean, double sd, int ncomp,
             out[(uint64_t)c * pitch + i] = mean + sd * orc_normal_f64(seed, (uint32_t)i, (uint64_t)c, mathmode);
     }
 }
+
+/* float planes: mean + sd * the float normal (random/detail/Backend.h:209-236, two words), as the float
+ * array fill computes it */
+void orc_fill_array_normal_f32(uint64_t seed, float mean, float sd, int ncomp, uint64_t n, uint64_t pitch,
+                               int mathmode, float* out)
+{
+    for (int c = 0; c < ncomp; ++c) {
+#pragma omp parallel for schedule(static)
+        for (uint64_t i = 0; i < n; ++i)
+            out[(uint64_t)c * pitch + i] = mean + sd * orc_normal_f32(seed, (uint32_t)i, (uint64_t)c, mathmode);
+    }
+}

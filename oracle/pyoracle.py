@@ -53,6 +53,7 @@ def lib():
             "orc_fill_array_uniform_f64": (None, [_u64, _f64, _f64, _i32, _u64, _u64, _pd]),
             "orc_fill_array_uniform_f32": (None, [_u64, _f32, _f32, _i32, _u64, _u64, _pf]),
             "orc_fill_array_normal_f64": (None, [_u64, _f64, _f64, _i32, _u64, _u64, _i32, _pd]),
+            "orc_fill_array_normal_f32": (None, [_u64, _f32, _f32, _i32, _u64, _u64, _i32, _pf]),
             "orc_math_f64": (None, [_i32, _i32, _u64, _pd, _pd]),
             "orc_math_f32": (None, [_i32, _i32, _u64, _pf, _pf]),
             "orc_math2_f64": (None, [_i32, _u64, _pd, _pd, _pd]),
@@ -164,6 +165,12 @@ def fill_array_uniform(seed, ncomp, n, pitch, dtype=np.float64, lo=-1.0, hi=1.0)
 def fill_array_normal_f64(seed, ncomp, n, pitch, mean=0.0, sd=1.0, mathmode=1):
     out = np.zeros(ncomp * pitch, np.float64)
     lib().orc_fill_array_normal_f64(seed, mean, sd, ncomp, n, pitch, mathmode, out)
+    return out
+
+
+def fill_array_normal_f32(seed, ncomp, n, pitch, mean=0.0, sd=1.0, mathmode=1):
+    out = np.zeros(ncomp * pitch, np.float32)
+    lib().orc_fill_array_normal_f32(seed, mean, sd, ncomp, n, pitch, mathmode, out)
     return out
 
 

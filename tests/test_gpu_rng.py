@@ -67,10 +67,11 @@ def test_array_fill_counter_is_component(dev, orc, dist, dt):
         ae.array_fill_uniform(arr, 42, -1.0, 1.0)
         exp = orc.fill_array_uniform(42, 3, n, pitch, np.float64 if dt == torch.float64 else np.float32)
     else:
-        if dt == torch.float32:
-            pytest.skip("the oracle's normal array fill is double only; float normals are covered by streams")
         ae.array_fill_normal(arr, 42, 0.5, 2.0)
-        exp = orc.fill_array_normal_f64(42, 3, n, pitch, 0.5, 2.0)
+        if dt == torch.float32:
+            exp = orc.fill_array_normal_f32(42, 3, n, pitch, 0.5, 2.0)
+        else:
+            exp = orc.fill_array_normal_f64(42, 3, n, pitch, 0.5, 2.0)
     torch.cuda.synchronize()
     got = arr.data.cpu().numpy().reshape(-1)
     exp = exp.reshape(3, pitch)[:, :n]

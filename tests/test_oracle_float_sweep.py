@@ -15,10 +15,10 @@ BOUND = {"sin": 0.67, "cos": 0.67, "exp": 0.78, "log": 0.65, "tan": 0.79, "atan"
 
 
 @pytest.fixture(scope="module")
-def sweep():
-    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "float_sweep"], check=True)
-    out = subprocess.run([os.path.join(ROOT, "oracle", "float_sweep"), "1009"], check=True, capture_output=True,
-                         text=True, timeout=300).stdout
+def sweep(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("sweep") / "float_sweep")  # one build per test process
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), f"SWEEP={exe}", exe], check=True)
+    out = subprocess.run([exe, "1009"], check=True, capture_output=True, text=True, timeout=300).stdout
     rows = {}
     for line in out.splitlines():
         m = re.match(r"(\w+)\s+max ([0-9.]+) ulp .* \(([0-9.]+)\)$", line)
