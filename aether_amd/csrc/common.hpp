@@ -12,6 +12,7 @@
 
 #include <cstdio>
 #include <string>
+#include <vector>
 
 namespace aeth {
 
@@ -80,5 +81,33 @@ private:
     void* ptr_ = nullptr;
     hipStream_t stream_;
 };
+
+// Queries and releases made while another thread or stream captures in global mode: the thread's
+// capture mode is relaxed for the call's duration, so they are legal and do not invalidate the capture.
+struct RelaxedCapture {
+    hipStreamCaptureMode prev = hipStreamCaptureModeRelaxed;
+    RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&prev); }
+    ~RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&prev); }
+};
+
+// Whether `s` is being captured, or a capture elsewhere makes work on it illegal (a failed query, such
+// as the implicit capture of the legacy stream under a blocking capture, answers yes).
+inline bool capturing(hipStream_t s)
+{
+    RelaxedCapture relaxed;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
+// Frees device allocations (hipMalloc) in a way that is safe while a stream capture is in progress:
+// with the thread's capture mode relaxed, and deferred -- kept on a process-wide list -- while the legacy
+// stream is captured or a free is refused for a capture. Every call (and drain_device_frees, which the
+// seam's drain() calls) first retries what earlier calls deferred. Never throws.
+void free_device_later(std::vector<void*> ptrs);
+void drain_device_frees();
 
 } // namespace aeth

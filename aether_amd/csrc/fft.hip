@@ -1000,8 +1000,8 @@ extern "C" int aeth_fft_batch_destroy(void* plan)
     return guarded([&] {
         auto* p = static_cast<BatchPlan*>(plan);
         if (!p) return;
-        (void)hipFree(p->descs);
-        (void)hipFree(p->sigs);
+        // safe while a capture is in progress: the tables are freed once no capture stands in the way
+        free_device_later({p->descs, p->sigs});
         delete p;
     });
 }
@@ -1304,42 +1304,59 @@ struct SeamPlan {
     void* half = nullptr;            // c2r with other axes: (outer, n/2+1, inner) complex work
     uint64_t outer = 1, n = 1, inner = 1;
     std::mutex mu; // cufft_plan.h:187 -- one execution at a time per plan
-    static void free_pass(BatchPlan* p)
+    // Plan-owned device memory goes through free_device_later: a plan destroyed while a stream
+    // capture is in progress (dsc/src/seam.h:50, "safe during a capture") neither breaks the capture
+    // nor leaks; its buffers are freed once no capture stands in the way.
+    static void free_pass(BatchPlan* p, std::vector<void*>& bufs)
     {
         if (!p) return;
-        (void)hipFree(p->descs);
-        (void)hipFree(p->sigs);
+        bufs.push_back(p->descs);
+        bufs.push_back(p->sigs);
         delete p;
     }
-    static void free_four(FourStepPass* f)
+    static void free_four(FourStepPass* f, std::vector<void*>& bufs)
     {
         if (!f) return;
-        free_pass(f->cols);
-        free_pass(f->rows);
-        (void)hipFree(f->work);
-        (void)hipFree(f->thi);
-        (void)hipFree(f->tlo);
+        free_pass(f->cols, bufs);
+        free_pass(f->rows, bufs);
+        bufs.push_back(f->work);
+        bufs.push_back(f->thi);
+        bufs.push_back(f->tlo);
         delete f;
     }
-    static void free_pass(Pass& p)
+    static void free_pass(Pass& p, std::vector<void*>& bufs)
     {
-        free_pass(p.stockham);
-        free_four(p.four);
+        free_pass(p.stockham, bufs);
+        free_four(p.four, bufs);
         if (p.blue) {
-            free_pass(p.blue->conv);
-            (void)hipFree(p.blue->work);
-            (void)hipFree(p.blue->chirp);
-            (void)hipFree(p.blue->bhat);
+            free_pass(p.blue->conv, bufs);
+            bufs.push_back(p.blue->work);
+            bufs.push_back(p.blue->chirp);
+            bufs.push_back(p.blue->bhat);
             delete p.blue;
         }
         p = Pass{};
     }
+    static void free_pass(BatchPlan* p)
+    {
+        std::vector<void*> bufs;
+        free_pass(p, bufs);
+        free_device_later(std::move(bufs));
+    }
+    static void free_pass(Pass& p)
+    {
+        std::vector<void*> bufs;
+        free_pass(p, bufs);
+        free_device_later(std::move(bufs));
+    }
     ~SeamPlan()
     {
-        for (auto& p : passes) free_pass(p);
-        free_pass(line);
-        (void)hipFree(full);
-        (void)hipFree(half);
+        std::vector<void*> bufs;
+        for (auto& p : passes) free_pass(p, bufs);
+        free_pass(line, bufs);
+        bufs.push_back(full);
+        bufs.push_back(half);
+        free_device_later(std::move(bufs));
     }
 };
 

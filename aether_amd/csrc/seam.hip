@@ -54,14 +54,6 @@ struct OnDevice {
     }
 };
 
-// Queries and releases made while another thread or stream captures in global mode: the thread's
-// capture mode is relaxed for the call's duration, so they are legal and do not invalidate the capture.
-struct RelaxedCapture {
-    hipStreamCaptureMode prev = hipStreamCaptureModeRelaxed;
-    RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&prev); }
-    ~RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&prev); }
-};
-
 // Never throws: it runs from chunk_release, which returns nothing.
 void free_now(Chunk* c)
 {
@@ -79,19 +71,6 @@ void free_now(Chunk* c)
     }
     (void)hipGetLastError();
     delete c;
-}
-
-// Whether `s` is being captured, or a capture elsewhere makes work on it illegal (a failed query, such
-// as the implicit capture of the legacy stream under a blocking capture, answers yes).
-bool capturing(hipStream_t s)
-{
-    RelaxedCapture relaxed;
-    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
-        (void)hipGetLastError();
-        return true;
-    }
-    return st != hipStreamCaptureStatusNone;
 }
 
 // A release a capture makes illegal now: a device free is ordered on its own stream, a pinned free
@@ -188,6 +167,7 @@ int s_chunk_alloc(int kind, int dev, size_t bytes, void* stream, void** chunk, v
 // Release every deferred chunk whose capture has ended; those still blocked stay queued.
 void s_drain()
 {
+    drain_device_frees(); // buffers of FFT plans destroyed during a capture
     std::vector<Chunk*> keep, now;
     {
         std::lock_guard<std::mutex> lock(g_deferred_mu);
@@ -316,6 +296,51 @@ const AetherMemoryV1 kTable = {AETHER_MEMORY_ABI, s_probe,        s_runtime_vers
                                s_fft_destroy};
 
 } // namespace
+
+namespace {
+std::mutex g_later_mu;
+std::vector<void*> g_later; // device allocations whose free a capture refused or would have broken
+
+bool capture_error(hipError_t e)
+{
+    return e == hipErrorStreamCaptureUnsupported || e == hipErrorStreamCaptureImplicit ||
+           e == hipErrorStreamCaptureInvalidated || e == hipErrorStreamCaptureWrongThread ||
+           e == hipErrorCapturedEvent;
+}
+} // namespace
+
+void free_device_later(std::vector<void*> ptrs)
+{
+    std::lock_guard<std::mutex> lock(g_later_mu);
+    for (void* p : g_later) ptrs.push_back(p);
+    g_later.clear();
+    RelaxedCapture relaxed;
+    // the legacy stream captured (or its query refused): a free, which synchronises the device, must wait
+    if (capturing(nullptr)) {
+        for (void* p : ptrs)
+            if (p) g_later.push_back(p);
+        return;
+    }
+    for (void* p : ptrs) {
+        if (!p) continue;
+        const hipError_t e = hipFree(p);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            if (capture_error(e)) g_later.push_back(p);
+        }
+    }
+}
+
+void drain_device_frees()
+{
+    bool any = false;
+    {
+        std::lock_guard<std::mutex> lock(g_later_mu);
+        any = !g_later.empty();
+    }
+    if (any) free_device_later({});
+}
+
 } // namespace aeth
 
 extern "C" __attribute__((visibility("default"))) const AetherMemoryV1* aether_memory_v1(void) { return &aeth::kTable; }

@@ -31,6 +31,7 @@
 #include "common.hpp"
 #include "dmath.hpp"
 #include "rng.hpp"
+#include "units.hpp"
 
 namespace aeth {
 namespace {
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(kThreads) void tape_kernel(const TapeArgs A, acc::P
         const int nl = P.n_loads;
         T* outp = static_cast<T*>(const_cast<void*>(A.leaf[P.out_leaf])) + (uint64_t)P.out_comp * A.pitch[P.out_leaf];
         acc::Pair<T> acc{T(0), T(0)};
-        const bool full = VECTOR && s0 + kTile <= n;
+        const bool full = VECTOR && units::full_tile(s0, kTile, n);
         for (int r = 0; r < ROUNDS; ++r) {
             const uint32_t i0 = s0 + r * (kThreads * VN) + threadIdx.x * VN;
             T L[NL][VN];
@@ -664,7 +665,7 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
     s += "  aeth::acc::Pair<T>* part = (aeth::acc::Pair<T>*)part_;\n  const uint32_t n = A.n;\n";
     s += "  for (uint32_t u = blockIdx.x; u < A.units; u += gridDim.x) {\n";
     s += "    const uint32_t pi = u / A.tiles;\n    const uint32_t tile = u - pi * A.tiles;\n";
-    s += "    const uint32_t s0 = tile * TILE;\n    const bool full = vec && s0 + TILE <= n;\n";
+    s += "    const uint32_t s0 = tile * TILE;\n    const bool full = vec && aeth::units::full_tile(s0, TILE, n);\n";
     s += "    aeth::acc::Pair<T> acc{T(0), T(0)};\n    switch (pi) {\n";
     std::vector<bool> done(np, false);
     for (int p = 0; p < np; ++p) {

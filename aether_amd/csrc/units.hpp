@@ -16,6 +16,14 @@ constexpr int kTile = 2048;
 constexpr int kThreads = 256;
 constexpr int kGroup = 1024;
 
+// Whether the tile of `tile_len` samples starting at s0 lies wholly below n. Sample counts reach 2^32-1
+// (the 32-bit offset_t span), so the end of the last tile is formed in 64 bits: s0 + tile_len can
+// exceed 2^32 and must not wrap.
+__host__ __device__ constexpr bool full_tile(uint32_t s0, uint32_t tile_len, uint64_t n)
+{
+    return (uint64_t)s0 + tile_len <= n;
+}
+
 template<class T>
 struct V16 {
     static constexpr int n = 16 / sizeof(T);
@@ -76,7 +84,7 @@ __device__ __forceinline__ acc::Pair<T> cfg0_tile(const T* __restrict__ a, const
     const uint32_t s0 = tile * kTile;
     const uint64_t base = (uint64_t)comp * pitch + s0;
     acc::Pair<T> p{T(0), T(0)};
-    if (VECTOR && s0 + kTile <= n) {
+    if (VECTOR && full_tile(s0, kTile, n)) {
         T va[ROUNDS][VN], vb[ROUNDS][VN], vc[ROUNDS][VN];
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r) {
@@ -124,7 +132,7 @@ __device__ __forceinline__ acc::Pair<T> reduce_tile(const T* __restrict__ x, uin
     const uint32_t s0 = tile * kTile;
     const uint64_t base = (uint64_t)comp * pitch + s0;
     acc::Pair<T> p{T(0), T(0)};
-    if (VECTOR && s0 + kTile <= n) {
+    if (VECTOR && full_tile(s0, kTile, n)) {
         T v[ROUNDS][VN];
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r)

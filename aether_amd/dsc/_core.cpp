@@ -343,10 +343,25 @@ void need_contiguous(const ArrayObj* a, const char* what)
     if (!contiguous(a)) fail(std::string(what) + " needs a contiguous array; this one is strided");
 }
 
+// Extents that are valid for an array of `itemsize`-byte elements: none negative, and neither the element
+// count nor the byte count overflows 64 bits (count() and nbytes() are computed from them, and the
+// allocation, DLPack export and __cuda_array_interface__ all trust them).
+void check_extents(const int64_t* shape, int rank, size_t itemsize)
+{
+    int64_t elems = 1;
+    for (int k = 0; k < rank; ++k) {
+        if (shape[k] < 0) fail("negative extent");
+        if (__builtin_mul_overflow(elems, shape[k], &elems)) fail("the shape's element count overflows 64 bits");
+    }
+    int64_t bytes = 0;
+    if (__builtin_mul_overflow(elems, (int64_t)itemsize, &bytes)) fail("the shape's byte count overflows 64 bits");
+}
+
 void set_dense(ArrayObj* a, const std::vector<int64_t>& shape)
 {
     if ((int)shape.size() > kRank)
         fail("rank " + std::to_string(shape.size()) + " exceeds " + std::to_string(kRank));
+    check_extents(shape.data(), (int)shape.size(), item(a->dt));
     a->rank = (int)shape.size();
     int64_t acc = 1;
     for (int k = a->rank - 1; k >= 0; --k) {
@@ -815,6 +830,7 @@ PyObject* from_managed(M* m, bool versioned)
     if (st->kind < 0)
         fail("DLPack device type " + std::to_string(type) + " is not supported (cpu 1, rocm 10, rocm host 11)");
     if (t.ndim < 0 || t.ndim > kRank) fail("DLPack rank " + std::to_string(t.ndim) + " exceeds " + std::to_string(kRank));
+    check_extents(t.shape, t.ndim, item(dt));
     st->dev = t.device.device_id;
     st->ptr = t.data ? static_cast<char*>(t.data) + t.byte_offset : nullptr;
     ArrayObj* a = new_array(st, dt);
@@ -908,6 +924,11 @@ PyObject* f_from_buffer(PyObject*, PyObject* obj)
         st->keep = obj;
         const int dt = type_by_format(view->format ? view->format : "B");
         if (view->ndim > kRank) fail("buffer rank " + std::to_string(view->ndim) + " exceeds " + std::to_string(kRank));
+        {
+            std::vector<int64_t> shape(view->ndim);
+            for (int k = 0; k < view->ndim; ++k) shape[k] = view->shape[k];
+            check_extents(shape.data(), view->ndim, item(dt));
+        }
         ArrayObj* a = new_array(st, dt);
         a->rank = view->ndim;
         for (int k = 0; k < view->ndim; ++k) {

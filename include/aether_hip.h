@@ -37,7 +37,11 @@ enum { AETH_F64 = 0, AETH_F32 = 1 };
  * 8192, four-step above) or any other length <= 2^25 (Bluestein); longer
  * axes return 2. For R2C/C2R `shape` is
  * the real side's and the last axis is n/2 + 1 long on the complex side
- * (dsc/src/cufft_plan.h:9-12). */
+ * (dsc/src/cufft_plan.h:9-12). fft_plan is refused while its stream is
+ * captured; fft_exec allocates nothing and may be captured; fft_destroy is
+ * safe during a capture (dsc/src/seam.h:48-50): the plan's device buffers are
+ * freed at once when no capture stands in the way, otherwise by a later
+ * destroy or drain(). chunk_release defers the same way. */
 #define AETHER_MEMORY_ABI 2
 enum { AETHER_MEM_CPU = 0, AETHER_MEM_DEVICE = 1, AETHER_MEM_PINNED = 2 };
 enum { AETHER_FFT_C2C = 0, AETHER_FFT_R2C = 1, AETHER_FFT_C2R = 2 };
@@ -243,6 +247,7 @@ enum { AETH_FFT_BACKWARD = 0, AETH_FFT_FORWARD = 1, AETH_FFT_ORTHO = 2 };
 int aeth_fft_batch_plan(int prec, uint64_t count, const uint64_t* lengths, const uint64_t* in_offsets,
                         const uint64_t* out_offsets, void** plan);
 int aeth_fft_batch_exec(void* plan, const void* in, void* out, int forward, int norm, void* stream);
+/* Safe during a stream capture, like the seam's fft_destroy. */
 int aeth_fft_batch_destroy(void* plan);
 
 /* ------------------------------------------------------------------ serving */

@@ -134,6 +134,70 @@ def test_release_under_capture_is_deferred(seam):
     assert torch.all(x == 1.0)
 
 
+def test_fft_destroy_under_capture(seam):
+    """fft_destroy is safe during a capture (dsc/src/seam.h:48-50): destroying one plan while another
+    plan's execution is being captured (global capture mode, torch's default) neither breaks the
+    capture nor the replays; the destroyed plan's buffers are freed once the capture has ended."""
+    shp = (C.c_int64 * 2)(8, 1024)
+    ax = (C.c_int32 * 1)(1)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((8, 1024)) + 1j * rng.standard_normal((8, 1024))
+    xin = torch.from_numpy(x).to("cuda")
+    out = torch.zeros_like(xin)
+    side = torch.cuda.Stream()
+    s = side.cuda_stream
+    keep, doomed, doomed_real = _vp(), _vp(), _vp()
+    assert seam.fft_plan(0, 0, 1, 0, 1, 2, shp, 1, ax, s, C.byref(keep)) == 0, seam.last_error()
+    assert seam.fft_plan(0, 0, 1, 0, 1, 2, shp, 1, ax, s, C.byref(doomed)) == 0, seam.last_error()
+    big = (C.c_int64 * 1)(1 << 14)  # four-step plan: work arrays and twiddle tables of its own
+    ax0 = (C.c_int32 * 1)(0)
+    assert seam.fft_plan(0, 1, 1, 0, 1, 1, big, 1, ax0, s, C.byref(doomed_real)) == 0, seam.last_error()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        assert seam.fft_exec(keep, xin.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream) == 0
+        seam.fft_destroy(doomed)
+        seam.fft_destroy(doomed_real)
+    seam.drain()
+    ref = np.fft.fft(x, axis=1)
+    for _ in range(3):
+        out.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.max(np.abs(out.cpu().numpy() - ref)) <= 1e-10 * np.abs(ref).max()
+    seam.fft_destroy(keep)
+    seam.drain()
+
+
+def test_batch_plan_destroy_under_capture(dev):
+    L = _lib.lib()
+    lengths = (C.c_uint64 * 2)(64, 128)
+    offs = (C.c_uint64 * 2)(0, 64)
+    keep, doomed = _vp(), _vp()
+    assert L.aeth_fft_batch_plan(0, 2, lengths, offs, offs, C.byref(keep)) == 0
+    assert L.aeth_fft_batch_plan(0, 2, lengths, offs, offs, C.byref(doomed)) == 0
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(192) + 1j * rng.standard_normal(192)
+    xin = torch.from_numpy(x).to("cuda")
+    out = torch.zeros_like(xin)
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        st = torch.cuda.current_stream().cuda_stream
+        assert L.aeth_fft_batch_exec(keep, _vp(xin.data_ptr()), _vp(out.data_ptr()), 1, 0, _vp(st)) == 0
+        assert L.aeth_fft_batch_destroy(doomed) == 0
+    ref = np.concatenate([np.fft.fft(x[:64]), np.fft.fft(x[64:])])
+    for _ in range(3):
+        out.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.max(np.abs(out.cpu().numpy() - ref)) <= 1e-12 * np.abs(ref).max()
+    assert L.aeth_fft_batch_destroy(keep) == 0
+
+
 @pytest.mark.parametrize("shape,axes", [((8, 64), (1,)), ((64, 8), (0,)), ((16, 32), (0, 1)), ((4, 8, 16), (1,)),
                                         ((4, 8, 16), (0, 2)), ((1024,), (0,))])
 @pytest.mark.parametrize("norm,forward", [(0, 1), (0, 0), (2, 1)])
