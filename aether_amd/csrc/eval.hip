@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "../../include/aether_hip.h"
 #include "accum.hpp"
@@ -141,15 +143,53 @@ void check_sizes(int ncomp, uint64_t n, uint64_t pitch, const char* op)
     if ((uint64_t)ncomp * pitch > 0xFFFFFFFFull) fail(op, "ncomp*pitch exceeds the 32-bit offset_t span");
 }
 
+// Scratch of the allocation-free entry point (aeth_eval_cfg0_noalloc): the tile partials and the group
+// tree's levels of its sums, in a file-scope device array that each GPU holds from the library's first
+// use on it until the process ends. Every call on a GPU shares that GPU's copy; nothing in it has to
+// start at zero (each partial is written before it is read), and calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = 4u << 20;
+} // namespace
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current device's copy of `scratch`, looked up once per device with that
+// device current. In host code the array's name is a host-side stand-in and never a device address.
+void* scratch_on_current_device()
+{
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) fail("aeth_eval_cfg0_noalloc", "device ordinal beyond 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) fail("aeth_eval_cfg0_noalloc", "hipGetSymbolAddress returned no address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// `fixed` (may be NULL): caller-owned scratch of kNoAllocScratchBytes used instead of a stream-ordered
+// allocation; sizes whose partials do not fit are refused before anything is launched.
 template<class T, int M, bool SUM>
 void launch_cfg0(const T* a, const T* b, const T* c, T* d, int ncomp, uint64_t n, uint64_t pitch, T* sums,
-                 hipStream_t st)
+                 hipStream_t st, void* fixed = nullptr)
 {
     const uint32_t tiles = n ? (uint32_t)((n + kTile - 1) / kTile) : 1u;
     const uint32_t units = tiles * (uint32_t)ncomp;
     const bool vec = aligned16(a) && aligned16(b) && aligned16(c) && aligned16(d) && (pitch * sizeof(T)) % 16 == 0;
-    StreamScratch scratch(SUM ? tree_scratch_pairs(tiles, ncomp) * sizeof(acc::Pair<T>) : 0, st);
-    auto* part = static_cast<acc::Pair<T>*>(scratch.get());
+    const size_t need = SUM ? tree_scratch_pairs(tiles, ncomp) * sizeof(acc::Pair<T>) : 0;
+    if (fixed && need > kNoAllocScratchBytes)
+        fail("aeth_eval_cfg0_noalloc",
+             "the sums' partials need " + std::to_string(need) + " bytes, beyond the fixed scratch of " +
+                 std::to_string(kNoAllocScratchBytes) + " bytes",
+             need);
+    StreamScratch scratch(fixed ? 0 : need, st);
+    auto* part = static_cast<acc::Pair<T>*>(fixed ? fixed : scratch.get());
     const int grid = grid_for(units);
     const bool nt = streaming(4ull * ncomp * n * sizeof(T));
     if (vec && nt)
@@ -167,19 +207,19 @@ void launch_cfg0(const T* a, const T* b, const T* c, T* d, int ncomp, uint64_t n
 
 template<class T>
 void dispatch_cfg0(const void* a, const void* b, const void* c, void* d, int ncomp, uint64_t n, uint64_t pitch,
-                   int mode, void* sums, hipStream_t st)
+                   int mode, void* sums, hipStream_t st, void* fixed = nullptr)
 {
     auto A = static_cast<const T*>(a), B = static_cast<const T*>(b), Cc = static_cast<const T*>(c);
     auto D = static_cast<T*>(d);
     auto S = static_cast<T*>(sums);
     if (mode != AETH_SUM_NONE && !sums) fail("aeth_eval_cfg0", "sums is NULL");
     switch (mode) {
-    case AETH_SUM_NONE: launch_cfg0<T, acc::kNeumaier, false>(A, B, Cc, D, ncomp, n, pitch, S, st); break;
-    case AETH_SUM_NEUMAIER: launch_cfg0<T, acc::kNeumaier, true>(A, B, Cc, D, ncomp, n, pitch, S, st); break;
-    case AETH_SUM_PLAIN: launch_cfg0<T, acc::kPlain, true>(A, B, Cc, D, ncomp, n, pitch, S, st); break;
+    case AETH_SUM_NONE: launch_cfg0<T, acc::kNeumaier, false>(A, B, Cc, D, ncomp, n, pitch, S, st, fixed); break;
+    case AETH_SUM_NEUMAIER: launch_cfg0<T, acc::kNeumaier, true>(A, B, Cc, D, ncomp, n, pitch, S, st, fixed); break;
+    case AETH_SUM_PLAIN: launch_cfg0<T, acc::kPlain, true>(A, B, Cc, D, ncomp, n, pitch, S, st, fixed); break;
     case AETH_SUM_FLOATFLOAT:
         if constexpr (sizeof(T) == 4) {
-            launch_cfg0<T, acc::kFloatFloat, true>(A, B, Cc, D, ncomp, n, pitch, S, st);
+            launch_cfg0<T, acc::kFloatFloat, true>(A, B, Cc, D, ncomp, n, pitch, S, st, fixed);
             break;
         } else {
             fail("aeth_eval_cfg0", "float-float accumulation needs dtype F32", 0, 2);
@@ -321,6 +361,26 @@ extern "C" int aeth_eval_cfg0(int dtype, const void* a, const void* b, const voi
             dispatch_cfg0<float>(a, b, c, d, ncomp, n, pitch, sum_mode, sums, as_stream(stream));
         } else {
             fail("aeth_eval_cfg0", "unknown dtype");
+        }
+    });
+}
+
+// aeth_eval_cfg0 for callers that cannot allocate (code under graph capture, tight serving loops):
+// the same kernels and bits, with the sums' partials in the fixed per-device `scratch` instead of a
+// stream-ordered allocation. Calls no hipMalloc/hipFree; refuses sizes whose partials exceed the scratch.
+extern "C" int aeth_eval_cfg0_noalloc(int dtype, const void* a, const void* b, const void* c, void* d, int ncomp,
+                                      uint64_t n, uint64_t pitch, int sum_mode, void* sums, void* stream)
+{
+    return guarded([&] {
+        void* fixed = scratch_on_current_device();
+        if (dtype == AETH_F64) {
+            check_sizes<double>(ncomp, n, pitch, "aeth_eval_cfg0_noalloc");
+            dispatch_cfg0<double>(a, b, c, d, ncomp, n, pitch, sum_mode, sums, as_stream(stream), fixed);
+        } else if (dtype == AETH_F32) {
+            check_sizes<float>(ncomp, n, pitch, "aeth_eval_cfg0_noalloc");
+            dispatch_cfg0<float>(a, b, c, d, ncomp, n, pitch, sum_mode, sums, as_stream(stream), fixed);
+        } else {
+            fail("aeth_eval_cfg0_noalloc", "unknown dtype");
         }
     });
 }

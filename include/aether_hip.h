@@ -119,6 +119,18 @@ enum { AETH_SUM_NONE = -1, AETH_SUM_NEUMAIER = 0, AETH_SUM_PLAIN = 1, AETH_SUM_F
 int aeth_eval_cfg0(int dtype, const void* a, const void* b, const void* c, void* d, int ncomp, uint64_t n,
                    uint64_t pitch, int sum_mode, void* sums, void* stream);
 
+/* aeth_eval_cfg0 for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. No
+ * counterpart in the reference (which has no C entry for evaluation). Same
+ * kernels and bits as aeth_eval_cfg0; the sums' tile partials live in a fixed
+ * 4 MiB device array per GPU (held until the process ends) instead of a
+ * stream-ordered allocation, so the call makes no hipMalloc/hipFree. Sizes
+ * whose partials need more (about 2^28 samples of FP64 over 3 components)
+ * return 1. Every call on one GPU shares that array: calls that overlap on one
+ * GPU (different streams, threads) race. */
+int aeth_eval_cfg0_noalloc(int dtype, const void* a, const void* b, const void* c, void* d, int ncomp, uint64_t n,
+                           uint64_t pitch, int sum_mode, void* sums, void* stream);
+
 /* Generic lowering of `dest = expr` (expr/Assign.h:124-160,264-271): each
  * output component is a PROGRAM -- a postfix tape over the planes it reads.
  * The tape is evaluated in registers for every sample; all planes a program
