@@ -827,7 +827,8 @@ extern "C" int aeth_server_replay(void* server, const aeth_job* jobs, uint64_t c
 // The plain per-call form: one job, its own allocations, synchronous -- how a
 // caller of the reference runs a request (Array ctor allocates, fill, launch,
 // download, free).
-extern "C" int aeth_run_job_plain(const aeth_job* job, aeth_job_result* out, void* stream)
+namespace {
+int run_plain(const aeth_job* job, aeth_job_result* out, void* ws, uint64_t ws_bytes, void* stream)
 {
     return guarded([&] {
         validate(*job);
@@ -835,8 +836,16 @@ extern "C" int aeth_run_job_plain(const aeth_job* job, aeth_job_result* out, voi
         hipStream_t st = as_stream(stream);
         const size_t es = esize(s.dtype);
         std::vector<void*> allocs;
+        uint64_t ws_used = 0;
         auto galloc = [&](size_t b) {
             void* p = nullptr;
+            if (ws) { // carve from the caller's workspace (256-byte aligned pieces)
+                const uint64_t need = (b + 255) / 256 * 256;
+                if (ws_used + need > ws_bytes) fail("aeth_run_job_plain_ws", "workspace too small", ws_used + need);
+                p = static_cast<char*>(ws) + ws_used;
+                ws_used += need;
+                return p;
+            }
             check(hipMalloc(&p, b ? b : 16), "hipMalloc", b);
             allocs.push_back(p);
             return p;
@@ -912,4 +921,17 @@ extern "C" int aeth_run_job_plain(const aeth_job* job, aeth_job_result* out, voi
             out->done_time = now_s();
         }
     });
+}
+} // namespace
+
+extern "C" int aeth_run_job_plain(const aeth_job* job, aeth_job_result* out, void* stream)
+{
+    return run_plain(job, out, nullptr, 0, stream);
+}
+
+extern "C" int aeth_run_job_plain_ws(const aeth_job* job, aeth_job_result* out, void* workspace,
+                                     uint64_t workspace_bytes, void* stream)
+{
+    if (!workspace) return guarded([] { fail("aeth_run_job_plain_ws", "workspace is NULL"); });
+    return run_plain(job, out, workspace, workspace_bytes, stream);
 }

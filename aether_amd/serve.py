@@ -48,6 +48,7 @@ def _bind():
         "aeth_server_destroy": [C.c_void_p],
         "aeth_server_replay": [C.c_void_p, C.POINTER(Job), C.c_uint64, C.POINTER(JobResult)],
         "aeth_run_job_plain": [C.POINTER(Job), C.POINTER(JobResult), C.c_void_p],
+        "aeth_run_job_plain_ws": [C.POINTER(Job), C.POINTER(JobResult), C.c_void_p, C.c_uint64, C.c_void_p],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -92,10 +93,11 @@ def make_mix(rate: float, duration: float, seed: int = 5):
     return jobs
 
 
-def job_bytes(j: Job) -> float:
-    """Algorithmic bytes of a job's hot-path work: eval 3 loads + 1 store per
+def job_bytes(j: Job, fp32_compensated: bool = False) -> float:
+    """Algorithmic bytes of a job's hot-path work in the precision the server executes it in (a server
+    in the FP32-compensated form runs every job in FP32 storage): eval 3 loads + 1 store per
     (component, sample); sampling: bytes written; FFT: read and write once per point."""
-    es = 8 if j.dtype == F64 else 4
+    es = 8 if (j.dtype == F64 and not fp32_compensated) else 4
     if j.kind == JOB_EVAL:
         return 3 * j.n * 4 * es
     if j.kind == JOB_SAMPLE:
@@ -159,4 +161,12 @@ def run_plain(job: Job, stream: int = 0) -> JobResult:
     _bind()
     r = JobResult()
     call("aeth_run_job_plain", C.byref(job), C.byref(r), stream)
+    return r
+
+
+def run_plain_ws(job: Job, workspace: int, workspace_bytes: int, stream: int = 0) -> JobResult:
+    """The plain per-call form with the job's arrays in a caller-owned device workspace."""
+    _bind()
+    r = JobResult()
+    call("aeth_run_job_plain_ws", C.byref(job), C.byref(r), workspace, workspace_bytes, stream)
     return r

@@ -9,9 +9,11 @@ step: 3 components x 2^27 samples x 32 B (3 loads + 1 store of a double).
 
 Prints ONE JSON line. `roofline.achieved` is the algorithmic bytes of one
 launch over its average duration, timed with HIP events on the launch
-stream; `cpu_baseline` runs the reference's own OpenMP CPU path (built from
-its sources into oracle/_ref) -- or the oracle's restatement when that build
-is absent -- on a bounded sample of the same workload on this host.
+stream; `roofline.stream_ceiling` is the same traffic (3 reads, 1 write per
+element) with trivial arithmetic on the same arrays, measured in the same run.
+`cpu_baseline` times the oracle's restatement of the reference's OpenMP CPU
+path (oracle/*.c) on a bounded sample of the same workload on this host's
+cores; the reference itself never runs here.
 """
 from __future__ import annotations
 
@@ -33,7 +35,7 @@ MEASURED_TRAFFIC_BYTES = {27: 12_890_958_603}
 
 
 # secondary workloads that run on their own (no arguments)
-SINGLE_WORKLOADS = {"philox": "wl_philox", "fp32": "wl_fp32", "trig": "wl_trig", "fft": "wl_fft",
+SINGLE_WORKLOADS = {"philox": "wl_philox", "fp32": "wl_fp32", "trig": "wl_trig", "fft": "wl_fft", "stream": "wl_stream_alone",
                     "accum": "wl_accum"}
 
 
@@ -81,38 +83,73 @@ def copy_calibration(steps: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "launches": steps}))
 
 
+def host_info() -> dict:
+    """The host the CPU baseline runs on: usable cores (affinity, capped by a cgroup CPU quota), model,
+    glibc."""
+    import math
+    import platform
+
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = max(1, math.ceil(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    model = platform.processor()
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"affinity_cpus": affinity, "cgroup_cpu_quota": quota,
+            "usable_cores": min(affinity, quota) if quota else affinity, "cpu_model": model,
+            "glibc": os.confstr("CS_GNU_LIBC_VERSION"), "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(log2n: int) -> dict:
-    """The reference's fused loop + its canonical compensated sums, on the host cores."""
-    n = 1 << log2n
-    cores = min(16, os.cpu_count() or 1)
-    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
-    ref = os.path.join(ROOT, "oracle", "_ref", "aether_ref")
-    if os.path.exists(ref):
-        out = subprocess.run([ref, "bench", str(n), "5"], capture_output=True, text=True, env=env, timeout=600,
-                             check=True).stdout
-        r = json.loads(out)
-        ms = r["median_ms"]
-        return {"value": round(3 * n * 32 / (ms * 1e-3) / 1e9, 3), "unit": "GB/s", "cores": r["threads"],
-                "kind": "reference",
-                "sample": f"2^{log2n} samples x 3 components, median of 5 after 1 warm-up; eval + 3 tree sums "
-                          f"(eval alone {r['eval_median_ms']:.1f} ms of {ms:.1f} ms)"}
+    """The reference's CPU path restated (oracle/*.c: its per-sample scalar loop under an OpenMP static
+    schedule, glibc sin/exp, as the reference evaluates this expression on the host), plus per-component
+    compensated sums (per-thread Neumaier blocks combined in order), timed on this host: all usable cores
+    and one thread, the bit-exact build (-ffp-contract=off) and the default-contraction build. Median of 5
+    after a warm-up; fills, allocation and first touch are outside the timed region."""
+    import ctypes as C
+
     import numpy as np
 
     from oracle import pyoracle as O
 
-    a = O.fill_array_uniform(42, 3, n, n)
-    b = O.fill_array_uniform(43, 3, n, n)
-    c = O.fill_array_uniform(44, 3, n, n)
-    ts = []
-    for _ in range(4):
-        t0 = time.perf_counter()
-        d = O.eval_cfg0(a, b, c, 3, n, n, mathmode=0, nthreads=cores)
-        for k in range(3):
-            O.tree_sum32(np.ascontiguousarray(d[k * n:(k + 1) * n]))
-        ts.append(time.perf_counter() - t0)
-    ms = sorted(ts[1:])[1] * 1e3
-    return {"value": round(3 * n * 32 / (ms * 1e-3) / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"2^{log2n} samples x 3 components, median of 3 after 1 warm-up"}
+    n = 1 << log2n
+    info = host_info()
+    cores = info["usable_cores"]
+    a, b, c = (O.fill_array_uniform(s, 3, n, n) for s in (42, 43, 44))
+    d = np.zeros_like(a)
+    runs = {}
+    for build, L in (("contract_off", O.lib()), ("default_contraction", O.lib_fast())):
+        for nt in sorted({cores, 1}, reverse=True):
+            ev, sm = [], []
+            for rep in range(6):
+                t0 = time.perf_counter()
+                L.orc_eval_cfg0_f64(a, b, c, d, 3, n, n, 0, nt)
+                t1 = time.perf_counter()
+                for k in range(3):
+                    h, cc = C.c_double(), C.c_double()
+                    L.orc_block_sum_f64(d[k * n:(k + 1) * n], n, nt, C.byref(h), C.byref(cc))
+                t2 = time.perf_counter()
+                if rep:
+                    ev.append(t1 - t0)
+                    sm.append(t2 - t1)
+            e_ms, s_ms = float(np.median(ev)) * 1e3, float(np.median(sm)) * 1e3
+            runs[f"{build}_{nt}t"] = {"threads": nt, "eval_ms": round(e_ms, 2), "sums_ms": round(s_ms, 2),
+                                     "GB/s": round(3 * n * 32 / ((e_ms + s_ms) * 1e-3) / 1e9, 3)}
+    head = runs[f"contract_off_{cores}t"]
+    return {"value": head["GB/s"], "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"2^{log2n} samples x 3 components of the same workload (eval + 3 compensated sums), "
+                      f"bit-exact build, {cores} threads; median of 5 after 1 warm-up",
+            "runs": runs, "host": info}
 
 
 def _time(fn, reps: int = 10, warm: int = 2) -> float:
@@ -155,6 +192,52 @@ def wl_tape(a, b, c, d, n) -> dict:
     out["no_sums"] = {"contract_shape_ms": round(ms0, 4), "tuned_shape": [shape["tile"], shape["vector_bytes"], shape["waves_per_simd"]],
                       "tuned_ms": round(ms1, 4), "tuned_GB/s": _gbs(3 * n * 32, ms1)}
     return out
+
+
+def wl_stream(a, b, c, d, n) -> dict:
+    """The device's 3-read/1-write ceiling on the headline's own arrays: d = a + b + c (three loads and a
+    store of a double per element, one add pair: no transcendental) through the runtime-compiled tape
+    kernel at the contract shape and at the shape the autotuner keeps; and the seam's copy kernel over
+    one array (read + write). The fused kernel's roofline fraction is read against these."""
+    import ctypes as C
+
+    import torch
+
+    from aether_amd import _lib
+    from aether_amd import expr as E
+
+    x = a + b + c
+    out = {"bytes_per_launch": 3 * n * 32}
+    ms0 = _time(lambda: E.assign(d, x))
+    shape = E.tune(d, x)
+    ms1 = _time(lambda: E.assign(d, x))
+    out["triad_contract_shape"] = {"ms": round(ms0, 4), "GB/s": _gbs(3 * n * 32, ms0)}
+    out["triad_tuned"] = {"ms": round(ms1, 4), "GB/s": _gbs(3 * n * 32, ms1),
+                          "shape": [shape["tile"], shape["vector_bytes"], shape["waves_per_simd"]]}
+    L = _lib.lib()
+    L.aether_memory_v1.restype = C.c_void_p
+    table = L.aether_memory_v1()
+    fp = C.cast(table + 8 + 8 * 8, C.POINTER(C.c_void_p)).contents.value  # field `copy`
+    copy = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t,
+                       C.c_void_p)(fp)
+    nb = a.data.numel() * 8
+    st = torch.cuda.current_stream().cuda_stream
+    msc = _time(lambda: copy(1, 0, d.ptr(), 1, 0, a.ptr(), nb, st))
+    out["copy_read_write"] = {"ms": round(msc, 4), "GB/s": _gbs(2 * nb, msc)}
+    out["ceiling_GB/s"] = max(out["triad_contract_shape"]["GB/s"], out["triad_tuned"]["GB/s"])
+    return out
+
+
+def wl_stream_alone(log2n: int = 27) -> dict:
+    import torch
+
+    import aether_amd as ae
+
+    n = 1 << log2n
+    a, b, c, d = (ae.Array(n, 3, torch.float64) for _ in range(4))
+    for arr, s in ((a, 42), (b, 43), (c, 44)):
+        ae.array_fill_uniform(arr, s, -1.0, 1.0)
+    return wl_stream(a, b, c, d, n)
 
 
 def wl_vec3(a, b, c, n) -> dict:
@@ -439,24 +522,30 @@ def fp32_error_vs_fp64(jobs, r32, per_kind: int = 12) -> dict:
             for k, v in errs.items()}
 
 
-def wl_serve(duration: float, rate: float = 20000.0, plain_seconds: float = 6.0) -> dict:
-    """Served mixed job stream (Poisson arrivals): continuous batching vs the plain per-call form.
+def wl_serve(duration: float, rate: float = 20000.0, plain_seconds: float = 4.0) -> dict:
+    """Served mixed job stream (Poisson arrivals): continuous batching against the plain per-call form.
 
     Served: open-loop replay of the stream into aeth_server (64 GB budget, 4 streams), once with each job's
-    requested precision and once in the FP32-compensated form. Plain: the same jobs, one at a time through
-    aeth_run_job_plain (own allocations, synchronous), closed loop for `plain_seconds`."""
+    requested precision and once in the FP32-compensated form; `window_s` is the length of the stream
+    replayed (the full configuration is 60 s: --serve-seconds 60). Capacity: one second's worth of the same
+    stream submitted at once (closed loop). Plain: the same jobs one at a time, synchronous, closed loop for
+    `plain_seconds` -- with per-call hipMalloc/hipFree (aeth_run_job_plain) and with the arrays in a reused
+    workspace (aeth_run_job_plain_ws), in the same two precisions. Algorithmic bytes count the precision a
+    job is executed in."""
     import time as _t
 
     import numpy as np
+    import torch
 
     from aether_amd import serve as S
 
     jobs = S.make_mix(rate, duration, seed=5)
-    nbytes = sum(S.job_bytes(j) for j in jobs)
-    out = {"offered_jobs_per_s": rate, "duration_s": duration, "jobs": len(jobs),
-           "offered_GB_per_s": round(nbytes / duration / 1e9, 1)}
+    nb = {False: sum(S.job_bytes(j) for j in jobs), True: sum(S.job_bytes(j, True) for j in jobs)}
+    out = {"offered_jobs_per_s": rate, "window_s": duration, "full_config_window_s": 60.0, "jobs": len(jobs),
+           "offered_GB_per_s": round(nb[False] / duration / 1e9, 1),
+           "offered_GB_per_s_fp32_compensated": round(nb[True] / duration / 1e9, 1)}
 
-    def summarize(res, label):
+    def summarize(res, label, nbytes):
         sub = np.array([r.submit_time for r in res])
         done = np.array([r.done_time for r in res])
         lat = (done - sub) * 1e3
@@ -468,36 +557,57 @@ def wl_serve(duration: float, rate: float = 20000.0, plain_seconds: float = 6.0)
         return res
 
     with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256) as srv:
-        r64 = summarize(srv.replay(jobs), "served")
+        summarize(srv.replay(jobs), "served", nb[False])
         out["served"]["batches"] = int(srv.stats().batches)
     with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True) as srv:
-        r32 = summarize(srv.replay(jobs), "served_fp32_compensated")
+        r32 = summarize(srv.replay(jobs), "served_fp32_compensated", nb[True])
         out["served_fp32_compensated"]["batches"] = int(srv.stats().batches)
     out["fp32_compensated_error_vs_fp64"] = fp32_error_vs_fp64(jobs, r32)
-    # capacity: the same kind of stream submitted all at once (closed loop), FP32-compensated form
-    cap_jobs = S.make_mix(rate, min(duration, 1.0), seed=6)
+    cap_jobs = S.make_mix(rate, 1.0, seed=6)
     for j in cap_jobs:
         j.arrival = 0.0
+    cap_nb = {f: sum(S.job_bytes(j, f) for j in cap_jobs) for f in (False, True)}
     with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True) as srv:
-        summarize(srv.replay(cap_jobs), "capacity_fp32_compensated_closed_loop")
+        summarize(srv.replay(cap_jobs), "capacity_fp32_compensated_closed_loop", cap_nb[True])
     with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256) as srv:
-        summarize(srv.replay(cap_jobs), "capacity_closed_loop")
-    # plain per-call form, closed loop over the head of the same stream
-    t0 = _t.perf_counter()
-    done = 0
-    lat = []
-    for j in jobs:
-        r = S.run_plain(j)
-        lat.append((r.done_time - r.submit_time) * 1e3)
-        done += 1
-        if _t.perf_counter() - t0 > plain_seconds:
-            break
-    el = _t.perf_counter() - t0
-    out["plain_per_call"] = {"jobs_per_s": round(done / el, 1), "jobs_run": done,
-                             "p50_ms_service": round(float(np.percentile(lat, 50)), 3)}
-    out["served_fp32_compensated_vs_plain"] = round(out["served_fp32_compensated"]["jobs_per_s"] /
-                                                    out["plain_per_call"]["jobs_per_s"], 2)
-    out["served_vs_plain"] = round(out["served"]["jobs_per_s"] / out["plain_per_call"]["jobs_per_s"], 2)
+        summarize(srv.replay(cap_jobs), "capacity_closed_loop", cap_nb[False])
+
+    # plain per-call forms, closed loop over the head of the capacity stream
+    ws_bytes = 12 * (1 << 24) * 8 + (64 << 20)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+
+    def plain(label, fp32, with_ws):
+        t0 = _t.perf_counter()
+        done, lat = 0, []
+        for j in cap_jobs:
+            job = S.Job.from_buffer_copy(j)
+            if fp32:
+                job.dtype = S.F32
+            r = S.run_plain_ws(job, ws.data_ptr(), ws_bytes) if with_ws else S.run_plain(job)
+            lat.append((r.done_time - r.submit_time) * 1e3)
+            done += 1
+            if _t.perf_counter() - t0 > plain_seconds:
+                break
+        el = _t.perf_counter() - t0
+        out[label] = {"jobs_per_s": round(done / el, 1), "jobs_run": done,
+                      "p50_ms_service": round(float(np.percentile(lat, 50)), 3)}
+
+    plain("plain_per_call", False, False)
+    plain("plain_per_call_reused_workspace", False, True)
+    plain("plain_per_call_fp32_compensated", True, False)
+    plain("plain_per_call_reused_workspace_fp32_compensated", True, True)
+    del ws
+    cap32 = out["capacity_fp32_compensated_closed_loop"]["jobs_per_s"]
+    cap64 = out["capacity_closed_loop"]["jobs_per_s"]
+    out["capacity_ratio_vs_plain"] = {
+        "fp32_compensated_vs_plain_reused_workspace": round(
+            cap32 / out["plain_per_call_reused_workspace_fp32_compensated"]["jobs_per_s"], 2),
+        "fp32_compensated_vs_plain_per_call": round(cap32 / out["plain_per_call_fp32_compensated"]["jobs_per_s"], 2),
+        "requested_precision_vs_plain_reused_workspace": round(
+            cap64 / out["plain_per_call_reused_workspace"]["jobs_per_s"], 2),
+        "fp32_compensated_vs_requested_precision_plain_reused_workspace": round(
+            cap32 / out["plain_per_call_reused_workspace"]["jobs_per_s"], 2)}
     return out
 
 
@@ -563,6 +673,10 @@ def main():
                       "avg": round(avg_launch_ms, 4)},
         "sums": [s[k, 0].item() + s[k, 1].item() for k in range(3)],
     }
+    # the device's own 3-read/1-write ceiling on the same arrays, beside the peak-based fraction
+    stream = wl_stream(a, b, c, d, n)
+    line["roofline"]["stream_ceiling"] = stream
+    line["roofline"]["frac_of_stream_ceiling"] = round(achieved / stream["ceiling_GB/s"], 4)
     if not args.no_extra:
         extra = {"fused_fp64_via_tape": wl_tape(a, b, c, d, n), "vec3_geometric_f64": wl_vec3(a, b, c, n)}
         del a, b, c, d

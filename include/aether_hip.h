@@ -321,6 +321,12 @@ int aeth_server_replay(void* server, const aeth_job* jobs, uint64_t count, aeth_
  * call entries above, synchronous (how a caller of the reference serves a
  * request). */
 int aeth_run_job_plain(const aeth_job* job, aeth_job_result* out, void* stream);
+/* The same, with the job's arrays carved from a caller-owned device workspace
+ * (256-byte aligned pieces; an EVAL job of n samples needs about 12*n*esize +
+ * 1 MiB) instead of hipMalloc/hipFree per call: the plain form without its
+ * allocation cost (an FFT job still builds its plan per call). */
+int aeth_run_job_plain_ws(const aeth_job* job, aeth_job_result* out, void* workspace, uint64_t workspace_bytes,
+                          void* stream);
 
 /* ------------------------------------------------------------------ accumulation */
 /* The device primitives are in include/aether/device/accum.hpp for HIP kernels of a caller's own;

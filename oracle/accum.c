@@ -12,6 +12,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "oracle.h"
 
@@ -104,6 +107,43 @@ float orc_tree_sum32_f32(const float* x, uint64_t n)
     for (uint64_t i = 0; i < n; ++i) orc_neumaier_add_f32(&h[i % 32], &c[i % 32], x[i]);
     reduce32_f(h, c, &oh, &oc);
     return oh + oc;
+}
+
+/* The host's parallel compensated sum over samples, as a caller of the reference writes it with its
+ * per-block primitives: each OpenMP thread walks one contiguous block (static schedule) with
+ * neumaierAdd_ (accum/atomic.h:142-149, the block commit of accum/run.h:345-410), and the block pairs
+ * are combined with pairCombine_ (accum/atomic.h:157-164) in block order. The result depends on the
+ * thread count (the reference's reduction order is a ULP contract, accum/run.h:34-37). Used to time
+ * the CPU baseline; `nthreads` <= 0 takes OpenMP's default. */
+void orc_block_sum_f64(const double* x, uint64_t n, int nthreads, double* oh, double* oc)
+{
+#ifdef _OPENMP
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+    const int nt = 1;
+    (void)nthreads;
+#endif
+    double* bh = calloc((size_t)nt, sizeof(double));
+    double* bc = calloc((size_t)nt, sizeof(double));
+#pragma omp parallel num_threads(nt)
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num(), m = omp_get_num_threads();
+#else
+        const int t = 0, m = 1;
+#endif
+        const uint64_t lo = n * (uint64_t)t / (uint64_t)m, hi = n * (uint64_t)(t + 1) / (uint64_t)m;
+        double h = 0.0, c = 0.0;
+        for (uint64_t i = lo; i < hi; ++i) orc_neumaier_add_f64(&h, &c, x[i]);
+        bh[t] = h;
+        bc[t] = c;
+    }
+    double h = 0.0, c = 0.0;
+    for (int t = 0; t < nt; ++t) orc_pair_combine_f64(h, c, bh[t], bc[t], &h, &c);
+    free(bh);
+    free(bc);
+    *oh = h;
+    *oc = c;
 }
 
 long double orc_sum_ld(const double* x, uint64_t n)

@@ -130,6 +130,7 @@ void orc_pair_combine_f32(float ah, float ac, float bh, float bc, float* oh, flo
  * (tests/test_CompensatedSum_common.h:71-80 -- the reference's canonical form). */
 double orc_tree_sum32_f64(const double* x, uint64_t n);
 float orc_tree_sum32_f32(const float* x, uint64_t n);
+void orc_block_sum_f64(const double* x, uint64_t n, int nthreads, double* oh, double* oc);
 /* The fixed, size-only reduction shape the device kernels use (restated
  * here so the device result can be checked bit for bit):
  *   tile = 2048 samples; lane l in 0..255 walks its 16-byte vectors in

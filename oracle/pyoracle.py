@@ -22,7 +22,10 @@ _pd = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _pf = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _pu = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 
+_FAST_PATH = os.path.join(_HERE, "liboracle_fast.so")
+
 _lib = None
+_lib_fast = None
 
 
 def build() -> None:
@@ -34,57 +37,72 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
-        sig = {
-            "orc_philox4x32_10": (None, [C.POINTER(_u32), _u32, _u32, C.POINTER(_u32)]),
-            "orc_random_bits64": (_u64, [_u64, _u32, _u64]),
-            "orc_uniform01_f64": (_f64, [_u64, _u32, _u64]),
-            "orc_uniform01_f32": (_f32, [_u64, _u32, _u64]),
-            "orc_normal_f64": (_f64, [_u64, _u32, _u64, _i32]),
-            "orc_normal_f32": (_f32, [_u64, _u32, _u64, _i32]),
-            "orc_splitmix64": (_u64, [_u64]),
-            "orc_generator_key": (_u64, [_u64, _u64]),
-            "orc_substream_key": (_u64, [_u64, _u64]),
-            "orc_fill_bits32": (None, [_u64, _u64, _u64, _pu]),
-            "orc_fill_uniform_f64": (None, [_u64, _u64, _u64, _pd]),
-            "orc_fill_normal_f64": (None, [_u64, _u64, _u64, _i32, _pd]),
-            "orc_fill_uniform_f32": (None, [_u64, _u64, _u64, _pf]),
-            "orc_fill_normal_f32": (None, [_u64, _u64, _u64, _i32, _pf]),
-            "orc_fill_array_uniform_f64": (None, [_u64, _f64, _f64, _i32, _u64, _u64, _pd]),
-            "orc_fill_array_uniform_f32": (None, [_u64, _f32, _f32, _i32, _u64, _u64, _pf]),
-            "orc_fill_array_normal_f64": (None, [_u64, _f64, _f64, _i32, _u64, _u64, _i32, _pd]),
-            "orc_fill_array_normal_f32": (None, [_u64, _f32, _f32, _i32, _u64, _u64, _i32, _pf]),
-            "orc_math_f64": (None, [_i32, _i32, _u64, _pd, _pd]),
-            "orc_math_f32": (None, [_i32, _i32, _u64, _pf, _pf]),
-            "orc_math2_f64": (None, [_i32, _u64, _pd, _pd, _pd]),
-            "orc_math2_f32": (None, [_i32, _u64, _pf, _pf, _pf]),
-            "orc_eval_cfg0_f64": (None, [_pd, _pd, _pd, _pd, _i32, _u64, _u64, _i32, _i32]),
-            "orc_eval_cfg0_f32": (None, [_pf, _pf, _pf, _pf, _i32, _u64, _u64, _i32, _i32]),
-            "orc_eval_vec3geo_f64": (None, [_pd, _pd, _pd, _pd, _u64, _u64, _i32]),
-            "orc_tree_sum32_f64": (_f64, [_pd, _u64]),
-            "orc_tree_sum32_f32": (_f32, [_pf, _u64]),
-            "orc_device_sum_f64": (None, [_pd, _u64, C.POINTER(_f64), C.POINTER(_f64)]),
-            "orc_device_sumsq_f64": (None, [_pd, _u64, C.POINTER(_f64), C.POINTER(_f64)]),
-            "orc_device_sum_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
-            "orc_device_sumsq_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
-            "orc_device_sum_ff_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
-            "orc_device_sumsq_ff_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
-            "orc_device_sum_plain_f32": (None, [_pf, _u64, C.POINTER(_f32)]),
-            "orc_device_sum_plain_f64": (None, [_pd, _u64, C.POINTER(_f64)]),
-            "orc_device_sumsq_plain_f64": (None, [_pd, _u64, C.POINTER(_f64)]),
-            "orc_device_sumsq_plain_f32": (None, [_pf, _u64, C.POINTER(_f32)]),
-            "orc_neumaier_add_f64": (None, [C.POINTER(_f64), C.POINTER(_f64), _f64]),
-            "orc_pair_combine_f64": (None, [_f64, _f64, _f64, _f64, C.POINTER(_f64), C.POINTER(_f64)]),
-            "orc_neumaier_add_f32": (None, [C.POINTER(_f32), C.POINTER(_f32), _f32]),
-            "orc_pair_combine_f32": (None, [_f32, _f32, _f32, _f32, C.POINTER(_f32), C.POINTER(_f32)]),
-            "orc_dft_c128": (None, [_pd, _pd, _u64, _i32, _f64]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = L
+        _lib = _bind(C.CDLL(_LIB_PATH))
     return _lib
+
+
+def lib_fast():
+    """The same restatement built with the compiler's default contraction (not bit-exact): the
+    performance variant of the CPU baseline only."""
+    global _lib_fast
+    if _lib_fast is None:
+        if not os.path.exists(_FAST_PATH):
+            build()
+        _lib_fast = _bind(C.CDLL(_FAST_PATH))
+    return _lib_fast
+
+
+def _bind(L):
+    sig = {
+        "orc_philox4x32_10": (None, [C.POINTER(_u32), _u32, _u32, C.POINTER(_u32)]),
+        "orc_random_bits64": (_u64, [_u64, _u32, _u64]),
+        "orc_uniform01_f64": (_f64, [_u64, _u32, _u64]),
+        "orc_uniform01_f32": (_f32, [_u64, _u32, _u64]),
+        "orc_normal_f64": (_f64, [_u64, _u32, _u64, _i32]),
+        "orc_normal_f32": (_f32, [_u64, _u32, _u64, _i32]),
+        "orc_splitmix64": (_u64, [_u64]),
+        "orc_generator_key": (_u64, [_u64, _u64]),
+        "orc_substream_key": (_u64, [_u64, _u64]),
+        "orc_fill_bits32": (None, [_u64, _u64, _u64, _pu]),
+        "orc_fill_uniform_f64": (None, [_u64, _u64, _u64, _pd]),
+        "orc_fill_normal_f64": (None, [_u64, _u64, _u64, _i32, _pd]),
+        "orc_fill_uniform_f32": (None, [_u64, _u64, _u64, _pf]),
+        "orc_fill_normal_f32": (None, [_u64, _u64, _u64, _i32, _pf]),
+        "orc_fill_array_uniform_f64": (None, [_u64, _f64, _f64, _i32, _u64, _u64, _pd]),
+        "orc_fill_array_uniform_f32": (None, [_u64, _f32, _f32, _i32, _u64, _u64, _pf]),
+        "orc_fill_array_normal_f64": (None, [_u64, _f64, _f64, _i32, _u64, _u64, _i32, _pd]),
+        "orc_fill_array_normal_f32": (None, [_u64, _f32, _f32, _i32, _u64, _u64, _i32, _pf]),
+        "orc_math_f64": (None, [_i32, _i32, _u64, _pd, _pd]),
+        "orc_math_f32": (None, [_i32, _i32, _u64, _pf, _pf]),
+        "orc_math2_f64": (None, [_i32, _u64, _pd, _pd, _pd]),
+        "orc_math2_f32": (None, [_i32, _u64, _pf, _pf, _pf]),
+        "orc_eval_cfg0_f64": (None, [_pd, _pd, _pd, _pd, _i32, _u64, _u64, _i32, _i32]),
+        "orc_eval_cfg0_f32": (None, [_pf, _pf, _pf, _pf, _i32, _u64, _u64, _i32, _i32]),
+        "orc_eval_vec3geo_f64": (None, [_pd, _pd, _pd, _pd, _u64, _u64, _i32]),
+        "orc_tree_sum32_f64": (_f64, [_pd, _u64]),
+        "orc_tree_sum32_f32": (_f32, [_pf, _u64]),
+        "orc_device_sum_f64": (None, [_pd, _u64, C.POINTER(_f64), C.POINTER(_f64)]),
+        "orc_device_sumsq_f64": (None, [_pd, _u64, C.POINTER(_f64), C.POINTER(_f64)]),
+        "orc_device_sum_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
+        "orc_device_sumsq_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
+        "orc_device_sum_ff_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
+        "orc_device_sumsq_ff_f32": (None, [_pf, _u64, C.POINTER(_f32), C.POINTER(_f32)]),
+        "orc_device_sum_plain_f32": (None, [_pf, _u64, C.POINTER(_f32)]),
+        "orc_device_sum_plain_f64": (None, [_pd, _u64, C.POINTER(_f64)]),
+        "orc_device_sumsq_plain_f64": (None, [_pd, _u64, C.POINTER(_f64)]),
+        "orc_device_sumsq_plain_f32": (None, [_pf, _u64, C.POINTER(_f32)]),
+        "orc_neumaier_add_f64": (None, [C.POINTER(_f64), C.POINTER(_f64), _f64]),
+        "orc_pair_combine_f64": (None, [_f64, _f64, _f64, _f64, C.POINTER(_f64), C.POINTER(_f64)]),
+        "orc_neumaier_add_f32": (None, [C.POINTER(_f32), C.POINTER(_f32), _f32]),
+        "orc_pair_combine_f32": (None, [_f32, _f32, _f32, _f32, C.POINTER(_f32), C.POINTER(_f32)]),
+        "orc_dft_c128": (None, [_pd, _pd, _u64, _i32, _f64]),
+        "orc_block_sum_f64": (None, [_pd, _u64, _i32, C.POINTER(_f64), C.POINTER(_f64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
 
 
 # ---------------------------------------------------------------- RNG
@@ -233,6 +251,14 @@ def tree_sum32(x):
     if x.dtype == np.float64:
         return lib().orc_tree_sum32_f64(x, x.size)
     return lib().orc_tree_sum32_f32(x, x.size)
+
+
+def block_sum(x, nthreads=0, fast=False):
+    """Per-thread Neumaier blocks combined in block order (the host's parallel compensated sum)."""
+    x = np.ascontiguousarray(x, np.float64)
+    h, c = _f64(), _f64()
+    (lib_fast() if fast else lib()).orc_block_sum_f64(x, x.size, nthreads, C.byref(h), C.byref(c))
+    return h.value, c.value
 
 
 def device_sum(x, square=False, mode="neumaier"):

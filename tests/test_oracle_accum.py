@@ -112,3 +112,22 @@ def test_device_shape_depends_on_size_only(orc):
     # the same values give the same bits whatever surrounds them (no launch dependence)
     x = np.random.default_rng(1).uniform(-1, 1, 50_000)
     assert orc.device_sum(x) == orc.device_sum(x.copy())
+
+
+def test_block_sum_matches_fsum_and_the_serial_walk(orc):
+    """The host's parallel compensated sum (per-thread Neumaier blocks combined in block order): one
+    block is the serial Neumaier walk exactly; any block count stays within 2 ulp of math.fsum."""
+    import math
+
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(300_001) * np.exp(rng.uniform(-20, 20, 300_001))
+    h1, c1 = orc.block_sum(x, 1)
+    h, c = 0.0, 0.0
+    for t in x[:5000]:
+        h, c = orc.neumaier_add(h, c, t)
+    hs, cs = orc.block_sum(x[:5000], 1)
+    assert (hs, cs) == (h, c)
+    exact = math.fsum(x)
+    for nt in (1, 2, 3, 8):
+        hh, cc = orc.block_sum(x, nt)
+        assert abs((hh + cc) - exact) <= 2 * math.ulp(exact)
