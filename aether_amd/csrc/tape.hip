@@ -19,6 +19,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -514,63 +515,24 @@ std::string tape_key(int dtype, int mode, int np, const aeth_program* P, bool nt
     return k;
 }
 
-std::string gen_program(const aeth_program& P, bool dbl, uint32_t vb)
+// The tape as straight-line code over one sample: `operand(q)` names loaded plane q's value, `sample` the
+// sample index (for random leaves). Appends the statements to `b` and returns the name of the result.
+std::string gen_body(const aeth_program& P, bool dbl, const std::function<std::string(int)>& operand,
+                     const std::string& sample, const std::string& ind, std::string& b)
 {
-    // 16-byte accesses go through the same vector load/store as the ahead-of-time kernels; narrower ones
-    // are element loops the compiler merges
-    const bool v16 = vb == 16;
-    const char* T = dbl ? "double" : "float";
-    std::string b;
-    b += "      " + std::string(T) + "* outp = (" + T + "*)A.plane[pi][0];\n";
-    for (int q = 0; q < P.n_loads; ++q) {
-        const std::string Q = std::to_string(q);
-        b += "      const " + std::string(T) + "* L" + Q + " = (const " + T + "*)A.plane[pi][" +
-             std::to_string(1 + q) + "];\n";
-    }
-    // Programs with few loads issue every load of the tile first (all rounds), then the arithmetic and
-    // the stores: as many 16-byte loads in flight per lane as the ahead-of-time kernels. Programs with
-    // many loads go round by round (one round's planes in registers).
-    const bool upfront = P.n_loads <= 6;
-    auto load_round = [&](const char* ind, const char* slot) {
-        std::string t;
-        t += std::string(ind) + "if (full) {\n";
-        for (int q = 0; q < P.n_loads; ++q)
-            t += v16 ? std::string(ind) + "  aeth::units::vload<NT>(L" + std::to_string(q) + " + i0, x" +
-                           std::to_string(q) + "[" + slot + "]);\n"
-                     : std::string(ind) + "  for (int k = 0; k < VN; ++k) x" + std::to_string(q) + "[" + slot +
-                           "][k] = L" + std::to_string(q) + "[i0 + k];\n";
-        t += std::string(ind) + "} else {\n";
-        for (int q = 0; q < P.n_loads; ++q)
-            t += std::string(ind) + "  for (int k = 0; k < VN; ++k) x" + std::to_string(q) + "[" + slot +
-                 "][k] = (i0 + k < n) ? L" + std::to_string(q) + "[i0 + k] : T(0);\n";
-        t += std::string(ind) + "}\n";
-        return t;
-    };
-    const std::string slot = upfront ? "r" : "0";
-    for (int q = 0; q < P.n_loads; ++q)
-        b += "      T x" + std::to_string(q) + (upfront ? "[ROUNDS][VN];\n" : "[1][VN];\n");
-    if (upfront) {
-        b += "      for (int r = 0; r < ROUNDS; ++r) {\n"
-             "        const uint32_t i0 = s0 + r * (256 * VN) + threadIdx.x * VN;\n";
-        b += load_round("        ", "r");
-        b += "      }\n";
-    }
-    b += "      for (int r = 0; r < ROUNDS; ++r) {\n";
-    b += "        const uint32_t i0 = s0 + r * (256 * VN) + threadIdx.x * VN;\n";
-    if (!upfront) b += load_round("        ", "0");
-    b += "        T v[VN];\n        for (int k = 0; k < VN; ++k) {\n";
     std::vector<std::string> st;
     int tmp = 0;
     auto fresh = [&] { return "t" + std::to_string(tmp++); };
     for (int i = 0; i < P.n_insns; ++i) {
         const int op = P.insns[i].op, arg = P.insns[i].arg;
         if (op == AETH_OP_PUSH) {
-            st.push_back("x" + std::to_string(arg) + "[" + slot + "][k]");
+            st.push_back(operand(arg));
         } else if (op == AETH_OP_CONST) {
             st.push_back("((T)A.consts[" + std::to_string(arg) + "])");
         } else if (op == AETH_OP_RNG) {
             const std::string t = fresh();
-            b += "          const T " + t + " = jit_draw<T>(A.rng[A.rng_arg[pi][" + std::to_string(i) + "]], i0 + k);\n";
+            b += ind + "const T " + t + " = jit_draw<T>(A.rng[A.rng_arg[pi][" + std::to_string(i) + "]], " + sample +
+                 ");\n";
             st.push_back(t);
         } else if (is_binary(op)) {
             const std::string y = st.back();
@@ -580,14 +542,14 @@ std::string gen_program(const aeth_program& P, bool dbl, uint32_t vb)
             const std::string t = fresh();
             if (op >= AETH_OP_ADD && op <= AETH_OP_DIV) {
                 const char* sym = op == AETH_OP_ADD ? "+" : op == AETH_OP_SUB ? "-" : op == AETH_OP_MUL ? "*" : "/";
-                b += "          const T " + t + " = " + x + " " + sym + " " + y + ";\n";
+                b += ind + "const T " + t + " = " + x + " " + sym + " " + y + ";\n";
             } else {
                 const char* fn = op == AETH_OP_FMIN    ? "aeth::dm::fmin("
                                  : op == AETH_OP_FMAX  ? "aeth::dm::fmax("
                                  : op == AETH_OP_SELGT ? "aeth::dm::selgt("
                                  : dbl                 ? "aeth::dm::pow("
                                                        : "aeth::dm::powf(";
-                b += "          const T " + t + " = " + fn + x + ", " + y + ");\n";
+                b += ind + "const T " + t + " = " + fn + x + ", " + y + ");\n";
             }
             st.push_back(t);
         } else {
@@ -610,16 +572,64 @@ std::string gen_program(const aeth_program& P, bool dbl, uint32_t vb)
             default: f = std::string(dbl ? "__builtin_fabs(" : "__builtin_fabsf(") + x + ")"; break;
             }
             const std::string t = fresh();
-            b += "          const T " + t + " = " + f + ";\n";
+            b += ind + "const T " + t + " = " + f + ";\n";
             st.push_back(t);
         }
     }
-    b += "          v[k] = " + st.back() + ";\n        }\n";
-    b += v16 ? "        if (full) {\n          aeth::units::vstore<NT>(outp + i0, v);\n"
-             : "        if (full) {\n          for (int k = 0; k < VN; ++k) outp[i0 + k] = v[k];\n";
-    b += "          if (SUM) for (int k = 0; k < VN; ++k) aeth::acc::accumulate<M, false>(acc, v[k]);\n";
-    b += "        } else {\n          for (int k = 0; k < VN; ++k) if (i0 + k < n) { outp[i0 + k] = v[k]; "
-         "if (SUM) aeth::acc::accumulate<M, false>(acc, v[k]); }\n        }\n      }\n";
+    return st.back();
+}
+
+// One program over one tile, in the structure of the ahead-of-time kernels (units::cfg0_tile): a whole
+// tile takes one branch that issues its 16-byte loads (all rounds at once for programs with few loads,
+// round by round otherwise), evaluates, accumulates in sample order and stores; the last, partial tile
+// goes sample by sample with scalar accesses.
+std::string gen_program(const aeth_program& P, bool dbl, uint32_t vb)
+{
+    const bool v16 = vb == 16; // narrower accesses are element loops the compiler merges
+    const char* T = dbl ? "double" : "float";
+    std::string b;
+    b += "      " + std::string(T) + "* outp = (" + T + "*)A.plane[pi][0];\n";
+    for (int q = 0; q < P.n_loads; ++q)
+        b += "      const " + std::string(T) + "* L" + std::to_string(q) + " = (const " + T + "*)A.plane[pi][" +
+             std::to_string(1 + q) + "];\n";
+    const bool upfront = P.n_loads <= 6;
+    const std::string slot = upfront ? "r" : "0";
+    auto load = [&](int q, const std::string& ind) {
+        const std::string Q = std::to_string(q);
+        return v16 ? ind + "aeth::units::vload<NT>(L" + Q + " + i0, x" + Q + "[" + slot + "]);\n"
+                   : ind + "for (int k = 0; k < VN; ++k) x" + Q + "[" + slot + "][k] = L" + Q + "[i0 + k];\n";
+    };
+    b += "      if (full) {\n";
+    for (int q = 0; q < P.n_loads; ++q)
+        b += "        T x" + std::to_string(q) + (upfront ? "[ROUNDS][VN];\n" : "[1][VN];\n");
+    if (upfront) {
+        b += "#pragma unroll\n        for (int r = 0; r < ROUNDS; ++r) {\n"
+             "          const uint32_t i0 = s0 + r * (256 * VN) + threadIdx.x * VN;\n";
+        for (int q = 0; q < P.n_loads; ++q) b += load(q, "          ");
+        b += "        }\n";
+    }
+    b += "#pragma unroll\n        for (int r = 0; r < ROUNDS; ++r) {\n"
+         "          const uint32_t i0 = s0 + r * (256 * VN) + threadIdx.x * VN;\n";
+    if (!upfront)
+        for (int q = 0; q < P.n_loads; ++q) b += load(q, "          ");
+    b += "          T v[VN];\n#pragma unroll\n          for (int k = 0; k < VN; ++k) {\n";
+    const std::string res = gen_body(
+        P, dbl, [&](int q) { return "x" + std::to_string(q) + "[" + slot + "][k]"; }, "i0 + k", "            ", b);
+    b += "            v[k] = " + res + ";\n            if (SUM) aeth::acc::accumulate<M, false>(acc, v[k]);\n"
+         "          }\n";
+    b += v16 ? "          aeth::units::vstore<NT>(outp + i0, v);\n"
+             : "          for (int k = 0; k < VN; ++k) outp[i0 + k] = v[k];\n";
+    b += "        }\n      } else {\n";
+    b += "#pragma unroll\n        for (int r = 0; r < ROUNDS; ++r)\n#pragma unroll\n"
+         "          for (int k = 0; k < VN; ++k) {\n"
+         "            const uint32_t i = s0 + r * (256 * VN) + threadIdx.x * VN + k;\n"
+         "            if (i < n) {\n";
+    for (int q = 0; q < P.n_loads; ++q)
+        b += "              const T y" + std::to_string(q) + " = L" + std::to_string(q) + "[i];\n";
+    const std::string res2 =
+        gen_body(P, dbl, [&](int q) { return "y" + std::to_string(q); }, "i", "              ", b);
+    b += "              outp[i] = " + res2 + ";\n              if (SUM) aeth::acc::accumulate<M, false>(acc, " + res2 +
+         ");\n            }\n          }\n      }\n";
     return b;
 }
 
