@@ -1,145 +1,106 @@
-"""Batched FFTs on aether arrays: the reference's ``aether.fft`` (dsc/aether/fft.py:1-148) on MI355X.
+"""Transforms of aether arrays on MI355X: the public surface of the reference's ``aether.fft``
+(dsc/aether/fft.py) -- ``fft``, ``ifft``, ``fftn``, ``ifftn``, ``rfft``, ``irfft``, ``rfftn``, ``irfftn`` and
+``plan`` -- over the seam's FFT plans.
 
-Complex data is a real float32 / float64 array whose LAST axis has extent 2 (real, imaginary); every
-``axis`` counts the data axes only. Normalisation is numpy's ``norm``: "backward" (forward unscaled, inverse
-scaled by 1/n), "forward" or "ortho". Every transform runs on the GPU through the seam's plans (Stockham
-up to 8192 points, four-step up to 2^26, Bluestein for other lengths up to 2^25): device arrays in place on
-their device; host and pinned arrays are staged through device memory and the result is a host array.
-Lengths beyond those raise ``FftUnavailable``.
+Conventions (the interface): complex data are float32 / float64 arrays whose last axis holds (re, im), and
+``axis`` / ``axes`` count the other ("data") axes; ``norm`` is numpy's ("backward", "forward", "ortho").
+Every transform runs on the GPU: device arrays on their device, host and pinned arrays staged through it
+(the result is then a host array). Power-of-two lengths up to 2^26 and other lengths up to 2^25 are
+supported; longer ones raise ``FftUnavailable``.
 
-Plans are built eagerly, outside a stream capture; ``plan(...)`` (and the first transform of a new shape)
-builds one, later calls reuse it. A device plan's ``execute(input, output, stream=None)`` allocates nothing
-and may run inside a capture; ``stream=None`` runs on the input's allocation stream.
+A plan is built once per (kind, dtype, shape, axes, direction, norm, place) and kept; building one is
+refused inside a stream capture, while a device plan's ``execute(input, output, stream=None)`` allocates
+nothing and may be captured (``stream=None``: the input's allocation stream).
 """
 from . import _core
 from ._core import Error
+from ._shapes import KINDS, NORMS, complex_data, data_axes, real_lengths, seam_code, spectrum
 
 __all__ = ["fft", "ifft", "rfft", "irfft", "fftn", "ifftn", "rfftn", "irfftn", "plan"]
 
-_KIND = {"c2c": 0, "r2c": 1, "c2r": 2}
-_NORM = {"backward": 0, "forward": 1, "ortho": 2}
-_plans = {}
-
-
-def _norm_code(norm):
-    try:
-        return _NORM[norm]
-    except (KeyError, TypeError):
-        raise Error(f"norm= wants 'backward', 'forward' or 'ortho', not {norm!r}") from None
+_cache = {}
 
 
 def plan(kind, dtype, shape, axes, forward=True, norm="backward", device="cpu", stream=None):
-    """A transform plan. ``kind``: "c2c", "r2c" or "c2r". ``shape`` is the data shape without the
-    trailing 2: the complex shape for c2c, the REAL side's for r2c / c2r (whose complex side has
-    ``shape[axes[-1]] // 2 + 1`` points along the last of ``axes``). ``forward`` applies to c2c only.
-    Cached per (kind, dtype, shape, axes, direction, norm, device)."""
-    from . import _device, _dtype
+    """The plan of one transform. ``kind`` is "c2c", "r2c" or "c2r"; ``shape`` the data shape (no
+    trailing 2): the complex one for c2c, the REAL side's for r2c / c2r, whose complex side has
+    ``shape[axes[-1]] // 2 + 1`` points along the last of ``axes``. ``forward`` matters for c2c only.
+    Equal requests return the same plan object."""
+    from . import _place, _resolve_dtype
 
-    if kind not in _KIND:
-        raise Error(f"kind= wants 'c2c', 'r2c' or 'c2r', not {kind!r}")
-    shape = tuple(int(s) for s in shape)
-    axes = tuple(int(a) % len(shape) for a in axes) if shape else tuple(int(a) for a in axes)
-    dkind, dindex = _device(device)
-    if dkind == 2:  # pinned memory is host memory
-        dkind = 0
-    name = _dtype(dtype).name
-    key = (kind, name, shape, axes, bool(forward), norm, dkind, dindex)
-    p = _plans.get(key)
-    if p is None:
-        p = _core._fft_plan(_KIND[kind], name, dkind, dindex, list(shape), list(axes), bool(forward),
-                            _norm_code(norm), stream)
-        _plans[key] = p
-    return p
-
-
-def _axes(axis, rank):
-    if axis is None:
-        axis = tuple(range(rank))
-    elif isinstance(axis, int):
-        axis = (axis,)
-    out = []
-    for a in axis:
-        if not -rank <= a < rank:
-            raise Error(f"axis {a} is out of range for {rank} data axes")
-        out.append(a % rank)
-    return tuple(out)
+    kind_code = seam_code(KINDS, kind, "kind", Error)
+    norm_code = seam_code(NORMS, norm, "norm", Error)
+    dims = tuple(int(e) for e in shape)
+    ax = tuple(int(a) % len(dims) for a in axes) if dims else tuple(int(a) for a in axes)
+    where = _place(device)
+    place = (1, where.index) if where.kind == "cuda" else (0, 0)  # pinned memory is host memory
+    name = _resolve_dtype(dtype).name
+    key = (kind_code, name, dims, ax, bool(forward), norm_code) + place
+    found = _cache.get(key)
+    if found is None:
+        found = _core._fft_plan(kind_code, name, place[0], place[1], list(dims), list(ax), bool(forward),
+                                norm_code, stream)
+        _cache[key] = found
+    return found
 
 
-def _where(x):
-    d = x.device
-    return f"cuda:{d.index}" if d.kind == "cuda" else "cpu"
-
-
-def _complex_shape(x):
-    if x.ndim < 2 or x.shape[-1] != 2:
-        raise Error("a complex array is a float array with a trailing axis of 2 (re, im); this one has shape "
-                    f"{tuple(x.shape)}")
-    return tuple(x.shape[:-1])
-
-
-def _run(kind, x, out_shape, shape, axes, forward, norm, stream):
+def _execute(kind, x, shape, axes, forward, norm, out_shape, stream):
     from . import empty
 
-    where = _where(x)
+    where = "cpu" if x.device.kind != "cuda" else f"cuda:{x.device.index}"
     p = plan(kind, x.dtype, shape, axes, forward, norm, where, stream)
     out = empty(out_shape, x.dtype, where, stream)
     p.execute(x, out, stream)
     return out
 
 
-def _c2c(x, axes, forward, norm, stream):
-    shape = _complex_shape(x)
-    _norm_code(norm)
-    return _run("c2c", x, shape + (2,), shape, _axes(axes, len(shape)), forward, norm, stream)
+def _complex(x, axes, forward, norm, stream):
+    seam_code(NORMS, norm, "norm", Error)
+    dims = complex_data(x.shape, Error)
+    return _execute("c2c", x, dims, data_axes(axes, len(dims), Error), forward, norm, tuple(x.shape), stream)
 
 
 def fft(x, axis=-1, norm="backward", stream=None):
-    """1-D transform of a complex array along ``axis``."""
-    return _c2c(x, (axis,), True, norm, stream)
+    """Transform of a complex array along one data axis."""
+    return _complex(x, (axis,), True, norm, stream)
 
 
 def ifft(x, axis=-1, norm="backward", stream=None):
-    return _c2c(x, (axis,), False, norm, stream)
+    """Inverse transform along one data axis."""
+    return _complex(x, (axis,), False, norm, stream)
 
 
 def fftn(x, axes=None, norm="backward", stream=None):
-    """N-D transform over ``axes`` (default: every data axis)."""
-    return _c2c(x, axes, True, norm, stream)
+    """Transform over several data axes (default: all)."""
+    return _complex(x, axes, True, norm, stream)
 
 
 def ifftn(x, axes=None, norm="backward", stream=None):
-    return _c2c(x, axes, False, norm, stream)
+    """Inverse transform over several data axes (default: all)."""
+    return _complex(x, axes, False, norm, stream)
 
 
 def rfftn(x, axes=None, norm="backward", stream=None):
-    """N-D transform of REAL data: the output keeps n // 2 + 1 points along the last of ``axes``."""
-    shape = tuple(x.shape)
-    ax = _axes(axes, len(shape))
-    half = list(shape)
-    half[ax[-1]] = shape[ax[-1]] // 2 + 1
-    return _run("r2c", x, tuple(half) + (2,), shape, ax, True, norm, stream)
+    """Transform of REAL data over ``axes``; n // 2 + 1 points are kept along the last of them."""
+    dims = tuple(x.shape)
+    ax = data_axes(axes, len(dims), Error)
+    return _execute("r2c", x, dims, ax, True, norm, spectrum(dims, ax) + (2,), stream)
 
 
 def rfft(x, axis=-1, norm="backward", stream=None):
+    """Transform of REAL data along one axis."""
     return rfftn(x, (axis,), norm, stream)
 
 
 def irfftn(x, s=None, axes=None, norm="backward", stream=None):
-    """Inverse of ``rfftn``. ``s``: the REAL output's lengths along ``axes``; by default the last axis
-    gets the even length 2 (m - 1), as numpy does."""
-    shape = _complex_shape(x)
-    ax = _axes(axes, len(shape))
-    real = list(shape)
-    if s is None:
-        real[ax[-1]] = 2 * (shape[ax[-1]] - 1)
-    else:
-        for a, n in zip(ax, s):
-            real[a] = int(n)
-    if real[ax[-1]] // 2 + 1 != shape[ax[-1]]:
-        raise Error(f"irfft: a real length of {real[ax[-1]]} has {real[ax[-1]] // 2 + 1} points, "
-                    f"not {shape[ax[-1]]}")
-    return _run("c2r", x, tuple(real), tuple(real), ax, False, norm, stream)
+    """Inverse of ``rfftn``: REAL output. ``s`` gives the real lengths along ``axes``; without it the last
+    of them becomes 2 (m - 1) for m complex points (numpy's default)."""
+    dims = complex_data(x.shape, Error)
+    ax = data_axes(axes, len(dims), Error)
+    real = real_lengths(dims, ax, s, Error)
+    return _execute("c2r", x, real, ax, False, norm, real, stream)
 
 
 def irfft(x, n=None, axis=-1, norm="backward", stream=None):
+    """Inverse of ``rfft`` along one axis; ``n`` is the real output length."""
     return irfftn(x, None if n is None else (n,), (axis,), norm, stream)

@@ -1,8 +1,9 @@
-"""Batched FFTs on device arrays, with the reference's Python API (dsc/aether/fft.py:1-148).
+"""Transforms of torch tensors with the reference's ``aether.fft`` API (dsc/aether/fft.py): the same
+functions as ``aether_amd.dsc.fft``, for callers whose arrays are torch tensors.
 
-Complex data is a real float32 / float64 array whose LAST axis has extent 2 (real, imaginary); every
-``axis`` counts the data axes only. Normalisation is numpy's ``norm``: "backward" (forward unscaled,
-inverse scaled by 1/n), "forward" or "ortho". Arrays are torch tensors (or anything torch can take by
+Complex data are float32 / float64 tensors whose last axis holds (re, im); axes count the other axes;
+``norm`` is numpy's ("backward", "forward", "ortho"); the shape rules are those of
+``aether_amd.dsc._shapes``. Arrays are torch tensors (or anything torch can take by
 DLPack, or numpy arrays); the transform always runs on the GPU through this library's seam plans
 (``aether_memory_v1().fft_plan``): a host array is copied to the device and the result copied back.
 
@@ -23,14 +24,18 @@ import torch
 
 from . import _seam
 from ._lib import AetherError
+from .dsc._shapes import NORMS, complex_data, data_axes, real_lengths, seam_code, spectrum
 
 __all__ = ["fft", "ifft", "rfft", "irfft", "fftn", "ifftn", "rfftn", "irfftn", "plan", "Plan", "FftUnavailable",
            "Error"]
 
 Error = AetherError
 _KINDS = {"c2c": _seam.C2C, "r2c": _seam.R2C, "c2r": _seam.C2R}
-_NORMS = {"backward": 0, "forward": 1, "ortho": 2}
 _cache: dict = {}
+
+
+def _error(message: str) -> AetherError:
+    return AetherError(1, message)
 
 
 class FftUnavailable(AetherError):
@@ -41,9 +46,7 @@ class FftUnavailable(AetherError):
 
 
 def _norm(norm) -> int:
-    if norm not in _NORMS:
-        raise Error(1, f"norm= wants 'backward', 'forward' or 'ortho', not {norm!r}")
-    return _NORMS[norm]
+    return seam_code(NORMS, norm, "norm", _error)
 
 
 def _dtype(dtype) -> torch.dtype:
@@ -84,9 +87,7 @@ class Plan:
         self.kind, self.dtype, self.shape, self.axes = kind, dtype, shape, axes
         self.forward = True if kind == "r2c" else False if kind == "c2r" else bool(forward)
         self.norm, self.device = norm, device
-        half = list(shape)
-        if kind != "c2c" and axes:
-            half[axes[-1]] = shape[axes[-1]] // 2 + 1
+        half = spectrum(tuple(shape), axes) if kind != "c2c" and axes else tuple(shape)
         full, halfc = tuple(shape) + (2,), tuple(half) + (2,)
         self.input_shape = {"c2c": full, "r2c": tuple(shape), "c2r": halfc}[kind]
         self.output_shape = {"c2c": full, "r2c": halfc, "c2r": tuple(shape)}[kind]
@@ -94,7 +95,7 @@ class Plan:
         shp = (C.c_int64 * max(1, len(shape)))(*shape)
         ax = (C.c_int32 * max(1, len(axes)))(*axes)
         h = C.c_void_p()
-        rc = t.fft_plan(device, _KINDS[kind], 1 if dtype == torch.float64 else 0, _NORMS[norm], int(self.forward),
+        rc = t.fft_plan(device, _KINDS[kind], 1 if dtype == torch.float64 else 0, _norm(norm), int(self.forward),
                         len(shape), shp, len(axes), ax, C.c_void_p(_stream(stream)), C.byref(h))
         if rc == 2:
             raise FftUnavailable(_seam.last_error())
@@ -128,10 +129,10 @@ def plan(kind, dtype, shape, axes, forward=True, norm="backward", device="cuda",
     shape for c2c, the REAL side's for r2c / c2r. Cached per (kind, dtype, shape, axes, direction, norm,
     device)."""
     if kind not in _KINDS:
-        raise Error(1, f"kind= wants 'c2c', 'r2c' or 'c2r', not {kind!r}")
+        raise _error(f"kind= must be one of 'c2c', 'r2c', 'c2r'; got {kind!r}")
     dt = _dtype(dtype)
     shape = tuple(int(s) for s in shape)
-    axes = tuple(_axes(axes, len(shape)))
+    axes = data_axes(axes, len(shape), _error)
     _norm(norm)
     dev = _device_index(device)
     fwd = True if kind == "r2c" else False if kind == "c2r" else bool(forward)
@@ -141,19 +142,6 @@ def plan(kind, dtype, shape, axes, forward=True, norm="backward", device="cuda",
         p = Plan(kind, dt, shape, axes, fwd, norm, dev, stream)
         _cache[key] = p
     return p
-
-
-def _axes(axis, rank):
-    if axis is None:
-        axis = tuple(range(rank))
-    elif isinstance(axis, int):
-        axis = (axis,)
-    out = []
-    for a in axis:
-        if not -rank <= a < rank:
-            raise Error(1, f"axis {a} is out of range for {rank} data axes")
-        out.append(a % rank)
-    return tuple(out)
 
 
 def _as_tensor(x):
@@ -172,13 +160,6 @@ def _as_tensor(x):
     return x.contiguous(), host
 
 
-def _complex_shape(x):
-    if x.dim() < 2 or x.shape[-1] != 2:
-        raise Error(1, "a complex array is a float array with a trailing axis of 2 (re, im); this one has shape "
-                       f"{tuple(x.shape)}")
-    return tuple(x.shape[:-1])
-
-
 def _run(kind, x, host, shape, axes, forward, norm, stream):
     p = plan(kind, x.dtype, shape, axes, forward, norm, x.device, stream)
     out = torch.empty(p.output_shape, dtype=x.dtype, device=x.device)
@@ -189,8 +170,8 @@ def _run(kind, x, host, shape, axes, forward, norm, stream):
 def _c2c(x, axes, forward, norm, stream):
     x, host = _as_tensor(x)
     _dtype(x.dtype)
-    shape = _complex_shape(x)
-    return _run("c2c", x, host, shape, _axes(axes, len(shape)), forward, norm, stream)
+    shape = complex_data(x.shape, _error)
+    return _run("c2c", x, host, shape, data_axes(axes, len(shape), _error), forward, norm, stream)
 
 
 def fft(x, axis=-1, norm="backward", stream=None):
@@ -216,7 +197,7 @@ def rfftn(x, axes=None, norm="backward", stream=None):
     x, host = _as_tensor(x)
     _dtype(x.dtype)
     shape = tuple(x.shape)
-    return _run("r2c", x, host, shape, _axes(axes, len(shape)), True, norm, stream)
+    return _run("r2c", x, host, shape, data_axes(axes, len(shape), _error), True, norm, stream)
 
 
 def rfft(x, axis=-1, norm="backward", stream=None):
@@ -228,21 +209,9 @@ def irfftn(x, s=None, axes=None, norm="backward", stream=None):
     length on the last axis (``2 * (m - 1)``), as numpy does. The input is left unchanged."""
     x, host = _as_tensor(x)
     _dtype(x.dtype)
-    shape = _complex_shape(x)
-    ax = _axes(axes, len(shape))
-    out = list(shape)
-    if s is not None:
-        for a, n in zip(ax, s):
-            out[a] = int(n)
-    else:
-        out[ax[-1]] = 2 * (shape[ax[-1]] - 1)
-    if out[ax[-1]] // 2 + 1 != shape[ax[-1]]:
-        raise Error(1, f"irfft: a real length of {out[ax[-1]]} has {out[ax[-1]] // 2 + 1} points, not "
-                       f"{shape[ax[-1]]}")
-    for a in ax[:-1]:
-        if out[a] != shape[a]:
-            raise Error(1, "irfftn: only the last of axes changes length (others are complex-to-complex)")
-    return _run("c2r", x, host, tuple(out), ax, False, norm, stream)
+    shape = complex_data(x.shape, _error)
+    ax = data_axes(axes, len(shape), _error)
+    return _run("c2r", x, host, real_lengths(shape, ax, s, _error), ax, False, norm, stream)
 
 
 def irfft(x, n=None, axis=-1, norm="backward", stream=None):

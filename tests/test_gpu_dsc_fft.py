@@ -1,9 +1,15 @@
-"""GPU: aether_amd.dsc.fft -- the reference's aether.fft rows (dsc/tests/test_fft.py:78-268) over aether
-arrays, on both the host ("cpu", staged through the device) and the device. numpy.fft is the oracle and the
-bound is the reference's ULP-scaled one: the largest absolute error of a transform over N points is at most
-(8 + 4 log2 N) * eps * ||x||_2 * scale. Plans, their errors, a plan captured into a graph and replayed, the
-refusal of plan creation inside a capture, and bitwise determinism of repeated transforms."""
+"""GPU: aether_amd.dsc.fft over aether arrays, on host arrays (staged through the device) and on device
+arrays.
+
+The acceptance rows are the reference package's FFT rows (dsc/tests/test_fft.py), restated: c2c over many
+lengths (powers of two, primes, composites), batched and multi-axis transforms and their round trips, the
+three norms, r2c / c2r including a spectrum built by numpy, plan caching and plan shapes, named misuse
+errors, the host path without numpy, a plan made before a capture executed inside it and replayed, plan
+creation refused inside a capture, and bit-identical repeats. numpy.fft in complex128 is the reference and
+the bound is the reference's: max |error| <= (8 + 4 log2(N + 1)) * eps * ||x||_2 * s, with s the
+transform's scale (1, 1/N or 1/sqrt(N))."""
 import math
+import os
 import pathlib
 import subprocess
 import sys
@@ -15,227 +21,183 @@ import torch
 
 import aether_amd.dsc as aether
 from aether_amd.dsc import fft as F
+from fft_rows import LENGTHS, MULTI_AXES, REAL_LENGTHS, Data, bound, scale, to_complex, to_pairs, worst
 
 pytestmark = pytest.mark.gpu
-ROOT = pathlib.Path(__file__).resolve().parents[1]
-DTYPES = ["float32", "float64"]
-DEVICES = ["cpu", "cuda"]
+REPO = pathlib.Path(__file__).resolve().parents[1]
+PLACES = ["cpu", "cuda"]
 
 
-def pairs(c, dtype):
-    return np.ascontiguousarray(np.stack([c.real, c.imag], -1).astype(dtype))
-
-
-def cplx(p):
-    p = np.asarray(p, dtype=np.float64)
-    return p[..., 0] + 1j * p[..., 1]
-
-
-def tol(dtype, points, x, scale):
-    return (8 + 4 * math.log2(points + 1)) * np.finfo(dtype).eps * float(np.linalg.norm(np.asarray(x, np.float64))) * scale
-
-
-def rand_c(shape, dtype, seed=0):
-    r = np.random.default_rng(seed)
-    return pairs(r.uniform(-1, 1, shape) + 1j * r.uniform(-1, 1, shape), dtype)
-
-
-def scale_of(norm, forward, n):
-    if norm == "ortho":
-        return 1 / math.sqrt(n)
-    return 1 / n if forward == (norm == "forward") else 1.0
-
-
-def to_host(a):
+def fetch(a):
+    """A host numpy copy of an aether array wherever it lives."""
     return np.array(aether.asarray(a, device="cpu"), copy=True)
 
 
-def on(x, device):
-    return aether.asarray(x, device=device)
+@pytest.mark.parametrize("place", PLACES)
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("forward", [True, False], ids=["fft", "ifft"])
+def test_c2c_over_many_lengths(dev, place, dtype, forward):
+    transform, numpy_ref = (F.fft, np.fft.fft) if forward else (F.ifft, np.fft.ifft)
+    for n in LENGTHS:
+        x = Data(1000 + n).complex_pairs((3, n), dtype)
+        y = transform(aether.asarray(x, device=place))
+        assert y.device.kind == place and y.shape == (3, n, 2)
+        err = worst(to_complex(fetch(y)), numpy_ref(to_complex(x), axis=-1))
+        assert err <= bound(dtype, n, x, scale("backward", forward, n)), (n, err)
 
 
-@pytest.mark.parametrize("device", DEVICES)
-@pytest.mark.parametrize("dtype", DTYPES)
-def test_c2c_lengths_match_numpy(dev, device, dtype):
-    for n in (1, 2, 3, 4, 5, 7, 8, 9, 12, 16, 17, 31, 64, 100, 127, 1000):
-        x = rand_c((3, n), dtype, n)
-        for forward in (True, False):
-            fn = F.fft if forward else F.ifft
-            y = fn(on(x, device))
-            assert y.device.kind == device
-            got = to_host(y)
-            ref = (np.fft.fft if forward else np.fft.ifft)(cplx(x), axis=-1)
-            err = np.abs(cplx(got) - ref).max()
-            assert err <= tol(dtype, n, x, scale_of("backward", forward, n)), (n, forward, err)
+@pytest.mark.parametrize("place", PLACES)
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("axes", MULTI_AXES, ids=str)
+def test_batched_and_multi_axis_round_trip(dev, place, dtype, axes):
+    x = Data(77).complex_pairs((4, 7, 6), dtype)
+    pts = math.prod(x.shape[a] for a in axes)
+    spectrum = F.fftn(aether.asarray(x, device=place), axes=axes)
+    got = fetch(spectrum)
+    assert worst(to_complex(got), np.fft.fftn(to_complex(x), axes=axes)) <= bound(dtype, pts, x, 1.0)
+    back = fetch(F.ifftn(aether.asarray(got, device=place), axes=axes))
+    assert worst(to_complex(back), to_complex(x)) <= 2 * bound(dtype, pts, got, 1.0 / pts)
 
 
-@pytest.mark.parametrize("device", DEVICES)
-@pytest.mark.parametrize("dtype", DTYPES)
-def test_multi_axis_and_batched_match_numpy(dev, device, dtype):
-    x = rand_c((4, 7, 6), dtype, 5)
-    xc = cplx(x)
-    cases = [((1,), np.fft.fft(xc, axis=1)), ((0, 2), np.fft.fftn(xc, axes=(0, 2))), ((0, 1, 2), np.fft.fftn(xc)),
-             ((2, 0), np.fft.fftn(xc, axes=(2, 0)))]
-    for axes, ref in cases:
-        got = to_host(F.fftn(on(x, device), axes=axes))
-        pts = int(np.prod([x.shape[a] for a in axes]))
-        assert np.abs(cplx(got) - ref).max() <= tol(dtype, pts, x, 1.0), axes
-        back = to_host(F.ifftn(on(got, device), axes=axes))
-        assert np.abs(cplx(back) - xc).max() <= 2 * tol(dtype, pts, got, 1.0 / pts), axes
-
-
-@pytest.mark.parametrize("device", DEVICES)
+@pytest.mark.parametrize("place", PLACES)
 @pytest.mark.parametrize("norm", ["backward", "forward", "ortho"])
-def test_norms_match_numpy(dev, device, norm):
-    x = rand_c((2, 12), "float64", 9)
-    for forward in (True, False):
-        fn = F.fft if forward else F.ifft
-        got = to_host(fn(on(x, device), norm=norm))
-        ref = (np.fft.fft if forward else np.fft.ifft)(cplx(x), norm=norm)
-        assert np.abs(cplx(got) - ref).max() <= tol("float64", 12, x, scale_of(norm, forward, 12))
+def test_norms(dev, place, norm):
+    x = Data(9).complex_pairs((2, 12), "float64")
+    for forward, transform, numpy_ref in ((True, F.fft, np.fft.fft), (False, F.ifft, np.fft.ifft)):
+        got = to_complex(fetch(transform(aether.asarray(x, device=place), norm=norm)))
+        assert worst(got, numpy_ref(to_complex(x), norm=norm)) <= bound("float64", 12, x, scale(norm, forward, 12))
 
 
-@pytest.mark.parametrize("device", DEVICES)
-@pytest.mark.parametrize("dtype", DTYPES)
-def test_rfft_irfft_match_numpy(dev, device, dtype):
-    for n in (2, 3, 7, 8, 9, 16, 17, 50, 128):
-        r = np.random.default_rng(n).uniform(-1, 1, (3, n)).astype(dtype)
-        got = to_host(F.rfft(on(r, device)))
-        ref = np.fft.rfft(r.astype(np.float64), axis=-1)
-        assert got.shape == ref.shape + (2,)
-        assert np.abs(cplx(got) - ref).max() <= tol(dtype, n, r, 1.0), n
-        back = to_host(F.irfft(on(got, device), n=n))
-        assert np.abs(back - r).max() <= 2 * tol(dtype, n, r, 1.0 / n), n
-        spec = pairs(ref, dtype)
-        c2r = to_host(F.irfft(on(spec, device), n=n))
-        assert np.abs(c2r - np.fft.irfft(ref, n=n, axis=-1)).max() <= 2 * tol(dtype, n, spec, 1.0 / n), n
+@pytest.mark.parametrize("place", PLACES)
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("n", REAL_LENGTHS)
+def test_real_transforms_one_axis(dev, place, dtype, n):
+    r = Data(n).real((3, n), dtype)
+    ref = np.fft.rfft(r.astype(np.float64), axis=-1)
+    spec = fetch(F.rfft(aether.asarray(r, device=place)))
+    assert spec.shape == (3, n // 2 + 1, 2)
+    assert worst(to_complex(spec), ref) <= bound(dtype, n, r, 1.0)
+    back = fetch(F.irfft(aether.asarray(spec, device=place), n=n))
+    assert worst(back, r) <= 2 * bound(dtype, n, r, 1.0 / n)
+    # a spectrum made by numpy, inverted by the plan
+    numpy_spec = to_pairs(ref, dtype)
+    inv = fetch(F.irfft(aether.asarray(numpy_spec, device=place), n=n))
+    assert worst(inv, np.fft.irfft(ref, n=n, axis=-1)) <= 2 * bound(dtype, n, numpy_spec, 1.0 / n)
 
 
-@pytest.mark.parametrize("device", DEVICES)
-@pytest.mark.parametrize("dtype", DTYPES)
-def test_rfftn_irfftn_match_numpy(dev, device, dtype):
-    r = np.random.default_rng(3).uniform(-1, 1, (4, 5, 6)).astype(dtype)
+@pytest.mark.parametrize("place", PLACES)
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_real_transforms_several_axes(dev, place, dtype):
+    r = Data(3).real((4, 5, 6), dtype)
     for axes in ((0, 1, 2), (1, 2), (2, 0)):
-        got = to_host(F.rfftn(on(r, device), axes=axes))
+        pts = math.prod(r.shape[a] for a in axes)
         ref = np.fft.rfftn(r.astype(np.float64), axes=axes)
-        pts = int(np.prod([r.shape[a] for a in axes]))
-        assert got.shape == ref.shape + (2,), axes
-        assert np.abs(cplx(got) - ref).max() <= tol(dtype, pts, r, 1.0), axes
-        s = [r.shape[a] for a in axes]
-        back = to_host(F.irfftn(on(got, device), s=s, axes=axes))
-        assert np.abs(back - r).max() <= 2 * tol(dtype, pts, r, 1.0 / pts), axes
-    # the input of a complex-to-real transform is left unchanged
-    spec = pairs(np.fft.rfftn(r.astype(np.float64)), dtype)
-    keep = spec.copy()
-    sd = on(spec, device)
-    F.irfftn(sd, s=r.shape, axes=(0, 1, 2))
-    assert np.array_equal(to_host(sd), keep)
+        spec = fetch(F.rfftn(aether.asarray(r, device=place), axes=axes))
+        assert spec.shape == ref.shape + (2,) and worst(to_complex(spec), ref) <= bound(dtype, pts, r, 1.0)
+        lengths = [r.shape[a] for a in axes]
+        back = fetch(F.irfftn(aether.asarray(spec, device=place), s=lengths, axes=axes))
+        assert worst(back, r) <= 2 * bound(dtype, pts, r, 1.0 / pts)
+    # the complex-to-real transform leaves its input as it was
+    spec = to_pairs(np.fft.rfftn(r.astype(np.float64)), dtype)
+    src = aether.asarray(spec.copy(), device=place)
+    F.irfftn(src, s=r.shape, axes=(0, 1, 2))
+    assert np.array_equal(fetch(src), spec)
 
 
-def test_plans_are_cached_and_carry_their_shapes(dev):
-    a = F.plan("r2c", "float32", (4, 9), (1,))
-    assert a is F.plan("r2c", "float32", (4, 9), (-1,))
-    assert a.input_shape == (4, 9) and a.output_shape == (4, 5, 2)
+def test_plan_cache_and_shapes(dev):
+    p = F.plan("r2c", "float32", (4, 9), (1,))
+    assert F.plan("r2c", "float32", (4, 9), (-1,)) is p
+    assert (p.input_shape, p.output_shape) == ((4, 9), (4, 5, 2))
     assert F.plan("c2c", "float64", (3, 4), (0, 1), forward=False).input_shape == (3, 4, 2)
     assert F.plan("c2c", "float32", (8,), (0,), device="cuda") is not F.plan("c2c", "float32", (8,), (0,))
 
 
-def test_misuse_is_a_named_error(dev):
-    p = F.plan("c2c", "float32", (8,), (0,))
-    with pytest.raises(aether.Error, match="plan wants float32"):
-        p.execute(aether.zeros((8, 2), "float64"), aether.zeros((8, 2), "float64"))
-    with pytest.raises(aether.Error, match="wrong shape"):
-        p.execute(aether.zeros((7, 2), "float32"), aether.zeros((8, 2), "float32"))
-    with pytest.raises(aether.Error, match="plan's device"):
-        p.execute(aether.zeros((8, 2), "float32", "cuda"), aether.zeros((8, 2), "float32"))
-    pd = F.plan("r2c", "float32", (8,), (0,), device="cuda")
-    x = aether.zeros((8,), "float32", "cuda")
-    with pytest.raises(aether.Error, match="wrong shape"):
-        pd.execute(x, x)
-    with pytest.raises(F.Error):
+def test_misuse_is_named(dev):
+    host_plan = F.plan("c2c", "float32", (8,), (0,))
+    wrong = [
+        (lambda: host_plan.execute(aether.zeros((8, 2), "float64"), aether.zeros((8, 2), "float64")),
+         "plan wants float32"),
+        (lambda: host_plan.execute(aether.zeros((7, 2), "float32"), aether.zeros((8, 2), "float32")), "wrong shape"),
+        (lambda: host_plan.execute(aether.zeros((8, 2), "float32", "cuda"), aether.zeros((8, 2), "float32")),
+         "plan's device"),
+    ]
+    dev_plan = F.plan("r2c", "float32", (8,), (0,), device="cuda")
+    real = aether.zeros(8, "float32", "cuda")
+    wrong.append((lambda: dev_plan.execute(real, real), "wrong shape"))
+    for call, words in wrong:
+        with pytest.raises(aether.Error, match=words):
+            call()
+    with pytest.raises(aether.Error):  # neither a power of two nor within Bluestein's 2^25
         F.plan("c2c", "float32", ((1 << 25) + 1,), (0,), device="cuda")
-    with pytest.raises(aether.FftUnavailable):
+    with pytest.raises(aether.FftUnavailable):  # a power of two beyond 2^26
         F.plan("c2c", "float64", (1 << 27,), (0,), device="cuda")
-    # pinned host memory is host memory: it runs the host path and gives a host array
     pin = aether.pinned((8, 2), "float32")
     pin.zero()
-    assert F.fft(pin).device.kind == "cpu"
+    assert F.fft(pin).device.kind == "cpu"  # pinned is host memory: host path, host result
 
 
-def test_the_host_path_needs_no_numpy(dev):
-    code = textwrap.dedent("""
+def test_host_path_without_numpy(dev):
+    source = textwrap.dedent("""
         import sys
-        for m in ('numpy', 'cupy', 'torch'):
-            sys.modules[m] = None
+        sys.modules.update(numpy=None, cupy=None, torch=None)
         import aether_amd.dsc as aether
-        from aether_amd.dsc import fft
-        x = aether.asarray([[1.0, 0.0], [0.0, 0.0], [0.0, 0.0], [0.0, 0.0]])
-        y = fft.fft(x)
-        assert [list(r) for r in memoryview(y).tolist()] == [[1.0, 0.0]] * 4, memoryview(y).tolist()
-        print('ok')
+        impulse = aether.asarray([[1.0, 0.0]] + [[0.0, 0.0]] * 3)
+        flat = aether.fft.fft(impulse)
+        print(memoryview(flat).tolist() == [[1.0, 0.0]] * 4)
     """)
-    import os
-
-    env = dict(os.environ, PYTHONPATH=str(ROOT))
-    out = subprocess.run([sys.executable, "-W", "ignore", "-c", code], capture_output=True, text=True, timeout=120,
-                         env=env)
-    assert out.stdout.strip() == "ok", out.stderr[-600:]
+    child = subprocess.run([sys.executable, "-W", "ignore", "-c", source], capture_output=True, text=True,
+                           timeout=120, env={**os.environ, "PYTHONPATH": str(REPO)})
+    assert child.stdout.strip() == "True", child.stderr[-800:]
 
 
-def test_a_plan_built_before_a_capture_runs_inside_it_and_replays(dev):
-    s = torch.cuda.Stream()
-    x = rand_c((16, 64), "float32", 1)
-    d = aether.asarray(x, device="cuda")
-    out = aether.empty((16, 64, 2), "float32", "cuda", stream=s)
-    p = F.plan("c2c", "float32", (16, 64), (1,), True, "backward", "cuda", s)  # eager, outside the capture
-    p2 = F.plan("c2r", "float32", (16, 64), (1,), False, "backward", "cuda", s)
-    real = aether.empty((16, 64), "float32", "cuda", stream=s)
-    ref = np.fft.fft(cplx(x), axis=-1)
-    s.synchronize()
+def replay_and_check(graph, target, expect, tol, times=2):
+    view = torch.from_dlpack(target)
+    for _ in range(times):
+        view.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert worst(expect[0](view.cpu().numpy()), expect[1]) <= tol
+
+
+def test_plans_made_before_a_capture_replay(dev):
+    side = torch.cuda.Stream()
+    data = Data(1)
+    x = data.complex_pairs((16, 64), "float32")
+    c2c = F.plan("c2c", "float32", (16, 64), (1,), True, "backward", "cuda", side)
+    c2r = F.plan("c2r", "float32", (16, 64), (1,), False, "backward", "cuda", side)
+    src = aether.asarray(x, device="cuda")
+    spec = aether.empty((16, 64, 2), "float32", "cuda", stream=side)
+    r = data.real((16, 64), "float32")
+    half = aether.asarray(to_pairs(np.fft.rfft(r.astype(np.float64), axis=-1), "float32"), device="cuda")
+    real = aether.empty((16, 64), "float32", "cuda", stream=side)
+    side.synchronize()
+    g1, g2 = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g1, stream=side):
+        c2c.execute(src, spec, side)
+    with torch.cuda.graph(g2, stream=side):
+        c2r.execute(half, real, side)
+    replay_and_check(g1, spec, (to_complex, np.fft.fft(to_complex(x), axis=-1)), bound("float32", 64, x, 1.0))
+    replay_and_check(g2, real, (lambda v: v, r), 2 * bound("float32", 64, r, 1.0 / 64))
+
+
+def test_no_plan_is_made_inside_a_capture(dev):
+    side = torch.cuda.Stream()
+    counter = torch.zeros(4, device="cuda")
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g, stream=s):
-        p.execute(d, out, s)
-    tout = torch.from_dlpack(out)
-    for _ in range(2):
-        tout.zero_()
-        torch.cuda.synchronize()
-        g.replay()
-        torch.cuda.synchronize()
-        assert np.abs(cplx(tout.cpu().numpy()) - ref).max() <= tol("float32", 64, x, 1.0)
-    # a complex-to-real plan (work copy, Hermitian extension, real part) also captures and replays
-    r = np.random.default_rng(2).uniform(-1, 1, (16, 64)).astype("float32")
-    sp = pairs(np.fft.rfft(r.astype(np.float64), axis=-1), "float32")
-    dsp = aether.asarray(sp, device="cuda")
-    g2 = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g2, stream=s):
-        p2.execute(dsp, real, s)
-    treal = torch.from_dlpack(real)
-    for _ in range(2):
-        treal.zero_()
-        torch.cuda.synchronize()
-        g2.replay()
-        torch.cuda.synchronize()
-        assert np.abs(treal.cpu().numpy() - r).max() <= 2 * tol("float32", 64, r, 1 / 64)
-
-
-def test_plan_creation_inside_a_capture_is_refused_by_name(dev):
-    s = torch.cuda.Stream()
-    y = torch.zeros(4, device="cuda")
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g, stream=s):
-        y += 1
+    with torch.cuda.graph(g, stream=side):
+        counter += 1
         with pytest.raises(aether.Error, match="plan creation is not allowed inside a stream capture"):
-            F.plan("c2c", "float32", (123, 45), (1,), True, "backward", "cuda", s)
+            F.plan("c2c", "float32", (123, 45), (1,), True, "backward", "cuda", side)
     g.replay()
     torch.cuda.synchronize()
-    assert float(y.sum()) == 4.0
+    assert counter.sum().item() == 4.0
 
 
-@pytest.mark.parametrize("device", DEVICES)
-def test_repeated_transforms_give_the_same_bits(dev, device):
-    x = rand_c((64, 256), "float32", 11)
-    a = to_host(F.fft(on(x, device)))
-    b = to_host(F.fft(on(x, device)))
-    assert a.tobytes() == b.tobytes()
-    r = np.random.default_rng(12).uniform(-1, 1, (64, 100)).astype("float64")
-    assert to_host(F.rfftn(on(r, device))).tobytes() == to_host(F.rfftn(on(r, device))).tobytes()
+@pytest.mark.parametrize("place", PLACES)
+def test_repeats_are_bit_identical(dev, place):
+    data = Data(11)
+    x = aether.asarray(data.complex_pairs((64, 256), "float32"), device=place)
+    assert fetch(F.fft(x)).tobytes() == fetch(F.fft(x)).tobytes()
+    r = aether.asarray(data.real((64, 100), "float64"), device=place)
+    assert fetch(F.rfftn(r)).tobytes() == fetch(F.rfftn(r)).tobytes()
