@@ -9,6 +9,8 @@
 //   cfg0   <n> <seedA> <seedB> <seedC>  : a,b,c,d (3 planes x n each, pitch = n
 //                                         compacted) then 3 tree sums of d
 //   cfg0g  <n> <seed>                   : same, inputs from Generator(seed).uniform
+//   gnormal <n> <seed>                  : Generator(seed).normal<double,3>(0.5, 2) planes, then
+//                                         Generator(seed).substream(3).normal<float,3>(-1, 0.25) planes
 //   cfg0f  <n> <seedA> <seedB> <seedC>  : the float variant
 //   vec3   <n> <seedA> <seedB> <seedC>  : e (n doubles)
 //   keys   <seed> <streamId> <k>        : generator key, substream key (text)
@@ -309,6 +311,23 @@ int main(int argc, char** argv)
         const auto leaf = g.uniform<double, 3>(-1.0, 1.0);
         cfg0<double>(u(2), leaf, g.substream(1).uniform<double, 3>(-1.0, 1.0),
             g.substream(2).uniform<double, 3>(-1.0, 1.0));
+    } else if (mode == "gnormal" && argc == 4) {
+        // Generator(seed).normal<double,3>(0.5, 2) and Generator(seed).substream(3).normal<float,3>(-1, 0.25)
+        const std::size_t n = u(2);
+        const aether::random::Generator g(u(3));
+        aether::Array<double, 3> x(n);
+        aether::Array<float, 3> y(n);
+        auto xv = x.hostView();
+        auto yv = y.hostView();
+        const auto lx = g.normal<double, 3>(0.5, 2.0);
+        const auto ly = g.substream(3).normal<float, 3>(-1.0f, 0.25f);
+        for (std::size_t k = 0; k < n; ++k) {
+            const auto i = SampleIndex::make(k);
+            xv[i] = lx;
+            yv[i] = ly;
+        }
+        writePlanes(x, n);
+        writePlanes(y, n);
     } else if (mode == "vec3" && argc == 6) {
         vec3(u(2), u(3), u(4), u(5));
     } else if (mode == "keys" && argc == 5) {

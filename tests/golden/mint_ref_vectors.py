@@ -3,7 +3,7 @@
 Runs oracle/_ref/aether_ref (the reference's header-only CPU path compiled
 from its sources by `make -C oracle ref`) and stores small input/output
 vectors next to this script. Run in the build container, where the reference
-tree exists: python tests/golden/mint_ref_vectors.py [members]
+tree exists: python tests/golden/mint_ref_vectors.py [members | gnormal]
 """
 import hashlib
 import os
@@ -97,9 +97,21 @@ def mint_members(n, seeds, dtype=np.float64):
     print(mode, n, len(MEMBER_PLANES))
 
 
+def mint_generator_normal(n, seed):
+    """Generator(seed).normal<double,3>(0.5, 2) and .substream(3).normal<float,3>(-1, 0.25) planes."""
+    raw = run("gnormal", n, seed)
+    d = np.frombuffer(raw[: 3 * n * 8], np.float64).reshape(3, n)
+    f = np.frombuffer(raw[3 * n * 8:], np.float32).reshape(3, n)
+    np.savez_compressed(os.path.join(HERE, "ref_generator_normal.npz"), n=n, seed=seed, d=d, f=f)
+    print("gnormal", d.shape, f.shape)
+
+
 def main(parts):
     if not os.path.exists(REF):
         sys.exit("build the reference driver first: make -C oracle ref")
+    if parts == ["gnormal"]:
+        mint_generator_normal(1000, 42)
+        return
     if parts == ["members"]:
         mint_members(512, (11, 12, 13))
         mint_members(512, (11, 12, 13), np.float32)
@@ -121,6 +133,7 @@ def main(parts):
         mint_fft(n, 7)
     mint_members(512, (11, 12, 13))
     mint_members(512, (11, 12, 13), np.float32)
+    mint_generator_normal(1000, 42)
 
 
 if __name__ == "__main__":

@@ -131,6 +131,11 @@ def test_vec3geo_unaligned_output(dev, orc):
 @pytest.mark.parametrize("n", [0, 5, 2048, 300_001])
 def test_reduce_parity(dev, orc, dt, square, mode, n):
     if mode == ae.SUM_FLOATFLOAT and dt == torch.float64:
+        # float-float accumulation is an FP32 mode: FP64 data are refused as unsupported (code 2)
+        (x,) = filled(max(n, 1), dt, seeds=(9,))
+        with pytest.raises(ae.AetherError) as e:
+            ae.reduce(x, mode, square)
+        assert e.value.code == 2
         return
     (x,) = filled(n, dt, seeds=(9,))
     out = ae.reduce(x, mode, square)

@@ -82,3 +82,28 @@ def test_rejects_out_of_span(dev):
     out = torch.empty(4, dtype=torch.float64, device="cuda")
     with pytest.raises(ae.AetherError):
         ae.rng_fill(ae.RNG_UNIFORM_F64, 1, out, first=(1 << 32) - 2)
+
+
+def test_generator_normal_leaf_on_device(dev, orc):
+    """The Python Generator's normal leaf on the device (E.normal lowered to a tape RNG op) against the
+    reference's Generator(42).normal outputs: within the reference's band, and bit for bit the oracle's
+    own-math stream (the device's math)."""
+    import numpy as np
+
+    from aether_amd import expr as E
+
+    z = np.load(os.path.join(GOLDEN, "ref_generator_normal.npz"))
+    n = int(z["n"])
+    g = ae.Generator.from_seed(42)
+    x = ae.Array(n, 3, torch.float64)
+    E.assign(x, E.normal(g, 3, 0.5, 2.0))
+    y = ae.Array(n, 3, torch.float32)
+    E.assign(y, E.normal(g.substream(3), 3, -1.0, 0.25))
+    torch.cuda.synchronize()
+    got_d, got_f = x.data[:, :n].cpu().numpy(), y.data[:, :n].cpu().numpy()
+    assert np.all(np.abs(got_d - z["d"]) <= 1e-12 * np.maximum(1.0, np.abs(z["d"])))
+    assert np.all(np.abs(got_f - z["f"]) <= 1e-5 * np.maximum(1.0, np.abs(z["f"])))
+    own_d = orc.fill_array_normal_f64(g.normal_seed(), 3, n, n, 0.5, 2.0, mathmode=1).reshape(3, n)
+    own_f = orc.fill_array_normal_f32(g.substream(3).normal_seed(), 3, n, n, -1.0, 0.25, mathmode=1).reshape(3, n)
+    assert np.array_equal(got_d.view(np.uint64), own_d.view(np.uint64))
+    assert np.array_equal(got_f.view(np.uint32), own_f.view(np.uint32))

@@ -94,19 +94,33 @@ def test_budget_admission_and_oversize(dev):
 
 
 def test_fp32_compensated_mode_error(dev):
-    jobs = [S.make_job(S.JOB_EVAL, 1 << 15, 8 * i + 3, S.F64) for i in range(6)]
-    ref = [S.run_plain(j) for j in jobs]
+    """A server in the FP32-compensated form evaluates in FP32 and accumulates in float-float. Its sums
+    are held against an FP64 compensated reduction of the same float values of d (recomputed by the
+    single-call FP32 kernel, which the served batch matches tile for tile) to 2^-44 sum|d| -- a bound that
+    a plain FP32 sum of those values does not meet."""
+    n = 1 << 15
+    jobs = [S.make_job(S.JOB_EVAL, n, 8 * i + 3, S.F64) for i in range(6)]
     with S.Server(budget_bytes=1 << 30, fp32_compensated=True) as srv:
         ids = [srv.submit(j) for j in jobs]
         srv.wait_idle()
         res = {r.id: r for r in srv.poll()}
-    for j, i, a in zip(jobs, ids, ref):
+    plain_errors, bounds = [], []
+    for j, i in zip(jobs, ids):
+        a, b, c, d = (ae.Array(n, 3, torch.float32) for _ in range(4))
+        for k, x in enumerate((a, b, c)):
+            ae.array_fill_uniform(x, j.seed + k, -1.0, 1.0)
+        plain = ae.eval_cfg0(a, b, c, d, ae.SUM_PLAIN).double().sum(1).cpu()
+        d64 = ae.Array(n, 3, torch.float64)
+        d64.data[:, :n].copy_(d.data[:, :n])
+        ref = ae.reduce(d64, ae.SUM_NEUMAIER).sum(1).cpu()
+        abs_sum = d64.data[:, :n].abs().sum(1).cpu()
         for k in range(3):
-            s32 = res[i].value[2 * k] + res[i].value[2 * k + 1]
-            s64 = a.value[2 * k] + a.value[2 * k + 1]
-            # FP32 inputs are the FP64 draws rounded to float; the sum error is that input rounding,
-            # not accumulation: relative to sum |d| it stays near 2^-24
-            assert abs(s32 - s64) <= 4 * 2.0**-24 * (1 << 15) * 2.5
+            served = res[i].value[2 * k] + res[i].value[2 * k + 1]
+            bound = 2.0**-44 * abs_sum[k].item()
+            assert abs(served - ref[k].item()) <= bound, (k, served, ref[k].item())
+            plain_errors.append(abs(plain[k].item() - ref[k].item()))
+            bounds.append(bound)
+    assert max(e / b for e, b in zip(plain_errors, bounds)) > 1.0, "plain FP32 would pass: the bound shows nothing"
 
 
 def test_replay_open_loop(dev):

@@ -111,3 +111,27 @@ def test_bits_moments(orc):
     z = orc.fill_normal_f64(42, 1 << 16)
     assert abs(z.mean()) < 6 / np.sqrt(z.size)
     assert abs(z.var() - 1) < 6 * np.sqrt(2 / z.size)
+
+
+def test_generator_normal_leaf_against_the_reference(orc):
+    """Generator(42).normal<double,3>(0.5, 2) and .substream(3).normal<float,3>(-1, 0.25), minted from
+    the reference's own CPU path (tests/golden/ref_generator_normal.npz): the normal leaf draws with seed
+    key ^ 0x13198A2E03707344 (random/detail/DistributionLeaf.h:184), counter = component. The oracle's
+    libm route reproduces the reference's bits; its own-math route (the device's) stays within the
+    reference's normal band (1e-12 double, 1e-5 float: tests/random/golden_random.h:437-441)."""
+    z = np.load(os.path.join(GOLDEN, "ref_generator_normal.npz"))
+    n = int(z["n"])
+    salt = 0x13198A2E03707344
+    key = orc.generator_key(42, 0)
+    sub = orc.substream_key(key, 3)
+    d0 = orc.fill_array_normal_f64(key ^ salt, 3, n, n, 0.5, 2.0, mathmode=0).reshape(3, n)
+    f0 = orc.fill_array_normal_f32(sub ^ salt, 3, n, n, -1.0, 0.25, mathmode=0).reshape(3, n)
+    assert np.array_equal(d0.view(np.uint64), z["d"].view(np.uint64))
+    assert np.array_equal(f0.view(np.uint32), z["f"].view(np.uint32))
+    d1 = orc.fill_array_normal_f64(key ^ salt, 3, n, n, 0.5, 2.0, mathmode=1).reshape(3, n)
+    f1 = orc.fill_array_normal_f32(sub ^ salt, 3, n, n, -1.0, 0.25, mathmode=1).reshape(3, n)
+    assert np.all(np.abs(d1 - z["d"]) <= 1e-12 * np.maximum(1.0, np.abs(z["d"])))
+    assert np.all(np.abs(f1 - z["f"]) <= 1e-5 * np.maximum(1.0, np.abs(z["f"])))
+    # the uniform leaf's salt in its place draws a different stream entirely
+    wrong = orc.fill_array_normal_f64(key ^ 0x243F6A8885A308D3, 3, n, n, 0.5, 2.0, mathmode=0).reshape(3, n)
+    assert np.mean(wrong == z["d"]) < 0.01
