@@ -18,6 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libaether_hip.so")
 API_CHECK = os.path.join(LIB_DIR, "api_check")
+RESIDENCY_CHECK = os.path.join(LIB_DIR, "residency_check")
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AETH_OFFLOAD_ARCH", "gfx950")
@@ -93,18 +94,23 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-def build_api_check() -> str:
-    """The C++ front end's test driver (tests/cpp/api_check.cpp): a host-only g++ build
-    against include/aether/aether.hpp and libaether_hip.so, no HIP headers involved."""
-    src = os.path.join(ROOT, "tests", "cpp", "api_check.cpp")
+def _host_driver(name: str, out: str) -> str:
+    src = os.path.join(ROOT, "tests", "cpp", name)
     cmd = ["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-Wextra", "-Werror",
-           "-I", os.path.join(ROOT, "include"), src, "-o", API_CHECK + ".tmp", "-L", LIB_DIR, "-laether_hip",
+           "-I", os.path.join(ROOT, "include"), src, "-o", out + ".tmp", "-L", LIB_DIR, "-laether_hip",
            "-Wl,-rpath,$ORIGIN"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"g++ failed for {src}:\n{res.stderr}")
-    os.replace(API_CHECK + ".tmp", API_CHECK)
-    return API_CHECK
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_api_check() -> str:
+    """The C++ front end's test drivers (tests/cpp/api_check.cpp, tests/cpp/residency_check.cpp): host-only
+    g++ builds against include/aether/*.hpp and libaether_hip.so, no HIP headers involved."""
+    _host_driver("residency_check.cpp", RESIDENCY_CHECK)
+    return _host_driver("api_check.cpp", API_CHECK)
 
 
 def build_dsc_core() -> str:

@@ -344,3 +344,79 @@ void drain_device_frees()
 } // namespace aeth
 
 extern "C" __attribute__((visibility("default"))) const AetherMemoryV1* aether_memory_v1(void) { return &aeth::kTable; }
+
+// ---------------------------------------------------------------- residency support
+namespace {
+std::mutex g_peer_mu;
+std::vector<std::pair<int, int>> g_peer_enabled; // (from, to) pairs with peer access turned on
+
+void enable_peer(int from, int to)
+{
+    for (const auto& e : g_peer_enabled)
+        if (e.first == from && e.second == to) return;
+    aeth::OnDevice on(from);
+    const hipError_t e = hipDeviceEnablePeerAccess(to, 0);
+    if (e != hipSuccess && e != hipErrorPeerAccessAlreadyEnabled) aeth::check(e, "hipDeviceEnablePeerAccess");
+    (void)hipGetLastError();
+    g_peer_enabled.emplace_back(from, to);
+}
+} // namespace
+
+extern "C" int aeth_peer_route(int kind_a, int dev_a, int kind_b, int dev_b, int force_staging, int* route)
+{
+    return aeth::guarded([&] {
+        if (!route) aeth::fail("aeth_peer_route", "route is NULL");
+        if (kind_a != AETHER_MEM_DEVICE || kind_b != AETHER_MEM_DEVICE) {
+            *route = AETH_ROUTE_HOST;
+            return;
+        }
+        if (force_staging) {
+            *route = AETH_ROUTE_STAGED;
+            return;
+        }
+        if (dev_a == dev_b) {
+            *route = AETH_ROUTE_SAME;
+            return;
+        }
+        int ab = 0, ba = 0;
+        aeth::check(hipDeviceCanAccessPeer(&ab, dev_a, dev_b), "hipDeviceCanAccessPeer");
+        aeth::check(hipDeviceCanAccessPeer(&ba, dev_b, dev_a), "hipDeviceCanAccessPeer");
+        if (!(ab && ba)) {
+            *route = AETH_ROUTE_STAGED;
+            return;
+        }
+        std::lock_guard<std::mutex> lock(g_peer_mu);
+        enable_peer(dev_a, dev_b);
+        enable_peer(dev_b, dev_a);
+        *route = AETH_ROUTE_P2P;
+    });
+}
+
+extern "C" int aeth_event_record(void* stream, void** event)
+{
+    return aeth::guarded([&] {
+        if (!event) aeth::fail("aeth_event_record", "event is NULL");
+        hipEvent_t ev = nullptr;
+        aeth::check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreateWithFlags");
+        const hipError_t e = hipEventRecord(ev, aeth::as_stream(stream));
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(ev);
+            aeth::check(e, "hipEventRecord");
+        }
+        *event = ev;
+    });
+}
+
+extern "C" int aeth_event_synchronize(void* event)
+{
+    return aeth::guarded([&] {
+        if (event) aeth::check(hipEventSynchronize(static_cast<hipEvent_t>(event)), "hipEventSynchronize");
+    });
+}
+
+extern "C" int aeth_event_destroy(void* event)
+{
+    return aeth::guarded([&] {
+        if (event) aeth::check(hipEventDestroy(static_cast<hipEvent_t>(event)), "hipEventDestroy");
+    });
+}

@@ -896,6 +896,17 @@ public:
     template<class E>
     void assign(const E& e, SumMode mode, double* sums_host, Stream stream)
     {
+        T pairs[2 * N];
+        assign_pairs(e, mode, pairs, stream);
+        if (mode != SumMode::None)
+            for (std::size_t c = 0; c < N; ++c) sums_host[c] = (double)pairs[2 * c] + (double)pairs[2 * c + 1];
+    }
+
+    /** dest = e with the sums as (hi, comp) pairs of T per component in pairs_host (2N values, when
+     *  mode != None): the form sums of partial arrays combine in (combine_ranks). */
+    template<class E>
+    void assign_pairs(const E& e, SumMode mode, T* pairs_host, Stream stream)
+    {
         static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
         if (!device_) throw Error("aether: whole-array assignment needs a device view");
         const detail::TapeCtx x = detail::lower<T, N>(p_, pitch_, e);
@@ -910,13 +921,10 @@ public:
                                       const_cast<void* const*>(x.leaves.data()), x.pitch.data(), (int)x.consts.size(),
                                       x.consts.data(), (int)x.rngs.size(), x.rngs.data(), samples_, (int)mode,
                                       sums_dev, stream);
-        if (rc == 0 && mode != SumMode::None) {
-            T h[2 * N];
-            detail::check(detail::seam().copy(AETHER_MEM_CPU, dev, h, AETHER_MEM_DEVICE, dev, sums_dev,
-                                              sizeof h, stream),
+        if (rc == 0 && mode != SumMode::None)
+            detail::check(detail::seam().copy(AETHER_MEM_CPU, dev, pairs_host, AETHER_MEM_DEVICE, dev, sums_dev,
+                                              2 * N * sizeof(T), stream),
                           "copy");
-            for (std::size_t c = 0; c < N; ++c) sums_host[c] = (double)h[2 * c] + (double)h[2 * c + 1];
-        }
         if (chunk) detail::seam().chunk_release(chunk);
         detail::check(rc, "aeth_eval_tape");
     }
@@ -1085,6 +1093,18 @@ std::array<double, N> evaluate(View<T, N> dest, const E& e, SumMode mode = SumMo
     std::array<double, N> sums{};
     dest.assign(e, mode, sums.data(), s);
     return sums;
+}
+
+/** dest = e with fused per-component sums kept as (hi, comp) pairs of T. */
+template<class T, std::size_t N, class E>
+std::array<std::array<T, 2>, N> evaluate_pairs(View<T, N> dest, const E& e, SumMode mode = SumMode::Neumaier,
+                                               Stream s = nullptr)
+{
+    T flat[2 * N];
+    dest.assign_pairs(e, mode, flat, s);
+    std::array<std::array<T, 2>, N> out{};
+    for (std::size_t c = 0; c < N; ++c) out[c] = {flat[2 * c], flat[2 * c + 1]};
+    return out;
 }
 
 /** Pick the fastest launch shape for this expression's structure on this device (no counterpart in the

@@ -369,6 +369,22 @@ enum { AETH_CLAIM_LANE = 0, AETH_CLAIM_WAVE = 1, AETH_CLAIM_TILE = 2 };
 int aeth_append_where(int dtype, const void* x, uint64_t n, double threshold, uint32_t* out, uint64_t capacity,
                       uint32_t* counter, uint32_t* overflow, int claim, void* stream);
 
+/* ------------------------------------------------------------------ residency */
+/* How a copy between two memory places travels (residency/PeerAccess.h:
+ * 57-129, peerRoute): HOST unless both places are DEVICE memory; STAGED when
+ * force_staging is set (through a pinned bounce buffer, the path a GPU pair
+ * without peer access takes); SAME for one device; P2P when each device can
+ * reach the other (peer access is then enabled both ways, once per process);
+ * STAGED otherwise. kinds are the seam's AETHER_MEM_* kinds. */
+enum { AETH_ROUTE_SAME = 0, AETH_ROUTE_P2P = 1, AETH_ROUTE_STAGED = 2, AETH_ROUTE_HOST = 3 };
+int aeth_peer_route(int kind_a, int dev_a, int kind_b, int dev_b, int force_staging, int* route);
+/* Completion markers for asynchronous transfers (residency/Transport.h:
+ * 175-205, the transport's lastEvent/wait/fence): record creates an event
+ * and records it on `stream`; synchronize waits for it; destroy frees it. */
+int aeth_event_record(void* stream, void** event);
+int aeth_event_synchronize(void* event);
+int aeth_event_destroy(void* event);
+
 /* ------------------------------------------------------------------ runtime evaluator */
 /* eval::runtimeEval over RuntimeViews (eval/Runtime.h:1-342, view/RuntimeView.h:71-120): elementwise
  * ops over strided N-d views of device memory, each operand addressed through its own element
