@@ -21,6 +21,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -856,6 +857,330 @@ bool eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_le
     }
 }
 
+// ---------------------------------------------------------------------------
+// Trees of any size (aeth_eval_tape_ex). A program that fits one launch's limits runs as before. A larger
+// one is cut into pieces that fit: the largest subtrees are evaluated first into temporary planes of T, and
+// the pieces above them load those planes. Every node still rounds to T exactly once, so the values are
+// the same as one straight-line evaluation; pieces run in dependency order on the caller's stream, and the
+// programs of an assignment that has to be cut run one after another in component order (the order in
+// which the reference stores a sample's components, expr/Assign.h:124-160).
+struct TreeNode {
+    int op;
+    uint32_t arg;
+    int a = -1, b = -1; // children (unary: a)
+};
+
+struct PieceCost {
+    int insns = 0, depth = 0;
+    std::vector<uint64_t> loads; // (leaf << 32 | comp), or temps as (1 << 63 | temp)
+    std::vector<uint32_t> consts, rngs;
+};
+
+void add_unique(std::vector<uint64_t>& v, uint64_t x)
+{
+    if (std::find(v.begin(), v.end(), x) == v.end()) v.push_back(x);
+}
+void add_unique(std::vector<uint32_t>& v, uint32_t x)
+{
+    if (std::find(v.begin(), v.end(), x) == v.end()) v.push_back(x);
+}
+
+struct Splitter {
+    const aeth_program_ex& P;
+    std::vector<TreeNode> nodes;
+    std::vector<int> temp_of; // node -> temp index, or -1
+    int root = -1;
+    int temps = 0;
+
+    explicit Splitter(const aeth_program_ex& p) : P(p)
+    {
+        std::vector<int> st;
+        for (uint32_t i = 0; i < P.n_insns; ++i) {
+            TreeNode t{P.insns[i].op, P.insns[i].arg};
+            if (is_binary(t.op)) {
+                t.b = st.back();
+                st.pop_back();
+                t.a = st.back();
+                st.pop_back();
+            } else if (is_unary(t.op)) {
+                t.a = st.back();
+                st.pop_back();
+            }
+            nodes.push_back(t);
+            st.push_back((int)nodes.size() - 1);
+        }
+        root = st.back();
+        temp_of.assign(nodes.size(), -1);
+    }
+
+    bool leafish(int k) const { return nodes[k].a < 0 || temp_of[k] >= 0; }
+
+    // The cost of evaluating node k's piece: its subtree down to leaves and materialized nodes.
+    void cost(int k, PieceCost& c, bool top) const
+    {
+        const TreeNode& t = nodes[k];
+        if (!top && temp_of[k] >= 0) {
+            add_unique(c.loads, (1ull << 63) | (uint64_t)temp_of[k]);
+            c.insns += 1;
+            c.depth = 1;
+            return;
+        }
+        if (t.op == AETH_OP_PUSH) {
+            add_unique(c.loads, ((uint64_t)P.load_leaf[t.arg] << 32) | P.load_comp[t.arg]);
+            c.insns += 1;
+            c.depth = 1;
+        } else if (t.op == AETH_OP_CONST) {
+            add_unique(c.consts, t.arg);
+            c.insns += 1;
+            c.depth = 1;
+        } else if (t.op == AETH_OP_RNG) {
+            add_unique(c.rngs, t.arg);
+            c.insns += 1;
+            c.depth = 1;
+        } else if (t.b < 0) {
+            cost(t.a, c, false);
+            c.insns += 1;
+        } else {
+            cost(t.a, c, false);
+            const int da = c.depth;
+            cost(t.b, c, false);
+            c.depth = std::max(da, c.depth + 1);
+            c.insns += 1;
+        }
+    }
+
+    static bool fits(const PieceCost& c)
+    {
+        // loads leave one leaf slot for the piece's output plane
+        return c.insns <= AETH_TAPE_MAX_INSNS && c.depth <= AETH_TAPE_MAX_DEPTH &&
+               (int)c.loads.size() < AETH_TAPE_MAX_LEAVES && (int)c.loads.size() <= AETH_TAPE_MAX_LOADS &&
+               (int)c.consts.size() <= AETH_TAPE_MAX_CONSTS && (int)c.rngs.size() <= AETH_TAPE_MAX_RNGS;
+    }
+
+    int size(int k) const
+    {
+        if (temp_of[k] >= 0 || nodes[k].a < 0) return 1;
+        return 1 + size(nodes[k].a) + (nodes[k].b >= 0 ? size(nodes[k].b) : 0);
+    }
+
+    // Make node k's piece fit by materializing its largest non-trivial children, deepest first.
+    void fit(int k)
+    {
+        for (;;) {
+            PieceCost c;
+            cost(k, c, true);
+            if (fits(c)) return;
+            const TreeNode& t = nodes[k];
+            int pick = -1;
+            for (int ch : {t.a, t.b})
+                if (ch >= 0 && !leafish(ch) && (pick < 0 || size(ch) > size(pick))) pick = ch;
+            if (pick < 0) fail("aeth_eval_tape_ex", "a node cannot be made to fit one program"); // unreachable
+            fit(pick);
+            temp_of[pick] = temps++;
+        }
+    }
+};
+
+// A tape for node k's piece, its planes in `loads` (leaf pointer, pitch, comp) and constants / random
+// leaves remapped into the piece's own tables.
+struct Piece {
+    std::vector<aeth_insn> insns;
+    std::vector<std::pair<const void*, uint64_t>> leaves; // pointer, pitch
+    std::vector<std::pair<int, uint32_t>> loads;          // (piece leaf, comp)
+    std::vector<uint32_t> consts, rngs;                   // indices into the caller's tables
+};
+
+void emit_piece(const Splitter& S, int k, bool top, Piece& pc, void* const* leaves, const uint64_t* pitch,
+                const std::vector<void*>& temp_ptr, uint64_t temp_pitch)
+{
+    const TreeNode& t = S.nodes[k];
+    auto load = [&](const void* ptr, uint64_t pt, uint32_t comp) {
+        int li = -1;
+        for (size_t q = 0; q < pc.leaves.size(); ++q)
+            if (pc.leaves[q].first == ptr) li = (int)q;
+        if (li < 0) {
+            pc.leaves.emplace_back(ptr, pt);
+            li = (int)pc.leaves.size() - 1;
+        }
+        const std::pair<int, uint32_t> key{li, comp};
+        size_t j = 0;
+        while (j < pc.loads.size() && pc.loads[j] != key) ++j;
+        if (j == pc.loads.size()) pc.loads.push_back(key);
+        pc.insns.push_back(aeth_insn{AETH_OP_PUSH, (uint8_t)j});
+    };
+    auto index_of = [](std::vector<uint32_t>& v, uint32_t x) {
+        size_t j = 0;
+        while (j < v.size() && v[j] != x) ++j;
+        if (j == v.size()) v.push_back(x);
+        return (uint8_t)j;
+    };
+    if (!top && S.temp_of[k] >= 0) {
+        load(temp_ptr[S.temp_of[k]], temp_pitch, 0);
+        return;
+    }
+    if (t.op == AETH_OP_PUSH) {
+        const uint32_t leaf = S.P.load_leaf[t.arg];
+        load(leaves[leaf], pitch[leaf], S.P.load_comp[t.arg]);
+    } else if (t.op == AETH_OP_CONST) {
+        pc.insns.push_back(aeth_insn{AETH_OP_CONST, index_of(pc.consts, t.arg)});
+    } else if (t.op == AETH_OP_RNG) {
+        pc.insns.push_back(aeth_insn{AETH_OP_RNG, index_of(pc.rngs, t.arg)});
+    } else {
+        emit_piece(S, t.a, false, pc, leaves, pitch, temp_ptr, temp_pitch);
+        if (t.b >= 0) emit_piece(S, t.b, false, pc, leaves, pitch, temp_ptr, temp_pitch);
+        pc.insns.push_back(aeth_insn{(uint8_t)t.op, 0});
+    }
+}
+
+// Run one piece as a one-program assignment into plane (out_ptr, comp 0 of a leaf of pitch out_pitch).
+void run_piece(int dtype, Piece& pc, void* out_ptr, uint64_t out_pitch, uint32_t out_comp, const double* consts,
+               const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums, hipStream_t st)
+{
+    int out_leaf = -1;
+    for (size_t q = 0; q < pc.leaves.size(); ++q)
+        if (pc.leaves[q].first == out_ptr) out_leaf = (int)q;
+    if (out_leaf < 0) {
+        pc.leaves.emplace_back(out_ptr, out_pitch);
+        out_leaf = (int)pc.leaves.size() - 1;
+    }
+    aeth_program prog;
+    std::memset(&prog, 0, sizeof prog);
+    prog.n_loads = (uint8_t)pc.loads.size();
+    prog.n_insns = (uint8_t)pc.insns.size();
+    prog.out_leaf = (uint8_t)out_leaf;
+    prog.out_comp = (uint8_t)out_comp;
+    for (size_t q = 0; q < pc.loads.size(); ++q) {
+        prog.load_leaf[q] = (uint8_t)pc.loads[q].first;
+        prog.load_comp[q] = (uint8_t)pc.loads[q].second;
+    }
+    for (size_t i = 0; i < pc.insns.size(); ++i) prog.insns[i] = pc.insns[i];
+    std::vector<void*> lp;
+    std::vector<uint64_t> lpitch;
+    for (const auto& l : pc.leaves) {
+        lp.push_back(const_cast<void*>(l.first));
+        lpitch.push_back(l.second);
+    }
+    std::vector<double> cs;
+    for (uint32_t c : pc.consts) cs.push_back(consts[c]);
+    std::vector<aeth_rng_leaf> rs;
+    for (uint32_t r : pc.rngs) rs.push_back(rngs[r]);
+    eval_tape(dtype, 1, &prog, (int)lp.size(), lp.data(), lpitch.data(), (int)cs.size(), cs.data(), (int)rs.size(),
+              rs.data(), n, sum_mode, sums, st, nullptr);
+}
+
+void validate_ex(const aeth_program_ex& p, int n_leaves, int n_consts, int n_rngs)
+{
+    if (p.n_insns == 0 || !p.insns) fail("aeth_eval_tape_ex", "a program needs instructions");
+    if (p.n_loads && (!p.load_leaf || !p.load_comp)) fail("aeth_eval_tape_ex", "load tables are NULL");
+    if ((int)p.out_leaf >= n_leaves) fail("aeth_eval_tape_ex", "output leaf out of range");
+    for (uint32_t k = 0; k < p.n_loads; ++k)
+        if ((int)p.load_leaf[k] >= n_leaves) fail("aeth_eval_tape_ex", "load leaf out of range");
+    int64_t d = 0;
+    for (uint32_t i = 0; i < p.n_insns; ++i) {
+        const int op = p.insns[i].op;
+        const uint32_t arg = p.insns[i].arg;
+        if (op == AETH_OP_PUSH) {
+            if (arg >= p.n_loads) fail("aeth_eval_tape_ex", "PUSH index out of range");
+            ++d;
+        } else if (op == AETH_OP_CONST) {
+            if ((int64_t)arg >= n_consts) fail("aeth_eval_tape_ex", "CONST index out of range");
+            ++d;
+        } else if (op == AETH_OP_RNG) {
+            if ((int64_t)arg >= n_rngs) fail("aeth_eval_tape_ex", "RNG index out of range");
+            ++d;
+        } else if (is_binary(op)) {
+            if (d < 2) fail("aeth_eval_tape_ex", "stack underflow");
+            --d;
+        } else if (is_unary(op)) {
+            if (d < 1) fail("aeth_eval_tape_ex", "stack underflow");
+        } else {
+            fail("aeth_eval_tape_ex", "unknown opcode");
+        }
+    }
+    if (d != 1) fail("aeth_eval_tape_ex", "a program must leave exactly one value");
+}
+
+bool fits_one_launch(const aeth_program_ex& p)
+{
+    if (p.n_loads > AETH_TAPE_MAX_LOADS || p.n_insns > AETH_TAPE_MAX_INSNS) return false;
+    int d = 0, dmax = 0;
+    for (uint32_t i = 0; i < p.n_insns; ++i) {
+        const int op = p.insns[i].op;
+        if (p.insns[i].arg > 255) return false;
+        if (op == AETH_OP_PUSH || op == AETH_OP_CONST || op == AETH_OP_RNG) ++d;
+        else if (is_binary(op)) --d;
+        dmax = std::max(dmax, d);
+    }
+    return dmax <= AETH_TAPE_MAX_DEPTH;
+}
+
+void eval_tape_ex(int dtype, int n_programs, const aeth_program_ex* programs, int n_leaves, void* const* leaves,
+                  const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs, const aeth_rng_leaf* rngs,
+                  uint64_t n, int sum_mode, void* sums, void* stream)
+{
+    if (dtype != AETH_F64 && dtype != AETH_F32) fail("aeth_eval_tape_ex", "unknown dtype");
+    if (n_programs < 1 || n_programs > 64) fail("aeth_eval_tape_ex", "1..64 programs");
+    if (n_leaves < 1 || !leaves || !leaf_pitch) fail("aeth_eval_tape_ex", "no leaves");
+    if (n_consts < 0 || n_rngs < 0) fail("aeth_eval_tape_ex", "negative table size");
+    if (n > 0xFFFFFFFFull) fail("aeth_eval_tape_ex", "n beyond the 32-bit offset_t");
+    if (sum_mode != AETH_SUM_NONE && !sums) fail("aeth_eval_tape_ex", "sums is NULL");
+    for (int p = 0; p < n_programs; ++p) validate_ex(programs[p], n_leaves, n_consts, n_rngs);
+    bool one = n_programs <= AETH_TAPE_MAX_PROGRAMS && n_leaves <= AETH_TAPE_MAX_LEAVES &&
+               n_consts <= AETH_TAPE_MAX_CONSTS && n_rngs <= AETH_TAPE_MAX_RNGS;
+    for (int p = 0; p < n_programs && one; ++p) one = fits_one_launch(programs[p]) && programs[p].out_leaf < 256 &&
+                                                      programs[p].out_comp < 256;
+    if (one) { // the plain tape: one launch over all programs
+        std::vector<aeth_program> ps(n_programs);
+        for (int p = 0; p < n_programs; ++p) {
+            const aeth_program_ex& e = programs[p];
+            aeth_program& q = ps[p];
+            std::memset(&q, 0, sizeof q);
+            q.n_loads = (uint8_t)e.n_loads;
+            q.n_insns = (uint8_t)e.n_insns;
+            q.out_leaf = (uint8_t)e.out_leaf;
+            q.out_comp = (uint8_t)e.out_comp;
+            for (uint32_t k = 0; k < e.n_loads; ++k) {
+                q.load_leaf[k] = (uint8_t)e.load_leaf[k];
+                q.load_comp[k] = (uint8_t)e.load_comp[k];
+            }
+            for (uint32_t i = 0; i < e.n_insns; ++i) q.insns[i] = aeth_insn{e.insns[i].op, (uint8_t)e.insns[i].arg};
+        }
+        eval_tape(dtype, n_programs, ps.data(), n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs, rngs, n,
+                  sum_mode, sums, stream, nullptr);
+        return;
+    }
+    // cut into pieces; the programs run in component order
+    const size_t es = dtype == AETH_F64 ? 8 : 4;
+    const uint64_t temp_pitch = (n + 31) / 32 * 32;
+    hipStream_t st = as_stream(stream);
+    for (int p = 0; p < n_programs; ++p) {
+        const aeth_program_ex& P = programs[p];
+        Splitter S(P);
+        S.fit(S.root);
+        std::vector<std::unique_ptr<StreamScratch>> temps;
+        std::vector<void*> temp_ptr(S.temps, nullptr);
+        for (int t = 0; t < S.temps; ++t) {
+            temps.emplace_back(new StreamScratch(std::max<uint64_t>(temp_pitch, 32) * es, st));
+            temp_ptr[t] = temps.back()->get();
+        }
+        // materialized nodes in the order they were cut (children before parents)
+        std::vector<int> order(S.temps, -1);
+        for (size_t k = 0; k < S.nodes.size(); ++k)
+            if (S.temp_of[k] >= 0) order[S.temp_of[k]] = (int)k;
+        for (int t = 0; t < S.temps; ++t) {
+            Piece pc;
+            emit_piece(S, order[t], true, pc, leaves, leaf_pitch, temp_ptr, temp_pitch);
+            run_piece(dtype, pc, temp_ptr[t], temp_pitch, 0, consts, rngs, n, AETH_SUM_NONE, nullptr, st);
+        }
+        Piece top;
+        emit_piece(S, S.root, true, top, leaves, leaf_pitch, temp_ptr, temp_pitch);
+        void* out = static_cast<char*>(leaves[P.out_leaf]);
+        void* psums = sums ? static_cast<char*>(sums) + 2 * es * (size_t)p : nullptr;
+        // the output plane goes in as comp P.out_comp of its leaf
+        run_piece(dtype, top, out, leaf_pitch[P.out_leaf], P.out_comp, consts, rngs, n, sum_mode, psums, st);
+    }
+}
+
 } // namespace
 
 extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_leaves,
@@ -866,6 +1191,17 @@ extern "C" int aeth_eval_tape(int dtype, int n_programs, const aeth_program* pro
     return guarded([&] {
         eval_tape(dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs, rngs, n,
                   sum_mode, sums, stream, nullptr);
+    });
+}
+
+extern "C" int aeth_eval_tape_ex(int dtype, int n_programs, const aeth_program_ex* programs, int n_leaves,
+                                 void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts,
+                                 int n_rngs, const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums,
+                                 void* stream)
+{
+    return guarded([&] {
+        eval_tape_ex(dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs, rngs, n,
+                     sum_mode, sums, stream);
     });
 }
 

@@ -39,6 +39,16 @@ class aeth_program(C.Structure):
                 ("insns", aeth_insn * MAX_INSNS)]
 
 
+class aeth_insn_ex(C.Structure):
+    _fields_ = [("op", C.c_uint8), ("pad", C.c_uint8 * 3), ("arg", C.c_uint32)]
+
+
+class aeth_program_ex(C.Structure):
+    _fields_ = [("n_loads", C.c_uint32), ("n_insns", C.c_uint32), ("out_leaf", C.c_uint32), ("out_comp", C.c_uint32),
+                ("load_leaf", C.POINTER(C.c_uint32)), ("load_comp", C.POINTER(C.c_uint32)),
+                ("insns", C.POINTER(aeth_insn_ex))]
+
+
 class aeth_rng_leaf(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("p0", C.c_double), ("p1", C.c_double), ("comp", C.c_uint32),
                 ("dist", C.c_uint32)]
@@ -381,8 +391,6 @@ class _Ctx:
         for k, x in enumerate(self.leaves):
             if x is arr:
                 return k
-        if len(self.leaves) >= MAX_LEAVES:
-            raise ValueError("too many leaves")
         self.leaves.append(arr)
         return len(self.leaves) - 1
 
@@ -407,36 +415,84 @@ class _Ctx:
         self.insns.append((OP[name], 0))
 
 
-def lower(dest: Array, expr: Expr):
-    """Lower `dest = expr` to (programs, leaves, consts, rngs)."""
+def _programs(dest: Array, expr):
+    """Per destination component: (loads, insns, out_leaf), with the shared leaf, constant and random-leaf
+    tables."""
     expr = _wrap(expr)
     if expr.ncomp not in (1, dest.ncomp):
         raise ValueError("expression and destination component counts differ")
-    leaves, consts, rngs = [], [], []
-    progs = (aeth_program * dest.ncomp)()
+    leaves, consts, rngs, out = [], [], [], []
     for c in range(dest.ncomp):
         ctx = _Ctx(dest.dtype, leaves, consts, rngs)
         expr.emit(c if expr.ncomp > 1 else 0, ctx)
-        out_leaf = ctx._leaf(dest)
-        p = progs[c]
-        if len(ctx.loads) > MAX_LOADS or len(ctx.insns) > MAX_INSNS:
+        out.append((ctx.loads, ctx.insns, ctx._leaf(dest)))
+    return out, leaves, consts, rngs
+
+
+def _stack_depth(insns):
+    d = peak = 0
+    for o, _ in insns:
+        if o in (OP["push"], OP["const"], OP["rng"]):
+            d += 1
+        elif o in _BINARY:
+            d -= 1
+        peak = max(peak, d)
+    return peak
+
+
+_BINARY = {OP[k] for k in ("+", "-", "*", "/", "fmin", "fmax", "selgt", "pow")}
+
+
+def lower(dest: Array, expr: Expr):
+    """Lower `dest = expr` to the programs of ONE launch (programs, leaves, consts, rngs); ValueError when the
+    tree is beyond one launch's limits (assign() still runs it, cut into pieces: lower_ex)."""
+    progs_list, leaves, consts, rngs = _programs(dest, expr)
+    if (dest.ncomp > MAX_PROGRAMS or len(leaves) > MAX_LEAVES or len(consts) > MAX_CONSTS
+            or len(rngs) > MAX_RNGS):
+        raise ValueError("too many components, arrays, constants or random leaves for one launch")
+    progs = (aeth_program * dest.ncomp)()
+    for c, (loads, insns, out_leaf) in enumerate(progs_list):
+        if len(loads) > MAX_LOADS or len(insns) > MAX_INSNS or _stack_depth(insns) > 8:
             raise ValueError("expression too large for one program")
-        p.n_loads, p.n_insns, p.out_leaf, p.out_comp = len(ctx.loads), len(ctx.insns), out_leaf, c
-        for k, (lf, cp) in enumerate(ctx.loads):
+        p = progs[c]
+        p.n_loads, p.n_insns, p.out_leaf, p.out_comp = len(loads), len(insns), out_leaf, c
+        for k, (lf, cp) in enumerate(loads):
             p.load_leaf[k], p.load_comp[k] = lf, cp
-        for k, (o, a) in enumerate(ctx.insns):
+        for k, (o, a) in enumerate(insns):
             p.insns[k].op, p.insns[k].arg = o, a
-    if len(consts) > MAX_CONSTS or len(rngs) > MAX_RNGS:
-        raise ValueError("too many constants or random leaves")
     return progs, leaves, consts, rngs
+
+
+def lower_ex(dest: Array, expr: Expr):
+    """Lower `dest = expr` without size limits (aeth_eval_tape_ex): (programs, keep-alive tables, leaves,
+    consts, rngs)."""
+    progs_list, leaves, consts, rngs = _programs(dest, expr)
+    progs = (aeth_program_ex * dest.ncomp)()
+    keep = []
+    for c, (loads, insns, out_leaf) in enumerate(progs_list):
+        ll = (C.c_uint32 * max(1, len(loads)))(*[lf for lf, _ in loads])
+        lc = (C.c_uint32 * max(1, len(loads)))(*[cp for _, cp in loads])
+        ins = (aeth_insn_ex * len(insns))()
+        for k, (o, a) in enumerate(insns):
+            ins[k].op, ins[k].arg = o, a
+        keep += [ll, lc, ins]
+        p = progs[c]
+        p.n_loads, p.n_insns, p.out_leaf, p.out_comp = len(loads), len(insns), out_leaf, c
+        p.load_leaf, p.load_comp = C.cast(ll, C.POINTER(C.c_uint32)), C.cast(lc, C.POINTER(C.c_uint32))
+        p.insns = C.cast(ins, C.POINTER(aeth_insn_ex))
+    return progs, keep, leaves, consts, rngs
 
 
 class aeth_tape_shape(C.Structure):
     _fields_ = [("tile", C.c_uint32), ("vector_bytes", C.c_uint32), ("waves_per_simd", C.c_uint32), ("ms", C.c_float)]
 
 
-def _abi_args(dest: Array, expr):
-    progs, leaves, consts, rngs = lower(dest, _wrap(expr))
+def _abi_args(dest: Array, expr, unbounded: bool = False):
+    if unbounded:
+        progs, keep, leaves, consts, rngs = lower_ex(dest, _wrap(expr))
+    else:
+        progs, leaves, consts, rngs = lower(dest, _wrap(expr))
+        keep = None
     for lf in leaves:
         if lf.n != dest.n or lf.dtype != dest.dtype:
             raise ValueError("all arrays of an assignment share sample count and dtype")
@@ -446,7 +502,7 @@ def _abi_args(dest: Array, expr):
     rg = (aeth_rng_leaf * max(1, len(rngs)))()
     for k, (seed, p0, p1, comp, dist) in enumerate(rngs):
         rg[k].seed, rg[k].p0, rg[k].p1, rg[k].comp, rg[k].dist = seed, p0, p1, comp, dist
-    return progs, len(leaves), ptrs, pitch, len(consts), cst, len(rngs), rg
+    return progs, len(leaves), ptrs, pitch, len(consts), cst, len(rngs), rg, keep
 
 
 def tune(dest: Array, expr, stream=None) -> dict:
@@ -454,7 +510,7 @@ def tune(dest: Array, expr, stream=None) -> dict:
     bytes per vector access, minimum waves per SIMD) runs on these arrays, the fastest is kept for this expression's structure
     (assignments without sums), and `dest` holds the result. Returns {"tile", "vector_bytes",
     "waves_per_simd", "ms"}."""
-    progs, nl, ptrs, pitch, nc, cst, nr, rg = _abi_args(dest, expr)
+    progs, nl, ptrs, pitch, nc, cst, nr, rg, _ = _abi_args(dest, expr)
     out = aeth_tape_shape()
     call("aeth_tape_tune", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg, dest.n,
          stream_handle(stream), C.byref(out))
@@ -462,10 +518,12 @@ def tune(dest: Array, expr, stream=None) -> dict:
 
 
 def assign(dest: Array, expr, sum_mode: int = -1, sums: torch.Tensor = None, stream=None):
-    """`dest = expr` over all samples as ONE kernel; returns (ncomp, 2) sums when sum_mode >= 0."""
-    progs, nl, ptrs, pitch, nc, cst, nr, rg = _abi_args(dest, expr)
+    """`dest = expr` over all samples: ONE kernel when the tree fits one launch, otherwise the tree cut into
+    pieces that do (same values); returns (ncomp, 2) sums when sum_mode >= 0."""
+    progs, nl, ptrs, pitch, nc, cst, nr, rg, keep = _abi_args(dest, expr, unbounded=True)
     if sum_mode >= 0 and sums is None:
         sums = torch.empty((dest.ncomp, 2), dtype=dest.dtype, device=dest.data.device)
-    call("aeth_eval_tape", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg, dest.n,
+    call("aeth_eval_tape_ex", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg, dest.n,
          sum_mode, sums.data_ptr() if sums is not None else None, stream_handle(stream))
+    del keep
     return sums

@@ -108,39 +108,47 @@ inline std::array<std::uint32_t, 4> philox_block(std::uint64_t seed, std::uint32
     return {x, y, z, w};
 }
 
-// ---- lowering context for aeth_eval_tape
+// ---- lowering context for aeth_eval_tape / aeth_eval_tape_ex
 struct TapeCtx {
     int dtype = AETH_F64;
     std::vector<const void*> leaves;
     std::vector<std::uint64_t> pitch;
     std::vector<double> consts;
     std::vector<aeth_rng_leaf> rngs;
-    std::vector<std::pair<std::uint8_t, std::uint8_t>> loads; // (leaf, comp) of the current program
-    std::vector<aeth_insn> insns;
+    std::vector<std::pair<std::uint32_t, std::uint32_t>> loads; // (leaf, comp) of the current program
+    std::vector<aeth_insn_ex> insns;                            // the current program's tape
+
+    struct Program {
+        std::vector<std::uint32_t> load_leaf, load_comp;
+        std::vector<aeth_insn_ex> insns;
+        std::uint32_t out_leaf = 0, out_comp = 0;
+    };
+    std::vector<Program> progs;         // one per destination component, any size
+    std::vector<aeth_program> programs; // the same as one launch's programs, when they fit (else empty)
 
     int leaf(const void* p, std::uint64_t pt)
     {
         for (std::size_t k = 0; k < leaves.size(); ++k)
             if (leaves[k] == p) return (int)k;
-        if (leaves.size() >= AETH_TAPE_MAX_LEAVES) throw Error("aether: too many arrays in one assignment");
         leaves.push_back(p);
         pitch.push_back(pt);
         return (int)leaves.size() - 1;
     }
+    void emit(int op, std::size_t arg) { insns.push_back(aeth_insn_ex{(std::uint8_t)op, {0, 0, 0}, (std::uint32_t)arg}); }
     void push(const void* p, std::uint64_t pt, std::size_t comp)
     {
-        const std::pair<std::uint8_t, std::uint8_t> key{(std::uint8_t)leaf(p, pt), (std::uint8_t)comp};
+        const std::pair<std::uint32_t, std::uint32_t> key{(std::uint32_t)leaf(p, pt), (std::uint32_t)comp};
         std::size_t j = 0;
         while (j < loads.size() && loads[j] != key) ++j;
         if (j == loads.size()) loads.push_back(key);
-        insns.push_back(aeth_insn{AETH_OP_PUSH, (std::uint8_t)j});
+        emit(AETH_OP_PUSH, j);
     }
     void constant(double v)
     {
         std::size_t j = 0;
         while (j < consts.size() && std::memcmp(&consts[j], &v, sizeof v) != 0) ++j;
         if (j == consts.size()) consts.push_back(v);
-        insns.push_back(aeth_insn{AETH_OP_CONST, (std::uint8_t)j});
+        emit(AETH_OP_CONST, j);
     }
     void rng(std::uint64_t seed, double p0, double p1, std::uint32_t comp, std::uint32_t dist)
     {
@@ -150,48 +158,85 @@ struct TapeCtx {
             if (r.seed == seed && r.p0 == p0 && r.p1 == p1 && r.comp == comp && r.dist == dist) break;
         }
         if (j == rngs.size()) rngs.push_back(aeth_rng_leaf{seed, p0, p1, comp, dist});
-        insns.push_back(aeth_insn{AETH_OP_RNG, (std::uint8_t)j});
+        emit(AETH_OP_RNG, j);
     }
-    void op(int o) { insns.push_back(aeth_insn{(std::uint8_t)o, 0}); }
+    void op(int o) { emit(o, 0); }
 
-    std::vector<aeth_program> programs; // one per destination component
+    /** The programs in aeth_eval_tape_ex's form (pointing into progs: valid while this context lives). */
+    std::vector<aeth_program_ex> ex() const
+    {
+        std::vector<aeth_program_ex> out(progs.size());
+        for (std::size_t c = 0; c < progs.size(); ++c) {
+            const Program& p = progs[c];
+            out[c] = aeth_program_ex{(std::uint32_t)p.load_leaf.size(), (std::uint32_t)p.insns.size(), p.out_leaf,
+                                     p.out_comp, p.load_leaf.data(), p.load_comp.data(), p.insns.data()};
+        }
+        return out;
+    }
 };
+
+inline int stack_depth(const std::vector<aeth_insn_ex>& insns)
+{
+    int d = 0, peak = 0;
+    for (const auto& i : insns) {
+        if (i.op == AETH_OP_PUSH || i.op == AETH_OP_CONST || i.op == AETH_OP_RNG) ++d;
+        else if ((i.op >= AETH_OP_ADD && i.op <= AETH_OP_DIV) || (i.op >= AETH_OP_FMIN && i.op <= AETH_OP_POW)) --d;
+        peak = std::max(peak, d);
+    }
+    return peak;
+}
 
 /**
  * Lower `dest = e` (dest: N planes `pitch` elements apart at `dest`) to one
- * program per component. Leaves are numbered in order of first use, the
- * destination after the first program's operands; constants and random
- * leaves are shared across programs.
+ * program per component, of any size. Leaves are numbered in order of first
+ * use, the destination after the first program's operands; constants and
+ * random leaves are shared across programs. When every program fits one
+ * launch (aeth_eval_tape's limits) `programs` holds that form too.
  */
 template<class T, std::size_t N, class E>
 TapeCtx lower(const void* dest, std::size_t pitch, const E& e)
 {
     static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
-    static_assert(N <= AETH_TAPE_MAX_PROGRAMS, "aether: too many components for one assignment");
+    static_assert(N <= 64, "aether: too many components for one assignment");
     TapeCtx x;
     x.dtype = dtype_code<T>();
-    x.programs.resize(N);
+    x.progs.resize(N);
+    bool fits = N <= AETH_TAPE_MAX_PROGRAMS;
     for (std::size_t c = 0; c < N; ++c) {
         x.loads.clear();
         x.insns.clear();
         e.emit(E::ncomp > 1 ? c : 0, x);
-        const int out = x.leaf(dest, pitch);
-        if (x.loads.size() > AETH_TAPE_MAX_LOADS || x.insns.size() > AETH_TAPE_MAX_INSNS)
-            throw Error("aether: expression too large for one program");
-        aeth_program& p = x.programs[c];
-        std::memset(&p, 0, sizeof p);
-        p.n_loads = (std::uint8_t)x.loads.size();
-        p.n_insns = (std::uint8_t)x.insns.size();
-        p.out_leaf = (std::uint8_t)out;
-        p.out_comp = (std::uint8_t)c;
-        for (std::size_t k = 0; k < x.loads.size(); ++k) {
-            p.load_leaf[k] = x.loads[k].first;
-            p.load_comp[k] = x.loads[k].second;
+        TapeCtx::Program& p = x.progs[c];
+        p.out_leaf = (std::uint32_t)x.leaf(dest, pitch);
+        p.out_comp = (std::uint32_t)c;
+        for (const auto& l : x.loads) {
+            p.load_leaf.push_back(l.first);
+            p.load_comp.push_back(l.second);
         }
-        for (std::size_t k = 0; k < x.insns.size(); ++k) p.insns[k] = x.insns[k];
+        p.insns = x.insns;
+        fits = fits && x.loads.size() <= AETH_TAPE_MAX_LOADS && x.insns.size() <= AETH_TAPE_MAX_INSNS &&
+               stack_depth(x.insns) <= AETH_TAPE_MAX_DEPTH;
     }
-    if (x.consts.size() > AETH_TAPE_MAX_CONSTS || x.rngs.size() > AETH_TAPE_MAX_RNGS)
-        throw Error("aether: too many constants or random leaves");
+    fits = fits && x.leaves.size() <= AETH_TAPE_MAX_LEAVES && x.consts.size() <= AETH_TAPE_MAX_CONSTS &&
+           x.rngs.size() <= AETH_TAPE_MAX_RNGS;
+    if (fits) {
+        x.programs.resize(N);
+        for (std::size_t c = 0; c < N; ++c) {
+            const TapeCtx::Program& p = x.progs[c];
+            aeth_program& q = x.programs[c];
+            std::memset(&q, 0, sizeof q);
+            q.n_loads = (std::uint8_t)p.load_leaf.size();
+            q.n_insns = (std::uint8_t)p.insns.size();
+            q.out_leaf = (std::uint8_t)p.out_leaf;
+            q.out_comp = (std::uint8_t)p.out_comp;
+            for (std::size_t k = 0; k < p.load_leaf.size(); ++k) {
+                q.load_leaf[k] = (std::uint8_t)p.load_leaf[k];
+                q.load_comp[k] = (std::uint8_t)p.load_comp[k];
+            }
+            for (std::size_t k = 0; k < p.insns.size(); ++k)
+                q.insns[k] = aeth_insn{p.insns[k].op, (std::uint8_t)p.insns[k].arg};
+        }
+    }
     return x;
 }
 
@@ -917,10 +962,11 @@ public:
             detail::check(detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, 2 * N * sizeof(T), stream, &chunk,
                                                      &sums_dev),
                           "chunk_alloc");
-        const int rc = aeth_eval_tape(x.dtype, (int)N, x.programs.data(), (int)x.leaves.size(),
-                                      const_cast<void* const*>(x.leaves.data()), x.pitch.data(), (int)x.consts.size(),
-                                      x.consts.data(), (int)x.rngs.size(), x.rngs.data(), samples_, (int)mode,
-                                      sums_dev, stream);
+        const std::vector<aeth_program_ex> progs = x.ex();
+        const int rc = aeth_eval_tape_ex(x.dtype, (int)N, progs.data(), (int)x.leaves.size(),
+                                         const_cast<void* const*>(x.leaves.data()), x.pitch.data(),
+                                         (int)x.consts.size(), x.consts.data(), (int)x.rngs.size(), x.rngs.data(),
+                                         samples_, (int)mode, sums_dev, stream);
         if (rc == 0 && mode != SumMode::None)
             detail::check(detail::seam().copy(AETHER_MEM_CPU, dev, pairs_host, AETHER_MEM_DEVICE, dev, sums_dev,
                                               2 * N * sizeof(T), stream),
@@ -936,6 +982,7 @@ public:
         static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
         if (!device_) throw Error("aether: tuning needs a device view");
         const detail::TapeCtx x = detail::lower<T, N>(p_, pitch_, e);
+        if (x.programs.empty()) throw Error("aether: tuning takes an expression that fits one launch");
         aeth_tape_shape chosen{};
         detail::check(aeth_tape_tune(x.dtype, (int)N, x.programs.data(), (int)x.leaves.size(),
                                      const_cast<void* const*>(x.leaves.data()), x.pitch.data(), (int)x.consts.size(),

@@ -190,6 +190,34 @@ int aeth_eval_tape(int dtype, int n_programs, const aeth_program* programs, int 
                    const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs,
                    const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums, void* stream);
 
+/* Trees of any size: programs whose loads, instructions and arguments are
+ * unbounded (32-bit indices), any number of leaves, constants and random
+ * leaves, up to 64 programs. What fits the limits above runs as aeth_eval_tape
+ * (one kernel). A larger program is cut into pieces that fit: its largest
+ * subtrees are evaluated first into temporary planes of dtype (stream-ordered
+ * allocations), the pieces above them load those planes. Every node still
+ * rounds once to dtype, so the values are the same as a single evaluation;
+ * sums apply to the outputs as above. Programs of an assignment that is cut
+ * run one after another in component order. No counterpart beyond the
+ * reference's templates, which lower any tree (expr/Assign.h:124-160). */
+typedef struct {
+    uint8_t op;
+    uint8_t pad[3];
+    uint32_t arg;
+} aeth_insn_ex;
+typedef struct {
+    uint32_t n_loads;
+    uint32_t n_insns;
+    uint32_t out_leaf;
+    uint32_t out_comp;
+    const uint32_t* load_leaf;  /* n_loads */
+    const uint32_t* load_comp;  /* n_loads */
+    const aeth_insn_ex* insns;  /* n_insns */
+} aeth_program_ex;
+int aeth_eval_tape_ex(int dtype, int n_programs, const aeth_program_ex* programs, int n_leaves, void* const* leaves,
+                      const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs,
+                      const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums, void* stream);
+
 /* Each distinct tape structure is lowered at run time (hipRTC) to its own
  * straight-line kernel, compiled once per process and cached; the resident
  * interpreter kernel runs a tape when the runtime compiler is unavailable or

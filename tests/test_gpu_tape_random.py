@@ -114,3 +114,69 @@ def test_random_trees_equal_numpy_and_oracle(dev, orc, dt):
         ok, bad = same_bits(got, want)
         assert ok, (checked, bad, got[bad], want[bad])
         checked += 1
+
+
+BIG_CONSTS = [0.5, 2.0, -1.25, 3.0, 0.1, 0.75, -0.3, 1.5, 2.5, -2.0, 0.125, 4.0, -0.7, 0.9, 1.1, -1.5, 0.05, 0.6,
+              -0.45, 1.75]
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_trees_beyond_one_launch_equal_numpy_and_oracle(dev, orc, dt):
+    """Trees of 200+ nodes over 24 arrays and 20 distinct constants -- beyond one launch's 16 loads, 64
+    instructions, stack of 8, 16 leaves and 16 constants -- are cut into pieces through temporary planes
+    (aeth_eval_tape_ex) and still give numpy's bits with the oracle's math; their fused sums equal the
+    standalone reduction of the output."""
+    global CONSTS
+    npdt = np.float64 if dt == torch.float64 else np.float32
+    n = 20_001
+    r = np.random.default_rng(31)
+    host = [r.uniform(-2, 2, n).astype(npdt) for _ in range(24)]
+    leaves = []
+    for h in host:
+        a = ae.Array(n, 1, dt)
+        a.data[0, :n].copy_(torch.from_numpy(h).to("cuda"))
+        leaves.append(a)
+    saved, CONSTS = CONSTS, BIG_CONSTS
+    try:
+        gen = Gen(orc, leaves, host, np.dtype(npdt), 41 if dt == torch.float64 else 42)
+        d = ae.Array(n, 1, dt)
+        checked = 0
+        while checked < 6:
+            ex, f = gen.tree(9)
+            progs, _keep, lv, cs, _ = E.lower_ex(d, ex)
+            if progs[0].n_insns < 200:
+                continue
+            with pytest.raises(ValueError):
+                E.lower(d, ex)  # beyond one launch
+            sums = E.assign(d, ex, ae.SUM_NEUMAIER if dt == torch.float64 else ae.SUM_FLOATFLOAT)
+            torch.cuda.synchronize()
+            with np.errstate(all="ignore"):
+                want = np.ascontiguousarray(f(), dtype=npdt)
+            got = d.data[0, :n].cpu().numpy()
+            ok, bad = same_bits(got, want)
+            assert ok, (checked, progs[0].n_insns, bad, got[bad], want[bad])
+            again = ae.reduce(d, ae.SUM_NEUMAIER if dt == torch.float64 else ae.SUM_FLOATFLOAT)
+            assert torch.equal(sums, again) or (torch.isnan(sums).any() and torch.isnan(again).any())
+            checked += 1
+    finally:
+        CONSTS = saved
+
+
+def test_many_components_and_arrays(dev):
+    """A 12-component destination (beyond one launch's 8 programs) of per-component sums over 20 arrays,
+    each component reading others' planes: the same as 12 separate assignments."""
+    n = 5003
+    srcs = [ae.Array(n, 12, torch.float64) for _ in range(20)]
+    for k, s in enumerate(srcs):
+        ae.array_fill_uniform(s, 100 + k, -1.0, 1.0)
+    expr = E.Leaf(srcs[0])
+    for k, s in enumerate(srcs[1:]):
+        expr = expr + E.Leaf(s) * (0.5 + 0.01 * k)
+    d = ae.Array(n, 12, torch.float64)
+    E.assign(d, expr)
+    torch.cuda.synchronize()
+    want = srcs[0].data[:, :n].clone()
+    for k, s in enumerate(srcs[1:]):
+        c = torch.tensor(0.5 + 0.01 * k, dtype=torch.float64, device="cuda")
+        want = want + s.data[:, :n] * c
+    assert torch.equal(d.data[:, :n], want)
