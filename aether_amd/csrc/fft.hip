@@ -793,6 +793,24 @@ __global__ __launch_bounds__(kRT, sizeof(C) == 8 ? 4 : 2) void fft_reg_kernel(co
     rchunk(v, d, sigs + d.first_sig, in, out, xch, TwGlobal{tw}, fwd_flag != 0, norm, fixed_scale);
 }
 
+// The persistent form: a fixed grid walks the descriptors, their number read from device memory, so one
+// launch (or one node of a captured graph) serves any table that fits the buffers it points at.
+template<class C>
+__global__ __launch_bounds__(kRT, sizeof(C) == 8 ? 4 : 2) void fft_reg_persist_kernel(
+    const Desc* __restrict__ descs, const Sig* __restrict__ sigs, const uint32_t* __restrict__ ndesc,
+    const void* __restrict__ in, void* __restrict__ out, TwTables tw, int fwd_flag, int norm, double fixed_scale)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    auto* xch = reinterpret_cast<typename Words<C>::W*>(lds_raw);
+    const uint32_t count = *ndesc;
+    for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+        __syncthreads(); // the previous chunk's exchanges are done with the LDS
+        const Desc d = descs[k];
+        C v[kRP];
+        rchunk(v, d, sigs + d.first_sig, in, out, xch, TwGlobal{tw}, fwd_flag != 0, norm, fixed_scale);
+    }
+}
+
 __global__ void twiddle_kernel(double2* d, float2* f, float4* ff)
 {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -851,6 +869,12 @@ TwTables tables(hipStream_t st)
         check(hipFuncSetAttribute((const void*)fft_reg_kernel<CW>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kMaxN * sizeof(Words<CW>::W)),
               "hipFuncSetAttribute(LDS)");
+        check(hipFuncSetAttribute((const void*)fft_reg_persist_kernel<C128>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kMaxN * sizeof(Words<C128>::W)),
+              "hipFuncSetAttribute(LDS)");
+        check(hipFuncSetAttribute((const void*)fft_reg_persist_kernel<CFF>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kMaxN * sizeof(Words<CFF>::W)),
+              "hipFuncSetAttribute(LDS)");
 
         it = g_tw.emplace(dev, t).first;
     }
@@ -893,6 +917,43 @@ void build_tables(const uint32_t* log2n, const uint64_t* in_off, const uint64_t*
         d.count = (uint16_t)c;
         descs.push_back(d);
     }
+}
+
+namespace {
+template<class C>
+int persist_grid(size_t lds)
+{
+    int dev = 0, per_cu = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fft_reg_persist_kernel<C>, kRT, lds),
+          "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    return std::max(1, per_cu) * device_cus();
+}
+} // namespace
+
+void prepare(hipStream_t st) { (void)tables(st); }
+
+int persistent_grid(int prec)
+{
+    if (prec == 2) return persist_grid<CFF>(kMaxN * sizeof(Words<CFF>::W));
+    if (prec == 0) return persist_grid<C128>(kMaxN * sizeof(Words<C128>::W));
+    fail("fft persistent_grid", "precision not served");
+    return 0;
+}
+
+void launch_persistent(int prec, int grid, const Desc* descs, const Sig* sigs, const uint32_t* ndesc, const void* in,
+                       void* out, int forward, int norm, hipStream_t st)
+{
+    const TwTables tw = tables(st);
+    if (prec == 2)
+        hipLaunchKernelGGL(fft_reg_persist_kernel<CFF>, dim3(grid), dim3(kRT), kMaxN * sizeof(Words<CFF>::W), st, descs,
+                           sigs, ndesc, in, out, tw, forward, norm, 1.0);
+    else if (prec == 0)
+        hipLaunchKernelGGL(fft_reg_persist_kernel<C128>, dim3(grid), dim3(kRT), kMaxN * sizeof(Words<C128>::W), st,
+                           descs, sigs, ndesc, in, out, tw, forward, norm, 1.0);
+    else
+        fail("fft launch_persistent", "precision not served");
+    check(hipGetLastError(), "fft_reg_persist_kernel launch");
 }
 
 void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm, hipStream_t st,

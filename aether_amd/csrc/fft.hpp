@@ -40,6 +40,14 @@ void build_tables(const uint32_t* log2n, const uint64_t* in_off, const uint64_t*
 // One launch over all descriptors of the plan. norm 0/1/2: numpy norms per length; 3: fixed_scale.
 void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm, hipStream_t st,
             double fixed_scale = 1.0);
+// The same transforms (contiguous signals; prec 0 complex128 or 2 complex64 float-float) with a fixed
+// persistent grid that reads the descriptor count from *ndesc on the device: capturable once, valid for
+// any table. persistent_grid(prec) is the grid that fills the device; prepare() builds the twiddle tables
+// (call both before a capture).
+void prepare(hipStream_t st);
+int persistent_grid(int prec);
+void launch_persistent(int prec, int grid, const Desc* descs, const Sig* sigs, const uint32_t* ndesc, const void* in,
+                       void* out, int forward, int norm, hipStream_t st);
 
 } // namespace fft
 } // namespace aeth

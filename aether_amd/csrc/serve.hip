@@ -88,6 +88,16 @@ struct TreeDesc {
     uint32_t tiles, dtype;
 };
 
+// The batch's counts, at the start of the slot's staging area on both sides. Kernels read them from device
+// memory, so a launch with a fixed grid (one node of a graph captured once per slot) serves every batch.
+struct BatchHeader {
+    uint32_t fills, evals, reds, trees;
+    uint32_t fft_ndesc[2]; // complex128, complex64 float-float
+    uint32_t jobs, pad_;
+    uint64_t fill_units, eval_units, red_units;
+    uint64_t region_bytes[9]; // bytes used in each region of the staging layout (header first)
+};
+
 template<class D>
 __device__ __forceinline__ uint32_t find_desc(const D* d, uint32_t count, uint64_t unit)
 {
@@ -100,10 +110,8 @@ __device__ __forceinline__ uint32_t find_desc(const D* d, uint32_t count, uint64
     return lo;
 }
 
-__global__ __launch_bounds__(kThreads) void fill_batch_kernel(const FillDesc* __restrict__ descs, uint32_t count)
+__device__ __forceinline__ void fill_unit(const FillDesc& D, uint64_t u)
 {
-    const uint64_t u = blockIdx.x;
-    const FillDesc& D = descs[find_desc(descs, count, u)];
     const uint64_t total = (uint64_t)D.ncomp * D.n;
     const uint64_t first = (u - D.unit_start) * kFillPerBlock;
     for (uint64_t k = first + threadIdx.x; k < first + kFillPerBlock && k < total; k += kThreads) {
@@ -126,12 +134,18 @@ __global__ __launch_bounds__(kThreads) void fill_batch_kernel(const FillDesc* __
     }
 }
 
-__global__ __launch_bounds__(kThreads) void eval_batch_kernel(const EvalDesc* __restrict__ descs, uint32_t count)
+__global__ __launch_bounds__(kThreads) void fill_batch_kernel(const BatchHeader* __restrict__ hdr,
+                                                             const FillDesc* __restrict__ descs)
 {
-    __shared__ acc::Pair<double> ldsd[4];
-    __shared__ acc::Pair<float> ldsf[4];
-    const uint64_t u = blockIdx.x;
-    const EvalDesc& D = descs[find_desc(descs, count, u)];
+    const uint32_t count = hdr->fills;
+    const uint64_t units = hdr->fill_units;
+    for (uint64_t u = blockIdx.x; u < units; u += gridDim.x)
+        fill_unit(descs[find_desc(descs, count, u)], u);
+}
+
+__device__ __forceinline__ void eval_unit(const EvalDesc& D, uint64_t u, acc::Pair<double>* ldsd,
+                                          acc::Pair<float>* ldsf)
+{
     const uint32_t local = (uint32_t)(u - D.unit_start);
     const uint32_t comp = local / D.tiles, tile = local - comp * D.tiles;
     if (D.dtype == AETH_F64) {
@@ -149,12 +163,19 @@ __global__ __launch_bounds__(kThreads) void eval_batch_kernel(const EvalDesc* __
     }
 }
 
-__global__ __launch_bounds__(kThreads) void reduce_batch_kernel(const RedDesc* __restrict__ descs, uint32_t count)
+__global__ __launch_bounds__(kThreads) void eval_batch_kernel(const BatchHeader* __restrict__ hdr,
+                                                             const EvalDesc* __restrict__ descs)
 {
     __shared__ acc::Pair<double> ldsd[4];
     __shared__ acc::Pair<float> ldsf[4];
-    const uint64_t u = blockIdx.x;
-    const RedDesc& D = descs[find_desc(descs, count, u)];
+    const uint32_t count = hdr->evals;
+    const uint64_t units = hdr->eval_units;
+    for (uint64_t u = blockIdx.x; u < units; u += gridDim.x) eval_unit(descs[find_desc(descs, count, u)], u, ldsd, ldsf);
+}
+
+__device__ __forceinline__ void reduce_unit(const RedDesc& D, uint64_t u, acc::Pair<double>* ldsd,
+                                            acc::Pair<float>* ldsf)
+{
     const uint32_t tile = (uint32_t)(u - D.unit_start);
     if (D.dtype == AETH_F64) {
         const double* x = static_cast<const double*>(D.x);
@@ -169,6 +190,17 @@ __global__ __launch_bounds__(kThreads) void reduce_batch_kernel(const RedDesc* _
         const auto r = acc::block_combine<acc::kFloatFloat>(p, ldsf);
         if (threadIdx.x == 0) static_cast<acc::Pair<float>*>(D.part)[tile] = r;
     }
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_batch_kernel(const BatchHeader* __restrict__ hdr,
+                                                               const RedDesc* __restrict__ descs)
+{
+    __shared__ acc::Pair<double> ldsd[4];
+    __shared__ acc::Pair<float> ldsf[4];
+    const uint32_t count = hdr->reds;
+    const uint64_t units = hdr->red_units;
+    for (uint64_t u = blockIdx.x; u < units; u += gridDim.x)
+        reduce_unit(descs[find_desc(descs, count, u)], u, ldsd, ldsf);
 }
 
 // The group tree of the single-call path, all levels inside one block.
@@ -201,12 +233,36 @@ __device__ __forceinline__ void tree_one(const TreeDesc& D, acc::Pair<T>* s)
     }
 }
 
-__global__ __launch_bounds__(512) void tree_batch_kernel(const TreeDesc* __restrict__ descs)
+__global__ __launch_bounds__(512) void tree_batch_kernel(const BatchHeader* __restrict__ hdr,
+                                                         const TreeDesc* __restrict__ descs)
 {
     __shared__ acc::Pair<double> s[kGroup];
-    const TreeDesc& D = descs[blockIdx.x];
-    if (D.dtype == AETH_F64) tree_one<double, acc::kNeumaier>(D, s);
-    else tree_one<float, acc::kFloatFloat>(D, reinterpret_cast<acc::Pair<float>*>(s));
+    const uint32_t count = hdr->trees;
+    for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+        __syncthreads(); // the previous tree is done with the LDS
+        const TreeDesc& D = descs[k];
+        if (D.dtype == AETH_F64) tree_one<double, acc::kNeumaier>(D, s);
+        else tree_one<float, acc::kFloatFloat>(D, reinterpret_cast<acc::Pair<float>*>(s));
+    }
+}
+
+// The staging area's used bytes, region by region, from the pinned host copy (read through its device
+// mapping) into device memory: one kernel in place of a copy whose size the batch decides.
+constexpr int kRegions = 9;
+__global__ __launch_bounds__(kThreads) void stage_in_kernel(const unsigned char* __restrict__ host,
+                                                           unsigned char* __restrict__ dev,
+                                                           const uint64_t* __restrict__ region_offset)
+{
+    __shared__ uint64_t used[kRegions];
+    if (threadIdx.x < kRegions) used[threadIdx.x] = reinterpret_cast<const BatchHeader*>(host)->region_bytes[threadIdx.x];
+    __syncthreads();
+    for (int r = 0; r < kRegions; ++r) {
+        const uint64_t at = region_offset[r], vecs = (used[r] + 15) / 16;
+        const uint4* src = reinterpret_cast<const uint4*>(host + at);
+        uint4* dst = reinterpret_cast<uint4*>(dev + at);
+        for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < vecs; i += (uint64_t)gridDim.x * kThreads)
+            dst[i] = src[i];
+    }
 }
 
 // ---------------------------------------------------------------- device pool
@@ -372,32 +428,61 @@ uint64_t job_need(const aeth_job& s)
 }
 
 // ---------------------------------------------------------------- batches
-constexpr size_t kStage = 8u << 20;
+// The staging area of a slot: a header and eight tables at fixed offsets, sized for cfg.max_batch_jobs
+// (the same layout in pinned host memory and in device memory).
+enum { kRegHeader = 0, kRegFill, kRegEval, kRegRed, kRegTree, kRegFd0, kRegFs0, kRegFd2, kRegFs2 };
+struct Layout {
+    uint64_t at[kRegions];
+    uint64_t cap[kRegions]; // bytes
+    uint64_t total;
+};
+
+Layout make_layout(uint32_t max_jobs)
+{
+    const uint64_t sigs = 64ull * max_jobs; // at most 64 signals a job, each at most one descriptor
+    const uint64_t caps[kRegions] = {sizeof(BatchHeader),           3ull * max_jobs * sizeof(FillDesc),
+                                     max_jobs * sizeof(EvalDesc),   2ull * max_jobs * sizeof(RedDesc),
+                                     3ull * max_jobs * sizeof(TreeDesc), sigs * sizeof(fft::Desc),
+                                     sigs * sizeof(fft::Sig),       sigs * sizeof(fft::Desc),
+                                     sigs * sizeof(fft::Sig)};
+    Layout L{};
+    uint64_t o = 0;
+    for (int r = 0; r < kRegions; ++r) {
+        o = (o + 255) & ~uint64_t(255);
+        L.at[r] = o;
+        L.cap[r] = caps[r];
+        o += caps[r];
+    }
+    L.total = (o + 255) & ~uint64_t(255);
+    return L;
+}
 
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    unsigned char* host_stage = nullptr;
+    unsigned char* host_stage = nullptr;     // pinned, mapped
+    unsigned char* host_stage_dev = nullptr; // its device address
     unsigned char* dev_stage = nullptr;
+    uint64_t* region_offset = nullptr;       // device copy of the layout's offsets
     double* host_res = nullptr;
     double* dev_res = nullptr;
+    hipGraphExec_t graph = nullptr;          // the batch as one captured graph (graph mode)
     std::vector<Job> jobs;
     bool busy = false;
     double launch_t = 0;
 };
 
 template<class T>
-size_t put(unsigned char* base, size_t& off, const std::vector<T>& v)
+void put(const Layout& L, unsigned char* base, int region, const std::vector<T>& v, BatchHeader& h)
 {
-    off = (off + 255) & ~size_t(255);
-    const size_t at = off;
-    if (!v.empty()) std::memcpy(base + at, v.data(), v.size() * sizeof(T));
-    off += v.size() * sizeof(T);
-    return at;
+    const uint64_t bytes = v.size() * sizeof(T);
+    if (bytes > L.cap[region]) fail("server", "batch descriptors exceed the staging area");
+    if (bytes) std::memcpy(base + L.at[region], v.data(), bytes);
+    h.region_bytes[region] = bytes;
 }
 
-// Launch every job of the slot as one batch on the slot's stream.
-void launch_batch(Slot& S, void* base)
+// Write the slot's batch into its staging area (descriptor tables and header); returns the header.
+BatchHeader build_batch(Slot& S, const Layout& L, void* base)
 {
     std::vector<FillDesc> fills;
     std::vector<EvalDesc> evals;
@@ -492,13 +577,12 @@ void launch_batch(Slot& S, void* base)
             trees.push_back(TreeDesc{J.part, J.scratch, res, J.tiles, s.dtype});
         }
     }
-    // FFT tables: input offsets are relative to a null base; outputs sit at the same offset in buf[1]
+    // FFT tables: input offsets are relative to the pool base; outputs sit at the same offset in buf[1]
     std::vector<fft::Desc> fdesc[3];
     std::vector<fft::Sig> fsig[3];
     for (int prec : {0, 2}) {
         if (l2[prec].empty()) continue;
         std::vector<uint64_t> out_off(off[prec].size());
-        // recover each signal's output offset: same position inside the job's second buffer
         size_t q = 0;
         for (size_t j = 0; j < S.jobs.size(); ++j) {
             const Job& J = S.jobs[j];
@@ -510,48 +594,124 @@ void launch_batch(Slot& S, void* base)
         }
         fft::build_tables(l2[prec].data(), off[prec].data(), out_off.data(), l2[prec].size(), fdesc[prec], fsig[prec]);
     }
-    size_t o = 0;
-    const size_t o_fill = put(S.host_stage, o, fills);
-    const size_t o_eval = put(S.host_stage, o, evals);
-    const size_t o_red = put(S.host_stage, o, reds);
-    const size_t o_tree = put(S.host_stage, o, trees);
-    const size_t o_fd0 = put(S.host_stage, o, fdesc[0]);
-    const size_t o_fs0 = put(S.host_stage, o, fsig[0]);
-    const size_t o_fd2 = put(S.host_stage, o, fdesc[2]);
-    const size_t o_fs2 = put(S.host_stage, o, fsig[2]);
-    if (o > kStage) fail("server", "batch descriptors exceed the staging area");
+    BatchHeader h{};
+    h.fills = (uint32_t)fills.size();
+    h.evals = (uint32_t)evals.size();
+    h.reds = (uint32_t)reds.size();
+    h.trees = (uint32_t)trees.size();
+    h.fft_ndesc[0] = (uint32_t)fdesc[0].size();
+    h.fft_ndesc[1] = (uint32_t)fdesc[2].size();
+    h.jobs = (uint32_t)S.jobs.size();
+    h.fill_units = fill_units;
+    h.eval_units = eval_units;
+    h.red_units = red_units;
+    put(L, S.host_stage, kRegFill, fills, h);
+    put(L, S.host_stage, kRegEval, evals, h);
+    put(L, S.host_stage, kRegRed, reds, h);
+    put(L, S.host_stage, kRegTree, trees, h);
+    put(L, S.host_stage, kRegFd0, fdesc[0], h);
+    put(L, S.host_stage, kRegFs0, fsig[0], h);
+    put(L, S.host_stage, kRegFd2, fdesc[2], h);
+    put(L, S.host_stage, kRegFs2, fsig[2], h);
+    h.region_bytes[kRegHeader] = sizeof(BatchHeader);
+    std::memcpy(S.host_stage + L.at[kRegHeader], &h, sizeof h);
+    return h;
+}
+
+template<class T>
+const T* at(const Slot& S, const Layout& L, int region)
+{
+    return reinterpret_cast<const T*>(S.dev_stage + L.at[region]);
+}
+
+// The batch's kernels on the slot's stream. grid_exact: size each grid to the batch (direct launches);
+// otherwise fixed persistent grids that read the counts on the device (the form captured into a graph).
+struct PersistentGrids {
+    int units = 0;    // fill / eval / reduce / tree / staging kernels
+    int fft[2] = {0, 0};
+};
+
+void enqueue_batch(Slot& S, const Layout& L, void* base, const BatchHeader* h, uint32_t max_jobs,
+                   const PersistentGrids& pg)
+{
     hipStream_t st = S.stream;
-    check(hipMemcpyAsync(S.dev_stage, S.host_stage, o, hipMemcpyHostToDevice, st), "hipMemcpyAsync(descriptors)");
-    if (!fills.empty())
-        hipLaunchKernelGGL(fill_batch_kernel, dim3((uint32_t)fill_units), dim3(kThreads), 0, st,
-                           reinterpret_cast<const FillDesc*>(S.dev_stage + o_fill), (uint32_t)fills.size());
-    if (!evals.empty())
-        hipLaunchKernelGGL(eval_batch_kernel, dim3((uint32_t)eval_units), dim3(kThreads), 0, st,
-                           reinterpret_cast<const EvalDesc*>(S.dev_stage + o_eval), (uint32_t)evals.size());
-    for (int prec : {0, 2}) {
-        if (fdesc[prec].empty()) continue;
-        fft::BatchPlan p;
-        p.prec = prec;
-        p.descs = reinterpret_cast<fft::Desc*>(S.dev_stage + (prec == 0 ? o_fd0 : o_fd2));
-        p.sigs = reinterpret_cast<fft::Sig*>(S.dev_stage + (prec == 0 ? o_fs0 : o_fs2));
-        p.ndesc = (uint32_t)fdesc[prec].size();
-        p.nsig = fsig[prec].size();
-        fft::launch(p, base, base, 1, 0, st);
+    const BatchHeader* hdr = at<BatchHeader>(S, L, kRegHeader);
+    const int persist = pg.units;
+    auto grid = [&](uint64_t units) { return h ? (uint32_t)std::max<uint64_t>(1, units) : (uint32_t)persist; };
+    constexpr int kStageBlocks = 64; // a few MB at most cross PCIe; 64 blocks keep the reads in flight
+    hipLaunchKernelGGL(stage_in_kernel, dim3(kStageBlocks), dim3(kThreads), 0, st, S.host_stage_dev, S.dev_stage,
+                       S.region_offset);
+    if (!h || h->fills)
+        hipLaunchKernelGGL(fill_batch_kernel, dim3(grid(h ? h->fill_units : 0)), dim3(kThreads), 0, st, hdr,
+                           at<FillDesc>(S, L, kRegFill));
+    if (!h || h->evals)
+        hipLaunchKernelGGL(eval_batch_kernel, dim3(grid(h ? h->eval_units : 0)), dim3(kThreads), 0, st, hdr,
+                           at<EvalDesc>(S, L, kRegEval));
+    for (int q = 0; q < 2; ++q) {
+        const int prec = q == 0 ? 0 : 2;
+        const auto* fd = at<fft::Desc>(S, L, q == 0 ? kRegFd0 : kRegFd2);
+        const auto* fs = at<fft::Sig>(S, L, q == 0 ? kRegFs0 : kRegFs2);
+        if (h) {
+            if (!h->fft_ndesc[q]) continue;
+            fft::BatchPlan p;
+            p.prec = prec;
+            p.descs = const_cast<fft::Desc*>(fd);
+            p.sigs = const_cast<fft::Sig*>(fs);
+            p.ndesc = h->fft_ndesc[q];
+            fft::launch(p, base, base, 1, 0, st);
+        } else {
+            fft::launch_persistent(prec, pg.fft[q], fd, fs, &hdr->fft_ndesc[q], base, base, 1, 0, st);
+        }
     }
-    if (!reds.empty())
-        hipLaunchKernelGGL(reduce_batch_kernel, dim3((uint32_t)red_units), dim3(kThreads), 0, st,
-                           reinterpret_cast<const RedDesc*>(S.dev_stage + o_red), (uint32_t)reds.size());
-    if (!trees.empty())
-        hipLaunchKernelGGL(tree_batch_kernel, dim3((uint32_t)trees.size()), dim3(512), 0, st,
-                           reinterpret_cast<const TreeDesc*>(S.dev_stage + o_tree));
+    if (!h || h->reds)
+        hipLaunchKernelGGL(reduce_batch_kernel, dim3(grid(h ? h->red_units : 0)), dim3(kThreads), 0, st, hdr,
+                           at<RedDesc>(S, L, kRegRed));
+    if (!h || h->trees)
+        hipLaunchKernelGGL(tree_batch_kernel, dim3(h ? std::max<uint32_t>(1, h->trees) : (uint32_t)persist),
+                           dim3(512), 0, st, hdr, at<TreeDesc>(S, L, kRegTree));
     check(hipGetLastError(), "serve batch launch");
-    check(hipMemcpyAsync(S.host_res, S.dev_res, 8 * sizeof(double) * S.jobs.size(), hipMemcpyDeviceToHost, st),
-          "hipMemcpyAsync(results)");
-    check(hipEventRecord(S.done, st), "hipEventRecord");
+    const size_t rb = 8 * sizeof(double) * (h ? std::max<size_t>(1, h->jobs) : max_jobs);
+    check(hipMemcpyAsync(S.host_res, S.dev_res, rb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(results)");
+}
+
+// Capture the persistent form of a batch on the slot's stream into a graph, once.
+void capture_batch_graph(Slot& S, const Layout& L, void* base, uint32_t max_jobs)
+{
+    // everything the launches look up is ready before the capture: twiddle tables, grid sizes
+    fft::prepare(S.stream);
+    PersistentGrids pg;
+    pg.units = device_cus() * 8;
+    pg.fft[0] = fft::persistent_grid(0);
+    pg.fft[1] = fft::persistent_grid(2);
+    check(hipStreamSynchronize(S.stream), "hipStreamSynchronize");
+    check(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipGraph_t g = nullptr;
+    try {
+        enqueue_batch(S, L, base, nullptr, max_jobs, pg);
+    } catch (...) {
+        (void)hipStreamEndCapture(S.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    check(hipStreamEndCapture(S.stream, &g), "hipStreamEndCapture");
+    const hipError_t e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    check(e, "hipGraphInstantiate");
+}
+
+// Launch every job of the slot as one batch on the slot's stream.
+void launch_batch(Slot& S, const Layout& L, void* base, bool graph)
+{
+    const BatchHeader h = build_batch(S, L, base);
+    if (graph) check(hipGraphLaunch(S.graph, S.stream), "hipGraphLaunch");
+    else enqueue_batch(S, L, base, &h, 0, PersistentGrids{});
+    check(hipEventRecord(S.done, S.stream), "hipEventRecord");
 }
 
 struct Server {
     aeth_server_config cfg;
+    Layout layout{};
+    bool graph = false;
     int dev = 0;
     std::unique_ptr<Pool> pool;
     std::vector<Slot> slots;
@@ -654,7 +814,7 @@ struct Server {
                 }
                 if (S.jobs.empty()) break; // pool full: wait for completions
                 try {
-                    launch_batch(S, pool->base());
+                    launch_batch(S, layout, pool->base(), graph);
                 } catch (const Failure&) {
                     fatal = last_error();
                     for (auto& J : S.jobs) {
@@ -691,16 +851,25 @@ extern "C" int aeth_server_create(const aeth_server_config* cfg, void** server)
         if (S->cfg.max_batch_jobs > 4096) S->cfg.max_batch_jobs = 4096;
         check(hipGetDevice(&S->dev), "hipGetDevice");
         S->pool = std::make_unique<Pool>(cfg->hbm_budget_bytes);
+        S->graph = (S->cfg.flags & AETH_SERVER_GRAPH_BATCHES) != 0;
+        S->layout = make_layout(S->cfg.max_batch_jobs);
+        const uint64_t stage = S->layout.total;
         S->slots.resize(S->cfg.streams);
         for (auto& sl : S->slots) {
             check(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
             check(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "hipEventCreateWithFlags");
-            check(hipHostMalloc(reinterpret_cast<void**>(&sl.host_stage), kStage, hipHostMallocDefault),
-                  "hipHostMalloc(stage)", kStage);
-            check(hipMalloc(reinterpret_cast<void**>(&sl.dev_stage), kStage), "hipMalloc(stage)", kStage);
+            check(hipHostMalloc(reinterpret_cast<void**>(&sl.host_stage), stage, hipHostMallocMapped),
+                  "hipHostMalloc(stage)", stage);
+            check(hipHostGetDevicePointer(reinterpret_cast<void**>(&sl.host_stage_dev), sl.host_stage, 0),
+                  "hipHostGetDevicePointer(stage)");
+            check(hipMalloc(reinterpret_cast<void**>(&sl.dev_stage), stage), "hipMalloc(stage)", stage);
+            check(hipMalloc(reinterpret_cast<void**>(&sl.region_offset), sizeof S->layout.at), "hipMalloc(layout)");
+            check(hipMemcpy(sl.region_offset, S->layout.at, sizeof S->layout.at, hipMemcpyHostToDevice),
+                  "hipMemcpy(layout)");
             const size_t rb = 8 * sizeof(double) * S->cfg.max_batch_jobs;
             check(hipHostMalloc(reinterpret_cast<void**>(&sl.host_res), rb, hipHostMallocDefault), "hipHostMalloc(res)");
             check(hipMalloc(reinterpret_cast<void**>(&sl.dev_res), rb), "hipMalloc(res)");
+            if (S->graph) capture_batch_graph(sl, S->layout, S->pool->base(), S->cfg.max_batch_jobs);
         }
         Server* raw = S.get();
         S->worker = std::thread([raw] { raw->loop(); });
@@ -780,6 +949,8 @@ extern "C" int aeth_server_destroy(void* server)
             (void)hipFree(sl.dev_stage);
             (void)hipHostFree(sl.host_res);
             (void)hipFree(sl.dev_res);
+            (void)hipFree(sl.region_offset);
+            if (sl.graph) (void)hipGraphExecDestroy(sl.graph);
         }
         delete S;
     });

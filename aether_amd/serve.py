@@ -14,6 +14,7 @@ import numpy as np
 from ._lib import call, lib
 
 JOB_EVAL, JOB_SAMPLE, JOB_FFT = 0, 1, 2
+SERVER_GRAPH_BATCHES = 1
 F64, F32 = 0, 1
 
 
@@ -29,7 +30,7 @@ class JobResult(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("streams", C.c_uint32), ("max_batch_jobs", C.c_uint32),
-                ("fp32_compensated", C.c_uint32), ("reserved", C.c_uint32)]
+                ("fp32_compensated", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -108,9 +109,10 @@ def job_bytes(j: Job, fp32_compensated: bool = False) -> float:
 
 class Server:
     def __init__(self, budget_bytes: int = 64 << 30, streams: int = 4, max_batch_jobs: int = 256,
-                 fp32_compensated: bool = False):
+                 fp32_compensated: bool = False, graph_batches: bool = False):
         self.L = _bind()
-        cfg = Config(budget_bytes, streams, max_batch_jobs, int(fp32_compensated), 0)
+        cfg = Config(budget_bytes, streams, max_batch_jobs, int(fp32_compensated),
+                     SERVER_GRAPH_BATCHES if graph_batches else 0)
         h = C.c_void_p()
         call("aeth_server_create", C.byref(cfg), C.byref(h))
         self.h = h

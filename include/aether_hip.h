@@ -330,8 +330,16 @@ typedef struct {
     uint32_t streams;        /* batches in flight, 1..4 */
     uint32_t max_batch_jobs; /* coalescing cap per batch */
     uint32_t fp32_compensated;
-    uint32_t reserved;
+    uint32_t flags;          /* AETH_SERVER_GRAPH_BATCHES: see below */
 } aeth_server_config;
+/* flags: AETH_SERVER_GRAPH_BATCHES -- every batch of a stream slot is ONE
+ * replay of a graph captured when the server is created: a staging kernel
+ * that pulls the batch's descriptor tables from pinned host memory, then
+ * persistent kernels (fixed grids, descriptor counts read on the device) for
+ * the fills, fused evaluations, FFTs, reductions and trees, and the copy of
+ * the results. The same kernels' per-unit bodies run either way, so results
+ * are bit for bit those of direct launches. */
+enum { AETH_SERVER_GRAPH_BATCHES = 1 };
 typedef struct {
     uint64_t submitted, completed, failed, batches;
     uint64_t pool_bytes_in_use, pool_bytes_budget;

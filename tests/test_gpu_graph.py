@@ -93,3 +93,36 @@ def test_fused_sums_in_a_graph_replay(dev, dt):
         g.replay()
         torch.cuda.synchronize()
         assert torch.equal(sums, want) and torch.equal(sums2, want)
+
+
+def test_served_batches_replay_one_graph(dev):
+    """Served batches as replays of a graph captured once per stream slot: three waves of jobs, with eager
+    GPU work and synchronisations between them, each job's result equal to the plain per-call path."""
+    from aether_amd import serve as S
+
+    rng = np.random.default_rng(17)
+    waves = []
+    for w in range(3):
+        wave = []
+        for i in range(12):
+            seed = 100 * w + 8 * i + 1
+            if i % 3 == 0:
+                wave.append(S.make_job(S.JOB_EVAL, int(2 ** rng.uniform(8, 16)), seed, S.F64))
+            elif i % 3 == 1:
+                wave.append(S.make_job(S.JOB_SAMPLE, int(2 ** rng.uniform(8, 17)), seed, S.F32))
+            else:
+                lens = [int(v) for v in rng.integers(2, 14, 3)]
+                wave.append(S.make_job(S.JOB_FFT, len(lens), seed, S.F64, fft_log2n=lens))
+        waves.append(wave)
+    busy = torch.ones(1 << 20, device="cuda")
+    with S.Server(budget_bytes=2 << 30, streams=1, max_batch_jobs=4, graph_batches=True) as srv:
+        for wave in waves:
+            ids = [srv.submit(j) for j in wave]
+            srv.wait_idle()
+            got = {r.id: r for r in srv.poll()}
+            busy.mul_(1.0001).add_(0.5)  # the program's own eager work between replays
+            torch.cuda.synchronize()
+            for j, i in zip(wave, ids):
+                assert got[i].status == 0
+                assert list(got[i].value) == list(S.run_plain(j).value), (j.kind, j.n)
+        assert srv.stats().batches >= 9  # 36 jobs, at most 4 a batch: the one graph replayed again and again

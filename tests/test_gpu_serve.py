@@ -48,6 +48,26 @@ def test_alone_equals_in_the_mix(dev):
         assert all(v == 0.0 for v in list(r.value)[used:])
 
 
+@pytest.mark.parametrize("fp32", [False, True], ids=["requested", "fp32_compensated"])
+def test_graph_batches_equal_direct_launches(dev, fp32):
+    """A server whose batches are replays of one captured graph per stream slot (persistent kernels reading
+    the batch's counts on the device) gives every job the bits of the direct launches."""
+    jobs = jobs_small_mix(count=48, seed=5)
+    results = {}
+    for graph in (False, True):
+        with S.Server(budget_bytes=4 << 30, streams=2, max_batch_jobs=16, fp32_compensated=fp32,
+                      graph_batches=graph) as srv:
+            ids = [srv.submit(j) for j in jobs]
+            srv.wait_idle()
+            got = {r.id: r for r in srv.poll()}
+            st = srv.stats()
+        assert st.completed == len(jobs) and st.failed == 0 and st.batches >= len(jobs) // 16
+        results[graph] = [got[i] for i in ids]
+    for j, a, b in zip(jobs, results[False], results[True]):
+        assert a.status == 0 and b.status == 0
+        assert list(a.value) == list(b.value), (j.kind, j.dtype, j.n)
+
+
 def test_eval_job_matches_oracle(dev, orc):
     n, seed = 10_000, 77
     r = S.run_plain(S.make_job(S.JOB_EVAL, n, seed, S.F64))
