@@ -110,26 +110,33 @@ __device__ __forceinline__ uint32_t find_desc(const D* d, uint32_t count, uint64
     return lo;
 }
 
+// One unit: kFillPerBlock consecutive samples of one component (units run component-major), so the
+// component and the sample range come from one 32-bit division per block, and a wave's stores are
+// consecutive.
 __device__ __forceinline__ void fill_unit(const FillDesc& D, uint64_t u)
 {
-    const uint64_t total = (uint64_t)D.ncomp * D.n;
-    const uint64_t first = (u - D.unit_start) * kFillPerBlock;
-    for (uint64_t k = first + threadIdx.x; k < first + kFillPerBlock && k < total; k += kThreads) {
-        if (D.kind == 0) {
-            const uint32_t c = (uint32_t)(k / D.n);
-            const uint32_t i = (uint32_t)(k - (uint64_t)c * D.n);
-            const rng::U4 b = rng::block(D.seed, i, c);
-            if (D.dtype == AETH_F64) {
-                static_cast<double*>(D.out)[(uint64_t)c * D.pitch + i] =
-                    D.p0 + (D.p1 - D.p0) * rng::u01_from_word_d(b.x);
-            } else {
-                const float p0 = (float)D.p0, p1 = (float)D.p1;
-                static_cast<float*>(D.out)[(uint64_t)c * D.pitch + i] = p0 + (p1 - p0) * rng::u01_from_word_f(b.x);
-            }
+    const uint32_t chunks = (uint32_t)((D.n + kFillPerBlock - 1) / kFillPerBlock);
+    const uint32_t local = (uint32_t)(u - D.unit_start);
+    const uint32_t c = local / chunks;
+    const uint32_t i0 = (local - c * chunks) * (uint32_t)kFillPerBlock;
+    const uint64_t stop = (uint64_t)i0 + kFillPerBlock;
+    const uint32_t end = (uint32_t)(stop < D.n ? stop : D.n);
+    if (D.kind == 0) {
+        if (D.dtype == AETH_F64) {
+            double* out = static_cast<double*>(D.out) + (uint64_t)c * D.pitch;
+            for (uint32_t i = i0 + threadIdx.x; i < end; i += kThreads)
+                out[i] = D.p0 + (D.p1 - D.p0) * rng::u01_from_word_d(rng::block(D.seed, i, c).x);
         } else {
-            const rng::U4 b = rng::block(D.seed, (uint32_t)k, 0);
-            if (D.dtype == AETH_F64) static_cast<double*>(D.out)[k] = rng::normal_from_block_d(b);
-            else static_cast<float*>(D.out)[k] = rng::normal_from_block_f(b);
+            float* out = static_cast<float*>(D.out) + (uint64_t)c * D.pitch;
+            const float p0 = (float)D.p0, p1 = (float)D.p1;
+            for (uint32_t i = i0 + threadIdx.x; i < end; i += kThreads)
+                out[i] = p0 + (p1 - p0) * rng::u01_from_word_f(rng::block(D.seed, i, c).x);
+        }
+    } else {
+        for (uint32_t i = i0 + threadIdx.x; i < end; i += kThreads) {
+            const rng::U4 b = rng::block(D.seed, i, 0);
+            if (D.dtype == AETH_F64) static_cast<double*>(D.out)[i] = rng::normal_from_block_d(b);
+            else static_cast<float*>(D.out)[i] = rng::normal_from_block_f(b);
         }
     }
 }
@@ -512,7 +519,7 @@ BatchHeader build_batch(Slot& S, const Layout& L, void* base)
                 f.dtype = s.dtype;
                 f.kind = 0;
                 f.unit_start = fill_units;
-                fill_units += (3 * s.n + kFillPerBlock - 1) / kFillPerBlock;
+                fill_units += 3 * ((s.n + kFillPerBlock - 1) / kFillPerBlock);
                 fills.push_back(f);
             }
             EvalDesc e{};

@@ -244,16 +244,28 @@ def wl_stream_alone(log2n: int = 27) -> dict:
 
 
 def wl_vec3(a, b, c, n) -> dict:
-    """e = cross(a,b).dot(c) + sqrt(a.dot(a) + 1): 9 planes read, 1 written (80 B/sample)."""
+    """e = cross(a,b).dot(c) + sqrt(a.dot(a) + 1): 9 planes read, 1 written (80 B/sample) -- the hand-written
+    kernel, and the same tree through the drop-in lowering at the contract shape and the autotuned shape."""
     import torch
 
     import aether_amd as ae
+    from aether_amd import expr as E
 
     e = torch.empty(n, dtype=torch.float64, device="cuda")
     ms = _time(lambda: ae.eval_vec3geo(a, b, c, e))
     del e
-    return {"samples": n, "ms": round(ms, 4), "GB/s": _gbs(80 * n, ms),
-            "frac": round(80 * n / (ms * 1e-3) / 1e9 / 8000, 4)}
+    out = {"samples": n, "ms": round(ms, 4), "GB/s": _gbs(80 * n, ms),
+           "frac": round(80 * n / (ms * 1e-3) / 1e9 / 8000, 4)}
+    dest = ae.Array(n, 1, torch.float64)
+    tree = a.cross(b).dot(c) + E.sqrt(a.dot(a) + 1.0)
+    ms0 = _time(lambda: E.assign(dest, tree))
+    shape = E.tune(dest, tree)
+    ms1 = _time(lambda: E.assign(dest, tree))
+    out["via_tape"] = {"contract_shape_ms": round(ms0, 4), "contract_GB/s": _gbs(80 * n, ms0),
+                       "tuned_shape": [shape["tile"], shape["vector_bytes"], shape["waves_per_simd"]],
+                       "tuned_ms": round(ms1, 4), "tuned_GB/s": _gbs(80 * n, ms1)}
+    del dest
+    return out
 
 
 def wl_philox() -> dict:
@@ -572,6 +584,14 @@ def wl_serve(duration: float, rate: float = 20000.0, plain_seconds: float = 4.0)
     cap_nb = {f: sum(S.job_bytes(j, f) for j in cap_jobs) for f in (False, True)}
     with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True) as srv:
         summarize(srv.replay(cap_jobs), "capacity_fp32_compensated_closed_loop", cap_nb[True])
+    # the same with every batch one replay of a graph captured per stream slot (persistent batch kernels)
+    with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True,
+                  graph_batches=True) as srv:
+        summarize(srv.replay(cap_jobs), "capacity_fp32_compensated_closed_loop_graph_batches", cap_nb[True])
+    with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True,
+                  graph_batches=True) as srv:
+        summarize(srv.replay(jobs), "served_fp32_compensated_graph_batches", nb[True])
+        out["served_fp32_compensated_graph_batches"]["batches"] = int(srv.stats().batches)
     with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256) as srv:
         summarize(srv.replay(cap_jobs), "capacity_closed_loop", cap_nb[False])
 
