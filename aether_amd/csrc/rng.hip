@@ -99,25 +99,26 @@ __global__ __launch_bounds__(kThreads) void f32_kernel(uint64_t seed, uint64_t f
     }
 }
 
-// Array fill: plane c, sample i = p0 + (p1 - p0) u01(seed, i, c) or p0 + p1 N(seed, i, c).
+// Array fill: plane c, sample i = p0 + (p1 - p0) u01(seed, i, c) or p0 + p1 N(seed, i, c). Blocks walk
+// the components in y and the samples of a plane in x, so no per-element division is needed.
 template<class T, int DIST>
 __global__ __launch_bounds__(kThreads) void array_fill_kernel(uint64_t seed, T p0, T p1, int ncomp, uint32_t n,
                                                               uint64_t pitch, T* __restrict__ out)
 {
-    const uint64_t total = (uint64_t)ncomp * n;
-    for (uint64_t k = (uint64_t)blockIdx.x * kThreads + threadIdx.x; k < total; k += (uint64_t)gridDim.x * kThreads) {
-        const uint32_t c = (uint32_t)(k / n);
-        const uint32_t i = (uint32_t)(k - (uint64_t)c * n);
-        const rng::U4 b = rng::block(seed, i, c);
-        T v;
-        if constexpr (sizeof(T) == 8) {
-            if constexpr (DIST == AETH_DIST_UNIFORM) v = p0 + (p1 - p0) * rng::u01_from_word_d(b.x);
-            else v = p0 + p1 * rng::normal_from_block_d(b);
-        } else {
-            if constexpr (DIST == AETH_DIST_UNIFORM) v = p0 + (p1 - p0) * rng::u01_from_word_f(b.x);
-            else v = p0 + p1 * rng::normal_from_block_f(b);
+    for (uint32_t c = blockIdx.y; c < (uint32_t)ncomp; c += gridDim.y) {
+        T* plane = out + (uint64_t)c * pitch;
+        for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads) {
+            const rng::U4 b = rng::block(seed, (uint32_t)i, c);
+            T v;
+            if constexpr (sizeof(T) == 8) {
+                if constexpr (DIST == AETH_DIST_UNIFORM) v = p0 + (p1 - p0) * rng::u01_from_word_d(b.x);
+                else v = p0 + p1 * rng::normal_from_block_d(b);
+            } else {
+                if constexpr (DIST == AETH_DIST_UNIFORM) v = p0 + (p1 - p0) * rng::u01_from_word_f(b.x);
+                else v = p0 + p1 * rng::normal_from_block_f(b);
+            }
+            plane[i] = v;
         }
-        out[(uint64_t)c * pitch + i] = v;
     }
 }
 
@@ -176,25 +177,27 @@ extern "C" int aeth_array_fill(int dist, int dtype, uint64_t seed, double p0, do
         if ((uint64_t)ncomp * pitch > 0xFFFFFFFFull) fail("aeth_array_fill", "span exceeds the 32-bit offset_t");
         if (n == 0) return;
         hipStream_t st = as_stream(stream);
-        const int grid = grid_for((uint64_t)ncomp * n);
+        const uint32_t planes = (uint32_t)std::min<int>(ncomp, 65535);
+        const uint64_t per_plane = std::max<uint64_t>(1, (uint64_t)grid_for((uint64_t)ncomp * n) / planes);
+        const dim3 grid((uint32_t)std::min<uint64_t>(per_plane, (n + kThreads - 1) / kThreads), planes);
         const uint32_t nn = (uint32_t)n;
         if (dtype == AETH_F64) {
             auto* o = static_cast<double*>(out);
             if (dist == AETH_DIST_UNIFORM)
-                hipLaunchKernelGGL((array_fill_kernel<double, AETH_DIST_UNIFORM>), dim3(grid), dim3(kThreads), 0, st,
+                hipLaunchKernelGGL((array_fill_kernel<double, AETH_DIST_UNIFORM>), grid, dim3(kThreads), 0, st,
                                    seed, p0, p1, ncomp, nn, pitch, o);
             else if (dist == AETH_DIST_NORMAL)
-                hipLaunchKernelGGL((array_fill_kernel<double, AETH_DIST_NORMAL>), dim3(grid), dim3(kThreads), 0, st,
+                hipLaunchKernelGGL((array_fill_kernel<double, AETH_DIST_NORMAL>), grid, dim3(kThreads), 0, st,
                                    seed, p0, p1, ncomp, nn, pitch, o);
             else
                 fail("aeth_array_fill", "unknown distribution");
         } else if (dtype == AETH_F32) {
             auto* o = static_cast<float*>(out);
             if (dist == AETH_DIST_UNIFORM)
-                hipLaunchKernelGGL((array_fill_kernel<float, AETH_DIST_UNIFORM>), dim3(grid), dim3(kThreads), 0, st,
+                hipLaunchKernelGGL((array_fill_kernel<float, AETH_DIST_UNIFORM>), grid, dim3(kThreads), 0, st,
                                    seed, (float)p0, (float)p1, ncomp, nn, pitch, o);
             else if (dist == AETH_DIST_NORMAL)
-                hipLaunchKernelGGL((array_fill_kernel<float, AETH_DIST_NORMAL>), dim3(grid), dim3(kThreads), 0, st,
+                hipLaunchKernelGGL((array_fill_kernel<float, AETH_DIST_NORMAL>), grid, dim3(kThreads), 0, st,
                                    seed, (float)p0, (float)p1, ncomp, nn, pitch, o);
             else
                 fail("aeth_array_fill", "unknown distribution");

@@ -124,17 +124,17 @@ __device__ __forceinline__ void fill_unit(const FillDesc& D, uint64_t u)
     if (D.kind == 0) {
         if (D.dtype == AETH_F64) {
             double* out = static_cast<double*>(D.out) + (uint64_t)c * D.pitch;
-            for (uint32_t i = i0 + threadIdx.x; i < end; i += kThreads)
-                out[i] = D.p0 + (D.p1 - D.p0) * rng::u01_from_word_d(rng::block(D.seed, i, c).x);
+            for (uint64_t i = (uint64_t)i0 + threadIdx.x; i < end; i += kThreads)
+                out[i] = D.p0 + (D.p1 - D.p0) * rng::u01_from_word_d(rng::block(D.seed, (uint32_t)i, c).x);
         } else {
             float* out = static_cast<float*>(D.out) + (uint64_t)c * D.pitch;
             const float p0 = (float)D.p0, p1 = (float)D.p1;
-            for (uint32_t i = i0 + threadIdx.x; i < end; i += kThreads)
-                out[i] = p0 + (p1 - p0) * rng::u01_from_word_f(rng::block(D.seed, i, c).x);
+            for (uint64_t i = (uint64_t)i0 + threadIdx.x; i < end; i += kThreads)
+                out[i] = p0 + (p1 - p0) * rng::u01_from_word_f(rng::block(D.seed, (uint32_t)i, c).x);
         }
     } else {
-        for (uint32_t i = i0 + threadIdx.x; i < end; i += kThreads) {
-            const rng::U4 b = rng::block(D.seed, i, 0);
+        for (uint64_t i = (uint64_t)i0 + threadIdx.x; i < end; i += kThreads) {
+            const rng::U4 b = rng::block(D.seed, (uint32_t)i, 0);
             if (D.dtype == AETH_F64) static_cast<double*>(D.out)[i] = rng::normal_from_block_d(b);
             else static_cast<float*>(D.out)[i] = rng::normal_from_block_f(b);
         }

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -634,6 +635,21 @@ std::string gen_program(const aeth_program& P, bool dbl, uint32_t vb)
     return b;
 }
 
+// The generated kernel's symbol: aeth_tape_ and a 64-bit FNV-1a hash of the tape's structure and shape,
+// the same in every process and on every device, so profiles name each structure apart.
+std::string kernel_name(int dtype, int mode, int np, const aeth_program* P, bool nt, JitShape sh)
+{
+    const std::string key = tape_key(dtype, mode, np, P, nt, 0, sh);
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (unsigned char ch : key) {
+        h ^= ch;
+        h *= 0x100000001b3ull;
+    }
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "aeth_tape_%016llx", (unsigned long long)h);
+    return buf;
+}
+
 std::string gen_kernel(int dtype, int mode, int np, const aeth_program* P, bool nt, JitShape sh)
 {
     const bool dbl = dtype == AETH_F64;
@@ -663,7 +679,7 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
 )K";
     s += std::string("extern \"C\" __global__ __launch_bounds__(256") +
          (sh.waves ? ", " + std::to_string(sh.waves) : std::string()) +
-         ") void aeth_tape_jit(const JitArgs A, void* part_, int vec)\n{\n";
+         ") void " + kernel_name(dtype, mode, np, P, nt, sh) + "(const JitArgs A, void* part_, int vec)\n{\n";
     s += std::string("  typedef ") + (dbl ? "double" : "float") + " T;\n";
     // nontemporal is compiled in: a run-time choice between two otherwise identical loads is merged
     // by the optimiser into one plain load
@@ -751,7 +767,8 @@ hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, b
     }
     JitEntry e;
     check(hipModuleLoadData(&e.module, code.data()), "hipModuleLoadData");
-    check(hipModuleGetFunction(&e.fn, e.module, "aeth_tape_jit"), "hipModuleGetFunction");
+    check(hipModuleGetFunction(&e.fn, e.module, kernel_name(dtype, mode, np, P, nt, sh).c_str()),
+          "hipModuleGetFunction");
     g_jit.emplace(key, e);
     ++g_jit_compiled;
     return e.fn;

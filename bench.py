@@ -32,9 +32,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fused SoA eval GB/s (% HBM peak); served jobs/s + p50/p99 at stated err vs FP64"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fused kernel at 3 x 2^27 samples, from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# scaled by the same counters over a known-size copy (profiles/cfg0_f64_2p27_hbm_traffic.md).
-MEASURED_TRAFFIC_BYTES = {27: 12_890_958_603}
+# HBM bytes per launch of the headline kernel (the runtime-compiled tape kernel of the fused tree with sums)
+# at 3 x 2^27 samples: rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the same counters over a copy of known
+# size, measured separately (profiles/tape_cfg0_f64_2p27_hbm_traffic.md); PMC counters cannot be read from
+# inside a timed run, so this is a recorded figure, not one from this run.
+MEASURED_TRAFFIC_BYTES = {27: 12_890_979_775}
 
 
 # secondary workloads that run on their own (no arguments)
@@ -50,6 +52,8 @@ def parse():
     p.add_argument("--log2n", type=int, default=27)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    p.add_argument("--no-stream", action="store_true",
+                   help="leave out the stream-ceiling measurement (profiling the headline kernel alone)")
     p.add_argument("--serve-seconds", type=float, default=3.0,
                    help="length of the Poisson job stream (20k jobs/s) replayed into the server")
     p.add_argument("--cpu-log2n", type=int, default=24)
@@ -701,15 +705,18 @@ def main():
                    "samples": n, "components": 3, "arrays": 4, "bytes_per_step": bytes_step},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC_BYTES.get(args.log2n)},
+                     "traffic": MEASURED_TRAFFIC_BYTES.get(args.log2n),
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of this kernel, calibrated by a known copy "
+                                       "(profiles/tape_cfg0_f64_2p27_hbm_traffic.md)"},
         "launch_ms": {"min": round(launch_ms[0], 4), "median": round(launch_ms[len(launch_ms) // 2], 4),
                       "avg": round(avg_launch_ms, 4)},
         "sums": [s[k, 0].item() + s[k, 1].item() for k in range(3)],
     }
     # the device's own 3-read/1-write ceiling on the same arrays, beside the peak-based fraction
-    stream = wl_stream(a, b, c, d, n)
-    line["roofline"]["stream_ceiling"] = stream
-    line["roofline"]["frac_of_stream_ceiling"] = round(achieved / stream["ceiling_GB/s"], 4)
+    if not args.no_stream:
+        stream = wl_stream(a, b, c, d, n)
+        line["roofline"]["stream_ceiling"] = stream
+        line["roofline"]["frac_of_stream_ceiling"] = round(achieved / stream["ceiling_GB/s"], 4)
     if not args.no_extra:
         extra = {"fused_fp64_specialised_and_tuned": wl_tape(a, b, c, d, n), "vec3_geometric_f64": wl_vec3(a, b, c, n)}
         del a, b, c, d
