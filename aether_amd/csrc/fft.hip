@@ -68,17 +68,28 @@ __device__ __forceinline__ C64 sub(C64 a, C64 b) { return {a.re - b.re, a.im - b
 __device__ __forceinline__ CW add(CW a, CW b) { return {a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ CW sub(CW a, CW b) { return {a.re - b.re, a.im - b.im}; }
 
-// float-float sum for the butterflies: TwoSum of the high parts, the low parts added to its error, one
-// fast renormalisation (11 flops). Its error is below 2^-44 (|a| + |b|): a transform's error is measured
-// against the norm of its input, so cancellation in a + b costs nothing there (the accurate Ff2 add of
-// accum.hpp, 23 flops, stays on the reductions).
+// float-float sum for the butterflies: TwoSum of the high parts, the low parts added to its error (8 flops).
+// The pair is left unnormalised: the high part is the rounded sum of the high parts and the low part carries
+// everything else, so the pair's value is exact up to the rounding of the low part (2^-24 of it, below
+// 2^-47 (|a| + |b|) while the low parts stay below 2^-23 of their high parts, which a butterfly network of
+// at most 13 levels keeps). TwoSum is exact for any pair of floats, so later sums need no normalised input;
+// the value is rounded once to float at the store. A transform's error is measured against the norm of its
+// input, so cancellation in a + b costs nothing there (the accurate Ff2 add of accum.hpp, 23 flops, stays on
+// the reductions).
 __device__ __forceinline__ acc::Pair<float> ffadd(float ah, float al, float bh, float bl)
 {
     const float s = ah + bh;
     const float bb = s - ah;
-    const float e = (ah - (s - bb)) + (bh - bb) + (al + bl);
-    const float hi = s + e;
-    return acc::Pair<float>{hi, e - (hi - s)};
+    const float e = ((ah - (s - bb)) + (bh - bb)) + (al + bl);
+    return acc::Pair<float>{s, e};
+}
+// a - b: ffadd(a, -b) written without the negations (the same operations, so the same bits)
+__device__ __forceinline__ acc::Pair<float> ffsub(float ah, float al, float bh, float bl)
+{
+    const float s = ah - bh;
+    const float bb = s - ah;
+    const float e = ((ah - (s - bb)) - (bh + bb)) + (al - bl);
+    return acc::Pair<float>{s, e};
 }
 __device__ __forceinline__ CFF add(CFF a, CFF b)
 {
@@ -87,7 +98,7 @@ __device__ __forceinline__ CFF add(CFF a, CFF b)
 }
 __device__ __forceinline__ CFF sub(CFF a, CFF b)
 {
-    const auto r = ffadd(a.rh, a.rl, -b.rh, -b.rl), i = ffadd(a.ih, a.il, -b.ih, -b.il);
+    const auto r = ffsub(a.rh, a.rl, b.rh, b.rl), i = ffsub(a.ih, a.il, b.ih, b.il);
     return {r.h, r.c, i.h, i.c};
 }
 
@@ -105,6 +116,23 @@ __device__ __forceinline__ CFF rot(CFF x, bool fwd)
 {
     return fwd ? CFF{x.ih, x.il, -x.rh, -x.rl} : CFF{-x.ih, -x.il, x.rh, x.rl};
 }
+
+// a + rot(t) and a - rot(t) (rot: -i t forward, +i t inverse)
+template<class C>
+__device__ __forceinline__ C addrot(C a, C t, bool fwd) { return add(a, rot(t, fwd)); }
+template<class C>
+__device__ __forceinline__ C subrot(C a, C t, bool fwd) { return sub(a, rot(t, fwd)); }
+// the compensated type with the rotation folded into the sums (no negated pairs)
+template<>
+__device__ __forceinline__ CFF addrot(CFF a, CFF t, bool fwd)
+{
+    // forward: (a.re + t.im, a.im - t.re); inverse: (a.re - t.im, a.im + t.re)
+    const auto r = fwd ? ffadd(a.rh, a.rl, t.ih, t.il) : ffsub(a.rh, a.rl, t.ih, t.il);
+    const auto i = fwd ? ffsub(a.ih, a.il, t.rh, t.rl) : ffadd(a.ih, a.il, t.rh, t.rl);
+    return {r.h, r.c, i.h, i.c};
+}
+template<>
+__device__ __forceinline__ CFF subrot(CFF a, CFF t, bool fwd) { return addrot(a, t, !fwd); }
 
 // float-float product of (ah + al)(bh + bl), 2Prod + cross terms, fast2Sum renormalised
 __device__ __forceinline__ acc::Pair<float> ffmul(float ah, float al, float bh, float bl)
@@ -155,7 +183,8 @@ __device__ __forceinline__ C64 cmul(C64 a, C64 w)
     return {__builtin_fmaf(a.re, w.re, -(a.im * w.im)), __builtin_fmaf(a.re, w.im, a.im * w.re)};
 }
 // float-float x*y + u*v as one dot product: two exact products (FMA residuals), TwoSum of their high
-// parts, every low-order term (residuals, cross products) gathered into one float, one fast renormalisation
+// parts, every low-order term (residuals, cross products) gathered into one float; left unnormalised like
+// ffadd's pairs (16 flops)
 __device__ __forceinline__ acc::Pair<float> ffdot2(float xh, float xl, float yh, float yl, float uh, float ul,
                                                    float vh, float vl)
 {
@@ -169,8 +198,7 @@ __device__ __forceinline__ acc::Pair<float> ffdot2(float xh, float xl, float yh,
     e = __builtin_fmaf(xl, yh, e);
     e = __builtin_fmaf(uh, vl, e);
     e = __builtin_fmaf(ul, vh, e);
-    const float hi = s + e;
-    return acc::Pair<float>{hi, e - (hi - s)};
+    return acc::Pair<float>{s, e};
 }
 __device__ __forceinline__ CFF cmul(CFF a, CFF w)
 {
@@ -339,26 +367,46 @@ constexpr int kRP = 16;  // points per thread
 
 template<class C>
 struct Words;
+// The FP64 types cross LDS as two 8-byte words (real parts, then imaginary parts): half the LDS of whole
+// 16-byte points, so two workgroups of 8192 points (or four of 4096) share a CU.
 template<>
 struct Words<C128> {
-    static constexpr int n = 1;
-    using W = double2;
-    __device__ static W get(const C128& v, int) { return make_double2(v.re, v.im); }
-    __device__ static void put(C128& v, int, W w) { v = C128{w.x, w.y}; }
+    static constexpr int n = 2;
+    using W = double;
+    __device__ static W get(const C128& v, int w) { return w == 0 ? v.re : v.im; }
+    __device__ static void put(C128& v, int w, W x)
+    {
+        if (w == 0) v.re = x;
+        else v.im = x;
+    }
 };
+// the compensated type as two 8-byte words: the real pair, then the imaginary pair
 template<>
 struct Words<CFF> {
-    static constexpr int n = 1;
-    using W = float4;
-    __device__ static W get(const CFF& v, int) { return make_float4(v.rh, v.rl, v.ih, v.il); }
-    __device__ static void put(CFF& v, int, W w) { v = CFF{w.x, w.y, w.z, w.w}; }
+    static constexpr int n = 2;
+    using W = float2;
+    __device__ static W get(const CFF& v, int w) { return w == 0 ? make_float2(v.rh, v.rl) : make_float2(v.ih, v.il); }
+    __device__ static void put(CFF& v, int w, W x)
+    {
+        if (w == 0) {
+            v.rh = x.x;
+            v.rl = x.y;
+        } else {
+            v.ih = x.x;
+            v.il = x.y;
+        }
+    }
 };
 template<>
 struct Words<CW> {
-    static constexpr int n = 1;
-    using W = double2;
-    __device__ static W get(const CW& v, int) { return make_double2(v.re, v.im); }
-    __device__ static void put(CW& v, int, W w) { v = CW{w.x, w.y}; }
+    static constexpr int n = 2;
+    using W = double;
+    __device__ static W get(const CW& v, int w) { return w == 0 ? v.re : v.im; }
+    __device__ static void put(CW& v, int w, W x)
+    {
+        if (w == 0) v.re = x;
+        else v.im = x;
+    }
 };
 template<>
 struct Words<C64> {
@@ -394,6 +442,31 @@ __device__ __forceinline__ C w16(C x, bool fwd)
         return x;
     } else if constexpr (E % 16 == 4) {
         return rot(x, fwd);
+    } else if constexpr (std::is_same_v<C, CFF> && E % 2 == 0) {
+        // an odd multiple of 45 degrees: W = c (+-1 +- i), c = sqrt(2)/2 as a float-float pair, so the
+        // product is two float-float sums and two products by c (24 flops instead of a complex product's 32)
+        constexpr double cd = 0.70710678118654752440;
+        const float ch = (float)cd, cl = (float)(cd - (double)ch);
+        auto byc = [&](acc::Pair<float> a) {
+            const float p = a.h * ch;
+            float e = __builtin_fmaf(a.h, ch, -p);
+            e = __builtin_fmaf(a.h, cl, e);
+            e = __builtin_fmaf(a.c, ch, e);
+            return acc::Pair<float>{p, e};
+        };
+        // forward W16^t = c ((1 - i), (-1 - i), (-1 + i), (1 + i)) for t = 2, 6, 10, 14; the inverse is the
+        // forward form of 16 - t
+        constexpr int tf = E % 16, ti = (16 - E % 16) % 16;
+        const int t = fwd ? tf : ti;
+        const auto sum = ffadd(x.rh, x.rl, x.ih, x.il); // xr + xi
+        const auto dif = ffsub(x.ih, x.il, x.rh, x.rl); // xi - xr
+        acc::Pair<float> re, im;
+        if (t == 2) { re = sum; im = dif; }
+        else if (t == 6) { re = dif; im = acc::Pair<float>{-sum.h, -sum.c}; }
+        else if (t == 10) { re = acc::Pair<float>{-sum.h, -sum.c}; im = acc::Pair<float>{-dif.h, -dif.c}; }
+        else { re = acc::Pair<float>{-dif.h, -dif.c}; im = sum; }
+        const auto r = byc(re), i = byc(im);
+        return CFF{r.h, r.c, i.h, i.c};
     } else if constexpr (std::is_same_v<C, CFF>) {
         // the constant as a float-float pair (hi, lo) from its double value
         const double cd = kCos16[E % 16], sd = kCos16[(E + 12) % 16];
@@ -410,11 +483,11 @@ template<class C>
 __device__ __forceinline__ void dft4(C& a, C& b, C& c, C& d, bool fwd)
 {
     const C s02 = add(a, c), d02 = sub(a, c);
-    const C s13 = add(b, d), d13 = rot(sub(b, d), fwd);
+    const C s13 = add(b, d), t13 = sub(b, d);
     a = add(s02, s13);
-    b = add(d02, d13);
+    b = addrot(d02, t13, fwd);
     c = sub(s02, s13);
-    d = sub(d02, d13);
+    d = subrot(d02, t13, fwd);
 }
 
 // In-place DFT of R points, natural order in and out. N = A B: y[c + B d] = DFT_A over a of
@@ -529,21 +602,35 @@ __device__ __forceinline__ void twiddle_apply(C* x, const TwBase<R, C>& t)
     }
 }
 
+// The compensated complex64 type runs on the vector ALU's full issue rate, so its products of base twiddles
+// (11 float-float complex products a radix-16 butterfly) cost more than loading every power from the table
+// (which is also exact to the table's rounding): it reads W^(r m) for r = 1..R-1 directly.
+template<class C>
+constexpr bool kDirectTw = std::is_same_v<C, CFF>;
+
+template<int R, class C, class TW>
+__device__ __forceinline__ void twiddle_direct(C* x, uint32_t m, const TW& q, bool fwd)
+{
+#pragma unroll
+    for (int r = 1; r < R; ++r) x[r] = cmul(x[r], q.template at<C>(r * m, fwd));
+}
+
 template<int R, class C, class TW>
 __device__ __forceinline__ void twiddle_run(C* x, uint32_t m, const TW& q, bool fwd)
 {
-    twiddle_apply<R>(x, twiddle_fetch<R, C>(m, q, fwd));
+    if constexpr (kDirectTw<C>) twiddle_direct<R>(x, m, q, fwd);
+    else twiddle_apply<R>(x, twiddle_fetch<R, C>(m, q, fwd));
 }
 
 // One pass of radix R (stride Ns = product of the earlier radices) over the thread's 16 / R butterflies
 // b = threadIdx.x + u * kRT, b < nb; butterfly u holds v[u R .. u R + R - 1]; 2^l2nr butterflies a signal.
-template<int R, class C, class TW>
+template<int T, int R, class C, class TW>
 __device__ __forceinline__ void rpass(C (&v)[kRP], uint32_t l2nr, uint32_t nb, uint32_t Ns, const TW& q, bool fwd)
 {
     constexpr int U = kRP / R;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint32_t b = threadIdx.x + u * kRT;
+        const uint32_t b = threadIdx.x + u * T;
         if (b < nb) {
             if (Ns > 1) {
                 const uint32_t j = b & ((1u << l2nr) - 1u);
@@ -580,7 +667,7 @@ __device__ __forceinline__ void rpass16(C (&v)[kRP], uint32_t nb, const TwBase<1
 }
 
 // The first pass's inputs: butterfly (s, j) reads x[j + r N / R] of signal s, for r in [R0, R1).
-template<int R, class C, int R0 = 0, int R1 = R, int NV>
+template<int T, int R, class C, int R0 = 0, int R1 = R, int NV>
 __device__ __forceinline__ void rload(C (&v)[NV], const Sig* __restrict__ sigs, const void* __restrict__ in,
                                       uint32_t l2nr, uint32_t nb)
 {
@@ -588,7 +675,7 @@ __device__ __forceinline__ void rload(C (&v)[NV], const Sig* __restrict__ sigs, 
     static_assert(U == 1 || (R0 == 0 && R1 == R), "a point range is for radix-16 loads");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint32_t b = threadIdx.x + u * kRT;
+        const uint32_t b = threadIdx.x + u * T;
         if (b < nb) {
             const uint32_t s = b >> l2nr, j = b & ((1u << l2nr) - 1u);
             const uint64_t base = sigs[s].in_off + j;
@@ -599,14 +686,14 @@ __device__ __forceinline__ void rload(C (&v)[NV], const Sig* __restrict__ sigs, 
 }
 
 // The last pass's outputs (Ns = N / R): butterfly (s, j) writes X[j + r Ns] of signal s.
-template<int R, class C>
+template<int T, int R, class C>
 __device__ __forceinline__ void rstore(const C (&v)[kRP], const Sig* __restrict__ sigs, void* __restrict__ out,
                                        uint32_t l2nr, uint32_t nb, double scale)
 {
     constexpr int U = kRP / R;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint32_t b = threadIdx.x + u * kRT;
+        const uint32_t b = threadIdx.x + u * T;
         if (b < nb) {
             const uint32_t s = b >> l2nr, j = b & ((1u << l2nr) - 1u);
             const uint64_t base = sigs[s].out_off + j;
@@ -629,7 +716,7 @@ __device__ __forceinline__ uint32_t xpos(uint32_t e)
 
 // Pass (R, Ns) outputs to LDS at their Stockham places, then the next pass's (R2) inputs back, one word
 // of every point at a time.
-template<int R, int R2, class C>
+template<int T, int R, int R2, class C>
 __device__ __forceinline__ void rexchange(C (&v)[kRP], typename Words<C>::W* lds, uint32_t log2n, uint32_t P,
                                           uint32_t Ns)
 {
@@ -641,7 +728,7 @@ __device__ __forceinline__ void rexchange(C (&v)[kRP], typename Words<C>::W* lds
     for (int w = 0; w < Words<C>::n; ++w) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t b = threadIdx.x + u * kRT;
+            const uint32_t b = threadIdx.x + u * T;
             if (b < nb) {
                 const uint32_t s = b / nr, j = b - s * nr;
                 const uint32_t k = j & (Ns - 1u);
@@ -653,7 +740,7 @@ __device__ __forceinline__ void rexchange(C (&v)[kRP], typename Words<C>::W* lds
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < U2; ++u) {
-            const uint32_t b = threadIdx.x + u * kRT;
+            const uint32_t b = threadIdx.x + u * T;
             if (b < nb2) {
                 const uint32_t s = b / nr2, j = b - s * nr2;
                 const uint32_t base = s * N + j;
@@ -665,13 +752,13 @@ __device__ __forceinline__ void rexchange(C (&v)[kRP], typename Words<C>::W* lds
     }
 }
 
-template<int R, class C, class TW>
+template<int T, int R, class C, class TW>
 __device__ __forceinline__ void rsingle(C (&v)[kRP], const Sig* sigs, const void* in, void* out, uint32_t P,
                                         double scale, const TW& q, bool fwd)
 {
-    rload<R>(v, sigs, in, 0, P / R);
-    rpass<R>(v, 0, P / R, 1, q, fwd);
-    rstore<R>(v, sigs, out, 0, P / R, scale);
+    rload<T, R>(v, sigs, in, 0, P / R);
+    rpass<T, R>(v, 0, P / R, 1, q, fwd);
+    rstore<T, R>(v, sigs, out, 0, P / R, scale);
 }
 
 // The tail pass of radix R (stride Ns = N / R) after the passes of 16. Butterfly u of a thread sits at
@@ -682,7 +769,8 @@ __device__ __forceinline__ void rsingle(C (&v)[kRP], const Sig* sigs, const void
 template<int R, class C, class TW>
 __device__ __forceinline__ TwBase<R, C> rfetch_tail(uint32_t Ns, const TW& q, bool fwd)
 {
-    return twiddle_fetch<R, C>((threadIdx.x & (Ns - 1u)) * (kMaxN / (R * Ns)), q, fwd);
+    if constexpr (kDirectTw<C>) return TwBase<R, C>{}; // read in the pass (rtail)
+    else return twiddle_fetch<R, C>((threadIdx.x & (Ns - 1u)) * (kMaxN / (R * Ns)), q, fwd);
 }
 
 template<int R, class C, int u>
@@ -706,91 +794,132 @@ __device__ __forceinline__ void rtail_whole(C (&v)[kRP], const TwBase<R, C>& t0,
     }
 }
 
-template<int R, class C>
+template<int T, int R, class C, class TW>
 __device__ __forceinline__ void rtail(C (&v)[kRP], const Sig* sigs, void* out, uint32_t log2n, uint32_t P,
-                                      uint32_t Ns, double scale, const TwBase<R, C>& t0, bool fwd)
+                                      uint32_t Ns, double scale, const TwBase<R, C>& t0, const TW& q, bool fwd)
 {
     constexpr uint32_t lr = R == 2 ? 1 : R == 4 ? 2 : 3;
     constexpr int U = kRP / R;
     const uint32_t nb = P / R;
-    if (Ns <= (uint32_t)kRT) {
+    if constexpr (kDirectTw<C>) {
+        // butterfly u sits at j = (threadIdx.x + u T) mod Ns (T is a multiple of Ns, or Ns = 4096 > T)
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (threadIdx.x + u * kRT < nb) {
+            const uint32_t b = threadIdx.x + u * T;
+            if (b < nb) {
+                twiddle_direct<R>(&v[u * R], (b & (Ns - 1u)) * (kMaxN / (R * Ns)), q, fwd);
+                dft<R>(&v[u * R], fwd);
+            }
+        }
+    } else if (Ns <= (uint32_t)T) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (threadIdx.x + u * T < nb) {
                 twiddle_apply<R>(&v[u * R], t0);
                 dft<R>(&v[u * R], fwd);
             }
         }
-    } else {
-        // Ns > kRT happens only for log2n = 13 (R = 2, Ns = 4096): R Ns = kMaxN and nb = U kRT
+    } else if constexpr (T == kRT) {
+        // Ns > T happens only for log2n = 13 in the 512-thread form (R = 2, Ns = 4096): R Ns = kMaxN and
+        // nb = U T
         rtail_whole<R>(v, t0, fwd);
     }
-    rstore<R>(v, sigs, out, log2n - lr, nb, scale);
+    rstore<T, R>(v, sigs, out, log2n - lr, nb, scale);
 }
 
-// Transform one chunk: read, passes, write.
-template<class C, class TW>
-__device__ __forceinline__ void rchunk(C (&v)[kRP], const Desc d, const Sig* s, const void* in, void* out,
-                                       typename Words<C>::W* xch, const TW& q, bool fwd, int norm, double fixed_scale)
+// Transform one chunk of signals of 2^L points: read, passes, write. Called with a constant L (rchunk), every
+// shift, mask, pass count and stride folds and the passes unroll, so a pass's outputs stay in the registers
+// the next pass reads (no moves between passes; half the registers of the form with L read at run time).
+// LC < 0: the length is read from the descriptor at run time.
+template<int T, int LC, class C, class TW>
+__device__ __forceinline__ void rchunk_n(C (&v)[kRP], const Desc d, const Sig* s, const void* in, void* out,
+                                         typename Words<C>::W* xch, const TW& q, bool fwd, int norm,
+                                         double fixed_scale)
 {
-    const uint32_t L = d.log2n, N = 1u << L, P = (uint32_t)d.count << L;
+    const uint32_t L = LC >= 0 ? (uint32_t)LC : (uint32_t)d.log2n;
+    const uint32_t N = 1u << L;
+    const uint32_t P = (uint32_t)d.count << L;
     // numpy norms (fft/Fft.h:73-76); norm 3: a caller-fixed scale
     const double scale = norm == 3 ? fixed_scale
                          : norm == 2 ? 1.0 / __builtin_sqrt((double)N)
                                      : ((fwd == (norm == 1)) ? 1.0 / (double)N : 1.0);
     if (L == 0) {
-        for (uint32_t e = threadIdx.x; e < P; e += kRT) gstore(out, s[e].out_off, gload(in, s[e].in_off, (C*)nullptr), scale);
+        for (uint32_t e = threadIdx.x; e < P; e += T) gstore(out, s[e].out_off, gload(in, s[e].in_off, (C*)nullptr), scale);
         return;
     }
     const uint32_t nq = L / 4, rem = L % 4;
     if (nq == 0) {
-        if (rem == 1) rsingle<2>(v, s, in, out, P, scale, q, fwd);
-        else if (rem == 2) rsingle<4>(v, s, in, out, P, scale, q, fwd);
-        else rsingle<8>(v, s, in, out, P, scale, q, fwd);
+        if (rem == 1) rsingle<T, 2>(v, s, in, out, P, scale, q, fwd);
+        else if (rem == 2) rsingle<T, 4>(v, s, in, out, P, scale, q, fwd);
+        else rsingle<T, 8>(v, s, in, out, P, scale, q, fwd);
         return;
     }
-    rload<16>(v, s, in, L - 4, P / 16);
+    rload<T, 16>(v, s, in, L - 4, P / 16);
     uint32_t Ns = 1;
     TwBase<16, C> next{};
     // complex128 fetches each radix-16 pass's twiddles ahead of the exchange before it; complex64 keeps its
     // registers for two workgroups a CU and fetches them in the pass
-    constexpr bool ahead = sizeof(C) == 16;
-    for (uint32_t p = 0; p < nq; ++p) {
-        if (p == 0 || !ahead) rpass<16>(v, L - 4, P / 16, Ns, q, fwd);
+    constexpr bool ahead = sizeof(C) == 16 && !kDirectTw<C>;
+    // at most three radix-16 passes (kMaxLog2 = 13): a constant trip count unrolls for every length
+#pragma unroll
+    for (uint32_t p = 0; p < (uint32_t)kMaxLog2 / 4; ++p) {
+        if (p >= nq) break;
+        if (p == 0 || !ahead) rpass<T, 16>(v, L - 4, P / 16, Ns, q, fwd);
         else rpass16(v, P / 16, next, fwd);
         if (p + 1 == nq) break;
         if constexpr (ahead) next = rfetch16<C>(L - 4, P / 16, Ns * 16, q, fwd);
-        rexchange<16, 16>(v, xch, L, P, Ns);
+        rexchange<T, 16, 16>(v, xch, L, P, Ns);
         Ns *= 16;
     }
     if (rem == 0) {
-        rstore<16>(v, s, out, L - 4, P / 16, scale);
+        rstore<T, 16>(v, s, out, L - 4, P / 16, scale);
     } else if (rem == 1) {
         const TwBase<2, C> t0 = rfetch_tail<2, C>(Ns * 16, q, fwd);
-        rexchange<16, 2>(v, xch, L, P, Ns);
-        rtail<2>(v, s, out, L, P, Ns * 16, scale, t0, fwd);
+        rexchange<T, 16, 2>(v, xch, L, P, Ns);
+        rtail<T, 2>(v, s, out, L, P, Ns * 16, scale, t0, q, fwd);
     } else if (rem == 2) {
         const TwBase<4, C> t0 = rfetch_tail<4, C>(Ns * 16, q, fwd);
-        rexchange<16, 4>(v, xch, L, P, Ns);
-        rtail<4>(v, s, out, L, P, Ns * 16, scale, t0, fwd);
+        rexchange<T, 16, 4>(v, xch, L, P, Ns);
+        rtail<T, 4>(v, s, out, L, P, Ns * 16, scale, t0, q, fwd);
     } else {
         const TwBase<8, C> t0 = rfetch_tail<8, C>(Ns * 16, q, fwd);
-        rexchange<16, 8>(v, xch, L, P, Ns);
-        rtail<8>(v, s, out, L, P, Ns * 16, scale, t0, fwd);
+        rexchange<T, 16, 8>(v, xch, L, P, Ns);
+        rtail<T, 8>(v, s, out, L, P, Ns * 16, scale, t0, q, fwd);
     }
 }
 
-// complex64 fits two workgroups a CU (4 waves a SIMD, 128 registers); complex128's passes need more
-template<class C>
-__global__ __launch_bounds__(kRT, sizeof(C) == 8 ? 4 : 2) void fft_reg_kernel(const Desc* __restrict__ descs, const Sig* __restrict__ sigs,
+template<int T, class C, class TW>
+__device__ __forceinline__ void rchunk(C (&v)[kRP], const Desc d, const Sig* s, const void* in, void* out,
+                                       typename Words<C>::W* xch, const TW& q, bool fwd, int norm, double fixed_scale)
+{
+    switch (d.log2n) {
+#define AETH_FFT_LEN(l)                                                                                               \
+    case l: rchunk_n<T, l>(v, d, s, in, out, xch, q, fwd, norm, fixed_scale); break;
+        AETH_FFT_LEN(0) AETH_FFT_LEN(1) AETH_FFT_LEN(2) AETH_FFT_LEN(3) AETH_FFT_LEN(4) AETH_FFT_LEN(5)
+        AETH_FFT_LEN(6) AETH_FFT_LEN(7) AETH_FFT_LEN(8) AETH_FFT_LEN(9) AETH_FFT_LEN(10) AETH_FFT_LEN(11)
+        AETH_FFT_LEN(12)
+#undef AETH_FFT_LEN
+    case 13:
+        if constexpr (T == kRT) rchunk_n<T, 13>(v, d, s, in, out, xch, q, fwd, norm, fixed_scale);
+        break;
+    default: break;
+    }
+}
+
+// T = 512 threads hold a chunk of up to 8192 points of one length. With the length a constant every form fits
+// 128 registers and 64 KiB of LDS (8-byte exchange words), so two workgroups (16 waves) share a CU.
+// The direction is a template parameter, so the conjugations of the inverse fold into the arithmetic's sign
+// modifiers instead of selects.
+template<class C, int T, bool FWD>
+__global__ __launch_bounds__(T, 4) void fft_reg_kernel(const Desc* __restrict__ descs, const Sig* __restrict__ sigs,
                                                          const void* __restrict__ in, void* __restrict__ out,
-                                                         TwTables tw, int fwd_flag, int norm, double fixed_scale)
+                                                         TwTables tw, int norm, double fixed_scale)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     auto* xch = reinterpret_cast<typename Words<C>::W*>(lds_raw);
     const Desc d = descs[blockIdx.x];
     C v[kRP];
-    rchunk(v, d, sigs + d.first_sig, in, out, xch, TwGlobal{tw}, fwd_flag != 0, norm, fixed_scale);
+    rchunk<T>(v, d, sigs + d.first_sig, in, out, xch, TwGlobal{tw}, FWD, norm, fixed_scale);
 }
 
 // The persistent form: a fixed grid walks the descriptors, their number read from device memory, so one
@@ -807,7 +936,9 @@ __global__ __launch_bounds__(kRT, sizeof(C) == 8 ? 4 : 2) void fft_reg_persist_k
         __syncthreads(); // the previous chunk's exchanges are done with the LDS
         const Desc d = descs[k];
         C v[kRP];
-        rchunk(v, d, sigs + d.first_sig, in, out, xch, TwGlobal{tw}, fwd_flag != 0, norm, fixed_scale);
+        // the length read at run time: inside the persistent loop the per-length forms' loop-invariant
+        // values would all be hoisted and held at once
+        rchunk_n<kRT, -1>(v, d, sigs + d.first_sig, in, out, xch, TwGlobal{tw}, fwd_flag != 0, norm, fixed_scale);
     }
 }
 
@@ -828,6 +959,15 @@ __global__ void twiddle_kernel(double2* d, float2* f, float4* ff)
     f[m] = make_float2((float)c, (float)s);
     const float ch = (float)c, sh = (float)s;
     ff[m] = make_float4(ch, (float)(c - (double)ch), sh, (float)(s - (double)sh));
+}
+
+template<class C>
+void set_reg_lds()
+{
+    for (auto fn : {(const void*)fft_reg_kernel<C, kRT, true>, (const void*)fft_reg_kernel<C, kRT, false>})
+        check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(kRT * kRP * sizeof(typename Words<C>::W))),
+              "hipFuncSetAttribute(LDS)");
 }
 
 // Per-device twiddle tables, created once.
@@ -857,18 +997,10 @@ TwTables tables(hipStream_t st)
             check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsElems * 16),
                   "hipFuncSetAttribute(LDS)");
         }
-        check(hipFuncSetAttribute((const void*)fft_reg_kernel<C128>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kMaxN * sizeof(Words<C128>::W)),
-              "hipFuncSetAttribute(LDS)");
-        check(hipFuncSetAttribute((const void*)fft_reg_kernel<C64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kMaxN * 8),
-              "hipFuncSetAttribute(LDS)");
-        check(hipFuncSetAttribute((const void*)fft_reg_kernel<CFF>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kMaxN * sizeof(Words<CFF>::W)),
-              "hipFuncSetAttribute(LDS)");
-        check(hipFuncSetAttribute((const void*)fft_reg_kernel<CW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kMaxN * sizeof(Words<CW>::W)),
-              "hipFuncSetAttribute(LDS)");
+        set_reg_lds<C128>();
+        set_reg_lds<C64>();
+        set_reg_lds<CFF>();
+        set_reg_lds<CW>();
         check(hipFuncSetAttribute((const void*)fft_reg_persist_kernel<C128>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, kMaxN * sizeof(Words<C128>::W)),
               "hipFuncSetAttribute(LDS)");
@@ -956,6 +1088,23 @@ void launch_persistent(int prec, int grid, const Desc* descs, const Sig* sigs, c
     check(hipGetLastError(), "fft_reg_persist_kernel launch");
 }
 
+namespace {
+// Contiguous signals: one launch over every descriptor.
+template<class C>
+void launch_reg(const BatchPlan& p, const void* in, void* out, const TwTables& tw, int forward, int norm,
+                double fixed_scale, hipStream_t st)
+{
+    const size_t lds = kRT * kRP * sizeof(typename Words<C>::W);
+    if (forward)
+        hipLaunchKernelGGL((fft_reg_kernel<C, kRT, true>), dim3(p.ndesc), dim3(kRT), lds, st, p.descs, p.sigs, in,
+                           out, tw, norm, fixed_scale);
+    else
+        hipLaunchKernelGGL((fft_reg_kernel<C, kRT, false>), dim3(p.ndesc), dim3(kRT), lds, st, p.descs, p.sigs, in,
+                           out, tw, norm, fixed_scale);
+    check(hipGetLastError(), "fft_reg_kernel launch");
+}
+} // namespace
+
 void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm, hipStream_t st,
             double fixed_scale)
 {
@@ -963,19 +1112,10 @@ void launch(const BatchPlan& p, const void* in, void* out, int forward, int norm
     const size_t lds = lds_bytes(p.prec);
     if (p.ndesc == 0) return;
     if (p.stride == 1) {
-        if (p.prec == 2)
-            hipLaunchKernelGGL(fft_reg_kernel<CFF>, dim3(p.ndesc), dim3(kRT), kMaxN * sizeof(Words<CFF>::W), st,
-                               p.descs, p.sigs, in, out, tw, forward, norm, fixed_scale);
-        else if (p.prec == 3)
-            hipLaunchKernelGGL(fft_reg_kernel<CW>, dim3(p.ndesc), dim3(kRT), kMaxN * sizeof(Words<CW>::W), st,
-                               p.descs, p.sigs, in, out, tw, forward, norm, fixed_scale);
-        else if (p.prec == 0)
-            hipLaunchKernelGGL(fft_reg_kernel<C128>, dim3(p.ndesc), dim3(kRT), kMaxN * sizeof(Words<C128>::W), st, p.descs, p.sigs, in,
-                               out, tw, forward, norm, fixed_scale);
-        else
-            hipLaunchKernelGGL(fft_reg_kernel<C64>, dim3(p.ndesc), dim3(kRT), kMaxN * 8, st, p.descs, p.sigs, in,
-                               out, tw, forward, norm, fixed_scale);
-        check(hipGetLastError(), "fft_reg_kernel launch");
+        if (p.prec == 2) launch_reg<CFF>(p, in, out, tw, forward, norm, fixed_scale, st);
+        else if (p.prec == 3) launch_reg<CW>(p, in, out, tw, forward, norm, fixed_scale, st);
+        else if (p.prec == 0) launch_reg<C128>(p, in, out, tw, forward, norm, fixed_scale, st);
+        else launch_reg<C64>(p, in, out, tw, forward, norm, fixed_scale, st);
         return;
     }
     switch (p.prec) {

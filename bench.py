@@ -41,7 +41,7 @@ MEASURED_TRAFFIC_BYTES = {27: 12_890_979_775}
 
 # secondary workloads that run on their own (no arguments)
 SINGLE_WORKLOADS = {"philox": "wl_philox", "fp32": "wl_fp32", "trig": "wl_trig", "fft": "wl_fft", "stream": "wl_stream_alone",
-                    "accum": "wl_accum"}
+                    "accum": "wl_accum", "vec3": "wl_vec3_alone"}
 
 
 def parse():
@@ -245,6 +245,18 @@ def wl_stream_alone(log2n: int = 27) -> dict:
     for arr, s in ((a, 42), (b, 43), (c, 44)):
         ae.array_fill_uniform(arr, s, -1.0, 1.0)
     return wl_stream(a, b, c, d, n)
+
+
+def wl_vec3_alone(log2n: int = 27) -> dict:
+    import torch
+
+    import aether_amd as ae
+
+    n = 1 << log2n
+    a, b, c = (ae.Array(n, 3, torch.float64) for _ in range(3))
+    for arr, s in ((a, 42), (b, 43), (c, 44)):
+        ae.array_fill_uniform(arr, s, -1.0, 1.0)
+    return wl_vec3(a, b, c, n)
 
 
 def wl_vec3(a, b, c, n) -> dict:
