@@ -280,6 +280,17 @@ def wl_vec3(a, b, c, n) -> dict:
     out["via_tape"] = {"contract_shape_ms": round(ms0, 4), "contract_GB/s": _gbs(80 * n, ms0),
                        "tuned_shape": [shape["tile"], shape["vector_bytes"], shape["waves_per_simd"]],
                        "tuned_ms": round(ms1, 4), "tuned_GB/s": _gbs(80 * n, ms1)}
+    # the device's ceiling for this traffic shape: the same 9 planes read and 1 written with two dot
+    # products and nothing else (a.b + c.c), at the contract shape and tuned
+    stream = a.dot(b) + c.dot(c)
+    ms2 = _time(lambda: E.assign(dest, stream))
+    sshape = E.tune(dest, stream)
+    ms3 = _time(lambda: E.assign(dest, stream))
+    best = min(ms2, ms3)
+    out["stream_9r1w"] = {"contract_shape_ms": round(ms2, 4), "tuned_ms": round(ms3, 4),
+                          "tuned_shape": [sshape["tile"], sshape["vector_bytes"], sshape["waves_per_simd"]],
+                          "GB/s": _gbs(80 * n, best)}
+    out["frac_of_stream_ceiling"] = round(best / min(ms, ms0, ms1), 4)
     del dest
     return out
 
