@@ -13,7 +13,7 @@ import torch
 from . import F32, F64, Array, _seam, stream_handle
 from ._lib import AetherError, call
 
-__all__ = ["keyed_sum", "Plane", "AppendCounter", "append_where", "ALIGNED", "TESTED", "SORTED"]
+__all__ = ["keyed_sum", "Plane", "DeviceFlag", "AppendCounter", "append_where", "ALIGNED", "TESTED", "SORTED"]
 
 ALIGNED, TESTED, SORTED = 0, 1, 2
 _FORMS = {"aligned": ALIGNED, "tested": TESTED, "sorted": SORTED}
@@ -108,14 +108,44 @@ class Plane:
         return out
 
 
+class DeviceFlag:
+    """The sticky device error word (err/DeviceFlag.h:86-160): kernels set it (an atomic OR of 1 through
+    `ptr`, DeviceFlagView in include/aether/device/flag.hpp), the host reads it (`is_set`, never clearing
+    it), clears it, or turns it into an error (`check_and_clear`)."""
+
+    def __init__(self, device="cuda"):
+        self._mem = torch.empty(4, dtype=torch.int32, device=device)  # one word on a 16-byte slot
+        self.clear()
+
+    @property
+    def ptr(self) -> int:
+        return self._mem.data_ptr()
+
+    def is_set(self) -> bool:
+        torch.cuda.synchronize(self._mem.device)
+        return bool(self._mem[0].item())
+
+    def clear(self) -> None:
+        torch.cuda.synchronize(self._mem.device)
+        _fill_zero(self._mem.data_ptr(), 16)
+        torch.cuda.synchronize(self._mem.device)
+
+    def check_and_clear(self, context: str) -> None:
+        """If set: clear it and raise AetherError shaped like err::fail (err/Error.h:63)."""
+        if self.is_set():
+            self.clear()
+            raise AetherError(1, f"{context} failed on DeviceFlag, size=4 bytes (device flag was set)")
+
+
 class AppendCounter:
-    """A device slot counter over `capacity` places with an overflow flag (accum/AppendCounter.h:82-192)."""
+    """A device slot counter over `capacity` places with a DeviceFlag for overflow
+    (accum/AppendCounter.h:82-192)."""
 
     def __init__(self, capacity: int, device="cuda"):
         self.capacity = int(capacity)
-        # [counter, pad, overflow, pad]: each word on its own 16-byte slot, so each is reset by one fill
-        self._mem = torch.empty(8, dtype=torch.int32, device=device)
-        self.reset(clear_overflow=True)
+        self._mem = torch.empty(4, dtype=torch.int32, device=device)  # the counter on a 16-byte slot
+        self.overflow = DeviceFlag(device)
+        self.reset()
 
     @property
     def counter_ptr(self) -> int:
@@ -123,23 +153,24 @@ class AppendCounter:
 
     @property
     def overflow_ptr(self) -> int:
-        return self._mem.data_ptr() + 16
+        return self.overflow.ptr
 
     def raw_count(self) -> int:
         """The raw counter (may exceed capacity: every overflowing claim still incremented it)."""
         return int(self._mem[0].item()) & 0xFFFFFFFF
 
     def overflowed(self) -> bool:
-        return bool(self._mem[4].item())
+        """Non-destructive peek at the overflow flag."""
+        return self.overflow.is_set()
 
     def reset(self, clear_overflow: bool = False, stream=None) -> None:
-        _fill_zero(self._mem.data_ptr(), 32 if clear_overflow else 16, stream)
+        _fill_zero(self._mem.data_ptr(), 16, stream)
+        if clear_overflow:
+            self.overflow.clear()
 
-    def check_overflow(self) -> None:
-        """Raise if the flag is set, and clear it (AppendCounter::checkOverflow)."""
-        if self.overflowed():
-            _fill_zero(self.overflow_ptr, 16)
-            raise AetherError(1, f"AppendCounter: more claims than capacity {self.capacity}")
+    def check_overflow(self, context: str = "AppendCounter::checkOverflow") -> None:
+        """Raise if the overflow flag is set, and clear it (AppendCounter::checkOverflow)."""
+        self.overflow.check_and_clear(context)
 
     def commit(self) -> int:
         """The claimed count clamped to capacity; resets the counter for the next step (never raises)."""

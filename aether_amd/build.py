@@ -19,6 +19,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libaether_hip.so")
 API_CHECK = os.path.join(LIB_DIR, "api_check")
 RESIDENCY_CHECK = os.path.join(LIB_DIR, "residency_check")
+FLAG_CHECK = os.path.join(LIB_DIR, "flag_check")
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AETH_OFFLOAD_ARCH", "gfx950")
@@ -110,7 +111,22 @@ def build_api_check() -> str:
     """The C++ front end's test drivers (tests/cpp/api_check.cpp, tests/cpp/residency_check.cpp): host-only
     g++ builds against include/aether/*.hpp and libaether_hip.so, no HIP headers involved."""
     _host_driver("residency_check.cpp", RESIDENCY_CHECK)
+    build_flag_check()
     return _host_driver("api_check.cpp", API_CHECK)
+
+
+def build_flag_check() -> str:
+    """tests/cpp/flag_check.hip: a HIP program whose own kernels set aether::DeviceFlag through
+    DeviceFlagView (include/aether/device/flag.hpp), the way a user's kernels would; hipcc against
+    libaether_hip.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "flag_check.hip")
+    cmd = [HIPCC, *FLAGS, "-Werror", "-I", os.path.join(ROOT, "include"), src, "-o", FLAG_CHECK + ".tmp",
+           "-L", LIB_DIR, "-laether_hip", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr}")
+    os.replace(FLAG_CHECK + ".tmp", FLAG_CHECK)
+    return FLAG_CHECK
 
 
 def build_dsc_core() -> str:

@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kLanes) void append_tile_kernel(const T* __restrict
             else over = true;
         }
     }
-    if (over) *v.overflow = 1u;
+    if (over) v.overflow.set();
 }
 
 } // namespace
@@ -234,7 +234,7 @@ extern "C" int aeth_append_where(int dtype, const void* x, uint64_t n, double th
         if (!counter || !overflow) fail("aeth_append_where", "counter and overflow flag are required");
         if (n == 0) return;
         if (n > 0xFFFFFFFFull) fail("aeth_append_where", "indices are 32-bit");
-        const aether::accum::AppendCounterView v{counter, capacity, overflow};
+        const aether::accum::AppendCounterView v{counter, capacity, aether::DeviceFlagView(overflow)};
         if (claim < AETH_CLAIM_LANE || claim > AETH_CLAIM_TILE) fail("aeth_append_where", "unknown claim granularity");
         hipStream_t st = as_stream(stream);
         const dim3 g = lanes_grid(n);

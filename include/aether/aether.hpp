@@ -40,6 +40,7 @@
 #include <vector>
 
 #include "../aether_hip.h"
+#include "device/flag.hpp"
 
 namespace aether {
 
@@ -1163,6 +1164,94 @@ aeth_tape_shape tune(View<T, N> dest, const E& e, Stream s = nullptr)
 }
 
 // ---------------------------------------------------------------- accumulation
+/**
+ * The owning sticky device error word (err/DeviceFlag.h:86-160): one 32-bit word in device memory that
+ * kernels set through deviceView() (DeviceFlagView::set, an atomic OR) and a pinned host mirror the host
+ * reads it through. Sticky: a read never clears it; clear() or checkAndClear() does. Move-only.
+ */
+class DeviceFlag {
+public:
+    DeviceFlag() : dev_(detail::current_device())
+    {
+        detail::check(detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev_, 16, nullptr, &dchunk_, &dword_),
+                      "DeviceFlag allocation");
+        if (detail::seam().chunk_alloc(AETHER_MEM_PINNED, dev_, 16, nullptr, &hchunk_, &hword_) != 0) {
+            detail::seam().chunk_release(dchunk_);
+            dchunk_ = nullptr;
+            detail::check(1, "DeviceFlag allocation");
+        }
+        clear();
+    }
+    DeviceFlag(const DeviceFlag&) = delete;
+    DeviceFlag& operator=(const DeviceFlag&) = delete;
+    DeviceFlag(DeviceFlag&& o) noexcept
+        : dev_(o.dev_), dchunk_(std::exchange(o.dchunk_, nullptr)), hchunk_(std::exchange(o.hchunk_, nullptr)),
+          dword_(std::exchange(o.dword_, nullptr)), hword_(std::exchange(o.hword_, nullptr))
+    {
+    }
+    DeviceFlag& operator=(DeviceFlag&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            dev_ = o.dev_;
+            dchunk_ = std::exchange(o.dchunk_, nullptr);
+            hchunk_ = std::exchange(o.hchunk_, nullptr);
+            dword_ = std::exchange(o.dword_, nullptr);
+            hword_ = std::exchange(o.hword_, nullptr);
+        }
+        return *this;
+    }
+    ~DeviceFlag() { release(); }
+
+    /** The view a kernel takes by value. */
+    DeviceFlagView deviceView() const { return DeviceFlagView(static_cast<std::uint32_t*>(dword_)); }
+    /** Is the flag set? Waits for the device's work, reads the device word; does not clear it. */
+    bool isSet() const
+    {
+        detail::check(detail::seam().stream_sync(dev_, nullptr), "DeviceFlag::isSet");
+        detail::check(detail::seam().copy(AETHER_MEM_PINNED, dev_, hword_, AETHER_MEM_DEVICE, dev_, dword_,
+                                          sizeof(std::uint32_t), nullptr),
+                      "DeviceFlag::isSet");
+        detail::check(detail::seam().stream_sync(dev_, nullptr), "DeviceFlag::isSet");
+        std::uint32_t v = 0;
+        std::memcpy(&v, hword_, sizeof v);
+        return v != 0;
+    }
+    /** Reset the flag on the host and device copies. */
+    void clear()
+    {
+        const std::uint32_t z = 0;
+        std::memcpy(hword_, &z, sizeof z);
+        detail::check(detail::seam().stream_sync(dev_, nullptr), "DeviceFlag::clear");
+        detail::check(detail::seam().copy(AETHER_MEM_DEVICE, dev_, dword_, AETHER_MEM_PINNED, dev_, hword_,
+                                          sizeof(std::uint32_t), nullptr),
+                      "DeviceFlag::clear");
+        detail::check(detail::seam().stream_sync(dev_, nullptr), "DeviceFlag::clear");
+    }
+    /** If the flag is set: clear it (so the caller may keep using it) and throw aether::Error naming
+     *  `context`, shaped like err::fail (err/Error.h:63). */
+    void checkAndClear(const std::string& context)
+    {
+        if (!isSet()) return;
+        clear();
+        throw Error(context + " failed on DeviceFlag, size=" + std::to_string(sizeof(std::uint32_t)) +
+                    " bytes (device flag was set)");
+    }
+
+private:
+    void release()
+    {
+        if (dchunk_) detail::seam().chunk_release(dchunk_);
+        if (hchunk_) detail::seam().chunk_release(hchunk_);
+        dchunk_ = hchunk_ = nullptr;
+    }
+    int dev_ = 0;
+    void* dchunk_ = nullptr;
+    void* hchunk_ = nullptr;
+    void* dword_ = nullptr;
+    void* hword_ = nullptr;
+};
+
 namespace accum {
 
 inline constexpr std::uint32_t npos = 0xFFFFFFFFu; // accum/AppendCounter.h:26-29
@@ -1223,15 +1312,15 @@ private:
     Array<double, VD> comp_;
 };
 
-/** A device slot counter over `capacity` places with an overflow flag (accum/AppendCounter.h:82-192). */
+/** A device slot counter over `capacity` places with a DeviceFlag for overflow (accum/AppendCounter.h:82-192). */
 class AppendCounter {
 public:
     explicit AppendCounter(std::size_t capacity) : capacity_(capacity)
     {
         const int dev = aether::detail::current_device();
-        aether::detail::check(aether::detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, 32, nullptr, &chunk_, &mem_),
+        aether::detail::check(aether::detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, 16, nullptr, &chunk_, &mem_),
                               "AppendCounter allocation");
-        reset(true);
+        reset();
         aether::detail::check(aether::detail::seam().stream_sync(dev, nullptr), "AppendCounter");
     }
     AppendCounter(const AppendCounter&) = delete;
@@ -1242,21 +1331,22 @@ public:
     }
     std::size_t capacity() const { return capacity_; }
     std::uint32_t* counter() const { return static_cast<std::uint32_t*>(mem_); }
-    std::uint32_t* overflowFlag() const { return static_cast<std::uint32_t*>(mem_) + 4; }
+    std::uint32_t* overflowFlag() const { return overflow_.deviceView().data(); }
+    const DeviceFlag& overflow() const { return overflow_; }
     /** The raw count (may exceed capacity: every overflowing claim still incremented it). */
     std::size_t rawCount() const { return read(0); }
-    bool overflowed() const { return read(4) != 0; }
-    /** Zero the counter (and the overflow flag when asked), stream-ordered. */
+    /** Non-destructive peek at the overflow flag. */
+    bool overflowed() const { return overflow_.isSet(); }
+    /** Zero the counter, stream-ordered (and the overflow flag when asked). */
     void reset(bool clearOverflow = false, Stream s = nullptr)
     {
-        detail::zero_device(mem_, clearOverflow ? 32 : 16, s);
+        detail::zero_device(mem_, 16, s);
+        if (clearOverflow) overflow_.clear();
     }
-    /** Throw if the flag is set, and clear it. */
+    /** Throw if the overflow flag is set, and clear it. */
     void checkOverflow(const std::string& context = "AppendCounter::checkOverflow")
     {
-        if (!overflowed()) return;
-        detail::zero_device(overflowFlag(), 16, nullptr);
-        throw Error(context + ": more claims than capacity " + std::to_string(capacity_));
+        overflow_.checkAndClear(context);
     }
     /** Adopt the claimed slots into `arr` (clamped to its spare capacity) and reset for the next step. */
     template<class T, std::size_t N>
@@ -1282,6 +1372,7 @@ private:
     std::size_t capacity_;
     void* chunk_ = nullptr;
     void* mem_ = nullptr;
+    DeviceFlag overflow_;
 };
 
 /**

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "flag.hpp"
+
 namespace aether {
 namespace accum {
 
@@ -128,14 +130,14 @@ struct PlaneView {
 struct AppendCounterView {
     uint32_t* counter;
     uint64_t capacity;
-    uint32_t* overflow; // set to 1 when a claim does not fit
+    DeviceFlagView overflow; // set when a claim does not fit
 
     /** One atomic per claim: a slot < capacity, or npos (and the overflow flag) when full. */
     __device__ __forceinline__ uint32_t try_append() const
     {
         const uint32_t slot = atomicAdd(counter, 1u);
         if ((uint64_t)slot >= capacity) {
-            *overflow = 1u;
+            overflow.set();
             return npos;
         }
         return slot;
@@ -154,7 +156,7 @@ struct AppendCounterView {
         if (!want) return npos;
         const uint32_t slot = base + (uint32_t)__popcll(ballot & ((1ull << lane) - 1ull));
         if ((uint64_t)slot >= capacity || slot < base) {
-            *overflow = 1u;
+            overflow.set();
             return npos;
         }
         return slot;

@@ -201,7 +201,7 @@ def test_append_overflow_is_flagged_and_clamped(dev, claim):
     k = ctr.commit()
     got = out[:k].cpu().numpy()
     assert k == 1000 and len(np.unique(got)) == 1000 and np.all(x.cpu().numpy()[got] > 0.5)
-    with pytest.raises(ae.AetherError, match="capacity"):
+    with pytest.raises(ae.AetherError, match="DeviceFlag"):
         ctr.check_overflow()
     assert not ctr.overflowed()
 
