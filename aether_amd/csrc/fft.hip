@@ -1292,25 +1292,28 @@ __global__ __launch_bounds__(kAuxThreads) void real_part_kernel(const T* __restr
 // exactly in integers before the one rounding. The convolution runs in complex128 for both widths
 // (a complex64 plan reads and writes float, rounding once at the end), so a float transform of any
 // length is as accurate as the double one rounded to float.
-// ---- four-step passes for powers of two above 8192 (up to 2^26 points per line): a line of
+// ---- four-step passes for powers of two above 8192 (up to 2^34 points per line): a line of
 // n = n1 n2 points is an n1 x n2 matrix (element (j1, j2) at n2 j1 + j2); transform its columns
 // (n1 points, stride n2), multiply by W_n^(j2 k1), transform its rows (n2 points), and read the result
 // out transposed: X[k1 + n1 k2]. All in complex128 in a plan-owned copy of the lines; W_n^m is the
-// product of two table entries, W_n^(8192 h) W_n^l with m = 8192 h + l.
+// product of two table entries, W_n^(8192 h) W_n^l with m = 8192 h + l. Up to 2^26, n1 and n2 are at
+// most 8192 (one batched launch each); above, n1 = 8192 and the rows (n2 > 8192 points) are themselves
+// a four-step pass over the dense rows, in place in `work`.
 struct FourStepPass {
     uint64_t n = 0, n1 = 0, n2 = 0, lines = 0, inner = 1;
-    int prec = 0;              // of the data
-    BatchPlan* cols = nullptr; // lines x n2 signals of n1 points, stride n2, in `work`
-    BatchPlan* rows = nullptr; // lines x n1 signals of n2 points, contiguous, in `work`
-    double* work = nullptr;    // lines x n complex128
-    double* thi = nullptr;     // W_n^(8192 h), h < n / 8192
-    double* tlo = nullptr;     // W_n^l, l < 8192
+    int prec = 0;                     // of the data
+    BatchPlan* cols = nullptr;        // lines x n2 signals of n1 points, stride n2, in `work`
+    BatchPlan* rows = nullptr;        // lines x n1 signals of n2 points, contiguous, in `work` (n2 <= 8192)
+    FourStepPass* rows_four = nullptr; // the same rows when n2 > 8192
+    double* work = nullptr;           // lines x n complex128
+    double* thi = nullptr;            // W_n^(8192 h), h < n / 8192
+    double* tlo = nullptr;            // W_n^l, l < 8192
 };
 
 struct BluesteinPass;
 
 // One axis of a plan: a Stockham pass (powers of two up to 8192), a four-step pass (powers of two up
-// to 2^26) or a Bluestein pass (other lengths up to 2^25).
+// to 2^34) or a Bluestein pass (other lengths below 2^32).
 struct Pass {
     BatchPlan* stockham = nullptr;
     FourStepPass* four = nullptr;
@@ -1520,6 +1523,7 @@ struct SeamPlan {
         if (!f) return;
         free_pass(f->cols, bufs);
         free_pass(f->rows, bufs);
+        free_four(f->rows_four, bufs);
         bufs.push_back(f->work);
         bufs.push_back(f->thi);
         bufs.push_back(f->tlo);
@@ -1594,8 +1598,9 @@ BatchPlan* make_batch(const std::vector<uint64_t>& off, uint32_t l2, uint64_t st
     return bp.release();
 }
 
-constexpr uint32_t kMaxFourLog2 = 2 * kMaxLog2;          // four-step: up to 2^26 points
-constexpr uint64_t kMaxBluestein = 1ull << (kMaxFourLog2 - 1); // 2n - 1 <= 2^26
+constexpr uint32_t kMaxFourLog2 = 34; // four-step: up to 2^34 points (two levels of 8192-point factors)
+// Bluestein: 2n - 1 <= 2^34, and n < 2^32 so the chirp's j^2 stays in 64 bits
+constexpr uint64_t kMaxBluestein = 0xFFFFFFFFull;
 
 FourStepPass* make_four(uint64_t n, uint64_t lines, uint64_t inner, int dev, bool dbl, hipStream_t st)
 {
@@ -1605,15 +1610,14 @@ FourStepPass* make_four(uint64_t n, uint64_t lines, uint64_t inner, int dev, boo
         ~Guard()
         {
             if (!p) return;
-            SeamPlan::free_pass(p->cols);
-            SeamPlan::free_pass(p->rows);
-            (void)hipFree(p->work);
-            (void)hipFree(p->thi);
-            (void)hipFree(p->tlo);
+            std::vector<void*> bufs;
+            SeamPlan::free_four(p.release(), bufs);
+            for (void* b : bufs) (void)hipFree(b);
         }
     } guard{f};
     const uint32_t L = log2_exact(n);
-    const uint32_t L1 = L / 2, L2 = L - L1; // both <= 13
+    // up to 2^26: both factors <= 8192; above: columns of 8192 points, rows of 2^(L - 13) (a four-step)
+    const uint32_t L1 = L <= 2 * (uint32_t)kMaxLog2 ? L / 2 : (uint32_t)kMaxLog2, L2 = L - L1;
     f->n = n;
     f->n1 = 1ull << L1;
     f->n2 = 1ull << L2;
@@ -1629,9 +1633,13 @@ FourStepPass* make_four(uint64_t n, uint64_t lines, uint64_t inner, int dev, boo
         for (uint64_t j2 = 0; j2 < f->n2; ++j2) off.push_back(li * n + j2);
     f->cols = make_batch(off, L1, f->n2, dev, true);
     off.clear();
-    for (uint64_t li = 0; li < lines; ++li)
-        for (uint64_t k1 = 0; k1 < f->n1; ++k1) off.push_back(li * n + k1 * f->n2);
-    f->rows = make_batch(off, L2, 1, dev, true);
+    if (L2 <= (uint32_t)kMaxLog2) {
+        for (uint64_t li = 0; li < lines; ++li)
+            for (uint64_t k1 = 0; k1 < f->n1; ++k1) off.push_back(li * n + k1 * f->n2);
+        f->rows = make_batch(off, L2, 1, dev, true);
+    } else {
+        f->rows_four = make_four(f->n2, lines * f->n1, 1, dev, true, st);
+    }
     hipLaunchKernelGGL(fs_table_kernel, aux_grid(kTwLo + n / kTwLo), dim3(kAuxThreads), 0, st, f->thi, f->tlo, n);
     check(hipGetLastError(), "four-step table kernel");
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -1647,7 +1655,8 @@ void run_four(const FourStepPass& f, const void* in, void* out, int forward, dou
     launch(*f.cols, f.work, f.work, forward, 3, st, 1.0);
     hipLaunchKernelGGL(fs_twiddle_kernel, aux_grid(count), dim3(kAuxThreads), 0, st, f.work, (const double*)f.thi,
                        (const double*)f.tlo, f.lines, f.n2, f.n, forward ? 1.0 : -1.0);
-    launch(*f.rows, f.work, f.work, forward, 3, st, 1.0);
+    if (f.rows_four) run_four<double>(*f.rows_four, f.work, f.work, forward, 1.0, st);
+    else launch(*f.rows, f.work, f.work, forward, 3, st, 1.0);
     hipLaunchKernelGGL(fs_scatter_kernel<T>, aux_grid(count), dim3(kAuxThreads), 0, st, (const double*)f.work,
                        static_cast<T*>(out), f.lines, f.n1, f.n2, f.inner, scale);
     check(hipGetLastError(), "four-step kernels");
@@ -1715,7 +1724,7 @@ BluesteinPass* make_bluestein(uint64_t n, uint64_t lines, uint64_t inner, int de
 }
 
 // One strided-line pass along `ax` of a dense row-major complex `shape`: Stockham for powers of two up
-// to 8192, four-step for powers of two up to 2^26, Bluestein for other lengths up to 2^25; anything else
+// to 8192, four-step for powers of two up to 2^34, Bluestein for other lengths below 2^32; anything else
 // is unsupported (2).
 Pass make_pass(const std::vector<int64_t>& shape, int ax, int dev, bool dbl, bool forward, hipStream_t st)
 {
@@ -1743,7 +1752,7 @@ Pass make_pass(const std::vector<int64_t>& shape, int ax, int dev, bool dbl, boo
         return p;
     }
     if (n == 0 || n > kMaxBluestein)
-        fail("fft_plan", "axis lengths must be powers of two up to 2^26 or other lengths up to 2^25", 0, 2);
+        fail("fft_plan", "axis lengths must be powers of two up to 2^34 or other lengths below 2^32", 0, 2);
     if (lines) p.blue = make_bluestein(n, lines, inner, dev, dbl, forward, st);
     return p;
 }

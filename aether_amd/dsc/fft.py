@@ -5,8 +5,8 @@
 Conventions (the interface): complex data are float32 / float64 arrays whose last axis holds (re, im), and
 ``axis`` / ``axes`` count the other ("data") axes; ``norm`` is numpy's ("backward", "forward", "ortho").
 Every transform runs on the GPU: device arrays on their device, host and pinned arrays staged through it
-(the result is then a host array). Power-of-two lengths up to 2^26 and other lengths up to 2^25 are
-supported; longer ones raise ``FftUnavailable``.
+(the result is then a host array). Power-of-two lengths up to 2^34 and other lengths below 2^32 are
+supported (device memory permitting); longer ones raise ``FftUnavailable``.
 
 A plan is built once per (kind, dtype, shape, axes, direction, norm, place) and kept; building one is
 refused inside a stream capture, while a device plan's ``execute(input, output, stream=None)`` allocates

@@ -7,9 +7,9 @@ Complex data are float32 / float64 tensors whose last axis holds (re, im); axes 
 DLPack, or numpy arrays); the transform always runs on the GPU through this library's seam plans
 (``aether_memory_v1().fft_plan``): a host array is copied to the device and the result copied back.
 
-Lengths: powers of two up to 8192 run the LDS-resident Stockham transform, powers of two up to 2^26 a
-four-step transform over it; other lengths up to 2^25 run Bluestein's chirp-z through the same
-transforms; anything longer raises ``FftUnavailable`` (code 2, as the reference's plugin does when its FFT
+Lengths: powers of two up to 8192 run the LDS-resident Stockham transform, powers of two up to 2^34 a
+four-step transform over it (two levels past 2^26); other lengths below 2^32 run Bluestein's chirp-z
+through the same transforms; anything longer raises ``FftUnavailable`` (code 2, as the reference's plugin does when its FFT
 library is missing).
 
 Plans are built eagerly, outside a stream capture; ``plan(...)`` (and the first call of a transform on a

@@ -33,9 +33,9 @@ enum { AETH_F64 = 0, AETH_F32 = 1 };
  * (dsc/src/seam.h:16-51), so the reference's loader (dsc/src/aether_core.cpp:
  * 95-113: dlopen, abi check, probe) binds this library unchanged. Kinds:
  * CPU, DEVICE (HIP device memory), PINNED (hipHostMalloc). FFT plans cover
- * C2C, R2C and C2R over axes of power-of-two length <= 2^26 (Stockham up to
- * 8192, four-step above) or any other length <= 2^25 (Bluestein); longer
- * axes return 2. For R2C/C2R `shape` is
+ * C2C, R2C and C2R over axes of power-of-two length <= 2^34 (Stockham up to
+ * 8192, four-step above, two levels past 2^26) or any other length < 2^32
+ * (Bluestein); longer axes return 2. For R2C/C2R `shape` is
  * the real side's and the last axis is n/2 + 1 long on the complex side
  * (dsc/src/cufft_plan.h:9-12). fft_plan is refused while its stream is
  * captured; fft_exec allocates nothing and may be captured; fft_destroy is

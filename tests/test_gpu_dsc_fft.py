@@ -127,10 +127,10 @@ def test_misuse_is_named(dev):
     for call, words in wrong:
         with pytest.raises(aether.Error, match=words):
             call()
-    with pytest.raises(aether.Error):  # neither a power of two nor within Bluestein's 2^25
-        F.plan("c2c", "float32", ((1 << 25) + 1,), (0,), device="cuda")
-    with pytest.raises(aether.FftUnavailable):  # a power of two beyond 2^26
-        F.plan("c2c", "float64", (1 << 27,), (0,), device="cuda")
+    with pytest.raises(aether.Error):  # neither a power of two nor below Bluestein's 2^32
+        F.plan("c2c", "float32", ((1 << 32) + 1,), (0,), device="cuda")
+    with pytest.raises(aether.FftUnavailable):  # a power of two beyond 2^34
+        F.plan("c2c", "float64", (1 << 35,), (0,), device="cuda")
     pin = aether.pinned((8, 2), "float32")
     pin.zero()
     assert F.fft(pin).device.kind == "cpu"  # pinned is host memory: host path, host result

@@ -1,7 +1,7 @@
 """GPU: aether_amd.fft -- the reference's aether.fft API for torch tensors -- against numpy.fft with the
 reference's error bound (tests/fft_rows.py). Beyond the rows of the array package's test
-(test_gpu_dsc_fft.py): lengths past the radix kernels' 8192 (four-step for powers of two, Bluestein's
-chirp-z with a four-step convolution otherwise), a long axis that is not the last one, long real
+(test_gpu_dsc_fft.py): lengths past the radix kernels' 8192 (four-step for powers of two, two levels of it
+past 2^26, Bluestein's chirp-z with a four-step convolution otherwise), a long axis that is not the last one, long real
 transforms, host tensors coming back on the host, and a Bluestein plan and a c2r plan captured into one
 graph and replayed."""
 import math
@@ -147,9 +147,23 @@ def test_named_misuse(dev):
     for call, words in cases:
         with pytest.raises(ae.AetherError, match=words):
             call()
-    for shape in (((1 << 25) + 1,), (1 << 27,)):  # beyond Bluestein's 2^25; a power of two beyond 2^26
+    for shape in (((1 << 32) + 1,), (1 << 35,)):  # Bluestein stops below 2^32, the four-step at 2^34
         with pytest.raises(F.FftUnavailable):
             F.plan("c2c", "float64", shape, (0,))
+
+
+@pytest.mark.parametrize("n", [1 << 27, (1 << 25) + 1], ids=["pow2_2^27", "bluestein_2^25+1"])
+def test_lengths_past_two_levels(dev, n):
+    """A power of two past 2^26 (columns of 8192 points, rows of 2^14 points that are a four-step pass
+    themselves) and a Bluestein length whose convolution is such a transform (m = 2^27), forward and
+    inverse, against numpy.fft with the reference's bound; the round trip within twice the bound."""
+    x = Data(27).complex_pairs((n,), "float64")
+    xt = gpu(x)
+    X = F.fft(xt)
+    err = worst(to_complex(back(X)), np.fft.fft(to_complex(x)))
+    assert err <= bound("float64", n, x, 1.0), err
+    xb = F.ifft(X)
+    assert worst(to_complex(back(xb)), to_complex(x)) <= 2 * bound("float64", n, x, 1.0 / n) * n
 
 
 def test_bluestein_and_c2r_plans_replay_in_one_graph(dev):

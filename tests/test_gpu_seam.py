@@ -228,12 +228,12 @@ def test_fft_plan_axes(seam, orc, shape, axes, norm, forward):
 
 def test_fft_plan_unsupported(seam):
     s = torch.cuda.current_stream().cuda_stream
-    shp = (C.c_int64 * 1)((1 << 25) + 1)  # not a power of two, and beyond Bluestein's 2^25
+    shp = (C.c_int64 * 1)((1 << 32) + 1)  # not a power of two, and not below Bluestein's 2^32
     ax = (C.c_int32 * 1)(0)
     plan = _vp()
     assert seam.fft_plan(0, 0, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 2
     assert seam.fft_plan(0, 1, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 2  # R2C of that length
-    shp[0] = 1 << 27  # a power of two beyond the four-step's 2^26
+    shp[0] = 1 << 35  # a power of two beyond the four-step's 2^34
     assert seam.fft_plan(0, 0, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 2
     assert seam.fft_plan(0, 3, 1, 0, 1, 1, shp, 1, ax, s, C.byref(plan)) == 1  # unknown kind
     ax2 = (C.c_int32 * 2)(0, 0)
