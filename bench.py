@@ -40,8 +40,8 @@ MEASURED_TRAFFIC_BYTES = {27: 12_890_979_775}
 
 
 # secondary workloads that run on their own (no arguments)
-SINGLE_WORKLOADS = {"philox": "wl_philox", "fp32": "wl_fp32", "trig": "wl_trig", "fft": "wl_fft", "stream": "wl_stream_alone",
-                    "accum": "wl_accum", "vec3": "wl_vec3_alone"}
+SINGLE_WORKLOADS = {"cfg0_small": "wl_cfg0_small", "philox": "wl_philox", "fp32": "wl_fp32", "trig": "wl_trig", "fft": "wl_fft", "stream": "wl_stream_alone",
+                    "accum": "wl_accum", "vec3": "wl_vec3_alone", "serve": "wl_serve_alone"}
 
 
 def parse():
@@ -199,6 +199,53 @@ def wl_tape(a, b, c, d, n) -> dict:
     out["no_sums"] = {"contract_shape_ms": round(ms0, 4), "tuned_shape": [shape["tile"], shape["vector_bytes"], shape["waves_per_simd"]],
                       "tuned_ms": round(ms1, 4), "tuned_GB/s": _gbs(3 * n * 32, ms1)}
     return out
+
+
+def wl_pcie(a, b, c, d, n) -> dict:
+    """The same step end to end from pinned host memory: upload a, b, c, the fused assignment with sums,
+    download d. Its rate counts the step's 32 B per (component, sample) against the whole time, PCIe
+    copies included; it is reported beside the headline, never as it."""
+    import torch
+
+    import aether_amd as ae
+    from aether_amd import expr as E
+
+    hosts = [torch.empty((3, n), dtype=torch.float64, pin_memory=True) for _ in range(4)]
+    for h, arr in zip(hosts, (a, b, c)):
+        h.copy_(arr.data[:, :n])
+    tree = a + b * c + E.sin(a) * E.exp(-1.0 * (b * b))
+    sums = torch.empty((3, 2), dtype=torch.float64, device="cuda")
+
+    def run():
+        for h, arr in zip(hosts, (a, b, c)):
+            arr.data[:, :n].copy_(h, non_blocking=True)
+        E.assign(d, tree, ae.SUM_NEUMAIER, sums)
+        hosts[3].copy_(d.data[:, :n], non_blocking=True)
+
+    ms = _time(run, reps=3, warm=1)
+    del hosts
+    return {"ms": round(ms, 3), "GB/s_step_bytes": _gbs(3 * n * 32, ms),
+            "pcie_bytes": 4 * 3 * n * 8, "pcie_GB/s": _gbs(4 * 3 * n * 8, ms)}
+
+
+def wl_cfg0_small(log2n: int = 20) -> dict:
+    """The CPU-runnable configuration's size on the GPU: 3 x Array<double,3> of 2^20 samples with the fused
+    sums through the drop-in path. Its 96 MiB working set fits the 256 MB Infinity Cache, so the rate is
+    not an HBM figure."""
+    import torch
+
+    import aether_amd as ae
+    from aether_amd import expr as E
+
+    n = 1 << log2n
+    a, b, c, d = (ae.Array(n, 3, torch.float64) for _ in range(4))
+    for arr, s in ((a, 42), (b, 43), (c, 44)):
+        ae.array_fill_uniform(arr, s, -1.0, 1.0)
+    tree = a + b * c + E.sin(a) * E.exp(-1.0 * (b * b))
+    sums = torch.empty((3, 2), dtype=torch.float64, device="cuda")
+    ms = _time(lambda: E.assign(d, tree, ae.SUM_NEUMAIER, sums), reps=50, warm=5)
+    return {"samples": n, "ms": round(ms, 4), "GB/s": _gbs(3 * n * 32, ms),
+            "note": "working set 96 MiB: Infinity-Cache resident, not an HBM rate"}
 
 
 def wl_stream(a, b, c, d, n) -> dict:
@@ -498,6 +545,10 @@ def wl_fft() -> dict:
     return res
 
 
+def wl_serve_alone() -> dict:
+    return wl_serve(2.0)
+
+
 def fp32_error_vs_fp64(jobs, r32, per_kind: int = 12) -> dict:
     """Error of FP32-compensated served results against an FP64 oracle on the SAME FP32 inputs:
     the job's float inputs are regenerated, widened exactly to double and pushed through the FP64
@@ -741,8 +792,11 @@ def main():
         line["roofline"]["stream_ceiling"] = stream
         line["roofline"]["frac_of_stream_ceiling"] = round(achieved / stream["ceiling_GB/s"], 4)
     if not args.no_extra:
-        extra = {"fused_fp64_specialised_and_tuned": wl_tape(a, b, c, d, n), "vec3_geometric_f64": wl_vec3(a, b, c, n)}
+        extra = {"fused_fp64_specialised_and_tuned": wl_tape(a, b, c, d, n), "vec3_geometric_f64": wl_vec3(a, b, c, n),
+                 "end_to_end_from_pinned_host": wl_pcie(a, b, c, d, n)}
         del a, b, c, d
+        torch.cuda.empty_cache()
+        extra["cfg0_2^20_cache_resident"] = wl_cfg0_small()
         torch.cuda.empty_cache()
         extra["philox_seed42"] = wl_philox()
         torch.cuda.empty_cache()
