@@ -33,6 +33,7 @@ SIGNATURES = {
     "aeth_tape_jit_stats": (_i32, [C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
     "aeth_tape_jit_build": (_i32, [_i32, _i32, _i32, _vp, C.c_char_p, C.POINTER(_u64)]),
     "aeth_tape_tune": (_i32, [_i32, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _u64, _vp, _vp]),
+    "aeth_tape_tune_sums": (_i32, [_i32, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _u64, _i32, _vp, _vp, _vp]),
     "aeth_reduce": (_i32, [_i32, _i32, _i32, _vp, _i32, _u64, _u64, _vp, _vp]),
     "aeth_math": (_i32, [_i32, _i32, _vp, _vp, _u64, _vp]),
     "aeth_fft_batch_plan": (_i32, [_i32, _u64, _vp, _vp, _vp, C.POINTER(_vp)]),

@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -468,10 +469,11 @@ struct JitEntry {
     hipFunction_t fn = nullptr;
 };
 
-// The launch shape of a runtime-compiled tape kernel: samples per workgroup tile and bytes per vector
-// access. Assignments with sums keep the contract shape (2048-sample tiles of 16-byte accesses: the
-// reduction order is part of the numerics); assignments without sums take the shape the autotuner picked
-// for their tape structure on this device (aeth_tape_tune), or the contract shape.
+// The launch shape of a runtime-compiled tape kernel: samples per workgroup tile, bytes per vector access
+// and the register allocator's occupancy target. Assignments with sums keep the contract tile and width
+// (2048-sample tiles of 16-byte accesses: the reduction order is part of the numerics) and take the
+// occupancy target aeth_tape_tune_sums picked, if any; assignments without sums take the shape
+// aeth_tape_tune picked for their tape structure on this device, or the contract shape.
 struct JitShape {
     uint32_t tile = 2048;
     uint32_t vb = 16;
@@ -479,6 +481,9 @@ struct JitShape {
 };
 constexpr JitShape kShapes[] = {{1024, 16, 0}, {2048, 16, 0}, {4096, 16, 0}, {2048, 8, 0}, {4096, 8, 0},
                                 {2048, 16, 8}};
+// Assignments with sums: the contract tile and access width fix the sums' bits, so only the register
+// allocator's occupancy target varies (it changes no operation).
+constexpr JitShape kSumShapes[] = {{2048, 16, 0}, {2048, 16, 4}, {2048, 16, 8}};
 std::mutex g_jit_mu;
 std::map<std::string, JitEntry> g_jit;
 std::map<std::string, JitShape> g_tuned; // base tape key -> tuned shape (guarded by g_jit_mu)
@@ -845,7 +850,8 @@ bool eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_le
             JitShape sh{};
             if (force) {
                 sh = *force;
-            } else if (sum_mode == AETH_SUM_NONE) {
+            } else {
+                // without sums any tuned shape; with sums only occupancy targets were tried (kSumShapes)
                 int dev = 0;
                 check(hipGetDevice(&dev), "hipGetDevice");
                 std::lock_guard<std::mutex> lock(g_jit_mu);
@@ -1222,81 +1228,109 @@ extern "C" int aeth_eval_tape_ex(int dtype, int n_programs, const aeth_program_e
     });
 }
 
+namespace {
+
+// Time every candidate shape of `dest = tape` (with `sum_mode`) on the caller's arrays, keep the fastest for
+// the tape's structure on this device. Every candidate writes the same values.
+void tune_tape(const char* who, int dtype, int n_programs, const aeth_program* programs, int n_leaves,
+               void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs,
+               const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums, void* stream,
+               const JitShape* cands, size_t ncands, aeth_tape_shape* chosen)
+{
+    hipStream_t st = as_stream(stream);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        fail(who, "tuning is not allowed inside a stream capture");
+    // every candidate writes the same values, so the destination ends holding the assignment's result; an
+    // assignment that reads any plane it writes (a = 2a + b) would not, and is refused
+    if (n_programs < 1 || n_programs > AETH_TAPE_MAX_PROGRAMS) fail(who, "1..8 programs");
+    if (n_leaves < 1 || n_leaves > AETH_TAPE_MAX_LEAVES) fail(who, "1..16 leaves");
+    if (sum_mode != AETH_SUM_NONE && !sums) fail(who, "sums is NULL");
+    const size_t es = dtype == AETH_F64 ? 8 : 4;
+    for (int p = 0; p < n_programs; ++p) {
+        validate(programs[p], n_leaves, n_consts, n_rngs);
+        for (int q = 0; q < n_programs; ++q) {
+            const auto& Q = programs[q];
+            const char* out = static_cast<const char*>(leaves[Q.out_leaf]) + (uint64_t)Q.out_comp * leaf_pitch[Q.out_leaf] * es;
+            for (int k = 0; k < programs[p].n_loads; ++k) {
+                const auto& P = programs[p];
+                const char* in = static_cast<const char*>(leaves[P.load_leaf[k]]) +
+                                 (uint64_t)P.load_comp[k] * leaf_pitch[P.load_leaf[k]] * es;
+                if (in == out) fail(who, "the destination is also read: tune an assignment into another array");
+            }
+        }
+    }
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "hipEventCreate");
+    check(hipEventCreate(&e1), "hipEventCreate");
+    JitShape best = cands[0];
+    float best_ms = -1.0f;
+    try {
+        for (size_t i = 0; i < ncands; ++i) {
+            const JitShape& sh = cands[i];
+            auto once = [&] {
+                eval_tape(dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs, rngs,
+                          n, sum_mode, sums, stream, &sh);
+            };
+            once(); // compiles and warms the variant
+            constexpr int kReps = 3;
+            check(hipEventRecord(e0, st), "hipEventRecord");
+            for (int r = 0; r < kReps; ++r) once();
+            check(hipEventRecord(e1, st), "hipEventRecord");
+            check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            ms /= kReps;
+            if (best_ms < 0.0f || ms < best_ms) {
+                best_ms = ms;
+                best = sh;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    uint64_t span_bytes = 0;
+    for (int p = 0; p < n_programs; ++p) span_bytes += (uint64_t)(programs[p].n_loads + 1) * n * es;
+    const bool nt = span_bytes >= (512ull << 20);
+    {
+        std::lock_guard<std::mutex> lock(g_jit_mu);
+        g_tuned[tape_key(dtype, sum_mode, n_programs, programs, nt, dev)] = best;
+    }
+    if (chosen) {
+        chosen->tile = best.tile;
+        chosen->vector_bytes = best.vb;
+        chosen->waves_per_simd = best.waves;
+        chosen->ms = best_ms;
+    }
+}
+
+} // namespace
+
 extern "C" int aeth_tape_tune(int dtype, int n_programs, const aeth_program* programs, int n_leaves,
                               void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts,
                               int n_rngs, const aeth_rng_leaf* rngs, uint64_t n, void* stream, aeth_tape_shape* chosen)
 {
     return guarded([&] {
-        hipStream_t st = as_stream(stream);
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
-            fail("aeth_tape_tune", "tuning is not allowed inside a stream capture");
-        // every candidate writes the same values (the assignment has no sums), so the destination ends
-        // holding the assignment's result; an assignment that reads any plane it writes (a = 2a + b)
-        // would not, and is refused
-        if (n_programs < 1 || n_programs > AETH_TAPE_MAX_PROGRAMS) fail("aeth_tape_tune", "1..8 programs");
-        if (n_leaves < 1 || n_leaves > AETH_TAPE_MAX_LEAVES) fail("aeth_tape_tune", "1..16 leaves");
-        const size_t es = dtype == AETH_F64 ? 8 : 4;
-        for (int p = 0; p < n_programs; ++p) {
-            validate(programs[p], n_leaves, n_consts, n_rngs);
-            for (int q = 0; q < n_programs; ++q) {
-                const auto& Q = programs[q];
-                const char* out = static_cast<const char*>(leaves[Q.out_leaf]) + (uint64_t)Q.out_comp * leaf_pitch[Q.out_leaf] * es;
-                for (int k = 0; k < programs[p].n_loads; ++k) {
-                    const auto& P = programs[p];
-                    const char* in = static_cast<const char*>(leaves[P.load_leaf[k]]) +
-                                     (uint64_t)P.load_comp[k] * leaf_pitch[P.load_leaf[k]] * es;
-                    if (in == out) fail("aeth_tape_tune", "the destination is also read: tune an assignment into another array");
-                }
-            }
-        }
-        hipEvent_t e0, e1;
-        check(hipEventCreate(&e0), "hipEventCreate");
-        check(hipEventCreate(&e1), "hipEventCreate");
-        JitShape best{};
-        float best_ms = -1.0f;
-        try {
-            for (const JitShape& sh : kShapes) {
-                auto once = [&] {
-                    eval_tape(dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs,
-                              rngs, n, AETH_SUM_NONE, nullptr, stream, &sh);
-                };
-                once(); // compiles and warms the variant
-                constexpr int kReps = 3;
-                check(hipEventRecord(e0, st), "hipEventRecord");
-                for (int r = 0; r < kReps; ++r) once();
-                check(hipEventRecord(e1, st), "hipEventRecord");
-                check(hipEventSynchronize(e1), "hipEventSynchronize");
-                float ms = 0.0f;
-                check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-                ms /= kReps;
-                if (best_ms < 0.0f || ms < best_ms) {
-                    best_ms = ms;
-                    best = sh;
-                }
-            }
-        } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            throw;
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        int dev = 0;
-        check(hipGetDevice(&dev), "hipGetDevice");
-        uint64_t span_bytes = 0;
-        for (int p = 0; p < n_programs; ++p) span_bytes += (uint64_t)(programs[p].n_loads + 1) * n * es;
-        const bool nt = span_bytes >= (512ull << 20);
-        {
-            std::lock_guard<std::mutex> lock(g_jit_mu);
-            g_tuned[tape_key(dtype, AETH_SUM_NONE, n_programs, programs, nt, dev)] = best;
-        }
-        if (chosen) {
-            chosen->tile = best.tile;
-            chosen->vector_bytes = best.vb;
-            chosen->waves_per_simd = best.waves;
-            chosen->ms = best_ms;
-        }
+        tune_tape("aeth_tape_tune", dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts,
+                  n_rngs, rngs, n, AETH_SUM_NONE, nullptr, stream, kShapes, std::size(kShapes), chosen);
+    });
+}
+
+extern "C" int aeth_tape_tune_sums(int dtype, int n_programs, const aeth_program* programs, int n_leaves,
+                                   void* const* leaves, const uint64_t* leaf_pitch, int n_consts,
+                                   const double* consts, int n_rngs, const aeth_rng_leaf* rngs, uint64_t n,
+                                   int sum_mode, void* sums, void* stream, aeth_tape_shape* chosen)
+{
+    return guarded([&] {
+        if (sum_mode == AETH_SUM_NONE) fail("aeth_tape_tune_sums", "a sum mode is required (aeth_tape_tune tunes without)");
+        tune_tape("aeth_tape_tune_sums", dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts,
+                  n_rngs, rngs, n, sum_mode, sums, stream, kSumShapes, std::size(kSumShapes), chosen);
     });
 }
 

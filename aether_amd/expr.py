@@ -505,15 +505,21 @@ def _abi_args(dest: Array, expr, unbounded: bool = False):
     return progs, len(leaves), ptrs, pitch, len(consts), cst, len(rngs), rg, keep
 
 
-def tune(dest: Array, expr, stream=None) -> dict:
+def tune(dest: Array, expr, stream=None, sum_mode: int = -1) -> dict:
     """Autotune the launch shape of `dest = expr` (aeth_tape_tune): every candidate shape (samples per tile,
-    bytes per vector access, minimum waves per SIMD) runs on these arrays, the fastest is kept for this expression's structure
-    (assignments without sums), and `dest` holds the result. Returns {"tile", "vector_bytes",
-    "waves_per_simd", "ms"}."""
+    bytes per vector access, minimum waves per SIMD) runs on these arrays, the fastest is kept for this
+    expression's structure, and `dest` holds the result. With a sum_mode (aeth_tape_tune_sums) only the
+    occupancy target varies, the sums' shape stays, and later assignments with that sum mode use the pick.
+    Returns {"tile", "vector_bytes", "waves_per_simd", "ms"}."""
     progs, nl, ptrs, pitch, nc, cst, nr, rg, _ = _abi_args(dest, expr)
     out = aeth_tape_shape()
-    call("aeth_tape_tune", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg, dest.n,
-         stream_handle(stream), C.byref(out))
+    if sum_mode >= 0:
+        sums = torch.empty((dest.ncomp, 2), dtype=dest.dtype, device=dest.data.device)
+        call("aeth_tape_tune_sums", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg,
+             dest.n, sum_mode, sums.data_ptr(), stream_handle(stream), C.byref(out))
+    else:
+        call("aeth_tape_tune", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg, dest.n,
+             stream_handle(stream), C.byref(out))
     return {"tile": out.tile, "vector_bytes": out.vector_bytes, "waves_per_simd": out.waves_per_simd, "ms": out.ms}
 
 

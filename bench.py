@@ -198,6 +198,13 @@ def wl_tape(a, b, c, d, n) -> dict:
     ms1 = _time(lambda: E.assign(d, x))
     out["no_sums"] = {"contract_shape_ms": round(ms0, 4), "tuned_shape": [shape["tile"], shape["vector_bytes"], shape["waves_per_simd"]],
                       "tuned_ms": round(ms1, 4), "tuned_GB/s": _gbs(3 * n * 32, ms1)}
+    # with sums the tile and access width stay (they define the sums' bits); only the occupancy target is tuned
+    ms2 = _time(lambda: E.assign(d, x, ae.SUM_NEUMAIER, sums))
+    sshape = E.tune(d, x, sum_mode=ae.SUM_NEUMAIER)
+    ms3 = _time(lambda: E.assign(d, x, ae.SUM_NEUMAIER, sums))
+    out["with_sums"] = {"contract_shape_ms": round(ms2, 4),
+                        "tuned_shape": [sshape["tile"], sshape["vector_bytes"], sshape["waves_per_simd"]],
+                        "tuned_ms": round(ms3, 4), "tuned_GB/s": _gbs(3 * n * 32, ms3)}
     return out
 
 

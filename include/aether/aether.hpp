@@ -976,19 +976,36 @@ public:
         detail::check(rc, "aeth_eval_tape");
     }
 
-    /** Autotune the launch shape of dest = e (aeth_tape_tune); this view holds the result afterwards. */
+    /** Autotune the launch shape of dest = e (aeth_tape_tune); this view holds the result afterwards. With a
+     *  sum mode (aeth_tape_tune_sums) only the occupancy target varies and later evaluate() calls of the
+     *  structure with that mode use the pick. */
     template<class E>
-    aeth_tape_shape tune(const E& e, Stream stream = nullptr)
+    aeth_tape_shape tune(const E& e, Stream stream = nullptr, SumMode mode = SumMode::None)
     {
         static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
         if (!device_) throw Error("aether: tuning needs a device view");
         const detail::TapeCtx x = detail::lower<T, N>(p_, pitch_, e);
         if (x.programs.empty()) throw Error("aether: tuning takes an expression that fits one launch");
         aeth_tape_shape chosen{};
-        detail::check(aeth_tape_tune(x.dtype, (int)N, x.programs.data(), (int)x.leaves.size(),
-                                     const_cast<void* const*>(x.leaves.data()), x.pitch.data(), (int)x.consts.size(),
-                                     x.consts.data(), (int)x.rngs.size(), x.rngs.data(), samples_, stream, &chosen),
-                      "aeth_tape_tune");
+        if (mode == SumMode::None) {
+            detail::check(aeth_tape_tune(x.dtype, (int)N, x.programs.data(), (int)x.leaves.size(),
+                                         const_cast<void* const*>(x.leaves.data()), x.pitch.data(),
+                                         (int)x.consts.size(), x.consts.data(), (int)x.rngs.size(), x.rngs.data(),
+                                         samples_, stream, &chosen),
+                          "aeth_tape_tune");
+            return chosen;
+        }
+        void* sums_dev = nullptr;
+        void* chunk = nullptr;
+        const int dev = detail::current_device();
+        detail::check(detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, 2 * N * sizeof(T), stream, &chunk, &sums_dev),
+                      "chunk_alloc");
+        const int rc = aeth_tape_tune_sums(x.dtype, (int)N, x.programs.data(), (int)x.leaves.size(),
+                                           const_cast<void* const*>(x.leaves.data()), x.pitch.data(),
+                                           (int)x.consts.size(), x.consts.data(), (int)x.rngs.size(), x.rngs.data(),
+                                           samples_, (int)mode, sums_dev, stream, &chosen);
+        detail::seam().chunk_release(chunk);
+        detail::check(rc, "aeth_tape_tune_sums");
         return chosen;
     }
 
@@ -1158,9 +1175,9 @@ std::array<std::array<T, 2>, N> evaluate_pairs(View<T, N> dest, const E& e, SumM
 /** Pick the fastest launch shape for this expression's structure on this device (no counterpart in the
  *  reference); later assignments of the same structure without sums use it. dest holds the result. */
 template<class T, std::size_t N, class E>
-aeth_tape_shape tune(View<T, N> dest, const E& e, Stream s = nullptr)
+aeth_tape_shape tune(View<T, N> dest, const E& e, Stream s = nullptr, SumMode mode = SumMode::None)
 {
-    return dest.tune(e, s);
+    return dest.tune(e, s, mode);
 }
 
 // ---------------------------------------------------------------- accumulation

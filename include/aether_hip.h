@@ -235,7 +235,7 @@ int aeth_tape_jit_build(int dtype, int sum_mode, int n_programs, const aeth_prog
  * 8) -- on `stream`, on the caller's arrays, and keeps the fastest for this
  * tape structure on this device for the rest of the process: later aeth_eval_tape calls of the same
  * structure without sums launch with it (calls with sums keep the 2048-sample, 16-byte shape, whose
- * reduction order the sums are defined by). The destination holds the assignment's result afterwards.
+ * reduction order the sums are defined by; aeth_tape_tune_sums below tunes their occupancy target). The destination holds the assignment's result afterwards.
  * Refused inside a stream capture and for an assignment whose destination is also one of its inputs.
  * Synchronises `stream`. chosen (may be NULL): the shape picked and its average time per launch. */
 typedef struct {
@@ -247,6 +247,15 @@ typedef struct {
 int aeth_tape_tune(int dtype, int n_programs, const aeth_program* programs, int n_leaves, void* const* leaves,
                    const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs,
                    const aeth_rng_leaf* rngs, uint64_t n, void* stream, aeth_tape_shape* chosen);
+/* The same for an assignment with sums (sum_mode != AETH_SUM_NONE; the arguments of aeth_eval_tape):
+ * the 2048-sample, 16-byte shape stays (it defines the sums' bits) and only the register allocator's
+ * occupancy target varies (none, 4 or 8 waves per SIMD), which changes no operation. Later calls of the
+ * structure with that sum mode launch with the pick. `sums` receives the assignment's sums. No
+ * counterpart in the reference. */
+int aeth_tape_tune_sums(int dtype, int n_programs, const aeth_program* programs, int n_leaves,
+                        void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts,
+                        int n_rngs, const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* sums, void* stream,
+                        aeth_tape_shape* chosen);
 
 /* e = cross(a, b).dot(c) + sqrt(a.dot(a) + 1) over Array<double,3> a, b, c
  * (pitch `pitch`) into a scalar plane e (expr/nodes/Geometric.h:32-73,

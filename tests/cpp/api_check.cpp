@@ -25,6 +25,8 @@
 //                                eps * sum|t| per slot (text)
 //   runtime <rows> <cols>      : runtimeEval AddScaled of a transposed and a dense view, and an
 //                                Assign cast to float, checked on the host (text)
+//   tune   <n> <seed>          : aether::tune with Neumaier sums and without, then the assignments:
+//                                values and sums equal the untuned evaluation's (text)
 //   ext    <n> <seed>          : q, r (quaternions) and a (3-vector) from Generator(seed) on the
 //                                device; quaternion, fold, norm, min/max/clamp and slice members
 //                                on the device, then the same per sample on the host; writes q, r,
@@ -154,6 +156,42 @@ int cfg0g(std::size_t n, std::uint64_t seed)
     write_planes(c);
     write_planes(d);
     std::fwrite(sums.data(), sizeof(double), 3, stdout);
+    return 0;
+}
+
+// aether::tune with and without sums, then the same assignments: values and sums bit-identical to the
+// untuned evaluation (text: "tune ok <tile> <bytes> <waves>" per form, or the first mismatch)
+int tune(std::size_t n, std::uint64_t seed)
+{
+    Array<double, 3> a(n), b(n), c(n), d1(n), d2(n), d3(n);
+    fill_generator(a, b, c, seed);
+    const auto e = cfg0(a.deviceView(), b.deviceView(), c.deviceView());
+    const auto s1 = aether::evaluate(d1.deviceView(), e, aether::SumMode::Neumaier);
+    const aeth_tape_shape ts = aether::tune(d2.deviceView(), e, nullptr, aether::SumMode::Neumaier);
+    const auto s2 = aether::evaluate(d2.deviceView(), e, aether::SumMode::Neumaier);
+    const aeth_tape_shape tn = aether::tune(d3.deviceView(), e);
+    d3.deviceView() = e;
+    d1.download();
+    d2.download();
+    d3.download();
+    auto v1 = d1.hostView(), v2 = d2.hostView(), v3 = d3.hostView();
+    for (std::size_t k = 0; k < 3; ++k)
+        if (s1[k] != s2[k]) {
+            std::printf("sum %zu differs\n", k);
+            return 1;
+        }
+    for (std::size_t comp = 0; comp < 3; ++comp)
+        for (std::size_t i = 0; i < n; ++i)
+            if (v1(comp, i) != v2(comp, i) || v1(comp, i) != v3(comp, i)) {
+                std::printf("value (%zu, %zu) differs\n", comp, i);
+                return 1;
+            }
+    if (ts.tile != 2048 || ts.vector_bytes != 16) {
+        std::printf("tuned with sums at %u x %u\n", ts.tile, ts.vector_bytes);
+        return 1;
+    }
+    std::printf("tune ok %u %u %u\ntune ok %u %u %u\n", ts.tile, ts.vector_bytes, ts.waves_per_simd, tn.tile,
+                tn.vector_bytes, tn.waves_per_simd);
     return 0;
 }
 
@@ -495,6 +533,7 @@ int main(int argc, char** argv)
         if (mode == "runtime" && argc == 4) return runtime(u(2), u(3));
         if (mode == "resize" && argc == 3) return resize(u(2));
         if (mode == "ext" && argc == 4) return ext(u(2), u(3));
+        if (mode == "tune" && argc == 4) return tune(u(2), u(3));
     } catch (const aether::Error& e) {
         std::fprintf(stderr, "aether error: %s\n", e.what());
         return 1;

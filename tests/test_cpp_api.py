@@ -201,3 +201,9 @@ def test_keyed_sum_compensated_within_one_eps(dev):
 @pytest.mark.gpu
 def test_runtime_eval_strided_views(dev):
     assert run("runtime", 123, 77).decode().startswith("runtime ok")
+
+
+@pytest.mark.gpu
+def test_tune_with_and_without_sums_keeps_the_bits(dev):
+    out = run("tune", (1 << 21) + 333, 9).decode().split("\n")
+    assert out[0].startswith("tune ok 2048 16 ") and out[1].startswith("tune ok"), out

@@ -2,7 +2,8 @@
 
 Tuning runs every candidate shape on the caller's arrays; whatever shape it keeps, an assignment without sums
 gives the same values as before tuning (bit for bit), the destination of the tuning call holds the result,
-and assignments with sums keep the contract shape (their sums equal the specialised kernel's). Tuning an
+and assignments with sums keep the contract shape (their sums equal the specialised kernel's), also after
+tuning their occupancy target (aeth_tape_tune_sums). Tuning an
 assignment that reads its own destination, or inside a stream capture, is refused by name."""
 import pytest
 import torch
@@ -49,6 +50,25 @@ def test_sums_keep_the_contract_shape_after_tuning(dev):
     torch.cuda.synchronize()
     assert torch.equal(s1, s2)
     assert torch.equal(d1.data[:, :n], d2.data[:, :n])
+
+
+@pytest.mark.parametrize("mode", [ae.SUM_NEUMAIER, ae.SUM_PLAIN], ids=["neumaier", "plain"])
+def test_tuning_with_sums_keeps_the_bits(dev, mode):
+    """With sums only the occupancy target varies: after tuning, the assignment's values and sums equal
+    the specialised kernel's (the contract shape), and the pick has the contract tile and width."""
+    n = (1 << 22) + 1000
+    a, b, c = filled(n, torch.float64, (8, 9, 10))
+    x = a + b * c + E.sin(a) * E.exp(-1.0 * (b * b))
+    d1, d2 = ae.Array(n, 3, torch.float64), ae.Array(n, 3, torch.float64)
+    got = E.tune(d2, x, sum_mode=mode)
+    assert (got["tile"], got["vector_bytes"]) == (2048, 16) and got["waves_per_simd"] in (0, 4, 8) and got["ms"] > 0
+    s1 = ae.eval_cfg0(a, b, c, d1, mode)
+    s2 = E.assign(d2, x, mode)
+    torch.cuda.synchronize()
+    assert torch.equal(s1, s2)
+    assert torch.equal(d1.data[:, :n], d2.data[:, :n])
+    with pytest.raises(ae.AetherError, match="destination is also read"):
+        E.tune(a, a * 2.0 + b, sum_mode=mode)
 
 
 def test_tuning_refusals(dev):
