@@ -473,11 +473,32 @@ struct Slot {
     uint64_t* region_offset = nullptr;       // device copy of the layout's offsets
     double* host_res = nullptr;
     double* dev_res = nullptr;
-    hipGraphExec_t graph = nullptr;          // the batch as one captured graph (graph mode)
+    hipGraphExec_t graph[4] = {};            // the batch as one captured graph per persistent-grid candidate
+    hipEvent_t t0 = nullptr, t1 = nullptr;   // timing of a batch the tuner measures
+    int cand = -1;                           // the candidate the in-flight batch was timed with (-1: untimed)
+    double work = 0;                         // its algorithmic bytes
     std::vector<Job> jobs;
     bool busy = false;
     double launch_t = 0;
 };
+
+// The persistent grids' candidates of the graph form: blocks of the batch kernels per CU. Every candidate
+// gives the same bits (grid-stride kernels over position-independent units); the tuner keeps the fastest.
+constexpr uint32_t kUnitsPerCu[4] = {2, 4, 8, 16};
+constexpr int kDefaultCand = 2; // 8 blocks a CU
+constexpr int kTuneSamples = 8; // timed batches per candidate before the pick
+
+// Algorithmic bytes of a job in the precision it executes in (eval: 3 loads + 1 store per (component,
+// sample); sampling: values written; FFT: each point read and written once).
+double job_work(const aeth_job& j)
+{
+    const double es = (double)esize(j.dtype);
+    if (j.kind == AETH_JOB_EVAL) return 12.0 * (double)j.n * es;
+    if (j.kind == AETH_JOB_SAMPLE) return (double)j.n * es;
+    double pts = 0;
+    for (uint64_t k = 0; k < j.n; ++k) pts += (double)(1ull << j.fft_log2n[k]);
+    return 4.0 * pts * es;
+}
 
 template<class T>
 void put(const Layout& L, unsigned char* base, int region, const std::vector<T>& v, BatchHeader& h)
@@ -681,13 +702,13 @@ void enqueue_batch(Slot& S, const Layout& L, void* base, const BatchHeader* h, u
     check(hipMemcpyAsync(S.host_res, S.dev_res, rb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(results)");
 }
 
-// Capture the persistent form of a batch on the slot's stream into a graph, once.
-void capture_batch_graph(Slot& S, const Layout& L, void* base, uint32_t max_jobs)
+// Capture the persistent form of a batch on the slot's stream into a graph (once per grid candidate).
+hipGraphExec_t capture_batch_graph(Slot& S, const Layout& L, void* base, uint32_t max_jobs, uint32_t units_per_cu)
 {
     // everything the launches look up is ready before the capture: twiddle tables, grid sizes
     fft::prepare(S.stream);
     PersistentGrids pg;
-    pg.units = device_cus() * 8;
+    pg.units = device_cus() * (int)units_per_cu;
     pg.fft[0] = fft::persistent_grid(0);
     pg.fft[1] = fft::persistent_grid(2);
     check(hipStreamSynchronize(S.stream), "hipStreamSynchronize");
@@ -701,17 +722,25 @@ void capture_batch_graph(Slot& S, const Layout& L, void* base, uint32_t max_jobs
         throw;
     }
     check(hipStreamEndCapture(S.stream, &g), "hipStreamEndCapture");
-    const hipError_t e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     check(e, "hipGraphInstantiate");
+    return exec;
 }
 
-// Launch every job of the slot as one batch on the slot's stream.
-void launch_batch(Slot& S, const Layout& L, void* base, bool graph)
+// Launch every job of the slot as one batch on the slot's stream; in graph form, with the graph of grid
+// candidate `cand`, timed between S.t0 and S.t1 when `timed`.
+void launch_batch(Slot& S, const Layout& L, void* base, bool graph, int cand, bool timed)
 {
     const BatchHeader h = build_batch(S, L, base);
-    if (graph) check(hipGraphLaunch(S.graph, S.stream), "hipGraphLaunch");
-    else enqueue_batch(S, L, base, &h, 0, PersistentGrids{});
+    if (graph) {
+        if (timed) check(hipEventRecord(S.t0, S.stream), "hipEventRecord");
+        check(hipGraphLaunch(S.graph[cand], S.stream), "hipGraphLaunch");
+        if (timed) check(hipEventRecord(S.t1, S.stream), "hipEventRecord");
+    } else {
+        enqueue_batch(S, L, base, &h, 0, PersistentGrids{});
+    }
     check(hipEventRecord(S.done, S.stream), "hipEventRecord");
 }
 
@@ -719,6 +748,12 @@ struct Server {
     aeth_server_config cfg;
     Layout layout{};
     bool graph = false;
+    // online tuning of the graph form's persistent grids (AETH_SERVER_TUNE): timed batches rotate over the
+    // candidates until each has kTuneSamples, then the lowest median time per byte is kept (under mu)
+    bool tune = false;
+    bool settled = false;
+    int chosen = kDefaultCand;
+    std::vector<float> samples[4];
     int dev = 0;
     std::unique_ptr<Pool> pool;
     std::vector<Slot> slots;
@@ -734,8 +769,45 @@ struct Server {
     aeth_server_stats stats{};
     std::string fatal;
 
+    void record_timing(Slot& S)
+    {
+        if (S.cand < 0) return;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, S.t0, S.t1) == hipSuccess && S.work > 0) samples[S.cand].push_back(ms / (float)(S.work * 1e-9));
+        S.cand = -1;
+        for (const auto& v : samples)
+            if ((int)v.size() < kTuneSamples) return;
+        int best = kDefaultCand;
+        float best_med = -1.0f;
+        for (int c = 0; c < 4; ++c) {
+            std::vector<float> v = samples[c];
+            std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+            const float med = v[v.size() / 2];
+            if (best_med < 0.0f || med < best_med) {
+                best_med = med;
+                best = c;
+            }
+        }
+        std::lock_guard<std::mutex> lock(mu);
+        chosen = best;
+        settled = true;
+    }
+
+    // the grid candidate of the next batch: the chosen one, or while tuning the one with fewest samples
+    int next_candidate(bool& timed) const
+    {
+        timed = false;
+        if (!graph || !tune || settled) return chosen;
+        int c = 0;
+        for (int k = 1; k < 4; ++k)
+            if (samples[k].size() < samples[c].size()) c = k;
+        timed = true;
+        return c;
+    }
+
     void complete(Slot& S)
     {
+        record_timing(S);
         const double t = now_s();
         std::vector<aeth_job_result> out;
         for (size_t j = 0; j < S.jobs.size(); ++j) {
@@ -821,7 +893,12 @@ struct Server {
                 }
                 if (S.jobs.empty()) break; // pool full: wait for completions
                 try {
-                    launch_batch(S, layout, pool->base(), graph);
+                    bool timed = false;
+                    const int cand = next_candidate(timed);
+                    S.work = 0;
+                    for (const auto& J : S.jobs) S.work += job_work(J.spec);
+                    launch_batch(S, layout, pool->base(), graph, cand, timed);
+                    S.cand = timed ? cand : -1;
                 } catch (const Failure&) {
                     fatal = last_error();
                     for (auto& J : S.jobs) {
@@ -859,6 +936,7 @@ extern "C" int aeth_server_create(const aeth_server_config* cfg, void** server)
         check(hipGetDevice(&S->dev), "hipGetDevice");
         S->pool = std::make_unique<Pool>(cfg->hbm_budget_bytes);
         S->graph = (S->cfg.flags & AETH_SERVER_GRAPH_BATCHES) != 0;
+        S->tune = S->graph && (S->cfg.flags & AETH_SERVER_TUNE) != 0;
         S->layout = make_layout(S->cfg.max_batch_jobs);
         const uint64_t stage = S->layout.total;
         S->slots.resize(S->cfg.streams);
@@ -876,7 +954,15 @@ extern "C" int aeth_server_create(const aeth_server_config* cfg, void** server)
             const size_t rb = 8 * sizeof(double) * S->cfg.max_batch_jobs;
             check(hipHostMalloc(reinterpret_cast<void**>(&sl.host_res), rb, hipHostMallocDefault), "hipHostMalloc(res)");
             check(hipMalloc(reinterpret_cast<void**>(&sl.dev_res), rb), "hipMalloc(res)");
-            if (S->graph) capture_batch_graph(sl, S->layout, S->pool->base(), S->cfg.max_batch_jobs);
+            if (S->tune) {
+                check(hipEventCreate(&sl.t0), "hipEventCreate");
+                check(hipEventCreate(&sl.t1), "hipEventCreate");
+                for (int c = 0; c < 4; ++c)
+                    sl.graph[c] = capture_batch_graph(sl, S->layout, S->pool->base(), S->cfg.max_batch_jobs, kUnitsPerCu[c]);
+            } else if (S->graph) {
+                sl.graph[kDefaultCand] = capture_batch_graph(sl, S->layout, S->pool->base(), S->cfg.max_batch_jobs,
+                                                             kUnitsPerCu[kDefaultCand]);
+            }
         }
         Server* raw = S.get();
         S->worker = std::thread([raw] { raw->loop(); });
@@ -940,6 +1026,17 @@ extern "C" int aeth_server_get_stats(void* server, aeth_server_stats* st)
     });
 }
 
+extern "C" int aeth_server_tuning(void* server, uint32_t* blocks_per_cu, uint32_t* settled)
+{
+    return guarded([&] {
+        auto* S = static_cast<Server*>(server);
+        if (!S || !blocks_per_cu || !settled) fail("aeth_server_tuning", "NULL argument");
+        std::lock_guard<std::mutex> lock(S->mu);
+        *blocks_per_cu = kUnitsPerCu[S->chosen];
+        *settled = S->settled ? 1u : 0u;
+    });
+}
+
 extern "C" int aeth_server_destroy(void* server)
 {
     return guarded([&] {
@@ -957,7 +1054,10 @@ extern "C" int aeth_server_destroy(void* server)
             (void)hipHostFree(sl.host_res);
             (void)hipFree(sl.dev_res);
             (void)hipFree(sl.region_offset);
-            if (sl.graph) (void)hipGraphExecDestroy(sl.graph);
+            for (auto& g : sl.graph)
+                if (g) (void)hipGraphExecDestroy(g);
+            if (sl.t0) (void)hipEventDestroy(sl.t0);
+            if (sl.t1) (void)hipEventDestroy(sl.t1);
         }
         delete S;
     });

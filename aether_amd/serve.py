@@ -15,6 +15,7 @@ from ._lib import call, lib
 
 JOB_EVAL, JOB_SAMPLE, JOB_FFT = 0, 1, 2
 SERVER_GRAPH_BATCHES = 1
+SERVER_TUNE = 2  # with SERVER_GRAPH_BATCHES: tune the graph form's persistent grids online
 F64, F32 = 0, 1
 
 
@@ -46,6 +47,7 @@ def _bind():
         "aeth_server_poll": [C.c_void_p, C.POINTER(JobResult), C.c_uint32, C.POINTER(C.c_uint32)],
         "aeth_server_wait_idle": [C.c_void_p],
         "aeth_server_get_stats": [C.c_void_p, C.POINTER(Stats)],
+        "aeth_server_tuning": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
         "aeth_server_destroy": [C.c_void_p],
         "aeth_server_replay": [C.c_void_p, C.POINTER(Job), C.c_uint64, C.POINTER(JobResult)],
         "aeth_run_job_plain": [C.POINTER(Job), C.POINTER(JobResult), C.c_void_p],
@@ -109,10 +111,10 @@ def job_bytes(j: Job, fp32_compensated: bool = False) -> float:
 
 class Server:
     def __init__(self, budget_bytes: int = 64 << 30, streams: int = 4, max_batch_jobs: int = 256,
-                 fp32_compensated: bool = False, graph_batches: bool = False):
+                 fp32_compensated: bool = False, graph_batches: bool = False, tune: bool = False):
         self.L = _bind()
-        cfg = Config(budget_bytes, streams, max_batch_jobs, int(fp32_compensated),
-                     SERVER_GRAPH_BATCHES if graph_batches else 0)
+        flags = (SERVER_GRAPH_BATCHES if graph_batches else 0) | (SERVER_TUNE if tune else 0)
+        cfg = Config(budget_bytes, streams, max_batch_jobs, int(fp32_compensated), flags)
         h = C.c_void_p()
         call("aeth_server_create", C.byref(cfg), C.byref(h))
         self.h = h
@@ -140,6 +142,12 @@ class Server:
         s = Stats()
         call("aeth_server_get_stats", self.h, C.byref(s))
         return s
+
+    def tuning(self) -> dict:
+        """The graph form's persistent grid in blocks per CU, and whether the online tuner has settled."""
+        b, done = C.c_uint32(), C.c_uint32()
+        call("aeth_server_tuning", self.h, C.byref(b), C.byref(done))
+        return {"blocks_per_cu": b.value, "settled": bool(done.value)}
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:

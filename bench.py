@@ -673,6 +673,14 @@ def wl_serve(duration: float, rate: float = 20000.0, plain_seconds: float = 4.0)
     with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True,
                   graph_batches=True) as srv:
         summarize(srv.replay(cap_jobs), "capacity_fp32_compensated_closed_loop_graph_batches", cap_nb[True])
+    # ... with the online tuner of the persistent grids: the stream's first batches rotate over 2, 4, 8 and
+    # 16 blocks per CU, then the pick serves the rest; a second pass runs at the settled pick
+    with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True,
+                  graph_batches=True, tune=True) as srv:
+        srv.replay(cap_jobs)
+        tuned = srv.tuning()
+        summarize(srv.replay(cap_jobs), "capacity_fp32_compensated_closed_loop_graph_batches_tuned", cap_nb[True])
+        out["capacity_fp32_compensated_closed_loop_graph_batches_tuned"]["tuning"] = tuned
     with S.Server(budget_bytes=64 << 30, streams=4, max_batch_jobs=256, fp32_compensated=True,
                   graph_batches=True) as srv:
         summarize(srv.replay(jobs), "served_fp32_compensated_graph_batches", nb[True])

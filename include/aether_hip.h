@@ -339,7 +339,7 @@ typedef struct {
     uint32_t streams;        /* batches in flight, 1..4 */
     uint32_t max_batch_jobs; /* coalescing cap per batch */
     uint32_t fp32_compensated;
-    uint32_t flags;          /* AETH_SERVER_GRAPH_BATCHES: see below */
+    uint32_t flags;          /* AETH_SERVER_GRAPH_BATCHES, AETH_SERVER_TUNE: see below */
 } aeth_server_config;
 /* flags: AETH_SERVER_GRAPH_BATCHES -- every batch of a stream slot is ONE
  * replay of a graph captured when the server is created: a staging kernel
@@ -347,8 +347,14 @@ typedef struct {
  * persistent kernels (fixed grids, descriptor counts read on the device) for
  * the fills, fused evaluations, FFTs, reductions and trees, and the copy of
  * the results. The same kernels' per-unit bodies run either way, so results
- * are bit for bit those of direct launches. */
-enum { AETH_SERVER_GRAPH_BATCHES = 1 };
+ * are bit for bit those of direct launches.
+ * AETH_SERVER_TUNE (with AETH_SERVER_GRAPH_BATCHES) -- the autotuner of the
+ * batched launch: each slot's graph is captured for persistent grids of 2,
+ * 4, 8 and 16 blocks per CU; the first batches rotate over them, timed with
+ * events, and once each has 8 timed batches the lowest median time per
+ * algorithmic byte is kept for the rest of the server's life. Every grid
+ * gives the same bits. Without the flag, 8 blocks per CU. */
+enum { AETH_SERVER_GRAPH_BATCHES = 1, AETH_SERVER_TUNE = 2 };
 typedef struct {
     uint64_t submitted, completed, failed, batches;
     uint64_t pool_bytes_in_use, pool_bytes_budget;
@@ -358,6 +364,10 @@ int aeth_server_submit(void* server, const aeth_job* job, uint64_t* id);
 int aeth_server_poll(void* server, aeth_job_result* out, uint32_t max, uint32_t* count);
 int aeth_server_wait_idle(void* server);
 int aeth_server_get_stats(void* server, aeth_server_stats* stats);
+/* The graph form's persistent grid in use (blocks per CU) and whether the
+ * tuner has settled (AETH_SERVER_TUNE; without it: 8 and 0). No counterpart
+ * in the reference. */
+int aeth_server_tuning(void* server, uint32_t* blocks_per_cu, uint32_t* settled);
 int aeth_server_destroy(void* server);
 /* Open-loop replay: submits jobs[i] at jobs[i].arrival seconds after the call
  * starts, waits for all, results[i] is job i's result. */

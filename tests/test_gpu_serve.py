@@ -68,6 +68,28 @@ def test_graph_batches_equal_direct_launches(dev, fp32):
         assert list(a.value) == list(b.value), (j.kind, j.dtype, j.n)
 
 
+def test_tuned_graph_batches_settle_and_equal_direct_launches(dev):
+    """The graph form's online tuner (AETH_SERVER_TUNE) times batches over four persistent grids, settles on
+    one of them after 8 timed batches each, and every job keeps the bits of the direct launches."""
+    jobs = jobs_small_mix(count=120, seed=13)
+    results = {}
+    for tune in (False, True):
+        with S.Server(budget_bytes=4 << 30, streams=2, max_batch_jobs=3, graph_batches=tune, tune=tune) as srv:
+            ids = [srv.submit(j) for j in jobs]
+            srv.wait_idle()
+            got = {r.id: r for r in srv.poll()}
+            st, tn = srv.stats(), srv.tuning()
+        assert st.completed == len(jobs) and st.failed == 0
+        results[tune] = [got[i] for i in ids]
+        if tune:
+            assert st.batches >= 32 and tn["settled"] and tn["blocks_per_cu"] in (2, 4, 8, 16), (st.batches, tn)
+        else:
+            assert tn == {"blocks_per_cu": 8, "settled": False}
+    for j, a, b in zip(jobs, results[False], results[True]):
+        assert a.status == 0 and b.status == 0
+        assert list(a.value) == list(b.value), (j.kind, j.dtype, j.n)
+
+
 def test_eval_job_matches_oracle(dev, orc):
     n, seed = 10_000, 77
     r = S.run_plain(S.make_job(S.JOB_EVAL, n, seed, S.F64))
