@@ -35,7 +35,7 @@ public:
 #if defined(__HIP_DEVICE_COMPILE__)
         atomicOr(word_, 1u);
 #else
-        reinterpret_cast<std::atomic<uint32_t>*>(word_)->store(1u, std::memory_order_relaxed);
+        std::atomic_ref<uint32_t>(*word_).store(1u, std::memory_order_relaxed);
 #endif
     }
 
