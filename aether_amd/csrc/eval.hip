@@ -35,7 +35,9 @@ __global__ __launch_bounds__(kThreads) void cfg0_kernel(const T* __restrict__ a,
                                                         acc::Pair<T>* __restrict__ part)
 {
     __shared__ acc::Pair<T> lds[4];
-    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+    constexpr uint32_t K = xcd_chunk(kTile * sizeof(T));
+    for (uint32_t v = blockIdx.x; v < units; v += gridDim.x) {
+        const uint32_t u = xcd_unit(v, units, K);
         const uint32_t comp = u / tiles;
         const uint32_t tile = u - comp * tiles;
         const acc::Pair<T> p = cfg0_tile<T, M, SUM, VECTOR, NT>(a, b, c, d, n, pitch, comp, tile);
@@ -53,7 +55,9 @@ __global__ __launch_bounds__(kThreads) void tile_reduce_kernel(const T* __restri
                                                                acc::Pair<T>* __restrict__ part)
 {
     __shared__ acc::Pair<T> lds[4];
-    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+    constexpr uint32_t K = xcd_chunk(kTile * sizeof(T));
+    for (uint32_t v = blockIdx.x; v < units; v += gridDim.x) {
+        const uint32_t u = xcd_unit(v, units, K);
         const uint32_t comp = u / tiles;
         const uint32_t tile = u - comp * tiles;
         const acc::Pair<T> p = reduce_tile<T, M, SQUARE, VECTOR, NT>(x, n, pitch, comp, tile);
@@ -304,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void vec3geo_kernel(const double* __restr
                                                            const double* __restrict__ c, double* __restrict__ e,
                                                            uint32_t n, uint64_t pitch)
 {
-    const uint64_t base = (uint64_t)blockIdx.x * kV3Tile;
+    const uint64_t base = (uint64_t)xcd_unit(blockIdx.x, gridDim.x, xcd_chunk(kV3Tile * sizeof(double))) * kV3Tile;
     if (VECTOR && base + kV3Tile <= n) {
         double A[kV3Rounds][3][2], B[kV3Rounds][3][2], C[kV3Rounds][3][2];
 #pragma unroll

@@ -257,7 +257,9 @@ __global__ __launch_bounds__(kThreads) void tape_kernel(const TapeArgs A, acc::P
     constexpr int ROUNDS = kTile / (kThreads * VN);
     __shared__ acc::Pair<T> lds[4];
     const uint32_t n = A.n;
-    for (uint32_t u = blockIdx.x; u < A.units; u += gridDim.x) {
+    constexpr uint32_t K = units::xcd_chunk(kTile * sizeof(T));
+    for (uint32_t v = blockIdx.x; v < A.units; v += gridDim.x) {
+        const uint32_t u = units::xcd_unit(v, A.units, K);
         const uint32_t pi = u / A.tiles;
         const uint32_t tile = u - pi * A.tiles;
         const uint32_t s0 = tile * kTile;
@@ -695,7 +697,9 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
     s += "  constexpr int VN = " + std::to_string(sh.vb) + " / sizeof(T);\n  constexpr int ROUNDS = TILE / (256 * VN);\n";
     s += "  __shared__ aeth::acc::Pair<T> lds[4];\n";
     s += "  aeth::acc::Pair<T>* part = (aeth::acc::Pair<T>*)part_;\n  const uint32_t n = A.n;\n";
-    s += "  for (uint32_t u = blockIdx.x; u < A.units; u += gridDim.x) {\n";
+    s += "  constexpr uint32_t XK = aeth::units::xcd_chunk(TILE * sizeof(T));\n";
+    s += "  for (uint32_t v = blockIdx.x; v < A.units; v += gridDim.x) {\n";
+    s += "    const uint32_t u = aeth::units::xcd_unit(v, A.units, XK);\n";
     s += "    const uint32_t pi = u / A.tiles;\n    const uint32_t tile = u - pi * A.tiles;\n";
     s += "    const uint32_t s0 = tile * TILE;\n    const bool full = vec && aeth::units::full_tile(s0, TILE, n);\n";
     s += "    aeth::acc::Pair<T> acc{T(0), T(0)};\n    switch (pi) {\n";

@@ -1,4 +1,4 @@
-"""The full-tile predicate of the tiled kernels near the 32-bit sample limit.
+"""The full-tile predicate of the tiled kernels near the 32-bit sample limit, and the XCD-aware unit order.
 
 tests/cpp/tile_predicate.hip holds static_asserts over aeth::units::full_tile (the predicate
 that picks a tile's unpredicated 16-byte path) at n = 2^32-4 and n = 2^32-1, where the last tile's
