@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # at 3 x 2^27 samples: rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the same counters over a copy of known
 # size, measured separately (profiles/tape_cfg0_f64_2p27_hbm_traffic.md); PMC counters cannot be read from
 # inside a timed run, so this is a recorded figure, not one from this run.
-MEASURED_TRAFFIC_BYTES = {27: 12_890_979_775}
+MEASURED_TRAFFIC_BYTES = {27: 12_888_048_651}
 
 
 # secondary workloads that run on their own (no arguments)

@@ -96,6 +96,39 @@ def test_fill_patterns_and_errors(seam):
     seam.chunk_release(ch)
 
 
+# Byte counts around the bulk kernels' 16 KiB units: below one unit, whole units, units plus spare vectors
+# plus tail bytes, and regions past the streaming threshold (fill 512 MiB, copy 256 MiB).
+BULK_SIZES = [1, 15, 16, 4095, 16384, 16384 + 16, 3 * 16384 + 1008 + 7, (1 << 22) + 48, (520 << 20) + 5]
+
+
+@pytest.mark.parametrize("nbytes", BULK_SIZES)
+@pytest.mark.parametrize("offset", [0, 3, 8])
+def test_bulk_fill_and_copy_bytes(seam, nbytes, offset):
+    """Device fill (1-byte and 8-byte patterns) and same-device copy against torch's own result, byte for
+    byte; `offset` moves the start off the 16-byte grid (the fill's bytewise head, the copy's byte path)."""
+    s = torch.cuda.current_stream().cuda_stream
+    buf = torch.zeros(nbytes + 32, dtype=torch.uint8, device="cuda")
+    base = buf.data_ptr() + offset
+    b1 = C.c_uint8(0xA5)
+    assert seam.fill(DEVICE, 0, base, nbytes, C.byref(b1), 1, s) == 0, seam.last_error()
+    torch.cuda.synchronize()
+    assert torch.all(buf[offset:offset + nbytes] == 0xA5)
+    assert torch.all(buf[:offset] == 0) and torch.all(buf[offset + nbytes:] == 0)
+    if offset % 8 == 0 and nbytes % 8 == 0:
+        pat = (C.c_uint8 * 8)(*range(1, 9))
+        assert seam.fill(DEVICE, 0, base, nbytes, pat, 8, s) == 0, seam.last_error()
+        torch.cuda.synchronize()
+        want = torch.arange(1, 9, dtype=torch.uint8, device="cuda").repeat(nbytes // 8)
+        assert torch.equal(buf[offset:offset + nbytes], want)
+    src = torch.randint(0, 256, (nbytes + 32,), dtype=torch.uint8, device="cuda")
+    dst = torch.zeros_like(src)
+    assert seam.copy(DEVICE, 0, dst.data_ptr() + offset, DEVICE, 0, src.data_ptr() + offset, nbytes, s) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(dst[offset:offset + nbytes], src[offset:offset + nbytes])
+    assert torch.all(dst[:offset] == 0) and torch.all(dst[offset + nbytes:] == 0)
+    del buf, src, dst
+
+
 def test_fill_and_copy_in_graph_replays(seam):
     # capture-legal: replay several times with eager work between, every replay checked
     n = 1 << 16
