@@ -62,7 +62,10 @@ inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 
 int device_cus();
 
 // Stream-ordered scratch: allocated from the device's default memory pool on
-// `stream` and freed on it when the guard dies. No host synchronisation.
+// `stream` and freed on it when the guard dies. Legal inside a stream capture
+// (the graph owns the allocation). On this ROCm the pair holds the host until
+// the stream has drained, so calls that run outside a capture take a
+// StreamLease instead.
 class StreamScratch {
 public:
     StreamScratch(size_t bytes, hipStream_t stream) : stream_(stream)
@@ -80,6 +83,29 @@ public:
 private:
     void* ptr_ = nullptr;
     hipStream_t stream_;
+};
+
+// Scratch kept per (device, stream): one growable device buffer per stream handle, reused by every call
+// on that stream. Calls on one stream run in stream order, so one call's kernels never meet another's in
+// the buffer; the lease holds the slot's lock while the call enqueues its kernels, so two host threads
+// issuing to one stream take turns. A buffer that must grow is released and replaced in stream order.
+// Measured on MI355X: a stream-ordered allocation and free per call held the host for ~1.9 ms of a 2.1 ms
+// fused evaluation and left the GPU idle 7-16 us before the next call's kernel; a lease costs neither.
+// Inside a stream capture, or when this thread already holds the stream's slot, the lease falls back to a
+// StreamScratch. A stream destroyed with work still queued must not have its handle reused while that work
+// runs (the slot is keyed by the handle's value).
+class StreamLease {
+public:
+    StreamLease(size_t bytes, hipStream_t stream);
+    ~StreamLease();
+    StreamLease(const StreamLease&) = delete;
+    StreamLease& operator=(const StreamLease&) = delete;
+    void* get() const { return ptr_; }
+
+private:
+    void* ptr_ = nullptr;
+    void* slot_ = nullptr;
+    StreamScratch* fallback_ = nullptr;
 };
 
 // Queries and releases made while another thread or stream captures in global mode: the thread's

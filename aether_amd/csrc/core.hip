@@ -1,8 +1,12 @@
 // core.hip -- error state, device queries and library identity.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/aether_hip.h"
@@ -32,6 +36,67 @@ int device_cus()
         g_cus[dev] = cus > 0 ? cus : 1;
     }
     return g_cus[dev];
+}
+
+namespace {
+
+struct LeaseSlot {
+    std::mutex mu;
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+std::mutex g_slots_mu;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<LeaseSlot>> g_slots; // never erased: leases point in
+thread_local std::vector<LeaseSlot*> t_held;
+
+LeaseSlot* slot_for(int dev, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lock(g_slots_mu);
+    auto& p = g_slots[{dev, st}];
+    if (!p) p.reset(new LeaseSlot);
+    return p.get();
+}
+
+} // namespace
+
+StreamLease::StreamLease(size_t bytes, hipStream_t stream)
+{
+    if (bytes == 0) return;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    LeaseSlot* slot = capturing(stream) ? nullptr : slot_for(dev, stream);
+    if (!slot || std::find(t_held.begin(), t_held.end(), slot) != t_held.end()) {
+        fallback_ = new StreamScratch(bytes, stream);
+        ptr_ = fallback_->get();
+        return;
+    }
+    std::unique_lock<std::mutex> lock(slot->mu);
+    if (slot->bytes < bytes) {
+        // earlier calls' kernels on this stream may still read the old buffer: both steps are stream-ordered
+        const size_t grown = (std::max(bytes, slot->bytes + slot->bytes / 2) + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+        if (slot->ptr) {
+            check(hipFreeAsync(slot->ptr, stream), "hipFreeAsync(lease)", slot->bytes);
+            slot->ptr = nullptr;
+            slot->bytes = 0;
+        }
+        check(hipMallocAsync(&slot->ptr, grown, stream), "hipMallocAsync(lease)", grown);
+        slot->bytes = grown;
+    }
+    ptr_ = slot->ptr;
+    slot_ = slot;
+    t_held.push_back(slot);
+    lock.release(); // held until the lease ends
+}
+
+StreamLease::~StreamLease()
+{
+    delete fallback_;
+    if (slot_) {
+        auto* slot = static_cast<LeaseSlot*>(slot_);
+        t_held.erase(std::find(t_held.begin(), t_held.end(), slot));
+        slot->mu.unlock();
+    }
 }
 
 } // namespace aeth

@@ -192,7 +192,7 @@ void launch_cfg0(const T* a, const T* b, const T* c, T* d, int ncomp, uint64_t n
              "the sums' partials need " + std::to_string(need) + " bytes, beyond the fixed scratch of " +
                  std::to_string(kNoAllocScratchBytes) + " bytes",
              need);
-    StreamScratch scratch(fixed ? 0 : need, st);
+    StreamLease scratch(fixed ? 0 : need, st);
     auto* part = static_cast<acc::Pair<T>*>(fixed ? fixed : scratch.get());
     const int grid = grid_for(units);
     const bool nt = streaming(4ull * ncomp * n * sizeof(T));
@@ -238,7 +238,7 @@ void launch_reduce(const T* x, int ncomp, uint64_t n, uint64_t pitch, T* out, hi
     const uint32_t tiles = n ? (uint32_t)((n + kTile - 1) / kTile) : 1u;
     const uint32_t units = tiles * (uint32_t)ncomp;
     const bool vec = aligned16(x) && (pitch * sizeof(T)) % 16 == 0;
-    StreamScratch scratch(tree_scratch_pairs(tiles, ncomp) * sizeof(acc::Pair<T>), st);
+    StreamLease scratch(tree_scratch_pairs(tiles, ncomp) * sizeof(acc::Pair<T>), st);
     auto* part = static_cast<acc::Pair<T>*>(scratch.get());
     const int grid = grid_for(units);
     const bool nt = streaming((uint64_t)ncomp * n * sizeof(T));

@@ -389,7 +389,7 @@ void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums
         t = (t + kGroup - 1) / kGroup;
         pairs += (size_t)t * np;
     }
-    StreamScratch scratch(SUM ? pairs * sizeof(acc::Pair<T>) : 0, st);
+    StreamLease scratch(SUM ? pairs * sizeof(acc::Pair<T>) : 0, st);
     auto* part = static_cast<acc::Pair<T>*>(scratch.get());
     const int grid = (int)std::max<uint32_t>(1u, A.units);
     if (jit) {

@@ -757,6 +757,10 @@ def main():
     def step():
         E.assign(d, tree, ae.SUM_NEUMAIER, sums)
 
+    # the device's own 3-read/1-write ceiling on the same arrays, measured first: its ~40 launches also take
+    # the GPU's clocks out of their idle state (a cold MI355X runs this pass up to 25% slower for its first
+    # ~10 launches), so the headline and its ceiling are both timed at the device's steady state
+    stream = None if args.no_stream else wl_stream(a, b, c, d, n)
     torch.cuda.synchronize()
     for _ in range(max(1, args.warmup)):
         step()
@@ -802,8 +806,7 @@ def main():
         "sums": [s[k, 0].item() + s[k, 1].item() for k in range(3)],
     }
     # the device's own 3-read/1-write ceiling on the same arrays, beside the peak-based fraction
-    if not args.no_stream:
-        stream = wl_stream(a, b, c, d, n)
+    if stream is not None:
         line["roofline"]["stream_ceiling"] = stream
         line["roofline"]["frac_of_stream_ceiling"] = round(achieved / stream["ceiling_GB/s"], 4)
     if not args.no_extra:
