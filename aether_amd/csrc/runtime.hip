@@ -2,7 +2,7 @@
 // view/RuntimeView.h:71-120): out = a (with a double <-> float cast), a + b, a - b, s * a, a + s * b,
 // a - s * b, where every operand has its own element strides over one logical row-major shape.
 //
-// Dense row-major operands of one dtype take a flat kernel of 16-byte lanes; any other layout takes
+// Dense row-major operands of one dtype take a flat kernel of 16 KiB units of 16-byte lanes; any other layout takes
 // the general kernel, which splits each flat index into a multi-index over the output's extents and
 // addresses every operand through its own strides (the reference's addressing). Nothing is contracted:
 // a + s * b rounds the product, then the sum.
@@ -14,6 +14,7 @@
 
 #include "../../include/aether_hip.h"
 #include "common.hpp"
+#include "units.hpp"
 
 namespace aeth {
 namespace {
@@ -66,22 +67,55 @@ __global__ __launch_bounds__(kThreads) void strided_kernel(int op, Strided A, St
     }
 }
 
-// dense, one dtype: 16-byte lanes over the flat range, then the scalar tail
-template<class T>
-__global__ __launch_bounds__(kThreads) void dense_kernel(int op, const T* __restrict__ a, const T* __restrict__ b,
-                                                         T* __restrict__ o, T s, uint64_t total, bool uses_b)
+// Dense, one dtype: 16 KiB units (four 16-byte vectors a lane, every load of a unit issued before its
+// stores), one workgroup a unit in the XCD-aware unit order of the evaluation kernels; the vectors past the
+// last whole unit grid-stride, then the scalar tail. The output may be one of the inputs (an in-place
+// update): every element is read and written by the same lane, its loads before its store, so the
+// pointers carry no __restrict__. NT: the streaming cache policy, for operands far beyond the last-level
+// cache.
+constexpr uint32_t kUnitVecs = kThreads * 4;
+
+template<class T, bool NT>
+__global__ __launch_bounds__(kThreads) void dense_kernel(int op, const T* a, const T* b, T* o, T s, uint64_t total,
+                                                         uint32_t units, bool uses_b)
 {
     using V = T __attribute__((ext_vector_type(16 / sizeof(T))));
     constexpr int VN = 16 / sizeof(T);
+    constexpr uint32_t K = units::xcd_chunk(kUnitVecs * 16);
+    const V* av = reinterpret_cast<const V*>(a);
+    const V* bv = reinterpret_cast<const V*>(b);
+    V* ov = reinterpret_cast<V*>(o);
     const uint64_t vecs = total / VN;
-    for (uint64_t v = (uint64_t)blockIdx.x * kThreads + threadIdx.x; v < vecs; v += (uint64_t)gridDim.x * kThreads) {
-        const V x = reinterpret_cast<const V*>(a)[v];
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const uint64_t base = (uint64_t)units::xcd_unit(u, units, K) * kUnitVecs + threadIdx.x;
+        V x[4], y[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if constexpr (NT) x[r] = __builtin_nontemporal_load(av + base + r * kThreads);
+            else x[r] = av[base + r * kThreads];
+            if (uses_b) {
+                if constexpr (NT) y[r] = __builtin_nontemporal_load(bv + base + r * kThreads);
+                else y[r] = bv[base + r * kThreads];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            V res;
+#pragma unroll
+            for (int k = 0; k < VN; ++k) res[k] = apply<T, T>(op, x[r][k], uses_b ? y[r][k] : T(0), s);
+            if constexpr (NT) __builtin_nontemporal_store(res, ov + base + r * kThreads);
+            else ov[base + r * kThreads] = res;
+        }
+    }
+    for (uint64_t v = (uint64_t)units * kUnitVecs + (uint64_t)blockIdx.x * kThreads + threadIdx.x; v < vecs;
+         v += (uint64_t)gridDim.x * kThreads) {
+        const V x = av[v];
         V y;
-        if (uses_b) y = reinterpret_cast<const V*>(b)[v];
+        if (uses_b) y = bv[v];
         V r;
 #pragma unroll
         for (int k = 0; k < VN; ++k) r[k] = apply<T, T>(op, x[k], uses_b ? y[k] : T(0), s);
-        reinterpret_cast<V*>(o)[v] = r;
+        ov[v] = r;
     }
     const uint64_t t = vecs * VN + (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (blockIdx.x == 0 && t < total) o[t] = apply<T, T>(op, a[t], uses_b ? b[t] : T(0), s);
@@ -151,9 +185,19 @@ void launch(int op, const aeth_runtime_view& a, const aeth_runtime_view& b, cons
         const bool vec = dense(a) && dense(o) && (!uses_b || dense(b)) && aligned16(a.data) && aligned16(o.data) &&
                          (!uses_b || aligned16(b.data));
         if (vec) {
-            hipLaunchKernelGGL(dense_kernel<TIn>, dim3(grid), dim3(kThreads), 0, st, op, static_cast<const TIn*>(a.data),
-                               static_cast<const TIn*>(uses_b ? b.data : a.data), static_cast<TIn*>(o.data), s, total,
-                               uses_b);
+            const uint64_t vecs = total / (16 / sizeof(TIn));
+            const uint32_t units = (uint32_t)std::min<uint64_t>(vecs / kUnitVecs, 0x7FFFFFFFull);
+            const unsigned dgrid = units ? units : grid;
+            const bool nt = (uses_b ? 3 : 2) * total * sizeof(TIn) >= (512ull << 20);
+            const TIn* A = static_cast<const TIn*>(a.data);
+            const TIn* B = static_cast<const TIn*>(uses_b ? b.data : a.data);
+            TIn* O = static_cast<TIn*>(o.data);
+            if (nt)
+                hipLaunchKernelGGL((dense_kernel<TIn, true>), dim3(dgrid), dim3(kThreads), 0, st, op, A, B, O, s, total,
+                                   units, uses_b);
+            else
+                hipLaunchKernelGGL((dense_kernel<TIn, false>), dim3(dgrid), dim3(kThreads), 0, st, op, A, B, O, s,
+                                   total, units, uses_b);
             check(hipGetLastError(), "dense_kernel launch");
             return;
         }

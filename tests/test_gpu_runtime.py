@@ -32,6 +32,32 @@ def test_dense_ops_match_numpy(dev, op, dt, shape):
     assert np.array_equal(got.cpu().numpy(), np.asarray(want, np_dt))
 
 
+@pytest.mark.parametrize("op", ["add_scaled", "scale"])
+@pytest.mark.parametrize("dt,n", [(torch.float64, (1 << 25) + 3), (torch.float32, (1 << 26) + 1029),
+                                  (torch.float64, 4 * 1024 * 2 + 2 * 256 * 2 + 1)])
+def test_dense_units_and_streaming(dev, op, dt, n):
+    """Sizes past the streaming threshold (three operands of 256 MiB) and around the dense kernel's 16 KiB
+    units (whole units, spare vectors, a scalar tail), bit-exact against numpy."""
+    a, b = rnd((n,), dt, 3), rnd((n,), dt, 4)
+    s = -0.7
+    got = R.runtime_eval(op, a.cuda(), b.cuda(), scalar=s).cpu().numpy()
+    np_dt = np.float64 if dt == torch.float64 else np.float32
+    want = np.asarray(REF[op](a.numpy(), b.numpy(), np_dt(s)), np_dt)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [5, 4096 + 3, (1 << 20) + 7])
+def test_dense_in_place(dev, n):
+    """out may be an input: a = a + s*b and b = s*b in place give the out-of-place bits."""
+    a, b = rnd((n,), torch.float64, 5), rnd((n,), torch.float64, 6)
+    want_a = np.asarray(a.numpy() + 0.25 * b.numpy())
+    want_b = np.asarray(0.25 * b.numpy())
+    ga, gb = a.cuda(), b.cuda()
+    R.runtime_eval("add_scaled", ga, gb, out=ga, scalar=0.25)
+    R.runtime_eval("scale", gb, out=gb, scalar=0.25)
+    assert np.array_equal(ga.cpu().numpy(), want_a) and np.array_equal(gb.cpu().numpy(), want_b)
+
+
 @pytest.mark.parametrize("op", ["add", "sub_scaled", "scale", "assign"])
 def test_strided_operands(dev, op):
     base = rnd((64, 96), torch.float64, 3).cuda()
