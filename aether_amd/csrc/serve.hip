@@ -93,10 +93,22 @@ struct TreeDesc {
 struct BatchHeader {
     uint32_t fills, evals, reds, trees;
     uint32_t fft_ndesc[2]; // complex128, complex64 float-float
-    uint32_t jobs, pad_;
+    uint32_t jobs, dtypes; // dtypes: bit AETH_F64 / AETH_F32 set when some job of the batch has that dtype
     uint64_t fill_units, eval_units, red_units;
     uint64_t region_bytes[9]; // bytes used in each region of the staging layout (header first)
 };
+
+// Which dtypes a batch kernel is compiled for: a batch whose jobs all have one dtype runs the variant of that
+// dtype (fewer registers: the other dtype's arithmetic is not compiled in), a mixed batch the variant for
+// both. The per-unit bodies are the same code either way, so the bits do not depend on the variant.
+enum DtVariant : int { kDtF64 = 0, kDtF32 = 1, kDtBoth = 2 };
+
+template<int DT>
+__device__ __forceinline__ bool is_f64(uint32_t dtype)
+{
+    if constexpr (DT == kDtBoth) return dtype == AETH_F64;
+    else return DT == kDtF64;
+}
 
 template<class D>
 __device__ __forceinline__ uint32_t find_desc(const D* d, uint32_t count, uint64_t unit)
@@ -113,6 +125,7 @@ __device__ __forceinline__ uint32_t find_desc(const D* d, uint32_t count, uint64
 // One unit: kFillPerBlock consecutive samples of one component (units run component-major), so the
 // component and the sample range come from one 32-bit division per block, and a wave's stores are
 // consecutive.
+template<int DT>
 __device__ __forceinline__ void fill_unit(const FillDesc& D, uint64_t u)
 {
     const uint32_t chunks = (uint32_t)((D.n + kFillPerBlock - 1) / kFillPerBlock);
@@ -122,7 +135,7 @@ __device__ __forceinline__ void fill_unit(const FillDesc& D, uint64_t u)
     const uint64_t stop = (uint64_t)i0 + kFillPerBlock;
     const uint32_t end = (uint32_t)(stop < D.n ? stop : D.n);
     if (D.kind == 0) {
-        if (D.dtype == AETH_F64) {
+        if (is_f64<DT>(D.dtype)) {
             double* out = static_cast<double*>(D.out) + (uint64_t)c * D.pitch;
             for (uint64_t i = (uint64_t)i0 + threadIdx.x; i < end; i += kThreads)
                 out[i] = D.p0 + (D.p1 - D.p0) * rng::u01_from_word_d(rng::block(D.seed, (uint32_t)i, c).x);
@@ -135,27 +148,29 @@ __device__ __forceinline__ void fill_unit(const FillDesc& D, uint64_t u)
     } else {
         for (uint64_t i = (uint64_t)i0 + threadIdx.x; i < end; i += kThreads) {
             const rng::U4 b = rng::block(D.seed, (uint32_t)i, 0);
-            if (D.dtype == AETH_F64) static_cast<double*>(D.out)[i] = rng::normal_from_block_d(b);
+            if (is_f64<DT>(D.dtype)) static_cast<double*>(D.out)[i] = rng::normal_from_block_d(b);
             else static_cast<float*>(D.out)[i] = rng::normal_from_block_f(b);
         }
     }
 }
 
+template<int DT>
 __global__ __launch_bounds__(kThreads) void fill_batch_kernel(const BatchHeader* __restrict__ hdr,
                                                              const FillDesc* __restrict__ descs)
 {
     const uint32_t count = hdr->fills;
     const uint64_t units = hdr->fill_units;
     for (uint64_t u = blockIdx.x; u < units; u += gridDim.x)
-        fill_unit(descs[find_desc(descs, count, u)], u);
+        fill_unit<DT>(descs[find_desc(descs, count, u)], u);
 }
 
+template<int DT>
 __device__ __forceinline__ void eval_unit(const EvalDesc& D, uint64_t u, acc::Pair<double>* ldsd,
                                           acc::Pair<float>* ldsf)
 {
     const uint32_t local = (uint32_t)(u - D.unit_start);
     const uint32_t comp = local / D.tiles, tile = local - comp * D.tiles;
-    if (D.dtype == AETH_F64) {
+    if (is_f64<DT>(D.dtype)) {
         const auto p = units::cfg0_tile<double, acc::kNeumaier, true, true, false>(
             static_cast<const double*>(D.a), static_cast<const double*>(D.b), static_cast<const double*>(D.c),
             static_cast<double*>(D.d), D.n, D.pitch, comp, tile);
@@ -170,6 +185,7 @@ __device__ __forceinline__ void eval_unit(const EvalDesc& D, uint64_t u, acc::Pa
     }
 }
 
+template<int DT>
 __global__ __launch_bounds__(kThreads) void eval_batch_kernel(const BatchHeader* __restrict__ hdr,
                                                              const EvalDesc* __restrict__ descs)
 {
@@ -177,14 +193,16 @@ __global__ __launch_bounds__(kThreads) void eval_batch_kernel(const BatchHeader*
     __shared__ acc::Pair<float> ldsf[4];
     const uint32_t count = hdr->evals;
     const uint64_t units = hdr->eval_units;
-    for (uint64_t u = blockIdx.x; u < units; u += gridDim.x) eval_unit(descs[find_desc(descs, count, u)], u, ldsd, ldsf);
+    for (uint64_t u = blockIdx.x; u < units; u += gridDim.x)
+        eval_unit<DT>(descs[find_desc(descs, count, u)], u, ldsd, ldsf);
 }
 
+template<int DT>
 __device__ __forceinline__ void reduce_unit(const RedDesc& D, uint64_t u, acc::Pair<double>* ldsd,
                                             acc::Pair<float>* ldsf)
 {
     const uint32_t tile = (uint32_t)(u - D.unit_start);
-    if (D.dtype == AETH_F64) {
+    if (is_f64<DT>(D.dtype)) {
         const double* x = static_cast<const double*>(D.x);
         const auto p = D.square ? units::reduce_tile<double, acc::kNeumaier, true, true, false>(x, D.n, 0, 0, tile)
                                 : units::reduce_tile<double, acc::kNeumaier, false, true, false>(x, D.n, 0, 0, tile);
@@ -199,6 +217,7 @@ __device__ __forceinline__ void reduce_unit(const RedDesc& D, uint64_t u, acc::P
     }
 }
 
+template<int DT>
 __global__ __launch_bounds__(kThreads) void reduce_batch_kernel(const BatchHeader* __restrict__ hdr,
                                                                const RedDesc* __restrict__ descs)
 {
@@ -207,7 +226,7 @@ __global__ __launch_bounds__(kThreads) void reduce_batch_kernel(const BatchHeade
     const uint32_t count = hdr->reds;
     const uint64_t units = hdr->red_units;
     for (uint64_t u = blockIdx.x; u < units; u += gridDim.x)
-        reduce_unit(descs[find_desc(descs, count, u)], u, ldsd, ldsf);
+        reduce_unit<DT>(descs[find_desc(descs, count, u)], u, ldsd, ldsf);
 }
 
 // The group tree of the single-call path, all levels inside one block.
@@ -630,6 +649,7 @@ BatchHeader build_batch(Slot& S, const Layout& L, void* base)
     h.fft_ndesc[0] = (uint32_t)fdesc[0].size();
     h.fft_ndesc[1] = (uint32_t)fdesc[2].size();
     h.jobs = (uint32_t)S.jobs.size();
+    for (const Job& J : S.jobs) h.dtypes |= 1u << J.spec.dtype;
     h.fill_units = fill_units;
     h.eval_units = eval_units;
     h.red_units = red_units;
@@ -659,8 +679,24 @@ struct PersistentGrids {
     int fft[2] = {0, 0};
 };
 
+template<int DT>
+void enqueue_kernels(Slot& S, const Layout& L, void* base, const BatchHeader* h, const PersistentGrids& pg);
+
+// `dtypes`: the dtype mask the batch kernels must cover (a direct launch takes its batch's; a captured
+// graph the mask of every batch it will replay).
 void enqueue_batch(Slot& S, const Layout& L, void* base, const BatchHeader* h, uint32_t max_jobs,
-                   const PersistentGrids& pg)
+                   const PersistentGrids& pg, uint32_t dtypes)
+{
+    if (h) dtypes = h->dtypes;
+    if (dtypes == (1u << AETH_F64)) enqueue_kernels<kDtF64>(S, L, base, h, pg);
+    else if (dtypes == (1u << AETH_F32)) enqueue_kernels<kDtF32>(S, L, base, h, pg);
+    else enqueue_kernels<kDtBoth>(S, L, base, h, pg);
+    const size_t rb = 8 * sizeof(double) * (h ? std::max<size_t>(1, h->jobs) : max_jobs);
+    check(hipMemcpyAsync(S.host_res, S.dev_res, rb, hipMemcpyDeviceToHost, S.stream), "hipMemcpyAsync(results)");
+}
+
+template<int DT>
+void enqueue_kernels(Slot& S, const Layout& L, void* base, const BatchHeader* h, const PersistentGrids& pg)
 {
     hipStream_t st = S.stream;
     const BatchHeader* hdr = at<BatchHeader>(S, L, kRegHeader);
@@ -670,10 +706,10 @@ void enqueue_batch(Slot& S, const Layout& L, void* base, const BatchHeader* h, u
     hipLaunchKernelGGL(stage_in_kernel, dim3(kStageBlocks), dim3(kThreads), 0, st, S.host_stage_dev, S.dev_stage,
                        S.region_offset);
     if (!h || h->fills)
-        hipLaunchKernelGGL(fill_batch_kernel, dim3(grid(h ? h->fill_units : 0)), dim3(kThreads), 0, st, hdr,
+        hipLaunchKernelGGL(fill_batch_kernel<DT>, dim3(grid(h ? h->fill_units : 0)), dim3(kThreads), 0, st, hdr,
                            at<FillDesc>(S, L, kRegFill));
     if (!h || h->evals)
-        hipLaunchKernelGGL(eval_batch_kernel, dim3(grid(h ? h->eval_units : 0)), dim3(kThreads), 0, st, hdr,
+        hipLaunchKernelGGL(eval_batch_kernel<DT>, dim3(grid(h ? h->eval_units : 0)), dim3(kThreads), 0, st, hdr,
                            at<EvalDesc>(S, L, kRegEval));
     for (int q = 0; q < 2; ++q) {
         const int prec = q == 0 ? 0 : 2;
@@ -692,18 +728,17 @@ void enqueue_batch(Slot& S, const Layout& L, void* base, const BatchHeader* h, u
         }
     }
     if (!h || h->reds)
-        hipLaunchKernelGGL(reduce_batch_kernel, dim3(grid(h ? h->red_units : 0)), dim3(kThreads), 0, st, hdr,
+        hipLaunchKernelGGL(reduce_batch_kernel<DT>, dim3(grid(h ? h->red_units : 0)), dim3(kThreads), 0, st, hdr,
                            at<RedDesc>(S, L, kRegRed));
     if (!h || h->trees)
         hipLaunchKernelGGL(tree_batch_kernel, dim3(h ? std::max<uint32_t>(1, h->trees) : (uint32_t)persist),
                            dim3(512), 0, st, hdr, at<TreeDesc>(S, L, kRegTree));
     check(hipGetLastError(), "serve batch launch");
-    const size_t rb = 8 * sizeof(double) * (h ? std::max<size_t>(1, h->jobs) : max_jobs);
-    check(hipMemcpyAsync(S.host_res, S.dev_res, rb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(results)");
 }
 
 // Capture the persistent form of a batch on the slot's stream into a graph (once per grid candidate).
-hipGraphExec_t capture_batch_graph(Slot& S, const Layout& L, void* base, uint32_t max_jobs, uint32_t units_per_cu)
+hipGraphExec_t capture_batch_graph(Slot& S, const Layout& L, void* base, uint32_t max_jobs, uint32_t units_per_cu,
+                                   uint32_t dtypes)
 {
     // everything the launches look up is ready before the capture: twiddle tables, grid sizes
     fft::prepare(S.stream);
@@ -715,7 +750,7 @@ hipGraphExec_t capture_batch_graph(Slot& S, const Layout& L, void* base, uint32_
     check(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipGraph_t g = nullptr;
     try {
-        enqueue_batch(S, L, base, nullptr, max_jobs, pg);
+        enqueue_batch(S, L, base, nullptr, max_jobs, pg, dtypes);
     } catch (...) {
         (void)hipStreamEndCapture(S.stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -739,7 +774,7 @@ void launch_batch(Slot& S, const Layout& L, void* base, bool graph, int cand, bo
         check(hipGraphLaunch(S.graph[cand], S.stream), "hipGraphLaunch");
         if (timed) check(hipEventRecord(S.t1, S.stream), "hipEventRecord");
     } else {
-        enqueue_batch(S, L, base, &h, 0, PersistentGrids{});
+        enqueue_batch(S, L, base, &h, 0, PersistentGrids{}, 0);
     }
     check(hipEventRecord(S.done, S.stream), "hipEventRecord");
 }
@@ -954,14 +989,17 @@ extern "C" int aeth_server_create(const aeth_server_config* cfg, void** server)
             const size_t rb = 8 * sizeof(double) * S->cfg.max_batch_jobs;
             check(hipHostMalloc(reinterpret_cast<void**>(&sl.host_res), rb, hipHostMallocDefault), "hipHostMalloc(res)");
             check(hipMalloc(reinterpret_cast<void**>(&sl.dev_res), rb), "hipMalloc(res)");
+            // an FP32-compensated server runs every job in FP32; otherwise batches may mix both dtypes
+            const uint32_t dtypes = S->cfg.fp32_compensated ? 1u << AETH_F32 : (1u << AETH_F64) | (1u << AETH_F32);
             if (S->tune) {
                 check(hipEventCreate(&sl.t0), "hipEventCreate");
                 check(hipEventCreate(&sl.t1), "hipEventCreate");
                 for (int c = 0; c < 4; ++c)
-                    sl.graph[c] = capture_batch_graph(sl, S->layout, S->pool->base(), S->cfg.max_batch_jobs, kUnitsPerCu[c]);
+                    sl.graph[c] = capture_batch_graph(sl, S->layout, S->pool->base(), S->cfg.max_batch_jobs,
+                                                      kUnitsPerCu[c], dtypes);
             } else if (S->graph) {
                 sl.graph[kDefaultCand] = capture_batch_graph(sl, S->layout, S->pool->base(), S->cfg.max_batch_jobs,
-                                                             kUnitsPerCu[kDefaultCand]);
+                                                             kUnitsPerCu[kDefaultCand], dtypes);
             }
         }
         Server* raw = S.get();
