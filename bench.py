@@ -553,7 +553,10 @@ def wl_fft() -> dict:
 
 
 def wl_serve_alone() -> dict:
-    return wl_serve(2.0)
+    return wl_serve(_SERVE_ALONE_SECONDS)
+
+
+_SERVE_ALONE_SECONDS = 2.0
 
 
 def fp32_error_vs_fp64(jobs, r32, per_kind: int = 12) -> dict:
@@ -646,6 +649,7 @@ def wl_serve(duration: float, rate: float = 20000.0, plain_seconds: float = 4.0)
            "offered_GB_per_s_fp32_compensated": round(nb[True] / duration / 1e9, 1)}
 
     def summarize(res, label, nbytes):
+        print(f"serving: {label} done ({len(res)} jobs)", file=sys.stderr, flush=True)
         sub = np.array([r.submit_time for r in res])
         done = np.array([r.done_time for r in res])
         lat = (done - sub) * 1e3
@@ -737,6 +741,9 @@ def main():
         copy_calibration(args.steps)
         return
     if args.workload:
+        global _SERVE_ALONE_SECONDS
+        if args.serve_seconds != 3.0:
+            _SERVE_ALONE_SECONDS = args.serve_seconds
         print(json.dumps({args.workload: globals()[SINGLE_WORKLOADS[args.workload]]()}), flush=True)
         return
     import torch
