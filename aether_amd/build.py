@@ -20,6 +20,7 @@ LIB = os.path.join(LIB_DIR, "libaether_hip.so")
 API_CHECK = os.path.join(LIB_DIR, "api_check")
 RESIDENCY_CHECK = os.path.join(LIB_DIR, "residency_check")
 FLAG_CHECK = os.path.join(LIB_DIR, "flag_check")
+EXPR_DEVICE_CHECK = os.path.join(LIB_DIR, "expr_device_check")
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AETH_OFFLOAD_ARCH", "gfx950")
@@ -115,18 +116,24 @@ def build_api_check() -> str:
     return _host_driver("api_check.cpp", API_CHECK)
 
 
-def build_flag_check() -> str:
-    """tests/cpp/flag_check.hip: a HIP program whose own kernels set aether::DeviceFlag through
-    DeviceFlagView (include/aether/device/flag.hpp), the way a user's kernels would; hipcc against
-    libaether_hip.so."""
-    src = os.path.join(ROOT, "tests", "cpp", "flag_check.hip")
-    cmd = [HIPCC, *FLAGS, "-Werror", "-I", os.path.join(ROOT, "include"), src, "-o", FLAG_CHECK + ".tmp",
+def _hip_driver(name: str, out: str) -> str:
+    src = os.path.join(ROOT, "tests", "cpp", name)
+    cmd = [HIPCC, *FLAGS, "-Werror", "-I", os.path.join(ROOT, "include"), src, "-o", out + ".tmp",
            "-L", LIB_DIR, "-laether_hip", "-Wl,-rpath,$ORIGIN"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr}")
-    os.replace(FLAG_CHECK + ".tmp", FLAG_CHECK)
-    return FLAG_CHECK
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_flag_check() -> str:
+    """tests/cpp/flag_check.hip: a HIP program whose own kernels set aether::DeviceFlag through
+    DeviceFlagView (include/aether/device/flag.hpp), the way a user's kernels would; and
+    tests/cpp/expr_device_check.hip, whose own kernels evaluate the C++ front end's expressions per sample
+    over device views (compared bit for bit with the whole-array assignment). hipcc against libaether_hip.so."""
+    _hip_driver("expr_device_check.hip", EXPR_DEVICE_CHECK)
+    return _hip_driver("flag_check.hip", FLAG_CHECK)
 
 
 def build_dsc_core() -> str:

@@ -23,7 +23,10 @@
 // per component; node semantics: -x is x*-1, x/s is x*(1/s) with the
 // reciprocal rounded in T, folds right-nested). The per-sample host spelling
 // `hv[SampleIndex::make(i)] = expr` evaluates on the host with std:: math,
-// like the reference's host route.
+// like the reference's host route. In a HIP translation unit the same
+// spelling works inside the caller's own kernels over device views
+// (`dv[SampleIndex::make(threadIdx.x, blockIdx.x, blockDim.x)] = expr`) and
+// writes the bits the whole-array assignment writes.
 #pragma once
 
 #include <algorithm>
@@ -41,6 +44,21 @@
 
 #include "../aether_hip.h"
 #include "device/flag.hpp"
+
+// AETHER_DEVICEHOST() (the reference's aether/macros.h): the expression nodes, views, SampleIndex, SampleRef
+// and the distribution leaves compile for the host and, in a HIP translation unit, for the device as well, so
+// a caller's own kernel can evaluate `view[SampleIndex::make(threadIdx.x, blockIdx.x, blockDim.x)] = expr` per
+// sample. On the device the nodes call this library's device math and Philox (aether_amd/csrc/dmath.hpp,
+// rng.hpp: the functions the runtime-compiled tape kernels call), so such a kernel writes the same bits as the
+// whole-array assignment `dv = expr`; on the host they keep std:: math (the reference's host route).
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+
+#include "../../aether_amd/csrc/rng.hpp"
+#define AETHER_DEVICEHOST() __host__ __device__
+#else
+#define AETHER_DEVICEHOST()
+#endif
 
 namespace aether {
 
@@ -241,15 +259,97 @@ TapeCtx lower(const void* dest, std::size_t pitch, const E& e)
     return x;
 }
 
+// One node operation on one value: std:: math on the host, on the device the tape kernels' own device math.
+template<int FN, class T>
+AETHER_DEVICEHOST() inline T unary_op(T a)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (FN == AETH_OP_SQRT) {
+        if constexpr (sizeof(T) == 8) return __builtin_sqrt(a);
+        else return __builtin_sqrtf(a);
+    } else if constexpr (FN == AETH_OP_ABS) {
+        if constexpr (sizeof(T) == 8) return __builtin_fabs(a);
+        else return __builtin_fabsf(a);
+    } else if constexpr (FN == AETH_OP_SIGN) {
+        return aeth::dm::sign(a);
+    } else if constexpr (FN == AETH_OP_NEG) {
+        return -a;
+    } else if constexpr (sizeof(T) == 8) {
+        if constexpr (FN == AETH_OP_SIN) return aeth::dm::sin(a);
+        else if constexpr (FN == AETH_OP_COS) return aeth::dm::cos(a);
+        else if constexpr (FN == AETH_OP_EXP) return aeth::dm::exp(a);
+        else if constexpr (FN == AETH_OP_LOG) return aeth::dm::log(a);
+        else if constexpr (FN == AETH_OP_TAN) return aeth::dm::tan(a);
+        else if constexpr (FN == AETH_OP_ATAN) return aeth::dm::atan(a);
+        else if constexpr (FN == AETH_OP_ASIN) return aeth::dm::asin(a);
+        else if constexpr (FN == AETH_OP_ACOS) return aeth::dm::acos(a);
+        else return aeth::dm::tanh(a);
+    } else {
+        if constexpr (FN == AETH_OP_SIN) return aeth::dm::sinf(a);
+        else if constexpr (FN == AETH_OP_COS) return aeth::dm::cosf(a);
+        else if constexpr (FN == AETH_OP_EXP) return aeth::dm::expf(a);
+        else if constexpr (FN == AETH_OP_LOG) return aeth::dm::logf(a);
+        else if constexpr (FN == AETH_OP_TAN) return aeth::dm::tanf(a);
+        else if constexpr (FN == AETH_OP_ATAN) return aeth::dm::atanf(a);
+        else if constexpr (FN == AETH_OP_ASIN) return aeth::dm::asinf(a);
+        else if constexpr (FN == AETH_OP_ACOS) return aeth::dm::acosf(a);
+        else return aeth::dm::tanhf(a);
+    }
+#else
+    if constexpr (FN == AETH_OP_SIN) return std::sin(a);
+    else if constexpr (FN == AETH_OP_COS) return std::cos(a);
+    else if constexpr (FN == AETH_OP_EXP) return std::exp(a);
+    else if constexpr (FN == AETH_OP_LOG) return std::log(a);
+    else if constexpr (FN == AETH_OP_SQRT) return std::sqrt(a);
+    else if constexpr (FN == AETH_OP_ABS) return std::fabs(a);
+    else if constexpr (FN == AETH_OP_SIGN) return a > 0 ? T(1) : (a < 0 ? T(-1) : T(0));
+    else if constexpr (FN == AETH_OP_TAN) return std::tan(a);
+    else if constexpr (FN == AETH_OP_ATAN) return std::atan(a);
+    else if constexpr (FN == AETH_OP_ASIN) return std::asin(a);
+    else if constexpr (FN == AETH_OP_ACOS) return std::acos(a);
+    else if constexpr (FN == AETH_OP_TANH) return std::tanh(a);
+    else return -a;
+#endif
+}
+
+template<int OP, class T>
+AETHER_DEVICEHOST() inline T binary_op(T a, T b)
+{
+    if constexpr (OP == AETH_OP_ADD) return a + b;
+    else if constexpr (OP == AETH_OP_SUB) return a - b;
+    else if constexpr (OP == AETH_OP_MUL) return a * b;
+    else if constexpr (OP == AETH_OP_DIV) return a / b;
+#if defined(__HIP_DEVICE_COMPILE__)
+    else if constexpr (OP == AETH_OP_FMIN) return aeth::dm::fmin(a, b);
+    else if constexpr (OP == AETH_OP_FMAX) return aeth::dm::fmax(a, b);
+    else if constexpr (OP == AETH_OP_SELGT) return aeth::dm::selgt(a, b);
+    else if constexpr (sizeof(T) == 8) return aeth::dm::pow(a, b);
+    else return aeth::dm::powf(a, b);
+#else
+    else if constexpr (OP == AETH_OP_FMIN) return std::fmin(a, b);
+    else if constexpr (OP == AETH_OP_FMAX) return std::fmax(a, b);
+    else if constexpr (OP == AETH_OP_SELGT) return a > b ? a : b;
+    else return std::pow(a, b);
+#endif
+}
+
 } // namespace detail
 
 // ---------------------------------------------------------------- index
 /** index/SampleIndex.h:36-61 */
 struct SampleIndex {
     offset_t g = 0, w = 0;
-    static constexpr SampleIndex make(std::size_t index) { return SampleIndex{(offset_t)index, (offset_t)index}; }
-    constexpr offset_t global() const { return g; }
-    constexpr offset_t work() const { return w; }
+    AETHER_DEVICEHOST() static constexpr SampleIndex make(std::size_t index)
+    {
+        return SampleIndex{(offset_t)index, (offset_t)index};
+    }
+    /** From a kernel's built-in indices: global = blockIdx * blockDim + threadIdx, work = threadIdx. */
+    AETHER_DEVICEHOST() static constexpr SampleIndex make(offset_t thread, offset_t block, offset_t block_dim)
+    {
+        return SampleIndex{block * block_dim + thread, thread};
+    }
+    AETHER_DEVICEHOST() constexpr offset_t global() const { return g; }
+    AETHER_DEVICEHOST() constexpr offset_t work() const { return w; }
 };
 
 // ---------------------------------------------------------------- expressions
@@ -290,85 +390,85 @@ struct SegmentNode;
  *  the folds keep the reference's order (expr/Reduce.h:150-164: acc = op(T(x_{k+1}), acc)). */
 template<class D>
 struct Expr {
-    const D& self() const { return static_cast<const D&>(*this); }
+    AETHER_DEVICEHOST() const D& self() const { return static_cast<const D&>(*this); }
     template<class E>
-    auto dot(const E& o) const
+    AETHER_DEVICEHOST() auto dot(const E& o) const
     {
         return DotNode<D, E>(self(), o);
     }
     template<class E>
-    auto cross(const E& o) const
+    AETHER_DEVICEHOST() auto cross(const E& o) const
     {
         return CrossNode<D, E>(self(), o);
     }
-    auto squaredNorm() const { return DotNode<D, D>(self(), self()); }
-    auto norm() const { return UnaryNode<DotNode<D, D>, AETH_OP_SQRT>(squaredNorm()); }
+    AETHER_DEVICEHOST() auto squaredNorm() const { return DotNode<D, D>(self(), self()); }
+    AETHER_DEVICEHOST() auto norm() const { return UnaryNode<DotNode<D, D>, AETH_OP_SQRT>(squaredNorm()); }
     /** 1 / sqrt(|v|^2) (Expression.h:202-206) */
-    auto rNorm() const { return BinaryNode<ScalarNode<typename D::element_type>, decltype(norm()), AETH_OP_DIV>(one(), norm()); }
+    AETHER_DEVICEHOST() auto rNorm() const { return BinaryNode<ScalarNode<typename D::element_type>, decltype(norm()), AETH_OP_DIV>(one(), norm()); }
     /** 1 / |v|^2 */
-    auto rSquaredNorm() const
+    AETHER_DEVICEHOST() auto rSquaredNorm() const
     {
         return BinaryNode<ScalarNode<typename D::element_type>, DotNode<D, D>, AETH_OP_DIV>(one(), squaredNorm());
     }
     /** |v|^2 sqrt(|v|^2) */
-    auto cubedNorm() const
+    AETHER_DEVICEHOST() auto cubedNorm() const
     {
         return BinaryNode<DotNode<D, D>, decltype(norm()), AETH_OP_MUL>(squaredNorm(), norm());
     }
     /** 1 / (|v|^2 sqrt(|v|^2)) */
-    auto rCubedNorm() const
+    AETHER_DEVICEHOST() auto rCubedNorm() const
     {
         return BinaryNode<ScalarNode<typename D::element_type>, decltype(cubedNorm()), AETH_OP_DIV>(one(), cubedNorm());
     }
     /** max over components of (x < 0 ? -x : x), the ordered max (expr/Reduce.h:74-130) */
-    auto maxNorm() const { return FoldNode<D, true>(self()); }
+    AETHER_DEVICEHOST() auto maxNorm() const { return FoldNode<D, true>(self()); }
     /** sum of the components */
-    auto sum() const { return FoldNode<D, false>(self()); }
+    AETHER_DEVICEHOST() auto sum() const { return FoldNode<D, false>(self()); }
     /** v * rNorm() (expr/nodes/Geometric.h UnitVector) */
-    auto unitVector() const { return BinaryNode<D, decltype(rNorm()), AETH_OP_MUL>(self(), rNorm()); }
+    AETHER_DEVICEHOST() auto unitVector() const { return BinaryNode<D, decltype(rNorm()), AETH_OP_MUL>(self(), rNorm()); }
     /** the Hadamard product (expr/nodes/Product.h) */
     template<class E>
-    auto cwiseMul(const E& o) const
+    AETHER_DEVICEHOST() auto cwiseMul(const E& o) const
     {
         return BinaryNode<D, E, AETH_OP_MUL>(self(), o);
     }
     /** Hamilton product (expr/nodes/Quaternion.h:35-80) */
     template<class E>
-    auto quatMul(const E& o) const
+    AETHER_DEVICEHOST() auto quatMul(const E& o) const
     {
         return QuatMulNode<D, E>(self(), o);
     }
-    auto quatConj() const { return QuatConjNode<D>(self()); }
-    auto quatReciprocal() const
+    AETHER_DEVICEHOST() auto quatConj() const { return QuatConjNode<D>(self()); }
+    AETHER_DEVICEHOST() auto quatReciprocal() const
     {
         return BinaryNode<QuatConjNode<D>, decltype(rSquaredNorm()), AETH_OP_MUL>(quatConj(), rSquaredNorm());
     }
     /** rotation of the 3-vector v by this unit quaternion (Quaternion.h:117-170) */
     template<class V>
-    auto quatRotate(const V& v) const
+    AETHER_DEVICEHOST() auto quatRotate(const V& v) const
     {
         return QuatRotateNode<D, V>(self(), v);
     }
-    auto asPureQuaternion() const { return PureQuatNode<D>(self()); }
+    AETHER_DEVICEHOST() auto asPureQuaternion() const { return PureQuatNode<D>(self()); }
     template<std::size_t OFF, std::size_t LEN>
-    auto segment() const
+    AETHER_DEVICEHOST() auto segment() const
     {
         return SegmentNode<D, OFF, LEN>(self());
     }
     template<std::size_t N>
-    auto head() const
+    AETHER_DEVICEHOST() auto head() const
     {
         return segment<0, N>();
     }
     template<std::size_t N>
-    auto tail() const
+    AETHER_DEVICEHOST() auto tail() const
     {
         return segment<D::ncomp - N, N>();
     }
-    auto asBack3DVector() const { return tail<3>(); }
+    AETHER_DEVICEHOST() auto asBack3DVector() const { return tail<3>(); }
 
 private:
-    static auto one() { return ScalarNode<typename D::element_type>(1); }
+    AETHER_DEVICEHOST() static auto one() { return ScalarNode<typename D::element_type>(1); }
 };
 
 template<class T>
@@ -376,9 +476,9 @@ struct ScalarNode : Expr<ScalarNode<T>> {
     using element_type = T;
     static constexpr std::size_t ncomp = 1;
     T v;
-    explicit ScalarNode(T x) : v(x) {}
+    AETHER_DEVICEHOST() explicit ScalarNode(T x) : v(x) {}
     void emit(std::size_t, detail::TapeCtx& c) const { c.constant((double)v); }
-    T eval(std::size_t, std::size_t) const { return v; }
+    AETHER_DEVICEHOST() T eval(std::size_t, std::size_t) const { return v; }
 };
 
 template<class L, class R, int OP>
@@ -389,24 +489,17 @@ struct BinaryNode : Expr<BinaryNode<L, R, OP>> {
     static_assert(std::is_same_v<elem_t<L>, elem_t<R>>, "aether: element types differ");
     L l;
     R r;
-    BinaryNode(const L& a, const R& b) : l(a), r(b) {}
+    AETHER_DEVICEHOST() BinaryNode(const L& a, const R& b) : l(a), r(b) {}
     void emit(std::size_t c, detail::TapeCtx& x) const
     {
         l.emit(L::ncomp > 1 ? c : 0, x);
         r.emit(R::ncomp > 1 ? c : 0, x);
         x.op(OP);
     }
-    element_type eval(std::size_t c, std::size_t i) const
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const
     {
         const element_type a = l.eval(L::ncomp > 1 ? c : 0, i), b = r.eval(R::ncomp > 1 ? c : 0, i);
-        if constexpr (OP == AETH_OP_ADD) return a + b;
-        else if constexpr (OP == AETH_OP_SUB) return a - b;
-        else if constexpr (OP == AETH_OP_MUL) return a * b;
-        else if constexpr (OP == AETH_OP_FMIN) return std::fmin(a, b);
-        else if constexpr (OP == AETH_OP_FMAX) return std::fmax(a, b);
-        else if constexpr (OP == AETH_OP_SELGT) return a > b ? a : b;
-        else if constexpr (OP == AETH_OP_POW) return std::pow(a, b);
-        else return a / b;
+        return detail::binary_op<OP>(a, b);
     }
 };
 
@@ -415,29 +508,13 @@ struct UnaryNode : Expr<UnaryNode<E, FN>> {
     using element_type = elem_t<E>;
     static constexpr std::size_t ncomp = E::ncomp;
     E e;
-    explicit UnaryNode(const E& x) : e(x) {}
+    AETHER_DEVICEHOST() explicit UnaryNode(const E& x) : e(x) {}
     void emit(std::size_t c, detail::TapeCtx& x) const
     {
         e.emit(c, x);
         x.op(FN);
     }
-    element_type eval(std::size_t c, std::size_t i) const
-    {
-        const element_type a = e.eval(c, i);
-        if constexpr (FN == AETH_OP_SIN) return std::sin(a);
-        else if constexpr (FN == AETH_OP_COS) return std::cos(a);
-        else if constexpr (FN == AETH_OP_EXP) return std::exp(a);
-        else if constexpr (FN == AETH_OP_LOG) return std::log(a);
-        else if constexpr (FN == AETH_OP_SQRT) return std::sqrt(a);
-        else if constexpr (FN == AETH_OP_ABS) return std::fabs(a);
-        else if constexpr (FN == AETH_OP_SIGN) return a > 0 ? element_type(1) : (a < 0 ? element_type(-1) : element_type(0));
-        else if constexpr (FN == AETH_OP_TAN) return std::tan(a);
-        else if constexpr (FN == AETH_OP_ATAN) return std::atan(a);
-        else if constexpr (FN == AETH_OP_ASIN) return std::asin(a);
-        else if constexpr (FN == AETH_OP_ACOS) return std::acos(a);
-        else if constexpr (FN == AETH_OP_TANH) return std::tanh(a);
-        else return -a;
-    }
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const { return detail::unary_op<FN>(e.eval(c, i)); }
 };
 
 /** expr/Reduce.h:150-164: acc = T(x_0), then acc = op(T(x_{k+1}), acc); sum: T = x, op = +; maxNorm:
@@ -448,7 +525,7 @@ struct FoldNode : Expr<FoldNode<E, MAX>> {
     using element_type = elem_t<E>;
     static constexpr std::size_t ncomp = 1;
     E e;
-    explicit FoldNode(const E& x) : e(x) {}
+    AETHER_DEVICEHOST() explicit FoldNode(const E& x) : e(x) {}
     void emit(std::size_t, detail::TapeCtx& x) const
     {
         for (std::size_t k = E::ncomp; k-- > 0;) {
@@ -461,13 +538,13 @@ struct FoldNode : Expr<FoldNode<E, MAX>> {
         }
         for (std::size_t k = 1; k < E::ncomp; ++k) x.op(MAX ? AETH_OP_SELGT : AETH_OP_ADD);
     }
-    element_type term(std::size_t k, std::size_t i) const
+    AETHER_DEVICEHOST() element_type term(std::size_t k, std::size_t i) const
     {
         const element_type a = e.eval(k, i);
         if constexpr (MAX) return -a > a ? -a : a;
         else return a;
     }
-    element_type eval(std::size_t, std::size_t i) const
+    AETHER_DEVICEHOST() element_type eval(std::size_t, std::size_t i) const
     {
         element_type acc = term(0, i);
         for (std::size_t k = 1; k < E::ncomp; ++k) {
@@ -487,7 +564,7 @@ struct QuatMulNode : Expr<QuatMulNode<L, R>> {
     static_assert(L::ncomp == 4 && R::ncomp == 4, "aether: quatMul needs quaternions");
     L l;
     R r;
-    QuatMulNode(const L& a, const R& b) : l(a), r(b) {}
+    AETHER_DEVICEHOST() QuatMulNode(const L& a, const R& b) : l(a), r(b) {}
     // component c = l[A0] r[B0] s1 l[A1] r[B1] s2 l[A2] r[B2] s3 l[A3] r[B3], s = +1 / -1
     static constexpr int A[4][4] = {{0, 1, 2, 3}, {0, 1, 2, 3}, {0, 1, 2, 3}, {0, 1, 2, 3}};
     static constexpr int B[4][4] = {{0, 1, 2, 3}, {1, 0, 3, 2}, {2, 3, 0, 1}, {3, 2, 1, 0}};
@@ -504,7 +581,7 @@ struct QuatMulNode : Expr<QuatMulNode<L, R>> {
             x.op(S[c][t - 1] > 0 ? AETH_OP_ADD : AETH_OP_SUB);
         }
     }
-    element_type eval(std::size_t c, std::size_t i) const
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const
     {
         element_type acc = l.eval(A[c][0], i) * r.eval(B[c][0], i);
         for (int t = 1; t < 4; ++t) {
@@ -522,13 +599,13 @@ struct QuatConjNode : Expr<QuatConjNode<E>> {
     static constexpr std::size_t ncomp = 4;
     static_assert(E::ncomp == 4, "aether: quatConj needs a quaternion");
     E e;
-    explicit QuatConjNode(const E& x) : e(x) {}
+    AETHER_DEVICEHOST() explicit QuatConjNode(const E& x) : e(x) {}
     void emit(std::size_t c, detail::TapeCtx& x) const
     {
         e.emit(c, x);
         if (c > 0) x.op(AETH_OP_NEG);
     }
-    element_type eval(std::size_t c, std::size_t i) const { return c > 0 ? -e.eval(c, i) : e.eval(c, i); }
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const { return c > 0 ? -e.eval(c, i) : e.eval(c, i); }
 };
 
 /** Quaternion.h:117-170: t = 2 (u x v); v' = v + w t + u x t, each sum as written there */
@@ -539,7 +616,7 @@ struct QuatRotateNode : Expr<QuatRotateNode<Q, V>> {
     static_assert(Q::ncomp == 4 && V::ncomp == 3, "aether: quatRotate needs a quaternion and a 3-vector");
     Q q;
     V v;
-    QuatRotateNode(const Q& a, const V& b) : q(a), v(b) {}
+    AETHER_DEVICEHOST() QuatRotateNode(const Q& a, const V& b) : q(a), v(b) {}
     // t_c = 2 * (u[I] v[J] - u[J] v[I]) with u = q[1..3]
     static constexpr std::size_t I[3] = {1, 2, 0}, J[3] = {2, 0, 1};
     void emit_t(std::size_t c, detail::TapeCtx& x) const
@@ -570,11 +647,11 @@ struct QuatRotateNode : Expr<QuatRotateNode<Q, V>> {
         x.op(AETH_OP_SUB);
         x.op(AETH_OP_ADD);
     }
-    element_type t(std::size_t c, std::size_t i) const
+    AETHER_DEVICEHOST() element_type t(std::size_t c, std::size_t i) const
     {
         return element_type(2) * (q.eval(1 + I[c], i) * v.eval(J[c], i) - q.eval(1 + J[c], i) * v.eval(I[c], i));
     }
-    element_type eval(std::size_t c, std::size_t i) const
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const
     {
         return v.eval(c, i) + q.eval(0, i) * t(c, i) + (q.eval(1 + I[c], i) * t(J[c], i) - q.eval(1 + J[c], i) * t(I[c], i));
     }
@@ -587,13 +664,13 @@ struct PureQuatNode : Expr<PureQuatNode<E>> {
     static constexpr std::size_t ncomp = 4;
     static_assert(E::ncomp == 3, "aether: asPureQuaternion needs a 3-vector");
     E e;
-    explicit PureQuatNode(const E& x) : e(x) {}
+    AETHER_DEVICEHOST() explicit PureQuatNode(const E& x) : e(x) {}
     void emit(std::size_t c, detail::TapeCtx& x) const
     {
         if (c == 0) x.constant(0.0);
         else e.emit(c - 1, x);
     }
-    element_type eval(std::size_t c, std::size_t i) const { return c == 0 ? element_type(0) : e.eval(c - 1, i); }
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const { return c == 0 ? element_type(0) : e.eval(c - 1, i); }
 };
 
 /** expr/nodes/Structural.h Segment: components OFF .. OFF + LEN - 1 */
@@ -603,9 +680,9 @@ struct SegmentNode : Expr<SegmentNode<E, OFF, LEN>> {
     static constexpr std::size_t ncomp = LEN;
     static_assert(OFF + LEN <= E::ncomp && LEN > 0, "aether: segment out of range");
     E e;
-    explicit SegmentNode(const E& x) : e(x) {}
+    AETHER_DEVICEHOST() explicit SegmentNode(const E& x) : e(x) {}
     void emit(std::size_t c, detail::TapeCtx& x) const { e.emit(OFF + c, x); }
-    element_type eval(std::size_t c, std::size_t i) const { return e.eval(OFF + c, i); }
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const { return e.eval(OFF + c, i); }
 };
 
 /** expr/Reduce.h:172-188: (x_{N-1} y_{N-1}) + (... + (x_0 y_0)) */
@@ -616,7 +693,7 @@ struct DotNode : Expr<DotNode<L, R>> {
     static_assert(L::ncomp == R::ncomp, "aether: dot operands differ in size");
     L l;
     R r;
-    DotNode(const L& a, const R& b) : l(a), r(b) {}
+    AETHER_DEVICEHOST() DotNode(const L& a, const R& b) : l(a), r(b) {}
     void emit(std::size_t, detail::TapeCtx& x) const
     {
         for (std::size_t k = L::ncomp; k-- > 0;) {
@@ -626,7 +703,7 @@ struct DotNode : Expr<DotNode<L, R>> {
         }
         for (std::size_t k = 1; k < L::ncomp; ++k) x.op(AETH_OP_ADD);
     }
-    element_type eval(std::size_t, std::size_t i) const
+    AETHER_DEVICEHOST() element_type eval(std::size_t, std::size_t i) const
     {
         element_type acc = l.eval(0, i) * r.eval(0, i);
         for (std::size_t k = 1; k < L::ncomp; ++k) acc = l.eval(k, i) * r.eval(k, i) + acc;
@@ -642,7 +719,7 @@ struct CrossNode : Expr<CrossNode<L, R>> {
     static_assert(L::ncomp == 3 && R::ncomp == 3, "aether: cross needs 3-vectors");
     L l;
     R r;
-    CrossNode(const L& a, const R& b) : l(a), r(b) {}
+    AETHER_DEVICEHOST() CrossNode(const L& a, const R& b) : l(a), r(b) {}
     static constexpr std::size_t I[3] = {1, 2, 0}, J[3] = {2, 0, 1};
     void emit(std::size_t c, detail::TapeCtx& x) const
     {
@@ -654,7 +731,7 @@ struct CrossNode : Expr<CrossNode<L, R>> {
         x.op(AETH_OP_MUL);
         x.op(AETH_OP_SUB);
     }
-    element_type eval(std::size_t c, std::size_t i) const
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const
     {
         return l.eval(I[c], i) * r.eval(J[c], i) - l.eval(J[c], i) * r.eval(I[c], i);
     }
@@ -662,180 +739,180 @@ struct CrossNode : Expr<CrossNode<L, R>> {
 
 // Scalars wrap into ScalarNode of the other operand's element type.
 template<class E, class S>
-auto wrap_scalar(const S& s)
+AETHER_DEVICEHOST() auto wrap_scalar(const S& s)
 {
     return ScalarNode<elem_t<E>>((elem_t<E>)s);
 }
 
 template<class L, class R>
     requires aether_expression<L> && aether_expression<R>
-auto operator+(const L& l, const R& r)
+AETHER_DEVICEHOST() auto operator+(const L& l, const R& r)
 {
     return BinaryNode<L, R, AETH_OP_ADD>(l, r);
 }
 template<class L, class R>
     requires aether_expression<L> && aether_expression<R>
-auto operator-(const L& l, const R& r)
+AETHER_DEVICEHOST() auto operator-(const L& l, const R& r)
 {
     return BinaryNode<L, R, AETH_OP_SUB>(l, r);
 }
 template<class L, class R>
     requires aether_expression<L> && aether_expression<R>
-auto operator*(const L& l, const R& r)
+AETHER_DEVICEHOST() auto operator*(const L& l, const R& r)
 {
     return BinaryNode<L, R, AETH_OP_MUL>(l, r);
 }
 template<class L, class R>
     requires aether_expression<L> && aether_expression<R>
-auto operator/(const L& l, const R& r)
+AETHER_DEVICEHOST() auto operator/(const L& l, const R& r)
 {
     return BinaryNode<L, R, AETH_OP_DIV>(l, r);
 }
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto operator+(const E& e, S s)
+AETHER_DEVICEHOST() auto operator+(const E& e, S s)
 {
     return e + wrap_scalar<E>(s);
 }
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto operator+(S s, const E& e)
+AETHER_DEVICEHOST() auto operator+(S s, const E& e)
 {
     return wrap_scalar<E>(s) + e;
 }
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto operator-(const E& e, S s)
+AETHER_DEVICEHOST() auto operator-(const E& e, S s)
 {
     return e - wrap_scalar<E>(s);
 }
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto operator-(S s, const E& e)
+AETHER_DEVICEHOST() auto operator-(S s, const E& e)
 {
     return wrap_scalar<E>(s) - e;
 }
 /** CWiseScale (expr/Operations.h:49-59): s * e */
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto operator*(S s, const E& e)
+AETHER_DEVICEHOST() auto operator*(S s, const E& e)
 {
     return wrap_scalar<E>(s) * e;
 }
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto operator*(const E& e, S s)
+AETHER_DEVICEHOST() auto operator*(const E& e, S s)
 {
     return e * wrap_scalar<E>(s);
 }
 /** expr/Operations.h:62-71: e / s is e * (1 / s), the reciprocal rounded in T */
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto operator/(const E& e, S s)
+AETHER_DEVICEHOST() auto operator/(const E& e, S s)
 {
     return e * wrap_scalar<E>(elem_t<E>(1) / (elem_t<E>)s);
 }
 /** -x is x * -1 (docs/content/numerics_contract.rst:66) */
 template<class E>
     requires aether_expression<E>
-auto operator-(const E& e)
+AETHER_DEVICEHOST() auto operator-(const E& e)
 {
     return e * wrap_scalar<E>(-1);
 }
 
 template<class E>
     requires aether_expression<E>
-auto cwiseSin(const E& e) { return UnaryNode<E, AETH_OP_SIN>(e); }
+AETHER_DEVICEHOST() auto cwiseSin(const E& e) { return UnaryNode<E, AETH_OP_SIN>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseCos(const E& e) { return UnaryNode<E, AETH_OP_COS>(e); }
+AETHER_DEVICEHOST() auto cwiseCos(const E& e) { return UnaryNode<E, AETH_OP_COS>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseExp(const E& e) { return UnaryNode<E, AETH_OP_EXP>(e); }
+AETHER_DEVICEHOST() auto cwiseExp(const E& e) { return UnaryNode<E, AETH_OP_EXP>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseLog(const E& e) { return UnaryNode<E, AETH_OP_LOG>(e); }
+AETHER_DEVICEHOST() auto cwiseLog(const E& e) { return UnaryNode<E, AETH_OP_LOG>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseSqrt(const E& e) { return UnaryNode<E, AETH_OP_SQRT>(e); }
+AETHER_DEVICEHOST() auto cwiseSqrt(const E& e) { return UnaryNode<E, AETH_OP_SQRT>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseAbs(const E& e) { return UnaryNode<E, AETH_OP_ABS>(e); }
+AETHER_DEVICEHOST() auto cwiseAbs(const E& e) { return UnaryNode<E, AETH_OP_ABS>(e); }
 template<class E>
     requires aether_expression<E>
-auto sqrt(const E& e) { return UnaryNode<E, AETH_OP_SQRT>(e); }
+AETHER_DEVICEHOST() auto sqrt(const E& e) { return UnaryNode<E, AETH_OP_SQRT>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseSign(const E& e) { return UnaryNode<E, AETH_OP_SIGN>(e); }
+AETHER_DEVICEHOST() auto cwiseSign(const E& e) { return UnaryNode<E, AETH_OP_SIGN>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseTan(const E& e) { return UnaryNode<E, AETH_OP_TAN>(e); }
+AETHER_DEVICEHOST() auto cwiseTan(const E& e) { return UnaryNode<E, AETH_OP_TAN>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseAtan(const E& e) { return UnaryNode<E, AETH_OP_ATAN>(e); }
+AETHER_DEVICEHOST() auto cwiseAtan(const E& e) { return UnaryNode<E, AETH_OP_ATAN>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseAsin(const E& e) { return UnaryNode<E, AETH_OP_ASIN>(e); }
+AETHER_DEVICEHOST() auto cwiseAsin(const E& e) { return UnaryNode<E, AETH_OP_ASIN>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseAcos(const E& e) { return UnaryNode<E, AETH_OP_ACOS>(e); }
+AETHER_DEVICEHOST() auto cwiseAcos(const E& e) { return UnaryNode<E, AETH_OP_ACOS>(e); }
 template<class E>
     requires aether_expression<E>
-auto cwiseTanh(const E& e) { return UnaryNode<E, AETH_OP_TANH>(e); }
+AETHER_DEVICEHOST() auto cwiseTanh(const E& e) { return UnaryNode<E, AETH_OP_TANH>(e); }
 /** math/root.h:74 (host): T(1) / sqrt(x), two correctly rounded operations */
 template<class E>
     requires aether_expression<E>
-auto cwiseRsqrt(const E& e) { return wrap_scalar<E>(1) / UnaryNode<E, AETH_OP_SQRT>(e); }
+AETHER_DEVICEHOST() auto cwiseRsqrt(const E& e) { return wrap_scalar<E>(1) / UnaryNode<E, AETH_OP_SQRT>(e); }
 /** Elementwise.h:198-210: min / max against a broadcast bound */
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto cwiseMin(const E& e, S bound)
+AETHER_DEVICEHOST() auto cwiseMin(const E& e, S bound)
 {
     return BinaryNode<E, ScalarNode<elem_t<E>>, AETH_OP_FMIN>(e, wrap_scalar<E>(bound));
 }
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto cwiseMax(const E& e, S bound)
+AETHER_DEVICEHOST() auto cwiseMax(const E& e, S bound)
 {
     return BinaryNode<E, ScalarNode<elem_t<E>>, AETH_OP_FMAX>(e, wrap_scalar<E>(bound));
 }
 /** Elementwise.h:761-775: min / max of two expressions */
 template<class L, class R>
     requires aether_expression<L> && aether_expression<R>
-auto cwiseMin(const L& l, const R& r)
+AETHER_DEVICEHOST() auto cwiseMin(const L& l, const R& r)
 {
     return BinaryNode<L, R, AETH_OP_FMIN>(l, r);
 }
 template<class L, class R>
     requires aether_expression<L> && aether_expression<R>
-auto cwiseMax(const L& l, const R& r)
+AETHER_DEVICEHOST() auto cwiseMax(const L& l, const R& r)
 {
     return BinaryNode<L, R, AETH_OP_FMAX>(l, r);
 }
 /** Elementwise.h:219-224: fmin(fmax(e, lo), hi) */
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto clamp(const E& e, S lo, S hi)
+AETHER_DEVICEHOST() auto clamp(const E& e, S lo, S hi)
 {
     return cwiseMin(cwiseMax(e, lo), hi);
 }
 /** Elementwise.h:742-746: l / r, both expressions */
 template<class L, class R>
     requires aether_expression<L> && aether_expression<R>
-auto cwiseDiv(const L& l, const R& r)
+AETHER_DEVICEHOST() auto cwiseDiv(const L& l, const R& r)
 {
     return BinaryNode<L, R, AETH_OP_DIV>(l, r);
 }
 /** Elementwise.h:749-760: e ^ exponent (a broadcast scalar) or e ^ r */
 template<class E, class S>
     requires aether_expression<E> && std::is_arithmetic_v<S>
-auto cwisePow(const E& e, S exponent)
+AETHER_DEVICEHOST() auto cwisePow(const E& e, S exponent)
 {
     return BinaryNode<E, ScalarNode<elem_t<E>>, AETH_OP_POW>(e, wrap_scalar<E>(exponent));
 }
 template<class L, class R>
     requires aether_expression<L> && aether_expression<R>
-auto cwisePow(const L& l, const R& r)
+AETHER_DEVICEHOST() auto cwisePow(const L& l, const R& r)
 {
     return BinaryNode<L, R, AETH_OP_POW>(l, r);
 }
@@ -852,7 +929,7 @@ struct ViewLeaf : Expr<ViewLeaf<T, N>> {
     T* p;
     std::size_t pitch;
     void emit(std::size_t c, detail::TapeCtx& x) const { x.push(p, pitch, c); }
-    element_type eval(std::size_t c, std::size_t i) const { return p[c * pitch + i]; }
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const { return p[c * pitch + i]; }
 };
 
 enum class SumMode : int { None = AETH_SUM_NONE, Neumaier = AETH_SUM_NEUMAIER, Plain = AETH_SUM_PLAIN,
@@ -862,26 +939,26 @@ enum class SumMode : int { None = AETH_SUM_NONE, Neumaier = AETH_SUM_NEUMAIER, P
 template<class T, std::size_t N>
 class SampleRef {
 public:
-    SampleRef(T* p, std::size_t pitch, std::size_t i) : p_(p), pitch_(pitch), i_(i) {}
+    AETHER_DEVICEHOST() SampleRef(T* p, std::size_t pitch, std::size_t i) : p_(p), pitch_(pitch), i_(i) {}
     template<class E>
-    SampleRef& operator=(const E& e)
+    AETHER_DEVICEHOST() SampleRef& operator=(const E& e)
     {
         for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] = (T)e.eval(E::ncomp > 1 ? c : 0, i_);
         return *this;
     }
     template<class E>
-    SampleRef& operator+=(const E& e)
+    AETHER_DEVICEHOST() SampleRef& operator+=(const E& e)
     {
         for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] += (T)e.eval(E::ncomp > 1 ? c : 0, i_);
         return *this;
     }
     template<class E>
-    SampleRef& operator-=(const E& e)
+    AETHER_DEVICEHOST() SampleRef& operator-=(const E& e)
     {
         for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] -= (T)e.eval(E::ncomp > 1 ? c : 0, i_);
         return *this;
     }
-    std::array<T, N> get() const
+    AETHER_DEVICEHOST() std::array<T, N> get() const
     {
         std::array<T, N> r{};
         for (std::size_t c = 0; c < N; ++c) r[c] = p_[c * pitch_ + i_];
@@ -905,21 +982,21 @@ public:
     using element_type = T;
     static constexpr std::size_t ncomp = N;
 
-    View(T* p, std::size_t samples, std::size_t pitch, bool device)
+    AETHER_DEVICEHOST() View(T* p, std::size_t samples, std::size_t pitch, bool device)
         : p_(p), samples_(samples), pitch_(pitch), device_(device)
     {
     }
     View(const View&) = default;
-    T* data() const { return p_; }
-    std::size_t samples() const { return samples_; }
-    std::size_t pitch() const { return pitch_; }
-    bool on_device() const { return device_; }
-    T& operator()(std::size_t c, std::size_t i) const { return p_[c * pitch_ + i]; }
-    T& operator()(std::size_t i) const { return p_[i]; }
-    SampleRef<T, N> operator[](SampleIndex i) const { return SampleRef<T, N>(p_, pitch_, i.global()); }
+    AETHER_DEVICEHOST() T* data() const { return p_; }
+    AETHER_DEVICEHOST() std::size_t samples() const { return samples_; }
+    AETHER_DEVICEHOST() std::size_t pitch() const { return pitch_; }
+    AETHER_DEVICEHOST() bool on_device() const { return device_; }
+    AETHER_DEVICEHOST() T& operator()(std::size_t c, std::size_t i) const { return p_[c * pitch_ + i]; }
+    AETHER_DEVICEHOST() T& operator()(std::size_t i) const { return p_[i]; }
+    AETHER_DEVICEHOST() SampleRef<T, N> operator[](SampleIndex i) const { return SampleRef<T, N>(p_, pitch_, i.global()); }
 
     void emit(std::size_t c, detail::TapeCtx& x) const { x.push(p_, pitch_, c); }
-    T eval(std::size_t c, std::size_t i) const { return p_[c * pitch_ + i]; }
+    AETHER_DEVICEHOST() T eval(std::size_t c, std::size_t i) const { return p_[c * pitch_ + i]; }
 
     /** Whole-array lazy assignment: one kernel (device views only). */
     template<class E>
@@ -1535,8 +1612,20 @@ struct DistLeaf : Expr<DistLeaf<Real, VD>> {
     double p0, p1;
     std::uint32_t dist;
     void emit(std::size_t c, detail::TapeCtx& x) const { x.rng(seed, p0, p1, (std::uint32_t)c, dist); }
-    Real eval(std::size_t c, std::size_t i) const
+    AETHER_DEVICEHOST() Real eval(std::size_t c, std::size_t i) const
     {
+#if defined(__HIP_DEVICE_COMPILE__)
+        // the tape kernels' draw: the same Philox block, conversions and device math
+        const aeth::rng::U4 d = aeth::rng::block(seed, (std::uint32_t)i, c);
+        if constexpr (std::is_same_v<Real, double>) {
+            if (dist == AETH_DIST_UNIFORM) return p0 + (p1 - p0) * aeth::rng::u01_from_word_d(d.x);
+            return p0 + p1 * aeth::rng::normal_from_block_d(d);
+        } else {
+            const float q0 = (float)p0, q1 = (float)p1;
+            if (dist == AETH_DIST_UNIFORM) return q0 + (q1 - q0) * aeth::rng::u01_from_word_f(d.x);
+            return q0 + q1 * aeth::rng::normal_from_block_f(d);
+        }
+#else
         const auto b = detail::philox_block(seed, (std::uint32_t)i, c);
         if (dist == AETH_DIST_UNIFORM) {
             if constexpr (std::is_same_v<Real, double>) return p0 + (p1 - p0) * ((double)b[0] * 0x1p-32 + 0x1p-33);
@@ -1555,6 +1644,7 @@ struct DistLeaf : Expr<DistLeaf<Real, VD>> {
             const float u = (float)(((b[0] >> 9) << 1) | 1u) * 0x1p-24f, v = (float)b[1] * k + (k / 2.0f);
             return (float)p0 + (float)p1 * (std::sqrt(-2.0f * std::log(u)) * std::sin(v));
         }
+#endif
     }
 };
 
@@ -1572,7 +1662,7 @@ public:
         return g;
     }
     template<class Real, std::size_t VD>
-    DistLeaf<Real, VD> uniform(Real lo, Real hi) const
+    AETHER_DEVICEHOST() DistLeaf<Real, VD> uniform(Real lo, Real hi) const
     {
         DistLeaf<Real, VD> l;
         l.seed = key_ ^ 0x243F6A8885A308D3ULL;
@@ -1582,7 +1672,7 @@ public:
         return l;
     }
     template<class Real, std::size_t VD>
-    DistLeaf<Real, VD> normal(Real mean, Real sd) const
+    AETHER_DEVICEHOST() DistLeaf<Real, VD> normal(Real mean, Real sd) const
     {
         DistLeaf<Real, VD> l;
         l.seed = key_ ^ 0x13198A2E03707344ULL;
