@@ -390,6 +390,10 @@ struct SegmentNode;
  *  the folds keep the reference's order (expr/Reduce.h:150-164: acc = op(T(x_{k+1}), acc)). */
 template<class D>
 struct Expr {
+    /** Whether the tree lowers to a tape (every built-in node does; a node type of the caller's own, which
+     *  derives from Expr and provides `ncomp`, `element_type` and an AETHER_DEVICEHOST() `eval(c, i)`,
+     *  does not: it assigns to a whole device view through a kernel of the caller's HIP translation unit). */
+    static constexpr bool lowerable = false;
     AETHER_DEVICEHOST() const D& self() const { return static_cast<const D&>(*this); }
     template<class E>
     AETHER_DEVICEHOST() auto dot(const E& o) const
@@ -475,6 +479,7 @@ template<class T>
 struct ScalarNode : Expr<ScalarNode<T>> {
     using element_type = T;
     static constexpr std::size_t ncomp = 1;
+    static constexpr bool lowerable = true;
     T v;
     AETHER_DEVICEHOST() explicit ScalarNode(T x) : v(x) {}
     void emit(std::size_t, detail::TapeCtx& c) const { c.constant((double)v); }
@@ -485,6 +490,7 @@ template<class L, class R, int OP>
 struct BinaryNode : Expr<BinaryNode<L, R, OP>> {
     using element_type = elem_t<L>;
     static constexpr std::size_t ncomp = L::ncomp > R::ncomp ? L::ncomp : R::ncomp;
+    static constexpr bool lowerable = L::lowerable && R::lowerable;
     static_assert(L::ncomp == R::ncomp || L::ncomp == 1 || R::ncomp == 1, "aether: component counts differ");
     static_assert(std::is_same_v<elem_t<L>, elem_t<R>>, "aether: element types differ");
     L l;
@@ -507,6 +513,7 @@ template<class E, int FN>
 struct UnaryNode : Expr<UnaryNode<E, FN>> {
     using element_type = elem_t<E>;
     static constexpr std::size_t ncomp = E::ncomp;
+    static constexpr bool lowerable = E::lowerable;
     E e;
     AETHER_DEVICEHOST() explicit UnaryNode(const E& x) : e(x) {}
     void emit(std::size_t c, detail::TapeCtx& x) const
@@ -524,6 +531,7 @@ template<class E, bool MAX>
 struct FoldNode : Expr<FoldNode<E, MAX>> {
     using element_type = elem_t<E>;
     static constexpr std::size_t ncomp = 1;
+    static constexpr bool lowerable = E::lowerable;
     E e;
     AETHER_DEVICEHOST() explicit FoldNode(const E& x) : e(x) {}
     void emit(std::size_t, detail::TapeCtx& x) const
@@ -561,6 +569,7 @@ template<class L, class R>
 struct QuatMulNode : Expr<QuatMulNode<L, R>> {
     using element_type = elem_t<L>;
     static constexpr std::size_t ncomp = 4;
+    static constexpr bool lowerable = L::lowerable && R::lowerable;
     static_assert(L::ncomp == 4 && R::ncomp == 4, "aether: quatMul needs quaternions");
     L l;
     R r;
@@ -597,6 +606,7 @@ template<class E>
 struct QuatConjNode : Expr<QuatConjNode<E>> {
     using element_type = elem_t<E>;
     static constexpr std::size_t ncomp = 4;
+    static constexpr bool lowerable = E::lowerable;
     static_assert(E::ncomp == 4, "aether: quatConj needs a quaternion");
     E e;
     AETHER_DEVICEHOST() explicit QuatConjNode(const E& x) : e(x) {}
@@ -613,6 +623,7 @@ template<class Q, class V>
 struct QuatRotateNode : Expr<QuatRotateNode<Q, V>> {
     using element_type = elem_t<Q>;
     static constexpr std::size_t ncomp = 3;
+    static constexpr bool lowerable = Q::lowerable && V::lowerable;
     static_assert(Q::ncomp == 4 && V::ncomp == 3, "aether: quatRotate needs a quaternion and a 3-vector");
     Q q;
     V v;
@@ -662,6 +673,7 @@ template<class E>
 struct PureQuatNode : Expr<PureQuatNode<E>> {
     using element_type = elem_t<E>;
     static constexpr std::size_t ncomp = 4;
+    static constexpr bool lowerable = E::lowerable;
     static_assert(E::ncomp == 3, "aether: asPureQuaternion needs a 3-vector");
     E e;
     AETHER_DEVICEHOST() explicit PureQuatNode(const E& x) : e(x) {}
@@ -678,6 +690,7 @@ template<class E, std::size_t OFF, std::size_t LEN>
 struct SegmentNode : Expr<SegmentNode<E, OFF, LEN>> {
     using element_type = elem_t<E>;
     static constexpr std::size_t ncomp = LEN;
+    static constexpr bool lowerable = E::lowerable;
     static_assert(OFF + LEN <= E::ncomp && LEN > 0, "aether: segment out of range");
     E e;
     AETHER_DEVICEHOST() explicit SegmentNode(const E& x) : e(x) {}
@@ -690,6 +703,7 @@ template<class L, class R>
 struct DotNode : Expr<DotNode<L, R>> {
     using element_type = elem_t<L>;
     static constexpr std::size_t ncomp = 1;
+    static constexpr bool lowerable = L::lowerable && R::lowerable;
     static_assert(L::ncomp == R::ncomp, "aether: dot operands differ in size");
     L l;
     R r;
@@ -716,6 +730,7 @@ template<class L, class R>
 struct CrossNode : Expr<CrossNode<L, R>> {
     using element_type = elem_t<L>;
     static constexpr std::size_t ncomp = 3;
+    static constexpr bool lowerable = L::lowerable && R::lowerable;
     static_assert(L::ncomp == 3 && R::ncomp == 3, "aether: cross needs 3-vectors");
     L l;
     R r;
@@ -926,6 +941,7 @@ template<class T, std::size_t N>
 struct ViewLeaf : Expr<ViewLeaf<T, N>> {
     using element_type = std::remove_const_t<T>;
     static constexpr std::size_t ncomp = N;
+    static constexpr bool lowerable = true;
     T* p;
     std::size_t pitch;
     void emit(std::size_t c, detail::TapeCtx& x) const { x.push(p, pitch, c); }
@@ -970,6 +986,27 @@ private:
     std::size_t pitch_, i_;
 };
 
+namespace detail {
+
+template<class E>
+inline constexpr bool always_false = false;
+
+#if defined(__HIPCC__)
+/** dest = e for a tree holding a node type of the caller's own: one sample per lane, grid-stride, every
+ *  component evaluated before any is stored; instantiated in the caller's HIP translation unit. */
+template<class T, std::size_t N, class E>
+__global__ __launch_bounds__(256) void assign_kernel(T* p, std::size_t pitch, std::uint32_t n, E e)
+{
+    for (std::uint64_t i = (std::uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (std::uint64_t)gridDim.x * 256u) {
+        T out[N];
+        for (std::size_t c = 0; c < N; ++c) out[c] = (T)e.eval(E::ncomp > 1 ? c : 0, (std::size_t)i);
+        for (std::size_t c = 0; c < N; ++c) p[c * pitch + i] = out[c];
+    }
+}
+#endif
+
+} // namespace detail
+
 /**
  * View<T, N>: `samples` items of N components, plane pitch `pitch`
  * (layout_stride with strides {pitch, 1}). A device view assigned an
@@ -981,6 +1018,7 @@ class View : public Expr<View<T, N>> {
 public:
     using element_type = T;
     static constexpr std::size_t ncomp = N;
+    static constexpr bool lowerable = true;
 
     AETHER_DEVICEHOST() View(T* p, std::size_t samples, std::size_t pitch, bool device)
         : p_(p), samples_(samples), pitch_(pitch), device_(device)
@@ -1032,25 +1070,65 @@ public:
     {
         static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
         if (!device_) throw Error("aether: whole-array assignment needs a device view");
-        const detail::TapeCtx x = detail::lower<T, N>(p_, pitch_, e);
+        if constexpr (!E::lowerable) {
+            assign_generic(e, mode, pairs_host, stream);
+        } else {
+            const detail::TapeCtx x = detail::lower<T, N>(p_, pitch_, e);
+            void* sums_dev = nullptr;
+            void* chunk = nullptr;
+            const int dev = detail::current_device();
+            if (mode != SumMode::None)
+                detail::check(detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, 2 * N * sizeof(T), stream, &chunk,
+                                                         &sums_dev),
+                              "chunk_alloc");
+            const std::vector<aeth_program_ex> progs = x.ex();
+            const int rc = aeth_eval_tape_ex(x.dtype, (int)N, progs.data(), (int)x.leaves.size(),
+                                             const_cast<void* const*>(x.leaves.data()), x.pitch.data(),
+                                             (int)x.consts.size(), x.consts.data(), (int)x.rngs.size(), x.rngs.data(),
+                                             samples_, (int)mode, sums_dev, stream);
+            if (rc == 0 && mode != SumMode::None)
+                detail::check(detail::seam().copy(AETHER_MEM_CPU, dev, pairs_host, AETHER_MEM_DEVICE, dev, sums_dev,
+                                                  2 * N * sizeof(T), stream),
+                              "copy");
+            if (chunk) detail::seam().chunk_release(chunk);
+            detail::check(rc, "aeth_eval_tape");
+        }
+    }
+
+    /** dest = e for a tree holding a node type of the caller's own (no tape lowering): a kernel of the
+     *  caller's HIP translation unit evaluates it per sample, and the sums (when asked) are the library's
+     *  reduction of dest in the fixed shape, which is bit for bit the fused sum of the same values. */
+    template<class E>
+    void assign_generic(const E& e, SumMode mode, T* pairs_host, Stream stream)
+    {
+#if defined(__HIPCC__)
+        if (samples_ > 0xFFFFFFFFull) throw Error("aether: a view of more than 2^32-1 samples");
+        if (samples_) {
+            const std::size_t blocks = std::min<std::size_t>((samples_ + 255) / 256, 16384);
+            hipLaunchKernelGGL((detail::assign_kernel<T, N, E>), dim3((unsigned)blocks), dim3(256), 0,
+                               reinterpret_cast<hipStream_t>(stream), p_, pitch_, (std::uint32_t)samples_, e);
+            const hipError_t err = hipGetLastError();
+            if (err != hipSuccess) throw Error(std::string("aether: assignment kernel launch: ") + hipGetErrorString(err));
+        }
+        if (mode == SumMode::None) return;
         void* sums_dev = nullptr;
         void* chunk = nullptr;
         const int dev = detail::current_device();
-        if (mode != SumMode::None)
-            detail::check(detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, 2 * N * sizeof(T), stream, &chunk,
-                                                     &sums_dev),
-                          "chunk_alloc");
-        const std::vector<aeth_program_ex> progs = x.ex();
-        const int rc = aeth_eval_tape_ex(x.dtype, (int)N, progs.data(), (int)x.leaves.size(),
-                                         const_cast<void* const*>(x.leaves.data()), x.pitch.data(),
-                                         (int)x.consts.size(), x.consts.data(), (int)x.rngs.size(), x.rngs.data(),
-                                         samples_, (int)mode, sums_dev, stream);
-        if (rc == 0 && mode != SumMode::None)
+        detail::check(detail::seam().chunk_alloc(AETHER_MEM_DEVICE, dev, 2 * N * sizeof(T), stream, &chunk, &sums_dev),
+                      "chunk_alloc");
+        const int rc = aeth_reduce(detail::dtype_code<T>(), (int)mode, 0, p_, (int)N, samples_, pitch_, sums_dev, stream);
+        if (rc == 0)
             detail::check(detail::seam().copy(AETHER_MEM_CPU, dev, pairs_host, AETHER_MEM_DEVICE, dev, sums_dev,
                                               2 * N * sizeof(T), stream),
                           "copy");
-        if (chunk) detail::seam().chunk_release(chunk);
-        detail::check(rc, "aeth_eval_tape");
+        detail::seam().chunk_release(chunk);
+        detail::check(rc, "aeth_reduce");
+#else
+        (void)e, (void)mode, (void)pairs_host, (void)stream;
+        static_assert(detail::always_false<E>, "aether: an expression holding a node type of the caller's own "
+                                               "assigns to a whole device view only in a HIP translation unit "
+                                               "(hipcc); a host translation unit lowers built-in nodes only");
+#endif
     }
 
     /** Autotune the launch shape of dest = e (aeth_tape_tune); this view holds the result afterwards. With a
@@ -1060,6 +1138,7 @@ public:
     aeth_tape_shape tune(const E& e, Stream stream = nullptr, SumMode mode = SumMode::None)
     {
         static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
+        static_assert(E::lowerable, "aether: tuning takes a tree of built-in nodes (it tunes the runtime-compiled kernel)");
         if (!device_) throw Error("aether: tuning needs a device view");
         const detail::TapeCtx x = detail::lower<T, N>(p_, pitch_, e);
         if (x.programs.empty()) throw Error("aether: tuning takes an expression that fits one launch");
@@ -1608,6 +1687,7 @@ template<class Real, std::size_t VD>
 struct DistLeaf : Expr<DistLeaf<Real, VD>> {
     using element_type = Real;
     static constexpr std::size_t ncomp = VD;
+    static constexpr bool lowerable = true;
     std::uint64_t seed; // salted
     double p0, p1;
     std::uint32_t dist;

@@ -2,7 +2,8 @@
 // kernels (`dv[SampleIndex::make(threadIdx.x, blockIdx.x, blockDim.x)] = expr`, expr/Assign.h:251-346 on the
 // device), compared bit for bit with the same expression assigned to the whole view (`dv = expr`: one
 // runtime-compiled kernel). Covers the fused benchmark tree and the Vec3 tree in double and float, the
-// elementwise functions, folds and norms, quaternions, Philox distribution leaves, `+=` / `-=` and `get()`.
+// elementwise functions, folds and norms, quaternions, Philox distribution leaves, `+=` / `-=`, `get()`, and a
+// node type of the caller's own (per sample, as a whole-array assignment and with fused sums).
 // Built by the package build with hipcc against libaether_hip.so; prints "ok" or the first mismatch, exit 1.
 #include <aether/aether.hpp>
 
@@ -51,6 +52,26 @@ void expect_same(const char* what, Array<T, N>& got, Array<T, N>& want)
         std::printf("mismatch: %s\n", what);
         ++failures;
     }
+}
+
+// ---- a node type of the caller's own: smoothstep x * x * (3 - 2 x), per component
+template<class E>
+struct Smoothstep : aether::Expr<Smoothstep<E>> {
+    using element_type = aether::elem_t<E>;
+    static constexpr std::size_t ncomp = E::ncomp;
+    E e;
+    AETHER_DEVICEHOST() explicit Smoothstep(const E& x) : e(x) {}
+    AETHER_DEVICEHOST() element_type eval(std::size_t c, std::size_t i) const
+    {
+        const element_type x = e.eval(c, i);
+        return x * x * (element_type(3) - element_type(2) * x);
+    }
+};
+
+template<class E>
+AETHER_DEVICEHOST() Smoothstep<E> smoothstep(const E& e)
+{
+    return Smoothstep<E>(e);
 }
 
 // ---- the caller's own kernels
@@ -117,6 +138,13 @@ __global__ void update_kernel(View<T, 3> d, View<T, 3> a, View<T, 3> b, std::uin
         d[i] += a * b;
         d[i] -= aether::cwiseExp(T(0.5) * b);
     }
+}
+
+template<class T>
+__global__ void user_node_kernel(View<T, 3> d, View<T, 3> a, View<T, 3> b, std::uint32_t n)
+{
+    const SampleIndex i = SampleIndex::make(threadIdx.x, blockIdx.x, blockDim.x);
+    if (i.global() < n) d[i] = smoothstep(a) + b * aether::cwiseSin(smoothstep(b));
 }
 
 template<class T>
@@ -205,6 +233,23 @@ void run_cases(std::size_t n)
         want.deviceView() = wv - aether::cwiseExp(T(0.5) * bv);
         w = std::string("+= and -= ") + tn;
         expect_same(w.c_str(), got, want);
+    }
+    {   // a node type of the caller's own: per sample in a kernel, as a whole-array assignment (a kernel of this
+        // translation unit) and with fused sums, against the same arithmetic spelled with built-in nodes
+        Array<T, 3> got(n), whole(n), want(n);
+        hipLaunchKernelGGL(user_node_kernel<T>, grid_of(n), dim3(256), 0, 0, got.deviceView(), av, bv, n32);
+        launch_check("user_node_kernel");
+        const auto ss = [](auto x) { return x * x * (T(3) - T(2) * x); };
+        const auto ps = aether::evaluate_pairs(whole.deviceView(), smoothstep(av) + bv * aether::cwiseSin(smoothstep(bv)));
+        const auto pw = aether::evaluate_pairs(want.deviceView(), ss(av) + bv * aether::cwiseSin(ss(bv)));
+        w = std::string("user node, per sample ") + tn;
+        expect_same(w.c_str(), got, want);
+        w = std::string("user node, whole array ") + tn;
+        expect_same(w.c_str(), whole, want);
+        if (std::memcmp(&ps, &pw, sizeof ps) != 0) {
+            std::printf("mismatch: user node sums %s\n", tn);
+            ++failures;
+        }
     }
     {   // get(): the sample's components, read in the kernel
         Array<T, 1> got(n), want(n);
