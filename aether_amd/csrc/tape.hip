@@ -793,6 +793,15 @@ namespace {
 // without sums use the tuned shape of their structure and assignments with sums the contract shape.
 // Returns whether the assignment could run as one launch over all programs (no program reads a plane
 // another one writes).
+// Set while aeth_tape_prepare walks an assignment: every kernel the assignment would launch is compiled and
+// loaded into this process's cache; nothing is allocated, written or launched.
+thread_local bool t_prepare_only = false;
+
+struct PrepareOnly {
+    PrepareOnly() { t_prepare_only = true; }
+    ~PrepareOnly() { t_prepare_only = false; }
+};
+
 bool eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_leaves, void* const* leaves,
                const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs, const aeth_rng_leaf* rngs,
                uint64_t n, int sum_mode, void* sums, void* stream, const JitShape* force)
@@ -804,7 +813,7 @@ bool eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_le
         if (n_consts < 0 || n_consts > AETH_TAPE_MAX_CONSTS) fail("aeth_eval_tape", "0..16 constants");
         if (n_rngs < 0 || n_rngs > AETH_TAPE_MAX_RNGS) fail("aeth_eval_tape", "0..8 rng leaves");
         if (n > 0xFFFFFFFFull) fail("aeth_eval_tape", "n beyond the 32-bit offset_t");
-        if (sum_mode != AETH_SUM_NONE && !sums) fail("aeth_eval_tape", "sums is NULL");
+        if (sum_mode != AETH_SUM_NONE && !sums && !t_prepare_only) fail("aeth_eval_tape", "sums is NULL");
         const size_t es = dtype == AETH_F64 ? 8 : 4;
         for (int p = 0; p < n_programs; ++p) validate(programs[p], n_leaves, n_consts, n_rngs);
         bool vec = true;
@@ -867,6 +876,7 @@ bool eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_le
                 if (force) fail("aeth_tape_tune", "the runtime compiler is unavailable");
                 sh = JitShape{}; // the resident interpreter runs the contract shape
             }
+            if (t_prepare_only) return;
             A.tiles = n ? (uint32_t)((n + sh.tile - 1) / sh.tile) : 1u;
             A.units = A.tiles * (uint32_t)np;
             if (dtype == AETH_F64)
@@ -1150,7 +1160,7 @@ void eval_tape_ex(int dtype, int n_programs, const aeth_program_ex* programs, in
     if (n_leaves < 1 || !leaves || !leaf_pitch) fail("aeth_eval_tape_ex", "no leaves");
     if (n_consts < 0 || n_rngs < 0) fail("aeth_eval_tape_ex", "negative table size");
     if (n > 0xFFFFFFFFull) fail("aeth_eval_tape_ex", "n beyond the 32-bit offset_t");
-    if (sum_mode != AETH_SUM_NONE && !sums) fail("aeth_eval_tape_ex", "sums is NULL");
+    if (sum_mode != AETH_SUM_NONE && !sums && !t_prepare_only) fail("aeth_eval_tape_ex", "sums is NULL");
     for (int p = 0; p < n_programs; ++p) validate_ex(programs[p], n_leaves, n_consts, n_rngs);
     bool one = n_programs <= AETH_TAPE_MAX_PROGRAMS && n_leaves <= AETH_TAPE_MAX_LEAVES &&
                n_consts <= AETH_TAPE_MAX_CONSTS && n_rngs <= AETH_TAPE_MAX_RNGS;
@@ -1187,6 +1197,10 @@ void eval_tape_ex(int dtype, int n_programs, const aeth_program_ex* programs, in
         std::vector<std::unique_ptr<StreamScratch>> temps;
         std::vector<void*> temp_ptr(S.temps, nullptr);
         for (int t = 0; t < S.temps; ++t) {
+            if (t_prepare_only) { // no planes: a piece's kernel depends on the tape's structure, not its pointers
+                temp_ptr[t] = reinterpret_cast<void*>((uintptr_t)256 * (t + 1));
+                continue;
+            }
             temps.emplace_back(new StreamScratch(std::max<uint64_t>(temp_pitch, 32) * es, st));
             temp_ptr[t] = temps.back()->get();
         }
@@ -1229,6 +1243,17 @@ extern "C" int aeth_eval_tape_ex(int dtype, int n_programs, const aeth_program_e
     return guarded([&] {
         eval_tape_ex(dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs, rngs, n,
                      sum_mode, sums, stream);
+    });
+}
+
+extern "C" int aeth_tape_prepare(int dtype, int n_programs, const aeth_program_ex* programs, int n_leaves,
+                                 void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts,
+                                 int n_rngs, const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* stream)
+{
+    return guarded([&] {
+        PrepareOnly scope;
+        eval_tape_ex(dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs, rngs, n,
+                     sum_mode, nullptr, stream);
     });
 }
 

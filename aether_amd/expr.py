@@ -523,6 +523,16 @@ def tune(dest: Array, expr, stream=None, sum_mode: int = -1) -> dict:
     return {"tile": out.tile, "vector_bytes": out.vector_bytes, "waves_per_simd": out.waves_per_simd, "ms": out.ms}
 
 
+def prepare(dest: Array, expr, sum_mode: int = -1, stream=None):
+    """Compile and load every kernel `assign(dest, expr, sum_mode)` would launch, without running anything, so
+    the first real assignment pays no runtime compile (no counterpart in the reference, whose kernels are
+    compiled ahead of time)."""
+    progs, nl, ptrs, pitch, nc, cst, nr, rg, keep = _abi_args(dest, expr, unbounded=True)
+    call("aeth_tape_prepare", _dtype_code(dest.dtype), dest.ncomp, progs, nl, ptrs, pitch, nc, cst, nr, rg, dest.n,
+         sum_mode, stream_handle(stream))
+    del keep
+
+
 def assign(dest: Array, expr, sum_mode: int = -1, sums: torch.Tensor = None, stream=None):
     """`dest = expr` over all samples: ONE kernel when the tree fits one launch, otherwise the tree cut into
     pieces that do (same values); returns (ncomp, 2) sums when sum_mode >= 0."""

@@ -1095,6 +1095,25 @@ public:
         }
     }
 
+    /** Compile and load every kernel that assigning e (with `mode`) would launch, without running anything
+     *  (aeth_tape_prepare): the first assignment then pays no runtime compile. A tree holding a node type of
+     *  the caller's own has nothing to compile at run time. */
+    template<class E>
+    void prepare(const E& e, SumMode mode = SumMode::None, Stream stream = nullptr) const
+    {
+        static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and view component counts differ");
+        if (!device_) throw Error("aether: preparing an assignment needs a device view");
+        if constexpr (E::lowerable) {
+            const detail::TapeCtx x = detail::lower<T, N>(p_, pitch_, e);
+            const std::vector<aeth_program_ex> progs = x.ex();
+            detail::check(aeth_tape_prepare(x.dtype, (int)N, progs.data(), (int)x.leaves.size(),
+                                            const_cast<void* const*>(x.leaves.data()), x.pitch.data(),
+                                            (int)x.consts.size(), x.consts.data(), (int)x.rngs.size(),
+                                            x.rngs.data(), samples_, (int)mode, stream),
+                          "aeth_tape_prepare");
+        }
+    }
+
     /** dest = e for a tree holding a node type of the caller's own (no tape lowering): a kernel of the
      *  caller's HIP translation unit evaluates it per sample, and the sums (when asked) are the library's
      *  reduction of dest in the fixed shape, which is bit for bit the fused sum of the same values. */
@@ -1326,6 +1345,14 @@ std::array<std::array<T, 2>, N> evaluate_pairs(View<T, N> dest, const E& e, SumM
     std::array<std::array<T, 2>, N> out{};
     for (std::size_t c = 0; c < N; ++c) out[c] = {flat[2 * c], flat[2 * c + 1]};
     return out;
+}
+
+/** Compile and load the kernels of `dest = e` (with `mode`) without running them (no counterpart in the
+ *  reference, whose kernels are compiled ahead of time). */
+template<class T, std::size_t N, class E>
+void prepare(View<T, N> dest, const E& e, SumMode mode = SumMode::None, Stream s = nullptr)
+{
+    dest.prepare(e, mode, s);
 }
 
 /** Pick the fastest launch shape for this expression's structure on this device (no counterpart in the

@@ -223,6 +223,17 @@ int aeth_eval_tape_ex(int dtype, int n_programs, const aeth_program_ex* programs
  * interpreter kernel runs a tape when the runtime compiler is unavailable or
  * AETHER_TAPE_INTERPRET is set. Both give the same bits. */
 int aeth_tape_jit_stats(uint64_t* compiled, uint64_t* cache_hits, uint64_t* fallbacks);
+/* Compile and load, without running, every kernel that aeth_eval_tape_ex
+ * with these arguments would launch (the same structure, launch shape and
+ * cache policy), so that the first real call pays no runtime compile (0.1-0.5 s
+ * per tape structure and process). Nothing is allocated, read or written; the
+ * pointers only decide which planes alias. No counterpart in the reference,
+ * whose kernels are compiled ahead of time: it is for callers whose first
+ * call must not stall, such as servers warming up and timed loops. */
+int aeth_tape_prepare(int dtype, int n_programs, const aeth_program_ex* programs, int n_leaves,
+                      void* const* leaves, const uint64_t* leaf_pitch, int n_consts, const double* consts,
+                      int n_rngs, const aeth_rng_leaf* rngs, uint64_t n, int sum_mode, void* stream);
+
 /* Compile (without loading or running) the kernel of a tape for `arch`
  * ("gfx950"); works without a GPU. code_bytes: size of the code object. */
 int aeth_tape_jit_build(int dtype, int sum_mode, int n_programs, const aeth_program* programs, const char* arch,

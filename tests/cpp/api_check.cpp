@@ -166,6 +166,7 @@ int tune(std::size_t n, std::uint64_t seed)
     Array<double, 3> a(n), b(n), c(n), d1(n), d2(n), d3(n);
     fill_generator(a, b, c, seed);
     const auto e = cfg0(a.deviceView(), b.deviceView(), c.deviceView());
+    aether::prepare(d1.deviceView(), e, aether::SumMode::Neumaier); // compiles and loads, runs nothing
     const auto s1 = aether::evaluate(d1.deviceView(), e, aether::SumMode::Neumaier);
     const aeth_tape_shape ts = aether::tune(d2.deviceView(), e, nullptr, aether::SumMode::Neumaier);
     const auto s2 = aether::evaluate(d2.deviceView(), e, aether::SumMode::Neumaier);
