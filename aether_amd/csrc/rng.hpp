@@ -5,7 +5,9 @@
 // philox4x32_10({lo(counter), hi(counter), lo(global), hi(global)},
 // key = (lo(seed), hi(seed))). The bijection is Salmon et al., SC'11.
 // Pure 32-bit integer VALU: each round is two 32x32->64 products
-// (v_mad_u64_u32) plus xors; the key bump is two adds.
+// (v_mad_u64_u32) plus two three-input xors (gfx950's v_bitop3_b32 with the
+// XOR truth table, one instruction where two-input xors take two); the key
+// bump is two scalar adds when the key is uniform.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +22,12 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
+// x ^ y ^ z in one instruction: truth table 0x96 is the parity of the three input bits.
+__device__ __forceinline__ uint32_t xor3(uint32_t x, uint32_t y, uint32_t z)
+{
+    return __builtin_amdgcn_bitop3_b32(x, y, z, 0x96);
+}
+
 __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1)
 {
 #pragma unroll
@@ -32,7 +40,10 @@ __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1)
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        // the first two rounds keep plain xors: callers' counters are often zero or uniform there, which the
+        // compiler folds through a xor but not through the three-input op
+        if (round < 2) c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        else c = U4{xor3(hi1, c.y, k0), lo1, xor3(hi0, c.w, k1), lo0};
     }
     return c;
 }
