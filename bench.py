@@ -160,19 +160,22 @@ def cpu_baseline(log2n: int) -> dict:
 
 
 def _time(fn, reps: int = 10, warm: int = 2) -> float:
-    """Average milliseconds of fn() over reps launches, HIP events on the launch stream."""
+    """Median milliseconds of fn() over reps back-to-back calls, each bracketed by HIP events on the launch
+    stream. The median, not the mean: this GPU now and then stalls one launch for ~20 ms (seen in per-launch
+    series of every kernel), which would move a 10-launch mean of a 2 ms kernel by up to 2x."""
     import torch
 
     for _ in range(warm):
         fn()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
     torch.cuda.synchronize()
-    e0.record()
-    for _ in range(reps):
+    for e0, e1 in ev:
+        e0.record()
         fn()
-    e1.record()
+        e1.record()
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+    t = sorted(e0.elapsed_time(e1) for e0, e1 in ev)
+    return t[len(t) // 2]
 
 
 def _gbs(nbytes: float, ms: float) -> float:
