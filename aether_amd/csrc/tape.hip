@@ -12,6 +12,7 @@
 // through a register. The tile and sum shape are those of eval.hip, so a tape
 // and the specialised kernel for the same tree agree bit for bit.
 #include <hip/hip_runtime.h>
+#include <unistd.h>
 
 #include <hip/hiprtc.h>
 
@@ -751,6 +752,51 @@ bool jit_compile(int dtype, int mode, int np, const aeth_program* P, bool nt, co
 }
 
 // Returns nullptr (and records why) when the runtime compiler is unavailable.
+// Compiled tape kernels persist across processes when AETHER_KERNEL_CACHE names a directory: one code object
+// per (architecture, generated source), named by a 64-bit FNV-1a hash of both. The source embeds the device
+// headers, so a library whose device code changed looks for other files instead of loading stale ones.
+// Files are written whole under a temporary name and renamed into place, so a concurrent reader never sees
+// a partial one; a file that does not load is compiled again and replaced.
+std::string cache_file(const std::string& arch, const std::string& src)
+{
+    static const char* dir = std::getenv("AETHER_KERNEL_CACHE");
+    if (!dir || !*dir) return std::string();
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (const std::string* part : {&arch, &src})
+        for (unsigned char ch : *part) {
+            h ^= ch;
+            h *= 0x100000001b3ull;
+        }
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "%016llx", (unsigned long long)h);
+    std::string tag = arch.substr(arch.find('=') + 1);
+    return std::string(dir) + "/aeth-" + tag + "-" + buf + ".co";
+}
+
+bool read_file(const std::string& path, std::vector<char>& out)
+{
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::vector<char> buf;
+    char chunk[1 << 16];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof chunk, f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+    const bool ok = !std::ferror(f) && !buf.empty();
+    std::fclose(f);
+    if (ok) out.swap(buf);
+    return ok;
+}
+
+void write_file(const std::string& path, const std::vector<char>& code)
+{
+    const std::string tmp = path + ".tmp" + std::to_string((unsigned long long)getpid());
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return; // an unwritable cache only costs the next process a compile
+    const bool ok = std::fwrite(code.data(), 1, code.size(), f) == code.size();
+    if (std::fclose(f) == 0 && ok && std::rename(tmp.c_str(), path.c_str()) == 0) return;
+    std::remove(tmp.c_str());
+}
+
 hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, bool nt, JitShape sh)
 {
     static const bool disabled = std::getenv("AETHER_TAPE_INTERPRET") != nullptr;
@@ -769,15 +815,27 @@ hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, b
     std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
     const size_t colon = arch.find(':');
     if (colon != std::string::npos) arch.resize(colon);
+    const std::string name = kernel_name(dtype, mode, np, P, nt, sh);
+    JitEntry e;
     std::vector<char> code;
+    const std::string file = cache_file(arch, gen_kernel(dtype, mode, np, P, nt, sh));
+    if (!file.empty() && read_file(file, code) && hipModuleLoadData(&e.module, code.data()) == hipSuccess) {
+        if (hipModuleGetFunction(&e.fn, e.module, name.c_str()) == hipSuccess) {
+            g_jit.emplace(key, e);
+            ++g_jit_hits;
+            return e.fn;
+        }
+        (void)hipModuleUnload(e.module);
+        e = JitEntry{};
+    }
+    (void)hipGetLastError();
     if (!jit_compile(dtype, mode, np, P, nt, arch, code, sh)) {
         ++g_jit_fallbacks;
         return nullptr;
     }
-    JitEntry e;
     check(hipModuleLoadData(&e.module, code.data()), "hipModuleLoadData");
-    check(hipModuleGetFunction(&e.fn, e.module, kernel_name(dtype, mode, np, P, nt, sh).c_str()),
-          "hipModuleGetFunction");
+    check(hipModuleGetFunction(&e.fn, e.module, name.c_str()), "hipModuleGetFunction");
+    if (!file.empty()) write_file(file, code);
     g_jit.emplace(key, e);
     ++g_jit_compiled;
     return e.fn;
@@ -789,10 +847,6 @@ using namespace aeth;
 
 namespace {
 
-// One tape assignment. `force` (the tuner) runs every launch with that shape; otherwise assignments
-// without sums use the tuned shape of their structure and assignments with sums the contract shape.
-// Returns whether the assignment could run as one launch over all programs (no program reads a plane
-// another one writes).
 // Set while aeth_tape_prepare walks an assignment: every kernel the assignment would launch is compiled and
 // loaded into this process's cache; nothing is allocated, written or launched.
 thread_local bool t_prepare_only = false;
@@ -802,6 +856,10 @@ struct PrepareOnly {
     ~PrepareOnly() { t_prepare_only = false; }
 };
 
+// One tape assignment. `force` (the tuner) runs every launch with that shape; otherwise assignments
+// without sums use the tuned shape of their structure and assignments with sums the contract shape.
+// Returns whether the assignment could run as one launch over all programs (no program reads a plane
+// another one writes).
 bool eval_tape(int dtype, int n_programs, const aeth_program* programs, int n_leaves, void* const* leaves,
                const uint64_t* leaf_pitch, int n_consts, const double* consts, int n_rngs, const aeth_rng_leaf* rngs,
                uint64_t n, int sum_mode, void* sums, void* stream, const JitShape* force)

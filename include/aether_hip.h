@@ -221,7 +221,11 @@ int aeth_eval_tape_ex(int dtype, int n_programs, const aeth_program_ex* programs
 /* Each distinct tape structure is lowered at run time (hipRTC) to its own
  * straight-line kernel, compiled once per process and cached; the resident
  * interpreter kernel runs a tape when the runtime compiler is unavailable or
- * AETHER_TAPE_INTERPRET is set. Both give the same bits. */
+ * AETHER_TAPE_INTERPRET is set. Both give the same bits. With the environment
+ * variable AETHER_KERNEL_CACHE naming a writable directory, compiled kernels
+ * are also kept there (one code object per architecture and generated source)
+ * and later processes load them instead of compiling. compiled: hipRTC
+ * compiles; cache_hits: kernels found in this process's cache or on disk. */
 int aeth_tape_jit_stats(uint64_t* compiled, uint64_t* cache_hits, uint64_t* fallbacks);
 /* Compile and load, without running, every kernel that aeth_eval_tape_ex
  * with these arguments would launch (the same structure, launch shape and
