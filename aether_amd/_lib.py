@@ -30,6 +30,8 @@ SIGNATURES = {
     "aeth_eval_vec3geo": (_i32, [_vp, _vp, _vp, _vp, _u64, _u64, _vp]),
     "aeth_eval_tape": (_i32, [_i32, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _u64, _i32, _vp, _vp]),
     "aeth_eval_tape_ex": (_i32, [_i32, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _u64, _i32, _vp, _vp]),
+    "aeth_tape_timing": (_i32, [_i32]),
+    "aeth_tape_timing_read": (_i32, [C.POINTER(C.c_float), _i32, C.POINTER(_i32)]),
     "aeth_tape_prepare": (_i32, [_i32, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _u64, _i32, _vp]),
     "aeth_tape_jit_stats": (_i32, [C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
     "aeth_tape_jit_build": (_i32, [_i32, _i32, _i32, _vp, C.c_char_p, C.POINTER(_u64)]),

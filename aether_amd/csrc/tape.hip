@@ -382,6 +382,45 @@ void launch_nl(const TapeArgs& A, bool vec, bool nt, acc::Pair<T>* part, int gri
         hipLaunchKernelGGL((tape_kernel<T, M, SUM, NL, false, false>), dim3(grid), dim3(kThreads), 0, st, A, part);
 }
 
+// Per-launch timing of the tape kernels themselves (measurement; no counterpart in the reference). While
+// enabled, every tape kernel launch outside a stream capture is bracketed by two HIP events on its stream
+// (not the sums' group tree after it); aeth_tape_timing_read waits for the pending launches and hands back
+// their elapsed milliseconds in launch order.
+std::atomic<bool> g_timing{false};
+std::mutex g_timing_mu;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_timed;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t pooled_event()
+{
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+struct TimedLaunch {
+    hipStream_t st;
+    hipEvent_t e1 = nullptr;
+    explicit TimedLaunch(hipStream_t s) : st(s)
+    {
+        if (!g_timing.load(std::memory_order_relaxed) || capturing(s)) return;
+        std::lock_guard<std::mutex> lock(g_timing_mu);
+        hipEvent_t e0 = pooled_event();
+        e1 = pooled_event();
+        check(hipEventRecord(e0, st), "hipEventRecord");
+        g_timed.emplace_back(e0, e1);
+    }
+    void done()
+    {
+        if (e1) check(hipEventRecord(e1, st), "hipEventRecord");
+    }
+};
+
 template<class T, int M, bool SUM>
 void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums, hipStream_t st, hipFunction_t jit)
 {
@@ -393,6 +432,7 @@ void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums
     StreamLease scratch(SUM ? pairs * sizeof(acc::Pair<T>) : 0, st);
     auto* part = static_cast<acc::Pair<T>*>(scratch.get());
     const int grid = (int)std::max<uint32_t>(1u, A.units);
+    TimedLaunch timed(st);
     if (jit) {
         JitArgs args;
         std::memset(&args, 0, sizeof args);
@@ -424,6 +464,7 @@ void launch(const TapeArgs& A, int max_loads, int np, bool vec, bool nt, T* sums
         launch_nl<T, M, SUM, 16>(A, vec, nt, part, grid, st);
     }
     check(hipGetLastError(), "tape_kernel launch");
+    timed.done();
     if constexpr (SUM) {
         acc::Pair<T>* cur = part;
         acc::Pair<T>* next = part + (size_t)A.tiles * np;
@@ -1301,6 +1342,30 @@ extern "C" int aeth_eval_tape_ex(int dtype, int n_programs, const aeth_program_e
     return guarded([&] {
         eval_tape_ex(dtype, n_programs, programs, n_leaves, leaves, leaf_pitch, n_consts, consts, n_rngs, rngs, n,
                      sum_mode, sums, stream);
+    });
+}
+
+extern "C" int aeth_tape_timing(int enable)
+{
+    return guarded([&] { g_timing.store(enable != 0); });
+}
+
+extern "C" int aeth_tape_timing_read(float* ms, int capacity, int* count)
+{
+    return guarded([&] {
+        if (!count || (capacity > 0 && !ms)) fail("aeth_tape_timing_read", "NULL output");
+        std::lock_guard<std::mutex> lock(g_timing_mu);
+        const size_t take = std::min<size_t>(g_timed.size(), (size_t)std::max(capacity, 0));
+        *count = (int)g_timed.size();
+        for (size_t k = 0; k < take; ++k) {
+            float t = 0.0f;
+            check(hipEventSynchronize(g_timed[k].second), "hipEventSynchronize");
+            check(hipEventElapsedTime(&t, g_timed[k].first, g_timed[k].second), "hipEventElapsedTime");
+            ms[k] = t;
+            g_event_pool.push_back(g_timed[k].first);
+            g_event_pool.push_back(g_timed[k].second);
+        }
+        g_timed.erase(g_timed.begin(), g_timed.begin() + (std::ptrdiff_t)take);
     });
 }
 

@@ -533,6 +533,23 @@ def prepare(dest: Array, expr, sum_mode: int = -1, stream=None):
     del keep
 
 
+def kernel_timing(enable: bool) -> None:
+    """Bracket every tape kernel launched from now on (outside stream capture) with two HIP events on its
+    stream, or stop doing so; the sums' group tree after each launch is outside the pair."""
+    call("aeth_tape_timing", 1 if enable else 0)
+
+
+def kernel_times() -> list:
+    """Milliseconds of each tape kernel launched while timing was on since the last read, in launch order
+    (waits for them to finish)."""
+    count = C.c_int32()
+    call("aeth_tape_timing_read", None, 0, C.byref(count))
+    n = count.value
+    buf = (C.c_float * max(n, 1))()
+    call("aeth_tape_timing_read", buf, n, C.byref(count))
+    return [float(buf[k]) for k in range(n)]
+
+
 def assign(dest: Array, expr, sum_mode: int = -1, sums: torch.Tensor = None, stream=None):
     """`dest = expr` over all samples: ONE kernel when the tree fits one launch, otherwise the tree cut into
     pieces that do (same values); returns (ncomp, 2) sums when sum_mode >= 0."""

@@ -11,8 +11,10 @@ Inputs are resident in HBM before the timed region. Algorithmic bytes per
 step: 3 components x 2^27 samples x 32 B (3 loads + 1 store of a double).
 
 Prints ONE JSON line. `roofline.achieved` is the algorithmic bytes of one
-launch over its average duration, timed with HIP events on the launch
-stream; `roofline.stream_ceiling` is the same traffic (3 reads, 1 write per
+launch of the tape kernel over its average duration, timed with HIP events
+that the library records around that kernel alone on its launch stream
+(`kernel_ms`; `launch_ms` is the whole step, the sums' tree included);
+`roofline.stream_ceiling` is the same traffic (3 reads, 1 write per
 element) with trivial arithmetic on the same arrays, measured in the same run.
 `cpu_baseline` times the oracle's restatement of the reference's OpenMP CPU
 path (oracle/*.c) on a bounded sample of the same workload on this host's
@@ -780,6 +782,8 @@ def main():
         step()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    E.kernel_times()  # nothing pending from before the timed region
+    E.kernel_timing(True)  # each step's tape kernel alone, by HIP events on its own stream
     t0 = time.perf_counter()
     for e0, e1 in ev:
         e0.record()
@@ -787,12 +791,16 @@ def main():
         e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    E.kernel_timing(False)
+    kernel_ms = sorted(E.kernel_times())
+    assert len(kernel_ms) == args.steps, (len(kernel_ms), args.steps)
+    avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)
     launch_ms = sorted(e0.elapsed_time(e1) for e0, e1 in ev)
     avg_launch_ms = sum(launch_ms) / len(launch_ms)
     bytes_step = 3 * n * 32
     ms_per_step = wall * 1e3 / args.steps
     value = bytes_step / (wall / args.steps) / 1e9
-    achieved = bytes_step / (avg_launch_ms * 1e-3) / 1e9
+    achieved = bytes_step / (avg_kernel_ms * 1e-3) / 1e9
     s = sums.cpu()
     line = {
         "metric": METRIC,
@@ -815,6 +823,8 @@ def main():
                      "traffic": MEASURED_TRAFFIC_BYTES.get(args.log2n),
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of this kernel, calibrated by a known copy "
                                        "(profiles/tape_cfg0_f64_2p27_hbm_traffic.md)"},
+        "kernel_ms": {"min": round(kernel_ms[0], 4), "median": round(kernel_ms[len(kernel_ms) // 2], 4),
+                      "avg": round(avg_kernel_ms, 4)},
         "launch_ms": {"min": round(launch_ms[0], 4), "median": round(launch_ms[len(launch_ms) // 2], 4),
                       "avg": round(avg_launch_ms, 4)},
         "sums": [s[k, 0].item() + s[k, 1].item() for k in range(3)],

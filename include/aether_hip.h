@@ -227,6 +227,16 @@ int aeth_eval_tape_ex(int dtype, int n_programs, const aeth_program_ex* programs
  * and later processes load them instead of compiling. compiled: hipRTC
  * compiles; cache_hits: kernels found in this process's cache or on disk. */
 int aeth_tape_jit_stats(uint64_t* compiled, uint64_t* cache_hits, uint64_t* fallbacks);
+/* Per-launch timing of the tape kernels (measurement; no counterpart in the
+ * reference). While enabled, each tape kernel launched outside a stream
+ * capture is bracketed by two HIP events on its stream (the sums' group tree
+ * that follows it is not). aeth_tape_timing_read sets *count to the number of
+ * launches recorded and not yet read, then waits for the oldest `capacity` of
+ * them and hands back their elapsed milliseconds in launch order (capacity 0
+ * only counts); the rest stay pending for the next read. */
+int aeth_tape_timing(int enable);
+int aeth_tape_timing_read(float* ms, int capacity, int* count);
+
 /* Compile and load, without running, every kernel that aeth_eval_tape_ex
  * with these arguments would launch (the same structure, launch shape and
  * cache policy), so that the first real call pays no runtime compile (0.1-0.5 s
