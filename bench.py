@@ -23,6 +23,7 @@ cores; the reference itself never runs here.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import subprocess
@@ -163,19 +164,28 @@ def cpu_baseline(log2n: int) -> dict:
 
 def _time(fn, reps: int = 10, warm: int = 2) -> float:
     """Median milliseconds of fn() over reps back-to-back calls, each bracketed by HIP events on the launch
-    stream. The median, not the mean: this GPU now and then stalls one launch for ~20 ms (seen in per-launch
-    series of every kernel), which would move a 10-launch mean of a 2 ms kernel by up to 2x."""
+    stream. The timed calls queue behind the warm-up calls with no synchronization in between, so the host
+    runs ahead of the GPU, and Python's garbage collector is off while they are issued (as timeit does): a
+    pause of the host (measured here: one call of 33 ms host time in a series of 0.06 ms calls) then falls
+    on queued work instead of idling the GPU between two events. The median, not the mean, for what
+    remains."""
     import torch
 
-    for _ in range(warm):
-        fn()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
     torch.cuda.synchronize()
-    for e0, e1 in ev:
-        e0.record()
-        fn()
-        e1.record()
-    torch.cuda.synchronize()
+    collecting = gc.isenabled()
+    gc.disable()
+    try:
+        for _ in range(warm):
+            fn()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for e0, e1 in ev:
+            e0.record()
+            fn()
+            e1.record()
+        torch.cuda.synchronize()
+    finally:
+        if collecting:
+            gc.enable()
     t = sorted(e0.elapsed_time(e1) for e0, e1 in ev)
     return t[len(t) // 2]
 
@@ -778,6 +788,10 @@ def main():
     E.prepare(d, tree, ae.SUM_NEUMAIER)
     stream = None if args.no_stream else wl_stream(a, b, c, d, n)
     torch.cuda.synchronize()
+    # no garbage collection while the steps are issued (as timeit does): a collection pause of the host
+    # between two launches would idle the GPU inside the timed region
+    gc.collect()
+    gc.disable()
     for _ in range(max(1, args.warmup)):
         step()
     torch.cuda.synchronize()
@@ -791,6 +805,7 @@ def main():
         e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    gc.enable()
     E.kernel_timing(False)
     kernel_ms = sorted(E.kernel_times())
     assert len(kernel_ms) == args.steps, (len(kernel_ms), args.steps)
