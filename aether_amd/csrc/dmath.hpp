@@ -14,9 +14,9 @@
 //             r^13, scale by ldexp.
 //   log     : x = 2^k m, m in [sqrt(2)/2, sqrt(2)); s = f/(2+f);
 //             log(1+f) = f - (hfsq - s(hfsq + R(s^2))), R from 2 atanh(s).
-//   float   : the argument widened to double, the same reductions, shorter
-//             polynomials (Taylor sin to r^11, cos to r^10, exp to r^8, log's
-//             R to s^10), one final rounding.
+//   float   : float arithmetic with float-float (unevaluated sum) steps for the
+//             reduced argument and the leading terms; double only in the
+//             reduction of trig arguments of magnitude 2^17 and up.
 // Every multiply-add that is fused is an explicit __builtin_fma; the library
 // is compiled with -ffp-contract=off so nothing else is fused.
 #pragma once

@@ -403,21 +403,41 @@ hipEvent_t pooled_event()
     return e;
 }
 
+// A pair joins the pending list only once both events are recorded: a launch that throws in between
+// returns its events to the pool and leaves nothing unreadable behind.
 struct TimedLaunch {
     hipStream_t st;
-    hipEvent_t e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     explicit TimedLaunch(hipStream_t s) : st(s)
     {
         if (!g_timing.load(std::memory_order_relaxed) || capturing(s)) return;
         std::lock_guard<std::mutex> lock(g_timing_mu);
-        hipEvent_t e0 = pooled_event();
+        e0 = pooled_event();
         e1 = pooled_event();
-        check(hipEventRecord(e0, st), "hipEventRecord");
-        g_timed.emplace_back(e0, e1);
+        if (hipEventRecord(e0, st) != hipSuccess) {
+            release();
+            fail("hipEventRecord", "timing the tape kernel");
+        }
     }
     void done()
     {
-        if (e1) check(hipEventRecord(e1, st), "hipEventRecord");
+        if (!e0) return;
+        check(hipEventRecord(e1, st), "hipEventRecord");
+        std::lock_guard<std::mutex> lock(g_timing_mu);
+        g_timed.emplace_back(e0, e1);
+        e0 = e1 = nullptr;
+    }
+    void release()
+    {
+        if (e0) g_event_pool.push_back(e0);
+        if (e1) g_event_pool.push_back(e1);
+        e0 = e1 = nullptr;
+    }
+    ~TimedLaunch()
+    {
+        if (!e0) return;
+        std::lock_guard<std::mutex> lock(g_timing_mu);
+        release();
     }
 };
 
@@ -1357,15 +1377,26 @@ extern "C" int aeth_tape_timing_read(float* ms, int capacity, int* count)
         std::lock_guard<std::mutex> lock(g_timing_mu);
         const size_t take = std::min<size_t>(g_timed.size(), (size_t)std::max(capacity, 0));
         *count = (int)g_timed.size();
-        for (size_t k = 0; k < take; ++k) {
+        size_t k = 0;
+        // the pairs read so far, and one that fails to read, leave the list
+        struct Drop {
+            size_t& k;
+            ~Drop()
+            {
+                for (size_t j = 0; j < k; ++j) {
+                    g_event_pool.push_back(g_timed[j].first);
+                    g_event_pool.push_back(g_timed[j].second);
+                }
+                g_timed.erase(g_timed.begin(), g_timed.begin() + (std::ptrdiff_t)k);
+            }
+        } drop{k};
+        while (k < take) {
+            const auto pr = g_timed[k++];
             float t = 0.0f;
-            check(hipEventSynchronize(g_timed[k].second), "hipEventSynchronize");
-            check(hipEventElapsedTime(&t, g_timed[k].first, g_timed[k].second), "hipEventElapsedTime");
-            ms[k] = t;
-            g_event_pool.push_back(g_timed[k].first);
-            g_event_pool.push_back(g_timed[k].second);
+            check(hipEventSynchronize(pr.second), "hipEventSynchronize");
+            check(hipEventElapsedTime(&t, pr.first, pr.second), "hipEventElapsedTime");
+            ms[k - 1] = t;
         }
-        g_timed.erase(g_timed.begin(), g_timed.begin() + (std::ptrdiff_t)take);
     });
 }
 
