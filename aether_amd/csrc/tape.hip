@@ -783,10 +783,8 @@ __device__ __forceinline__ T jit_draw(const aeth_rng_leaf& r, uint32_t i)
 }
 
 // Compile a tape's kernel for `arch_opt` ("--offload-arch=gfx950"); false (log in last error) on failure.
-bool jit_compile(int dtype, int mode, int np, const aeth_program* P, bool nt, const std::string& arch_opt,
-                 std::vector<char>& code, JitShape sh = JitShape{})
+bool jit_compile(const std::string& src, const std::string& arch_opt, std::vector<char>& code)
 {
-    const std::string src = gen_kernel(dtype, mode, np, P, nt, sh);
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "aeth_tape_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
         set_error("hiprtcCreateProgram failed");
@@ -814,7 +812,8 @@ bool jit_compile(int dtype, int mode, int np, const aeth_program* P, bool nt, co
 
 // Returns nullptr (and records why) when the runtime compiler is unavailable.
 // Compiled tape kernels persist across processes when AETHER_KERNEL_CACHE names a directory: one code object
-// per (architecture, generated source), named by a 64-bit FNV-1a hash of both. The source embeds the device
+// per (architecture, generated source), named by two independent 64-bit hashes of both (FNV-1a and a
+// multiply-rotate hash), so an accidental collision of names is out of reach. The source embeds the device
 // headers, so a library whose device code changed looks for other files instead of loading stale ones.
 // Files are written whole under a temporary name and renamed into place, so a concurrent reader never sees
 // a partial one; a file that does not load is compiled again and replaced.
@@ -822,14 +821,16 @@ std::string cache_file(const std::string& arch, const std::string& src)
 {
     static const char* dir = std::getenv("AETHER_KERNEL_CACHE");
     if (!dir || !*dir) return std::string();
-    uint64_t h = 0xcbf29ce484222325ull;
+    uint64_t h = 0xcbf29ce484222325ull, g = 0x9e3779b97f4a7c15ull;
     for (const std::string* part : {&arch, &src})
         for (unsigned char ch : *part) {
             h ^= ch;
             h *= 0x100000001b3ull;
+            g = (g ^ ch) * 0xff51afd7ed558ccdull;
+            g = (g << 29) | (g >> 35);
         }
     char buf[40];
-    std::snprintf(buf, sizeof buf, "%016llx", (unsigned long long)h);
+    std::snprintf(buf, sizeof buf, "%016llx%016llx", (unsigned long long)h, (unsigned long long)g);
     std::string tag = arch.substr(arch.find('=') + 1);
     return std::string(dir) + "/aeth-" + tag + "-" + buf + ".co";
 }
@@ -879,7 +880,8 @@ hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, b
     const std::string name = kernel_name(dtype, mode, np, P, nt, sh);
     JitEntry e;
     std::vector<char> code;
-    const std::string file = cache_file(arch, gen_kernel(dtype, mode, np, P, nt, sh));
+    const std::string src = gen_kernel(dtype, mode, np, P, nt, sh);
+    const std::string file = cache_file(arch, src);
     if (!file.empty() && read_file(file, code) && hipModuleLoadData(&e.module, code.data()) == hipSuccess) {
         if (hipModuleGetFunction(&e.fn, e.module, name.c_str()) == hipSuccess) {
             g_jit.emplace(key, e);
@@ -890,7 +892,7 @@ hipFunction_t jit_function(int dtype, int mode, int np, const aeth_program* P, b
         e = JitEntry{};
     }
     (void)hipGetLastError();
-    if (!jit_compile(dtype, mode, np, P, nt, arch, code, sh)) {
+    if (!jit_compile(src, arch, code)) {
         ++g_jit_fallbacks;
         return nullptr;
     }
@@ -1534,7 +1536,8 @@ extern "C" int aeth_tape_jit_build(int dtype, int sum_mode, int n_programs, cons
         for (int p = 0; p < n_programs; ++p)
             validate(programs[p], AETH_TAPE_MAX_LEAVES, AETH_TAPE_MAX_CONSTS, AETH_TAPE_MAX_RNGS);
         std::vector<char> code;
-        if (!jit_compile(dtype, sum_mode, n_programs, programs, true, std::string("--offload-arch=") + arch, code))
+        if (!jit_compile(gen_kernel(dtype, sum_mode, n_programs, programs, true, JitShape{}),
+                         std::string("--offload-arch=") + arch, code))
             throw Failure{1};
         if (code_bytes) *code_bytes = code.size();
     });
