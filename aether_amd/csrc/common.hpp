@@ -93,7 +93,10 @@ private:
 // fused evaluation and left the GPU idle 7-16 us before the next call's kernel; a lease costs neither.
 // Inside a stream capture, or when this thread already holds the stream's slot, the lease falls back to a
 // StreamScratch. A stream destroyed with work still queued must not have its handle reused while that work
-// runs (the slot is keyed by the handle's value).
+// runs (the slot is keyed by the handle's value). Each lease records an event on its stream as it ends (the
+// buffer's last use); once more than a few slots hold buffers, a lease on a stream new to the library
+// releases the buffers of idle slots (unlocked, their last use complete), so a program that creates and
+// destroys streams keeps a bounded amount of scratch.
 class StreamLease {
 public:
     StreamLease(size_t bytes, hipStream_t stream);
@@ -105,6 +108,7 @@ public:
 private:
     void* ptr_ = nullptr;
     void* slot_ = nullptr;
+    hipStream_t stream_ = nullptr;
     StreamScratch* fallback_ = nullptr;
 };
 

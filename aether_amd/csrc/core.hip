@@ -44,17 +44,44 @@ struct LeaseSlot {
     std::mutex mu;
     void* ptr = nullptr;
     size_t bytes = 0;
+    hipEvent_t last = nullptr; // recorded on the stream as each lease ends: the buffer's last use
+    bool kept = false;         // a record of `last` failed: it no longer marks the last use, never release
 };
 
 std::mutex g_slots_mu;
-std::map<std::pair<int, hipStream_t>, std::unique_ptr<LeaseSlot>> g_slots; // never erased: leases point in
+// never erased (leases point in); only the buffers of idle slots are released
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<LeaseSlot>> g_slots;
+size_t g_slot_buffers = 0; // slots holding a buffer (guarded by g_slots_mu)
+constexpr size_t kKeptBuffers = 16;
 thread_local std::vector<LeaseSlot*> t_held;
+
+// Release the buffers of idle slots of device `dev` in the order of `st` (a stream of that device, not
+// capturing): a slot no lease holds (its lock is free) whose last use has completed. Called with
+// g_slots_mu held; try_lock never waits, so the lock order slot -> g_slots_mu elsewhere cannot deadlock.
+void release_idle(int dev, hipStream_t st, const LeaseSlot* keep)
+{
+    for (auto& kv : g_slots) {
+        LeaseSlot* s = kv.second.get();
+        if (kv.first.first != dev || s == keep || !s->ptr || !s->mu.try_lock()) continue;
+        if (s->ptr && !s->kept && s->last && hipEventQuery(s->last) == hipSuccess &&
+            hipFreeAsync(s->ptr, st) == hipSuccess) {
+            s->ptr = nullptr;
+            s->bytes = 0;
+            --g_slot_buffers;
+        }
+        (void)hipGetLastError();
+        s->mu.unlock();
+    }
+}
 
 LeaseSlot* slot_for(int dev, hipStream_t st)
 {
     std::lock_guard<std::mutex> lock(g_slots_mu);
     auto& p = g_slots[{dev, st}];
-    if (!p) p.reset(new LeaseSlot);
+    if (!p) {
+        p.reset(new LeaseSlot);
+        if (g_slot_buffers >= kKeptBuffers) release_idle(dev, st, p.get());
+    }
     return p.get();
 }
 
@@ -72,6 +99,7 @@ StreamLease::StreamLease(size_t bytes, hipStream_t stream)
         return;
     }
     std::unique_lock<std::mutex> lock(slot->mu);
+    if (!slot->last) check(hipEventCreateWithFlags(&slot->last, hipEventDisableTiming), "hipEventCreateWithFlags");
     if (slot->bytes < bytes) {
         // earlier calls' kernels on this stream may still read the old buffer: both steps are stream-ordered
         const size_t grown = (std::max(bytes, slot->bytes + slot->bytes / 2) + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
@@ -79,12 +107,17 @@ StreamLease::StreamLease(size_t bytes, hipStream_t stream)
             check(hipFreeAsync(slot->ptr, stream), "hipFreeAsync(lease)", slot->bytes);
             slot->ptr = nullptr;
             slot->bytes = 0;
+            std::lock_guard<std::mutex> g(g_slots_mu);
+            --g_slot_buffers;
         }
         check(hipMallocAsync(&slot->ptr, grown, stream), "hipMallocAsync(lease)", grown);
         slot->bytes = grown;
+        std::lock_guard<std::mutex> g(g_slots_mu);
+        ++g_slot_buffers;
     }
     ptr_ = slot->ptr;
     slot_ = slot;
+    stream_ = stream;
     t_held.push_back(slot);
     lock.release(); // held until the lease ends
 }
@@ -94,6 +127,11 @@ StreamLease::~StreamLease()
     delete fallback_;
     if (slot_) {
         auto* slot = static_cast<LeaseSlot*>(slot_);
+        // the call's kernels are queued: this event completes after its last use of the buffer
+        if (hipEventRecord(slot->last, stream_) != hipSuccess) {
+            slot->kept = true;
+            (void)hipGetLastError();
+        }
         t_held.erase(std::find(t_held.begin(), t_held.end(), slot));
         slot->mu.unlock();
     }

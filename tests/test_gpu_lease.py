@@ -5,6 +5,7 @@ stream. The results must not depend on it: calls of different sizes back to back
 two host threads into one stream, and calls on several streams at once all give the bits the same call
 gives alone on a fresh stream. Issuing a call must also leave the host free while the GPU works.
 """
+import ctypes as C
 import threading
 import time
 
@@ -154,3 +155,42 @@ def test_host_is_not_held(dev):
     torch.cuda.synchronize()
     gpu_ms = e0.elapsed_time(e1)
     assert host_ms < 0.5 * gpu_ms, (host_ms, gpu_ms)
+
+
+def test_idle_streams_release_their_buffers(dev):
+    """48 streams of the HIP runtime's own (distinct handles), one call each: the device's default memory
+    pool, where the buffers live, stays near its use after the first 20 instead of growing by a buffer a
+    stream, and every call gives the bits of the call alone."""
+    hip = C.CDLL("libamdhip64.so")
+    n = 1 << 22
+    (a, b, c), [(v1, w1), _] = _alone(n)
+    tree = _tree(a, b, c)
+    pool = C.c_void_p()
+    assert hip.hipDeviceGetDefaultMemPool(C.byref(pool), 0) == 0
+
+    def used():
+        v = C.c_uint64()
+        assert hip.hipMemPoolGetAttribute(pool, 0x7, C.byref(v)) == 0  # hipMemPoolAttrUsedMemCurrent
+        return v.value
+
+    streams, base = [], None
+    try:
+        for k in range(48):
+            s = C.c_void_p()
+            assert hip.hipStreamCreate(C.byref(s)) == 0
+            streams.append(s)
+            ext = torch.cuda.ExternalStream(s.value)
+            d = ae.Array(n, 3, torch.float64)
+            with torch.cuda.stream(ext):
+                sums = E.assign(d, tree, ae.SUM_NEUMAIER)
+            ext.synchronize()
+            assert np.array_equal(sums.cpu().numpy(), w1) and np.array_equal(_values(d), v1), k
+            if k == 19:
+                base = used()
+        torch.cuda.synchronize()
+        # one buffer a stream is 1 MiB here: 28 more streams unreleased would add 28 MiB
+        assert used() <= base + (4 << 20), (used(), base)
+    finally:
+        torch.cuda.synchronize()
+        for s in streams:
+            hip.hipStreamDestroy(s)
