@@ -997,11 +997,11 @@ inline constexpr bool always_false = false;
 template<class T, std::size_t N, class E>
 __global__ __launch_bounds__(256) void assign_kernel(T* p, std::size_t pitch, std::uint32_t n, E e)
 {
-    for (std::uint64_t i = (std::uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (std::uint64_t)gridDim.x * 256u) {
-        T out[N];
-        for (std::size_t c = 0; c < N; ++c) out[c] = (T)e.eval(E::ncomp > 1 ? c : 0, (std::size_t)i);
-        for (std::size_t c = 0; c < N; ++c) p[c * pitch + i] = out[c];
-    }
+    // component c is stored before component c + 1 is evaluated, as the reference assigns a sample
+    // (expr/Assign.h:124-160): a tree that reads the destination (v = f(v).cross(w)) sees the components
+    // of this sample already stored
+    for (std::uint64_t i = (std::uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (std::uint64_t)gridDim.x * 256u)
+        for (std::size_t c = 0; c < N; ++c) p[c * pitch + i] = (T)e.eval(E::ncomp > 1 ? c : 0, (std::size_t)i);
 }
 #endif
 

@@ -251,6 +251,18 @@ void run_cases(std::size_t n)
             ++failures;
         }
     }
+    {   // a tree holding a user node that reads its own destination: component 0 of a sample is stored before
+        // component 1 is evaluated, as the built-in lowering does (one launch a component in that case)
+        Array<T, 3> got(n), want(n);
+        got.deviceView() = cv * T(1);
+        want.deviceView() = cv * T(1);
+        const auto gv = got.deviceView(), wv = want.deviceView();
+        const auto ss = [](auto x) { return x * x * (T(3) - T(2) * x); };
+        got.deviceView() = smoothstep(gv).cross(bv);
+        want.deviceView() = ss(wv).cross(bv);
+        w = std::string("user node reading its destination ") + tn;
+        expect_same(w.c_str(), got, want);
+    }
     {   // get(): the sample's components, read in the kernel
         Array<T, 1> got(n), want(n);
         hipLaunchKernelGGL(get_kernel<T>, grid_of(n), dim3(256), 0, 0, got.deviceView(), av, n32);
