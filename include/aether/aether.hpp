@@ -352,6 +352,56 @@ struct SampleIndex {
     AETHER_DEVICEHOST() constexpr offset_t work() const { return w; }
 };
 
+/** index/BundleIndex.h:56-112: W consecutive samples [base, base + W) handled by one thread. Lane k is the
+ *  sample base + k (its work index is k). A bundle always names a whole W-wide span; the part of a tail
+ *  bundle inside an n-sample array is asked for separately with mask(n). */
+template<std::size_t W>
+struct BundleIndex {
+    static constexpr std::size_t width = W;
+    offset_t base_ = 0;
+
+    /** Thread `thread` of block `block` (blocks of `block_dim`) takes samples W (block block_dim + thread) on. */
+    AETHER_DEVICEHOST() static constexpr BundleIndex make(offset_t thread, offset_t block, offset_t block_dim)
+    {
+        return BundleIndex{(offset_t)W * (block * block_dim + thread)};
+    }
+    /** Bundle `i` of a host loop: samples W i on. */
+    AETHER_DEVICEHOST() static constexpr BundleIndex make(std::size_t i) { return BundleIndex{(offset_t)(W * i)}; }
+    AETHER_DEVICEHOST() constexpr offset_t base() const { return base_; }
+    AETHER_DEVICEHOST() constexpr SampleIndex lane(std::size_t k) const
+    {
+        return SampleIndex{base_ + (offset_t)k, (offset_t)k};
+    }
+
+    /** The lanes of a bundle that lie inside the array: the first `active_` ones. */
+    struct Mask {
+        offset_t active_;
+        AETHER_DEVICEHOST() constexpr bool lane(std::size_t k) const { return (offset_t)k < active_; }
+        AETHER_DEVICEHOST() constexpr bool full() const { return active_ == (offset_t)W; }
+    };
+    AETHER_DEVICEHOST() constexpr Mask mask(offset_t n) const
+    {
+        const offset_t left = n > base_ ? n - base_ : offset_t(0);
+        return Mask{left < (offset_t)W ? left : (offset_t)W};
+    }
+};
+
+/** index/SampleIndex.h:78-99: `Width` consecutive samples from base_, of which the first active_ are in
+ *  use (a tail packet has fewer). */
+template<std::size_t Width>
+struct PacketIndex {
+    static constexpr std::size_t width = Width;
+    std::size_t base_;
+    std::size_t active_;
+    AETHER_DEVICEHOST() static constexpr PacketIndex make(std::size_t base) { return PacketIndex{base, Width}; }
+    AETHER_DEVICEHOST() static constexpr PacketIndex makeTail(std::size_t base, std::size_t remaining)
+    {
+        return PacketIndex{base, remaining};
+    }
+    AETHER_DEVICEHOST() constexpr bool full() const { return active_ == Width; }
+    AETHER_DEVICEHOST() constexpr SampleIndex scalar(std::size_t k) const { return SampleIndex::make(base_ + k); }
+};
+
 // ---------------------------------------------------------------- expressions
 template<class D>
 struct Expr;
@@ -988,12 +1038,58 @@ private:
 
 namespace detail {
 
+/** Store the W values of one component of a bundle: whole 16-byte stores when the bundle is full and its
+ *  first sample 16-byte aligned, lane by lane otherwise (a tail bundle, or a view whose planes start off a
+ *  16-byte boundary). */
+template<class T, std::size_t W>
+AETHER_DEVICEHOST() inline void store_lanes(T* dst, const T (&v)[W], std::size_t active)
+{
+    if constexpr ((W * sizeof(T)) % 16 == 0) {
+        if (active == W && reinterpret_cast<std::uintptr_t>(dst) % 16 == 0) {
+            __builtin_memcpy(static_cast<T*>(__builtin_assume_aligned(dst, 16)), v, W * sizeof(T));
+            return;
+        }
+    }
+    for (std::size_t k = 0; k < active; ++k) dst[k] = v[k];
+}
+
+} // namespace detail
+
+/** View[BundleIndex<W>] (backend/cuda/bundle/Assign.h:56-265): the W samples of a bundle, assigned one
+ *  component at a time over its lanes. Each sample gets the bits of the per-sample assignment (the same
+ *  node arithmetic; a sample's component c is stored before its component c + 1 is evaluated). */
+template<class T, std::size_t N, std::size_t W>
+class BundleRef {
+public:
+    AETHER_DEVICEHOST() BundleRef(T* p, std::size_t pitch, std::size_t base, std::size_t active)
+        : p_(p), pitch_(pitch), base_(base), active_(active)
+    {
+    }
+    template<class E>
+    AETHER_DEVICEHOST() BundleRef& operator=(const E& e)
+    {
+        for (std::size_t c = 0; c < N; ++c) {
+            T v[W];
+            for (std::size_t k = 0; k < W; ++k)
+                v[k] = k < active_ ? (T)e.eval(E::ncomp > 1 ? c : 0, base_ + k) : T(0);
+            detail::store_lanes<T, W>(p_ + c * pitch_ + base_, v, active_);
+        }
+        return *this;
+    }
+
+private:
+    T* p_;
+    std::size_t pitch_, base_, active_;
+};
+
+namespace detail {
+
 template<class E>
 inline constexpr bool always_false = false;
 
 #if defined(__HIPCC__)
-/** dest = e for a tree holding a node type of the caller's own: one sample per lane, grid-stride, every
- *  component evaluated before any is stored; instantiated in the caller's HIP translation unit. */
+/** dest = e for a tree holding a node type of the caller's own: one sample per lane, grid-stride, its
+ *  components stored in order; instantiated in the caller's HIP translation unit. */
 template<class T, std::size_t N, class E>
 __global__ __launch_bounds__(256) void assign_kernel(T* p, std::size_t pitch, std::uint32_t n, E e)
 {
@@ -1032,6 +1128,12 @@ public:
     AETHER_DEVICEHOST() T& operator()(std::size_t c, std::size_t i) const { return p_[c * pitch_ + i]; }
     AETHER_DEVICEHOST() T& operator()(std::size_t i) const { return p_[i]; }
     AETHER_DEVICEHOST() SampleRef<T, N> operator[](SampleIndex i) const { return SampleRef<T, N>(p_, pitch_, i.global()); }
+    /** A whole bundle (every lane inside the view); a tail bundle goes through bundleAssign with its mask. */
+    template<std::size_t W>
+    AETHER_DEVICEHOST() BundleRef<T, N, W> operator[](BundleIndex<W> b) const
+    {
+        return BundleRef<T, N, W>(p_, pitch_, b.base(), W);
+    }
 
     void emit(std::size_t c, detail::TapeCtx& x) const { x.push(p_, pitch_, c); }
     AETHER_DEVICEHOST() T eval(std::size_t c, std::size_t i) const { return p_[c * pitch_ + i]; }
@@ -1325,6 +1427,23 @@ using VecView = View<T, N>;
 using Vec3d = Array<double, 3>; // aether.h:234-260
 using Vec3f = Array<float, 3>;
 using Vec3dView = View<double, 3>;
+
+/** dest[b] = e over the lanes of bundle `b` that `m` marks as inside the view (a tail bundle:
+ *  bi.mask(n)), the spelling of backend/cuda/bundle/Assign.h's bundleAssign; lanes past the mask are
+ *  neither evaluated nor stored. */
+template<class T, std::size_t N, std::size_t W, class E>
+AETHER_DEVICEHOST() void bundleAssign(const View<T, N>& dest, const E& e, BundleIndex<W> b,
+                                      typename BundleIndex<W>::Mask m)
+{
+    BundleRef<T, N, W>(dest.data(), dest.pitch(), b.base(), m.active_) = e;
+}
+
+/** dest[b] = e over a whole bundle. */
+template<class T, std::size_t N, std::size_t W, class E>
+AETHER_DEVICEHOST() void bundleAssign(const View<T, N>& dest, const E& e, BundleIndex<W> b)
+{
+    dest[b] = e;
+}
 
 /** dest = e with fused per-component sums (hi + comp per component) copied to the host. */
 template<class T, std::size_t N, class E>

@@ -514,6 +514,66 @@ int ext(std::size_t n, std::uint64_t seed)
     return 0;
 }
 
+// Bundles against per-sample assignment on host views over plain buffers (no device involved): n = 37 is odd,
+// so every width has a masked tail bundle; W = 2 and 4 for doubles, 4 for floats; an axpy, a transcendental
+// tree and a quaternion chain; and views whose planes start off a 16-byte boundary (the lane-by-lane store).
+template<class T, std::size_t N, std::size_t W, class F>
+int bundle_case(const char* what, std::size_t offset, F make_expr)
+{
+    constexpr std::size_t n = 37, pitch = 40;
+    std::vector<T> a(N * pitch + offset), b(N * pitch + offset), x(N * pitch + offset, T(0)), y(N * pitch + offset, T(0));
+    for (std::size_t c = 0; c < N; ++c)
+        for (std::size_t i = 0; i < n; ++i) {
+            a[offset + c * pitch + i] = T(0.75) * T(i) + T(0.1) * T(c) - T(3);
+            b[offset + c * pitch + i] = T(-0.5) * T(i) + T(c) + T(1.25);
+        }
+    const aether::View<T, N> av(a.data() + offset, n, pitch, false), bv(b.data() + offset, n, pitch, false);
+    const aether::View<T, N> xv(x.data() + offset, n, pitch, false), yv(y.data() + offset, n, pitch, false);
+    const auto e = make_expr(av, bv);
+    for (std::size_t k = 0; k * W < n; ++k) {
+        const auto bi = aether::BundleIndex<W>::make(k);
+        if (bi.base() + W <= n) xv[bi] = e;
+        else aether::bundleAssign(xv, e, bi, bi.mask((aether::offset_t)n));
+    }
+    for (std::size_t i = 0; i < n; ++i) yv[SampleIndex::make(i)] = e;
+    if (std::memcmp(x.data(), y.data(), x.size() * sizeof(T)) != 0) {
+        std::printf("mismatch: bundle %s W=%zu offset %zu\n", what, W, offset);
+        return 1;
+    }
+    return 0;
+}
+
+int bundle()
+{
+    int bad = 0;
+    auto axpy = [](auto a, auto b) { return a + 2.5 * b; };
+    auto trig = [](auto a, auto b) { return a * aether::cwiseSin(b) + aether::cwiseExp(-1.0 * (a * a)); };
+    auto quat = [](auto a, auto b) { return a.quatMul(b).quatMul(a.quatConj()) + b.quatConj(); };
+    for (std::size_t off : {std::size_t(0), std::size_t(1)}) {
+        bad += bundle_case<double, 3, 2>("axpy", off, axpy);
+        bad += bundle_case<double, 3, 4>("axpy", off, axpy);
+        bad += bundle_case<double, 3, 2>("trig", off, trig);
+        bad += bundle_case<double, 4, 2>("quat", off, quat);
+        bad += bundle_case<float, 3, 4>("axpy f32", off, [](auto a, auto b) { return a + 2.5f * b; });
+    }
+    // the masks and the packet index
+    const auto tail = aether::BundleIndex<4>::make(9); // samples 36..39 of a 37-sample array: one lane
+    const auto m = tail.mask(37);
+    if (!(m.lane(0) && !m.lane(1) && !m.full() && aether::BundleIndex<4>::make(8).mask(37).full() &&
+          tail.lane(0).global() == 36 && tail.lane(0).work() == 0 && aether::BundleIndex<4>::make(10).mask(37).active_ == 0 &&
+          aether::BundleIndex<2>::make(3, 2, 64).base() == 2 * (2 * 64 + 3))) {
+        std::printf("mismatch: bundle masks\n");
+        ++bad;
+    }
+    const auto pk = aether::PacketIndex<8>::makeTail(32, 5);
+    if (!(aether::PacketIndex<8>::make(8).full() && !pk.full() && pk.scalar(4).global() == 36)) {
+        std::printf("mismatch: packet index\n");
+        ++bad;
+    }
+    if (!bad) std::printf("bundles ok\n");
+    return bad ? 1 : 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) return 2;
@@ -521,6 +581,7 @@ int main(int argc, char** argv)
     auto u = [&](int k) { return (std::uint64_t)std::strtoull(argv[k], nullptr, 0); };
     try {
         if (mode == "tape") return tape();
+        if (mode == "bundle") return bundle();
         if (mode == "cfg0g" && argc == 4) return cfg0g(u(2), u(3));
         if (mode == "host" && argc == 4) return host(u(2), u(3));
         if (mode == "vec3" && argc == 6) return vec3(u(2), u(3), u(4), u(5));

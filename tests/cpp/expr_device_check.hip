@@ -82,6 +82,19 @@ __global__ void cfg0_kernel(View<T, 3> d, View<T, 3> a, View<T, 3> b, View<T, 3>
     if (i.global() < n) d[i] = a + b * c + aether::cwiseSin(a) * aether::cwiseExp(T(-1) * (b * b));
 }
 
+// the same tree a bundle of W samples a thread (16 bytes of each plane: W = 2 doubles, 4 floats); the tail
+// bundle takes the masked form
+template<class T>
+__global__ void bundle_kernel(View<T, 3> d, View<T, 3> a, View<T, 3> b, View<T, 3> c, std::uint32_t n)
+{
+    constexpr std::size_t W = 16 / sizeof(T);
+    const auto bi = aether::BundleIndex<W>::make(threadIdx.x, blockIdx.x, blockDim.x);
+    const auto m = bi.mask(n);
+    const auto e = a + b * c + aether::cwiseSin(a) * aether::cwiseExp(T(-1) * (b * b));
+    if (m.full()) d[bi] = e;
+    else if (m.active_ > 0) aether::bundleAssign(d, e, bi, m);
+}
+
 template<class T>
 __global__ void vec3_kernel(View<T, 1> e, View<T, 3> a, View<T, 3> b, View<T, 3> c, std::uint32_t n)
 {
@@ -177,6 +190,15 @@ void run_cases(std::size_t n)
         launch_check("cfg0_kernel");
         want.deviceView() = av + bv * cv + aether::cwiseSin(av) * aether::cwiseExp(T(-1) * (bv * bv));
         w = std::string("cfg0 ") + tn;
+        expect_same(w.c_str(), got, want);
+    }
+    {   // the same tree in bundles of 16 bytes a plane, with a masked tail bundle
+        Array<T, 3> got(n), want(n);
+        const std::size_t bundles = (n + 16 / sizeof(T) - 1) / (16 / sizeof(T));
+        hipLaunchKernelGGL(bundle_kernel<T>, grid_of(bundles), dim3(256), 0, 0, got.deviceView(), av, bv, cv, n32);
+        launch_check("bundle_kernel");
+        want.deviceView() = av + bv * cv + aether::cwiseSin(av) * aether::cwiseExp(T(-1) * (bv * bv));
+        w = std::string("bundles ") + tn;
         expect_same(w.c_str(), got, want);
     }
     {   // Vec3 geometry

@@ -47,6 +47,13 @@ def render(name, lowered, ids):
     return out
 
 
+def test_bundles_equal_per_sample_assignment():
+    """BundleIndex / bundleAssign / View[BundleIndex] (index/BundleIndex.h:56-112, backend/cuda/bundle/Assign.h)
+    on host views: every bundle width and a masked tail give the per-sample assignment's bits, aligned and
+    not (the reference's own gate, tests/test_Bundle.cpp, with n = 37); the masks and PacketIndex."""
+    assert run("bundle").decode().strip() == "bundles ok"
+
+
 def test_cpp_lowering_equals_python_lowering():
     got = run("tape").decode().splitlines()
     arr = lambda k: ae.Array(10, k, torch.float64, device="cpu")  # noqa: E731
