@@ -50,8 +50,8 @@ SINGLE_WORKLOADS = {"cfg0_small": "wl_cfg0_small", "philox": "wl_philox", "fp32"
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--log2n", type=int, default=27)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-extra", action="store_true", help="only the headline workload")
