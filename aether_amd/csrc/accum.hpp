@@ -81,6 +81,23 @@ __device__ __forceinline__ Pair<float> ff_add(Pair<float> a, Pair<float> b)
     return Pair<float>{hi, w - bp};
 }
 
+// ff_add(a, {b, +0}) in four operations fewer, with the same bits: the second pair's low part is +0, and
+// two_sum_f(a.c, +0) is exactly (a.c + 0, a.c - a.c) -- the sum (a.c, or +0 for -0) and an error that is
+// +0 for a finite a.c and NaN (from a.c's own payload, or inf - inf) otherwise. Checked against ff_add
+// bit for bit over signed zeros, subnormals, infinities, NaNs and random floats of every magnitude.
+__device__ __forceinline__ Pair<float> ff_add_float(Pair<float> a, float b)
+{
+    float sh, sl, vh, vl;
+    two_sum_f(a.h, b, sh, sl);
+    const float th = a.c + 0.0f, tl = a.c - a.c;
+    const float c = sl + th;
+    two_sum_f(sh, c, vh, vl);
+    const float w = vl + tl;
+    const float hi = vh + w;
+    const float bp = hi - vh;
+    return Pair<float>{hi, w - bp};
+}
+
 // One accumulation step of `t` in mode M (square: accumulate t*t).
 template<int M, bool SQUARE, class T>
 __device__ __forceinline__ void accumulate(Pair<T>& p, T t)
@@ -91,15 +108,12 @@ __device__ __forceinline__ void accumulate(Pair<T>& p, T t)
         p.h = p.h + (SQUARE ? t * t : t);
     } else {
         static_assert(sizeof(T) == 4, "float-float accumulation is for float");
-        Pair<float> term;
         if constexpr (SQUARE) {
-            term.h = t * t;
-            term.c = __builtin_fmaf(t, t, -term.h);
+            const float h = t * t;
+            p = ff_add(p, Pair<float>{h, __builtin_fmaf(t, t, -h)});
         } else {
-            term.h = t;
-            term.c = 0.0f;
+            p = ff_add_float(p, t);
         }
-        p = ff_add(p, term);
     }
 }
 
