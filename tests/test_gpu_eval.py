@@ -147,6 +147,33 @@ def test_reduce_parity(dev, orc, dt, square, mode, n):
         assert out[k, 0].item() == h and out[k, 1].item() == c
 
 
+@pytest.mark.parametrize("with_nonfinite", [False, True])
+def test_reduce_floatfloat_special_values(dev, orc, with_nonfinite):
+    """Float-float sums over planes thick with signed zeros, subnormals and values of every magnitude (and,
+    in the second case, infinities and NaNs): the (hi, lo) pair of every plane is the oracle's bit for bit,
+    zero signs and NaN included, plain and squared."""
+    rng = np.random.default_rng(21)
+    n = 70_001
+    x = ae.Array(n, 3, torch.float32)
+    vals = (rng.standard_normal((3, n)) * 10.0 ** rng.integers(-44, 38, (3, n))).astype(np.float32)
+    pick = rng.integers(0, 8, (3, n))
+    vals[pick == 0] = -0.0
+    vals[pick == 1] = 0.0
+    vals[pick == 2] = np.float32(1e-45) * rng.integers(-50, 50, int((pick == 2).sum())).astype(np.float32)
+    if with_nonfinite:
+        for k, v in enumerate([np.inf, -np.inf, np.nan]):
+            vals[k, rng.integers(0, n, 3)] = v
+    x.data[:, :n] = torch.from_numpy(vals).to("cuda")
+    for square in (False, True):
+        out = ae.reduce(x, ae.SUM_FLOATFLOAT, square).cpu().numpy()
+        for k in range(3):
+            h, c = orc.device_sum(np.ascontiguousarray(vals[k]), square=square, mode="ff")
+            got = out[k].astype(np.float32)
+            want = np.array([h, c], dtype=np.float32)
+            same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+            assert same.all(), (k, square, got, want)
+
+
 def test_reduce_nonfinite(dev):
     x = ae.Array(5000, 1, torch.float64)
     x.data.fill_(1.0)
