@@ -42,6 +42,8 @@ void orc_philox_block(uint64_t seed, uint32_t global, uint64_t counter, uint32_t
 uint64_t orc_random_bits64(uint64_t seed, uint32_t global, uint64_t counter);
 /* random/detail/Backend.h:184-195 (+ :123-134 conversions) */
 double orc_uniform01_f64(uint64_t seed, uint32_t global, uint64_t counter);
+double orc_u01_f64_from_word(uint32_t w);
+float orc_u01_f32_from_word(uint32_t w);
 float orc_uniform01_f32(uint64_t seed, uint32_t global, uint64_t counter);
 /* random/detail/Backend.h:209-236; mathmode 0 = libm (the reference host route),
  * 1 = own math (the algorithm the device kernels use). */

@@ -70,6 +70,10 @@ static float u01_float_from_word(uint32_t w)
     return (float)(((w >> 9) << 1) | 1u) * 0x1p-24f;
 }
 
+/* the word conversions themselves, for checks at chosen words (tests/test_RandomIndependence.cpp:98-119) */
+double orc_u01_f64_from_word(uint32_t w) { return u01_double_from_word(w); }
+float orc_u01_f32_from_word(uint32_t w) { return u01_float_from_word(w); }
+
 double orc_uniform01_f64(uint64_t seed, uint32_t global, uint64_t counter)
 {
     uint32_t b[4];

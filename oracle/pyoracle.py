@@ -55,9 +55,12 @@ def lib_fast():
 def _bind(L):
     sig = {
         "orc_philox4x32_10": (None, [C.POINTER(_u32), _u32, _u32, C.POINTER(_u32)]),
+        "orc_philox_block": (None, [_u64, _u32, _u64, C.POINTER(_u32)]),
         "orc_random_bits64": (_u64, [_u64, _u32, _u64]),
         "orc_uniform01_f64": (_f64, [_u64, _u32, _u64]),
         "orc_uniform01_f32": (_f32, [_u64, _u32, _u64]),
+        "orc_u01_f64_from_word": (_f64, [_u32]),
+        "orc_u01_f32_from_word": (_f32, [_u32]),
         "orc_normal_f64": (_f64, [_u64, _u32, _u64, _i32]),
         "orc_normal_f32": (_f32, [_u64, _u32, _u64, _i32]),
         "orc_splitmix64": (_u64, [_u64]),
@@ -123,6 +126,21 @@ def u01_f64(seed, g, counter):
 
 def u01_f32(seed, g, counter):
     return lib().orc_uniform01_f32(seed, g, counter)
+
+
+def u01_f64_from_word(w):
+    return lib().orc_u01_f64_from_word(w)
+
+
+def u01_f32_from_word(w):
+    return np.float32(lib().orc_u01_f32_from_word(w))
+
+
+def philox_block(seed, g, counter):
+    """random/detail/Backend.h:144-150: the block of (seed, global, counter)."""
+    o = (_u32 * 4)()
+    lib().orc_philox_block(seed, g, counter, o)
+    return tuple(o)
 
 
 def normal_f64(seed, g, counter, mathmode=1):

@@ -96,12 +96,56 @@ def test_streams_and_word_layout(orc):
     assert all(u[k] == orc.u01_f64(42, 5 + k, 0) for k in range(16))
 
 
-def test_uniform_open_interval_extremes():
-    # Backend.h:123-134: w*2^-32 + 2^-33 and (2(w>>9)+1)*2^-24 never reach 0 or 1
-    for w in (0, 1, 0xFFFFFFFF, 0xFFFFFE00):
-        d = w * 2.0**-32 + 2.0**-33
-        f = np.float32(((w >> 9) << 1) | 1) * np.float32(2.0**-24)
-        assert 0.0 < d < 1.0 and np.float32(0) < f < np.float32(1)
+BOUNDARY_WORDS = (0, 1, 0x1FF, 0x200, 0x7FFFFFFF, 0x80000000, 0xFFFFFE00, 0xFFFFFFFE, 0xFFFFFFFF)
+
+
+def test_uniform_open_interval_at_boundary_words(orc):
+    """The oracle's word -> real conversions at the reference's boundary words
+    (tests/test_RandomIndependence.cpp:98-119): inside (0, 1), monotone in the word, log of the draw and of
+    its complement finite; and the known answers at the ends (a wrong shift or offset moves them): double
+    2^-33 and 1 - 2^-33, float 2^-24 and 1 - 2^-24, the float keeping the word's top 23 bits."""
+    d = [orc.u01_f64_from_word(w) for w in BOUNDARY_WORDS]
+    f = [orc.u01_f32_from_word(w) for w in BOUNDARY_WORDS]
+    assert all(0.0 < x < 1.0 for x in d) and all(np.float32(0) < x < np.float32(1) for x in f)
+    assert d == sorted(d) and f == sorted(f)
+    for x, y in zip(d, f):
+        assert np.isfinite(np.log(x)) and np.isfinite(np.log(1.0 - x)) and np.isfinite(np.log(np.float32(1) - y))
+    assert d[0] == 2.0**-33 and d[-1] == 1.0 - 2.0**-33
+    assert f[0] == np.float32(2.0**-24) and f[-1] == np.float32(1.0 - 2.0**-24)
+    assert orc.u01_f32_from_word(0x1FF) == f[0] and orc.u01_f32_from_word(0x200) == np.float32(3 * 2.0**-24)
+    # the stream's uniform is word 0 of its block, through the same conversion
+    for seed, g, c in ((42, 0, 0), (7, 123, 2), (2**64 - 1, 2**32 - 1, 5)):
+        b = orc.philox_block(seed, g, c)
+        assert orc.u01_f64(seed, g, c) == orc.u01_f64_from_word(b[0])
+        assert np.float32(orc.u01_f32(seed, g, c)) == orc.u01_f32_from_word(b[0])
+
+
+def test_blocks_word_disjoint_across_counters(orc):
+    """tests/test_RandomIndependence.cpp:60-96: the 256 blocks of counters 0..255 at one (seed, global) share
+    no word, and bits64 is (word 1 << 32) | word 0 of its own block, never overlapping the next counter's."""
+    for seed in (0, 0xC0FFEE, 2**64 - 1):
+        for g in (0, 3, 2**32 - 1):
+            words = [w for c in range(256) for w in orc.philox_block(seed, g, c)]
+            assert len(set(words)) == len(words), (seed, g)
+        for c in range(0, 512, 7):
+            b = orc.philox_block(seed, 7, c)
+            assert orc.bits64(seed, 7, c) == (b[1] << 32) | b[0]
+            assert orc.bits64(seed, 7, c) >> 32 != orc.bits64(seed, 7, c + 1) & 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_normals_independent_across_counters(orc, dt):
+    """tests/test_RandomIndependence.cpp:29-57: for independent standard normals E[z^2 z'^2] = 1; the planes
+    of an array draw with counter = component, so component 5 against component 5 + lag (lags 1..8, 16, 64)
+    over 2^17 samples must give a mean of z^2 z'^2 within 5 standard errors of 1."""
+    n, ncomp = 1 << 17, 70
+    planes = (orc.fill_array_normal_f64 if dt == "f64" else orc.fill_array_normal_f32)(0x1234567, ncomp, n, n)
+    planes = np.asarray(planes, np.float64).reshape(ncomp, n)
+    z = planes[5]
+    for lag in (1, 2, 3, 4, 5, 6, 7, 8, 16, 64):
+        q = z * z * planes[5 + lag] ** 2
+        se = np.sqrt(q.var() / n)
+        assert abs(q.mean() - 1.0) < 5 * se, (lag, q.mean(), se)
 
 
 def test_bits_moments(orc):
