@@ -152,6 +152,53 @@ def test_plane_concurrent_producers_deliver_the_exact_sum(dev):
     assert np.max(np.abs(got2 - got) / ulp) <= 1.0
 
 
+def _body_terms(cancel):
+    """The value families of tests/test_AccumOrderPermutation.cu:90-115 (8 bodies x 3 components x 128
+    samples): magnitudes spread over eleven decades with hashed signs, or bodies that cancel pairwise to
+    within 0 .. 1e-3 relative; each value carries the same hashed jitter."""
+    kb, ks = 8, 128
+    i = np.arange(ks, dtype=np.uint64)
+    v = np.zeros((kb, 3, ks))
+    rel = [1.0, 1.1e-3, 3.1e-6, 6.0e-7, 5.0e-8, 1.0e-10, 4.0e-12, 8.0e-13]
+    dl = np.array([0.0, 2.22e-16, 9.1e-13, 9.3e-10, 9.5e-7, 9.8e-4])[(i % 6).astype(np.int64)]
+    for k in range(kb):
+        for d in range(3):
+            h = (i * np.uint64(2654435761) + np.uint64(d * 40503) + np.uint64(k * 2246822519)) & np.uint64(0xFFFFFFFF)
+            jit = 1.0 + 0.4375 * ((h >> np.uint64(8)) & np.uint64(0xFFFF)).astype(np.float64) / 65536.0
+            if cancel:
+                a = 1.0e-2 * jit
+                v[k, d] = a if k % 2 == 0 else -a * (1.0 - dl)
+            else:
+                sgn = np.where((h >> np.uint64(3)) & np.uint64(1), 1.0, -1.0)
+                v[k, d] = 8.0e-3 * rel[k] * jit * (1.0 if k == 0 else sgn)
+    return v
+
+
+@pytest.mark.parametrize("cancel", [False, True], ids=["spread", "cancelling"])
+def test_plane_bitwise_under_permuted_launch_orders(dev, cancel):
+    """The reference's admissibility bar for the compensated plane (tests/test_AccumOrderPermutation.cu:182):
+    the same 8 bodies added one launch each in five orders (identity, reverse, three shuffles) deliver planes
+    that are bit-identical, and equal to the correctly rounded sum of each sample's 8 terms."""
+    v = _body_terms(cancel)
+    bodies = []
+    for k in range(v.shape[0]):
+        x = ae.Array(v.shape[2], 3, torch.float64)
+        x.data[:, : v.shape[2]] = torch.from_numpy(v[k]).to("cuda")
+        bodies.append(x)
+    rng = np.random.default_rng(0xACC0)
+    orders = [list(range(8)), list(range(7, -1, -1))] + [list(rng.permutation(8)) for _ in range(3)]
+    planes = []
+    for order in orders:
+        plane = A.Plane(v.shape[2], 3)
+        for k in order:
+            plane.add(bodies[k])
+        planes.append(plane.delivered().data[:, : v.shape[2]].cpu().numpy().copy())
+    for p in planes[1:]:
+        assert np.array_equal(p.view(np.uint64), planes[0].view(np.uint64))
+    exact = np.array([[math.fsum(v[:, d, j]) for j in range(v.shape[2])] for d in range(3)])
+    assert np.array_equal(planes[0], exact)
+
+
 def test_serialized_plane_is_the_sequential_sum(dev):
     n, vd = 100_003, 2
     plane = A.Plane(n, vd, policy="serialized")
