@@ -536,6 +536,82 @@ struct ScalarNode : Expr<ScalarNode<T>> {
     AETHER_DEVICEHOST() T eval(std::size_t, std::size_t) const { return v; }
 };
 
+/**
+ * Item<T, N> (view/Item.h:60-330, vector rank): N values held by value, a register-resident vector.
+ * Constructed from N values, filled(v), Zeros(), Ones(), or the value of an expression at sample 0 (a
+ * sample-free tree of Items and constants; `SampleRef::get()` gives a view's sample as one). As an
+ * expression leaf it is the same N values at every sample (a whole-array assignment lowers them to
+ * constants). Assigning a tree stores component c before it evaluates component c + 1, as a view's sample
+ * assignment does.
+ */
+template<class T, std::size_t N>
+class Item : public Expr<Item<T, N>> {
+public:
+    using element_type = T;
+    static constexpr std::size_t ncomp = N;
+    static constexpr std::size_t Size = N;
+    static constexpr bool lowerable = true;
+
+    constexpr Item() = default;
+    template<class... Vs>
+        requires(sizeof...(Vs) == N && N >= 2 && (std::is_arithmetic_v<Vs> && ...))
+    AETHER_DEVICEHOST() constexpr Item(Vs... vs) : data_{static_cast<T>(vs)...}
+    {
+    }
+    AETHER_DEVICEHOST() constexpr explicit Item(const T& v)
+        requires(N == 1)
+        : data_{v}
+    {
+    }
+    template<class E>
+        requires(std::is_base_of_v<Expr<E>, E> && !std::is_same_v<E, Item>)
+    AETHER_DEVICEHOST() Item(const E& e)
+    {
+        *this = e;
+    }
+    template<class E>
+        requires(std::is_base_of_v<Expr<E>, E> && !std::is_same_v<E, Item>)
+    AETHER_DEVICEHOST() Item& operator=(const E& e)
+    {
+        static_assert(E::ncomp == N || E::ncomp == 1, "aether: expression and item component counts differ");
+        for (std::size_t c = 0; c < N; ++c) data_[c] = (T)e.eval(E::ncomp > 1 ? c : 0, 0);
+        return *this;
+    }
+
+    AETHER_DEVICEHOST() static constexpr Item filled(const T& v)
+    {
+        Item r;
+        for (std::size_t c = 0; c < N; ++c) r.data_[c] = v;
+        return r;
+    }
+    AETHER_DEVICEHOST() static constexpr Item Zeros() { return filled(T(0)); }
+    AETHER_DEVICEHOST() static constexpr Item Ones() { return filled(T(1)); }
+
+    AETHER_DEVICEHOST() constexpr T& operator()(std::size_t c) { return data_[c]; }
+    AETHER_DEVICEHOST() constexpr const T& operator()(std::size_t c) const { return data_[c]; }
+    template<std::size_t I>
+    AETHER_DEVICEHOST() constexpr T& get()
+    {
+        static_assert(I < N, "aether: Item::get<I>() out of range");
+        return data_[I];
+    }
+    template<std::size_t I>
+    AETHER_DEVICEHOST() constexpr const T& get() const
+    {
+        static_assert(I < N, "aether: Item::get<I>() out of range");
+        return data_[I];
+    }
+    AETHER_DEVICEHOST() constexpr T* data() { return data_; }
+    AETHER_DEVICEHOST() constexpr const T* data() const { return data_; }
+    AETHER_DEVICEHOST() static constexpr std::size_t size() { return N; }
+
+    void emit(std::size_t c, detail::TapeCtx& x) const { x.constant((double)data_[c]); }
+    AETHER_DEVICEHOST() T eval(std::size_t c, std::size_t) const { return data_[c]; }
+
+private:
+    T data_[N] = {};
+};
+
 template<class L, class R, int OP>
 struct BinaryNode : Expr<BinaryNode<L, R, OP>> {
     using element_type = elem_t<L>;
@@ -1007,27 +1083,57 @@ class SampleRef {
 public:
     AETHER_DEVICEHOST() SampleRef(T* p, std::size_t pitch, std::size_t i) : p_(p), pitch_(pitch), i_(i) {}
     template<class E>
+        requires aether_expression<E>
     AETHER_DEVICEHOST() SampleRef& operator=(const E& e)
     {
         for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] = (T)e.eval(E::ncomp > 1 ? c : 0, i_);
         return *this;
     }
     template<class E>
+        requires aether_expression<E>
     AETHER_DEVICEHOST() SampleRef& operator+=(const E& e)
     {
         for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] += (T)e.eval(E::ncomp > 1 ? c : 0, i_);
         return *this;
     }
     template<class E>
+        requires aether_expression<E>
     AETHER_DEVICEHOST() SampleRef& operator-=(const E& e)
     {
         for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] -= (T)e.eval(E::ncomp > 1 ? c : 0, i_);
         return *this;
     }
-    AETHER_DEVICEHOST() std::array<T, N> get() const
+    /** A scalar broadcast to every component (expr/Assign.h:296-306): = sets, += adds, -= subtracts. */
+    template<class S>
+        requires(!aether_expression<S> && std::is_arithmetic_v<S>)
+    AETHER_DEVICEHOST() SampleRef& operator=(const S& s)
     {
-        std::array<T, N> r{};
-        for (std::size_t c = 0; c < N; ++c) r[c] = p_[c * pitch_ + i_];
+        for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] = (T)s;
+        return *this;
+    }
+    template<class S>
+        requires(!aether_expression<S> && std::is_arithmetic_v<S>)
+    AETHER_DEVICEHOST() SampleRef& operator+=(const S& s)
+    {
+        for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] += (T)s;
+        return *this;
+    }
+    template<class S>
+        requires(!aether_expression<S> && std::is_arithmetic_v<S>)
+    AETHER_DEVICEHOST() SampleRef& operator-=(const S& s)
+    {
+        for (std::size_t c = 0; c < N; ++c) p_[c * pitch_ + i_] -= (T)s;
+        return *this;
+    }
+    /** Not rebindable: `v[i] = w[j]` is not an assignment of values (the reference's SampleRef holds its
+     *  view by reference); assign `w[j].get()` instead. */
+    SampleRef& operator=(const SampleRef&) = delete;
+    SampleRef(const SampleRef&) = default;
+    /** The sample's components as an Item (expr/Assign.h:336-341). */
+    AETHER_DEVICEHOST() Item<T, N> get() const
+    {
+        Item<T, N> r;
+        for (std::size_t c = 0; c < N; ++c) r(c) = p_[c * pitch_ + i_];
         return r;
     }
 
@@ -1422,11 +1528,17 @@ private:
     void* dev_ = nullptr;
 };
 
+// aether.h:234-260: Vec is a register-resident Item; the *View aliases are batched views
+template<class T, std::size_t N>
+using Vec = Item<T, N>;
+using Vec3d = Vec<double, 3>;
+using Vec4d = Vec<double, 4>;
+using Vec6d = Vec<double, 6>;
+template<class T>
+using ScalarView = View<T, 1>;
 template<class T, std::size_t N>
 using VecView = View<T, N>;
-using Vec3d = Array<double, 3>; // aether.h:234-260
-using Vec3f = Array<float, 3>;
-using Vec3dView = View<double, 3>;
+using Vec3dView = VecView<double, 3>;
 
 /** dest[b] = e over the lanes of bundle `b` that `m` marks as inside the view (a tail bundle:
  *  bi.mask(n)), the spelling of backend/cuda/bundle/Assign.h's bundleAssign; lanes past the mask are

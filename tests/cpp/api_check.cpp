@@ -259,7 +259,7 @@ int rfft(std::size_t rows, std::size_t n)
 
 int append(std::size_t n)
 {
-    aether::Vec3d x(0);
+    Array<double, 3> x(0);
     bool ok = x.samples() == 0 && x.capacity() == 0;
     for (std::size_t i = 0; i < n; ++i) x.push_back({double(i), 2.0 * double(i), -double(i)});
     ok = ok && x.samples() == n && x.capacity() >= n && x.capacity() % 32 == 0;
@@ -270,7 +270,7 @@ int append(std::size_t n)
     r += v;         // x -> 2x
     r -= 0.25 * v;  // reads the updated sample: 2x - x/2
     const auto g = r.get();
-    ok = ok && g[0] == 1.5 * double(n / 2) && g[1] == 3.0 * double(n / 2);
+    ok = ok && g(0) == 1.5 * double(n / 2) && g(1) == 3.0 * double(n / 2);
     const std::size_t spare = x.capacity() - x.samples();
     auto sp = x.hostSpare();
     if (spare > 0) sp(0, 0) = 42.0;
@@ -574,6 +574,54 @@ int bundle()
     return bad ? 1 : 0;
 }
 
+// Item (view/Item.h) and scalar broadcast through SampleRef (tests/test_ScalarBroadcast.cpp's rows) on host
+// views over plain buffers
+int item()
+{
+    using aether::Vec3d;
+    int bad = 0;
+    auto fail = [&](const char* what) {
+        std::printf("mismatch: item %s\n", what);
+        ++bad;
+    };
+    constexpr std::size_t n = 4;
+    std::vector<double> buf(3 * n, -1.0), wbuf(3 * n, -9.0);
+    const aether::View<double, 3> v(buf.data(), n, n, false), w(wbuf.data(), n, n, false);
+    for (std::size_t i = 0; i < n; ++i) v[SampleIndex::make(i)] = 2.0;
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t c = 0; c < 3; ++c)
+            if (v(c, i) != 2.0) fail("scalar assignment");
+    const Vec3d a{1.0, 2.0, 3.0};
+    v[SampleIndex::make(1)] = a;
+    v[SampleIndex::make(1)] += 0.5;
+    if (!(v(0, 1) == 1.5 && v(1, 1) == 2.5 && v(2, 1) == 3.5)) fail("+= scalar");
+    v[SampleIndex::make(1)] -= 0.25;
+    if (!(v(0, 1) == 1.25 && v(1, 1) == 2.25 && v(2, 1) == 3.25)) fail("-= scalar");
+    // a scalar store and the same store of a filled Item give the same bits
+    w[SampleIndex::make(0)] = Vec3d::filled(2.0);
+    if (std::memcmp(&v(0, 0), &w(0, 0), sizeof(double)) || std::memcmp(&v(2, 0), &w(2, 0), sizeof(double)))
+        fail("scalar store vs filled item");
+    // an Item from a sample-free tree, accessors, get()
+    const Vec3d out = a + Vec3d::filled(10.0);
+    if (!(out(0) == 11.0 && out.get<1>() == 12.0 && out.data()[2] == 13.0 && Vec3d::size() == 3)) fail("tree value");
+    const Vec3d b{0.5, -1.0, 4.0};
+    const Vec3d cr = a.cross(b);
+    if (!(cr(0) == 2.0 * 4.0 - 3.0 * -1.0 && cr(1) == 3.0 * 0.5 - 1.0 * 4.0 && cr(2) == 1.0 * -1.0 - 2.0 * 0.5))
+        fail("cross");
+    const aether::Vec<double, 1> d1 = a.dot(b);
+    if (d1(0) != 3.0 * 4.0 + (2.0 * -1.0 + 1.0 * 0.5)) fail("dot fold order");
+    if (!(Vec3d::Zeros()(1) == 0.0 && Vec3d::Ones()(2) == 1.0 && Vec3d{}(0) == 0.0)) fail("factories");
+    const Vec3d g = v[SampleIndex::make(1)].get();
+    if (!(g(0) == 1.25 && g(2) == 3.25)) fail("get");
+    // an Item as a leaf of a per-sample tree: the same value at every sample
+    for (std::size_t i = 0; i < n; ++i) w[SampleIndex::make(i)] = v * 2.0 + a;
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t c = 0; c < 3; ++c)
+            if (w(c, i) != v(c, i) * 2.0 + a(c)) fail("leaf");
+    if (!bad) std::printf("items ok\n");
+    return bad ? 1 : 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) return 2;
@@ -582,6 +630,7 @@ int main(int argc, char** argv)
     try {
         if (mode == "tape") return tape();
         if (mode == "bundle") return bundle();
+        if (mode == "item") return item();
         if (mode == "cfg0g" && argc == 4) return cfg0g(u(2), u(3));
         if (mode == "host" && argc == 4) return host(u(2), u(3));
         if (mode == "vec3" && argc == 6) return vec3(u(2), u(3), u(4), u(5));

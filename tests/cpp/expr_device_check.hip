@@ -82,6 +82,14 @@ __global__ void cfg0_kernel(View<T, 3> d, View<T, 3> a, View<T, 3> b, View<T, 3>
     if (i.global() < n) d[i] = a + b * c + aether::cwiseSin(a) * aether::cwiseExp(T(-1) * (b * b));
 }
 
+// an Item leaf: the same three values at every sample
+template<class T>
+__global__ void item_kernel(View<T, 3> d, View<T, 3> a, View<T, 3> b, aether::Item<T, 3> k, std::uint32_t n)
+{
+    const SampleIndex i = SampleIndex::make(threadIdx.x, blockIdx.x, blockDim.x);
+    if (i.global() < n) d[i] = a + k * b - aether::cwiseSin(k);
+}
+
 // the same tree a bundle of W samples a thread (16 bytes of each plane: W = 2 doubles, 4 floats); the tail
 // bundle takes the masked form
 template<class T>
@@ -166,7 +174,7 @@ __global__ void get_kernel(View<T, 1> s, View<T, 3> a, std::uint32_t n)
     const SampleIndex i = SampleIndex::make(threadIdx.x, blockIdx.x, blockDim.x);
     if (i.global() < n) {
         const auto item = a[i].get();
-        s(i.global()) = item[2] + (item[1] + item[0]);
+        s(i.global()) = item.template get<2>() + (item(1) + item(0));
     }
 }
 
@@ -190,6 +198,15 @@ void run_cases(std::size_t n)
         launch_check("cfg0_kernel");
         want.deviceView() = av + bv * cv + aether::cwiseSin(av) * aether::cwiseExp(T(-1) * (bv * bv));
         w = std::string("cfg0 ") + tn;
+        expect_same(w.c_str(), got, want);
+    }
+    {   // an Item leaf: per sample in a kernel against the whole-array assignment, where it lowers to constants
+        Array<T, 3> got(n), want(n);
+        const aether::Item<T, 3> k{T(0.75), T(-1.5), T(2.25)};
+        hipLaunchKernelGGL(item_kernel<T>, grid_of(n), dim3(256), 0, 0, got.deviceView(), av, bv, k, n32);
+        launch_check("item_kernel");
+        want.deviceView() = av + k * bv - aether::cwiseSin(k);
+        w = std::string("item leaf ") + tn;
         expect_same(w.c_str(), got, want);
     }
     {   // the same tree in bundles of 16 bytes a plane, with a masked tail bundle

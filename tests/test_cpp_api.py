@@ -54,6 +54,13 @@ def test_bundles_equal_per_sample_assignment():
     assert run("bundle").decode().strip() == "bundles ok"
 
 
+def test_items_and_scalar_broadcast():
+    """Item / Vec3d (view/Item.h, aether.h:234-260) and scalar = += -= through SampleRef (the rows of the
+    reference's tests/test_ScalarBroadcast.cpp), Items in sample-free trees (cross, the dot fold order),
+    get() as an Item, and an Item as a leaf of a per-sample tree, on host views."""
+    assert run("item").decode().strip() == "items ok"
+
+
 def test_cpp_lowering_equals_python_lowering():
     got = run("tape").decode().splitlines()
     arr = lambda k: ae.Array(10, k, torch.float64, device="cpu")  # noqa: E731
