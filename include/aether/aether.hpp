@@ -1418,11 +1418,27 @@ public:
     }
     ~Array() { release(); }
 
+    /** array/Array.h:89-94: the view types (planes at pitch capacity(); the packed form at pitch samples()). */
+    using ViewT = View<T, N>;
+    using PackedViewT = View<T, N>;
+
     std::size_t samples() const { return samples_; }
     std::size_t size() const { return N * samples_; }
     std::size_t capacity() const { return capacity_; }
     View<T, N> hostView() const { return View<T, N>(static_cast<T*>(host_), samples_, capacity_, false); }
     View<T, N> deviceView() const { return View<T, N>(static_cast<T*>(dev_), samples_, capacity_, true); }
+    /** array/Array.h:176-218: the compact view (plane pitch = samples()), only for an Array whose capacity is
+     *  its sample count (throws otherwise). */
+    View<T, N> hostPacked() const
+    {
+        require_packed();
+        return View<T, N>(static_cast<T*>(host_), samples_, samples_, false);
+    }
+    View<T, N> devicePacked() const
+    {
+        require_packed();
+        return View<T, N>(static_cast<T*>(dev_), samples_, samples_, true);
+    }
     void upload(Stream s = nullptr)
     {
         const int dev = detail::current_device();
@@ -1489,6 +1505,12 @@ public:
     }
 
 private:
+    void require_packed() const
+    {
+        if (capacity_ != samples_)
+            throw Error("aether: a packed view needs capacity() == samples() (" + std::to_string(capacity_) + " != " +
+                        std::to_string(samples_) + ")");
+    }
     Array(std::size_t n_samples, std::size_t capacity) : samples_(n_samples), capacity_(capacity) { allocate(); }
     static std::size_t quantize(std::size_t n)
     {

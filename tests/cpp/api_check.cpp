@@ -436,6 +436,17 @@ int resize(std::size_t n)
     auto v = x.hostView();
     bool ok = x.samples() == n + 100 && x.capacity() >= n + 100 && x.capacity() % 32 == 0 && cap0 % 32 == 0;
     for (std::size_t i = 0; i < n; ++i) ok = ok && v(0, i) == 3.0 * double(i) && v(1, i) == -3.0 * double(i);
+    // the packed views: pitch samples() when the capacity is the sample count, refused otherwise
+    Array<double, 2> full(64);
+    const auto pk = full.devicePacked();
+    ok = ok && pk.pitch() == 64 && pk.samples() == 64 && full.hostPacked().data() == full.hostView().data();
+    bool refused = false;
+    try {
+        (void)x.hostPacked(); // capacity() > samples() after the resize
+    } catch (const aether::Error&) {
+        refused = x.capacity() != x.samples();
+    }
+    ok = ok && refused;
     std::printf("resize %s samples %zu capacity %zu\n", ok ? "ok" : "BAD", x.samples(), x.capacity());
     return ok ? 0 : 1;
 }
