@@ -163,6 +163,30 @@ def normal(generator, ncomp: int, mean: float, sd: float) -> Random:
     return Random(generator.normal_seed(), ncomp, AETH_RNG_DIST_NORMAL, mean, sd)
 
 
+# the other distributions' stream salts (random/detail/DistributionLeaf.h:249, 317, 378)
+LOGNORMAL_SALT, EXPONENTIAL_SALT, BERNOULLI_SALT = 0xA4093822299F31D0, 0x082EFA98EC4E6C89, 0x452821E638D01377
+
+
+def lognormal(generator, ncomp: int, mu: float, sigma: float) -> "Expr":
+    """Generator(s).lognormal<Real, ncomp>(mu, sigma) (random/detail/DistributionLeaf.h:240-300):
+    exp(mu + sigma z), z the standard normal of the lognormal stream."""
+    return exp(Random(generator.key ^ LOGNORMAL_SALT, ncomp, AETH_RNG_DIST_NORMAL, mu, sigma))
+
+
+def exponential(generator, ncomp: int, lam: float) -> "Expr":
+    """Generator(s).exponential<Real, ncomp>(lambda) (DistributionLeaf.h:305-365): -log(u) / lambda, a
+    division (not a product by 1/lambda)."""
+    u = Random(generator.key ^ EXPONENTIAL_SALT, ncomp, AETH_RNG_DIST_UNIFORM, 0.0, 1.0)
+    return div(-log(u), Const(float(lam)))
+
+
+def bernoulli(generator, ncomp: int, p: float) -> "Expr":
+    """Generator(s).bernoulli<Real, ncomp>(p) (DistributionLeaf.h:370-430): 1 when u < p, else 0; sign(p - u)
+    is +1 exactly when u < p (a difference of two floats is 0 only when they are equal)."""
+    u = Random(generator.key ^ BERNOULLI_SALT, ncomp, AETH_RNG_DIST_UNIFORM, 0.0, 1.0)
+    return maximum(sign(Const(float(p)) - u), 0.0)
+
+
 class Bin(Expr):
     def __init__(self, op, l: Expr, r: Expr):
         self.op, self.l, self.r = op, l, r

@@ -2041,9 +2041,71 @@ public:
         l.dist = AETH_DIST_NORMAL;
         return l;
     }
+    /** Per-component bounds and parameters (random/Generator.h:90-111): lo + (hi - lo) u and mean + sd z, the
+     *  same draws as the scalar forms with the parameters of component c (a leaf on [0, 1) gives u exactly,
+     *  one of mean 0 and sd 1 gives z exactly). */
+    template<class Real, std::size_t VD>
+    AETHER_DEVICEHOST() auto uniform(const Item<Real, VD>& lo, const Item<Real, VD>& hi) const
+    {
+        return lo + (hi - lo) * uniform<Real, VD>(Real(0), Real(1));
+    }
+    template<class Real, std::size_t VD>
+    AETHER_DEVICEHOST() auto normal(const Item<Real, VD>& mean, const Item<Real, VD>& sd) const
+    {
+        return mean + sd * normal<Real, VD>(Real(0), Real(1));
+    }
+    /** random/detail/DistributionLeaf.h:240-300: exp(mu + sigma z), z the standard normal of the lognormal
+     *  stream's own salt. */
+    template<class Real, std::size_t VD>
+    AETHER_DEVICEHOST() auto lognormal(Real mu, Real sigma) const
+    {
+        return cwiseExp(leaf<Real, VD>(0xA4093822299F31D0ULL, AETH_DIST_NORMAL, mu, sigma));
+    }
+    template<class Real, std::size_t VD>
+    AETHER_DEVICEHOST() auto lognormal(const Item<Real, VD>& mu, const Item<Real, VD>& sigma) const
+    {
+        return cwiseExp(mu + sigma * leaf<Real, VD>(0xA4093822299F31D0ULL, AETH_DIST_NORMAL, Real(0), Real(1)));
+    }
+    /** DistributionLeaf.h:305-365: -log(u) / lambda, a division (not a product by 1/lambda). */
+    template<class Real, std::size_t VD>
+    AETHER_DEVICEHOST() auto exponential(Real lambda) const
+    {
+        return cwiseDiv(-cwiseLog(leaf<Real, VD>(0x082EFA98EC4E6C89ULL, AETH_DIST_UNIFORM, Real(0), Real(1))),
+                        ScalarNode<Real>(lambda));
+    }
+    template<class Real, std::size_t VD>
+    AETHER_DEVICEHOST() auto exponential(const Item<Real, VD>& lambda) const
+    {
+        return cwiseDiv(-cwiseLog(leaf<Real, VD>(0x082EFA98EC4E6C89ULL, AETH_DIST_UNIFORM, Real(0), Real(1))), lambda);
+    }
+    /** DistributionLeaf.h:370-430: 1 when u < p, else 0. The difference of two floats is 0 only when they are
+     *  equal, so sign(p - u) is +1 exactly when u < p, and max(sign(p - u), 0) is the draw (0 for a NaN p). */
+    template<class Real, std::size_t VD>
+    AETHER_DEVICEHOST() auto bernoulli(Real p) const
+    {
+        return cwiseMax(
+            cwiseSign(ScalarNode<Real>(p) - leaf<Real, VD>(0x452821E638D01377ULL, AETH_DIST_UNIFORM, Real(0), Real(1))),
+            ScalarNode<Real>(Real(0)));
+    }
+    template<class Real, std::size_t VD>
+    AETHER_DEVICEHOST() auto bernoulli(const Item<Real, VD>& p) const
+    {
+        return cwiseMax(cwiseSign(p - leaf<Real, VD>(0x452821E638D01377ULL, AETH_DIST_UNIFORM, Real(0), Real(1))),
+                        ScalarNode<Real>(Real(0)));
+    }
     constexpr std::uint64_t key() const { return key_; }
 
 private:
+    template<class Real, std::size_t VD>
+    AETHER_DEVICEHOST() DistLeaf<Real, VD> leaf(std::uint64_t salt, std::uint32_t dist, Real p0, Real p1) const
+    {
+        DistLeaf<Real, VD> l;
+        l.seed = key_ ^ salt;
+        l.p0 = p0;
+        l.p1 = p1;
+        l.dist = dist;
+        return l;
+    }
     std::uint64_t key_;
 };
 

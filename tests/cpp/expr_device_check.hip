@@ -151,6 +151,17 @@ __global__ void random_kernel(View<T, 3> d, View<T, 3> a, aether::random::Genera
         d[i] = g.uniform<T, 3>(T(-1), T(1)) * a + g.substream(3).normal<T, 3>(T(0.5), T(2));
 }
 
+// the other distributions (lognormal, exponential, bernoulli), scalar and per-component parameters
+template<class T>
+__global__ void dist_kernel(View<T, 3> d, aether::random::Generator g, std::uint32_t n)
+{
+    const SampleIndex i = SampleIndex::make(threadIdx.x, blockIdx.x, blockDim.x);
+    const aether::Item<T, 3> mu{T(0.25), T(-0.5), T(1)}, lam{T(1.5), T(0.5), T(3)};
+    if (i.global() < n)
+        d[i] = g.lognormal<T, 3>(T(0.25), T(0.5)) + g.exponential<T, 3>(T(1.5)) * g.bernoulli<T, 3>(T(0.3)) +
+               g.lognormal(mu, lam) - g.exponential(lam) + g.uniform(mu, lam) * g.normal(mu, lam);
+}
+
 template<class T>
 __global__ void update_kernel(View<T, 3> d, View<T, 3> a, View<T, 3> b, std::uint32_t n)
 {
@@ -259,6 +270,16 @@ void run_cases(std::size_t n)
         launch_check("random_kernel");
         want.deviceView() = g.uniform<T, 3>(T(-1), T(1)) * av + g.substream(3).normal<T, 3>(T(0.5), T(2));
         w = std::string("random leaves ") + tn;
+        expect_same(w.c_str(), got, want);
+    }
+    {   // lognormal / exponential / bernoulli and the per-component forms inside the kernel
+        Array<T, 3> got(n), want(n);
+        hipLaunchKernelGGL(dist_kernel<T>, grid_of(n), dim3(256), 0, 0, got.deviceView(), g, n32);
+        launch_check("dist_kernel");
+        const aether::Item<T, 3> mu{T(0.25), T(-0.5), T(1)}, lam{T(1.5), T(0.5), T(3)};
+        want.deviceView() = g.lognormal<T, 3>(T(0.25), T(0.5)) + g.exponential<T, 3>(T(1.5)) * g.bernoulli<T, 3>(T(0.3)) +
+                            g.lognormal(mu, lam) - g.exponential(lam) + g.uniform(mu, lam) * g.normal(mu, lam);
+        w = std::string("distributions ") + tn;
         expect_same(w.c_str(), got, want);
     }
     {   // += and -= on the device against the same updates as whole-array assignments
