@@ -2111,6 +2111,33 @@ private:
 
 inline Generator seed(std::uint64_t s) { return Generator(s); }
 
+// random/free.h:46-79: the distributions of Generator(DEFAULT_SEED)
+template<class Real, std::size_t VD>
+AETHER_DEVICEHOST() auto uniform(Real lo, Real hi)
+{
+    return Generator(DEFAULT_SEED).uniform<Real, VD>(lo, hi);
+}
+template<class Real, std::size_t VD>
+AETHER_DEVICEHOST() auto normal(Real mean, Real sd)
+{
+    return Generator(DEFAULT_SEED).normal<Real, VD>(mean, sd);
+}
+template<class Real, std::size_t VD>
+AETHER_DEVICEHOST() auto lognormal(Real mu, Real sigma)
+{
+    return Generator(DEFAULT_SEED).lognormal<Real, VD>(mu, sigma);
+}
+template<class Real, std::size_t VD>
+AETHER_DEVICEHOST() auto exponential(Real lambda)
+{
+    return Generator(DEFAULT_SEED).exponential<Real, VD>(lambda);
+}
+template<class Real, std::size_t VD>
+AETHER_DEVICEHOST() auto bernoulli(Real p)
+{
+    return Generator(DEFAULT_SEED).bernoulli<Real, VD>(p);
+}
+
 } // namespace random
 
 // ---------------------------------------------------------------- fft

@@ -629,6 +629,13 @@ int item()
     for (std::size_t i = 0; i < n; ++i)
         for (std::size_t c = 0; c < 3; ++c)
             if (w(c, i) != v(c, i) * 2.0 + a(c)) fail("leaf");
+    // the free distributions are Generator(DEFAULT_SEED)'s (random/free.h:46-79)
+    const aether::random::Generator dg(aether::random::DEFAULT_SEED);
+    for (std::size_t i = 0; i < 3; ++i)
+        if (aether::random::uniform<double, 3>(-1.0, 1.0).eval(1, i) != dg.uniform<double, 3>(-1.0, 1.0).eval(1, i) ||
+            aether::random::bernoulli<double, 3>(0.5).eval(2, i) != dg.bernoulli<double, 3>(0.5).eval(2, i) ||
+            aether::random::exponential<double, 3>(2.0).eval(0, i) != dg.exponential<double, 3>(2.0).eval(0, i))
+            fail("free distributions");
     if (!bad) std::printf("items ok\n");
     return bad ? 1 : 0;
 }
