@@ -54,8 +54,13 @@ __device__ __forceinline__ uint64_t twopi_window(int j)
     return v;
 }
 
-// |x| >= 2^20, finite. Returns the quadrant of |x|; (rh, rl) ~ |x| - q pi/2.
-__device__ __noinline__ int reduce_large(double ax, double* rh, double* rl)
+// |x| >= 2^20, finite: the quadrant q of |x| and (rh, rl) ~ |x| - q pi/2. Out of line (waves rarely take it)
+// and returned by value, so the result comes back in registers and the callers keep no stack.
+struct LargeReduction {
+    double rh, rl;
+    int q;
+};
+__device__ __noinline__ LargeReduction reduce_large(double ax)
 {
     const uint64_t u = bits_of(ax);
     const int E = (int)(u >> 52) - 1075;
@@ -107,9 +112,7 @@ __device__ __noinline__ int reduce_large(double ax, double* rh, double* rl)
     const double l = __builtin_fma(sh, kPio2_1, -h) + __builtin_fma(sh, kPio2_2, sl * kPio2_1);
     const double r = h + l;
     const double t = l - (r - h);
-    *rh = neg ? -r : r;
-    *rl = neg ? -t : t;
-    return q & 3;
+    return LargeReduction{neg ? -r : r, neg ? -t : t, q & 3};
 }
 
 __device__ __forceinline__ int reduce_pio2(double x, double* rh, double* rl)
@@ -129,8 +132,9 @@ __device__ __forceinline__ int reduce_pio2(double x, double* rh, double* rl)
     int q = ((int)(long long)n) & 3;
     const double ax = __builtin_fabs(x);
     if (ax >= 0x1p20 && ax <= 0x1.fffffffffffffp+1023) {
-        double lh, ll;
-        int ql = reduce_large(ax, &lh, &ll);
+        const LargeReduction lr = reduce_large(ax);
+        double lh = lr.rh, ll = lr.rl;
+        int ql = lr.q;
         if (x < 0) {
             lh = -lh;
             ll = -ll;
@@ -301,8 +305,9 @@ __device__ __forceinline__ int reduce_pio2_f(double x, double* r)
     int q = ((int)(long long)n) & 3;
     const double ax = __builtin_fabs(x);
     if (ax >= 0x1p20 && ax <= 0x1.fffffffffffffp+1023) {
-        double lh, ll;
-        int ql = reduce_large(ax, &lh, &ll);
+        const LargeReduction lr = reduce_large(ax);
+        double lh = lr.rh, ll = lr.rl;
+        int ql = lr.q;
         if (x < 0) {
             lh = -lh;
             ll = -ll;
@@ -351,20 +356,28 @@ __device__ __forceinline__ int reduce_ff(float x, float& rh, float& rl)
     return ((int)n) & 3;
 }
 
-// 2^17 and above (and non-finite): the double reduction, its result split into two floats
-__device__ __noinline__ int reduce_big_f(float x, float& rh, float& rl)
+// 2^17 and above (and non-finite): the double reduction, its result split into two floats. Inline: the
+// Payne-Hanek part it may reach is the one out-of-line call, a leaf returning in registers, so kernels keep
+// no stack.
+struct BigReductionF {
+    float rh, rl;
+    int q;
+};
+__device__ __forceinline__ BigReductionF reduce_big_f(float x)
 {
     double r;
     const int q = reduce_pio2_f((double)x, &r);
-    rh = (float)r;
-    rl = (float)(r - (double)rh);
-    return q;
+    const float rh = (float)r;
+    return BigReductionF{rh, (float)(r - (double)rh), q};
 }
 
 __device__ __forceinline__ int reduce_f32(float x, float& rh, float& rl)
 {
     if (__builtin_fabsf(x) < 0x1p17f) return reduce_ff(x, rh, rl);
-    return reduce_big_f(x, rh, rl);
+    const BigReductionF b = reduce_big_f(x);
+    rh = b.rh;
+    rl = b.rl;
+    return b.q;
 }
 
 // sin(rh + rl) on |rh| <= pi/4: rh + (rh^3 P(rh^2) + rl (1 - rh^2/2)), Taylor to r^11; the leading
